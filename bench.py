@@ -1,0 +1,177 @@
+#!/usr/bin/env python3
+"""Benchmark: one equi-join per step on one MI355X, inputs resident in HBM.
+
+Default workload (the headline configuration): phj_radix_bc on
+|R| = 2^24, |S| = 2^28 tuples of 16 bytes {u64 key, u64 payload}, R keys a
+seeded permutation of 1..2^24 (payload = key), S keys uniform over R's keys
+(payload = 1-based row index).  One step = hj_join() = partition + join +
+result read-back.  `value` is (|R|+|S|) tuples per second over the timed
+steps (host wall clock, synchronised on both sides).
+
+Also reported:
+  roofline      the dominant kernel's algorithmic bytes per launch / its
+                average HIP-event duration in the timed steps, vs 8 TB/s;
+                `traffic` is its measured HBM bytes per launch from the
+                committed rocprofv3 --pmc summary, when one exists;
+  cpu_baseline  the CPU oracle's restatement of the reference's DRAM-mode
+                join (same algorithm, 11 radix bits, one pass) timed on this
+                host's cores on a bounded sample of the same workload.
+
+Prints exactly one JSON line on stdout; everything else goes to stderr.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "hashjoin-scm_amd"))
+
+METRIC = "(|R|+|S|) tuples/s + % HBM peak, 16M⋈256M; batched joins/s & p99 vs plain"
+PEAK_GBPS = 8000.0
+CONFIGS = {
+    "C1": {"algo": "nphj_sc", "workload": "pkfk", "nr": 1 << 20, "ns": 1 << 24, "width": 16, "theta": 1.0},
+    "C2": {"algo": "phj_radix_bc", "workload": "pkfk", "nr": 1 << 24, "ns": 1 << 28, "width": 16, "theta": 1.0},
+    "C3": {"algo": "phj_radix_bc", "workload": "zipf", "nr": 1 << 24, "ns": 1 << 27, "width": 64, "theta": 1.0},
+}
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def kernel_bytes(name, W, nr, ns):
+    """Algorithmic HBM bytes one launch of `name` must move (no histogram
+    passes, table zeroing or partial lines): a radix scatter reads and writes
+    every tuple once, the radix join reads every tuple once, an NPHJ build
+    reads R and writes a 16-byte entry per tuple, a probe reads S and one
+    16-byte entry per probe."""
+    N = nr + ns
+    return {
+        "radix_scatter": 2 * W * N,
+        "radix_join": W * N,
+        "lp_build": (W + 16) * nr,
+        "sc_build": (W + 16) * nr,
+        "lp_probe": (W + 16) * ns,
+        "sc_probe": (W + 16) * ns,
+    }.get(name)
+
+
+def load_traffic(kernel, workload_tag):
+    p = os.path.join(ROOT, "profiles", f"{kernel}_{workload_tag}_traffic.json")
+    if os.path.exists(p):
+        try:
+            return json.load(open(p)).get("hbm_bytes_per_launch")
+        except Exception:
+            return None
+    return None
+
+
+def cpu_baseline(cfg, algo):
+    """Time the oracle's restatement of the reference join on host cores on a
+    1/4-scale sample of the workload (|R|/4 x |S|/4), 3 runs, first dropped."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    threads = min(16, os.cpu_count() or 1)
+    nr, ns = cfg["nr"] // 4, cfg["ns"] // 4
+    R, S = O.generate(cfg["workload"], nr, ns, cfg["width"], theta=cfg["theta"], threads=threads)
+    base_algo = algo.replace("_bw_reg", "")
+    times = []
+    for _ in range(3):
+        r = O.join(base_algo, R, S, cfg["width"], threads=threads, radix_bits=11, passes=1)
+        times.append(r["total_time"])
+    t = sum(times[1:]) / len(times[1:])
+    import platform
+    model = ""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"value": (nr + ns) / t, "unit": "tuples/s", "cores": threads, "kind": "port",
+            "sample": f"{base_algo} restated (oracle/hjoracle.c), {nr}x{ns} {cfg['width']}B {cfg['workload']}, "
+                      f"11 radix bits 1 pass, mean of runs 2-3 of 3, {threads} threads on {model or platform.machine()}"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--config", default="C2", choices=sorted(CONFIGS))
+    ap.add_argument("--algo", default=None)
+    ap.add_argument("--bits", type=int, default=0)
+    ap.add_argument("--passes", type=int, default=0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        log("bench.py: this build measures one GPU; multi-GPU sharding is a design note "
+            "(see the sharding paragraph of the design document), so --gpus must be 1")
+        sys.exit(2)
+    import hjmi
+
+    cfg = dict(CONFIGS[args.config])
+    algo = args.algo or cfg["algo"]
+    nr, ns, W = cfg["nr"], cfg["ns"], cfg["width"]
+    ctx = hjmi.Context(0)
+    info = ctx.device_info()
+    log(f"device {info}")
+    R, S = ctx.generate(cfg["workload"], nr, ns, W, theta=cfg["theta"])
+    for _ in range(args.warmup):
+        ref = ctx.join(algo, R, S, radix_bits=args.bits, passes=args.passes)
+    ctx.sync()
+    t0 = time.perf_counter()
+    res = []
+    for _ in range(args.steps):
+        res.append(ctx.join(algo, R, S, radix_bits=args.bits, passes=args.passes, kernel_times=True))
+    ctx.sync()
+    elapsed = time.perf_counter() - t0
+    for r in res:
+        if (r["matched_cnt"], r["check_sum"]) != (res[0]["matched_cnt"], res[0]["check_sum"]):
+            raise SystemExit("results differ between steps")
+    value = (nr + ns) * args.steps / elapsed
+    ms_per_step = elapsed / args.steps * 1e3
+
+    # per-kernel device time over the timed steps
+    tot, cnt = {}, {}
+    for r in res:
+        for k, lst in r["kernel_ms"].items():
+            tot[k] = tot.get(k, 0.0) + sum(lst)
+            cnt[k] = cnt.get(k, 0) + len(lst)
+    dom = max((k for k in tot if kernel_bytes(k, W, nr, ns)), key=lambda k: tot[k])
+    avg_ms = tot[dom] / cnt[dom]
+    achieved = kernel_bytes(dom, W, nr, ns) / (avg_ms * 1e-3) / 1e9
+    job_s = sum(r["total_time"] for r in res) / len(res)
+    job_gbps = res[0]["algo_bytes"] / job_s / 1e9
+    tag = f"{args.config}_{algo}"
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_GBPS, "unit": "GB/s",
+                "frac": round(achieved / PEAK_GBPS, 4), "traffic": load_traffic(dom, tag),
+                "kernel": dom, "kernel_avg_ms": round(avg_ms, 4),
+                "job_algo_bytes": res[0]["algo_bytes"], "job_device_ms": round(job_s * 1e3, 4),
+                "job_frac": round(job_gbps / PEAK_GBPS, 4),
+                "kernel_ms_per_step": {k: round(v / args.steps, 4) for k, v in sorted(tot.items())}}
+    cpu = None
+    if not args.no_cpu_baseline:
+        try:
+            cpu = cpu_baseline(cfg, algo)
+        except Exception as e:  # reported, never fatal for the GPU number
+            log(f"cpu baseline failed: {e}")
+    out = {
+        "metric": METRIC, "value": value, "unit": "tuples/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "u64", "data": "synthetic (seeded on device)",
+        "config": {"workload": f"{args.config}: {algo} |R|={nr} |S|={ns} {W}B {cfg['workload']}",
+                   "radix_bits": res[0]["radix_bits"], "passes": res[0]["passes"],
+                   "matched_cnt": res[0]["matched_cnt"], "check_sum": res[0]["check_sum"],
+                   "device": info["arch"], "cu": info["cu_count"]},
+        "roofline": roofline, "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,105 @@
+// hj_device.h -- device-side helpers: tuple access, wave and block reductions.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define HJ_WAVE 64
+
+// A tuple is WV 16-byte words; word 0 holds {key, payload}.
+template <int WV>
+struct hj_tuple {
+	uint4 w[WV];
+};
+
+__device__ __forceinline__ uint64_t hj_key_of(const uint4 &w0) {
+	return (uint64_t)w0.x | ((uint64_t)w0.y << 32);
+}
+__device__ __forceinline__ uint64_t hj_pay_of(const uint4 &w0) {
+	return (uint64_t)w0.z | ((uint64_t)w0.w << 32);
+}
+
+// Plain (cached) 16-byte load of word 0 of tuple i.
+template <int WV>
+__device__ __forceinline__ uint4 hj_load_head(const uint4 *__restrict__ base, uint64_t i) {
+	return base[i * WV];
+}
+
+typedef unsigned int hj_u32x4 __attribute__((ext_vector_type(4)));
+
+// Streaming load hint for data read exactly once.
+__device__ __forceinline__ uint4 hj_load_stream(const uint4 *p) {
+	const hj_u32x4 v = __builtin_nontemporal_load((const hj_u32x4 *)p);
+	return make_uint4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ void hj_store_stream(uint4 *p, const uint4 &v) {
+	hj_u32x4 t = {v.x, v.y, v.z, v.w};
+	__builtin_nontemporal_store(t, (hj_u32x4 *)p);
+}
+
+__device__ __forceinline__ unsigned long long hj_wave_sum(unsigned long long v) {
+#pragma unroll
+	for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, HJ_WAVE);
+	return v;
+}
+
+// Reduce (a, b) over the block and add the totals to dst[0], dst[1] with one
+// 64-bit atomic each (integer sums mod 2^64: any order gives the same bits).
+// All threads of the block must call it; `scratch` holds 2 * waves entries.
+__device__ __forceinline__ void hj_block_add2(unsigned long long a, unsigned long long b,
+                                              unsigned long long *scratch,
+                                              unsigned long long *dst0, unsigned long long *dst1) {
+	a = hj_wave_sum(a);
+	b = hj_wave_sum(b);
+	const int lane = threadIdx.x & (HJ_WAVE - 1);
+	const int wave = threadIdx.x / HJ_WAVE;
+	const int nw = (blockDim.x + HJ_WAVE - 1) / HJ_WAVE;
+	__syncthreads();
+	if (lane == 0) {
+		scratch[2 * wave] = a;
+		scratch[2 * wave + 1] = b;
+	}
+	__syncthreads();
+	if (threadIdx.x == 0) {
+		unsigned long long ta = 0, tb = 0;
+		for (int w = 0; w < nw; w++) {
+			ta += scratch[2 * w];
+			tb += scratch[2 * w + 1];
+		}
+		if (ta) atomicAdd(dst0, ta);
+		if (tb) atomicAdd(dst1, tb);
+	}
+	__syncthreads();
+}
+
+// Exclusive scan of `v` (one value per thread) across the block; returns the
+// exclusive prefix and writes the block total to *total.  `scratch` holds
+// (waves + 1) entries.  All threads must call it.
+template <typename T>
+__device__ __forceinline__ T hj_block_exscan(T v, T *scratch, T *total) {
+	const int lane = threadIdx.x & (HJ_WAVE - 1);
+	const int wave = threadIdx.x / HJ_WAVE;
+	const int nw = (blockDim.x + HJ_WAVE - 1) / HJ_WAVE;
+	T incl = v;
+#pragma unroll
+	for (int off = 1; off < HJ_WAVE; off <<= 1) {
+		T t = __shfl_up(incl, off, HJ_WAVE);
+		if (lane >= off) incl += t;
+	}
+	if (lane == HJ_WAVE - 1) scratch[wave] = incl;
+	__syncthreads();
+	if (threadIdx.x == 0) {
+		T run = 0;
+		for (int w = 0; w < nw; w++) {
+			T t = scratch[w];
+			scratch[w] = run;
+			run += t;
+		}
+		scratch[nw] = run;
+	}
+	__syncthreads();
+	T res = scratch[wave] + incl - v;
+	*total = scratch[nw];
+	__syncthreads();
+	return res;
+}

@@ -1,0 +1,122 @@
+// hj_internal.h -- library-private declarations shared by the HIP sources.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stddef.h>
+#include <string>
+#include <mutex>
+
+#include "hjmi.h"
+
+// Kernel ids for per-kernel timing (hj_kernel_name maps them to text).
+enum hj_kid : uint32_t {
+	HJ_K_FILL = 0,
+	HJ_K_LP_BUILD,
+	HJ_K_LP_PROBE,
+	HJ_K_LP_FIXUP,
+	HJ_K_SC_BUILD,
+	HJ_K_SC_PROBE,
+	HJ_K_HIST,
+	HJ_K_SCAN,
+	HJ_K_SCATTER,
+	HJ_K_PLAN,
+	HJ_K_JOIN,
+	HJ_K_COUNT
+};
+
+// One growable device buffer of the context's scratch arena.
+struct hj_buf {
+	void *p = nullptr;
+	size_t cap = 0;
+};
+
+struct hj_ctx {
+	int device = 0;
+	hipStream_t stream = nullptr;
+	hipDeviceProp_t prop{};
+	int num_cu = 0;
+	std::mutex mu;
+	std::string last_error;
+
+	// scratch arena: grown on demand, kept across calls so repeated joins do
+	// no allocation inside their timed region
+	hj_buf table;      // NPHJ tables
+	hj_buf pool;       // NPHJ-SC overflow nodes
+	hj_buf aux;        // NPHJ-LP duplicate fix-up indices
+	hj_buf part[2];    // radix partition outputs (pass 1, pass 2)
+	hj_buf hist;       // radix histogram matrix / scan
+	hj_buf meta;       // segments, chunk maps, bounds, work lists
+	hj_buf scan_tmp;   // scan partials
+	hj_buf stats;      // device counters and results
+	hj_buf hostcopy[2];// device copies of host-resident relations
+
+	// timing
+	static const int NEV = 2 * HJ_MAX_KTIMES + 4;
+	hipEvent_t ev[NEV] = {};
+	bool kernel_times = false;
+	int n_kt = 0;
+	uint32_t kt_id[HJ_MAX_KTIMES] = {};
+};
+
+// Set ctx->last_error and return `code`.
+int hj_fail(hj_ctx_t *ctx, int code, const char *fmt, ...);
+// Ensure `b` holds at least `bytes`; returns HJ_OK or HJ_ENOMEM.
+int hj_reserve(hj_ctx_t *ctx, hj_buf &b, size_t bytes);
+
+#define HJ_HIP(ctx, call)                                                                   \
+	do {                                                                                    \
+		hipError_t e_ = (call);                                                             \
+		if (e_ != hipSuccess)                                                               \
+			return hj_fail((ctx), HJ_EHIP, "%s failed: %s (%s:%d)", #call,                  \
+			               hipGetErrorString(e_), __FILE__, __LINE__);                       \
+	} while (0)
+
+// Per-kernel timing: bracket a launch with kt_begin/kt_end.
+void hj_kt_begin(hj_ctx_t *ctx, uint32_t kid);
+void hj_kt_end(hj_ctx_t *ctx);
+
+// Device counters block (lives in ctx->stats).
+struct hj_stats {
+	unsigned long long matched;    // match count
+	unsigned long long checksum;   // payload checksum mod 2^64
+	unsigned int flags;            // HJ_DF_* device-detected conditions
+	unsigned int pad0;
+	unsigned long long pool_next;  // NPHJ-SC overflow node cursor
+	unsigned long long work_next;  // persistent-kernel work cursor
+	unsigned long long sent_cnt;   // LP: R tuples whose key is the empty marker
+	unsigned long long sent_min;   // LP: smallest R row index with that key
+	unsigned long long sent_sum;   // payload of that row
+	unsigned long long total_work; // work items produced by the planner
+	unsigned long long total_chunks;
+	unsigned long long pad1[6];
+};
+
+#define HJ_DF_DUPKEY 1u       // duplicate build key met in an LP table
+#define HJ_DF_POOL_FULL 2u    // SC overflow pool exhausted
+#define HJ_DF_OVERSIZE 4u     // LP partition larger than its LDS table
+#define HJ_DF_TABLE_FULL 8u   // LP global table full on insert
+
+// ---- launchers (each returns HJ_OK or an error code) -----------------------
+int hj_launch_generate(hj_ctx_t *ctx, const hj_gen_spec_t *spec, void *r, void *s);
+
+int hj_run_nphj_lp(hj_ctx_t *ctx, const hj_relation_t *R, const hj_relation_t *S,
+                   const hj_params_t *prm, hj_result_t *out);
+int hj_run_nphj_sc(hj_ctx_t *ctx, const hj_relation_t *R, const hj_relation_t *S,
+                   const hj_params_t *prm, hj_result_t *out);
+int hj_run_radix(hj_ctx_t *ctx, const hj_relation_t *R, const hj_relation_t *S,
+                 const hj_params_t *prm, bool lp, hj_result_t *out);
+
+// Time between two recorded events of ctx->ev, in seconds.
+double hj_event_seconds(hj_ctx_t *ctx, int a, int b);
+
+static inline uint64_t hj_next_pow2(uint64_t x) {
+	uint64_t p = 1;
+	while (p < x) p <<= 1;
+	return p;
+}
+static inline uint32_t hj_log2_ceil(uint64_t x) {
+	uint32_t b = 0;
+	while ((1ULL << b) < x) b++;
+	return b;
+}

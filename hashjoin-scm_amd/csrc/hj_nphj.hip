@@ -1,0 +1,475 @@
+// hj_nphj.hip -- non-partitioned hash joins on one global table in HBM.
+//
+//  nphj_lp  (reference: src/algo/nphj.c:1056-1269 build/probe, :1487-1848
+//           driver).  Open addressing with linear probing over
+//           next_pow_2(|R|) slots, identity hash key & mask.  The reference
+//           claims a slot with a CAS on a separate hashkey word and then
+//           stores the 16-byte tuple (32-byte entries); here a slot is the
+//           16-byte {key, payload} pair and the claim is one 64-bit
+//           atomicCAS of the key word against an empty marker (all ones).
+//           R tuples whose key equals the marker are counted on the side.
+//           Probes stop at the first match, as lp_lookup_with_pos does
+//           (:1162-1165); a miss adds nothing (the reference dereferences
+//           NULL there, :1253).  Duplicate build keys keep one slot per key
+//           holding the payload of the smallest R row index (the match a
+//           sequential build would find first); they are fixed up in two
+//           extra passes that run only when the build saw a duplicate.
+//
+//  nphj_sc  (reference: src/algo/nphj.c:132-442, :677-1037).  Separate
+//           chaining.  The reference uses 2-tuple buckets with a spinlock
+//           and per-thread overflow buffers; here a bucket is one 64-byte
+//           line {3 tuples, u32 count, u32 next}: slots are claimed with an
+//           atomicAdd on count, overflow nodes come from a pool cursor and
+//           are pushed wait-free with atomicExch on next.  Probes compare
+//           every occupied slot along the chain (all matches count).
+//
+// Both probe kernels read S with coalesced 16-byte loads, keep U tuples in
+// flight per lane, and reduce count/checksum with wave shuffles + one 64-bit
+// atomic per block.
+
+#include "hj_device.h"
+#include "hj_internal.h"
+
+#define LP_EMPTY 0xFFFFFFFFFFFFFFFFULL
+#define NPHJ_THREADS 256
+#define PROBE_U 4
+
+// ---------------------------------------------------------------- utilities
+__global__ void k_fill_u4(uint4 *p, uint64_t n, uint4 v) {
+	const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+	for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = v;
+}
+
+__global__ void k_stats_reset(hj_stats *st) {
+	unsigned long long *p = (unsigned long long *)st;
+	for (unsigned i = threadIdx.x; i < sizeof(hj_stats) / 8; i += blockDim.x) p[i] = 0;
+	if (threadIdx.x == 0) st->sent_min = ~0ULL;
+}
+
+static unsigned grid_for(hj_ctx_t *ctx, uint64_t n, unsigned threads, unsigned per_cu = 16) {
+	uint64_t b = (n + threads - 1) / threads;
+	uint64_t cap = (uint64_t)ctx->num_cu * per_cu;
+	if (b > cap) b = cap;
+	if (b == 0) b = 1;
+	return (unsigned)b;
+}
+
+int hj_fill16(hj_ctx_t *ctx, void *p, uint64_t n16, uint4 v) {
+	if (!n16) return HJ_OK;
+	hipLaunchKernelGGL(k_fill_u4, dim3(grid_for(ctx, n16, 256, 32)), dim3(256), 0, ctx->stream, (uint4 *)p, n16, v);
+	return HJ_OK;
+}
+
+int hj_stats_reset(hj_ctx_t *ctx) {
+	hipLaunchKernelGGL(k_stats_reset, dim3(1), dim3(64), 0, ctx->stream, (hj_stats *)ctx->stats.p);
+	return HJ_OK;
+}
+
+int hj_stats_read(hj_ctx_t *ctx, hj_stats *h) {
+	HJ_HIP(ctx, hipMemcpyAsync(h, ctx->stats.p, sizeof *h, hipMemcpyDeviceToHost, ctx->stream));
+	HJ_HIP(ctx, hipStreamSynchronize(ctx->stream));
+	return HJ_OK;
+}
+
+// ------------------------------------------------------------------ nphj_lp
+template <int WV>
+__global__ __launch_bounds__(NPHJ_THREADS) void k_lp_build(const uint4 *__restrict__ R, uint64_t n,
+                                                           unsigned long long *__restrict__ tab, uint64_t mask,
+                                                           hj_stats *st) {
+	const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+	bool dup = false, full = false;
+	for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+		const uint4 t = R[i * WV];
+		const uint64_t key = hj_key_of(t), pay = hj_pay_of(t);
+		if (key == LP_EMPTY) {
+			atomicAdd(&st->sent_cnt, 1ULL);
+			atomicMin(&st->sent_min, (unsigned long long)i);
+			continue;
+		}
+		uint64_t pos = key & mask;
+		uint64_t step = 0;
+		for (; step <= mask; step++) {
+			const unsigned long long old = atomicCAS(&tab[2 * pos], LP_EMPTY, key);
+			if (old == LP_EMPTY) {
+				tab[2 * pos + 1] = pay;
+				break;
+			}
+			if (old == key) {
+				dup = true;
+				break;
+			}
+			pos = (pos + 1) & mask;
+		}
+		if (step > mask) full = true;
+	}
+	// one flag write per wave instead of one per duplicate
+	if (__any(dup) && (threadIdx.x & (HJ_WAVE - 1)) == 0) atomicOr(&st->flags, HJ_DF_DUPKEY);
+	if (__any(full) && (threadIdx.x & (HJ_WAVE - 1)) == 0) atomicOr(&st->flags, HJ_DF_TABLE_FULL);
+}
+
+// slot holding `key` (it must be present); used by the duplicate fix-up
+__device__ __forceinline__ uint64_t lp_find(const unsigned long long *tab, uint64_t mask, uint64_t key) {
+	uint64_t pos = key & mask;
+	for (uint64_t step = 0; step <= mask; step++) {
+		if (tab[2 * pos] == key) return pos;
+		pos = (pos + 1) & mask;
+	}
+	return ~0ULL;
+}
+
+template <int WV>
+__global__ __launch_bounds__(NPHJ_THREADS) void k_lp_minidx(const uint4 *__restrict__ R, uint64_t n,
+                                                            const unsigned long long *__restrict__ tab, uint64_t mask,
+                                                            unsigned long long *__restrict__ minidx) {
+	const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+	for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+		const uint64_t key = hj_key_of(R[i * WV]);
+		if (key == LP_EMPTY) continue;
+		const uint64_t pos = lp_find(tab, mask, key);
+		if (pos != ~0ULL) atomicMin(&minidx[pos], (unsigned long long)i);
+	}
+}
+
+template <int WV>
+__global__ __launch_bounds__(NPHJ_THREADS) void k_lp_setpay(const uint4 *__restrict__ R, uint64_t n,
+                                                            unsigned long long *__restrict__ tab, uint64_t mask,
+                                                            const unsigned long long *__restrict__ minidx) {
+	const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+	for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+		const uint4 t = R[i * WV];
+		const uint64_t key = hj_key_of(t);
+		if (key == LP_EMPTY) continue;
+		const uint64_t pos = lp_find(tab, mask, key);
+		if (pos != ~0ULL && minidx[pos] == i) tab[2 * pos + 1] = hj_pay_of(t);
+	}
+}
+
+template <int WV>
+__global__ __launch_bounds__(NPHJ_THREADS) void k_lp_probe(const uint4 *__restrict__ S, uint64_t n,
+                                                           const ulonglong2 *__restrict__ tab, uint64_t mask,
+                                                           const uint4 *__restrict__ R, hj_stats *st) {
+	__shared__ unsigned long long red[2 * (NPHJ_THREADS / HJ_WAVE)];
+	unsigned long long cnt = 0, sum = 0;
+	const uint64_t sent_cnt = st->sent_cnt;
+	const uint64_t sent_pay = sent_cnt ? hj_pay_of(R[st->sent_min * WV]) : 0;
+	const uint64_t tile = (uint64_t)blockDim.x * PROBE_U;
+	for (uint64_t base = (uint64_t)blockIdx.x * tile; base < n; base += (uint64_t)gridDim.x * tile) {
+		uint64_t key[PROBE_U], pay[PROBE_U], pos[PROBE_U];
+		ulonglong2 e[PROBE_U];
+		bool live[PROBE_U];
+#pragma unroll
+		for (int u = 0; u < PROBE_U; u++) {
+			const uint64_t i = base + (uint64_t)u * blockDim.x + threadIdx.x;
+			live[u] = i < n;
+			uint4 t = live[u] ? hj_load_stream(&S[i * WV]) : make_uint4(0, 0, 0, 0);
+			key[u] = hj_key_of(t);
+			pay[u] = hj_pay_of(t);
+		}
+#pragma unroll
+		for (int u = 0; u < PROBE_U; u++) {
+			pos[u] = key[u] & mask;
+			if (live[u] && key[u] != LP_EMPTY) e[u] = tab[pos[u]];
+		}
+#pragma unroll
+		for (int u = 0; u < PROBE_U; u++) {
+			if (!live[u]) continue;
+			if (key[u] == LP_EMPTY) {
+				if (sent_cnt) {
+					cnt++;
+					sum += sent_pay + pay[u];
+				}
+				continue;
+			}
+			for (uint64_t step = 0; step <= mask; step++) {
+				if (e[u].x == key[u]) {
+					cnt++;
+					sum += e[u].y + pay[u];
+					break;
+				}
+				if (e[u].x == LP_EMPTY) break;
+				pos[u] = (pos[u] + 1) & mask;
+				e[u] = tab[pos[u]];
+			}
+		}
+	}
+	hj_block_add2(cnt, sum, red, &st->matched, &st->checksum);
+}
+
+// Build (and, if duplicates were seen, fix up) the LP table for R.
+// `tab` has `slots` 16-byte slots.  Returns device flags through *flags.
+static int lp_build_table(hj_ctx_t *ctx, const hj_relation_t *R, unsigned long long *tab, uint64_t slots,
+                          hj_result_t *out) {
+	const uint64_t mask = slots - 1;
+	const uint32_t wv = R->tuple_size / 16;
+	hj_stats *st = (hj_stats *)ctx->stats.p;
+	const uint64_t n = R->tuple_num;
+	const unsigned g = grid_for(ctx, n, NPHJ_THREADS);
+	hj_kt_begin(ctx, HJ_K_LP_BUILD);
+#define LP_B(W) hipLaunchKernelGGL(k_lp_build<W>, dim3(g), dim3(NPHJ_THREADS), 0, ctx->stream, (const uint4 *)R->tuples, n, tab, mask, st)
+	switch (wv) {
+	case 1: LP_B(1); break;
+	case 2: LP_B(2); break;
+	case 4: LP_B(4); break;
+	case 8: LP_B(8); break;
+	default: LP_B(16); break;
+	}
+#undef LP_B
+	hj_kt_end(ctx);
+	HJ_HIP(ctx, hipGetLastError());
+	// The duplicate flag is read back only to choose the fix-up path; PK
+	// builds (unique keys) never take it.
+	unsigned int flags = 0;
+	HJ_HIP(ctx, hipMemcpyAsync(&flags, &st->flags, sizeof flags, hipMemcpyDeviceToHost, ctx->stream));
+	HJ_HIP(ctx, hipStreamSynchronize(ctx->stream));
+	if (flags & HJ_DF_TABLE_FULL) return hj_fail(ctx, HJ_EINVAL, "nphj_lp table full: |R| exceeds slots");
+	if (flags & HJ_DF_DUPKEY) {
+		int rc = hj_reserve(ctx, ctx->aux, slots * 8);
+		if (rc) return rc;
+		hj_fill16(ctx, ctx->aux.p, (slots + 1) / 2, make_uint4(~0u, ~0u, ~0u, ~0u));
+		hj_kt_begin(ctx, HJ_K_LP_FIXUP);
+#define LP_F(W)                                                                                                  \
+	hipLaunchKernelGGL(k_lp_minidx<W>, dim3(g), dim3(NPHJ_THREADS), 0, ctx->stream, (const uint4 *)R->tuples, n, tab, \
+	                   mask, (unsigned long long *)ctx->aux.p);                                                  \
+	hipLaunchKernelGGL(k_lp_setpay<W>, dim3(g), dim3(NPHJ_THREADS), 0, ctx->stream, (const uint4 *)R->tuples, n, tab, \
+	                   mask, (const unsigned long long *)ctx->aux.p)
+		switch (wv) {
+		case 1: LP_F(1); break;
+		case 2: LP_F(2); break;
+		case 4: LP_F(4); break;
+		case 8: LP_F(8); break;
+		default: LP_F(16); break;
+		}
+#undef LP_F
+		hj_kt_end(ctx);
+		HJ_HIP(ctx, hipGetLastError());
+		out->fallback |= HJ_FB_LP_DUPLICATES;
+	}
+	return HJ_OK;
+}
+
+int hj_run_nphj_lp(hj_ctx_t *ctx, const hj_relation_t *R, const hj_relation_t *S, const hj_params_t *prm,
+                   hj_result_t *out) {
+	uint64_t slots = prm->lp_slots_log2 ? (1ULL << prm->lp_slots_log2) : hj_next_pow2(R->tuple_num ? R->tuple_num : 1);
+	if (slots < R->tuple_num) return hj_fail(ctx, HJ_EINVAL, "lp_slots_log2 too small for |R|");
+	int rc = hj_reserve(ctx, ctx->table, slots * 16);
+	if (rc) return rc;
+	unsigned long long *tab = (unsigned long long *)ctx->table.p;
+	hj_stats *st = (hj_stats *)ctx->stats.p;
+
+	HJ_HIP(ctx, hipEventRecord(ctx->ev[0], ctx->stream));
+	hj_stats_reset(ctx);
+	hj_kt_begin(ctx, HJ_K_FILL);
+	hj_fill16(ctx, tab, slots, make_uint4(~0u, ~0u, 0u, 0u));
+	hj_kt_end(ctx);
+	HJ_HIP(ctx, hipEventRecord(ctx->ev[1], ctx->stream));
+	rc = lp_build_table(ctx, R, tab, slots, out);
+	if (rc) return rc;
+	HJ_HIP(ctx, hipEventRecord(ctx->ev[2], ctx->stream));
+
+	const uint32_t wv = S->tuple_size / 16;
+	const uint64_t n = S->tuple_num;
+	const unsigned g = grid_for(ctx, (n + PROBE_U - 1) / PROBE_U, NPHJ_THREADS, 8);
+	hj_kt_begin(ctx, HJ_K_LP_PROBE);
+#define LP_P(W)                                                                                                  \
+	hipLaunchKernelGGL(k_lp_probe<W>, dim3(g), dim3(NPHJ_THREADS), 0, ctx->stream, (const uint4 *)S->tuples, n, \
+	                   (const ulonglong2 *)tab, slots - 1, (const uint4 *)R->tuples, st)
+	switch (wv) {
+	case 1: LP_P(1); break;
+	case 2: LP_P(2); break;
+	case 4: LP_P(4); break;
+	case 8: LP_P(8); break;
+	default: LP_P(16); break;
+	}
+#undef LP_P
+	hj_kt_end(ctx);
+	HJ_HIP(ctx, hipGetLastError());
+	HJ_HIP(ctx, hipEventRecord(ctx->ev[3], ctx->stream));
+	hj_stats h;
+	rc = hj_stats_read(ctx, &h);
+	if (rc) return rc;
+	out->matched_cnt = h.matched;
+	out->check_sum = h.checksum;
+	out->buildpart_time = hj_event_seconds(ctx, 1, 2);
+	out->probejoin_time = hj_event_seconds(ctx, 2, 3);
+	out->total_time = hj_event_seconds(ctx, 0, 3);
+	out->algo_bytes = (uint64_t)(R->tuple_size + 16) * (R->tuple_num + S->tuple_num);
+	return HJ_OK;
+}
+
+// ------------------------------------------------------------------ nphj_sc
+// 64-byte bucket / overflow node: slots at u64 words 0..5 (key, payload)
+// x3, u32 count at u32 word 12, u32 next (node index + 1) at u32 word 13.
+#define SC_SLOTS 3
+
+struct __align__(64) sc_node {
+	unsigned long long kv[2 * SC_SLOTS];
+	unsigned int count;
+	unsigned int next;
+	unsigned long long pad;
+};
+
+template <int WV>
+__global__ __launch_bounds__(NPHJ_THREADS) void k_sc_build(const uint4 *__restrict__ R, uint64_t n,
+                                                           sc_node *__restrict__ tab, uint64_t mask,
+                                                           sc_node *__restrict__ pool, uint64_t pool_cap,
+                                                           hj_stats *st) {
+	const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+	bool pool_full = false;
+	for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+		const uint4 t = R[i * WV];
+		const uint64_t key = hj_key_of(t);
+		sc_node *b = &tab[key & mask];
+		const unsigned c = atomicAdd(&b->count, 1u);
+		if (c < SC_SLOTS) {
+			*(uint4 *)&b->kv[2 * c] = t;
+			continue;
+		}
+		// overflow: append to the bucket's first node, or push a new node
+		unsigned head = __hip_atomic_load(&b->next, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+		if (head) {
+			const unsigned c2 = atomicAdd(&pool[head - 1].count, 1u);
+			if (c2 < SC_SLOTS) {
+				*(uint4 *)&pool[head - 1].kv[2 * c2] = t;
+				continue;
+			}
+		}
+		const unsigned long long nn = atomicAdd(&st->pool_next, 1ULL);
+		if (nn >= pool_cap) {
+			pool_full = true;
+			continue;
+		}
+		sc_node *nd = &pool[nn];
+		*(uint4 *)&nd->kv[0] = t;
+		nd->count = 1u;
+		__threadfence();
+		// wait-free push: inserters only ever touch the head node, so the
+		// link to the previous head can be stored after publication; the
+		// probe kernel runs after this kernel has completed
+		nd->next = atomicExch(&b->next, (unsigned)(nn + 1));
+	}
+	if (__any(pool_full) && (threadIdx.x & (HJ_WAVE - 1)) == 0) atomicOr(&st->flags, HJ_DF_POOL_FULL);
+}
+
+__device__ __forceinline__ void sc_match_node(const uint4 *nd, uint64_t key, uint64_t pay, unsigned long long &cnt,
+                                              unsigned long long &sum, unsigned &next) {
+	const uint4 w0 = nd[0], w1 = nd[1], w2 = nd[2], w3 = nd[3];
+	const unsigned c = w3.x < SC_SLOTS ? w3.x : SC_SLOTS;
+	next = w3.y;
+	if (c > 0 && hj_key_of(w0) == key) { cnt++; sum += hj_pay_of(w0) + pay; }
+	if (c > 1 && hj_key_of(w1) == key) { cnt++; sum += hj_pay_of(w1) + pay; }
+	if (c > 2 && hj_key_of(w2) == key) { cnt++; sum += hj_pay_of(w2) + pay; }
+}
+
+template <int WV>
+__global__ __launch_bounds__(NPHJ_THREADS) void k_sc_probe(const uint4 *__restrict__ S, uint64_t n,
+                                                           const sc_node *__restrict__ tab, uint64_t mask,
+                                                           const sc_node *__restrict__ pool, uint64_t pool_cap,
+                                                           hj_stats *st) {
+	__shared__ unsigned long long red[2 * (NPHJ_THREADS / HJ_WAVE)];
+	unsigned long long cnt = 0, sum = 0;
+	const uint64_t tile = (uint64_t)blockDim.x * PROBE_U;
+	for (uint64_t base = (uint64_t)blockIdx.x * tile; base < n; base += (uint64_t)gridDim.x * tile) {
+		uint64_t key[PROBE_U], pay[PROBE_U];
+		bool live[PROBE_U];
+		unsigned next[PROBE_U];
+#pragma unroll
+		for (int u = 0; u < PROBE_U; u++) {
+			const uint64_t i = base + (uint64_t)u * blockDim.x + threadIdx.x;
+			live[u] = i < n;
+			uint4 t = live[u] ? hj_load_stream(&S[i * WV]) : make_uint4(0, 0, 0, 0);
+			key[u] = hj_key_of(t);
+			pay[u] = hj_pay_of(t);
+		}
+#pragma unroll
+		for (int u = 0; u < PROBE_U; u++) {
+			next[u] = 0;
+			if (live[u]) sc_match_node((const uint4 *)&tab[key[u] & mask], key[u], pay[u], cnt, sum, next[u]);
+		}
+#pragma unroll
+		for (int u = 0; u < PROBE_U; u++) {
+			uint64_t hops = 0;
+			while (next[u] && hops++ < pool_cap) {
+				unsigned nx;
+				sc_match_node((const uint4 *)&pool[next[u] - 1], key[u], pay[u], cnt, sum, nx);
+				next[u] = nx;
+			}
+		}
+	}
+	hj_block_add2(cnt, sum, red, &st->matched, &st->checksum);
+}
+
+int hj_run_nphj_sc(hj_ctx_t *ctx, const hj_relation_t *R, const hj_relation_t *S, const hj_params_t *prm,
+                   hj_result_t *out) {
+	(void)prm;
+	// |R|/2 buckets as src/algo/nphj.c:746 (BUCKET_CAPACITY 2), rounded to a
+	// power of two so that key & mask is a valid identity hash.
+	uint64_t nb = hj_next_pow2(R->tuple_num ? R->tuple_num : 1) / 2;
+	if (nb == 0) nb = 1;
+	uint64_t pool_cap = R->tuple_num / 4 + 1024;
+	hj_stats *st = (hj_stats *)ctx->stats.p;
+	const uint32_t wv = R->tuple_size / 16;
+	int rc = hj_reserve(ctx, ctx->table, nb * sizeof(sc_node));
+	if (rc) return rc;
+	for (int attempt = 0; attempt < 2; attempt++) {
+		rc = hj_reserve(ctx, ctx->pool, pool_cap * sizeof(sc_node));
+		if (rc) return rc;
+		sc_node *tab = (sc_node *)ctx->table.p, *pool = (sc_node *)ctx->pool.p;
+		HJ_HIP(ctx, hipEventRecord(ctx->ev[0], ctx->stream));
+		hj_stats_reset(ctx);
+		hj_kt_begin(ctx, HJ_K_FILL);
+		hj_fill16(ctx, tab, nb * 4, make_uint4(0, 0, 0, 0));
+		hj_kt_end(ctx);
+		HJ_HIP(ctx, hipEventRecord(ctx->ev[1], ctx->stream));
+		const unsigned g = grid_for(ctx, R->tuple_num, NPHJ_THREADS);
+		hj_kt_begin(ctx, HJ_K_SC_BUILD);
+#define SC_B(W)                                                                                                   \
+	hipLaunchKernelGGL(k_sc_build<W>, dim3(g), dim3(NPHJ_THREADS), 0, ctx->stream, (const uint4 *)R->tuples, \
+	                   R->tuple_num, tab, nb - 1, pool, pool_cap, st)
+		switch (wv) {
+		case 1: SC_B(1); break;
+		case 2: SC_B(2); break;
+		case 4: SC_B(4); break;
+		case 8: SC_B(8); break;
+		default: SC_B(16); break;
+		}
+#undef SC_B
+		hj_kt_end(ctx);
+		HJ_HIP(ctx, hipGetLastError());
+		HJ_HIP(ctx, hipEventRecord(ctx->ev[2], ctx->stream));
+		const uint64_t n = S->tuple_num;
+		const unsigned gp = grid_for(ctx, (n + PROBE_U - 1) / PROBE_U, NPHJ_THREADS, 8);
+		hj_kt_begin(ctx, HJ_K_SC_PROBE);
+#define SC_P(W)                                                                                                   \
+	hipLaunchKernelGGL(k_sc_probe<W>, dim3(gp), dim3(NPHJ_THREADS), 0, ctx->stream, (const uint4 *)S->tuples, n, \
+	                   (const sc_node *)tab, nb - 1, (const sc_node *)pool, pool_cap, st)
+		switch (wv) {
+		case 1: SC_P(1); break;
+		case 2: SC_P(2); break;
+		case 4: SC_P(4); break;
+		case 8: SC_P(8); break;
+		default: SC_P(16); break;
+		}
+#undef SC_P
+		hj_kt_end(ctx);
+		HJ_HIP(ctx, hipGetLastError());
+		HJ_HIP(ctx, hipEventRecord(ctx->ev[3], ctx->stream));
+		hj_stats h;
+		rc = hj_stats_read(ctx, &h);
+		if (rc) return rc;
+		if (h.flags & HJ_DF_POOL_FULL) {
+			// every node holds at least one tuple, so |R| nodes always suffice
+			pool_cap = R->tuple_num + 1024;
+			out->fallback |= HJ_FB_SC_REBUILD;
+			ctx->n_kt = 0;
+			continue;
+		}
+		out->matched_cnt = h.matched;
+		out->check_sum = h.checksum;
+		out->buildpart_time = hj_event_seconds(ctx, 1, 2);
+		out->probejoin_time = hj_event_seconds(ctx, 2, 3);
+		out->total_time = hj_event_seconds(ctx, 0, 3);
+		out->algo_bytes = (uint64_t)(R->tuple_size + 16) * (R->tuple_num + S->tuple_num);
+		return HJ_OK;
+	}
+	return hj_fail(ctx, HJ_ENOMEM, "nphj_sc overflow pool exhausted after rebuild");
+}

@@ -1,0 +1,705 @@
+// hj_radix.hip -- radix-partitioned hash join (phj_radix_bc / phj_radix_lp).
+//
+// Reference: src/algo/phj.c -- join_init_run (:1875-2512), partitioning
+// (:147-359 one pass, :366-491 second pass), bc_join (:496-571), lp_join
+// (:574-680), task queue src/inc/task_pthread.h.
+//
+// Partitioning (one "multi-split" per pass).  The input of a pass is a list
+// of segments (pass 1: R and S; pass 2: every pass-1 partition), each cut
+// into chunks of CH tuples; one workgroup owns one chunk, the way one
+// reference thread owns one slice (phj.c:2046-2053).
+//   k_hist     per-chunk histogram of the radix digit in LDS, written to a
+//              flat matrix laid out [segment][digit][chunk];
+//   k_scan     one exclusive scan over the flat matrix gives every
+//              (segment, digit, chunk) its absolute output position -- the
+//              reference's cross-thread prefix (phj.c:299-311) for all
+//              segments at once;
+//   k_scatter  re-reads the chunk tile by tile; each tile is counting-sorted
+//              by digit in LDS and written out as contiguous runs per digit,
+//              so a wavefront stores whole lines instead of one tuple per
+//              partition.  This LDS staging replaces the reference's 64-byte
+//              software write-combining buffers + AVX-512 non-temporal
+//              flushes (phj.c:218-254, :314-358).
+// Digit bits follow the reference: pass 1 uses key bits [0, B/P), pass 2
+// bits [B/P, B) (params.h:34-35); the in-partition hash uses key >> B.
+//
+// Join.  A planner turns every final partition pair (R_q, S_q) into tiles of
+// at most s_tile probe tuples; a persistent kernel pulls (partition, tile)
+// items from a device counter -- the GPU form of the reference's LIFO task
+// queue (task_pthread.h:110-133) -- builds the R_q table in LDS and probes
+// the tile.  bc: bucket chains (heads + next, 1-based, head insertion, as
+// bc_join).  lp: linear probing over next_pow_2(|R_q|) slots, first match
+// only (as lp_join).  Count and checksum are wave-reduced and added with one
+// 64-bit atomic per workgroup.
+
+#include "hj_device.h"
+#include "hj_internal.h"
+
+#include <algorithm>
+
+int hj_fill16(hj_ctx_t *ctx, void *p, uint64_t n16, uint4 v);
+int hj_stats_reset(hj_ctx_t *ctx);
+int hj_stats_read(hj_ctx_t *ctx, hj_stats *h);
+
+struct hj_seg {
+	const uint4 *in;     // first input tuple
+	uint64_t len;        // tuples
+	uint64_t out_start;  // absolute output position of the segment
+	uint64_t hist_base;  // flat index of (digit 0, chunk 0)
+	uint32_t chunk_base; // first global chunk id
+	uint32_t nchunks;
+};
+
+struct hj_pair {
+	uint64_t r0, r1, s0, s1;  // tuple ranges in the partitioned buffer
+};
+
+#define SCAN_NT 256
+#define SCAN_ITEMS 16
+#define SCAN_CH (SCAN_NT * SCAN_ITEMS)
+
+// ------------------------------------------------------------------ scans
+// In-place exclusive scan of a[0..L) (u64); a[L] receives the total.
+__global__ __launch_bounds__(SCAN_NT) void k_scan_reduce(const uint64_t *__restrict__ a, uint64_t L,
+                                                         uint64_t *__restrict__ part) {
+	__shared__ uint64_t red[SCAN_NT / HJ_WAVE + 1];
+	const uint64_t base = (uint64_t)blockIdx.x * SCAN_CH;
+	uint64_t s = 0;
+	for (int k = 0; k < SCAN_ITEMS; k++) {
+		uint64_t i = base + (uint64_t)k * SCAN_NT + threadIdx.x;
+		if (i < L) s += a[i];
+	}
+	uint64_t tot;
+	hj_block_exscan<uint64_t>(s, red, &tot);
+	if (threadIdx.x == 0) part[blockIdx.x] = tot;
+}
+
+__global__ __launch_bounds__(1024) void k_scan_top(uint64_t *__restrict__ part, uint32_t nparts,
+                                                   uint64_t *__restrict__ total_out) {
+	__shared__ uint64_t red[1024 / HJ_WAVE + 1];
+	// each thread owns a contiguous run of partials
+	const uint32_t per = (nparts + blockDim.x - 1) / blockDim.x;
+	const uint32_t b0 = threadIdx.x * per;
+	uint64_t s = 0;
+	for (uint32_t k = 0; k < per; k++)
+		if (b0 + k < nparts) s += part[b0 + k];
+	uint64_t tot;
+	uint64_t run = hj_block_exscan<uint64_t>(s, red, &tot);
+	for (uint32_t k = 0; k < per; k++)
+		if (b0 + k < nparts) {
+			uint64_t t = part[b0 + k];
+			part[b0 + k] = run;
+			run += t;
+		}
+	if (threadIdx.x == 0) *total_out = tot;
+}
+
+__global__ __launch_bounds__(SCAN_NT) void k_scan_down(uint64_t *__restrict__ a, uint64_t L,
+                                                       const uint64_t *__restrict__ part) {
+	__shared__ uint64_t red[SCAN_NT / HJ_WAVE + 1];
+	const uint64_t base = (uint64_t)blockIdx.x * SCAN_CH + (uint64_t)threadIdx.x * SCAN_ITEMS;
+	uint64_t v[SCAN_ITEMS];
+	uint64_t s = 0;
+	for (int k = 0; k < SCAN_ITEMS; k++) {
+		uint64_t i = base + k;
+		v[k] = i < L ? a[i] : 0;
+		s += v[k];
+	}
+	uint64_t tot;
+	uint64_t run = hj_block_exscan<uint64_t>(s, red, &tot) + part[blockIdx.x];
+	for (int k = 0; k < SCAN_ITEMS; k++) {
+		uint64_t i = base + k;
+		if (i < L) a[i] = run;
+		run += v[k];
+	}
+}
+
+// Exclusive scan of a[0..L) in place, a[L] = total.  `tmp` holds
+// ceil(L / SCAN_CH) partials (<= 16384 * 1024 entries supported).
+static int scan_u64(hj_ctx_t *ctx, uint64_t *a, uint64_t L, uint64_t *tmp) {
+	uint64_t nb = (L + SCAN_CH - 1) / SCAN_CH;
+	if (nb == 0) nb = 1;
+	if (nb > 1024 * 64) return hj_fail(ctx, HJ_EINVAL, "scan too long (%llu)", (unsigned long long)L);
+	hipLaunchKernelGGL(k_scan_reduce, dim3((unsigned)nb), dim3(SCAN_NT), 0, ctx->stream, a, L, tmp);
+	hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, ctx->stream, tmp, (uint32_t)nb, a + L);
+	hipLaunchKernelGGL(k_scan_down, dim3((unsigned)nb), dim3(SCAN_NT), 0, ctx->stream, a, L, tmp);
+	return HJ_OK;
+}
+
+// ------------------------------------------------------------ segments
+// Pass 1: R and S as two segments; chunk map filled for both.
+__global__ void k_seg_pair(const uint4 *R, uint64_t nr, const uint4 *S, uint64_t ns, uint64_t ch, uint32_t F,
+                           hj_seg *segs, uint32_t *chunk_seg, hj_stats *st) {
+	const uint32_t cr = (uint32_t)((nr + ch - 1) / ch), cs = (uint32_t)((ns + ch - 1) / ch);
+	if (blockIdx.x == 0 && threadIdx.x == 0) {
+		segs[0] = hj_seg{R, nr, 0, 0, 0, cr};
+		segs[1] = hj_seg{S, ns, nr, (uint64_t)cr * F, cr, cs};
+		st->total_chunks = cr + cs;
+	}
+	for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < cr + cs; c += gridDim.x * blockDim.x)
+		chunk_seg[c] = c < cr ? 0u : 1u;
+}
+
+// Partition bounds: pb[s*F + p] = first output position of digit p of
+// segment s; pb[nseg*F] = end of the last segment.
+__global__ void k_bounds(const hj_seg *__restrict__ segs, uint32_t nseg, uint32_t F, const uint64_t *__restrict__ scan,
+                         uint64_t *__restrict__ pb) {
+	const uint64_t total = (uint64_t)nseg * F;
+	for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i <= total;
+	     i += (uint64_t)gridDim.x * blockDim.x) {
+		if (i == total) {
+			const hj_seg &l = segs[nseg - 1];
+			pb[i] = l.out_start + l.len;
+			continue;
+		}
+		const uint32_t s = (uint32_t)(i / F), p = (uint32_t)(i % F);
+		const hj_seg &g = segs[s];
+		pb[i] = g.nchunks ? scan[g.hist_base + (uint64_t)p * g.nchunks] : g.out_start;
+	}
+}
+
+// Pass 2 segments: one per pass-1 partition (s2 = s1*F1 + p1).
+__global__ void k_seg2_len(const uint64_t *__restrict__ pb1, uint32_t nseg2, uint64_t ch, const uint4 *in1,
+                           uint32_t wv, hj_seg *__restrict__ segs2, uint64_t *__restrict__ nch) {
+	for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < nseg2; s += gridDim.x * blockDim.x) {
+		const uint64_t a = pb1[s], b = pb1[s + 1];
+		hj_seg g;
+		g.in = in1 + a * wv;
+		g.len = b - a;
+		g.out_start = a;
+		g.nchunks = (uint32_t)((g.len + ch - 1) / ch);
+		g.chunk_base = 0;
+		g.hist_base = 0;
+		segs2[s] = g;
+		nch[s] = g.nchunks;
+	}
+}
+
+__global__ void k_seg2_fin(const uint64_t *__restrict__ nch_scan, uint32_t nseg2, uint32_t F2,
+                           hj_seg *__restrict__ segs2, uint32_t *__restrict__ chunk_seg, hj_stats *st) {
+	// one workgroup per segment
+	for (uint32_t s = blockIdx.x; s < nseg2; s += gridDim.x) {
+		const uint32_t cb = (uint32_t)nch_scan[s];
+		const uint32_t nc = segs2[s].nchunks;
+		if (threadIdx.x == 0) {
+			segs2[s].chunk_base = cb;
+			segs2[s].hist_base = (uint64_t)cb * F2;
+		}
+		for (uint32_t c = threadIdx.x; c < nc; c += blockDim.x) chunk_seg[cb + c] = s;
+	}
+	if (blockIdx.x == 0 && threadIdx.x == 0) st->total_chunks = nch_scan[nseg2];
+}
+
+// ------------------------------------------------------------ histogram
+template <int WV>
+__global__ __launch_bounds__(512) void k_hist(const hj_seg *__restrict__ segs, const uint32_t *__restrict__ chunk_seg,
+                                              const hj_stats *__restrict__ st, uint64_t ch, uint32_t shift,
+                                              uint32_t bits, uint64_t *__restrict__ hist) {
+	extern __shared__ uint32_t cnt[];
+	const uint32_t c = blockIdx.x;
+	if (c >= st->total_chunks) return;
+	const uint32_t F = 1u << bits, fmask = F - 1;
+	const hj_seg g = segs[chunk_seg[c]];
+	const uint32_t cl = c - g.chunk_base;
+	const uint64_t b = (uint64_t)cl * ch, e = min(g.len, b + ch);
+	for (uint32_t p = threadIdx.x; p < F; p += blockDim.x) cnt[p] = 0;
+	__syncthreads();
+	const uint64_t *keys = (const uint64_t *)g.in;
+	constexpr int U = 8;
+	for (uint64_t i0 = b + threadIdx.x; i0 < e; i0 += (uint64_t)U * blockDim.x) {
+		uint64_t k[U];
+#pragma unroll
+		for (int u = 0; u < U; u++) {
+			const uint64_t i = i0 + (uint64_t)u * blockDim.x;
+			k[u] = i < e ? __builtin_nontemporal_load(&keys[i * 2 * WV]) : 0;
+		}
+#pragma unroll
+		for (int u = 0; u < U; u++) {
+			const uint64_t i = i0 + (uint64_t)u * blockDim.x;
+			if (i < e) atomicAdd(&cnt[(uint32_t)(k[u] >> shift) & fmask], 1u);
+		}
+	}
+	__syncthreads();
+	for (uint32_t p = threadIdx.x; p < F; p += blockDim.x) hist[g.hist_base + (uint64_t)p * g.nchunks + cl] = cnt[p];
+}
+
+// ------------------------------------------------------------ scatter
+// LDS: tile[TILE*WV] uint4 | tcnt[F] u32 | tbase[F] u32 | cursor[F] u64 |
+//      pid[TILE] u16 | scan scratch
+template <int WV, int NT, int ITEMS>
+__global__ __launch_bounds__(NT) void k_scatter(const hj_seg *__restrict__ segs,
+                                                const uint32_t *__restrict__ chunk_seg,
+                                                const hj_stats *__restrict__ st, uint64_t ch, uint32_t shift,
+                                                uint32_t bits, const uint64_t *__restrict__ scan,
+                                                uint4 *__restrict__ out) {
+	constexpr int TILE = NT * ITEMS;
+	extern __shared__ uint4 lds_u4[];
+	const uint32_t F = 1u << bits, fmask = F - 1;
+	uint4 *tile = lds_u4;
+	uint64_t *cursor = (uint64_t *)(tile + TILE * WV);
+	uint32_t *tcnt = (uint32_t *)(cursor + F);
+	uint32_t *tbase = tcnt + F;
+	uint16_t *pid = (uint16_t *)(tbase + F);
+	uint32_t *red = (uint32_t *)(pid + TILE + 2);
+
+	const uint32_t c = blockIdx.x;
+	if (c >= st->total_chunks) return;
+	const hj_seg g = segs[chunk_seg[c]];
+	const uint32_t cl = c - g.chunk_base;
+	const uint64_t b = (uint64_t)cl * ch, e = min(g.len, b + ch);
+	for (uint32_t p = threadIdx.x; p < F; p += NT) {
+		cursor[p] = scan[g.hist_base + (uint64_t)p * g.nchunks + cl];
+		tcnt[p] = 0;
+	}
+	__syncthreads();
+	const uint32_t per = (F + NT - 1) / NT;  // digits per thread in the tile scan
+	for (uint64_t t0 = b; t0 < e; t0 += TILE) {
+		const uint32_t n = (uint32_t)min((uint64_t)TILE, e - t0);
+		uint4 v[ITEMS][WV];
+		uint32_t dig[ITEMS], rank[ITEMS];
+#pragma unroll
+		for (int k = 0; k < ITEMS; k++) {
+			const uint32_t idx = k * NT + threadIdx.x;
+			if (idx < n) {
+				const uint4 *src = g.in + (t0 + idx) * WV;
+#pragma unroll
+				for (int w = 0; w < WV; w++) v[k][w] = hj_load_stream(src + w);
+				dig[k] = (uint32_t)(hj_key_of(v[k][0]) >> shift) & fmask;
+				rank[k] = atomicAdd(&tcnt[dig[k]], 1u);
+			}
+		}
+		__syncthreads();
+		// exclusive scan of tcnt -> tbase
+		{
+			const uint32_t p0 = threadIdx.x * per;
+			uint32_t s = 0;
+			for (uint32_t q = 0; q < per; q++)
+				if (p0 + q < F) s += tcnt[p0 + q];
+			uint32_t tot;
+			uint32_t run = hj_block_exscan<uint32_t>(s, red, &tot);
+			for (uint32_t q = 0; q < per; q++)
+				if (p0 + q < F) {
+					tbase[p0 + q] = run;
+					run += tcnt[p0 + q];
+				}
+		}
+		__syncthreads();
+#pragma unroll
+		for (int k = 0; k < ITEMS; k++) {
+			const uint32_t idx = k * NT + threadIdx.x;
+			if (idx < n) {
+				const uint32_t pos = tbase[dig[k]] + rank[k];
+#pragma unroll
+				for (int w = 0; w < WV; w++) tile[pos * WV + w] = v[k][w];
+				pid[pos] = (uint16_t)dig[k];
+			}
+		}
+		__syncthreads();
+		// write the sorted tile out: consecutive lanes -> consecutive words
+		for (uint32_t u = threadIdx.x; u < n * WV; u += NT) {
+			const uint32_t i = u / WV, w = u % WV;
+			const uint32_t p = pid[i];
+			const uint64_t dest = cursor[p] + (i - tbase[p]);
+			out[dest * WV + w] = tile[u];
+		}
+		__syncthreads();
+		for (uint32_t p = threadIdx.x; p < F; p += NT) {
+			cursor[p] += tcnt[p];
+			tcnt[p] = 0;
+		}
+		__syncthreads();
+	}
+}
+
+// ------------------------------------------------------------ planner
+// pairs[q] and tiles[q] for every final partition q of one join.
+//  passes 0: one pair (whole R, whole S) at offsets r_base / s_base.
+//  passes 1: R digit q at pb[q], S digit q at pb[F1 + q].
+//  passes 2: q = (p2 << b1) | p1; relation rel's partition at
+//            pb[((rel*F1 + p1) * F2) + p2].
+__global__ void k_plan(const uint64_t *__restrict__ pb, uint32_t passes, uint32_t Q, uint32_t b1, uint32_t F2,
+                       uint64_t nr, uint64_t ns, uint64_t s_tile, hj_pair *__restrict__ pairs,
+                       uint64_t *__restrict__ tiles) {
+	const uint32_t F1 = 1u << b1;
+	for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < Q; q += gridDim.x * blockDim.x) {
+		hj_pair pr;
+		if (passes == 0) {
+			pr = hj_pair{0, nr, 0, ns};
+		} else if (passes == 1) {
+			pr = hj_pair{pb[q], pb[q + 1], pb[F1 + q], pb[F1 + q + 1]};
+		} else {
+			const uint32_t p1 = q & (F1 - 1), p2 = q >> b1;
+			const uint64_t ir = ((uint64_t)p1 * F2) + p2, is = ((uint64_t)(F1 + p1) * F2) + p2;
+			pr = hj_pair{pb[ir], pb[ir + 1], pb[is], pb[is + 1]};
+		}
+		pairs[q] = pr;
+		const uint64_t rl = pr.r1 - pr.r0, sl = pr.s1 - pr.s0;
+		tiles[q] = (rl && sl) ? (sl + s_tile - 1) / s_tile : 0;
+	}
+}
+
+// ------------------------------------------------------------ join
+// LDS (bc): keys u64[cap] | pays u64[cap] | heads u32[np2(cap)] | next u16[cap]
+// LDS (lp): keys u64[cap] | pays u64[cap] | occ   u32[np2(cap)]
+template <int WV, int NT, bool LP>
+__global__ __launch_bounds__(NT) void k_join(const uint4 *__restrict__ Rb, const uint4 *__restrict__ Sb,
+                                             const hj_pair *__restrict__ pairs,
+                                             const uint64_t *__restrict__ tprefix, uint32_t npairs, uint32_t B,
+                                             uint64_t s_tile, uint32_t cap, hj_stats *st) {
+	extern __shared__ uint64_t lds_u64[];
+	__shared__ unsigned long long red[2 * (NT / HJ_WAVE)];
+	__shared__ uint64_t s_item[4];
+	uint64_t *keys = lds_u64;
+	uint64_t *pays = keys + cap;
+	uint32_t *heads = (uint32_t *)(pays + cap);
+	const uint32_t nbmax = 1u << (32 - __clz(cap - 1));
+	uint16_t *next = (uint16_t *)(heads + nbmax);
+	const uint64_t total = tprefix[npairs];
+	unsigned long long cnt = 0, sum = 0;
+	bool dup = false, oversize = false;
+
+	for (;;) {
+		if (threadIdx.x == 0) {
+			const uint64_t w = atomicAdd(&st->work_next, 1ULL);
+			uint32_t lo = 0, hi = npairs;  // last pair with tprefix <= w
+			while (hi - lo > 1) {
+				const uint32_t mid = (lo + hi) >> 1;
+				if (tprefix[mid] <= w) lo = mid; else hi = mid;
+			}
+			s_item[0] = w;
+			s_item[1] = lo;
+			s_item[2] = w < total ? w - tprefix[lo] : 0;
+		}
+		__syncthreads();
+		const uint64_t w = s_item[0];
+		const uint32_t q = (uint32_t)s_item[1];
+		const uint64_t tix = s_item[2];
+		__syncthreads();
+		if (w >= total) break;
+		const hj_pair pr = pairs[q];
+		const uint64_t sb = pr.s0 + tix * s_tile, se = min(pr.s1, sb + s_tile);
+		if (LP && pr.r1 - pr.r0 > cap) {
+			oversize = true;  // whole join is recomputed by the host's exact path
+			continue;
+		}
+		for (uint64_t rc = pr.r0; rc < pr.r1; rc += cap) {
+			const uint32_t m = (uint32_t)min((uint64_t)cap, pr.r1 - rc);
+			const uint32_t nb = m <= 1 ? 1u : 1u << (32 - __clz(m - 1));
+			const uint32_t hmask = nb - 1;
+			for (uint32_t i = threadIdx.x; i < nb; i += NT) heads[i] = 0;
+			__syncthreads();
+			for (uint32_t i = threadIdx.x; i < m; i += NT) {
+				const uint4 t = Rb[(rc + i) * WV];
+				const uint64_t key = hj_key_of(t);
+				keys[i] = key;
+				pays[i] = hj_pay_of(t);
+				if (!LP) next[i] = (uint16_t)atomicExch(&heads[(uint32_t)(key >> B) & hmask], i + 1);
+			}
+			__syncthreads();
+			if (LP) {
+				for (uint32_t i = threadIdx.x; i < m; i += NT) {
+					const uint64_t key = keys[i];
+					uint32_t pos = (uint32_t)(key >> B) & hmask;
+					for (uint32_t step = 0; step < nb; step++) {
+						const uint32_t old = atomicCAS(&heads[pos], 0u, i + 1);
+						if (old == 0) break;
+						if (keys[old - 1] == key) {
+							dup = true;
+							break;
+						}
+						pos = (pos + 1) & hmask;
+					}
+				}
+				__syncthreads();
+			}
+			constexpr int U = 4;
+			for (uint64_t j0 = sb + threadIdx.x; j0 < se; j0 += (uint64_t)U * NT) {
+				uint64_t sk[U], sp[U];
+				bool live[U];
+#pragma unroll
+				for (int u = 0; u < U; u++) {
+					const uint64_t j = j0 + (uint64_t)u * NT;
+					live[u] = j < se;
+					const uint4 t = live[u] ? Sb[j * WV] : make_uint4(0, 0, 0, 0);
+					sk[u] = hj_key_of(t);
+					sp[u] = hj_pay_of(t);
+				}
+#pragma unroll
+				for (int u = 0; u < U; u++) {
+					if (!live[u]) continue;
+					const uint64_t key = sk[u];
+					uint32_t h = (uint32_t)(key >> B) & hmask;
+					if (!LP) {
+						for (uint32_t hit = heads[h]; hit; hit = next[hit - 1]) {
+							if (keys[hit - 1] == key) {
+								cnt++;
+								sum += pays[hit - 1] + sp[u];
+							}
+						}
+					} else {
+						for (uint32_t step = 0; step < nb; step++) {
+							const uint32_t o = heads[h];
+							if (!o) break;
+							if (keys[o - 1] == key) {
+								cnt++;
+								sum += pays[o - 1] + sp[u];
+								break;
+							}
+							h = (h + 1) & hmask;
+						}
+					}
+				}
+			}
+			__syncthreads();
+		}
+	}
+	const unsigned fl = (__any(dup) ? HJ_DF_DUPKEY : 0u) | (__any(oversize) ? HJ_DF_OVERSIZE : 0u);
+	if (fl && (threadIdx.x & (HJ_WAVE - 1)) == 0) atomicOr(&st->flags, fl);
+	hj_block_add2(cnt, sum, red, &st->matched, &st->checksum);
+}
+
+// ------------------------------------------------------------ host side
+struct part_cfg {
+	int nt, items, tile;
+};
+
+static part_cfg cfg_for(uint32_t wv) {
+	// 32 KB LDS tile of tuples, 512 threads (256 for 128-byte tuples and up)
+	const int nt = wv >= 8 ? 256 : 512;
+	int items = (int)(32768 / (16 * wv) / nt);
+	if (items < 1) items = 1;
+	return part_cfg{nt, items, nt * items};
+}
+
+static size_t scatter_lds(const part_cfg &c, uint32_t wv, uint32_t F) {
+	return (size_t)c.tile * wv * 16 + (size_t)F * 16 + (size_t)(c.tile + 2) * 2 + 64 * 4;
+}
+
+template <int WV>
+static void launch_hist(hj_ctx_t *ctx, unsigned grid, const hj_seg *segs, const uint32_t *cmap, uint64_t ch,
+                        uint32_t shift, uint32_t bits, uint64_t *hist) {
+	hipLaunchKernelGGL(k_hist<WV>, dim3(grid), dim3(512), (size_t)4 << bits, ctx->stream, segs, cmap,
+	                   (const hj_stats *)ctx->stats.p, ch, shift, bits, hist);
+}
+
+template <int WV, int NT, int ITEMS>
+static void launch_scatter_t(hj_ctx_t *ctx, unsigned grid, size_t lds, const hj_seg *segs, const uint32_t *cmap,
+                             uint64_t ch, uint32_t shift, uint32_t bits, const uint64_t *scan, void *out) {
+	auto kfn = k_scatter<WV, NT, ITEMS>;
+	(void)hipFuncSetAttribute((const void *)kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+	hipLaunchKernelGGL(kfn, dim3(grid), dim3(NT), lds, ctx->stream, segs, cmap, (const hj_stats *)ctx->stats.p, ch,
+	                   shift, bits, scan, (uint4 *)out);
+}
+
+static int launch_pass(hj_ctx_t *ctx, uint32_t wv, unsigned grid, const hj_seg *segs, const uint32_t *cmap,
+                       uint64_t ch, uint32_t shift, uint32_t bits, uint64_t *hist, uint64_t hist_len, void *out) {
+	const part_cfg c = cfg_for(wv);
+	const uint32_t F = 1u << bits;
+	const size_t lds = scatter_lds(c, wv, F);
+	if (lds > ctx->prop.sharedMemPerBlock)
+		return hj_fail(ctx, HJ_EINVAL, "%u radix bits in one pass need %zu B of LDS (device: %zu)", bits, lds,
+		               (size_t)ctx->prop.sharedMemPerBlock);
+	if (grid == 0) return HJ_OK;
+	hj_kt_begin(ctx, HJ_K_HIST);
+	switch (wv) {
+	case 1: launch_hist<1>(ctx, grid, segs, cmap, ch, shift, bits, hist); break;
+	case 2: launch_hist<2>(ctx, grid, segs, cmap, ch, shift, bits, hist); break;
+	case 4: launch_hist<4>(ctx, grid, segs, cmap, ch, shift, bits, hist); break;
+	case 8: launch_hist<8>(ctx, grid, segs, cmap, ch, shift, bits, hist); break;
+	default: launch_hist<16>(ctx, grid, segs, cmap, ch, shift, bits, hist); break;
+	}
+	hj_kt_end(ctx);
+	hj_kt_begin(ctx, HJ_K_SCAN);
+	int rc = scan_u64(ctx, hist, hist_len, (uint64_t *)ctx->scan_tmp.p);
+	hj_kt_end(ctx);
+	if (rc) return rc;
+	hj_kt_begin(ctx, HJ_K_SCATTER);
+	switch (wv) {
+	case 1: launch_scatter_t<1, 512, 4>(ctx, grid, lds, segs, cmap, ch, shift, bits, hist, out); break;
+	case 2: launch_scatter_t<2, 512, 2>(ctx, grid, lds, segs, cmap, ch, shift, bits, hist, out); break;
+	case 4: launch_scatter_t<4, 512, 1>(ctx, grid, lds, segs, cmap, ch, shift, bits, hist, out); break;
+	case 8: launch_scatter_t<8, 256, 1>(ctx, grid, lds, segs, cmap, ch, shift, bits, hist, out); break;
+	default: launch_scatter_t<16, 256, 1>(ctx, grid, lds, segs, cmap, ch, shift, bits, hist, out); break;
+	}
+	hj_kt_end(ctx);
+	HJ_HIP(ctx, hipGetLastError());
+	return HJ_OK;
+}
+
+// LDS bytes of a join table holding `cap` build tuples.
+static size_t join_lds(uint32_t cap, bool lp) {
+	const size_t nb = hj_next_pow2(cap);
+	return (size_t)cap * 16 + nb * 4 + (lp ? 0 : (size_t)cap * 2 + 8);
+}
+
+template <int WV, bool LP>
+static void launch_join_t(hj_ctx_t *ctx, unsigned grid, size_t lds, const void *Rb, const void *Sb, const hj_pair *pairs,
+                          const uint64_t *tpre, uint32_t npairs, uint32_t B, uint64_t s_tile, uint32_t cap) {
+	auto kfn = k_join<WV, 256, LP>;
+	(void)hipFuncSetAttribute((const void *)kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+	hipLaunchKernelGGL(kfn, dim3(grid), dim3(256), lds, ctx->stream, (const uint4 *)Rb, (const uint4 *)Sb, pairs,
+	                   tpre, npairs, B, s_tile, cap, (hj_stats *)ctx->stats.p);
+}
+
+static void launch_join(hj_ctx_t *ctx, uint32_t wv, bool lp, unsigned grid, size_t lds, const void *Rb,
+                        const void *Sb, const hj_pair *pairs, const uint64_t *tpre, uint32_t npairs, uint32_t B,
+                        uint64_t s_tile, uint32_t cap) {
+#define J(W)                                                                                      \
+	if (lp) launch_join_t<W, true>(ctx, grid, lds, Rb, Sb, pairs, tpre, npairs, B, s_tile, cap); \
+	else launch_join_t<W, false>(ctx, grid, lds, Rb, Sb, pairs, tpre, npairs, B, s_tile, cap)
+	switch (wv) {
+	case 1: J(1); break;
+	case 2: J(2); break;
+	case 4: J(4); break;
+	case 8: J(8); break;
+	default: J(16); break;
+	}
+#undef J
+}
+
+// Default radix configuration for |R| (the size-class autotuner's base
+// rule): aim for ~2K build tuples per final partition so that the LDS table
+// stays near 40 KB; two passes once one pass would exceed 2^11 partitions.
+void hj_radix_default(uint64_t nr, uint32_t *bits, uint32_t *passes) {
+	uint32_t b = 0;
+	while (b < 18 && (nr >> b) > 2048) b++;
+	*bits = b;
+	*passes = b == 0 ? 0 : (b > 11 ? 2 : 1);
+}
+
+int hj_run_radix(hj_ctx_t *ctx, const hj_relation_t *R, const hj_relation_t *S, const hj_params_t *prm, bool lp,
+                 hj_result_t *out) {
+	const uint32_t W = R->tuple_size, wv = W / 16;
+	const uint64_t nr = R->tuple_num, ns = S->tuple_num, N = nr + ns;
+	uint32_t B = prm->radix_bits, P = prm->passes;
+	if (B == 0 && P == 0) hj_radix_default(nr, &B, &P);
+	if (P == 0) P = B == 0 ? 0 : (B > 11 ? 2 : 1);
+	if (B == 0) P = 0;
+	if (P == 2 && B < 2) P = 1;
+	const uint32_t b1 = P ? B / P : 0, b2 = B - b1;
+	const uint32_t F1 = 1u << b1, F2 = 1u << b2, Q = 1u << B;
+	if (b1 > 12 || (P == 2 && b2 > 12)) return hj_fail(ctx, HJ_EINVAL, "at most 12 radix bits per pass");
+	out->radix_bits = B;
+	out->passes = P;
+	hj_stats *st = (hj_stats *)ctx->stats.p;
+	const part_cfg pc = cfg_for(wv);
+
+	// chunk size: a multiple of the scatter tile, ~4 chunks per CU
+	auto chunk_for = [&](uint64_t n) {
+		uint64_t target = (uint64_t)ctx->num_cu * 4;
+		uint64_t ch = (n + target - 1) / target;
+		ch = (ch + pc.tile - 1) / pc.tile * pc.tile;
+		if (ch < (uint64_t)pc.tile) ch = pc.tile;
+		return ch;
+	};
+	const uint64_t ch1 = chunk_for(N), ch2 = chunk_for(N);
+	const uint64_t chunks1 = (nr + ch1 - 1) / ch1 + (ns + ch1 - 1) / ch1;
+	const uint32_t nseg2 = 2 * F1;
+	const uint64_t chunks2_max = N / ch2 + nseg2 + 1;
+	const uint64_t hist1_len = chunks1 * F1, hist2_len = (P == 2) ? chunks2_max * F2 : 0;
+
+	// scratch
+	int rc;
+	if (P >= 1 && (rc = hj_reserve(ctx, ctx->part[0], N * W + 64))) return rc;
+	if (P == 2 && (rc = hj_reserve(ctx, ctx->part[1], N * W + 64))) return rc;
+	const uint64_t hist_len = std::max(hist1_len, hist2_len) + 1;
+	if ((rc = hj_reserve(ctx, ctx->hist, hist_len * 8))) return rc;
+	if ((rc = hj_reserve(ctx, ctx->scan_tmp, ((hist_len + Q + nseg2) / SCAN_CH + 2) * 8 + 4096))) return rc;
+	// meta: segs1[2] | segs2[nseg2] | cmap[max chunks] | pb1 | pb2 | nch | pairs | tiles
+	const uint64_t cmap_n = std::max<uint64_t>(chunks1, chunks2_max);
+	size_t off = 0;
+	auto take = [&](size_t bytes) {
+		size_t o = off;
+		off += (bytes + 255) & ~(size_t)255;
+		return o;
+	};
+	const size_t o_seg1 = take(2 * sizeof(hj_seg)), o_seg2 = take(nseg2 * sizeof(hj_seg));
+	const size_t o_cmap = take(cmap_n * 4), o_pb1 = take(((uint64_t)2 * F1 + 1) * 8);
+	const size_t o_pb2 = take(((uint64_t)nseg2 * F2 + 1) * 8), o_nch = take(((uint64_t)nseg2 + 1) * 8);
+	const size_t o_pairs = take((size_t)Q * sizeof(hj_pair)), o_tiles = take(((size_t)Q + 1) * 8);
+	if ((rc = hj_reserve(ctx, ctx->meta, off))) return rc;
+	char *meta = (char *)ctx->meta.p;
+	hj_seg *segs1 = (hj_seg *)(meta + o_seg1), *segs2 = (hj_seg *)(meta + o_seg2);
+	uint32_t *cmap = (uint32_t *)(meta + o_cmap);
+	uint64_t *pb1 = (uint64_t *)(meta + o_pb1), *pb2 = (uint64_t *)(meta + o_pb2);
+	uint64_t *nch = (uint64_t *)(meta + o_nch), *tiles = (uint64_t *)(meta + o_tiles);
+	hj_pair *pairs = (hj_pair *)(meta + o_pairs);
+	uint64_t *hist = (uint64_t *)ctx->hist.p;
+
+	HJ_HIP(ctx, hipEventRecord(ctx->ev[0], ctx->stream));
+	hj_stats_reset(ctx);
+	const void *Rb = R->tuples, *Sb = S->tuples;
+	const uint64_t *pbF = nullptr;
+	if (P >= 1) {
+		hipLaunchKernelGGL(k_seg_pair, dim3(std::max<unsigned>(1, (unsigned)((chunks1 + 255) / 256))), dim3(256), 0,
+		                   ctx->stream, (const uint4 *)R->tuples, nr, (const uint4 *)S->tuples, ns, ch1, F1, segs1,
+		                   cmap, st);
+		rc = launch_pass(ctx, wv, (unsigned)chunks1, segs1, cmap, ch1, 0, b1, hist, hist1_len, ctx->part[0].p);
+		if (rc) return rc;
+		hipLaunchKernelGGL(k_bounds, dim3((unsigned)std::min<uint64_t>(1024, (2ull * F1 + 256) / 256)), dim3(256), 0,
+		                   ctx->stream, segs1, 2u, F1, hist, pb1);
+		pbF = pb1;
+		Rb = Sb = ctx->part[0].p;
+		if (P == 2) {
+			hipLaunchKernelGGL(k_seg2_len, dim3((nseg2 + 255) / 256), dim3(256), 0, ctx->stream, pb1, nseg2, ch2,
+			                   (const uint4 *)ctx->part[0].p, wv, segs2, nch);
+			if ((rc = scan_u64(ctx, nch, nseg2, (uint64_t *)ctx->scan_tmp.p))) return rc;
+			hipLaunchKernelGGL(k_seg2_fin, dim3(std::min<uint32_t>(nseg2, 4096)), dim3(256), 0, ctx->stream, nch,
+			                   nseg2, F2, segs2, cmap, st);
+			rc = launch_pass(ctx, wv, (unsigned)chunks2_max, segs2, cmap, ch2, b1, b2, hist, hist2_len,
+			                 ctx->part[1].p);
+			if (rc) return rc;
+			hipLaunchKernelGGL(k_bounds, dim3((unsigned)std::min<uint64_t>(4096, ((uint64_t)nseg2 * F2 + 256) / 256)),
+			                   dim3(256), 0, ctx->stream, segs2, nseg2, F2, hist, pb2);
+			pbF = pb2;
+			Rb = Sb = ctx->part[1].p;
+		}
+	}
+	HJ_HIP(ctx, hipGetLastError());
+	HJ_HIP(ctx, hipEventRecord(ctx->ev[1], ctx->stream));
+
+	// join: tile the probe side so a skewed partition spreads over workgroups
+	const uint64_t avg_s = (ns >> B) + 1;
+	uint64_t s_tile = std::max<uint64_t>(8192, 2 * avg_s);
+	const uint64_t avg_r = (nr + Q - 1) / Q;
+	const size_t lds_max = std::min<size_t>(ctx->prop.sharedMemPerBlock, 160 * 1024) - 2048;
+	uint32_t cap = (uint32_t)std::min<uint64_t>(65535, std::max<uint64_t>(256, avg_r + avg_r / 4 + 64));
+	while (cap > 256 && join_lds(cap, lp) > lds_max) cap -= 256;
+	const size_t jlds = join_lds(cap, lp);
+	hj_kt_begin(ctx, HJ_K_PLAN);
+	hipLaunchKernelGGL(k_plan, dim3((Q + 255) / 256), dim3(256), 0, ctx->stream, pbF, P, Q, b1, F2, nr, ns, s_tile,
+	                   pairs, tiles);
+	rc = scan_u64(ctx, tiles, Q, (uint64_t *)ctx->scan_tmp.p);
+	hj_kt_end(ctx);
+	if (rc) return rc;
+	const int per_cu = jlds <= 40 * 1024 ? 4 : (jlds <= 76 * 1024 ? 2 : 1);
+	const unsigned jgrid = (unsigned)std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)ctx->num_cu * per_cu, (uint64_t)Q * 64));
+	hj_kt_begin(ctx, HJ_K_JOIN);
+	launch_join(ctx, wv, lp, jgrid, jlds, Rb, Sb, pairs, tiles, Q, B, s_tile, cap);
+	hj_kt_end(ctx);
+	HJ_HIP(ctx, hipGetLastError());
+	HJ_HIP(ctx, hipEventRecord(ctx->ev[2], ctx->stream));
+	hj_stats h;
+	if ((rc = hj_stats_read(ctx, &h))) return rc;
+	out->buildpart_time = hj_event_seconds(ctx, 0, 1);
+	out->probejoin_time = hj_event_seconds(ctx, 1, 2);
+	out->total_time = hj_event_seconds(ctx, 0, 2);
+	out->algo_bytes = (uint64_t)(2 * P + 1) * W * N;
+	if (lp && (h.flags & (HJ_DF_DUPKEY | HJ_DF_OVERSIZE))) {
+		// Exact path: linear probing keeps only the first build tuple per key
+		// (src/algo/phj.c:655-666 with a stable partition = smallest R row),
+		// which is what nphj_lp computes with its duplicate fix-up.
+		hj_result_t r2{};
+		hj_params_t p2{};
+		rc = hj_run_nphj_lp(ctx, R, S, &p2, &r2);
+		if (rc) return rc;
+		out->matched_cnt = r2.matched_cnt;
+		out->check_sum = r2.check_sum;
+		out->fallback |= (h.flags & HJ_DF_DUPKEY) ? HJ_FB_LP_DUPLICATES : 0;
+		out->fallback |= (h.flags & HJ_DF_OVERSIZE) ? HJ_FB_LP_OVERSIZE : 0;
+		return HJ_OK;
+	}
+	out->matched_cnt = h.matched;
+	out->check_sum = h.checksum;
+	return HJ_OK;
+}

@@ -1,0 +1,279 @@
+"""Python binding of libhjmi.so, the MI355X equi-join library.
+
+Host-side mirror of the reference's join entry points (src/main.c:66-143):
+``Context.join(algo, R, S)`` runs one join selected by the reference's
+algorithm name and returns ``matched_cnt``, ``check_sum`` and the phase times
+(``buildpart_time``, ``probejoin_time``, ``total_time``) that the reference
+prints (src/algo/nphj.c:881, src/algo/phj.c:2206, src/main.c:439-445).
+
+The library is loaded with ctypes from ``hashjoin-scm_amd/libhjmi.so``.  There
+is no CPU fallback: if the shared library or a GPU is missing, constructing a
+``Context`` raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+_PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(_PKG_DIR, "libhjmi.so")
+
+HJ_OK, HJ_EINVAL, HJ_ENOMEM, HJ_EHIP, HJ_EREJECT, HJ_ENODEV, HJ_EBUSY = 0, -1, -2, -3, -4, -5, -6
+HJ_LOC_DEVICE, HJ_LOC_HOST = 0, 1
+HJ_F_KERNEL_TIMES = 1
+HJ_FB_LP_DUPLICATES, HJ_FB_LP_OVERSIZE, HJ_FB_SC_REBUILD = 1, 2, 4
+
+WL_PKFK_UNIFORM, WL_PKFK_REF, WL_PAYLOAD_REF, WL_ZIPF, WL_RANDOM = 0, 1, 2, 3, 4
+WORKLOADS = {"pkfk": WL_PKFK_UNIFORM, "pkfk_ref": WL_PKFK_REF, "payload_ref": WL_PAYLOAD_REF,
+             "zipf": WL_ZIPF, "random": WL_RANDOM}
+
+ALGOS = ("nphj_sc", "nphj_lp", "phj_radix_bc", "phj_radix_lp", "phj_radix_bw_reg_bc", "phj_radix_bw_reg_lp")
+MAX_KTIMES = 16
+
+
+class Relation(C.Structure):
+    _fields_ = [("tuples", C.c_void_p), ("tuple_num", C.c_uint64), ("tuple_size", C.c_uint32),
+                ("location", C.c_uint32)]
+
+
+class Params(C.Structure):
+    _fields_ = [("radix_bits", C.c_uint32), ("passes", C.c_uint32), ("lp_slots_log2", C.c_uint32),
+                ("flags", C.c_uint32), ("reserved", C.c_uint32 * 4)]
+
+
+class Result(C.Structure):
+    _fields_ = [("matched_cnt", C.c_uint64), ("check_sum", C.c_uint64), ("buildpart_time", C.c_double),
+                ("probejoin_time", C.c_double), ("total_time", C.c_double), ("algo_bytes", C.c_uint64),
+                ("radix_bits", C.c_uint32), ("passes", C.c_uint32), ("fallback", C.c_uint32),
+                ("n_ktimes", C.c_uint32), ("ktime_ms", C.c_double * MAX_KTIMES),
+                ("ktime_id", C.c_uint32 * MAX_KTIMES)]
+
+
+class GenSpec(C.Structure):
+    _fields_ = [("workload", C.c_uint32), ("width", C.c_uint32), ("seed", C.c_uint64), ("r_num", C.c_uint64),
+                ("s_num", C.c_uint64), ("key_range", C.c_uint64), ("theta", C.c_double)]
+
+
+class DeviceInfo(C.Structure):
+    _fields_ = [("cu_count", C.c_int), ("lds_per_block", C.c_int), ("lds_per_cu", C.c_int),
+                ("warp_size", C.c_int), ("hbm_bytes", C.c_uint64), ("clock_khz", C.c_int),
+                ("arch", C.c_char * 64)]
+
+
+class HJError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"{msg} (status {code})")
+        self.code = code
+
+
+_lib = None
+
+
+def load_library(path: str = LIB_PATH):
+    """Load libhjmi.so once; raise if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise ImportError(f"{path} is missing: build it with __graft_entry__.build() or "
+                          f"`make -C hashjoin-scm_amd`")
+    lib = C.CDLL(path)
+    P, U64, U32, I = C.c_void_p, C.c_uint64, C.c_uint32, C.c_int
+    sig = {
+        "hj_ctx_create": (I, [I, C.POINTER(P)]),
+        "hj_ctx_destroy": (I, [P]),
+        "hj_strerror": (C.c_char_p, [I]),
+        "hj_last_error": (C.c_char_p, [P]),
+        "hj_algo_count": (I, []),
+        "hj_algo_name": (C.c_char_p, [I]),
+        "hj_kernel_name": (C.c_char_p, [U32]),
+        "hj_join": (I, [P, C.c_char_p, C.POINTER(Relation), C.POINTER(Relation), C.POINTER(Params),
+                        C.POINTER(Result)]),
+        "hj_dev_alloc": (I, [P, U64, C.POINTER(P)]),
+        "hj_dev_free": (I, [P, P]),
+        "hj_copy_h2d": (I, [P, P, P, U64]),
+        "hj_copy_d2h": (I, [P, P, P, U64]),
+        "hj_sync": (I, [P]),
+        "hj_generate": (I, [P, C.POINTER(GenSpec), P, P]),
+        "hj_device_info": (I, [P, C.POINTER(DeviceInfo)]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype, fn.argtypes = res, args
+    _lib = lib
+    return lib
+
+
+def exported_symbols():
+    """Names of the C entry points this binding requires."""
+    return ["hj_ctx_create", "hj_ctx_destroy", "hj_strerror", "hj_last_error", "hj_algo_count",
+            "hj_algo_name", "hj_kernel_name", "hj_join", "hj_dev_alloc", "hj_dev_free", "hj_copy_h2d",
+            "hj_copy_d2h", "hj_sync", "hj_generate", "hj_device_info"]
+
+
+@dataclass
+class DeviceRelation:
+    """Tuples resident in HBM: the device analogue of relation_t (store.h:147-150)."""
+    ptr: int
+    tuple_num: int
+    tuple_size: int
+    owner: "Context | None" = None
+
+    def as_c(self) -> Relation:
+        return Relation(self.ptr, self.tuple_num, self.tuple_size, HJ_LOC_DEVICE)
+
+    @property
+    def nbytes(self) -> int:
+        return self.tuple_num * self.tuple_size
+
+
+def kernel_name(kid: int) -> str:
+    s = load_library().hj_kernel_name(kid)
+    return s.decode() if s else f"k{kid}"
+
+
+class Context:
+    """One GPU context (stream, events, scratch arena)."""
+
+    def __init__(self, device: int = 0):
+        self.lib = load_library()
+        h = C.c_void_p()
+        rc = self.lib.hj_ctx_create(device, C.byref(h))
+        if rc != HJ_OK:
+            raise HJError(rc, "hj_ctx_create: " + self.lib.hj_strerror(rc).decode())
+        self.h = h
+        self._owned = []
+
+    # -- errors
+    def _check(self, rc: int, what: str):
+        if rc != HJ_OK:
+            detail = self.lib.hj_last_error(self.h).decode()
+            raise HJError(rc, f"{what}: {self.lib.hj_strerror(rc).decode()}: {detail}")
+
+    def close(self):
+        if getattr(self, "h", None):
+            for p in self._owned:
+                self.lib.hj_dev_free(self.h, C.c_void_p(p))
+            self._owned = []
+            self.lib.hj_ctx_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- memory
+    def alloc(self, nbytes: int) -> int:
+        p = C.c_void_p()
+        self._check(self.lib.hj_dev_alloc(self.h, int(nbytes), C.byref(p)), "hj_dev_alloc")
+        self._owned.append(p.value)
+        return p.value
+
+    def free(self, ptr: int):
+        if ptr in self._owned:
+            self._owned.remove(ptr)
+            self._check(self.lib.hj_dev_free(self.h, C.c_void_p(ptr)), "hj_dev_free")
+
+    def free_relation(self, rel: DeviceRelation):
+        if rel is not None and rel.ptr:
+            self.free(rel.ptr)
+            rel.ptr = 0
+
+    def upload(self, arr: np.ndarray, tuple_size: int) -> DeviceRelation:
+        """Copy a host tuple array (bytes, row-major) into HBM."""
+        a = np.ascontiguousarray(arr)
+        nbytes = a.nbytes
+        if nbytes % tuple_size:
+            raise ValueError("array size is not a multiple of tuple_size")
+        ptr = self.alloc(max(nbytes, 1))
+        if nbytes:
+            self._check(self.lib.hj_copy_h2d(self.h, C.c_void_p(ptr), a.ctypes.data_as(C.c_void_p), nbytes),
+                        "hj_copy_h2d")
+        return DeviceRelation(ptr, nbytes // tuple_size, tuple_size, self)
+
+    def download(self, rel: DeviceRelation) -> np.ndarray:
+        out = np.empty(rel.nbytes, dtype=np.uint8)
+        if rel.nbytes:
+            self._check(self.lib.hj_copy_d2h(self.h, out.ctypes.data_as(C.c_void_p), C.c_void_p(rel.ptr),
+                                             rel.nbytes), "hj_copy_d2h")
+        return out
+
+    def sync(self):
+        self._check(self.lib.hj_sync(self.h), "hj_sync")
+
+    # -- generator
+    def generate(self, workload="pkfk", r_num=1 << 10, s_num=1 << 14, width=16, seed=12345,
+                 theta=1.0, key_range=0):
+        spec = make_spec(workload, r_num, s_num, width, seed, theta, key_range)
+        r = self.alloc(max(spec.r_num * spec.width, 1))
+        s = self.alloc(max(spec.s_num * spec.width, 1))
+        self._check(self.lib.hj_generate(self.h, C.byref(spec), C.c_void_p(r), C.c_void_p(s)), "hj_generate")
+        return (DeviceRelation(r, spec.r_num, spec.width, self), DeviceRelation(s, spec.s_num, spec.width, self))
+
+    # -- join
+    def join(self, algo: str, R, S, radix_bits: int = 0, passes: int = 0, lp_slots_log2: int = 0,
+             kernel_times: bool = False) -> dict:
+        """Run one join; R/S are DeviceRelation or host numpy uint8 arrays."""
+        rr, keep = [], []
+        for rel, ts in ((R, None), (S, None)):
+            if isinstance(rel, DeviceRelation):
+                rr.append(rel.as_c())
+            else:
+                raise TypeError("pass DeviceRelation (use Context.upload or Context.host_relation)")
+        prm = Params(radix_bits, passes, lp_slots_log2, HJ_F_KERNEL_TIMES if kernel_times else 0)
+        res = Result()
+        self._check(self.lib.hj_join(self.h, algo.encode(), C.byref(rr[0]), C.byref(rr[1]), C.byref(prm),
+                                     C.byref(res)), f"hj_join({algo})")
+        return result_dict(res)
+
+    def join_host(self, algo: str, r_host: np.ndarray, s_host: np.ndarray, tuple_size: int, **kw) -> dict:
+        """Join host-resident relations: the library copies them in (untimed)."""
+        ra, sa = np.ascontiguousarray(r_host), np.ascontiguousarray(s_host)
+        R = Relation(ra.ctypes.data_as(C.c_void_p), ra.nbytes // tuple_size, tuple_size, HJ_LOC_HOST)
+        S = Relation(sa.ctypes.data_as(C.c_void_p), sa.nbytes // tuple_size, tuple_size, HJ_LOC_HOST)
+        prm = Params(kw.get("radix_bits", 0), kw.get("passes", 0), kw.get("lp_slots_log2", 0), 0)
+        res = Result()
+        self._check(self.lib.hj_join(self.h, algo.encode(), C.byref(R), C.byref(S), C.byref(prm), C.byref(res)),
+                    f"hj_join({algo})")
+        return result_dict(res)
+
+    def device_info(self) -> dict:
+        d = DeviceInfo()
+        self._check(self.lib.hj_device_info(self.h, C.byref(d)), "hj_device_info")
+        return {"cu_count": d.cu_count, "lds_per_block": d.lds_per_block, "lds_per_cu": d.lds_per_cu,
+                "warp_size": d.warp_size, "hbm_bytes": d.hbm_bytes, "clock_khz": d.clock_khz,
+                "arch": d.arch.decode()}
+
+
+def make_spec(workload="pkfk", r_num=1 << 10, s_num=1 << 14, width=16, seed=12345, theta=1.0, key_range=0):
+    wl = WORKLOADS[workload] if isinstance(workload, str) else int(workload)
+    return GenSpec(wl, width, seed, r_num, s_num, key_range or r_num, theta)
+
+
+def result_dict(res: Result) -> dict:
+    kt = {}
+    for i in range(res.n_ktimes):
+        name = kernel_name(res.ktime_id[i])
+        kt.setdefault(name, []).append(res.ktime_ms[i])
+    return {
+        "matched_cnt": res.matched_cnt, "check_sum": res.check_sum,
+        "buildpart_time": res.buildpart_time, "probejoin_time": res.probejoin_time,
+        "total_time": res.total_time, "algo_bytes": res.algo_bytes, "radix_bits": res.radix_bits,
+        "passes": res.passes, "fallback": res.fallback, "kernel_ms": kt,
+    }
+
+
+def algo_names():
+    lib = load_library()
+    return [lib.hj_algo_name(i).decode() for i in range(lib.hj_algo_count())]

@@ -1,0 +1,193 @@
+/*
+ * main.c -- command-line driver, the drop-in for the reference's bin/main
+ * (src/main.c:147-550).
+ *
+ *   hjmain --algo=<name> --workload=<w> --subtype=<s> --param=<p> [options]
+ *
+ * The four reference arguments keep their meaning: the algorithm is looked
+ * up by name in the library's registry (src/main.c:204-218, unknown name ->
+ * error exit), and (workload, param, subtype) name a workload of the
+ * reference's config tree (src/main.c:244-255, config/data/<w>/<p>_<s>.cfg).
+ * Instead of reading pmem files, the relations are generated on the GPU with
+ * the same layout and closed-form results as the reference's generator.
+ * The trial protocol is src/main.c:429-459: TRIALS runs, each printing
+ * matched_cnt/check_sum and the phase times, then the mean of trials 2..N.
+ *
+ * Extra options (all optional):
+ *   --r=N --s=N --width=W --gen=pkfk|pkfk_ref|payload_ref|zipf|random
+ *   --theta=T --key-range=K --seed=S --bits=B --passes=P --trials=T
+ *   --device=D --list
+ * Workload "baseline" with subtype C1, C2, C3 selects the benchmark
+ * configurations: C1 2^20 x 2^24 uniform PK-FK, C2 2^24 x 2^28, C3 64-byte
+ * Zipf 2^24 x 2^27 (theta from --theta, default 1.0).
+ */
+#include <getopt.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "hjmi.h"
+
+typedef struct {
+	const char *workload, *param, *subtype;
+	uint64_t r_num, s_num;
+	uint32_t width, gen;
+} wl_t;
+
+/* Sizes from the reference's config/data and config/fndly-data files. */
+static const wl_t k_workloads[] = {
+    {"uniform", "pkfk", "A", 1ull << 24, 1ull << 28, 16, HJ_WL_PKFK_REF},
+    {"uniform", "pkfk", "B", 1ull << 26, 1ull << 28, 16, HJ_WL_PKFK_REF},
+    {"uniform", "pkfk", "C", 1ull << 28, 1ull << 28, 16, HJ_WL_PKFK_REF},
+    {"payload", "pkfk", "16", 1ull << 24, 1ull << 28, 16, HJ_WL_PAYLOAD_REF},
+    {"payload", "pkfk", "32", 1ull << 24, 1ull << 28, 32, HJ_WL_PAYLOAD_REF},
+    {"payload", "pkfk", "64", 1ull << 24, 1ull << 28, 64, HJ_WL_PAYLOAD_REF},
+    {"payload", "pkfk", "128", 1ull << 24, 1ull << 28, 128, HJ_WL_PAYLOAD_REF},
+    {"payload", "pkfk", "256", 1ull << 24, 1ull << 28, 256, HJ_WL_PAYLOAD_REF},
+    {"fndly", "pkfk", "Q", 1ull << 28, 1ull << 30, 16, HJ_WL_PKFK_REF},
+    {"fndly", "pkfk", "R", 1ull << 28, 1ull << 31, 16, HJ_WL_PKFK_REF},
+    {"fndly", "pkfk", "H", 1ull << 28, 1ull << 32, 16, HJ_WL_PKFK_REF},
+    {"fndly", "pkfk", "T", 1ull << 29, 1ull << 32, 16, HJ_WL_PKFK_REF},
+    {"baseline", "pkfk", "C1", 1ull << 20, 1ull << 24, 16, HJ_WL_PKFK_UNIFORM},
+    {"baseline", "pkfk", "C2", 1ull << 24, 1ull << 28, 16, HJ_WL_PKFK_UNIFORM},
+    {"baseline", "zipf", "C3", 1ull << 24, 1ull << 27, 64, HJ_WL_ZIPF},
+    {NULL, NULL, NULL, 0, 0, 0, 0},
+};
+
+static int die(hj_ctx_t *ctx, int rc, const char *what) {
+	fprintf(stderr, "[ERROR] %s: %s (%s)\n", what, hj_strerror(rc), ctx ? hj_last_error(ctx) : "");
+	return 1;
+}
+
+int main(int argc, char **argv) {
+	const char *algo = NULL, *workload = "baseline", *subtype = "C2", *param = "pkfk", *gen = NULL;
+	uint64_t r_num = 0, s_num = 0, key_range = 0, seed = 12345;
+	uint32_t width = 0, bits = 0, passes = 0;
+	int trials = 6, device = 0, list = 0;
+	double theta = 1.0;
+	static struct option opts[] = {
+	    {"algo", required_argument, 0, 'a'},    {"workload", required_argument, 0, 'w'},
+	    {"subtype", required_argument, 0, 't'}, {"param", required_argument, 0, 'p'},
+	    {"r", required_argument, 0, 'r'},       {"s", required_argument, 0, 's'},
+	    {"width", required_argument, 0, 'W'},   {"gen", required_argument, 0, 'g'},
+	    {"theta", required_argument, 0, 'z'},   {"key-range", required_argument, 0, 'k'},
+	    {"seed", required_argument, 0, 'S'},    {"bits", required_argument, 0, 'b'},
+	    {"passes", required_argument, 0, 'P'},  {"trials", required_argument, 0, 'T'},
+	    {"device", required_argument, 0, 'd'},  {"list", no_argument, 0, 'l'},
+	    {0, 0, 0, 0}};
+	int c;
+	while ((c = getopt_long(argc, argv, "", opts, NULL)) != -1) {
+		switch (c) {
+		case 'a': algo = optarg; break;
+		case 'w': workload = optarg; break;
+		case 't': subtype = optarg; break;
+		case 'p': param = optarg; break;
+		case 'r': r_num = strtoull(optarg, NULL, 0); break;
+		case 's': s_num = strtoull(optarg, NULL, 0); break;
+		case 'W': width = (uint32_t)strtoul(optarg, NULL, 0); break;
+		case 'g': gen = optarg; break;
+		case 'z': theta = atof(optarg); break;
+		case 'k': key_range = strtoull(optarg, NULL, 0); break;
+		case 'S': seed = strtoull(optarg, NULL, 0); break;
+		case 'b': bits = (uint32_t)strtoul(optarg, NULL, 0); break;
+		case 'P': passes = (uint32_t)strtoul(optarg, NULL, 0); break;
+		case 'T': trials = atoi(optarg); break;
+		case 'd': device = atoi(optarg); break;
+		case 'l': list = 1; break;
+		default: fprintf(stderr, "unknown option\n"); return 2;
+		}
+	}
+	if (list) {
+		for (int i = 0; i < hj_algo_count(); i++) printf("%s\n", hj_algo_name(i));
+		return 0;
+	}
+	if (!algo) {
+		fprintf(stderr, "[ERROR] --algo is required\n");
+		return 2;
+	}
+	int found = 0;
+	for (int i = 0; i < hj_algo_count(); i++)
+		if (strcmp(hj_algo_name(i), algo) == 0) found = 1;
+	if (!found) {
+		printf("[ERROR] Join algorithm named `%s' does not exist!\n", algo);
+		return 1;
+	}
+	hj_gen_spec_t spec;
+	memset(&spec, 0, sizeof spec);
+	spec.seed = seed;
+	spec.theta = theta;
+	const wl_t *wl = NULL;
+	for (const wl_t *w = k_workloads; w->workload; w++)
+		if (!strcmp(w->workload, workload) && !strcmp(w->param, param) && !strcmp(w->subtype, subtype)) wl = w;
+	if (wl) {
+		spec.r_num = wl->r_num;
+		spec.s_num = wl->s_num;
+		spec.width = wl->width;
+		spec.workload = wl->gen;
+	} else if (!r_num || !s_num) {
+		fprintf(stderr, "[ERROR] unknown workload %s/%s_%s and no --r/--s given\n", workload, param, subtype);
+		return 1;
+	} else {
+		spec.width = 16;
+		spec.workload = HJ_WL_PKFK_UNIFORM;
+	}
+	if (r_num) spec.r_num = r_num;
+	if (s_num) spec.s_num = s_num;
+	if (width) spec.width = width;
+	if (gen) {
+		if (!strcmp(gen, "pkfk")) spec.workload = HJ_WL_PKFK_UNIFORM;
+		else if (!strcmp(gen, "pkfk_ref")) spec.workload = HJ_WL_PKFK_REF;
+		else if (!strcmp(gen, "payload_ref")) spec.workload = HJ_WL_PAYLOAD_REF;
+		else if (!strcmp(gen, "zipf")) spec.workload = HJ_WL_ZIPF;
+		else if (!strcmp(gen, "random")) spec.workload = HJ_WL_RANDOM;
+		else {
+			fprintf(stderr, "[ERROR] unknown --gen %s\n", gen);
+			return 1;
+		}
+	}
+	spec.key_range = key_range ? key_range : spec.r_num;
+
+	hj_ctx_t *ctx = NULL;
+	int rc = hj_ctx_create(device, &ctx);
+	if (rc) return die(NULL, rc, "hj_ctx_create");
+	void *r_dev = NULL, *s_dev = NULL;
+	if ((rc = hj_dev_alloc(ctx, spec.r_num * spec.width, &r_dev))) return die(ctx, rc, "alloc R");
+	if ((rc = hj_dev_alloc(ctx, spec.s_num * spec.width, &s_dev))) return die(ctx, rc, "alloc S");
+	if ((rc = hj_generate(ctx, &spec, r_dev, s_dev))) return die(ctx, rc, "generate");
+
+	printf("CONFIGURATION:\tTUPLE_T_SIZE: %u\tNUM_RADIX_BITS: %u\tNUM_PASSES: %u\tTRIALS: %d\n", spec.width, bits,
+	       passes, trials);
+	hj_relation_t R = {r_dev, spec.r_num, spec.width, HJ_LOC_DEVICE};
+	hj_relation_t S = {s_dev, spec.s_num, spec.width, HJ_LOC_DEVICE};
+	hj_params_t prm;
+	memset(&prm, 0, sizeof prm);
+	prm.radix_bits = bits;
+	prm.passes = passes;
+	double sum_b = 0, sum_p = 0, sum_t = 0;
+	int counted = 0;
+	char wname[128];
+	snprintf(wname, sizeof wname, "%s_%s_%s", workload, param, subtype);
+	for (int t = 0; t < trials; t++) {
+		hj_result_t res;
+		if ((rc = hj_join(ctx, algo, &R, &S, &prm, &res))) return die(ctx, rc, "hj_join");
+		printf("matched_cnt: %llu\tcheck_sum: %llu\n", (unsigned long long)res.matched_cnt,
+		       (unsigned long long)res.check_sum);
+		printf("memory_type: hbm\talgo: %s\tworkload: %s\tbuildpart_time: %.9f\tprobejoin_time: %.9f\ttotal_time: "
+		       "%.9f\tradix_bits: %u\tpasses: %u\ttuples_per_s: %.4e\thbm_GBps: %.1f\n",
+		       algo, wname, res.buildpart_time, res.probejoin_time, res.total_time, res.radix_bits, res.passes,
+		       (double)(spec.r_num + spec.s_num) / res.total_time, res.algo_bytes / res.total_time * 1e-9);
+		if (t > 0 || trials == 1) {
+			sum_b += res.buildpart_time;
+			sum_p += res.probejoin_time;
+			sum_t += res.total_time;
+			counted++;
+		}
+	}
+	printf("\nnum_trials: %d\tmemory_type: hbm\talgo: %s\tworkload: %s\tbuildpart_time: %.9f\tprobejoin_time: "
+	       "%.9f\ttotal_time: %.9f\n",
+	       counted, algo, wname, sum_b / counted, sum_p / counted, sum_t / counted);
+	hj_dev_free(ctx, r_dev);
+	hj_dev_free(ctx, s_dev);
+	hj_ctx_destroy(ctx);
+	return 0;
+}

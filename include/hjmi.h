@@ -1,0 +1,144 @@
+/*
+ * hjmi.h -- C ABI of the MI355X equi-join library (libhjmi.so).
+ *
+ * Drop-in boundary for the reference's join hot path.  The reference selects
+ * a join by name from a table of `void algo(const datameta_t, timekeeper_t*)`
+ * entries (src/main.c:66-143, src/algo/nphj.h:80-81, src/algo/phj.h:136-142);
+ * each entry loads its own relations, runs build/partition and probe/join
+ * phases on pthreads, and prints `matched_cnt` and `check_sum`
+ * (src/algo/nphj.c:881, src/algo/phj.c:2206).  This library keeps that shape
+ * -- one entry point, algorithm chosen by the same names, two relations in,
+ * match count + 64-bit checksum + phase times out -- but takes relations by
+ * pointer instead of by file, returns status codes instead of calling exit(),
+ * and keeps all state in a caller-owned context instead of file-statics.
+ *
+ * Only plain C types cross this boundary.  All functions are thread-safe per
+ * context; a context serialises its own calls.  Nothing here calls exit().
+ */
+#ifndef HJMI_H
+#define HJMI_H
+
+#include <stdint.h>
+#include "hjgen.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+/* the library is built with hidden visibility; only this API is exported */
+#pragma GCC visibility push(default)
+
+/* ---- status codes (the reference has none: it asserts or exits,
+ *      src/main.c:215-217, src/inc/utils.c:459-461) ------------------------ */
+#define HJ_OK 0
+#define HJ_EINVAL -1   /* bad argument, unknown algorithm name           */
+#define HJ_ENOMEM -2   /* device or host allocation failed               */
+#define HJ_EHIP -3     /* a HIP runtime call failed                      */
+#define HJ_EREJECT -4  /* serving admission rejected the request         */
+#define HJ_ENODEV -5   /* no usable GPU                                  */
+#define HJ_EBUSY -6    /* request still running (serve poll)             */
+
+/* ---- relation: replaces relation_t {tuples, tuple_num}
+ *      (src/inc/store.h:147-150) plus the tuple width that the reference
+ *      fixes at compile time (TUPLE_T_SIZE, src/inc/store.h:109-113) ------- */
+#define HJ_LOC_DEVICE 0u  /* `tuples` is a device (HBM) pointer            */
+#define HJ_LOC_HOST 1u    /* `tuples` is host memory; copied in, untimed  */
+
+typedef struct {
+	void *tuples;         /* packed AoS records {u64 key; u64 row_id; u8 rest[]} */
+	uint64_t tuple_num;
+	uint32_t tuple_size;  /* bytes per tuple: 16, 32, 64, 128 or 256 */
+	uint32_t location;    /* HJ_LOC_* */
+} hj_relation_t;
+
+/* ---- tuning: the reference's compile-time knobs (src/algo/params.h:26-35,
+ *      CMakeLists.txt:48-375) become run-time values; 0 = library default
+ *      chosen by the size-class autotuner ------------------------------------ */
+typedef struct {
+	uint32_t radix_bits;     /* total radix bits B (NUM_RADIX_BITS), 0 = auto */
+	uint32_t passes;         /* 1 or 2 (NUM_PASSES), 0 = auto                 */
+	uint32_t lp_slots_log2;  /* nphj_lp slots = 2^x; 0 = next_pow_2(|R|) as
+	                            src/algo/nphj.c:1556-1558                      */
+	uint32_t flags;          /* HJ_F_*                                        */
+	uint32_t reserved[4];
+} hj_params_t;
+
+#define HJ_F_KERNEL_TIMES 1u  /* record a HIP event pair around every kernel */
+
+/* ---- result: matched_cnt/check_sum as printed by the reference plus the
+ *      timekeeper_t phase times (src/inc/store.h:74-78) ---------------------- */
+#define HJ_MAX_KTIMES 16
+typedef struct {
+	uint64_t matched_cnt;    /* pairs (r,s) with r.key == s.key (LP: first match) */
+	uint64_t check_sum;      /* sum of r.row_id + s.row_id over matches, mod 2^64  */
+	double buildpart_time;   /* seconds, device events: build or partition   */
+	double probejoin_time;   /* seconds, device events: probe or join        */
+	double total_time;
+	uint64_t algo_bytes;     /* algorithmic HBM bytes of the algorithm run    */
+	uint32_t radix_bits;     /* what actually ran                             */
+	uint32_t passes;
+	uint32_t fallback;       /* HJ_FB_* bitmask of exact slow paths taken     */
+	uint32_t n_ktimes;
+	double ktime_ms[HJ_MAX_KTIMES];  /* per-kernel device ms (HJ_F_KERNEL_TIMES) */
+	uint32_t ktime_id[HJ_MAX_KTIMES];
+} hj_result_t;
+
+#define HJ_FB_LP_DUPLICATES 1u  /* duplicate build keys under LP: re-run exact */
+#define HJ_FB_LP_OVERSIZE 2u    /* partition above LDS capacity under LP        */
+#define HJ_FB_SC_REBUILD 4u     /* nphj_sc overflow pool grown and rebuilt      */
+
+typedef struct hj_ctx hj_ctx_t;
+
+/* Context on one GPU: stream, events and a grow-only scratch arena.
+ * Replaces the reference's per-run allocations (src/algo/phj.c:1917-1950,
+ * src/algo/nphj.c:746-757) and its file-static hashmask/bitskip
+ * (src/algo/nphj.c:95-96). */
+int hj_ctx_create(int device, hj_ctx_t **out);
+int hj_ctx_destroy(hj_ctx_t *ctx);
+const char *hj_strerror(int status);
+const char *hj_last_error(hj_ctx_t *ctx);   /* detail of the last failure */
+
+/* Algorithm registry, same names as the reference's algos[] table
+ * (src/main.c:72-143): nphj_sc, nphj_lp, phj_radix_bc, phj_radix_lp,
+ * phj_radix_bw_reg_bc, phj_radix_bw_reg_lp.  Index past the end -> NULL
+ * (the reference table has no sentinel, src/main.c:206). */
+int hj_algo_count(void);
+const char *hj_algo_name(int idx);
+const char *hj_kernel_name(uint32_t id);
+
+/* One join: the drop-in for `algo(datameta, &timekeeper)`.  R is the build
+ * side, S the probe side.  Borrowed read-only; all scratch is the context's.
+ * On HJ_OK `out` holds count, checksum and phase times. */
+int hj_join(hj_ctx_t *ctx, const char *algo, const hj_relation_t *R, const hj_relation_t *S,
+            const hj_params_t *params, hj_result_t *out);
+
+/* ---- device memory helpers for hosts without an allocator of their own --- */
+int hj_dev_alloc(hj_ctx_t *ctx, uint64_t bytes, void **ptr);
+int hj_dev_free(hj_ctx_t *ctx, void *ptr);
+int hj_copy_h2d(hj_ctx_t *ctx, void *dst, const void *src, uint64_t bytes);
+int hj_copy_d2h(hj_ctx_t *ctx, void *dst, const void *src, uint64_t bytes);
+int hj_sync(hj_ctx_t *ctx);
+
+/* ---- seeded generator on the device (replaces bin/gen, src/gen.c:29-158).
+ * Fills R (spec->r_num tuples) and S (spec->s_num tuples) of width
+ * spec->width in device memory allocated by the caller.  Byte-identical to
+ * the host functions in hjgen.h. */
+int hj_generate(hj_ctx_t *ctx, const hj_gen_spec_t *spec, void *r_dev, void *s_dev);
+
+/* ---- device properties used by the autotuner and the benchmark ---------- */
+typedef struct {
+	int cu_count;
+	int lds_per_block;
+	int lds_per_cu;
+	int warp_size;
+	uint64_t hbm_bytes;
+	int clock_khz;
+	char arch[64];
+} hj_device_info_t;
+int hj_device_info(hj_ctx_t *ctx, hj_device_info_t *info);
+
+#pragma GCC visibility pop
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* HJMI_H */

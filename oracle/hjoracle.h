@@ -1,0 +1,62 @@
+/*
+ * hjoracle.h -- TEST INFRASTRUCTURE ONLY.  CPU oracle for the equi-join hot
+ * path.  Only tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg
+ * may load it, and only as the checker; the product never links it.
+ *
+ * Two kinds of functions:
+ *   - restatements of the reference's DRAM-mode joins (USE_NVM off), in
+ *     their structure: pthreads, static input slices, barriers, per-thread
+ *     histograms + cross-thread prefix, 64-byte software write-combining
+ *     buffers, a mutex task queue, bc_join / lp_join per partition;
+ *   - independent exact checkers: per-key aggregation by sorting, and a
+ *     brute-force nested loop for tiny inputs.
+ *
+ * Semantics shared by every function (the reference's, src/algo/nphj.c:
+ * 348-350, src/algo/phj.c:555-557):
+ *   matched_cnt = #pairs (r,s) with r.key == s.key
+ *   check_sum   = sum of (r.row_id + s.row_id) over those pairs, mod 2^64
+ * Linear probing (lp) keeps only the first build tuple of a key in R's input
+ * order (src/algo/nphj.c:1162-1165, src/algo/phj.c:655-666).  Departures from
+ * the reference, each a reference bug or undefined case:
+ *   - a probe miss adds 0 (nphj_lp dereferences NULL there, nphj.c:1253);
+ *   - probes are bounded by the table size (the reference spins forever on a
+ *     miss in a full table unless TEST_SELECTIVITY is set, phj.c:633-653);
+ *   - lp tables mark occupancy separately, so key 0 is an ordinary key
+ *     (phj.c:614 uses key 0 as "empty").
+ */
+#ifndef HJORACLE_H
+#define HJORACLE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include "../include/hjgen.h"
+
+typedef struct {
+	uint64_t matched_cnt;
+	uint64_t check_sum;
+	double buildpart_time; /* seconds, thread 0 wall clock between barriers */
+	double probejoin_time;
+	double total_time;
+} or_result_t;
+
+/* fill host buffers with the generator of hjgen.h (threads >= 1) */
+void or_generate(const hj_gen_spec_t *spec, void *r, void *s, int threads);
+
+/* restatements of the reference joins; width in bytes (16..256) */
+int or_nphj_sc(const void *R, uint64_t nr, const void *S, uint64_t ns, uint32_t width, int threads, or_result_t *out);
+int or_nphj_lp(const void *R, uint64_t nr, const void *S, uint64_t ns, uint32_t width, int threads, or_result_t *out);
+int or_phj_radix(const void *R, uint64_t nr, const void *S, uint64_t ns, uint32_t width, uint32_t radix_bits,
+                 uint32_t passes, int lp, int threads, or_result_t *out);
+
+/* stable radix partition of one relation by key bits [shift, shift+bits)
+ * (the reference's parallel_radix_partition_optimized, phj.c:257-359, on
+ * `threads` slices); hist receives 2^bits partition sizes */
+int or_radix_partition(const void *in, uint64_t n, uint32_t width, uint32_t shift, uint32_t bits, int threads,
+                       void *out, uint64_t *hist);
+
+/* independent checkers */
+int or_aggregate(const void *R, uint64_t nr, const void *S, uint64_t ns, uint32_t width, int lp, or_result_t *out);
+int or_bruteforce(const void *R, uint64_t nr, const void *S, uint64_t ns, uint32_t width, int lp, or_result_t *out);
+
+#endif

@@ -1,0 +1,164 @@
+"""TEST INFRASTRUCTURE ONLY -- the CPU oracle for the equi-join hot path.
+
+Only tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg may
+import this module, and only as the checker; the product never does.
+
+* ``liboracle.so`` (hjoracle.c): restatements of the reference's DRAM-mode
+  joins (src/algo/nphj.c, src/algo/phj.c) plus sort-based aggregation and a
+  brute-force nested loop.
+* numpy closed forms for the generated PK-FK workloads (derived from the
+  reference's generator, src/gen.c:174-283, and checked against the
+  reference's logged results in tests/golden/reference_logs.json).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+DIR = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(DIR, "liboracle.so")
+M64 = (1 << 64) - 1
+
+
+class OrResult(C.Structure):
+    _fields_ = [("matched_cnt", C.c_uint64), ("check_sum", C.c_uint64), ("buildpart_time", C.c_double),
+                ("probejoin_time", C.c_double), ("total_time", C.c_double)]
+
+
+class GenSpec(C.Structure):
+    _fields_ = [("workload", C.c_uint32), ("width", C.c_uint32), ("seed", C.c_uint64), ("r_num", C.c_uint64),
+                ("s_num", C.c_uint64), ("key_range", C.c_uint64), ("theta", C.c_double)]
+
+
+WORKLOADS = {"pkfk": 0, "pkfk_ref": 1, "payload_ref": 2, "zipf": 3, "random": 4}
+_lib = None
+
+
+def build():
+    import subprocess
+    subprocess.run(["make", "-s", "-C", DIR], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = C.CDLL(LIB_PATH)
+        P, U64, U32, I = C.c_void_p, C.c_uint64, C.c_uint32, C.c_int
+        L.or_generate.argtypes = [C.POINTER(GenSpec), P, P, I]
+        L.or_generate.restype = None
+        for name in ("or_nphj_sc", "or_nphj_lp"):
+            getattr(L, name).argtypes = [P, U64, P, U64, U32, I, C.POINTER(OrResult)]
+            getattr(L, name).restype = I
+        L.or_phj_radix.argtypes = [P, U64, P, U64, U32, U32, U32, I, I, C.POINTER(OrResult)]
+        L.or_phj_radix.restype = I
+        L.or_radix_partition.argtypes = [P, U64, U32, U32, U32, I, P, P]
+        L.or_radix_partition.restype = I
+        for name in ("or_aggregate", "or_bruteforce"):
+            getattr(L, name).argtypes = [P, U64, P, U64, U32, I, C.POINTER(OrResult)]
+            getattr(L, name).restype = I
+        _lib = L
+    return _lib
+
+
+def _ptr(a: np.ndarray):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def generate(workload="pkfk", r_num=1 << 10, s_num=1 << 14, width=16, seed=12345, theta=1.0, key_range=0,
+             threads=8):
+    """Host copy of the generated relations (uint8 arrays, row-major tuples)."""
+    wl = WORKLOADS[workload] if isinstance(workload, str) else int(workload)
+    spec = GenSpec(wl, width, seed, r_num, s_num, key_range or r_num, theta)
+    r = np.empty(r_num * width, dtype=np.uint8)
+    s = np.empty(s_num * width, dtype=np.uint8)
+    lib().or_generate(C.byref(spec), _ptr(r), _ptr(s), threads)
+    return r, s
+
+
+def _res(r: OrResult) -> dict:
+    return {"matched_cnt": r.matched_cnt, "check_sum": r.check_sum, "buildpart_time": r.buildpart_time,
+            "probejoin_time": r.probejoin_time, "total_time": r.total_time}
+
+
+def join(algo: str, R: np.ndarray, S: np.ndarray, width: int, threads: int = 8, radix_bits: int = 11,
+         passes: int = 1) -> dict:
+    """Run the restated reference join `algo` on host tuples."""
+    out = OrResult()
+    nr, ns = R.nbytes // width, S.nbytes // width
+    L = lib()
+    algo = algo.replace("_bw_reg", "")
+    if algo == "nphj_sc":
+        rc = L.or_nphj_sc(_ptr(R), nr, _ptr(S), ns, width, threads, C.byref(out))
+    elif algo == "nphj_lp":
+        rc = L.or_nphj_lp(_ptr(R), nr, _ptr(S), ns, width, threads, C.byref(out))
+    elif algo in ("phj_radix_bc", "phj_radix_lp"):
+        rc = L.or_phj_radix(_ptr(R), nr, _ptr(S), ns, width, radix_bits, passes, int(algo.endswith("lp")),
+                            threads, C.byref(out))
+    else:
+        raise ValueError(algo)
+    if rc:
+        raise RuntimeError(f"oracle {algo} failed: {rc}")
+    return _res(out)
+
+
+def aggregate(R: np.ndarray, S: np.ndarray, width: int, lp: bool = False) -> dict:
+    out = OrResult()
+    rc = lib().or_aggregate(_ptr(R), R.nbytes // width, _ptr(S), S.nbytes // width, width, int(lp), C.byref(out))
+    if rc:
+        raise RuntimeError("aggregate failed")
+    return _res(out)
+
+
+def bruteforce(R: np.ndarray, S: np.ndarray, width: int, lp: bool = False) -> dict:
+    out = OrResult()
+    lib().or_bruteforce(_ptr(R), R.nbytes // width, _ptr(S), S.nbytes // width, width, int(lp), C.byref(out))
+    return _res(out)
+
+
+def radix_partition(rel: np.ndarray, width: int, shift: int, bits: int, threads: int = 4):
+    n = rel.nbytes // width
+    out = np.empty_like(rel)
+    hist = np.zeros(1 << bits, dtype=np.uint64)
+    lib().or_radix_partition(_ptr(rel), n, width, shift, bits, threads, _ptr(out), _ptr(hist))
+    return out, hist
+
+
+# --------------------------------------------------------------- closed forms
+def keys_payloads(rel: np.ndarray, width: int):
+    w = rel.view(np.uint64).reshape(-1, width // 8)
+    return w[:, 0], w[:, 1]
+
+
+def pkfk_closed_form(S: np.ndarray, width: int) -> dict:
+    """PK-FK with unique R keys, R.payload = key, every S key present in R:
+    count = |S|, checksum = sum over S of (key + payload) mod 2^64."""
+    k, p = keys_payloads(S, width)
+    return {"matched_cnt": int(k.size), "check_sum": (int(k.sum(dtype=np.uint64)) + int(p.sum(dtype=np.uint64))) & M64}
+
+
+def pkfk_ref_closed_form(r_num: int, s_num: int) -> dict:
+    """Reference-style PK-FK (src/gen.c:199-223): S = |S|/|R| copies of R's
+    keys, row_id = key on both sides: checksum = |S| * (|R| + 1) mod 2^64."""
+    return {"matched_cnt": s_num, "check_sum": (s_num * (r_num + 1)) & M64}
+
+
+def payload_ref_closed_form(r_num: int, s_num: int) -> dict:
+    """Reference payload-style PK-FK (src/gen.c:226-283): row_id = 1-based
+    position on both sides: checksum = |S| * (|S| + |R| + 2) / 2 mod 2^64."""
+    return {"matched_cnt": s_num, "check_sum": (s_num * (s_num + r_num + 2) // 2) & M64}
+
+
+def direct_address(R: np.ndarray, S: np.ndarray, width: int) -> dict:
+    """R keys a permutation of 1..|R|: P[key] = payload, then
+    count = |S|, checksum = sum_s (P[s.key] + s.payload)."""
+    rk, rp = keys_payloads(R, width)
+    sk, sp = keys_payloads(S, width)
+    P = np.zeros(rk.size + 1, dtype=np.uint64)
+    P[rk.astype(np.int64)] = rp
+    with np.errstate(over="ignore"):
+        s = int(P[sk.astype(np.int64)].sum(dtype=np.uint64)) + int(sp.sum(dtype=np.uint64))
+    return {"matched_cnt": int(sk.size), "check_sum": s & M64}

@@ -1,0 +1,170 @@
+"""GPU parity: every join through the C ABI (libhjmi.so) against the CPU
+oracle on the same seeded inputs, bit-exact count and 64-bit checksum.
+
+Small and medium sizes are checked against the restated reference joins and
+the sort-based aggregation; full benchmark and reference sizes are checked
+against closed forms (the reference's logged golden vectors)."""
+import numpy as np
+import pytest
+
+import hjmi
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+ALGOS = list(hjmi.ALGOS)
+
+
+def _check(got, want, what=""):
+    assert (got["matched_cnt"], got["check_sum"]) == (want["matched_cnt"], want["check_sum"]), (what, got, want)
+
+
+def _dev(ctx, arr, width):
+    return ctx.upload(arr, width)
+
+
+# ----------------------------------------------------------------- generator
+@pytest.mark.parametrize("wl,width,theta", [("pkfk", 16, 1.0), ("pkfk_ref", 16, 1.0), ("payload_ref", 32, 1.0),
+                                            ("zipf", 64, 1.5), ("zipf", 16, 0.5), ("random", 128, 1.0)])
+def test_device_generator_matches_host(ctx, wl, width, theta):
+    nr, ns = 3001, 20011
+    R, S = ctx.generate(wl, nr, ns, width, seed=4242, theta=theta, key_range=2000)
+    hr, hs = O.generate(wl, nr, ns, width, seed=4242, theta=theta, key_range=2000)
+    assert np.array_equal(ctx.download(R), hr)
+    assert np.array_equal(ctx.download(S), hs)
+    ctx.free_relation(R)
+    ctx.free_relation(S)
+
+
+# ------------------------------------------------------- small/medium parity
+@pytest.mark.parametrize("algo", ALGOS)
+@pytest.mark.parametrize("wl,nr,ns,width", [("pkfk", 1 << 12, 1 << 16, 16), ("pkfk_ref", 5000, 80000, 16),
+                                           ("payload_ref", 1 << 14, 1 << 18, 32), ("zipf", 1 << 13, 1 << 17, 64)])
+def test_join_matches_oracle(ctx, algo, wl, nr, ns, width):
+    R, S = O.generate(wl, nr, ns, width, theta=1.0)
+    want = O.join(algo, R, S, width, threads=4, radix_bits=6, passes=1)
+    _check(want, O.aggregate(R, S, width, lp=algo.endswith("lp")), "oracle self-check")
+    dR, dS = _dev(ctx, R, width), _dev(ctx, S, width)
+    _check(ctx.join(algo, dR, dS), want, algo)
+    ctx.free_relation(dR)
+    ctx.free_relation(dS)
+
+
+@pytest.mark.parametrize("algo", ["phj_radix_bc", "phj_radix_lp"])
+@pytest.mark.parametrize("bits,passes", [(0, 0), (1, 1), (4, 1), (8, 1), (11, 1), (12, 1), (2, 2), (9, 2),
+                                         (12, 2), (14, 2), (16, 2), (20, 2)])
+def test_radix_configurations(ctx, algo, bits, passes):
+    nr, ns = 1 << 16, 1 << 20
+    R, S = O.generate("pkfk", nr, ns, 16, seed=11)
+    want = O.pkfk_closed_form(S, 16)
+    dR, dS = _dev(ctx, R, 16), _dev(ctx, S, 16)
+    got = ctx.join(algo, dR, dS, radix_bits=bits, passes=passes)
+    _check(got, want, (bits, passes))
+    assert got["fallback"] == 0
+    ctx.free_relation(dR)
+    ctx.free_relation(dS)
+
+
+@pytest.mark.parametrize("algo", ALGOS)
+@pytest.mark.parametrize("width", [16, 64])
+def test_duplicates_and_misses(ctx, algo, width):
+    """Keys uniform in [1, 3000] on both sides: duplicate build keys, misses.
+    bc/sc count every pair; lp keeps the first build row of each key."""
+    R, S = O.generate("random", 9000, 50000, width, key_range=3000, seed=5)
+    want = O.aggregate(R, S, width, lp=algo.endswith("lp"))
+    dR, dS = _dev(ctx, R, width), _dev(ctx, S, width)
+    for bits, passes in ((0, 0), (3, 1), (6, 2)):
+        got = ctx.join(algo, dR, dS, radix_bits=bits, passes=passes)
+        _check(got, want, (algo, bits, passes))
+        if algo.endswith("lp"):
+            assert got["fallback"] & hjmi.HJ_FB_LP_DUPLICATES
+    ctx.free_relation(dR)
+    ctx.free_relation(dS)
+
+
+def _tuples(keys, pays, width=16):
+    a = np.zeros((len(keys), width // 8), dtype=np.uint64)
+    a[:, 0] = np.array(keys, dtype=np.uint64)
+    a[:, 1] = np.array(pays, dtype=np.uint64)
+    return a.view(np.uint8).reshape(-1)
+
+
+@pytest.mark.parametrize("algo", ALGOS)
+def test_extreme_keys(ctx, algo):
+    """Key 0, key 2^64-1 (the LP empty marker), 2^63; checksums that wrap."""
+    big = (1 << 64) - 1
+    rng = np.random.default_rng(1)
+    rk = [0, big, 5, big, 1 << 63] + list(rng.integers(1, 1 << 62, 500, dtype=np.uint64))
+    rp = [1, 2, 3, 4, big] + list(rng.integers(0, 1 << 63, 500, dtype=np.uint64))
+    sk = [big, 0, 7, 1 << 63, big] + list(rng.choice(np.array(rk, dtype=np.uint64), 4000))
+    sp = [10, 20, 30, 40, big] + list(rng.integers(0, 1 << 63, 4000, dtype=np.uint64))
+    R, S = _tuples(rk, rp), _tuples(sk, sp)
+    want = O.bruteforce(R, S, 16, lp=algo.endswith("lp"))
+    dR, dS = _dev(ctx, R, 16), _dev(ctx, S, 16)
+    for bits, passes in ((0, 0), (2, 1), (7, 2)):
+        _check(ctx.join(algo, dR, dS, radix_bits=bits, passes=passes), want, (algo, bits))
+    ctx.free_relation(dR)
+    ctx.free_relation(dS)
+
+
+@pytest.mark.parametrize("algo", ALGOS)
+def test_empty_and_tiny_relations(ctx, algo):
+    one = _tuples([9], [1])
+    many = _tuples([9, 8, 9, 1], [2, 3, 4, 5])
+    empty = np.zeros(0, dtype=np.uint8)
+    for a, b in ((empty, many), (many, empty), (empty, empty), (one, many), (many, one)):
+        want = O.bruteforce(a, b, 16, lp=algo.endswith("lp"))
+        dA, dB = _dev(ctx, a, 16), _dev(ctx, b, 16)
+        for bits, passes in ((0, 0), (1, 1), (4, 2)):
+            _check(ctx.join(algo, dA, dB, radix_bits=bits, passes=passes), want, (algo, len(a), len(b)))
+        ctx.free_relation(dA)
+        ctx.free_relation(dB)
+
+
+@pytest.mark.parametrize("algo", ["nphj_sc", "phj_radix_bc", "phj_radix_lp", "nphj_lp"])
+def test_one_hot_build_key(ctx, algo):
+    """Every build tuple has the same key: one bucket chain of |R| entries
+    (SC overflow pool grows and rebuilds; radix bc walks an LDS chain over
+    several build chunks; lp falls back to its exact path)."""
+    nr, ns = 20000, 3000
+    R = _tuples([77] * nr, list(range(nr)))
+    S = _tuples([77, 78] * (ns // 2), list(range(ns)))
+    want = O.aggregate(R, S, 16, lp=algo.endswith("lp"))
+    dR, dS = _dev(ctx, R, 16), _dev(ctx, S, 16)
+    _check(ctx.join(algo, dR, dS), want, algo)
+    ctx.free_relation(dR)
+    ctx.free_relation(dS)
+
+
+def test_host_resident_relations(ctx):
+    R, S = O.generate("pkfk", 1 << 12, 1 << 16, 16)
+    want = O.pkfk_closed_form(S, 16)
+    for algo in ALGOS:
+        _check(ctx.join_host(algo, R, S, 16), want, algo)
+
+
+def test_errors_are_status_codes(ctx):
+    R, S = ctx.generate("pkfk", 64, 256, 16)
+    with pytest.raises(hjmi.HJError) as e:
+        ctx.join("phj_radix_zz", R, S)
+    assert e.value.code == hjmi.HJ_EINVAL
+    with pytest.raises(hjmi.HJError) as e:
+        ctx.join("phj_radix_bc", R, S, radix_bits=21)
+    assert e.value.code == hjmi.HJ_EINVAL
+    bad = hjmi.DeviceRelation(S.ptr, S.tuple_num, 24)
+    with pytest.raises(hjmi.HJError):
+        ctx.join("nphj_sc", R, bad)
+    # the context still works after errors
+    assert ctx.join("nphj_sc", R, S)["matched_cnt"] == 256
+    ctx.free_relation(R)
+    ctx.free_relation(S)
+
+
+def test_repeated_joins_reuse_scratch(ctx):
+    R, S = ctx.generate("pkfk", 1 << 15, 1 << 19, 16)
+    first = ctx.join("phj_radix_bc", R, S)
+    for algo in ALGOS * 2:
+        got = ctx.join(algo, R, S)
+        _check(got, first, algo)
+    ctx.free_relation(R)
+    ctx.free_relation(S)

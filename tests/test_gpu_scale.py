@@ -1,0 +1,89 @@
+"""GPU parity at full benchmark and reference sizes, through size-independent
+checks: the reference's logged golden vectors (closed forms of its
+generator) and closed forms of the benchmark workloads."""
+import numpy as np
+import pytest
+
+import hjmi
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+ALGOS = list(hjmi.ALGOS)
+
+
+def _check(got, cnt, csum, what=""):
+    assert (got["matched_cnt"], got["check_sum"]) == (cnt, csum), (what, got)
+
+
+@pytest.mark.parametrize("name,nr,ns", [("uniform_pkfk_A", 1 << 24, 1 << 28), ("uniform_pkfk_B", 1 << 26, 1 << 28),
+                                        ("uniform_pkfk_C", 1 << 28, 1 << 28), ("uniform_pkfk_V", 1 << 27, 1 << 30),
+                                        ("uniform_pkfk_Q", 1 << 28, 1 << 30)])
+def test_reference_logged_pkfk(ctx, name, nr, ns):
+    """Logged: A 268435456/4503599895805952, B .../18014398777917440,
+    C .../72057594306363392, V 1073741824/144115189149597696,
+    Q 1073741824/288230377225453568 (tests/golden/reference_logs.json)."""
+    logged = {"uniform_pkfk_A": 4503599895805952, "uniform_pkfk_B": 18014398777917440,
+              "uniform_pkfk_C": 72057594306363392, "uniform_pkfk_V": 144115189149597696,
+              "uniform_pkfk_Q": 288230377225453568}[name]
+    R, S = ctx.generate("pkfk_ref", nr, ns, 16)
+    algos = ALGOS if name == "uniform_pkfk_A" else ["phj_radix_bc", "phj_radix_lp", "nphj_lp", "nphj_sc"]
+    for algo in algos:
+        _check(ctx.join(algo, R, S), ns, logged, (name, algo))
+    if name == "uniform_pkfk_A":
+        for bits, passes in ((11, 1), (12, 1), (10, 2), (14, 2), (16, 2)):
+            _check(ctx.join("phj_radix_bc", R, S, radix_bits=bits, passes=passes), ns, logged, (bits, passes))
+    ctx.free_relation(R)
+    ctx.free_relation(S)
+
+
+@pytest.mark.parametrize("width", [16, 64])
+def test_reference_logged_payload(ctx, width):
+    """payload_pkfk_16 / _64: 268435456 / 38280597101084672."""
+    R, S = ctx.generate("payload_ref", 1 << 24, 1 << 28, width)
+    for algo in ("phj_radix_bc", "nphj_sc", "phj_radix_lp"):
+        _check(ctx.join(algo, R, S), 268435456, 38280597101084672, (width, algo))
+    ctx.free_relation(R)
+    ctx.free_relation(S)
+
+
+def _s_closed_form(nr, ns, width=16, seed=12345):
+    _, S = O.generate("pkfk", nr, ns, width, seed=seed, threads=16)
+    return O.pkfk_closed_form(S, width)
+
+
+def test_benchmark_config_c1_nphj_sc(ctx):
+    nr, ns = 1 << 20, 1 << 24
+    want = _s_closed_form(nr, ns)
+    R, S = ctx.generate("pkfk", nr, ns, 16)
+    for algo in ALGOS:
+        _check(ctx.join(algo, R, S), want["matched_cnt"], want["check_sum"], algo)
+    ctx.free_relation(R)
+    ctx.free_relation(S)
+
+
+def test_benchmark_config_c2(ctx):
+    nr, ns = 1 << 24, 1 << 28
+    want = _s_closed_form(nr, ns)
+    R, S = ctx.generate("pkfk", nr, ns, 16)
+    for algo in ("phj_radix_bc", "phj_radix_lp", "nphj_lp", "nphj_sc"):
+        _check(ctx.join(algo, R, S), want["matched_cnt"], want["check_sum"], algo)
+    for bits, passes in ((10, 1), (11, 1), (12, 1), (12, 2), (14, 2), (16, 2)):
+        _check(ctx.join("phj_radix_bc", R, S, radix_bits=bits, passes=passes), want["matched_cnt"],
+               want["check_sum"], (bits, passes))
+    ctx.free_relation(R)
+    ctx.free_relation(S)
+
+
+@pytest.mark.parametrize("theta", [0.5, 1.0, 1.5])
+def test_benchmark_config_c3_zipf_wide(ctx, theta):
+    """2^24 x 2^27 64-byte tuples, Zipf probe keys; direct-address oracle."""
+    nr, ns = 1 << 24, 1 << 27
+    hR, hS = O.generate("zipf", nr, ns, 64, theta=theta, threads=16)
+    want = O.direct_address(hR, hS, 64)
+    del hR, hS
+    R, S = ctx.generate("zipf", nr, ns, 64, theta=theta)
+    for algo in ("phj_radix_bc", "phj_radix_lp", "nphj_sc", "nphj_lp"):
+        _check(ctx.join(algo, R, S), want["matched_cnt"], want["check_sum"], (theta, algo))
+    ctx.free_relation(R)
+    ctx.free_relation(S)
