@@ -486,7 +486,8 @@ template <int WV, int NT, int ITEMS>
 static void launch_scatter_t(hj_ctx_t *ctx, unsigned grid, size_t lds, const hj_seg *segs, const uint32_t *cmap,
                              uint64_t ch, uint32_t shift, uint32_t bits, const uint64_t *scan, void *out) {
 	auto kfn = k_scatter<WV, NT, ITEMS>;
-	(void)hipFuncSetAttribute((const void *)kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+	if (hipFuncSetAttribute((const void *)kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess)
+		(void)hipGetLastError();
 	hipLaunchKernelGGL(kfn, dim3(grid), dim3(NT), lds, ctx->stream, segs, cmap, (const hj_stats *)ctx->stats.p, ch,
 	                   shift, bits, scan, (uint4 *)out);
 }
@@ -536,7 +537,8 @@ template <int WV, bool LP>
 static void launch_join_t(hj_ctx_t *ctx, unsigned grid, size_t lds, const void *Rb, const void *Sb, const hj_pair *pairs,
                           const uint64_t *tpre, uint32_t npairs, uint32_t B, uint64_t s_tile, uint32_t cap) {
 	auto kfn = k_join<WV, 256, LP>;
-	(void)hipFuncSetAttribute((const void *)kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+	if (hipFuncSetAttribute((const void *)kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess)
+		(void)hipGetLastError();
 	hipLaunchKernelGGL(kfn, dim3(grid), dim3(256), lds, ctx->stream, (const uint4 *)Rb, (const uint4 *)Sb, pairs,
 	                   tpre, npairs, B, s_tile, cap, (hj_stats *)ctx->stats.p);
 }

@@ -60,7 +60,10 @@ def test_radix_configurations(ctx, algo, bits, passes):
     dR, dS = _dev(ctx, R, 16), _dev(ctx, S, 16)
     got = ctx.join(algo, dR, dS, radix_bits=bits, passes=passes)
     _check(got, want, (bits, passes))
-    assert got["fallback"] == 0
+    if (nr >> got["radix_bits"]) <= 4096:  # partitions fit the LDS table: no exact-path re-run
+        assert got["fallback"] == 0
+    elif algo.endswith("lp"):
+        assert got["fallback"] == hjmi.HJ_FB_LP_OVERSIZE
     ctx.free_relation(dR)
     ctx.free_relation(dS)
 
