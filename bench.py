@@ -106,12 +106,27 @@ def main():
     ap.add_argument("--bits", type=int, default=0)
     ap.add_argument("--passes", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="instead of a join, time a plain device copy of MIB mebibytes (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         log("bench.py: this build measures one GPU; multi-GPU sharding is a design note "
             "(see the sharding paragraph of the design document), so --gpus must be 1")
         sys.exit(2)
     import hjmi
+
+    if args.copy_calibration:
+        ctx = hjmi.Context(0)
+        nb = args.copy_calibration << 20
+        a, b = ctx.alloc(nb), ctx.alloc(nb)
+        for _ in range(args.warmup):
+            ctx.copy_d2d(b, a, nb)
+        ms = [ctx.copy_d2d(b, a, nb) for _ in range(args.steps)]
+        avg = sum(ms) / len(ms)
+        print(json.dumps({"copy_bytes_read": nb, "copy_bytes_written": nb, "avg_ms": avg,
+                          "rw_GBps": 2 * nb / (avg * 1e-3) / 1e9, "launches": len(ms)}), flush=True)
+        ctx.close()
+        return
 
     cfg = dict(CONFIGS[args.config])
     algo = args.algo or cfg["algo"]

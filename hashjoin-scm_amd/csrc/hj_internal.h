@@ -22,6 +22,7 @@ enum hj_kid : uint32_t {
 	HJ_K_SCATTER,
 	HJ_K_PLAN,
 	HJ_K_JOIN,
+	HJ_K_SORTCHUNK,
 	HJ_K_COUNT
 };
 

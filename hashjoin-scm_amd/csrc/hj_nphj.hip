@@ -54,6 +54,27 @@ static unsigned grid_for(hj_ctx_t *ctx, uint64_t n, unsigned threads, unsigned p
 	return (unsigned)b;
 }
 
+__global__ __launch_bounds__(256) void k_copy_u4(uint4 *__restrict__ dst, const uint4 *__restrict__ src, uint64_t n) {
+	const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+	for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+		hj_store_stream(&dst[i], hj_load_stream(&src[i]));
+}
+
+extern "C" int hj_copy_d2d(hj_ctx_t *ctx, void *dst, const void *src, uint64_t bytes, double *ms) {
+	if (!ctx || (bytes && (!dst || !src)) || (bytes & 15)) return HJ_EINVAL;
+	if (!bytes) return HJ_OK;
+	std::lock_guard<std::mutex> lk(ctx->mu);
+	HJ_HIP(ctx, hipSetDevice(ctx->device));
+	HJ_HIP(ctx, hipEventRecord(ctx->ev[0], ctx->stream));
+	hipLaunchKernelGGL(k_copy_u4, dim3(grid_for(ctx, bytes / 16, 256, 8)), dim3(256), 0, ctx->stream, (uint4 *)dst,
+	                   (const uint4 *)src, bytes / 16);
+	HJ_HIP(ctx, hipGetLastError());
+	HJ_HIP(ctx, hipEventRecord(ctx->ev[1], ctx->stream));
+	HJ_HIP(ctx, hipStreamSynchronize(ctx->stream));
+	if (ms) *ms = hj_event_seconds(ctx, 0, 1) * 1e3;
+	return HJ_OK;
+}
+
 int hj_fill16(hj_ctx_t *ctx, void *p, uint64_t n16, uint4 v) {
 	if (!n16) return HJ_OK;
 	hipLaunchKernelGGL(k_fill_u4, dim3(grid_for(ctx, n16, 256, 32)), dim3(256), 0, ctx->stream, (uint4 *)p, n16, v);

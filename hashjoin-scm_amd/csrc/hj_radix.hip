@@ -253,10 +253,11 @@ __global__ __launch_bounds__(NT) void k_scatter(const hj_seg *__restrict__ segs,
 	}
 	__syncthreads();
 	const uint32_t per = (F + NT - 1) / NT;  // digits per thread in the tile scan
-	for (uint64_t t0 = b; t0 < e; t0 += TILE) {
+	// the next tile's tuples are loaded into registers while the current tile
+	// is written out (barriers on gfx950 wait only for LDS, not for loads)
+	uint4 v[ITEMS][WV];
+	auto load_tile = [&](uint64_t t0) {
 		const uint32_t n = (uint32_t)min((uint64_t)TILE, e - t0);
-		uint4 v[ITEMS][WV];
-		uint32_t dig[ITEMS], rank[ITEMS];
 #pragma unroll
 		for (int k = 0; k < ITEMS; k++) {
 			const uint32_t idx = k * NT + threadIdx.x;
@@ -264,12 +265,24 @@ __global__ __launch_bounds__(NT) void k_scatter(const hj_seg *__restrict__ segs,
 				const uint4 *src = g.in + (t0 + idx) * WV;
 #pragma unroll
 				for (int w = 0; w < WV; w++) v[k][w] = hj_load_stream(src + w);
+			}
+		}
+	};
+	if (b < e) load_tile(b);
+	for (uint64_t t0 = b; t0 < e; t0 += TILE) {
+		const uint32_t n = (uint32_t)min((uint64_t)TILE, e - t0);
+		uint32_t dig[ITEMS], rank[ITEMS];
+#pragma unroll
+		for (int k = 0; k < ITEMS; k++) {
+			const uint32_t idx = k * NT + threadIdx.x;
+			if (idx < n) {
 				dig[k] = (uint32_t)(hj_key_of(v[k][0]) >> shift) & fmask;
 				rank[k] = atomicAdd(&tcnt[dig[k]], 1u);
 			}
 		}
 		__syncthreads();
-		// exclusive scan of tcnt -> tbase
+		// exclusive scan of tcnt -> tbase; cursor becomes cursor - tbase so
+		// that a sorted position i maps to cursor[p] + i
 		{
 			const uint32_t p0 = threadIdx.x * per;
 			uint32_t s = 0;
@@ -280,6 +293,7 @@ __global__ __launch_bounds__(NT) void k_scatter(const hj_seg *__restrict__ segs,
 			for (uint32_t q = 0; q < per; q++)
 				if (p0 + q < F) {
 					tbase[p0 + q] = run;
+					cursor[p0 + q] -= run;
 					run += tcnt[p0 + q];
 				}
 		}
@@ -295,16 +309,17 @@ __global__ __launch_bounds__(NT) void k_scatter(const hj_seg *__restrict__ segs,
 			}
 		}
 		__syncthreads();
-		// write the sorted tile out: consecutive lanes -> consecutive words
+		if (t0 + TILE < e) load_tile(t0 + TILE);
+		// write the sorted tile out with streaming stores: consecutive lanes
+		// store consecutive 16-byte words of a run
 		for (uint32_t u = threadIdx.x; u < n * WV; u += NT) {
 			const uint32_t i = u / WV, w = u % WV;
-			const uint32_t p = pid[i];
-			const uint64_t dest = cursor[p] + (i - tbase[p]);
-			out[dest * WV + w] = tile[u];
+			const uint64_t dest = cursor[pid[i]] + i;
+			hj_store_stream(&out[dest * WV + w], tile[u]);
 		}
 		__syncthreads();
 		for (uint32_t p = threadIdx.x; p < F; p += NT) {
-			cursor[p] += tcnt[p];
+			cursor[p] += tbase[p] + tcnt[p];
 			tcnt[p] = 0;
 		}
 		__syncthreads();
@@ -420,7 +435,7 @@ __global__ __launch_bounds__(NT) void k_join(const uint4 *__restrict__ Rb, const
 				for (int u = 0; u < U; u++) {
 					const uint64_t j = j0 + (uint64_t)u * NT;
 					live[u] = j < se;
-					const uint4 t = live[u] ? Sb[j * WV] : make_uint4(0, 0, 0, 0);
+					const uint4 t = live[u] ? hj_load_stream(&Sb[j * WV]) : make_uint4(0, 0, 0, 0);
 					sk[u] = hj_key_of(t);
 					sp[u] = hj_pay_of(t);
 				}
@@ -458,13 +473,314 @@ __global__ __launch_bounds__(NT) void k_join(const uint4 *__restrict__ Rb, const
 	hj_block_add2(cnt, sum, red, &st->matched, &st->checksum);
 }
 
+// ------------------------------------------------------------ pass 2
+// Chunk-local second pass: every chunk (one LDS tile) of a pass-1 partition
+// is counting-sorted by the pass-2 digit and written back to the same range
+// of the output buffer, so the stores of a chunk are one contiguous stream.
+// run[c*(F+1) + p] = start of digit p's run inside chunk c (u16), run[..F] =
+// chunk length; cstart[c] = absolute position of chunk c.  A final
+// partition (p1, p2) is then the union of run p2 of every chunk of p1.
+template <int WV, int NT, int ITEMS>
+__global__ __launch_bounds__(NT) void k_sortchunk(const hj_seg *__restrict__ segs, const uint32_t *__restrict__ cmap,
+                                                  const hj_stats *__restrict__ st, uint32_t shift, uint32_t bits,
+                                                  uint4 *__restrict__ out, uint16_t *__restrict__ run,
+                                                  uint64_t *__restrict__ cstart) {
+	constexpr int TILE = NT * ITEMS;
+	extern __shared__ uint4 lds_u4[];
+	const uint32_t F = 1u << bits, fmask = F - 1;
+	uint4 *tile = lds_u4;
+	uint32_t *tcnt = (uint32_t *)(tile + TILE * WV);
+	uint32_t *tbase = tcnt + F;
+	uint32_t *red = tbase + F + 1;
+	const uint32_t total = (uint32_t)st->total_chunks;
+	const uint32_t per = (F + NT - 1) / NT;
+	uint4 v[ITEMS][WV];
+	auto chunk_of = [&](uint32_t c, const uint4 *&src, uint64_t &pos, uint32_t &n) {
+		const hj_seg g = segs[cmap[c]];
+		const uint64_t cl = c - g.chunk_base;
+		pos = g.out_start + cl * TILE;
+		src = g.in + cl * TILE * WV;
+		n = (uint32_t)min((uint64_t)TILE, g.len - cl * TILE);
+	};
+	auto load = [&](const uint4 *src, uint32_t n) {
+#pragma unroll
+		for (int k = 0; k < ITEMS; k++) {
+			const uint32_t idx = k * NT + threadIdx.x;
+			if (idx < n)
+#pragma unroll
+				for (int w = 0; w < WV; w++) v[k][w] = hj_load_stream(src + (uint64_t)idx * WV + w);
+		}
+	};
+	for (uint32_t p = threadIdx.x; p < F; p += NT) tcnt[p] = 0;
+	const uint4 *src;
+	uint64_t pos;
+	uint32_t n;
+	if (blockIdx.x < total) {
+		chunk_of(blockIdx.x, src, pos, n);
+		load(src, n);
+	}
+	__syncthreads();
+	for (uint32_t c = blockIdx.x; c < total; c += gridDim.x) {
+		uint32_t dig[ITEMS], rank[ITEMS];
+#pragma unroll
+		for (int k = 0; k < ITEMS; k++) {
+			const uint32_t idx = k * NT + threadIdx.x;
+			if (idx < n) {
+				dig[k] = (uint32_t)(hj_key_of(v[k][0]) >> shift) & fmask;
+				rank[k] = atomicAdd(&tcnt[dig[k]], 1u);
+			}
+		}
+		__syncthreads();
+		{
+			const uint32_t p0 = threadIdx.x * per;
+			uint32_t sum = 0;
+			for (uint32_t q = 0; q < per; q++)
+				if (p0 + q < F) sum += tcnt[p0 + q];
+			uint32_t tot;
+			uint32_t r = hj_block_exscan<uint32_t>(sum, red, &tot);
+			for (uint32_t q = 0; q < per; q++)
+				if (p0 + q < F) {
+					tbase[p0 + q] = r;
+					run[(uint64_t)c * (F + 1) + p0 + q] = (uint16_t)r;
+					r += tcnt[p0 + q];
+				}
+			if (threadIdx.x == 0) {
+				run[(uint64_t)c * (F + 1) + F] = (uint16_t)n;
+				cstart[c] = pos;
+			}
+		}
+		__syncthreads();
+#pragma unroll
+		for (int k = 0; k < ITEMS; k++) {
+			const uint32_t idx = k * NT + threadIdx.x;
+			if (idx < n) {
+				const uint32_t q = tbase[dig[k]] + rank[k];
+#pragma unroll
+				for (int w = 0; w < WV; w++) tile[q * WV + w] = v[k][w];
+			}
+		}
+		__syncthreads();
+		const uint32_t n_cur = n;
+		const uint64_t pos_cur = pos;
+		const uint32_t cn = c + gridDim.x;
+		if (cn < total) {
+			chunk_of(cn, src, pos, n);
+			load(src, n);
+		}
+		uint4 *dst = out + pos_cur * WV;
+		for (uint32_t u = threadIdx.x; u < n_cur * WV; u += NT) hj_store_stream(&dst[u], tile[u]);
+		for (uint32_t p = threadIdx.x; p < F; p += NT) tcnt[p] = 0;
+		__syncthreads();
+	}
+}
+
+// Work list of the two-pass join: pair q = (p2 << b1) | p1 holds the chunk id
+// ranges of pass-1 partition p1 of R (segment p1) and of S (segment F1+p1).
+__global__ void k_plan2(const hj_seg *__restrict__ segs2, uint32_t Q, uint32_t b1, uint32_t cpt,
+                        hj_pair *__restrict__ pairs, uint64_t *__restrict__ tiles) {
+	const uint32_t F1 = 1u << b1;
+	for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < Q; q += gridDim.x * blockDim.x) {
+		const hj_seg &r = segs2[q & (F1 - 1)], &s = segs2[F1 + (q & (F1 - 1))];
+		hj_pair pr{r.chunk_base, (uint64_t)r.chunk_base + r.nchunks, s.chunk_base, (uint64_t)s.chunk_base + s.nchunks};
+		pairs[q] = pr;
+		tiles[q] = (r.len && s.len) ? (s.nchunks + cpt - 1) / cpt : 0;
+	}
+}
+
+// Join over chunk runs.  Lanes work in groups of G = 2^glog; a group copies
+// or probes one run at a time.  The build side of a partition is gathered
+// run by run into the LDS table (bc: chained, inserted while copying; lp:
+// inserted after the gather).  A bc build side larger than `cap` is joined
+// in pieces; an lp one sets the oversize flag (exact path on the host).
+// LDS: keys u64[cap] | pays u64[cap] | heads u32[np2(cap)] | next u16[cap]
+template <int WV, int NT, bool LP>
+__global__ __launch_bounds__(NT) void k_join2(const uint4 *__restrict__ P, const uint64_t *__restrict__ cstart,
+                                              const uint16_t *__restrict__ run, uint32_t F2, uint32_t b1,
+                                              const hj_pair *__restrict__ pairs,
+                                              const uint64_t *__restrict__ tprefix, uint32_t npairs, uint32_t B,
+                                              uint32_t cpt, uint32_t cap, uint32_t glog, hj_stats *st) {
+	extern __shared__ uint64_t lds_u64[];
+	__shared__ unsigned long long red[2 * (NT / HJ_WAVE)];
+	__shared__ uint32_t sred[NT / HJ_WAVE + 1];
+	__shared__ uint64_t s_item[4];
+	__shared__ uint64_t r_src[NT];
+	__shared__ uint32_t r_dst[NT];
+	__shared__ uint16_t r_len[NT];
+	uint64_t *keys = lds_u64;
+	uint64_t *pays = keys + cap;
+	uint32_t *heads = (uint32_t *)(pays + cap);
+	const uint32_t nb = 1u << (32 - __clz(cap - 1));
+	const uint32_t hmask = nb - 1;
+	uint16_t *next = (uint16_t *)(heads + nb);
+	const uint32_t G = 1u << glog, gl = threadIdx.x & (G - 1), grp = threadIdx.x >> glog, ngrp = NT >> glog;
+	const uint64_t total = tprefix[npairs];
+	const uint32_t stride = F2 + 1;
+	constexpr int RU = 8;  // runs in flight per lane group
+	unsigned long long cnt = 0, sum = 0;
+	bool dup = false, oversize = false;
+
+	for (;;) {
+		if (threadIdx.x == 0) {
+			const uint64_t w = atomicAdd(&st->work_next, 1ULL);
+			uint32_t lo = 0, hi = npairs;
+			while (hi - lo > 1) {
+				const uint32_t mid = (lo + hi) >> 1;
+				if (tprefix[mid] <= w) lo = mid; else hi = mid;
+			}
+			s_item[0] = w;
+			s_item[1] = lo;
+			s_item[2] = w < total ? w - tprefix[lo] : 0;
+		}
+		__syncthreads();
+		const uint64_t w = s_item[0];
+		const uint32_t q = (uint32_t)s_item[1];
+		const uint64_t tix = s_item[2];
+		__syncthreads();
+		if (w >= total) break;
+		const hj_pair pr = pairs[q];
+		const uint32_t p2 = q >> b1;
+		const uint64_t sb = pr.s0 + tix * cpt, se = min(pr.s1, sb + cpt);
+		uint64_t rc = pr.r0;
+		while (rc < pr.r1) {
+			// ---- gather build runs into the table until it is full
+			for (uint32_t i = threadIdx.x; i < nb; i += NT) heads[i] = 0;
+			uint32_t m = 0;
+			__syncthreads();
+			while (rc < pr.r1) {
+				const uint64_t c = rc + threadIdx.x;
+				uint32_t len = 0, a = 0;
+				if (c < pr.r1) {
+					a = run[c * stride + p2];
+					len = run[c * stride + p2 + 1] - a;
+				}
+				uint32_t tot;
+				const uint32_t ex = hj_block_exscan<uint32_t>(len, sred, &tot);
+				const bool fits = c < pr.r1 && m + ex + len <= cap;
+				const uint32_t take = __syncthreads_count(fits);
+				if (fits) {
+					r_src[threadIdx.x] = cstart[c] + a;
+					r_dst[threadIdx.x] = m + ex;
+					r_len[threadIdx.x] = (uint16_t)len;
+				}
+				__syncthreads();
+				// a group copies RU runs at once so that RU loads are in flight
+				for (uint32_t k0 = grp * RU; k0 < take; k0 += ngrp * RU) {
+					uint32_t Lmax = 0;
+#pragma unroll
+					for (int u = 0; u < RU; u++)
+						if (k0 + u < take) Lmax = max(Lmax, (uint32_t)r_len[k0 + u]);
+					for (uint32_t j = gl; j < Lmax; j += G) {
+						uint4 t[RU];
+#pragma unroll
+						for (int u = 0; u < RU; u++)
+							if (k0 + u < take && j < r_len[k0 + u]) t[u] = P[(r_src[k0 + u] + j) * WV];
+#pragma unroll
+						for (int u = 0; u < RU; u++) {
+							if (!(k0 + u < take && j < r_len[k0 + u])) continue;
+							const uint32_t i = r_dst[k0 + u] + j;
+							const uint64_t key = hj_key_of(t[u]);
+							keys[i] = key;
+							pays[i] = hj_pay_of(t[u]);
+							if (!LP) next[i] = (uint16_t)atomicExch(&heads[(uint32_t)(key >> B) & hmask], i + 1);
+						}
+					}
+				}
+				if (take) m = r_dst[take - 1] + r_len[take - 1];
+				rc += take;
+				__syncthreads();
+				if (take < NT) break;  // table full, or all runs taken
+			}
+			if (LP) {
+				if (rc < pr.r1) {
+					oversize = true;  // the host recomputes this join exactly
+					break;
+				}
+				for (uint32_t i = threadIdx.x; i < m; i += NT) {
+					const uint64_t key = keys[i];
+					uint32_t p = (uint32_t)(key >> B) & hmask;
+					for (uint32_t step = 0; step < nb; step++) {
+						const uint32_t old = atomicCAS(&heads[p], 0u, i + 1);
+						if (old == 0) break;
+						if (keys[old - 1] == key) {
+							dup = true;
+							break;
+						}
+						p = (p + 1) & hmask;
+					}
+				}
+				__syncthreads();
+			}
+			// ---- probe the tile's runs of the probe side
+			// run descriptors of the probe tile are staged in LDS NT at a time;
+			// a group then probes RU runs at once
+			for (uint64_t kb = sb; m && kb < se; kb += NT) {
+				const uint32_t nrun = (uint32_t)min((uint64_t)NT, se - kb);
+				if (threadIdx.x < nrun) {
+					const uint64_t c = kb + threadIdx.x;
+					const uint32_t a = run[c * stride + p2];
+					r_src[threadIdx.x] = cstart[c] + a;
+					r_len[threadIdx.x] = (uint16_t)(run[c * stride + p2 + 1] - a);
+				}
+				__syncthreads();
+				for (uint32_t k0 = grp * RU; k0 < nrun; k0 += ngrp * RU) {
+					uint32_t Lmax = 0;
+#pragma unroll
+					for (int u = 0; u < RU; u++)
+						if (k0 + u < nrun) Lmax = max(Lmax, (uint32_t)r_len[k0 + u]);
+					for (uint32_t j = gl; j < Lmax; j += G) {
+						uint4 tt[RU];
+#pragma unroll
+						for (int u = 0; u < RU; u++)
+							if (k0 + u < nrun && j < r_len[k0 + u]) tt[u] = hj_load_stream(&P[(r_src[k0 + u] + j) * WV]);
+#pragma unroll
+						for (int u = 0; u < RU; u++) {
+						if (!(k0 + u < nrun && j < r_len[k0 + u])) continue;
+						const uint4 t = tt[u];
+						const uint64_t key = hj_key_of(t), pay = hj_pay_of(t);
+						uint32_t h = (uint32_t)(key >> B) & hmask;
+						if (!LP) {
+							for (uint32_t hit = heads[h]; hit; hit = next[hit - 1])
+								if (keys[hit - 1] == key) {
+									cnt++;
+									sum += pays[hit - 1] + pay;
+								}
+						} else {
+							for (uint32_t step = 0; step < nb; step++) {
+								const uint32_t o = heads[h];
+								if (!o) break;
+								if (keys[o - 1] == key) {
+									cnt++;
+									sum += pays[o - 1] + pay;
+									break;
+								}
+								h = (h + 1) & hmask;
+							}
+						}
+						}
+					}
+				}
+				__syncthreads();
+			}
+			__syncthreads();
+		}
+	}
+	const unsigned fl = (__any(dup) ? HJ_DF_DUPKEY : 0u) | (__any(oversize) ? HJ_DF_OVERSIZE : 0u);
+	if (fl && (threadIdx.x & (HJ_WAVE - 1)) == 0) atomicOr(&st->flags, fl);
+	hj_block_add2(cnt, sum, red, &st->matched, &st->checksum);
+}
+
 // ------------------------------------------------------------ host side
 struct part_cfg {
 	int nt, items, tile;
 };
 
-static part_cfg cfg_for(uint32_t wv) {
-	// 32 KB LDS tile of tuples, 512 threads (256 for 128-byte tuples and up)
+// Tile shapes (threads x tuples per thread).  16-byte tuples have several
+// shapes for tuning sweeps; wider tuples use one 32 KB tile.
+static const part_cfg k_shapes16[] = {{1024, 4, 4096}, {512, 4, 2048}, {512, 8, 4096},
+                                      {256, 8, 2048}, {1024, 4, 4096}, {256, 16, 4096}};
+
+static part_cfg cfg_for(uint32_t wv, uint32_t variant = 0) {
+	if (wv == 1) return k_shapes16[variant < 6 ? variant : 0];
 	const int nt = wv >= 8 ? 256 : 512;
 	int items = (int)(32768 / (16 * wv) / nt);
 	if (items < 1) items = 1;
@@ -492,12 +808,13 @@ static void launch_scatter_t(hj_ctx_t *ctx, unsigned grid, size_t lds, const hj_
 	                   shift, bits, scan, (uint4 *)out);
 }
 
-static int launch_pass(hj_ctx_t *ctx, uint32_t wv, unsigned grid, const hj_seg *segs, const uint32_t *cmap,
-                       uint64_t ch, uint32_t shift, uint32_t bits, uint64_t *hist, uint64_t hist_len, void *out) {
-	const part_cfg c = cfg_for(wv);
+static int launch_pass(hj_ctx_t *ctx, uint32_t wv, uint32_t variant, unsigned grid, const hj_seg *segs,
+                       const uint32_t *cmap, uint64_t ch, uint32_t shift, uint32_t bits, uint64_t *hist,
+                       uint64_t hist_len, void *out) {
+	const part_cfg c = cfg_for(wv, variant);
 	const uint32_t F = 1u << bits;
 	const size_t lds = scatter_lds(c, wv, F);
-	if (lds > ctx->prop.sharedMemPerBlock)
+	if (lds + 1024 > ctx->prop.sharedMemPerBlock)
 		return hj_fail(ctx, HJ_EINVAL, "%u radix bits in one pass need %zu B of LDS (device: %zu)", bits, lds,
 		               (size_t)ctx->prop.sharedMemPerBlock);
 	if (grid == 0) return HJ_OK;
@@ -516,7 +833,16 @@ static int launch_pass(hj_ctx_t *ctx, uint32_t wv, unsigned grid, const hj_seg *
 	if (rc) return rc;
 	hj_kt_begin(ctx, HJ_K_SCATTER);
 	switch (wv) {
-	case 1: launch_scatter_t<1, 512, 4>(ctx, grid, lds, segs, cmap, ch, shift, bits, hist, out); break;
+	case 1:
+		switch (variant < 6 ? variant : 0) {
+		case 2: launch_scatter_t<1, 512, 8>(ctx, grid, lds, segs, cmap, ch, shift, bits, hist, out); break;
+		case 3: launch_scatter_t<1, 256, 8>(ctx, grid, lds, segs, cmap, ch, shift, bits, hist, out); break;
+		case 4: launch_scatter_t<1, 1024, 4>(ctx, grid, lds, segs, cmap, ch, shift, bits, hist, out); break;
+		case 5: launch_scatter_t<1, 256, 16>(ctx, grid, lds, segs, cmap, ch, shift, bits, hist, out); break;
+		case 1: launch_scatter_t<1, 512, 4>(ctx, grid, lds, segs, cmap, ch, shift, bits, hist, out); break;
+		default: launch_scatter_t<1, 1024, 4>(ctx, grid, lds, segs, cmap, ch, shift, bits, hist, out); break;
+		}
+		break;
 	case 2: launch_scatter_t<2, 512, 2>(ctx, grid, lds, segs, cmap, ch, shift, bits, hist, out); break;
 	case 4: launch_scatter_t<4, 512, 1>(ctx, grid, lds, segs, cmap, ch, shift, bits, hist, out); break;
 	case 8: launch_scatter_t<8, 256, 1>(ctx, grid, lds, segs, cmap, ch, shift, bits, hist, out); break;
@@ -525,6 +851,24 @@ static int launch_pass(hj_ctx_t *ctx, uint32_t wv, unsigned grid, const hj_seg *
 	hj_kt_end(ctx);
 	HJ_HIP(ctx, hipGetLastError());
 	return HJ_OK;
+}
+
+// Static LDS of a kernel (its __shared__ arrays), so that static + dynamic
+// never exceeds the device's per-workgroup LDS.
+static size_t static_lds(const void *kfn) {
+	hipFuncAttributes a{};
+	if (hipFuncGetAttributes(&a, kfn) != hipSuccess) {
+		(void)hipGetLastError();
+		return 16384;  // conservative if the query fails
+	}
+	return a.sharedSizeBytes;
+}
+
+static size_t join_static_lds(uint32_t wv, bool lp, bool two_pass) {
+	// every instance of one kernel template has the same __shared__ arrays
+	(void)wv;
+	if (two_pass) return lp ? static_lds((const void *)k_join2<1, 512, true>) : static_lds((const void *)k_join2<1, 512, false>);
+	return lp ? static_lds((const void *)k_join<1, 256, true>) : static_lds((const void *)k_join<1, 256, false>);
 }
 
 // LDS bytes of a join table holding `cap` build tuples.
@@ -559,6 +903,56 @@ static void launch_join(hj_ctx_t *ctx, uint32_t wv, bool lp, unsigned grid, size
 #undef J
 }
 
+template <int WV, int NT, int ITEMS>
+static void launch_sortchunk_t(hj_ctx_t *ctx, unsigned grid, size_t lds, const hj_seg *segs, const uint32_t *cmap,
+                               uint32_t shift, uint32_t bits, void *out, uint16_t *run, uint64_t *cst) {
+	auto kfn = k_sortchunk<WV, NT, ITEMS>;
+	if (hipFuncSetAttribute((const void *)kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess)
+		(void)hipGetLastError();
+	hipLaunchKernelGGL(kfn, dim3(grid), dim3(NT), lds, ctx->stream, segs, cmap, (const hj_stats *)ctx->stats.p, shift,
+	                   bits, (uint4 *)out, run, cst);
+}
+
+static void launch_sortchunk(hj_ctx_t *ctx, uint32_t wv, unsigned grid, size_t lds, const hj_seg *segs,
+                             const uint32_t *cmap, uint32_t shift, uint32_t bits, void *out, uint16_t *run,
+                             uint64_t *cst) {
+	// shapes must match cfg_for(wv, 1)
+	switch (wv) {
+	case 1: launch_sortchunk_t<1, 512, 4>(ctx, grid, lds, segs, cmap, shift, bits, out, run, cst); break;
+	case 2: launch_sortchunk_t<2, 512, 2>(ctx, grid, lds, segs, cmap, shift, bits, out, run, cst); break;
+	case 4: launch_sortchunk_t<4, 512, 1>(ctx, grid, lds, segs, cmap, shift, bits, out, run, cst); break;
+	case 8: launch_sortchunk_t<8, 256, 1>(ctx, grid, lds, segs, cmap, shift, bits, out, run, cst); break;
+	default: launch_sortchunk_t<16, 256, 1>(ctx, grid, lds, segs, cmap, shift, bits, out, run, cst); break;
+	}
+}
+
+template <int WV, bool LP>
+static void launch_join2_t(hj_ctx_t *ctx, unsigned grid, size_t lds, const void *P, const uint64_t *cst,
+                           const uint16_t *run, uint32_t F2, uint32_t b1, const hj_pair *pairs, const uint64_t *tpre,
+                           uint32_t npairs, uint32_t B, uint32_t cpt, uint32_t cap, uint32_t glog) {
+	auto kfn = k_join2<WV, 512, LP>;
+	if (hipFuncSetAttribute((const void *)kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess)
+		(void)hipGetLastError();
+	hipLaunchKernelGGL(kfn, dim3(grid), dim3(512), lds, ctx->stream, (const uint4 *)P, cst, run, F2, b1, pairs, tpre,
+	                   npairs, B, cpt, cap, glog, (hj_stats *)ctx->stats.p);
+}
+
+static void launch_join2(hj_ctx_t *ctx, uint32_t wv, bool lp, unsigned grid, size_t lds, const void *P,
+                         const uint64_t *cst, const uint16_t *run, uint32_t F2, uint32_t b1, const hj_pair *pairs,
+                         const uint64_t *tpre, uint32_t npairs, uint32_t B, uint32_t cpt, uint32_t cap, uint32_t glog) {
+#define J(W)                                                                                                   \
+	if (lp) launch_join2_t<W, true>(ctx, grid, lds, P, cst, run, F2, b1, pairs, tpre, npairs, B, cpt, cap, glog); \
+	else launch_join2_t<W, false>(ctx, grid, lds, P, cst, run, F2, b1, pairs, tpre, npairs, B, cpt, cap, glog)
+	switch (wv) {
+	case 1: J(1); break;
+	case 2: J(2); break;
+	case 4: J(4); break;
+	case 8: J(8); break;
+	default: J(16); break;
+	}
+#undef J
+}
+
 // Default radix configuration for |R| (the size-class autotuner's base
 // rule): aim for ~2K build tuples per final partition so that the LDS table
 // stays near 40 KB; two passes once one pass would exceed 2^11 partitions.
@@ -578,13 +972,16 @@ int hj_run_radix(hj_ctx_t *ctx, const hj_relation_t *R, const hj_relation_t *S, 
 	if (P == 0) P = B == 0 ? 0 : (B > 11 ? 2 : 1);
 	if (B == 0) P = 0;
 	if (P == 2 && B < 2) P = 1;
-	const uint32_t b1 = P ? B / P : 0, b2 = B - b1;
+	// two passes: the first takes the larger half of the bits, so the chunk-
+	// local second pass sorts each tile into at most as many runs as pass 1
+	const uint32_t b1 = P == 2 ? (B + 1) / 2 : (P ? B : 0), b2 = B - b1;
 	const uint32_t F1 = 1u << b1, F2 = 1u << b2, Q = 1u << B;
 	if (b1 > 12 || (P == 2 && b2 > 12)) return hj_fail(ctx, HJ_EINVAL, "at most 12 radix bits per pass");
 	out->radix_bits = B;
 	out->passes = P;
 	hj_stats *st = (hj_stats *)ctx->stats.p;
-	const part_cfg pc = cfg_for(wv);
+	const uint32_t variant = prm->part_variant;
+	const part_cfg pc = cfg_for(wv, variant);
 
 	// chunk size: a multiple of the scatter tile, ~4 chunks per CU
 	auto chunk_for = [&](uint64_t n) {
@@ -594,11 +991,12 @@ int hj_run_radix(hj_ctx_t *ctx, const hj_relation_t *R, const hj_relation_t *S, 
 		if (ch < (uint64_t)pc.tile) ch = pc.tile;
 		return ch;
 	};
-	const uint64_t ch1 = chunk_for(N), ch2 = chunk_for(N);
+	const part_cfg pc2 = cfg_for(wv, 1);  // pass-2 chunk = one 32 KB sort tile
+	const uint64_t ch1 = chunk_for(N), ch2 = pc2.tile;
 	const uint64_t chunks1 = (nr + ch1 - 1) / ch1 + (ns + ch1 - 1) / ch1;
 	const uint32_t nseg2 = 2 * F1;
 	const uint64_t chunks2_max = N / ch2 + nseg2 + 1;
-	const uint64_t hist1_len = chunks1 * F1, hist2_len = (P == 2) ? chunks2_max * F2 : 0;
+	const uint64_t hist1_len = chunks1 * F1, hist2_len = 0;
 
 	// scratch
 	int rc;
@@ -607,7 +1005,7 @@ int hj_run_radix(hj_ctx_t *ctx, const hj_relation_t *R, const hj_relation_t *S, 
 	const uint64_t hist_len = std::max(hist1_len, hist2_len) + 1;
 	if ((rc = hj_reserve(ctx, ctx->hist, hist_len * 8))) return rc;
 	if ((rc = hj_reserve(ctx, ctx->scan_tmp, ((hist_len + Q + nseg2) / SCAN_CH + 2) * 8 + 4096))) return rc;
-	// meta: segs1[2] | segs2[nseg2] | cmap[max chunks] | pb1 | pb2 | nch | pairs | tiles
+	// meta: segs1[2] | segs2[nseg2] | cmap[max chunks] | pb1 | nch | pairs | tiles | runs | chunk starts
 	const uint64_t cmap_n = std::max<uint64_t>(chunks1, chunks2_max);
 	size_t off = 0;
 	auto take = [&](size_t bytes) {
@@ -617,15 +1015,19 @@ int hj_run_radix(hj_ctx_t *ctx, const hj_relation_t *R, const hj_relation_t *S, 
 	};
 	const size_t o_seg1 = take(2 * sizeof(hj_seg)), o_seg2 = take(nseg2 * sizeof(hj_seg));
 	const size_t o_cmap = take(cmap_n * 4), o_pb1 = take(((uint64_t)2 * F1 + 1) * 8);
-	const size_t o_pb2 = take(((uint64_t)nseg2 * F2 + 1) * 8), o_nch = take(((uint64_t)nseg2 + 1) * 8);
+	const size_t o_nch = take(((uint64_t)nseg2 + 1) * 8);
 	const size_t o_pairs = take((size_t)Q * sizeof(hj_pair)), o_tiles = take(((size_t)Q + 1) * 8);
+	const size_t o_run = take(P == 2 ? chunks2_max * (F2 + 1) * 2 : 8);
+	const size_t o_cst = take(P == 2 ? chunks2_max * 8 : 8);
 	if ((rc = hj_reserve(ctx, ctx->meta, off))) return rc;
 	char *meta = (char *)ctx->meta.p;
 	hj_seg *segs1 = (hj_seg *)(meta + o_seg1), *segs2 = (hj_seg *)(meta + o_seg2);
 	uint32_t *cmap = (uint32_t *)(meta + o_cmap);
-	uint64_t *pb1 = (uint64_t *)(meta + o_pb1), *pb2 = (uint64_t *)(meta + o_pb2);
+	uint64_t *pb1 = (uint64_t *)(meta + o_pb1);
 	uint64_t *nch = (uint64_t *)(meta + o_nch), *tiles = (uint64_t *)(meta + o_tiles);
 	hj_pair *pairs = (hj_pair *)(meta + o_pairs);
+	uint16_t *runs = (uint16_t *)(meta + o_run);
+	uint64_t *cst = (uint64_t *)(meta + o_cst);
 	uint64_t *hist = (uint64_t *)ctx->hist.p;
 
 	HJ_HIP(ctx, hipEventRecord(ctx->ev[0], ctx->stream));
@@ -636,7 +1038,7 @@ int hj_run_radix(hj_ctx_t *ctx, const hj_relation_t *R, const hj_relation_t *S, 
 		hipLaunchKernelGGL(k_seg_pair, dim3(std::max<unsigned>(1, (unsigned)((chunks1 + 255) / 256))), dim3(256), 0,
 		                   ctx->stream, (const uint4 *)R->tuples, nr, (const uint4 *)S->tuples, ns, ch1, F1, segs1,
 		                   cmap, st);
-		rc = launch_pass(ctx, wv, (unsigned)chunks1, segs1, cmap, ch1, 0, b1, hist, hist1_len, ctx->part[0].p);
+		rc = launch_pass(ctx, wv, variant, (unsigned)chunks1, segs1, cmap, ch1, 0, b1, hist, hist1_len, ctx->part[0].p);
 		if (rc) return rc;
 		hipLaunchKernelGGL(k_bounds, dim3((unsigned)std::min<uint64_t>(1024, (2ull * F1 + 256) / 256)), dim3(256), 0,
 		                   ctx->stream, segs1, 2u, F1, hist, pb1);
@@ -648,12 +1050,12 @@ int hj_run_radix(hj_ctx_t *ctx, const hj_relation_t *R, const hj_relation_t *S, 
 			if ((rc = scan_u64(ctx, nch, nseg2, (uint64_t *)ctx->scan_tmp.p))) return rc;
 			hipLaunchKernelGGL(k_seg2_fin, dim3(std::min<uint32_t>(nseg2, 4096)), dim3(256), 0, ctx->stream, nch,
 			                   nseg2, F2, segs2, cmap, st);
-			rc = launch_pass(ctx, wv, (unsigned)chunks2_max, segs2, cmap, ch2, b1, b2, hist, hist2_len,
-			                 ctx->part[1].p);
-			if (rc) return rc;
-			hipLaunchKernelGGL(k_bounds, dim3((unsigned)std::min<uint64_t>(4096, ((uint64_t)nseg2 * F2 + 256) / 256)),
-			                   dim3(256), 0, ctx->stream, segs2, nseg2, F2, hist, pb2);
-			pbF = pb2;
+			const size_t slds = (size_t)pc2.tile * wv * 16 + (2 * (size_t)F2 + 1) * 4 + 64 * 4;
+			if (slds + 1024 > ctx->prop.sharedMemPerBlock) return hj_fail(ctx, HJ_EINVAL, "pass-2 bits too large for LDS");
+			const unsigned sgrid = (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(chunks2_max, (uint64_t)ctx->num_cu * 4));
+			hj_kt_begin(ctx, HJ_K_SORTCHUNK);
+			launch_sortchunk(ctx, wv, sgrid, slds, segs2, cmap, b1, b2, ctx->part[1].p, runs, cst);
+			hj_kt_end(ctx);
 			Rb = Sb = ctx->part[1].p;
 		}
 	}
@@ -664,20 +1066,35 @@ int hj_run_radix(hj_ctx_t *ctx, const hj_relation_t *R, const hj_relation_t *S, 
 	const uint64_t avg_s = (ns >> B) + 1;
 	uint64_t s_tile = std::max<uint64_t>(8192, 2 * avg_s);
 	const uint64_t avg_r = (nr + Q - 1) / Q;
-	const size_t lds_max = std::min<size_t>(ctx->prop.sharedMemPerBlock, 160 * 1024) - 2048;
+	const size_t lds_max = std::min<size_t>(ctx->prop.sharedMemPerBlock, 160 * 1024) - join_static_lds(wv, lp, P == 2) - 256;
 	uint32_t cap = (uint32_t)std::min<uint64_t>(65535, std::max<uint64_t>(256, avg_r + avg_r / 4 + 64));
-	while (cap > 256 && join_lds(cap, lp) > lds_max) cap -= 256;
+	if (P == 2 && cap < (uint32_t)pc2.tile) cap = (uint32_t)pc2.tile;  // a chunk run always fits
+	while (cap > (P == 2 ? (uint32_t)pc2.tile : 256u) && join_lds(cap, lp) > lds_max) cap -= 256;
 	const size_t jlds = join_lds(cap, lp);
+	if (jlds > lds_max) return hj_fail(ctx, HJ_EINVAL, "join table of %u tuples does not fit LDS", cap);
 	hj_kt_begin(ctx, HJ_K_PLAN);
-	hipLaunchKernelGGL(k_plan, dim3((Q + 255) / 256), dim3(256), 0, ctx->stream, pbF, P, Q, b1, F2, nr, ns, s_tile,
-	                   pairs, tiles);
+	uint32_t cpt = 0;
+	if (P == 2) {
+		const uint64_t avg_chunks = (ns / F1 + pc2.tile - 1) / pc2.tile;
+		cpt = (uint32_t)std::max<uint64_t>(16, avg_chunks + avg_chunks / 4 + 2);  // one item per uniform partition
+		hipLaunchKernelGGL(k_plan2, dim3((Q + 255) / 256), dim3(256), 0, ctx->stream, segs2, Q, b1, cpt, pairs, tiles);
+	} else {
+		hipLaunchKernelGGL(k_plan, dim3((Q + 255) / 256), dim3(256), 0, ctx->stream, pbF, P, Q, b1, F2, nr, ns, s_tile,
+		                   pairs, tiles);
+	}
 	rc = scan_u64(ctx, tiles, Q, (uint64_t *)ctx->scan_tmp.p);
 	hj_kt_end(ctx);
 	if (rc) return rc;
 	const int per_cu = jlds <= 40 * 1024 ? 4 : (jlds <= 76 * 1024 ? 2 : 1);
 	const unsigned jgrid = (unsigned)std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)ctx->num_cu * per_cu, (uint64_t)Q * 64));
 	hj_kt_begin(ctx, HJ_K_JOIN);
-	launch_join(ctx, wv, lp, jgrid, jlds, Rb, Sb, pairs, tiles, Q, B, s_tile, cap);
+	if (P == 2) {
+		uint32_t glog = 4;
+		while (glog < 6 && (pc2.tile >> b2) > (1u << glog)) glog++;
+		launch_join2(ctx, wv, lp, jgrid, jlds, Rb, cst, runs, F2, b1, pairs, tiles, Q, B, cpt, cap, glog);
+	} else {
+		launch_join(ctx, wv, lp, jgrid, jlds, Rb, Sb, pairs, tiles, Q, B, s_tile, cap);
+	}
 	hj_kt_end(ctx);
 	HJ_HIP(ctx, hipGetLastError());
 	HJ_HIP(ctx, hipEventRecord(ctx->ev[2], ctx->stream));

@@ -41,7 +41,7 @@ class Relation(C.Structure):
 
 class Params(C.Structure):
     _fields_ = [("radix_bits", C.c_uint32), ("passes", C.c_uint32), ("lp_slots_log2", C.c_uint32),
-                ("flags", C.c_uint32), ("reserved", C.c_uint32 * 4)]
+                ("flags", C.c_uint32), ("part_variant", C.c_uint32), ("reserved", C.c_uint32 * 3)]
 
 
 class Result(C.Structure):
@@ -99,6 +99,7 @@ def load_library(path: str = LIB_PATH):
         "hj_sync": (I, [P]),
         "hj_generate": (I, [P, C.POINTER(GenSpec), P, P]),
         "hj_device_info": (I, [P, C.POINTER(DeviceInfo)]),
+        "hj_copy_d2d": (I, [P, P, P, U64, C.POINTER(C.c_double)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -111,7 +112,7 @@ def exported_symbols():
     """Names of the C entry points this binding requires."""
     return ["hj_ctx_create", "hj_ctx_destroy", "hj_strerror", "hj_last_error", "hj_algo_count",
             "hj_algo_name", "hj_kernel_name", "hj_join", "hj_dev_alloc", "hj_dev_free", "hj_copy_h2d",
-            "hj_copy_d2h", "hj_sync", "hj_generate", "hj_device_info"]
+            "hj_copy_d2h", "hj_sync", "hj_generate", "hj_device_info", "hj_copy_d2d"]
 
 
 @dataclass
@@ -209,6 +210,13 @@ class Context:
                                              rel.nbytes), "hj_copy_d2h")
         return out
 
+    def copy_d2d(self, dst: int, src: int, nbytes: int) -> float:
+        """Streaming device copy; returns its device time in ms."""
+        ms = C.c_double()
+        self._check(self.lib.hj_copy_d2d(self.h, C.c_void_p(dst), C.c_void_p(src), int(nbytes), C.byref(ms)),
+                    "hj_copy_d2d")
+        return ms.value
+
     def sync(self):
         self._check(self.lib.hj_sync(self.h), "hj_sync")
 
@@ -223,7 +231,7 @@ class Context:
 
     # -- join
     def join(self, algo: str, R, S, radix_bits: int = 0, passes: int = 0, lp_slots_log2: int = 0,
-             kernel_times: bool = False) -> dict:
+             kernel_times: bool = False, part_variant: int = 0) -> dict:
         """Run one join; R/S are DeviceRelation or host numpy uint8 arrays."""
         rr, keep = [], []
         for rel, ts in ((R, None), (S, None)):
@@ -231,7 +239,7 @@ class Context:
                 rr.append(rel.as_c())
             else:
                 raise TypeError("pass DeviceRelation (use Context.upload or Context.host_relation)")
-        prm = Params(radix_bits, passes, lp_slots_log2, HJ_F_KERNEL_TIMES if kernel_times else 0)
+        prm = Params(radix_bits, passes, lp_slots_log2, HJ_F_KERNEL_TIMES if kernel_times else 0, part_variant)
         res = Result()
         self._check(self.lib.hj_join(self.h, algo.encode(), C.byref(rr[0]), C.byref(rr[1]), C.byref(prm),
                                      C.byref(res)), f"hj_join({algo})")

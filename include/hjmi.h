@@ -59,7 +59,9 @@ typedef struct {
 	uint32_t lp_slots_log2;  /* nphj_lp slots = 2^x; 0 = next_pow_2(|R|) as
 	                            src/algo/nphj.c:1556-1558                      */
 	uint32_t flags;          /* HJ_F_*                                        */
-	uint32_t reserved[4];
+	uint32_t part_variant;   /* partition tile shape for 16-byte tuples, for
+	                            tuning sweeps: 0 default, 1..5 fixed shapes  */
+	uint32_t reserved[3];
 } hj_params_t;
 
 #define HJ_F_KERNEL_TIMES 1u  /* record a HIP event pair around every kernel */
@@ -117,6 +119,11 @@ int hj_dev_free(hj_ctx_t *ctx, void *ptr);
 int hj_copy_h2d(hj_ctx_t *ctx, void *dst, const void *src, uint64_t bytes);
 int hj_copy_d2h(hj_ctx_t *ctx, void *dst, const void *src, uint64_t bytes);
 int hj_sync(hj_ctx_t *ctx);
+/* Device-to-device copy by a plain streaming kernel on the context's stream
+ * (no reference counterpart; used to duplicate relations in HBM and as the
+ * known-size copy that calibrates HBM byte counters).  Returns the kernel's
+ * device time in *ms when ms is not NULL.  bytes must be a multiple of 16. */
+int hj_copy_d2d(hj_ctx_t *ctx, void *dst, const void *src, uint64_t bytes, double *ms);
 
 /* ---- seeded generator on the device (replaces bin/gen, src/gen.c:29-158).
  * Fills R (spec->r_num tuples) and S (spec->s_num tuples) of width
