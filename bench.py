@@ -50,6 +50,7 @@ def kernel_bytes(name, W, nr, ns):
     N = nr + ns
     return {
         "radix_scatter": 2 * W * N,
+        "radix_sortchunk": 2 * W * N,
         "radix_join": W * N,
         "lp_build": (W + 16) * nr,
         "sc_build": (W + 16) * nr,
