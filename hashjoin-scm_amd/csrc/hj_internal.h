@@ -52,6 +52,11 @@ struct hj_ctx {
 	hj_buf stats;      // device counters and results
 	hj_buf hostcopy[2];// device copies of host-resident relations
 
+	// size-class autotuner: radix bits / passes per log2 |R| (0xff = unset)
+	uint8_t tune_bits[40];
+	uint8_t tune_passes[40];
+	bool tuned = false;
+
 	// timing
 	static const int NEV = 2 * HJ_MAX_KTIMES + 4;
 	hipEvent_t ev[NEV] = {};
@@ -107,6 +112,10 @@ int hj_run_nphj_sc(hj_ctx_t *ctx, const hj_relation_t *R, const hj_relation_t *S
                    const hj_params_t *prm, hj_result_t *out);
 int hj_run_radix(hj_ctx_t *ctx, const hj_relation_t *R, const hj_relation_t *S,
                  const hj_params_t *prm, bool lp, hj_result_t *out);
+// radix configuration for |R|: the tuned class if the context has one, else
+// the default rule
+void hj_radix_config(hj_ctx_t *ctx, uint64_t nr, uint32_t *bits, uint32_t *passes);
+extern "C" bool hj_algo_is_lp(const char *algo, bool *radix);
 
 // Time between two recorded events of ctx->ev, in seconds.
 double hj_event_seconds(hj_ctx_t *ctx, int a, int b);

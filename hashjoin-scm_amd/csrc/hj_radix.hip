@@ -36,6 +36,7 @@
 #include "hj_internal.h"
 
 #include <algorithm>
+#include <vector>
 
 int hj_fill16(hj_ctx_t *ctx, void *p, uint64_t n16, uint4 v);
 int hj_stats_reset(hj_ctx_t *ctx);
@@ -48,6 +49,8 @@ struct hj_seg {
 	uint64_t hist_base;  // flat index of (digit 0, chunk 0)
 	uint32_t chunk_base; // first global chunk id
 	uint32_t nchunks;
+	uint32_t bits;       // radix digit width of this segment
+	uint32_t pb_base;    // first entry of its partition bounds (batches)
 };
 
 struct hj_pair {
@@ -132,8 +135,9 @@ __global__ void k_seg_pair(const uint4 *R, uint64_t nr, const uint4 *S, uint64_t
                            hj_seg *segs, uint32_t *chunk_seg, hj_stats *st) {
 	const uint32_t cr = (uint32_t)((nr + ch - 1) / ch), cs = (uint32_t)((ns + ch - 1) / ch);
 	if (blockIdx.x == 0 && threadIdx.x == 0) {
-		segs[0] = hj_seg{R, nr, 0, 0, 0, cr};
-		segs[1] = hj_seg{S, ns, nr, (uint64_t)cr * F, cr, cs};
+		const uint32_t bits = 31 - __clz(F);
+		segs[0] = hj_seg{R, nr, 0, 0, 0, cr, bits, 0};
+		segs[1] = hj_seg{S, ns, nr, (uint64_t)cr * F, cr, cs, bits, F + 1};
 		st->total_chunks = cr + cs;
 	}
 	for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < cr + cs; c += gridDim.x * blockDim.x)
@@ -160,7 +164,7 @@ __global__ void k_bounds(const hj_seg *__restrict__ segs, uint32_t nseg, uint32_
 
 // Pass 2 segments: one per pass-1 partition (s2 = s1*F1 + p1).
 __global__ void k_seg2_len(const uint64_t *__restrict__ pb1, uint32_t nseg2, uint64_t ch, const uint4 *in1,
-                           uint32_t wv, hj_seg *__restrict__ segs2, uint64_t *__restrict__ nch) {
+                           uint32_t wv, uint32_t bits2, hj_seg *__restrict__ segs2, uint64_t *__restrict__ nch) {
 	for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < nseg2; s += gridDim.x * blockDim.x) {
 		const uint64_t a = pb1[s], b = pb1[s + 1];
 		hj_seg g;
@@ -170,6 +174,8 @@ __global__ void k_seg2_len(const uint64_t *__restrict__ pb1, uint32_t nseg2, uin
 		g.nchunks = (uint32_t)((g.len + ch - 1) / ch);
 		g.chunk_base = 0;
 		g.hist_base = 0;
+		g.bits = bits2;
+		g.pb_base = 0;
 		segs2[s] = g;
 		nch[s] = g.nchunks;
 	}
@@ -198,8 +204,8 @@ __global__ __launch_bounds__(512) void k_hist(const hj_seg *__restrict__ segs, c
 	extern __shared__ uint32_t cnt[];
 	const uint32_t c = blockIdx.x;
 	if (c >= st->total_chunks) return;
-	const uint32_t F = 1u << bits, fmask = F - 1;
 	const hj_seg g = segs[chunk_seg[c]];
+	const uint32_t F = 1u << min(bits, g.bits), fmask = F - 1;  // `bits` = launch maximum (LDS size)
 	const uint32_t cl = c - g.chunk_base;
 	const uint64_t b = (uint64_t)cl * ch, e = min(g.len, b + ch);
 	for (uint32_t p = threadIdx.x; p < F; p += blockDim.x) cnt[p] = 0;
@@ -234,17 +240,18 @@ __global__ __launch_bounds__(NT) void k_scatter(const hj_seg *__restrict__ segs,
                                                 uint4 *__restrict__ out) {
 	constexpr int TILE = NT * ITEMS;
 	extern __shared__ uint4 lds_u4[];
-	const uint32_t F = 1u << bits, fmask = F - 1;
+	const uint32_t FL = 1u << bits;  // launch maximum: LDS layout
 	uint4 *tile = lds_u4;
 	uint64_t *cursor = (uint64_t *)(tile + TILE * WV);
-	uint32_t *tcnt = (uint32_t *)(cursor + F);
-	uint32_t *tbase = tcnt + F;
-	uint16_t *pid = (uint16_t *)(tbase + F);
+	uint32_t *tcnt = (uint32_t *)(cursor + FL);
+	uint32_t *tbase = tcnt + FL;
+	uint16_t *pid = (uint16_t *)(tbase + FL);
 	uint32_t *red = (uint32_t *)(pid + TILE + 2);
 
 	const uint32_t c = blockIdx.x;
 	if (c >= st->total_chunks) return;
 	const hj_seg g = segs[chunk_seg[c]];
+	const uint32_t F = 1u << min(bits, g.bits), fmask = F - 1;
 	const uint32_t cl = c - g.chunk_base;
 	const uint64_t b = (uint64_t)cl * ch, e = min(g.len, b + ch);
 	for (uint32_t p = threadIdx.x; p < F; p += NT) {
@@ -791,26 +798,37 @@ static size_t scatter_lds(const part_cfg &c, uint32_t wv, uint32_t F) {
 	return (size_t)c.tile * wv * 16 + (size_t)F * 16 + (size_t)(c.tile + 2) * 2 + 64 * 4;
 }
 
+// Launchers take the stats block explicitly: a context's single joins use
+// ctx->stats, a batch uses its own.  `st_dev` is read by the kernels for the
+// chunk count.
+static const hj_stats *g_launch_stats = nullptr;
+
 template <int WV>
 static void launch_hist(hj_ctx_t *ctx, unsigned grid, const hj_seg *segs, const uint32_t *cmap, uint64_t ch,
                         uint32_t shift, uint32_t bits, uint64_t *hist) {
-	hipLaunchKernelGGL(k_hist<WV>, dim3(grid), dim3(512), (size_t)4 << bits, ctx->stream, segs, cmap,
-	                   (const hj_stats *)ctx->stats.p, ch, shift, bits, hist);
+	hipLaunchKernelGGL(k_hist<WV>, dim3(grid), dim3(512), (size_t)4 << bits, ctx->stream, segs, cmap, g_launch_stats,
+	                   ch, shift, bits, hist);
 }
 
 template <int WV, int NT, int ITEMS>
 static void launch_scatter_t(hj_ctx_t *ctx, unsigned grid, size_t lds, const hj_seg *segs, const uint32_t *cmap,
                              uint64_t ch, uint32_t shift, uint32_t bits, const uint64_t *scan, void *out) {
 	auto kfn = k_scatter<WV, NT, ITEMS>;
-	if (hipFuncSetAttribute((const void *)kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess)
+	// attributes are not stream work: set them outside any graph capture
+	// (a batch sets them before it starts capturing)
+	hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+	if (hipStreamIsCapturing(ctx->stream, &cs) != hipSuccess) (void)hipGetLastError();
+	if (cs == hipStreamCaptureStatusNone &&
+	    hipFuncSetAttribute((const void *)kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess)
 		(void)hipGetLastError();
-	hipLaunchKernelGGL(kfn, dim3(grid), dim3(NT), lds, ctx->stream, segs, cmap, (const hj_stats *)ctx->stats.p, ch,
-	                   shift, bits, scan, (uint4 *)out);
+	hipLaunchKernelGGL(kfn, dim3(grid), dim3(NT), lds, ctx->stream, segs, cmap, g_launch_stats, ch, shift, bits, scan,
+	                   (uint4 *)out);
 }
 
 static int launch_pass(hj_ctx_t *ctx, uint32_t wv, uint32_t variant, unsigned grid, const hj_seg *segs,
                        const uint32_t *cmap, uint64_t ch, uint32_t shift, uint32_t bits, uint64_t *hist,
-                       uint64_t hist_len, void *out) {
+                       uint64_t hist_len, void *out, const hj_stats *st, uint64_t *scan_tmp) {
+	g_launch_stats = st;
 	const part_cfg c = cfg_for(wv, variant);
 	const uint32_t F = 1u << bits;
 	const size_t lds = scatter_lds(c, wv, F);
@@ -828,7 +846,7 @@ static int launch_pass(hj_ctx_t *ctx, uint32_t wv, uint32_t variant, unsigned gr
 	}
 	hj_kt_end(ctx);
 	hj_kt_begin(ctx, HJ_K_SCAN);
-	int rc = scan_u64(ctx, hist, hist_len, (uint64_t *)ctx->scan_tmp.p);
+	int rc = scan_u64(ctx, hist, hist_len, scan_tmp);
 	hj_kt_end(ctx);
 	if (rc) return rc;
 	hj_kt_begin(ctx, HJ_K_SCATTER);
@@ -869,6 +887,20 @@ static size_t join_static_lds(uint32_t wv, bool lp, bool two_pass) {
 	(void)wv;
 	if (two_pass) return lp ? static_lds((const void *)k_join2<1, 512, true>) : static_lds((const void *)k_join2<1, 512, false>);
 	return lp ? static_lds((const void *)k_join<1, 256, true>) : static_lds((const void *)k_join<1, 256, false>);
+}
+
+// Dynamic-LDS attribute of the default-shape scatter kernel for width wv.
+static void set_scatter_attr(uint32_t wv, size_t lds) {
+	const void *k = nullptr;
+	switch (wv) {
+	case 1: k = (const void *)k_scatter<1, 1024, 4>; break;
+	case 2: k = (const void *)k_scatter<2, 512, 2>; break;
+	case 4: k = (const void *)k_scatter<4, 512, 1>; break;
+	case 8: k = (const void *)k_scatter<8, 256, 1>; break;
+	default: k = (const void *)k_scatter<16, 256, 1>; break;
+	}
+	if (hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess)
+		(void)hipGetLastError();
 }
 
 // LDS bytes of a join table holding `cap` build tuples.
@@ -963,12 +995,22 @@ void hj_radix_default(uint64_t nr, uint32_t *bits, uint32_t *passes) {
 	*passes = b == 0 ? 0 : (b > 11 ? 2 : 1);
 }
 
+void hj_radix_config(hj_ctx_t *ctx, uint64_t nr, uint32_t *bits, uint32_t *passes) {
+	const uint32_t c = hj_log2_ceil(nr ? nr : 1);
+	if (ctx && ctx->tuned && c < 40 && ctx->tune_bits[c] != 0xff) {
+		*bits = ctx->tune_bits[c];
+		*passes = ctx->tune_passes[c];
+		return;
+	}
+	hj_radix_default(nr, bits, passes);
+}
+
 int hj_run_radix(hj_ctx_t *ctx, const hj_relation_t *R, const hj_relation_t *S, const hj_params_t *prm, bool lp,
                  hj_result_t *out) {
 	const uint32_t W = R->tuple_size, wv = W / 16;
 	const uint64_t nr = R->tuple_num, ns = S->tuple_num, N = nr + ns;
 	uint32_t B = prm->radix_bits, P = prm->passes;
-	if (B == 0 && P == 0) hj_radix_default(nr, &B, &P);
+	if (B == 0 && P == 0) hj_radix_config(ctx, nr, &B, &P);
 	if (P == 0) P = B == 0 ? 0 : (B > 11 ? 2 : 1);
 	if (B == 0) P = 0;
 	if (P == 2 && B < 2) P = 1;
@@ -1038,7 +1080,8 @@ int hj_run_radix(hj_ctx_t *ctx, const hj_relation_t *R, const hj_relation_t *S, 
 		hipLaunchKernelGGL(k_seg_pair, dim3(std::max<unsigned>(1, (unsigned)((chunks1 + 255) / 256))), dim3(256), 0,
 		                   ctx->stream, (const uint4 *)R->tuples, nr, (const uint4 *)S->tuples, ns, ch1, F1, segs1,
 		                   cmap, st);
-		rc = launch_pass(ctx, wv, variant, (unsigned)chunks1, segs1, cmap, ch1, 0, b1, hist, hist1_len, ctx->part[0].p);
+		rc = launch_pass(ctx, wv, variant, (unsigned)chunks1, segs1, cmap, ch1, 0, b1, hist, hist1_len, ctx->part[0].p,
+		                 st, (uint64_t *)ctx->scan_tmp.p);
 		if (rc) return rc;
 		hipLaunchKernelGGL(k_bounds, dim3((unsigned)std::min<uint64_t>(1024, (2ull * F1 + 256) / 256)), dim3(256), 0,
 		                   ctx->stream, segs1, 2u, F1, hist, pb1);
@@ -1046,7 +1089,7 @@ int hj_run_radix(hj_ctx_t *ctx, const hj_relation_t *R, const hj_relation_t *S, 
 		Rb = Sb = ctx->part[0].p;
 		if (P == 2) {
 			hipLaunchKernelGGL(k_seg2_len, dim3((nseg2 + 255) / 256), dim3(256), 0, ctx->stream, pb1, nseg2, ch2,
-			                   (const uint4 *)ctx->part[0].p, wv, segs2, nch);
+			                   (const uint4 *)ctx->part[0].p, wv, b2, segs2, nch);
 			if ((rc = scan_u64(ctx, nch, nseg2, (uint64_t *)ctx->scan_tmp.p))) return rc;
 			hipLaunchKernelGGL(k_seg2_fin, dim3(std::min<uint32_t>(nseg2, 4096)), dim3(256), 0, ctx->stream, nch,
 			                   nseg2, F2, segs2, cmap, st);
@@ -1120,5 +1163,605 @@ int hj_run_radix(hj_ctx_t *ctx, const hj_relation_t *R, const hj_relation_t *S, 
 	}
 	out->matched_cnt = h.matched;
 	out->check_sum = h.checksum;
+	return HJ_OK;
+}
+
+// ======================================================================
+// Batched joins: many independent joins in one captured HIP graph.
+//
+// Planning (host, once per batch shape): each join gets a size class -- a
+// build side that fits one LDS table is not partitioned; larger ones get the
+// radix bits of their class (one pass).  All partitioned joins' R and S
+// become segments of ONE multi-split launch (per-segment digit width), then
+// a planner turns every (join, partition) into probe tiles and one
+// persistent kernel joins all tiles of all joins, adding each tile's count
+// and checksum into that join's result slot.  Integer sums mod 2^64 are
+// order-independent, so each join's result equals its stand-alone result.
+// ======================================================================
+
+struct hj_bjoin {
+	const uint4 *R;
+	const uint4 *S;
+	uint64_t nr, ns;
+	uint32_t bits;       // 0: not partitioned
+	uint32_t seg;        // index of its R segment (S is seg + 1)
+	uint32_t pair_base;  // first pair index of this join
+	uint32_t pad;
+	uint64_t s_tile;     // probe tuples per work item
+};
+
+struct hj_bpair {
+	const uint4 *rb, *sb;
+	uint64_t r0, r1, s0, s1;
+	uint32_t join, shift;
+};
+
+__global__ void k_batch_reset(hj_stats *st, uint64_t nchunks, unsigned long long *res, uint32_t njoins,
+                              unsigned int *jflags) {
+	const uint32_t stride = gridDim.x * blockDim.x;
+	for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < njoins; i += stride) {
+		res[2 * i] = 0;
+		res[2 * i + 1] = 0;
+		jflags[i] = 0;
+	}
+	if (blockIdx.x == 0 && threadIdx.x == 0) {
+		st->work_next = 0;
+		st->flags = 0;
+		st->total_chunks = nchunks;
+	}
+}
+
+// pb[seg.pb_base + p] = first position of digit p, pb[seg.pb_base + F] = end
+__global__ void k_bounds_b(const hj_seg *__restrict__ segs, uint32_t nseg, const uint64_t *__restrict__ scan,
+                           uint64_t *__restrict__ pb) {
+	for (uint32_t s = blockIdx.x; s < nseg; s += gridDim.x) {
+		const hj_seg g = segs[s];
+		const uint32_t F = 1u << g.bits;
+		for (uint32_t p = threadIdx.x; p <= F; p += blockDim.x)
+			pb[g.pb_base + p] = p == F ? g.out_start + g.len
+			                           : (g.nchunks ? scan[g.hist_base + (uint64_t)p * g.nchunks] : g.out_start);
+	}
+}
+
+__global__ void k_plan_b(const hj_bjoin *__restrict__ joins, const uint32_t *__restrict__ pair_join, uint32_t npairs,
+                         const hj_seg *__restrict__ segs, const uint64_t *__restrict__ pb, const uint4 *part,
+                         hj_bpair *__restrict__ pairs, uint64_t *__restrict__ tiles) {
+	for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < npairs; i += gridDim.x * blockDim.x) {
+		const uint32_t j = pair_join[i];
+		const hj_bjoin J = joins[j];
+		hj_bpair pr;
+		pr.join = j;
+		pr.shift = J.bits;
+		if (J.bits == 0) {
+			pr.rb = J.R;
+			pr.sb = J.S;
+			pr.r0 = 0;
+			pr.r1 = J.nr;
+			pr.s0 = 0;
+			pr.s1 = J.ns;
+		} else {
+			const uint32_t q = i - J.pair_base;
+			const uint64_t rb = segs[J.seg].pb_base + q, sb = segs[J.seg + 1].pb_base + q;
+			pr.rb = pr.sb = part;
+			pr.r0 = pb[rb];
+			pr.r1 = pb[rb + 1];
+			pr.s0 = pb[sb];
+			pr.s1 = pb[sb + 1];
+		}
+		pairs[i] = pr;
+		const uint64_t rl = pr.r1 - pr.r0, sl = pr.s1 - pr.s0;
+		tiles[i] = (rl && sl) ? (sl + J.s_tile - 1) / J.s_tile : 0;
+	}
+}
+
+// Persistent join over (join, partition, tile) items; per-item results are
+// wave-reduced and added to res[2*join], res[2*join+1].
+template <int WV, int NT, bool LP>
+__global__ __launch_bounds__(NT) void k_join_b(const hj_bpair *__restrict__ pairs,
+                                               const uint64_t *__restrict__ tprefix, uint32_t npairs,
+                                               const hj_bjoin *__restrict__ joins, uint32_t cap, hj_stats *st,
+                                               unsigned long long *__restrict__ res, unsigned int *__restrict__ jflags) {
+	extern __shared__ uint64_t lds_u64[];
+	__shared__ uint64_t s_item[4];
+	uint64_t *keys = lds_u64;
+	uint64_t *pays = keys + cap;
+	uint32_t *heads = (uint32_t *)(pays + cap);
+	const uint32_t nbmax = 1u << (32 - __clz(cap - 1));
+	uint16_t *next = (uint16_t *)(heads + nbmax);
+	const uint64_t total = tprefix[npairs];
+	const bool lane0 = (threadIdx.x & (HJ_WAVE - 1)) == 0;
+
+	for (;;) {
+		if (threadIdx.x == 0) {
+			const uint64_t w = atomicAdd(&st->work_next, 1ULL);
+			uint32_t lo = 0, hi = npairs;
+			while (hi - lo > 1) {
+				const uint32_t mid = (lo + hi) >> 1;
+				if (tprefix[mid] <= w) lo = mid; else hi = mid;
+			}
+			s_item[0] = w;
+			s_item[1] = lo;
+			s_item[2] = w < total ? w - tprefix[lo] : 0;
+		}
+		__syncthreads();
+		const uint64_t w = s_item[0];
+		const uint32_t q = (uint32_t)s_item[1];
+		const uint64_t tix = s_item[2];
+		__syncthreads();
+		if (w >= total) break;
+		const hj_bpair pr = pairs[q];
+		const uint32_t B = pr.shift;
+		const uint64_t s_tile = joins[pr.join].s_tile;
+		const uint64_t sb = pr.s0 + tix * s_tile, se = min(pr.s1, sb + s_tile);
+		if (LP && pr.r1 - pr.r0 > cap) {
+			if (threadIdx.x == 0) atomicOr(&jflags[pr.join], HJ_DF_OVERSIZE);
+			continue;
+		}
+		unsigned long long cnt = 0, sum = 0;
+		bool dup = false;
+		for (uint64_t rc = pr.r0; rc < pr.r1; rc += cap) {
+			const uint32_t m = (uint32_t)min((uint64_t)cap, pr.r1 - rc);
+			const uint32_t nb = m <= 1 ? 1u : 1u << (32 - __clz(m - 1));
+			const uint32_t hmask = nb - 1;
+			for (uint32_t i = threadIdx.x; i < nb; i += NT) heads[i] = 0;
+			__syncthreads();
+			for (uint32_t i = threadIdx.x; i < m; i += NT) {
+				const uint4 t = pr.rb[(rc + i) * WV];
+				const uint64_t key = hj_key_of(t);
+				keys[i] = key;
+				pays[i] = hj_pay_of(t);
+				if (!LP) next[i] = (uint16_t)atomicExch(&heads[(uint32_t)(key >> B) & hmask], i + 1);
+			}
+			__syncthreads();
+			if (LP) {
+				for (uint32_t i = threadIdx.x; i < m; i += NT) {
+					const uint64_t key = keys[i];
+					uint32_t pos = (uint32_t)(key >> B) & hmask;
+					for (uint32_t step = 0; step < nb; step++) {
+						const uint32_t old = atomicCAS(&heads[pos], 0u, i + 1);
+						if (old == 0) break;
+						if (keys[old - 1] == key) {
+							dup = true;
+							break;
+						}
+						pos = (pos + 1) & hmask;
+					}
+				}
+				__syncthreads();
+			}
+			constexpr int U = 4;
+			for (uint64_t j0 = sb + threadIdx.x; j0 < se; j0 += (uint64_t)U * NT) {
+				uint64_t sk[U], sp[U];
+				bool live[U];
+#pragma unroll
+				for (int u = 0; u < U; u++) {
+					const uint64_t j = j0 + (uint64_t)u * NT;
+					live[u] = j < se;
+					const uint4 t = live[u] ? hj_load_stream(&pr.sb[j * WV]) : make_uint4(0, 0, 0, 0);
+					sk[u] = hj_key_of(t);
+					sp[u] = hj_pay_of(t);
+				}
+#pragma unroll
+				for (int u = 0; u < U; u++) {
+					if (!live[u]) continue;
+					const uint64_t key = sk[u];
+					uint32_t h = (uint32_t)(key >> B) & hmask;
+					if (!LP) {
+						for (uint32_t hit = heads[h]; hit; hit = next[hit - 1])
+							if (keys[hit - 1] == key) {
+								cnt++;
+								sum += pays[hit - 1] + sp[u];
+							}
+					} else {
+						for (uint32_t step = 0; step < nb; step++) {
+							const uint32_t o = heads[h];
+							if (!o) break;
+							if (keys[o - 1] == key) {
+								cnt++;
+								sum += pays[o - 1] + sp[u];
+								break;
+							}
+							h = (h + 1) & hmask;
+						}
+					}
+				}
+			}
+			__syncthreads();
+		}
+		cnt = hj_wave_sum(cnt);
+		sum = hj_wave_sum(sum);
+		if (lane0 && cnt) {
+			atomicAdd(&res[2 * pr.join], cnt);
+			atomicAdd(&res[2 * pr.join + 1], sum);
+		}
+		if (__any(dup) && lane0) atomicOr(&jflags[pr.join], HJ_DF_DUPKEY);
+	}
+}
+
+template <int WV, bool LP>
+static void launch_join_b_t(hj_ctx_t *ctx, unsigned grid, size_t lds, const hj_bpair *pairs, const uint64_t *tiles,
+                            uint32_t npairs, const hj_bjoin *joins, uint32_t cap, hj_stats *st, unsigned long long *res,
+                            unsigned int *jflags) {
+	hipLaunchKernelGGL((k_join_b<WV, 256, LP>), dim3(grid), dim3(256), lds, ctx->stream, pairs, tiles, npairs, joins, cap,
+	                   st, res, jflags);
+}
+
+static void launch_join_b(hj_ctx_t *ctx, uint32_t wv, bool lp, unsigned grid, size_t lds, const hj_bpair *pairs,
+                          const uint64_t *tiles, uint32_t npairs, const hj_bjoin *joins, uint32_t cap, hj_stats *st,
+                          unsigned long long *res, unsigned int *jflags) {
+#define JB(W)                                                                                          \
+	if (lp) launch_join_b_t<W, true>(ctx, grid, lds, pairs, tiles, npairs, joins, cap, st, res, jflags); \
+	else launch_join_b_t<W, false>(ctx, grid, lds, pairs, tiles, npairs, joins, cap, st, res, jflags)
+	switch (wv) {
+	case 1: JB(1); break;
+	case 2: JB(2); break;
+	case 4: JB(4); break;
+	case 8: JB(8); break;
+	default: JB(16); break;
+	}
+#undef JB
+}
+
+struct hj_batch {
+	hj_ctx_t *ctx = nullptr;
+	bool lp = false;
+	uint32_t njoins = 0, wv = 1, W = 16;
+	std::vector<hj_relation_t> R, S;
+	hj_buf part, hist, scan_tmp, meta, res, stats;
+	hipGraph_t graph = nullptr;
+	hipGraphExec_t exec = nullptr;
+	hipEvent_t e0 = nullptr, e1 = nullptr;
+	hj_batch_info_t info{};
+	// device pointers into meta
+	unsigned long long *d_res = nullptr;
+	unsigned int *d_jflags = nullptr;
+};
+
+static void batch_free(hj_batch *b) {
+	if (b->exec) (void)hipGraphExecDestroy(b->exec);
+	if (b->graph) (void)hipGraphDestroy(b->graph);
+	hj_buf *bufs[] = {&b->part, &b->hist, &b->scan_tmp, &b->meta, &b->res, &b->stats};
+	for (hj_buf *x : bufs)
+		if (x->p) (void)hipFree(x->p);
+	if (b->e0) (void)hipEventDestroy(b->e0);
+	if (b->e1) (void)hipEventDestroy(b->e1);
+	delete b;
+}
+
+template <int WV, bool LP>
+static void *join_b_fn() {
+	return (void *)k_join_b<WV, 256, LP>;
+}
+
+static const void *join_b_kernel(uint32_t wv, bool lp) {
+	switch (wv) {
+	case 1: return lp ? join_b_fn<1, true>() : join_b_fn<1, false>();
+	case 2: return lp ? join_b_fn<2, true>() : join_b_fn<2, false>();
+	case 4: return lp ? join_b_fn<4, true>() : join_b_fn<4, false>();
+	case 8: return lp ? join_b_fn<8, true>() : join_b_fn<8, false>();
+	default: return lp ? join_b_fn<16, true>() : join_b_fn<16, false>();
+	}
+}
+
+extern "C" int hj_batch_create(hj_ctx_t *ctx, const char *algo, uint32_t njoins, const hj_relation_t *R,
+                               const hj_relation_t *S, const hj_params_t *params, hj_batch_t **out) {
+	if (!ctx || !algo || !out || (njoins && (!R || !S))) return hj_fail(ctx, HJ_EINVAL, "null argument");
+	*out = nullptr;
+	bool radix = false;
+	const bool lp = hj_algo_is_lp(algo, &radix);
+	if (!radix) return hj_fail(ctx, HJ_EINVAL, "batches run the radix joins (phj_radix_bc/lp), not '%s'", algo);
+	if (params && (params->radix_bits || params->passes))
+		return hj_fail(ctx, HJ_EINVAL, "batches choose radix bits per join: pass radix_bits = passes = 0");
+	std::lock_guard<std::mutex> lk(ctx->mu);
+	HJ_HIP(ctx, hipSetDevice(ctx->device));
+	const uint32_t W = njoins ? R[0].tuple_size : 16;
+	for (uint32_t j = 0; j < njoins; j++) {
+		if (R[j].tuple_size != W || S[j].tuple_size != W)
+			return hj_fail(ctx, HJ_EINVAL, "join %u: every relation of a batch needs the same tuple size", j);
+		if (R[j].location != HJ_LOC_DEVICE || S[j].location != HJ_LOC_DEVICE)
+			return hj_fail(ctx, HJ_EINVAL, "join %u: batch relations must be device resident", j);
+		if ((R[j].tuple_num && !R[j].tuples) || (S[j].tuple_num && !S[j].tuples))
+			return hj_fail(ctx, HJ_EINVAL, "join %u: null tuples", j);
+	}
+	if (W != 16 && W != 32 && W != 64 && W != 128 && W != 256) return hj_fail(ctx, HJ_EINVAL, "tuple size %u", W);
+	hj_batch *b = new hj_batch();
+	b->ctx = ctx;
+	b->lp = lp;
+	b->njoins = njoins;
+	b->W = W;
+	b->wv = W / 16;
+	b->R.assign(R, R + njoins);
+	b->S.assign(S, S + njoins);
+	const uint32_t wv = b->wv;
+	const part_cfg pc = cfg_for(wv, 0);
+
+	// ---- plan on the host
+	const uint32_t small_max = 2048;  // build sides up to this size are joined unpartitioned
+	std::vector<hj_bjoin> joins(njoins);
+	std::vector<hj_seg> segs;
+	std::vector<uint32_t> pair_join;
+	uint64_t part_n = 0, tuples = 0;
+	uint32_t maxbits = 0, pb_total = 0;
+	uint64_t max_rq = 1;
+	for (uint32_t j = 0; j < njoins; j++) {
+		hj_bjoin &J = joins[j];
+		J.R = (const uint4 *)R[j].tuples;
+		J.S = (const uint4 *)S[j].tuples;
+		J.nr = R[j].tuple_num;
+		J.ns = S[j].tuple_num;
+		tuples += J.nr + J.ns;
+		uint32_t bits = 0, passes = 0;
+		if (J.nr > small_max) {
+			hj_radix_config(ctx, J.nr, &bits, &passes);
+			if (bits > 12) bits = 12;  // one pass in a batch
+		}
+		J.bits = bits;
+		J.seg = ~0u;
+		J.pair_base = (uint32_t)pair_join.size();
+		const uint32_t Q = 1u << bits;
+		for (uint32_t q = 0; q < Q; q++) pair_join.push_back(j);
+		J.s_tile = std::max<uint64_t>(8192, 2 * ((J.ns >> bits) + 1));
+		max_rq = std::max<uint64_t>(max_rq, (J.nr >> bits) + 1);
+		if (bits) {
+			J.seg = (uint32_t)segs.size();
+			maxbits = std::max(maxbits, bits);
+			for (int side = 0; side < 2; side++) {
+				hj_seg g{};
+				g.in = side ? J.S : J.R;
+				g.len = side ? J.ns : J.nr;
+				g.out_start = part_n;
+				g.bits = bits;
+				g.pb_base = pb_total;
+				pb_total += Q + 1;
+				part_n += g.len;
+				segs.push_back(g);
+			}
+		} else {
+			b->info.n_unpartitioned++;
+		}
+	}
+	b->info.n_partitioned = njoins - b->info.n_unpartitioned;
+	b->info.njoins = njoins;
+	b->info.tuples = tuples;
+	// chunks: a multiple of the scatter tile, ~4 per CU over the whole batch
+	uint64_t ch = (part_n + (uint64_t)ctx->num_cu * 4 - 1) / ((uint64_t)ctx->num_cu * 4);
+	ch = std::max<uint64_t>(pc.tile, (ch + pc.tile - 1) / pc.tile * pc.tile);
+	std::vector<uint32_t> cmap;
+	uint64_t hist_len = 0;
+	for (uint32_t s = 0; s < segs.size(); s++) {
+		hj_seg &g = segs[s];
+		g.chunk_base = (uint32_t)cmap.size();
+		g.nchunks = (uint32_t)((g.len + ch - 1) / ch);
+		g.hist_base = hist_len;
+		hist_len += (uint64_t)g.nchunks << g.bits;
+		for (uint32_t c = 0; c < g.nchunks; c++) cmap.push_back(s);
+	}
+	const uint64_t nchunks = cmap.size();
+	const uint32_t npairs = (uint32_t)pair_join.size();
+	const uint32_t nseg = (uint32_t)segs.size();
+	const size_t slds = scatter_lds(pc, wv, 1u << maxbits);
+	// LDS table: expected partition size with headroom; bigger build sides
+	// are handled by the kernels (bc: in pieces, lp: exact host path)
+	const size_t lds_max = std::min<size_t>(ctx->prop.sharedMemPerBlock, 160 * 1024) - 1024;
+	uint32_t cap = (uint32_t)std::min<uint64_t>(65535, std::max<uint64_t>(small_max, max_rq + max_rq / 4 + 64));
+	while (cap > 256 && join_lds(cap, lp) > lds_max) cap -= 256;
+	const size_t jlds = join_lds(cap, lp);
+
+	// ---- device buffers owned by the batch
+	int rc = HJ_OK;
+	auto fail = [&](int code) {
+		batch_free(b);
+		return code;
+	};
+	if ((rc = hj_reserve(ctx, b->part, part_n * W + 64))) return fail(rc);
+	if ((rc = hj_reserve(ctx, b->hist, (hist_len + 1) * 8))) return fail(rc);
+	if ((rc = hj_reserve(ctx, b->scan_tmp, ((hist_len + npairs) / SCAN_CH + 4) * 8 + 4096))) return fail(rc);
+	if ((rc = hj_reserve(ctx, b->stats, sizeof(hj_stats)))) return fail(rc);
+	if ((rc = hj_reserve(ctx, b->res, (size_t)njoins * 20 + 64))) return fail(rc);
+	size_t off = 0;
+	auto take = [&](size_t bytes) {
+		size_t o = off;
+		off += (bytes + 255) & ~(size_t)255;
+		return o;
+	};
+	const size_t o_segs = take(nseg * sizeof(hj_seg)), o_cmap = take(nchunks * 4), o_pb = take((pb_total + 1) * 8);
+	const size_t o_joins = take(njoins * sizeof(hj_bjoin)), o_pj = take((size_t)npairs * 4);
+	const size_t o_pairs = take((size_t)npairs * sizeof(hj_bpair)), o_tiles = take(((size_t)npairs + 1) * 8);
+	if ((rc = hj_reserve(ctx, b->meta, off))) return fail(rc);
+	char *meta = (char *)b->meta.p;
+	hj_seg *d_segs = (hj_seg *)(meta + o_segs);
+	uint32_t *d_cmap = (uint32_t *)(meta + o_cmap);
+	uint64_t *d_pb = (uint64_t *)(meta + o_pb), *d_tiles = (uint64_t *)(meta + o_tiles);
+	hj_bjoin *d_joins = (hj_bjoin *)(meta + o_joins);
+	uint32_t *d_pj = (uint32_t *)(meta + o_pj);
+	hj_bpair *d_pairs = (hj_bpair *)(meta + o_pairs);
+	b->d_res = (unsigned long long *)b->res.p;
+	b->d_jflags = (unsigned int *)((char *)b->res.p + (size_t)njoins * 16);
+	hj_stats *d_st = (hj_stats *)b->stats.p;
+	auto up = [&](void *dst, const void *src, size_t bytes) -> int {
+		if (!bytes) return HJ_OK;
+		hipError_t e = hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice);
+		return e == hipSuccess ? HJ_OK : hj_fail(ctx, HJ_EHIP, "batch upload: %s", hipGetErrorString(e));
+	};
+	if ((rc = up(d_segs, segs.data(), nseg * sizeof(hj_seg))) || (rc = up(d_cmap, cmap.data(), nchunks * 4)) ||
+	    (rc = up(d_joins, joins.data(), njoins * sizeof(hj_bjoin))) || (rc = up(d_pj, pair_join.data(), npairs * 4)))
+		return fail(rc);
+	b->info.scratch_bytes = b->part.cap + b->hist.cap + b->scan_tmp.cap + b->meta.cap + b->res.cap + b->stats.cap;
+
+	// kernel attributes are set before capture (not stream work)
+	const void *jk = join_b_kernel(wv, lp);
+	if (hipFuncSetAttribute(jk, hipFuncAttributeMaxDynamicSharedMemorySize, (int)jlds) != hipSuccess)
+		(void)hipGetLastError();
+	if (jlds + static_lds(jk) > ctx->prop.sharedMemPerBlock) return fail(hj_fail(ctx, HJ_EINVAL, "batch LDS table too big"));
+	if (nseg && slds + 1024 > ctx->prop.sharedMemPerBlock) return fail(hj_fail(ctx, HJ_EINVAL, "batch radix bits too big"));
+	if (nseg) set_scatter_attr(wv, slds);
+
+	// ---- capture the batch as one graph
+	const bool kt = ctx->kernel_times;
+	ctx->kernel_times = false;
+	if (hipEventCreate(&b->e0) != hipSuccess || hipEventCreate(&b->e1) != hipSuccess) {
+		(void)hipGetLastError();
+		ctx->kernel_times = kt;
+		return fail(hj_fail(ctx, HJ_EHIP, "event create"));
+	}
+	hipError_t e = hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal);
+	if (e != hipSuccess) {
+		ctx->kernel_times = kt;
+		return fail(hj_fail(ctx, HJ_EHIP, "begin capture: %s", hipGetErrorString(e)));
+	}
+	hipLaunchKernelGGL(k_batch_reset, dim3(std::max<uint32_t>(1, std::min<uint32_t>(1024, (njoins + 255) / 256))),
+	                   dim3(256), 0, ctx->stream, d_st, nchunks, b->d_res, njoins, b->d_jflags);
+	if (nseg) {
+		rc = launch_pass(ctx, wv, 0, (unsigned)nchunks, d_segs, d_cmap, ch, 0, maxbits, (uint64_t *)b->hist.p, hist_len,
+		                 b->part.p, d_st, (uint64_t *)b->scan_tmp.p);
+		if (!rc)
+			hipLaunchKernelGGL(k_bounds_b, dim3(std::min<uint32_t>(nseg, 65535)), dim3(256), 0, ctx->stream, d_segs, nseg,
+			                   (const uint64_t *)b->hist.p, d_pb);
+	}
+	if (!rc && npairs) {
+		hipLaunchKernelGGL(k_plan_b, dim3(std::min<uint32_t>(65535, (npairs + 255) / 256)), dim3(256), 0, ctx->stream,
+		                   d_joins, d_pj, npairs, d_segs, d_pb, (const uint4 *)b->part.p, d_pairs, d_tiles);
+		rc = scan_u64(ctx, d_tiles, npairs, (uint64_t *)b->scan_tmp.p);
+		const int per_cu = jlds <= 40 * 1024 ? 4 : (jlds <= 76 * 1024 ? 2 : 1);
+		const unsigned jgrid = (unsigned)std::max<uint64_t>(1, (uint64_t)ctx->num_cu * per_cu);
+		if (!rc) launch_join_b(ctx, wv, lp, jgrid, jlds, d_pairs, d_tiles, npairs, d_joins, cap, d_st, b->d_res, b->d_jflags);
+	}
+	hipGraph_t g = nullptr;
+	e = hipStreamEndCapture(ctx->stream, &g);
+	ctx->kernel_times = kt;
+	if (rc || e != hipSuccess) {
+		if (g) (void)hipGraphDestroy(g);
+		return fail(rc ? rc : hj_fail(ctx, HJ_EHIP, "end capture: %s", hipGetErrorString(e)));
+	}
+	b->graph = g;
+	e = hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0);
+	if (e != hipSuccess) return fail(hj_fail(ctx, HJ_EHIP, "graph instantiate: %s", hipGetErrorString(e)));
+	*out = b;
+	return HJ_OK;
+}
+
+extern "C" int hj_batch_run(hj_batch_t *b, hj_join_result_t *results, hj_batch_info_t *info) {
+	if (!b) return HJ_EINVAL;
+	hj_ctx_t *ctx = b->ctx;
+	if (b->njoins && !results) return hj_fail(ctx, HJ_EINVAL, "null results");
+	std::lock_guard<std::mutex> lk(ctx->mu);
+	HJ_HIP(ctx, hipSetDevice(ctx->device));
+	HJ_HIP(ctx, hipEventRecord(b->e0, ctx->stream));
+	HJ_HIP(ctx, hipGraphLaunch(b->exec, ctx->stream));
+	HJ_HIP(ctx, hipEventRecord(b->e1, ctx->stream));
+	std::vector<unsigned char> host((size_t)b->njoins * 20 + 8);
+	if (b->njoins)
+		HJ_HIP(ctx, hipMemcpyAsync(host.data(), b->res.p, (size_t)b->njoins * 20, hipMemcpyDeviceToHost, ctx->stream));
+	HJ_HIP(ctx, hipStreamSynchronize(ctx->stream));
+	float ms = 0.f;
+	HJ_HIP(ctx, hipEventElapsedTime(&ms, b->e0, b->e1));
+	b->info.device_ms = ms;
+	b->info.fallback_ms = 0;
+	b->info.n_fallback = 0;
+	const unsigned long long *res = (const unsigned long long *)host.data();
+	const unsigned int *fl = (const unsigned int *)(host.data() + (size_t)b->njoins * 16);
+	for (uint32_t j = 0; j < b->njoins; j++) {
+		results[j].matched_cnt = res[2 * j];
+		results[j].check_sum = res[2 * j + 1];
+		if (b->lp && (fl[j] & (HJ_DF_DUPKEY | HJ_DF_OVERSIZE))) {
+			// exact first-match path for this join, outside the graph
+			hj_result_t r{};
+			hj_params_t p{};
+			const bool kt = ctx->kernel_times;
+			ctx->kernel_times = false;
+			int rc = hj_run_nphj_lp(ctx, &b->R[j], &b->S[j], &p, &r);
+			ctx->kernel_times = kt;
+			if (rc) return rc;
+			results[j].matched_cnt = r.matched_cnt;
+			results[j].check_sum = r.check_sum;
+			b->info.n_fallback++;
+			b->info.fallback_ms += r.total_time * 1e3;
+		}
+	}
+	if (info) *info = b->info;
+	return HJ_OK;
+}
+
+extern "C" int hj_batch_destroy(hj_batch_t *b) {
+	if (!b) return HJ_EINVAL;
+	hj_ctx_t *ctx = b->ctx;
+	std::lock_guard<std::mutex> lk(ctx->mu);
+	(void)hipSetDevice(ctx->device);
+	(void)hipStreamSynchronize(ctx->stream);
+	batch_free(b);
+	return HJ_OK;
+}
+
+// ---------------------------------------------------------------- autotuner
+extern "C" int hj_autotune(hj_ctx_t *ctx, uint32_t width, uint32_t s_ratio, uint32_t min_log2, uint32_t max_log2) {
+	if (!ctx || s_ratio == 0 || min_log2 > max_log2 || max_log2 > 28) return hj_fail(ctx, HJ_EINVAL, "bad tuning range");
+	if (width != 16 && width != 32 && width != 64 && width != 128 && width != 256)
+		return hj_fail(ctx, HJ_EINVAL, "tuple size %u", width);
+	std::lock_guard<std::mutex> lk(ctx->mu);
+	HJ_HIP(ctx, hipSetDevice(ctx->device));
+	const uint64_t nr_max = 1ull << max_log2, ns_max = nr_max * s_ratio;
+	void *r = nullptr, *s = nullptr;
+	if (hipMalloc(&r, nr_max * width) != hipSuccess || hipMalloc(&s, ns_max * width) != hipSuccess) {
+		(void)hipGetLastError();
+		if (r) (void)hipFree(r);
+		return hj_fail(ctx, HJ_ENOMEM, "autotune buffers");
+	}
+	int rc = HJ_OK;
+	const bool kt = ctx->kernel_times;
+	ctx->kernel_times = false;
+	for (uint32_t c = min_log2; c <= max_log2 && rc == HJ_OK; c++) {
+		hj_gen_spec_t spec{};
+		spec.workload = HJ_WL_PKFK_UNIFORM;
+		spec.width = width;
+		spec.seed = 777 + c;
+		spec.r_num = 1ull << c;
+		spec.s_num = (1ull << c) * s_ratio;
+		if ((rc = hj_launch_generate(ctx, &spec, r, s))) break;
+		hj_relation_t R{r, spec.r_num, width, HJ_LOC_DEVICE}, S{s, spec.s_num, width, HJ_LOC_DEVICE};
+		double best = 1e30;
+		uint32_t bb = 0, bp = 0;
+		// candidates: partitions of 512..8192 build tuples, one or two passes
+		for (int b = (int)c - 13; b <= (int)c - 9; b++) {
+			if (b < 0) continue;
+			for (uint32_t p = (b == 0 ? 0 : 1); p <= (b >= 12 ? 2u : (b == 0 ? 0u : 1u)); p++) {
+				if (b > 12 && p == 1) continue;
+				hj_params_t prm{};
+				prm.radix_bits = (uint32_t)b;
+				prm.passes = p;
+				if (b == 0) prm.passes = 0;
+				double t = 1e30;
+				for (int rep = 0; rep < 3; rep++) {
+					hj_result_t out{};
+					rc = hj_run_radix(ctx, &R, &S, &prm, false, &out);
+					if (rc) break;
+					t = std::min(t, out.total_time);
+				}
+				if (rc == HJ_EINVAL) {  // configuration not supported here: skip it
+					rc = HJ_OK;
+					continue;
+				}
+				if (rc) break;
+				if (t < best) {
+					best = t;
+					bb = (uint32_t)b;
+					bp = b ? p : 0;
+				}
+			}
+			if (rc) break;
+		}
+		ctx->tune_bits[c] = (uint8_t)bb;
+		ctx->tune_passes[c] = (uint8_t)bp;
+	}
+	ctx->kernel_times = kt;
+	(void)hipStreamSynchronize(ctx->stream);
+	(void)hipFree(r);
+	(void)hipFree(s);
+	if (rc == HJ_OK) ctx->tuned = true;
+	return rc;
+}
+
+extern "C" int hj_tuned_config(hj_ctx_t *ctx, uint64_t r_num, uint32_t *radix_bits, uint32_t *passes) {
+	if (!ctx || !radix_bits || !passes) return HJ_EINVAL;
+	hj_radix_config(ctx, r_num, radix_bits, passes);
 	return HJ_OK;
 }

@@ -109,6 +109,8 @@ int hj_ctx_create(int device, hj_ctx_t **out) {
 		return HJ_EHIP;
 	}
 	ctx->num_cu = ctx->prop.multiProcessorCount;
+	memset(ctx->tune_bits, 0xff, sizeof ctx->tune_bits);
+	memset(ctx->tune_passes, 0xff, sizeof ctx->tune_passes);
 	for (int i = 0; i < hj_ctx::NEV; i++) {
 		if (hipEventCreate(&ctx->ev[i]) != hipSuccess) {
 			(void)hipGetLastError();
@@ -171,6 +173,16 @@ int hj_algo_count(void) { return (int)(sizeof k_algos / sizeof k_algos[0]) - 1; 
 const char *hj_algo_name(int idx) {
 	if (idx < 0 || idx >= hj_algo_count()) return nullptr;
 	return k_algos[idx].name;
+}
+
+bool hj_algo_is_lp(const char *algo, bool *radix) {
+	*radix = strncmp(algo, "phj_radix_", 10) == 0;
+	bool known = false;
+	for (const hj_algo_entry *a = k_algos; a->name; a++)
+		if (strcmp(a->name, algo) == 0) known = true;
+	if (!known) *radix = false;
+	const size_t n = strlen(algo);
+	return n >= 3 && strcmp(algo + n - 3, "_lp") == 0;
 }
 
 static bool width_ok(uint32_t w) { return w == 16 || w == 32 || w == 64 || w == 128 || w == 256; }

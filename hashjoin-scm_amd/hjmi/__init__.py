@@ -63,6 +63,16 @@ class DeviceInfo(C.Structure):
                 ("arch", C.c_char * 64)]
 
 
+class JoinResult(C.Structure):
+    _fields_ = [("matched_cnt", C.c_uint64), ("check_sum", C.c_uint64)]
+
+
+class BatchInfo(C.Structure):
+    _fields_ = [("device_ms", C.c_double), ("fallback_ms", C.c_double), ("njoins", C.c_uint32),
+                ("n_unpartitioned", C.c_uint32), ("n_partitioned", C.c_uint32), ("n_fallback", C.c_uint32),
+                ("work_items", C.c_uint64), ("tuples", C.c_uint64), ("scratch_bytes", C.c_uint64)]
+
+
 class HJError(RuntimeError):
     def __init__(self, code: int, msg: str):
         super().__init__(f"{msg} (status {code})")
@@ -100,6 +110,12 @@ def load_library(path: str = LIB_PATH):
         "hj_generate": (I, [P, C.POINTER(GenSpec), P, P]),
         "hj_device_info": (I, [P, C.POINTER(DeviceInfo)]),
         "hj_copy_d2d": (I, [P, P, P, U64, C.POINTER(C.c_double)]),
+        "hj_batch_create": (I, [P, C.c_char_p, U32, C.POINTER(Relation), C.POINTER(Relation), C.POINTER(Params),
+                                C.POINTER(P)]),
+        "hj_batch_run": (I, [P, C.POINTER(JoinResult), C.POINTER(BatchInfo)]),
+        "hj_batch_destroy": (I, [P]),
+        "hj_autotune": (I, [P, U32, U32, U32, U32]),
+        "hj_tuned_config": (I, [P, U64, C.POINTER(U32), C.POINTER(U32)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -112,7 +128,8 @@ def exported_symbols():
     """Names of the C entry points this binding requires."""
     return ["hj_ctx_create", "hj_ctx_destroy", "hj_strerror", "hj_last_error", "hj_algo_count",
             "hj_algo_name", "hj_kernel_name", "hj_join", "hj_dev_alloc", "hj_dev_free", "hj_copy_h2d",
-            "hj_copy_d2h", "hj_sync", "hj_generate", "hj_device_info", "hj_copy_d2d"]
+            "hj_copy_d2h", "hj_sync", "hj_generate", "hj_device_info", "hj_copy_d2d", "hj_batch_create",
+            "hj_batch_run", "hj_batch_destroy", "hj_autotune", "hj_tuned_config"]
 
 
 @dataclass
@@ -256,6 +273,17 @@ class Context:
                     f"hj_join({algo})")
         return result_dict(res)
 
+    def batch(self, algo: str, Rs, Ss) -> "Batch":
+        return Batch(self, algo, Rs, Ss)
+
+    def autotune(self, width: int = 16, s_ratio: int = 16, min_log2: int = 10, max_log2: int = 20):
+        self._check(self.lib.hj_autotune(self.h, width, s_ratio, min_log2, max_log2), "hj_autotune")
+
+    def tuned_config(self, r_num: int):
+        b, p = C.c_uint32(), C.c_uint32()
+        self._check(self.lib.hj_tuned_config(self.h, r_num, C.byref(b), C.byref(p)), "hj_tuned_config")
+        return b.value, p.value
+
     def device_info(self) -> dict:
         d = DeviceInfo()
         self._check(self.lib.hj_device_info(self.h, C.byref(d)), "hj_device_info")
@@ -285,3 +313,37 @@ def result_dict(res: Result) -> dict:
 def algo_names():
     lib = load_library()
     return [lib.hj_algo_name(i).decode() for i in range(lib.hj_algo_count())]
+
+
+class Batch:
+    """Many joins as one captured HIP graph (hj_batch_*); per-join results."""
+
+    def __init__(self, ctx: Context, algo: str, Rs, Ss):
+        if len(Rs) != len(Ss):
+            raise ValueError("one build and one probe relation per join")
+        self.ctx, self.n = ctx, len(Rs)
+        self._R = (Relation * max(1, self.n))(*[r.as_c() for r in Rs])
+        self._S = (Relation * max(1, self.n))(*[x.as_c() for x in Ss])
+        h = C.c_void_p()
+        prm = Params()
+        ctx._check(ctx.lib.hj_batch_create(ctx.h, algo.encode(), self.n, self._R, self._S, C.byref(prm),
+                                           C.byref(h)), "hj_batch_create")
+        self.h = h
+
+    def run(self):
+        res = (JoinResult * max(1, self.n))()
+        info = BatchInfo()
+        self.ctx._check(self.ctx.lib.hj_batch_run(self.h, res, C.byref(info)), "hj_batch_run")
+        out = [(res[i].matched_cnt, res[i].check_sum) for i in range(self.n)]
+        return out, {f: getattr(info, f) for f, _ in BatchInfo._fields_}
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.ctx.lib.hj_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -113,6 +113,47 @@ const char *hj_kernel_name(uint32_t id);
 int hj_join(hj_ctx_t *ctx, const char *algo, const hj_relation_t *R, const hj_relation_t *S,
             const hj_params_t *params, hj_result_t *out);
 
+/* ---- batched joins (no reference counterpart: the reference runs one join
+ * per process).  Many independent joins run as one captured HIP graph: one
+ * partition launch over all joins that need partitioning (radix bits chosen
+ * per join by size class), then one persistent join kernel pulling (join,
+ * partition, probe tile) items from a device work queue.  Results are
+ * per join and bit-identical to hj_join on the same relations.
+ * Relations are borrowed for the life of the batch and must stay in device
+ * memory; their contents may change between runs. */
+typedef struct {
+	uint64_t matched_cnt;
+	uint64_t check_sum;
+} hj_join_result_t;
+
+typedef struct {
+	double device_ms;        /* graph execution, device events             */
+	double fallback_ms;      /* host-driven exact re-runs (LP duplicates)  */
+	uint32_t njoins;
+	uint32_t n_unpartitioned;/* joins whose build side fits one LDS table  */
+	uint32_t n_partitioned;
+	uint32_t n_fallback;     /* joins recomputed by the exact LP path      */
+	uint64_t work_items;     /* (join, partition, tile) items in the queue */
+	uint64_t tuples;         /* sum of |R|+|S|                             */
+	uint64_t scratch_bytes;  /* device memory held by the batch            */
+} hj_batch_info_t;
+
+typedef struct hj_batch hj_batch_t;
+
+/* algo: phj_radix_bc or phj_radix_lp (or their bw_reg names).
+ * params: radix_bits/passes must be 0 (chosen per join); flags honoured. */
+int hj_batch_create(hj_ctx_t *ctx, const char *algo, uint32_t njoins, const hj_relation_t *R,
+                    const hj_relation_t *S, const hj_params_t *params, hj_batch_t **out);
+int hj_batch_run(hj_batch_t *batch, hj_join_result_t *results, hj_batch_info_t *info);
+int hj_batch_destroy(hj_batch_t *batch);
+
+/* Size-class autotuner: times each candidate (radix bits, passes) on seeded
+ * joins of |R| = 2^c, |S| = s_ratio * |R| for c in [min_log2, max_log2] and
+ * keeps the fastest per class in the context; batches and hj_join with
+ * radix_bits = 0 then use it.  hj_tuned_config reports the choice for |R|. */
+int hj_autotune(hj_ctx_t *ctx, uint32_t width, uint32_t s_ratio, uint32_t min_log2, uint32_t max_log2);
+int hj_tuned_config(hj_ctx_t *ctx, uint64_t r_num, uint32_t *radix_bits, uint32_t *passes);
+
 /* ---- device memory helpers for hosts without an allocator of their own --- */
 int hj_dev_alloc(hj_ctx_t *ctx, uint64_t bytes, void **ptr);
 int hj_dev_free(hj_ctx_t *ctx, void *ptr);
