@@ -36,6 +36,7 @@
 #include "hj_internal.h"
 
 #include <algorithm>
+#include <string.h>
 #include <vector>
 
 int hj_fill16(hj_ctx_t *ctx, void *p, uint64_t n16, uint4 v);
@@ -1184,10 +1185,12 @@ struct hj_bjoin {
 	const uint4 *S;
 	uint64_t nr, ns;
 	uint32_t bits;       // 0: not partitioned
-	uint32_t seg;        // index of its R segment (S is seg + 1)
+	uint32_t seg;        // index of its first segment (R, then S; S only if R is resident)
 	uint32_t pair_base;  // first pair index of this join
 	uint32_t pad;
 	uint64_t s_tile;     // probe tuples per work item
+	const uint64_t *rpb; // resident, already partitioned build side: bounds ...
+	const uint4 *rpart;  // ... and tuples (null when R is partitioned here)
 };
 
 struct hj_bpair {
@@ -1239,6 +1242,15 @@ __global__ void k_plan_b(const hj_bjoin *__restrict__ joins, const uint32_t *__r
 			pr.r1 = J.nr;
 			pr.s0 = 0;
 			pr.s1 = J.ns;
+		} else if (J.rpb) {
+			const uint32_t q = i - J.pair_base;
+			const uint64_t sb = segs[J.seg].pb_base + q;
+			pr.rb = J.rpart;
+			pr.sb = part;
+			pr.r0 = J.rpb[q];
+			pr.r1 = J.rpb[q + 1];
+			pr.s0 = pb[sb];
+			pr.s1 = pb[sb + 1];
 		} else {
 			const uint32_t q = i - J.pair_base;
 			const uint64_t rb = segs[J.seg].pb_base + q, sb = segs[J.seg + 1].pb_base + q;
@@ -1485,6 +1497,8 @@ extern "C" int hj_batch_create(hj_ctx_t *ctx, const char *algo, uint32_t njoins,
 	uint64_t max_rq = 1;
 	for (uint32_t j = 0; j < njoins; j++) {
 		hj_bjoin &J = joins[j];
+		J.rpb = nullptr;
+		J.rpart = nullptr;
 		J.R = (const uint4 *)R[j].tuples;
 		J.S = (const uint4 *)S[j].tuples;
 		J.nr = R[j].tuple_num;
@@ -1763,5 +1777,558 @@ extern "C" int hj_autotune(hj_ctx_t *ctx, uint32_t width, uint32_t s_ratio, uint
 extern "C" int hj_tuned_config(hj_ctx_t *ctx, uint64_t r_num, uint32_t *radix_bits, uint32_t *passes) {
 	if (!ctx || !radix_bits || !passes) return HJ_EINVAL;
 	hj_radix_config(ctx, r_num, radix_bits, passes);
+	return HJ_OK;
+}
+
+// ======================================================================
+// Serving: continuous batching over per-slot streams and HBM arenas.
+// ======================================================================
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <thread>
+#include <unordered_map>
+
+typedef std::chrono::steady_clock hj_clock;
+
+struct hj_build_res {
+	hj_relation_t R{};
+	uint32_t bits = 0;
+	void *part = nullptr;    // partitioned copy of R (bits > 0)
+	uint64_t *pb = nullptr;  // 2^bits + 1 partition bounds
+	size_t bytes = 0;
+};
+
+struct hj_sreq {
+	int32_t build = -1;
+	hj_relation_t R{}, S{};
+	uint64_t footprint = 0;
+	hj_clock::time_point t_arr, t_done;
+	int state = 0;  // 0 queued, 1 running, 2 done
+	int err = HJ_OK; // status reported by poll for a request that could not run
+	hj_join_result_t res{};
+};
+
+struct hj_slot {
+	hipStream_t stream = nullptr;
+	hipEvent_t done = nullptr;
+	char *arena = nullptr;
+	size_t arena_bytes = 0;
+	unsigned char *hbuf = nullptr;  // pinned: plan upload, then results
+	size_t hbuf_bytes = 0;
+	size_t res_off = 0;
+	std::vector<uint64_t> reqs;
+	uint64_t admitted = 0;
+	bool busy = false;
+};
+
+struct hj_server {
+	hj_ctx_t *ctx = nullptr;
+	bool lp = false;
+	hj_serve_config_t cfg{};
+	uint32_t W = 0;
+	std::vector<hj_slot> slots;
+	std::vector<hj_build_res> builds;
+	std::mutex mu;
+	std::condition_variable cv;
+	std::deque<uint64_t> queue;
+	std::unordered_map<uint64_t, hj_sreq> reqs;
+	uint64_t next_id = 1;
+	bool stop = false;
+	std::thread th;
+	hj_serve_stats_t st{};
+	uint64_t admitted_now = 0;
+};
+
+// tuples per partition chunk when serving: a multiple of every scatter tile
+static const uint64_t kServeChunk = 16384;
+
+static size_t al256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+// Exact scratch a request adds to a batch (its share of every arena region).
+struct serve_req_plan {
+	uint32_t bits;
+	bool r_here;  // R partitioned in this batch
+	uint64_t part_tuples, nchunks, hist_entries, pb_entries, npairs;
+	uint32_t nseg;
+};
+
+static serve_req_plan serve_plan_req(hj_server *s, const hj_sreq &r) {
+	serve_req_plan p{};
+	const uint64_t nr = r.R.tuple_num, ns = r.S.tuple_num;
+	if (r.build >= 0) {
+		p.bits = s->builds[r.build].bits;
+		p.r_here = false;
+	} else {
+		uint32_t b = 0, passes = 0;
+		if (nr > 2048) hj_radix_config(s->ctx, nr, &b, &passes);
+		p.bits = b > 12 ? 12 : b;
+		p.r_here = p.bits > 0;
+	}
+	const uint64_t F = 1ull << p.bits;
+	p.npairs = F;
+	if (p.bits) {
+		const uint64_t nsides[2] = {p.r_here ? nr : UINT64_MAX, ns};
+		for (uint64_t len : nsides) {
+			if (len == UINT64_MAX) continue;
+			const uint64_t c = (len + kServeChunk - 1) / kServeChunk;
+			p.part_tuples += len;
+			p.nchunks += c;
+			p.hist_entries += c * F;
+			p.pb_entries += F + 1;
+			p.nseg++;
+		}
+	}
+	return p;
+}
+
+static uint64_t serve_footprint(const serve_req_plan &p, uint32_t W) {
+	return p.part_tuples * W + p.hist_entries * 8 + p.nchunks * 4 + p.pb_entries * 8 + p.nseg * sizeof(hj_seg) +
+	       sizeof(hj_bjoin) + p.npairs * (sizeof(hj_bpair) + 4 + 8) + 20 + (p.hist_entries + p.npairs) / 512 + 64;
+}
+
+static const uint64_t kServeBatchOverhead = 1u << 20;  // alignment, scan partials, stats
+
+static int serve_grow_pinned(hj_server *s, hj_slot &sl, size_t bytes) {
+	if (sl.hbuf_bytes >= bytes) return HJ_OK;
+	if (sl.hbuf) (void)hipHostFree(sl.hbuf);
+	sl.hbuf = nullptr;
+	sl.hbuf_bytes = 0;
+	size_t want = std::max<size_t>(bytes, 1u << 20);
+	if (hipHostMalloc((void **)&sl.hbuf, want, hipHostMallocDefault) != hipSuccess) {
+		(void)hipGetLastError();
+		return hj_fail(s->ctx, HJ_ENOMEM, "pinned staging buffer");
+	}
+	sl.hbuf_bytes = want;
+	return HJ_OK;
+}
+
+// Plan and enqueue one batch of queued requests on a free slot.  Called with
+// s->mu held; takes ctx->mu for the launches.
+static int serve_launch(hj_server *s, hj_slot &sl, const std::vector<uint64_t> &ids) {
+	hj_ctx_t *ctx = s->ctx;
+	const uint32_t n = (uint32_t)ids.size(), W = s->W, wv = W / 16;
+	std::vector<hj_bjoin> joins(n);
+	std::vector<hj_seg> segs;
+	std::vector<uint32_t> cmap, pj;
+	uint64_t part_n = 0, hist_len = 0, max_rq = 1;
+	uint32_t maxbits = 0, pb_total = 0;
+	for (uint32_t j = 0; j < n; j++) {
+		const hj_sreq &r = s->reqs[ids[j]];
+		const serve_req_plan p = serve_plan_req(s, r);
+		hj_bjoin &J = joins[j];
+		J.R = (const uint4 *)r.R.tuples;
+		J.S = (const uint4 *)r.S.tuples;
+		J.nr = r.R.tuple_num;
+		J.ns = r.S.tuple_num;
+		J.bits = p.bits;
+		J.seg = (uint32_t)segs.size();
+		J.pair_base = (uint32_t)pj.size();
+		J.rpb = nullptr;
+		J.rpart = nullptr;
+		if (r.build >= 0 && p.bits) {
+			J.rpb = s->builds[r.build].pb;
+			J.rpart = (const uint4 *)s->builds[r.build].part;
+		}
+		J.s_tile = std::max<uint64_t>(8192, 2 * ((J.ns >> p.bits) + 1));
+		max_rq = std::max<uint64_t>(max_rq, (J.nr >> p.bits) + 1);
+		for (uint64_t q = 0; q < (1ull << p.bits); q++) pj.push_back(j);
+		if (p.bits) {
+			maxbits = std::max(maxbits, p.bits);
+			for (int side = p.r_here ? 0 : 1; side < 2; side++) {
+				hj_seg g{};
+				g.in = side ? J.S : J.R;
+				g.len = side ? J.ns : J.nr;
+				g.out_start = part_n;
+				g.bits = p.bits;
+				g.pb_base = pb_total;
+				pb_total += (1u << p.bits) + 1;
+				g.chunk_base = (uint32_t)cmap.size();
+				g.nchunks = (uint32_t)((g.len + kServeChunk - 1) / kServeChunk);
+				g.hist_base = hist_len;
+				hist_len += (uint64_t)g.nchunks << p.bits;
+				for (uint32_t c = 0; c < g.nchunks; c++) cmap.push_back((uint32_t)segs.size());
+				part_n += g.len;
+				segs.push_back(g);
+			}
+		}
+	}
+	const uint32_t nseg = (uint32_t)segs.size(), npairs = (uint32_t)pj.size();
+	const uint64_t nchunks = cmap.size();
+	// arena layout
+	size_t off = 0;
+	auto take = [&](size_t b) {
+		size_t o = off;
+		off += al256(b);
+		return o;
+	};
+	const size_t o_part = take(part_n * W + 64), o_hist = take((hist_len + 1) * 8);
+	const size_t o_scan = take(((hist_len + npairs) / SCAN_CH + 4) * 8 + 4096), o_st = take(sizeof(hj_stats));
+	const size_t o_res = take((size_t)n * 20 + 64), o_pb = take(((size_t)pb_total + 1) * 8);
+	const size_t o_pairs = take((size_t)npairs * sizeof(hj_bpair)), o_tiles = take(((size_t)npairs + 1) * 8);
+	// uploaded together: segs | cmap | joins | pair->join
+	const size_t up0 = off;
+	const size_t o_segs = take(nseg * sizeof(hj_seg)), o_cmap = take(nchunks * 4);
+	const size_t o_joins = take((size_t)n * sizeof(hj_bjoin)), o_pj = take((size_t)npairs * 4);
+	const size_t up_bytes = off - up0;
+	if (off > sl.arena_bytes) return hj_fail(ctx, HJ_ENOMEM, "batch does not fit its arena (%zu > %zu)", off, sl.arena_bytes);
+	int rc = serve_grow_pinned(s, sl, up_bytes + (size_t)n * 20 + 4096);
+	if (rc) return rc;
+	memcpy(sl.hbuf + (o_segs - up0), segs.data(), nseg * sizeof(hj_seg));
+	memcpy(sl.hbuf + (o_cmap - up0), cmap.data(), nchunks * 4);
+	memcpy(sl.hbuf + (o_joins - up0), joins.data(), (size_t)n * sizeof(hj_bjoin));
+	memcpy(sl.hbuf + (o_pj - up0), pj.data(), (size_t)npairs * 4);
+	sl.res_off = al256(up_bytes);
+	char *A = sl.arena;
+	hj_seg *d_segs = (hj_seg *)(A + o_segs);
+	uint32_t *d_cmap = (uint32_t *)(A + o_cmap), *d_pj = (uint32_t *)(A + o_pj);
+	hj_bjoin *d_joins = (hj_bjoin *)(A + o_joins);
+	hj_bpair *d_pairs = (hj_bpair *)(A + o_pairs);
+	uint64_t *d_tiles = (uint64_t *)(A + o_tiles), *d_pb = (uint64_t *)(A + o_pb), *d_hist = (uint64_t *)(A + o_hist);
+	hj_stats *d_st = (hj_stats *)(A + o_st);
+	unsigned long long *d_res = (unsigned long long *)(A + o_res);
+	unsigned int *d_fl = (unsigned int *)(A + o_res + (size_t)n * 16);
+	const part_cfg pc = cfg_for(wv, 0);
+	const size_t slds = scatter_lds(pc, wv, 1u << maxbits);
+	const size_t lds_max = std::min<size_t>(ctx->prop.sharedMemPerBlock, 160 * 1024) - 1024;
+	uint32_t cap = (uint32_t)std::min<uint64_t>(65535, std::max<uint64_t>(2048, max_rq + max_rq / 4 + 64));
+	while (cap > 256 && join_lds(cap, s->lp) > lds_max) cap -= 256;
+	const size_t jlds = join_lds(cap, s->lp);
+
+	std::lock_guard<std::mutex> lk(ctx->mu);
+	hipStream_t saved = ctx->stream;
+	const bool kt = ctx->kernel_times;
+	ctx->kernel_times = false;
+	ctx->stream = sl.stream;
+	hipError_t e = hipMemcpyAsync(A + up0, sl.hbuf, up_bytes, hipMemcpyHostToDevice, sl.stream);
+	if (e == hipSuccess) {
+		if (nseg) set_scatter_attr(wv, slds);
+		const void *jk = join_b_kernel(wv, s->lp);
+		if (hipFuncSetAttribute(jk, hipFuncAttributeMaxDynamicSharedMemorySize, (int)jlds) != hipSuccess)
+			(void)hipGetLastError();
+		hipLaunchKernelGGL(k_batch_reset, dim3(std::max<uint32_t>(1, std::min<uint32_t>(1024, (n + 255) / 256))),
+		                   dim3(256), 0, sl.stream, d_st, nchunks, d_res, n, d_fl);
+		if (nseg) {
+			rc = launch_pass(ctx, wv, 0, (unsigned)nchunks, d_segs, d_cmap, kServeChunk, 0, maxbits, d_hist, hist_len,
+			                 A + o_part, d_st, (uint64_t *)(A + o_scan));
+			if (!rc)
+				hipLaunchKernelGGL(k_bounds_b, dim3(std::min<uint32_t>(nseg, 65535)), dim3(256), 0, sl.stream, d_segs,
+				                   nseg, (const uint64_t *)d_hist, d_pb);
+		}
+		if (!rc) {
+			hipLaunchKernelGGL(k_plan_b, dim3(std::min<uint32_t>(65535, (npairs + 255) / 256)), dim3(256), 0, sl.stream,
+			                   d_joins, d_pj, npairs, d_segs, d_pb, (const uint4 *)(A + o_part), d_pairs, d_tiles);
+			rc = scan_u64(ctx, d_tiles, npairs, (uint64_t *)(A + o_scan));
+		}
+		if (!rc) {
+			const int per_cu = jlds <= 40 * 1024 ? 4 : (jlds <= 76 * 1024 ? 2 : 1);
+			launch_join_b(ctx, wv, s->lp, (unsigned)ctx->num_cu * per_cu, jlds, d_pairs, d_tiles, npairs, d_joins, cap,
+			              d_st, d_res, d_fl);
+		}
+		e = hipGetLastError();
+		if (e == hipSuccess) e = hipMemcpyAsync(sl.hbuf + sl.res_off, d_res, (size_t)n * 20, hipMemcpyDeviceToHost, sl.stream);
+		if (e == hipSuccess) e = hipEventRecord(sl.done, sl.stream);
+	}
+	ctx->stream = saved;
+	ctx->kernel_times = kt;
+	if (rc) return rc;
+	if (e != hipSuccess) return hj_fail(ctx, HJ_EHIP, "serve launch: %s", hipGetErrorString(e));
+	sl.reqs = ids;
+	sl.busy = true;
+	return HJ_OK;
+}
+
+static void serve_finish(hj_server *s, hj_slot &sl) {
+	const uint32_t n = (uint32_t)sl.reqs.size();
+	const unsigned long long *res = (const unsigned long long *)(sl.hbuf + sl.res_off);
+	const unsigned int *fl = (const unsigned int *)(sl.hbuf + sl.res_off + (size_t)n * 16);
+	for (uint32_t j = 0; j < n; j++) {
+		hj_sreq &r = s->reqs[sl.reqs[j]];
+		r.res.matched_cnt = res[2 * j];
+		r.res.check_sum = res[2 * j + 1];
+		if (s->lp && (fl[j] & (HJ_DF_DUPKEY | HJ_DF_OVERSIZE))) {
+			// exact first-match path for this request
+			std::lock_guard<std::mutex> lk(s->ctx->mu);
+			hj_result_t out{};
+			hj_params_t p{};
+			const hj_relation_t &R = r.build >= 0 ? s->builds[r.build].R : r.R;
+			r.err = hj_run_nphj_lp(s->ctx, &R, &r.S, &p, &out);
+			r.res.matched_cnt = out.matched_cnt;
+			r.res.check_sum = out.check_sum;
+		}
+		r.t_done = hj_clock::now();
+		r.state = 2;
+		s->st.completed++;
+		s->admitted_now -= r.footprint;
+	}
+	sl.reqs.clear();
+	sl.busy = false;
+}
+
+static void serve_loop(hj_server *s) {
+	(void)hipSetDevice(s->ctx->device);
+	std::unique_lock<std::mutex> lk(s->mu);
+	for (;;) {
+		bool progress = false;
+		uint32_t busy = 0;
+		for (hj_slot &sl : s->slots) {
+			if (!sl.busy) continue;
+			const hipError_t q = hipEventQuery(sl.done);
+			if (q == hipSuccess) {
+				serve_finish(s, sl);
+				progress = true;
+			} else {
+				if (q != hipErrorNotReady) (void)hipGetLastError();
+				busy++;
+			}
+		}
+		for (hj_slot &sl : s->slots) {
+			if (sl.busy || s->queue.empty()) continue;
+			std::vector<uint64_t> ids;
+			uint64_t fp = kServeBatchOverhead;
+			while (!s->queue.empty() && ids.size() < s->cfg.max_batch) {
+				hj_sreq &r = s->reqs[s->queue.front()];
+				if (fp + r.footprint + 64 * 256 > sl.arena_bytes) break;  // FIFO: wait for memory
+				fp += r.footprint + 64 * 256;
+				ids.push_back(s->queue.front());
+				s->queue.pop_front();
+			}
+			if (ids.empty()) continue;
+			for (uint64_t id : ids) s->reqs[id].state = 1;
+			const int rc = serve_launch(s, sl, ids);
+			if (rc != HJ_OK) {  // these requests fail with the launch's status
+				for (uint64_t id : ids) {
+					hj_sreq &r = s->reqs[id];
+					r.state = 2;
+					r.err = rc;
+					r.t_done = hj_clock::now();
+					s->st.completed++;
+				}
+				continue;
+			}
+			for (uint64_t id : ids) s->admitted_now += s->reqs[id].footprint;
+			s->st.batches++;
+			s->st.max_batch_joins = std::max<uint64_t>(s->st.max_batch_joins, ids.size());
+			s->st.peak_admitted_bytes = std::max(s->st.peak_admitted_bytes, s->admitted_now);
+			busy++;
+			progress = true;
+		}
+		s->st.max_busy_slots = std::max(s->st.max_busy_slots, busy);
+		if (progress) s->cv.notify_all();
+		if (s->stop && s->queue.empty() && busy == 0) break;
+		if (!progress) {
+			if (busy == 0 && s->queue.empty())
+				s->cv.wait_for(lk, std::chrono::milliseconds(5));
+			else {
+				lk.unlock();
+				std::this_thread::sleep_for(std::chrono::microseconds(20));
+				lk.lock();
+			}
+		}
+	}
+}
+
+extern "C" int hj_serve_open(hj_ctx_t *ctx, const char *algo, const hj_serve_config_t *cfg, hj_server_t **out) {
+	if (!ctx || !algo || !cfg || !out) return hj_fail(ctx, HJ_EINVAL, "null argument");
+	*out = nullptr;
+	bool radix = false;
+	const bool lp = hj_algo_is_lp(algo, &radix);
+	if (!radix) return hj_fail(ctx, HJ_EINVAL, "serving runs the radix joins (phj_radix_bc/lp), not '%s'", algo);
+	if (cfg->slots < 1 || cfg->slots > 8 || cfg->max_batch < 1 || cfg->hbm_cap_bytes < ((uint64_t)cfg->slots << 24))
+		return hj_fail(ctx, HJ_EINVAL, "serve config: 1..8 slots, max_batch >= 1, cap >= 16 MiB per slot");
+	hj_server *s = new hj_server();
+	s->ctx = ctx;
+	s->lp = lp;
+	s->cfg = *cfg;
+	s->slots.resize(cfg->slots);
+	const size_t per = (size_t)(cfg->hbm_cap_bytes / cfg->slots) & ~(size_t)4095;
+	int rc = HJ_OK;
+	{
+		std::lock_guard<std::mutex> lk(ctx->mu);
+		(void)hipSetDevice(ctx->device);
+		for (hj_slot &sl : s->slots) {
+			if (hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking) != hipSuccess ||
+			    hipEventCreateWithFlags(&sl.done, hipEventDisableTiming) != hipSuccess) {
+				(void)hipGetLastError();
+				rc = hj_fail(ctx, HJ_EHIP, "slot stream/event");
+				break;
+			}
+			if (hipMalloc(&sl.arena, per) != hipSuccess) {
+				(void)hipGetLastError();
+				sl.arena = nullptr;
+				rc = hj_fail(ctx, HJ_ENOMEM, "slot arena of %zu bytes", per);
+				break;
+			}
+			sl.arena_bytes = per;
+			s->st.arena_bytes += per;
+		}
+	}
+	if (rc) {
+		for (hj_slot &sl : s->slots) {
+			if (sl.arena) (void)hipFree(sl.arena);
+			if (sl.done) (void)hipEventDestroy(sl.done);
+			if (sl.stream) (void)hipStreamDestroy(sl.stream);
+		}
+		delete s;
+		return rc;
+	}
+	s->th = std::thread(serve_loop, s);
+	*out = s;
+	return HJ_OK;
+}
+
+extern "C" int hj_serve_register_build(hj_server_t *s, const hj_relation_t *R, uint32_t *build_id) {
+	if (!s || !R || !build_id) return HJ_EINVAL;
+	hj_ctx_t *ctx = s->ctx;
+	if (R->location != HJ_LOC_DEVICE || (R->tuple_num && !R->tuples)) return hj_fail(ctx, HJ_EINVAL, "device build relation required");
+	std::lock_guard<std::mutex> lk(s->mu);
+	if (s->W == 0) s->W = R->tuple_size;
+	if (R->tuple_size != s->W || (s->W != 16 && s->W != 32 && s->W != 64 && s->W != 128 && s->W != 256))
+		return hj_fail(ctx, HJ_EINVAL, "tuple size %u", R->tuple_size);
+	hj_build_res b;
+	b.R = *R;
+	uint32_t bits = 0, passes = 0;
+	if (R->tuple_num > 2048) hj_radix_config(ctx, R->tuple_num, &bits, &passes);
+	b.bits = bits > 12 ? 12 : bits;
+	if (b.bits) {
+		std::lock_guard<std::mutex> lk2(ctx->mu);
+		const uint32_t wv = s->W / 16, F = 1u << b.bits;
+		const uint64_t n = R->tuple_num, nch = (n + kServeChunk - 1) / kServeChunk;
+		hj_seg g{};
+		g.in = (const uint4 *)R->tuples;
+		g.len = n;
+		g.bits = b.bits;
+		g.nchunks = (uint32_t)nch;
+		std::vector<uint32_t> cmap(nch, 0);
+		char *tmp = nullptr;
+		const size_t t_hist = al256((nch * F + 1) * 8), t_scan = al256(((nch * F) / SCAN_CH + 4) * 8 + 4096);
+		const size_t t_seg = al256(sizeof(hj_seg)), t_cmap = al256(nch * 4), t_st = al256(sizeof(hj_stats));
+		if (hipMalloc(&b.part, n * s->W + 64) != hipSuccess || hipMalloc((void **)&b.pb, (F + 1) * 8) != hipSuccess ||
+		    hipMalloc(&tmp, t_hist + t_scan + t_seg + t_cmap + t_st) != hipSuccess) {
+			(void)hipGetLastError();
+			if (b.part) (void)hipFree(b.part);
+			if (b.pb) (void)hipFree(b.pb);
+			return hj_fail(ctx, HJ_ENOMEM, "resident build of %llu tuples", (unsigned long long)n);
+		}
+		uint64_t *d_hist = (uint64_t *)tmp, *d_scan = (uint64_t *)(tmp + t_hist);
+		hj_seg *d_seg = (hj_seg *)(tmp + t_hist + t_scan);
+		uint32_t *d_cmap = (uint32_t *)(tmp + t_hist + t_scan + t_seg);
+		hj_stats *d_st = (hj_stats *)(tmp + t_hist + t_scan + t_seg + t_cmap);
+		int rc = HJ_OK;
+		if (hipMemcpy(d_seg, &g, sizeof g, hipMemcpyHostToDevice) != hipSuccess ||
+		    hipMemcpy(d_cmap, cmap.data(), nch * 4, hipMemcpyHostToDevice) != hipSuccess)
+			rc = hj_fail(ctx, HJ_EHIP, "build plan upload");
+		if (!rc) {
+			const bool kt = ctx->kernel_times;
+			ctx->kernel_times = false;
+			hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(64), 0, ctx->stream, d_st, nch, (unsigned long long *)nullptr, 0u,
+			                   (unsigned int *)nullptr);
+			rc = launch_pass(ctx, wv, 0, (unsigned)nch, d_seg, d_cmap, kServeChunk, 0, b.bits, d_hist, nch * F, b.part,
+			                 d_st, d_scan);
+			if (!rc) hipLaunchKernelGGL(k_bounds_b, dim3(1), dim3(256), 0, ctx->stream, d_seg, 1u, (const uint64_t *)d_hist, b.pb);
+			ctx->kernel_times = kt;
+			hipError_t e = hipStreamSynchronize(ctx->stream);
+			if (!rc && e != hipSuccess) rc = hj_fail(ctx, HJ_EHIP, "build partition: %s", hipGetErrorString(e));
+		}
+		(void)hipFree(tmp);
+		if (rc) {
+			(void)hipFree(b.part);
+			(void)hipFree(b.pb);
+			return rc;
+		}
+		b.bytes = n * s->W + 64 + (F + 1) * 8;
+	}
+	s->builds.push_back(b);
+	s->st.n_builds++;
+	s->st.resident_bytes += b.bytes;
+	*build_id = (uint32_t)s->builds.size() - 1;
+	return HJ_OK;
+}
+
+extern "C" int hj_serve_submit(hj_server_t *s, int32_t build_id, const hj_relation_t *R, const hj_relation_t *S,
+                               uint64_t *req_id) {
+	if (!s || !S || !req_id) return HJ_EINVAL;
+	hj_ctx_t *ctx = s->ctx;
+	std::unique_lock<std::mutex> lk(s->mu);
+	hj_sreq r;
+	r.build = build_id;
+	if (build_id >= 0) {
+		if ((size_t)build_id >= s->builds.size()) return hj_fail(ctx, HJ_EINVAL, "unknown build %d", build_id);
+		r.R = s->builds[build_id].R;
+	} else {
+		if (!R) return hj_fail(ctx, HJ_EINVAL, "null build relation");
+		r.R = *R;
+	}
+	r.S = *S;
+	if (s->W == 0) s->W = r.R.tuple_size;
+	if (r.R.tuple_size != s->W || r.S.tuple_size != s->W) return hj_fail(ctx, HJ_EINVAL, "tuple sizes must match the server's");
+	if (r.R.location != HJ_LOC_DEVICE || r.S.location != HJ_LOC_DEVICE || (r.R.tuple_num && !r.R.tuples) ||
+	    (r.S.tuple_num && !r.S.tuples))
+		return hj_fail(ctx, HJ_EINVAL, "device relations required");
+	r.footprint = serve_footprint(serve_plan_req(s, r), s->W);
+	s->st.submitted++;
+	if (r.footprint + kServeBatchOverhead + 64 * 256 > s->slots[0].arena_bytes) {
+		s->st.rejected++;
+		return hj_fail(ctx, HJ_EREJECT, "request needs %llu bytes of scratch, a slot holds %zu",
+		               (unsigned long long)r.footprint, s->slots[0].arena_bytes);
+	}
+	r.t_arr = hj_clock::now();
+	const uint64_t id = s->next_id++;
+	s->reqs.emplace(id, r);
+	s->queue.push_back(id);
+	*req_id = id;
+	lk.unlock();
+	s->cv.notify_all();
+	return HJ_OK;
+}
+
+extern "C" int hj_serve_poll(hj_server_t *s, uint64_t req_id, hj_join_result_t *res, double *latency_ms) {
+	if (!s) return HJ_EINVAL;
+	std::lock_guard<std::mutex> lk(s->mu);
+	auto it = s->reqs.find(req_id);
+	if (it == s->reqs.end()) return hj_fail(s->ctx, HJ_EINVAL, "unknown request %llu", (unsigned long long)req_id);
+	if (it->second.state != 2) return HJ_EBUSY;
+	if (it->second.err != HJ_OK) return hj_fail(s->ctx, it->second.err, "request %llu failed", (unsigned long long)req_id);
+	if (res) *res = it->second.res;
+	if (latency_ms)
+		*latency_ms = std::chrono::duration<double, std::milli>(it->second.t_done - it->second.t_arr).count();
+	return HJ_OK;
+}
+
+extern "C" int hj_serve_drain(hj_server_t *s) {
+	if (!s) return HJ_EINVAL;
+	std::unique_lock<std::mutex> lk(s->mu);
+	s->cv.wait(lk, [&] { return s->st.completed + s->st.rejected >= s->st.submitted; });
+	return HJ_OK;
+}
+
+extern "C" int hj_serve_stats(hj_server_t *s, hj_serve_stats_t *st) {
+	if (!s || !st) return HJ_EINVAL;
+	std::lock_guard<std::mutex> lk(s->mu);
+	*st = s->st;
+	return HJ_OK;
+}
+
+extern "C" int hj_serve_close(hj_server_t *s) {
+	if (!s) return HJ_EINVAL;
+	{
+		std::lock_guard<std::mutex> lk(s->mu);
+		s->stop = true;
+	}
+	s->cv.notify_all();
+	if (s->th.joinable()) s->th.join();
+	(void)hipSetDevice(s->ctx->device);
+	for (hj_slot &sl : s->slots) {
+		if (sl.stream) (void)hipStreamSynchronize(sl.stream);
+		if (sl.arena) (void)hipFree(sl.arena);
+		if (sl.hbuf) (void)hipHostFree(sl.hbuf);
+		if (sl.done) (void)hipEventDestroy(sl.done);
+		if (sl.stream) (void)hipStreamDestroy(sl.stream);
+	}
+	for (hj_build_res &b : s->builds) {
+		if (b.part) (void)hipFree(b.part);
+		if (b.pb) (void)hipFree(b.pb);
+	}
+	delete s;
 	return HJ_OK;
 }

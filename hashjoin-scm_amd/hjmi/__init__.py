@@ -73,6 +73,18 @@ class BatchInfo(C.Structure):
                 ("work_items", C.c_uint64), ("tuples", C.c_uint64), ("scratch_bytes", C.c_uint64)]
 
 
+class ServeConfig(C.Structure):
+    _fields_ = [("hbm_cap_bytes", C.c_uint64), ("slots", C.c_uint32), ("max_batch", C.c_uint32),
+                ("reserved", C.c_uint32 * 4)]
+
+
+class ServeStats(C.Structure):
+    _fields_ = [("submitted", C.c_uint64), ("completed", C.c_uint64), ("rejected", C.c_uint64),
+                ("batches", C.c_uint64), ("max_batch_joins", C.c_uint64), ("max_busy_slots", C.c_uint32),
+                ("n_builds", C.c_uint32), ("resident_bytes", C.c_uint64), ("arena_bytes", C.c_uint64),
+                ("peak_admitted_bytes", C.c_uint64)]
+
+
 class HJError(RuntimeError):
     def __init__(self, code: int, msg: str):
         super().__init__(f"{msg} (status {code})")
@@ -116,6 +128,13 @@ def load_library(path: str = LIB_PATH):
         "hj_batch_destroy": (I, [P]),
         "hj_autotune": (I, [P, U32, U32, U32, U32]),
         "hj_tuned_config": (I, [P, U64, C.POINTER(U32), C.POINTER(U32)]),
+        "hj_serve_open": (I, [P, C.c_char_p, C.POINTER(ServeConfig), C.POINTER(P)]),
+        "hj_serve_register_build": (I, [P, C.POINTER(Relation), C.POINTER(U32)]),
+        "hj_serve_submit": (I, [P, C.c_int32, C.POINTER(Relation), C.POINTER(Relation), C.POINTER(U64)]),
+        "hj_serve_poll": (I, [P, U64, C.POINTER(JoinResult), C.POINTER(C.c_double)]),
+        "hj_serve_drain": (I, [P]),
+        "hj_serve_stats": (I, [P, C.POINTER(ServeStats)]),
+        "hj_serve_close": (I, [P]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -129,7 +148,9 @@ def exported_symbols():
     return ["hj_ctx_create", "hj_ctx_destroy", "hj_strerror", "hj_last_error", "hj_algo_count",
             "hj_algo_name", "hj_kernel_name", "hj_join", "hj_dev_alloc", "hj_dev_free", "hj_copy_h2d",
             "hj_copy_d2h", "hj_sync", "hj_generate", "hj_device_info", "hj_copy_d2d", "hj_batch_create",
-            "hj_batch_run", "hj_batch_destroy", "hj_autotune", "hj_tuned_config"]
+            "hj_batch_run", "hj_batch_destroy", "hj_autotune", "hj_tuned_config", "hj_serve_open",
+            "hj_serve_register_build", "hj_serve_submit", "hj_serve_poll", "hj_serve_drain", "hj_serve_stats",
+            "hj_serve_close"]
 
 
 @dataclass
@@ -340,6 +361,61 @@ class Batch:
     def close(self):
         if getattr(self, "h", None):
             self.ctx.lib.hj_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Server:
+    """Continuous-batching join server (hj_serve_*)."""
+
+    def __init__(self, ctx: Context, algo: str = "phj_radix_bc", hbm_cap_bytes: int = 8 << 30, slots: int = 3,
+                 max_batch: int = 64):
+        self.ctx = ctx
+        cfg = ServeConfig(hbm_cap_bytes, slots, max_batch)
+        h = C.c_void_p()
+        ctx._check(ctx.lib.hj_serve_open(ctx.h, algo.encode(), C.byref(cfg), C.byref(h)), "hj_serve_open")
+        self.h = h
+
+    def register_build(self, R: DeviceRelation) -> int:
+        bid = C.c_uint32()
+        r = R.as_c()
+        self.ctx._check(self.ctx.lib.hj_serve_register_build(self.h, C.byref(r), C.byref(bid)),
+                        "hj_serve_register_build")
+        return bid.value
+
+    def submit(self, S: DeviceRelation, R: DeviceRelation = None, build: int = -1) -> int:
+        rid = C.c_uint64()
+        s = S.as_c()
+        r = R.as_c() if R is not None else Relation()
+        self.ctx._check(self.ctx.lib.hj_serve_submit(self.h, build, C.byref(r), C.byref(s), C.byref(rid)),
+                        "hj_serve_submit")
+        return rid.value
+
+    def poll(self, rid: int):
+        """(matched_cnt, check_sum), latency_ms; or None while running."""
+        res, lat = JoinResult(), C.c_double()
+        rc = self.ctx.lib.hj_serve_poll(self.h, rid, C.byref(res), C.byref(lat))
+        if rc == HJ_EBUSY:
+            return None
+        self.ctx._check(rc, "hj_serve_poll")
+        return (res.matched_cnt, res.check_sum), lat.value
+
+    def drain(self):
+        self.ctx._check(self.ctx.lib.hj_serve_drain(self.h), "hj_serve_drain")
+
+    def stats(self) -> dict:
+        st = ServeStats()
+        self.ctx._check(self.ctx.lib.hj_serve_stats(self.h, C.byref(st)), "hj_serve_stats")
+        return {f: getattr(st, f) for f, _ in ServeStats._fields_}
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.ctx.lib.hj_serve_close(self.h)
             self.h = None
 
     def __del__(self):

@@ -154,6 +154,49 @@ int hj_batch_destroy(hj_batch_t *batch);
 int hj_autotune(hj_ctx_t *ctx, uint32_t width, uint32_t s_ratio, uint32_t min_log2, uint32_t max_log2);
 int hj_tuned_config(hj_ctx_t *ctx, uint64_t r_num, uint32_t *radix_bits, uint32_t *passes);
 
+/* ---- serving: continuous batching of arriving joins (no reference
+ * counterpart).  A scheduler thread owns `slots` HIP streams, each with an
+ * HBM arena of hbm_cap_bytes / slots.  Requests queue in arrival order;
+ * whenever a slot is free, requests from the head of the queue are admitted
+ * while their exact scratch footprint fits that slot's arena (at most
+ * max_batch per launch), planned as one batch and launched asynchronously,
+ * so batches on different slots overlap on the device.  Registered build
+ * relations are partitioned once and stay resident in HBM; requests naming
+ * them only partition their probe side.  A request whose footprint exceeds
+ * one arena is rejected (HJ_EREJECT).  Results are bit-identical to hj_join
+ * on the same relations.  All relations are borrowed until their request
+ * (or, for builds, the server) is done. */
+typedef struct {
+	uint64_t hbm_cap_bytes;  /* scratch cap over all slots              */
+	uint32_t slots;          /* concurrent batches (streams), 1..8      */
+	uint32_t max_batch;      /* joins per launch                        */
+	uint32_t reserved[4];
+} hj_serve_config_t;
+
+typedef struct {
+	uint64_t submitted, completed, rejected, batches;
+	uint64_t max_batch_joins;      /* largest batch launched               */
+	uint32_t max_busy_slots;       /* most batches in flight at once       */
+	uint32_t n_builds;
+	uint64_t resident_bytes;       /* HBM held by resident build tables    */
+	uint64_t arena_bytes;          /* HBM held by the slot arenas          */
+	uint64_t peak_admitted_bytes;  /* largest footprint in flight at once  */
+} hj_serve_stats_t;
+
+typedef struct hj_server hj_server_t;
+
+int hj_serve_open(hj_ctx_t *ctx, const char *algo, const hj_serve_config_t *cfg, hj_server_t **out);
+/* partition R once and keep it resident; returns its id */
+int hj_serve_register_build(hj_server_t *srv, const hj_relation_t *R, uint32_t *build_id);
+/* build_id < 0: R is the build side; otherwise R is ignored */
+int hj_serve_submit(hj_server_t *srv, int32_t build_id, const hj_relation_t *R, const hj_relation_t *S,
+                    uint64_t *req_id);
+/* HJ_EBUSY until done; latency = completion - submission, milliseconds */
+int hj_serve_poll(hj_server_t *srv, uint64_t req_id, hj_join_result_t *res, double *latency_ms);
+int hj_serve_drain(hj_server_t *srv);
+int hj_serve_stats(hj_server_t *srv, hj_serve_stats_t *st);
+int hj_serve_close(hj_server_t *srv);
+
 /* ---- device memory helpers for hosts without an allocator of their own --- */
 int hj_dev_alloc(hj_ctx_t *ctx, uint64_t bytes, void **ptr);
 int hj_dev_free(hj_ctx_t *ctx, void *ptr);
