@@ -152,12 +152,117 @@ def run_batched(args, hjmi):
     print(json.dumps(res), flush=True)
 
 
+def run_served(args, hjmi):
+    """C5: Poisson arrivals of PK-FK joins, |R| log-uniform in [2^12, 2^22],
+    |S| = U{4..64} |R|, 16-byte tuples; 30% of requests reuse one of 64
+    resident build relations.  The continuous-batching server (admission by
+    HBM footprint under --cap-gb) is compared with the unbatched plain path
+    (one synchronous hj_join per request, first come first served) on the
+    same arrival sequence.  Every request's result must equal its join run
+    alone."""
+    import random
+    ctx = hjmi.Context(0)
+    rng = random.Random(777)
+    algo = args.algo or "phj_radix_bc"
+
+    def rel_sizes():
+        nr = int(round(2 ** rng.uniform(12, 22)))
+        return nr, nr * rng.randint(4, 64)
+
+    pool = []
+    for i in range(args.pool):
+        nr, ns = rel_sizes()
+        pool.append(ctx.generate("pkfk", nr, ns, 16, seed=50000 + i))
+    builds = []
+    for i in range(64):
+        nr, ns = rel_sizes()
+        builds.append(ctx.generate("pkfk", nr, ns, 16, seed=90000 + i))
+    ctx.autotune(16, 16, 12, 22)
+    want = [ctx.join(algo, R, S) for R, S in pool] + [ctx.join(algo, R, S) for R, S in builds]
+    want = [(w["matched_cnt"], w["check_sum"]) for w in want]
+    srv = hjmi.Server(ctx, algo, hbm_cap_bytes=int(args.cap_gb * (1 << 30)), slots=args.slots,
+                      max_batch=args.max_batch, solo_lanes=args.solo_lanes, solo_min_log2=args.solo_min_log2)
+    bids = [srv.register_build(R) for R, _ in builds]
+    rates = [int(x) for x in args.rates.split(",")]
+    out_rates = []
+    for rate in rates:
+        arr, t = [], 0.0
+        picks = []
+        for _ in range(args.requests):
+            t += rng.expovariate(rate)
+            arr.append(t)
+            picks.append(("b", rng.randrange(64)) if rng.random() < 0.3 else ("p", rng.randrange(len(pool))))
+        # served
+        ids = []
+        t0 = time.perf_counter()
+        for a, (kind, i) in zip(arr, picks):
+            while time.perf_counter() - t0 < a:
+                pass
+            if kind == "b":
+                ids.append(srv.submit(builds[i][1], build=bids[i]))
+            else:
+                ids.append(srv.submit(pool[i][1], R=pool[i][0]))
+        srv.drain()
+        t_end = time.perf_counter()
+        lat, ok = [], True
+        for rid, (kind, i) in zip(ids, picks):
+            res, l = srv.poll(rid)
+            lat.append(l)
+            ok &= res == want[len(pool) + i if kind == "b" else i]
+        lat.sort()
+        served = {"joins_per_s": args.requests / (t_end - t0), "p50_ms": lat[len(lat) // 2],
+                  "p99_ms": lat[int(len(lat) * 0.99) - 1], "identical_results": ok}
+        # plain: one synchronous join at a time, in arrival order
+        plat = []
+        t0 = time.perf_counter()
+        for a, (kind, i) in zip(arr, picks):
+            while time.perf_counter() - t0 < a:
+                pass
+            R, S = (builds[i] if kind == "b" else pool[i])
+            ctx.join(algo, R, S)
+            plat.append((time.perf_counter() - t0 - a) * 1e3)
+        t_end = time.perf_counter()
+        plat.sort()
+        plain = {"joins_per_s": args.requests / (t_end - t0), "p50_ms": plat[len(plat) // 2],
+                 "p99_ms": plat[int(len(plat) * 0.99) - 1]}
+        out_rates.append({"offered_per_s": rate, "served": served, "plain": plain})
+        log(f"rate {rate}: served {served} plain {plain}")
+    st = srv.stats()
+    srv.close()
+    best = max(out_rates, key=lambda r: r["served"]["joins_per_s"])
+    res = {
+        "metric": "served joins/s and p50/p99 latency (continuous batching) vs unbatched plain joins",
+        "value": best["served"]["joins_per_s"], "unit": "joins/s", "n_gpus": 1, "steps": args.requests,
+        "warmup": 0, "ms_per_step": 1e3 / best["served"]["joins_per_s"], "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "u64", "data": "synthetic (seeded on device)",
+        "config": {"workload": f"C5: Poisson arrivals, {algo}, |R| log-uniform [2^12, 2^22], |S| = U{{4..64}}|R|, "
+                               f"30% reuse of 64 resident builds, cap {args.cap_gb} GB, {args.slots} slots, "
+                               f"{args.solo_lanes} solo lanes for |R| >= 2^{args.solo_min_log2}",
+                   "requests_per_rate": args.requests},
+        "rates": out_rates, "server": st,
+        "all_identical": all(r["served"]["identical_results"] for r in out_rates),
+    }
+    for R, S in pool + builds:
+        ctx.free_relation(R)
+        ctx.free_relation(S)
+    ctx.close()
+    print(json.dumps(res), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--config", default="C2", choices=sorted(CONFIGS) + ["C4"])
+    ap.add_argument("--config", default="C2", choices=sorted(CONFIGS) + ["C4", "C5"])
+    ap.add_argument("--rates", default="1000,2000,5000,10000,20000", help="C5: offered joins/s")
+    ap.add_argument("--requests", type=int, default=1000, help="C5: requests per rate")
+    ap.add_argument("--pool", type=int, default=64, help="C5: distinct non-resident joins")
+    ap.add_argument("--cap-gb", type=float, default=200.0, help="C5: admission cap (HBM GB)")
+    ap.add_argument("--slots", type=int, default=3, help="C5: concurrent batches")
+    ap.add_argument("--max-batch", type=int, default=64, help="C5: joins per launch")
+    ap.add_argument("--solo-lanes", type=int, default=2, help="C5: lanes running large joins alone")
+    ap.add_argument("--solo-min-log2", type=int, default=16, help="C5: |R| >= 2^x runs in a solo lane")
     ap.add_argument("--joins", type=int, default=2048, help="C4: joins per batch")
     ap.add_argument("--batch-lo", type=int, default=10, help="C4: log2 of the smallest |R|")
     ap.add_argument("--batch-hi", type=int, default=20, help="C4: log2 of the largest |R|")
@@ -189,6 +294,9 @@ def main():
 
     if args.config == "C4":
         run_batched(args, hjmi)
+        return
+    if args.config == "C5":
+        run_served(args, hjmi)
         return
 
     cfg = dict(CONFIGS[args.config])

@@ -799,79 +799,6 @@ static size_t scatter_lds(const part_cfg &c, uint32_t wv, uint32_t F) {
 	return (size_t)c.tile * wv * 16 + (size_t)F * 16 + (size_t)(c.tile + 2) * 2 + 64 * 4;
 }
 
-// Launchers take the stats block explicitly: a context's single joins use
-// ctx->stats, a batch uses its own.  `st_dev` is read by the kernels for the
-// chunk count.
-static const hj_stats *g_launch_stats = nullptr;
-
-template <int WV>
-static void launch_hist(hj_ctx_t *ctx, unsigned grid, const hj_seg *segs, const uint32_t *cmap, uint64_t ch,
-                        uint32_t shift, uint32_t bits, uint64_t *hist) {
-	hipLaunchKernelGGL(k_hist<WV>, dim3(grid), dim3(512), (size_t)4 << bits, ctx->stream, segs, cmap, g_launch_stats,
-	                   ch, shift, bits, hist);
-}
-
-template <int WV, int NT, int ITEMS>
-static void launch_scatter_t(hj_ctx_t *ctx, unsigned grid, size_t lds, const hj_seg *segs, const uint32_t *cmap,
-                             uint64_t ch, uint32_t shift, uint32_t bits, const uint64_t *scan, void *out) {
-	auto kfn = k_scatter<WV, NT, ITEMS>;
-	// attributes are not stream work: set them outside any graph capture
-	// (a batch sets them before it starts capturing)
-	hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
-	if (hipStreamIsCapturing(ctx->stream, &cs) != hipSuccess) (void)hipGetLastError();
-	if (cs == hipStreamCaptureStatusNone &&
-	    hipFuncSetAttribute((const void *)kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess)
-		(void)hipGetLastError();
-	hipLaunchKernelGGL(kfn, dim3(grid), dim3(NT), lds, ctx->stream, segs, cmap, g_launch_stats, ch, shift, bits, scan,
-	                   (uint4 *)out);
-}
-
-static int launch_pass(hj_ctx_t *ctx, uint32_t wv, uint32_t variant, unsigned grid, const hj_seg *segs,
-                       const uint32_t *cmap, uint64_t ch, uint32_t shift, uint32_t bits, uint64_t *hist,
-                       uint64_t hist_len, void *out, const hj_stats *st, uint64_t *scan_tmp) {
-	g_launch_stats = st;
-	const part_cfg c = cfg_for(wv, variant);
-	const uint32_t F = 1u << bits;
-	const size_t lds = scatter_lds(c, wv, F);
-	if (lds + 1024 > ctx->prop.sharedMemPerBlock)
-		return hj_fail(ctx, HJ_EINVAL, "%u radix bits in one pass need %zu B of LDS (device: %zu)", bits, lds,
-		               (size_t)ctx->prop.sharedMemPerBlock);
-	if (grid == 0) return HJ_OK;
-	hj_kt_begin(ctx, HJ_K_HIST);
-	switch (wv) {
-	case 1: launch_hist<1>(ctx, grid, segs, cmap, ch, shift, bits, hist); break;
-	case 2: launch_hist<2>(ctx, grid, segs, cmap, ch, shift, bits, hist); break;
-	case 4: launch_hist<4>(ctx, grid, segs, cmap, ch, shift, bits, hist); break;
-	case 8: launch_hist<8>(ctx, grid, segs, cmap, ch, shift, bits, hist); break;
-	default: launch_hist<16>(ctx, grid, segs, cmap, ch, shift, bits, hist); break;
-	}
-	hj_kt_end(ctx);
-	hj_kt_begin(ctx, HJ_K_SCAN);
-	int rc = scan_u64(ctx, hist, hist_len, scan_tmp);
-	hj_kt_end(ctx);
-	if (rc) return rc;
-	hj_kt_begin(ctx, HJ_K_SCATTER);
-	switch (wv) {
-	case 1:
-		switch (variant < 6 ? variant : 0) {
-		case 2: launch_scatter_t<1, 512, 8>(ctx, grid, lds, segs, cmap, ch, shift, bits, hist, out); break;
-		case 3: launch_scatter_t<1, 256, 8>(ctx, grid, lds, segs, cmap, ch, shift, bits, hist, out); break;
-		case 4: launch_scatter_t<1, 1024, 4>(ctx, grid, lds, segs, cmap, ch, shift, bits, hist, out); break;
-		case 5: launch_scatter_t<1, 256, 16>(ctx, grid, lds, segs, cmap, ch, shift, bits, hist, out); break;
-		case 1: launch_scatter_t<1, 512, 4>(ctx, grid, lds, segs, cmap, ch, shift, bits, hist, out); break;
-		default: launch_scatter_t<1, 1024, 4>(ctx, grid, lds, segs, cmap, ch, shift, bits, hist, out); break;
-		}
-		break;
-	case 2: launch_scatter_t<2, 512, 2>(ctx, grid, lds, segs, cmap, ch, shift, bits, hist, out); break;
-	case 4: launch_scatter_t<4, 512, 1>(ctx, grid, lds, segs, cmap, ch, shift, bits, hist, out); break;
-	case 8: launch_scatter_t<8, 256, 1>(ctx, grid, lds, segs, cmap, ch, shift, bits, hist, out); break;
-	default: launch_scatter_t<16, 256, 1>(ctx, grid, lds, segs, cmap, ch, shift, bits, hist, out); break;
-	}
-	hj_kt_end(ctx);
-	HJ_HIP(ctx, hipGetLastError());
-	return HJ_OK;
-}
-
 // Static LDS of a kernel (its __shared__ arrays), so that static + dynamic
 // never exceeds the device's per-workgroup LDS.
 static size_t static_lds(const void *kfn) {
@@ -883,6 +810,87 @@ static size_t static_lds(const void *kfn) {
 	return a.sharedSizeBytes;
 }
 
+// Allow a kernel the device's whole LDS beyond its static arrays.  Every
+// caller sets the same value, so launches from several host threads (server
+// lanes, batch slots) cannot lower the limit under one another.  Attributes
+// are not stream work: callers skip this while capturing a graph (a batch
+// sets them before it starts capturing).
+static void allow_lds(hj_ctx_t *ctx, const void *kfn) {
+	const size_t dyn = ctx->prop.sharedMemPerBlock - std::min<size_t>(ctx->prop.sharedMemPerBlock, static_lds(kfn));
+	if (hipFuncSetAttribute(kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn) != hipSuccess)
+		(void)hipGetLastError();
+}
+
+static bool capturing(hj_ctx_t *ctx) {
+	hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+	if (hipStreamIsCapturing(ctx->stream, &cs) != hipSuccess) (void)hipGetLastError();
+	return cs != hipStreamCaptureStatusNone;
+}
+
+// Launchers take the stats block explicitly: a context's single joins use
+// ctx->stats, a batch uses its own.  The kernels read the chunk count there.
+template <int WV>
+static void launch_hist(hj_ctx_t *ctx, unsigned grid, const hj_seg *segs, const uint32_t *cmap, const hj_stats *st,
+                        uint64_t ch, uint32_t shift, uint32_t bits, uint64_t *hist) {
+	hipLaunchKernelGGL(k_hist<WV>, dim3(grid), dim3(512), (size_t)4 << bits, ctx->stream, segs, cmap, st, ch, shift,
+	                   bits, hist);
+}
+
+template <int WV, int NT, int ITEMS>
+static void launch_scatter_t(hj_ctx_t *ctx, unsigned grid, size_t lds, const hj_seg *segs, const uint32_t *cmap,
+                             const hj_stats *st, uint64_t ch, uint32_t shift, uint32_t bits, const uint64_t *scan,
+                             void *out) {
+	auto kfn = k_scatter<WV, NT, ITEMS>;
+	if (!capturing(ctx)) allow_lds(ctx, (const void *)kfn);
+	hipLaunchKernelGGL(kfn, dim3(grid), dim3(NT), lds, ctx->stream, segs, cmap, st, ch, shift, bits, scan,
+	                   (uint4 *)out);
+}
+
+static int launch_pass(hj_ctx_t *ctx, uint32_t wv, uint32_t variant, unsigned grid, const hj_seg *segs,
+                       const uint32_t *cmap, uint64_t ch, uint32_t shift, uint32_t bits, uint64_t *hist,
+                       uint64_t hist_len, void *out, const hj_stats *st, uint64_t *scan_tmp) {
+	const part_cfg c = cfg_for(wv, variant);
+	const uint32_t F = 1u << bits;
+	const size_t lds = scatter_lds(c, wv, F);
+	if (lds + 1024 > ctx->prop.sharedMemPerBlock)
+		return hj_fail(ctx, HJ_EINVAL, "%u radix bits in one pass need %zu B of LDS (device: %zu)", bits, lds,
+		               (size_t)ctx->prop.sharedMemPerBlock);
+	if (grid == 0) return HJ_OK;
+	hj_kt_begin(ctx, HJ_K_HIST);
+	switch (wv) {
+	case 1: launch_hist<1>(ctx, grid, segs, cmap, st, ch, shift, bits, hist); break;
+	case 2: launch_hist<2>(ctx, grid, segs, cmap, st, ch, shift, bits, hist); break;
+	case 4: launch_hist<4>(ctx, grid, segs, cmap, st, ch, shift, bits, hist); break;
+	case 8: launch_hist<8>(ctx, grid, segs, cmap, st, ch, shift, bits, hist); break;
+	default: launch_hist<16>(ctx, grid, segs, cmap, st, ch, shift, bits, hist); break;
+	}
+	hj_kt_end(ctx);
+	hj_kt_begin(ctx, HJ_K_SCAN);
+	int rc = scan_u64(ctx, hist, hist_len, scan_tmp);
+	hj_kt_end(ctx);
+	if (rc) return rc;
+	hj_kt_begin(ctx, HJ_K_SCATTER);
+	switch (wv) {
+	case 1:
+		switch (variant < 6 ? variant : 0) {
+		case 2: launch_scatter_t<1, 512, 8>(ctx, grid, lds, segs, cmap, st, ch, shift, bits, hist, out); break;
+		case 3: launch_scatter_t<1, 256, 8>(ctx, grid, lds, segs, cmap, st, ch, shift, bits, hist, out); break;
+		case 4: launch_scatter_t<1, 1024, 4>(ctx, grid, lds, segs, cmap, st, ch, shift, bits, hist, out); break;
+		case 5: launch_scatter_t<1, 256, 16>(ctx, grid, lds, segs, cmap, st, ch, shift, bits, hist, out); break;
+		case 1: launch_scatter_t<1, 512, 4>(ctx, grid, lds, segs, cmap, st, ch, shift, bits, hist, out); break;
+		default: launch_scatter_t<1, 1024, 4>(ctx, grid, lds, segs, cmap, st, ch, shift, bits, hist, out); break;
+		}
+		break;
+	case 2: launch_scatter_t<2, 512, 2>(ctx, grid, lds, segs, cmap, st, ch, shift, bits, hist, out); break;
+	case 4: launch_scatter_t<4, 512, 1>(ctx, grid, lds, segs, cmap, st, ch, shift, bits, hist, out); break;
+	case 8: launch_scatter_t<8, 256, 1>(ctx, grid, lds, segs, cmap, st, ch, shift, bits, hist, out); break;
+	default: launch_scatter_t<16, 256, 1>(ctx, grid, lds, segs, cmap, st, ch, shift, bits, hist, out); break;
+	}
+	hj_kt_end(ctx);
+	HJ_HIP(ctx, hipGetLastError());
+	return HJ_OK;
+}
+
 static size_t join_static_lds(uint32_t wv, bool lp, bool two_pass) {
 	// every instance of one kernel template has the same __shared__ arrays
 	(void)wv;
@@ -891,7 +899,7 @@ static size_t join_static_lds(uint32_t wv, bool lp, bool two_pass) {
 }
 
 // Dynamic-LDS attribute of the default-shape scatter kernel for width wv.
-static void set_scatter_attr(uint32_t wv, size_t lds) {
+static void set_scatter_attr(hj_ctx_t *ctx, uint32_t wv) {
 	const void *k = nullptr;
 	switch (wv) {
 	case 1: k = (const void *)k_scatter<1, 1024, 4>; break;
@@ -900,8 +908,7 @@ static void set_scatter_attr(uint32_t wv, size_t lds) {
 	case 8: k = (const void *)k_scatter<8, 256, 1>; break;
 	default: k = (const void *)k_scatter<16, 256, 1>; break;
 	}
-	if (hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess)
-		(void)hipGetLastError();
+	allow_lds(ctx, k);
 }
 
 // LDS bytes of a join table holding `cap` build tuples.
@@ -914,8 +921,7 @@ template <int WV, bool LP>
 static void launch_join_t(hj_ctx_t *ctx, unsigned grid, size_t lds, const void *Rb, const void *Sb, const hj_pair *pairs,
                           const uint64_t *tpre, uint32_t npairs, uint32_t B, uint64_t s_tile, uint32_t cap) {
 	auto kfn = k_join<WV, 256, LP>;
-	if (hipFuncSetAttribute((const void *)kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess)
-		(void)hipGetLastError();
+	allow_lds(ctx, (const void *)kfn);
 	hipLaunchKernelGGL(kfn, dim3(grid), dim3(256), lds, ctx->stream, (const uint4 *)Rb, (const uint4 *)Sb, pairs,
 	                   tpre, npairs, B, s_tile, cap, (hj_stats *)ctx->stats.p);
 }
@@ -940,8 +946,7 @@ template <int WV, int NT, int ITEMS>
 static void launch_sortchunk_t(hj_ctx_t *ctx, unsigned grid, size_t lds, const hj_seg *segs, const uint32_t *cmap,
                                uint32_t shift, uint32_t bits, void *out, uint16_t *run, uint64_t *cst) {
 	auto kfn = k_sortchunk<WV, NT, ITEMS>;
-	if (hipFuncSetAttribute((const void *)kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess)
-		(void)hipGetLastError();
+	allow_lds(ctx, (const void *)kfn);
 	hipLaunchKernelGGL(kfn, dim3(grid), dim3(NT), lds, ctx->stream, segs, cmap, (const hj_stats *)ctx->stats.p, shift,
 	                   bits, (uint4 *)out, run, cst);
 }
@@ -964,8 +969,7 @@ static void launch_join2_t(hj_ctx_t *ctx, unsigned grid, size_t lds, const void 
                            const uint16_t *run, uint32_t F2, uint32_t b1, const hj_pair *pairs, const uint64_t *tpre,
                            uint32_t npairs, uint32_t B, uint32_t cpt, uint32_t cap, uint32_t glog) {
 	auto kfn = k_join2<WV, 512, LP>;
-	if (hipFuncSetAttribute((const void *)kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess)
-		(void)hipGetLastError();
+	allow_lds(ctx, (const void *)kfn);
 	hipLaunchKernelGGL(kfn, dim3(grid), dim3(512), lds, ctx->stream, (const uint4 *)P, cst, run, F2, b1, pairs, tpre,
 	                   npairs, B, cpt, cap, glog, (hj_stats *)ctx->stats.p);
 }
@@ -1604,11 +1608,10 @@ extern "C" int hj_batch_create(hj_ctx_t *ctx, const char *algo, uint32_t njoins,
 
 	// kernel attributes are set before capture (not stream work)
 	const void *jk = join_b_kernel(wv, lp);
-	if (hipFuncSetAttribute(jk, hipFuncAttributeMaxDynamicSharedMemorySize, (int)jlds) != hipSuccess)
-		(void)hipGetLastError();
+	allow_lds(ctx, jk);
 	if (jlds + static_lds(jk) > ctx->prop.sharedMemPerBlock) return fail(hj_fail(ctx, HJ_EINVAL, "batch LDS table too big"));
 	if (nseg && slds + 1024 > ctx->prop.sharedMemPerBlock) return fail(hj_fail(ctx, HJ_EINVAL, "batch radix bits too big"));
-	if (nseg) set_scatter_attr(wv, slds);
+	if (nseg) set_scatter_attr(ctx, wv);
 
 	// ---- capture the batch as one graph
 	const bool kt = ctx->kernel_times;
@@ -1822,9 +1825,19 @@ struct hj_slot {
 	bool busy = false;
 };
 
+struct hj_solo_lane {
+	hj_ctx_t *ctx = nullptr;  // own stream and scratch
+	std::thread th;
+};
+
 struct hj_server {
 	hj_ctx_t *ctx = nullptr;
 	bool lp = false;
+	std::string algo;
+	std::vector<hj_solo_lane> solo;
+	std::deque<uint64_t> solo_queue;
+	uint64_t solo_min = 1ull << 16;
+	uint64_t solo_budget = 0;  // scratch one solo lane may use
 	hj_serve_config_t cfg{};
 	uint32_t W = 0;
 	std::vector<hj_slot> slots;
@@ -1988,12 +2001,12 @@ static int serve_launch(hj_server *s, hj_slot &sl, const std::vector<uint64_t> &
 	hj_stats *d_st = (hj_stats *)(A + o_st);
 	unsigned long long *d_res = (unsigned long long *)(A + o_res);
 	unsigned int *d_fl = (unsigned int *)(A + o_res + (size_t)n * 16);
-	const part_cfg pc = cfg_for(wv, 0);
-	const size_t slds = scatter_lds(pc, wv, 1u << maxbits);
-	const size_t lds_max = std::min<size_t>(ctx->prop.sharedMemPerBlock, 160 * 1024) - 1024;
+	const size_t lds_max =
+	    std::min<size_t>(ctx->prop.sharedMemPerBlock, 160 * 1024) - static_lds(join_b_kernel(wv, s->lp)) - 256;
 	uint32_t cap = (uint32_t)std::min<uint64_t>(65535, std::max<uint64_t>(2048, max_rq + max_rq / 4 + 64));
 	while (cap > 256 && join_lds(cap, s->lp) > lds_max) cap -= 256;
 	const size_t jlds = join_lds(cap, s->lp);
+	if (jlds > lds_max) return hj_fail(ctx, HJ_EINVAL, "serve join table does not fit LDS");
 
 	std::lock_guard<std::mutex> lk(ctx->mu);
 	hipStream_t saved = ctx->stream;
@@ -2002,10 +2015,8 @@ static int serve_launch(hj_server *s, hj_slot &sl, const std::vector<uint64_t> &
 	ctx->stream = sl.stream;
 	hipError_t e = hipMemcpyAsync(A + up0, sl.hbuf, up_bytes, hipMemcpyHostToDevice, sl.stream);
 	if (e == hipSuccess) {
-		if (nseg) set_scatter_attr(wv, slds);
-		const void *jk = join_b_kernel(wv, s->lp);
-		if (hipFuncSetAttribute(jk, hipFuncAttributeMaxDynamicSharedMemorySize, (int)jlds) != hipSuccess)
-			(void)hipGetLastError();
+		if (nseg) set_scatter_attr(ctx, wv);
+		allow_lds(ctx, join_b_kernel(wv, s->lp));
 		hipLaunchKernelGGL(k_batch_reset, dim3(std::max<uint32_t>(1, std::min<uint32_t>(1024, (n + 255) / 256))),
 		                   dim3(256), 0, sl.stream, d_st, nchunks, d_res, n, d_fl);
 		if (nseg) {
@@ -2086,9 +2097,13 @@ static void serve_loop(hj_server *s) {
 			if (sl.busy || s->queue.empty()) continue;
 			std::vector<uint64_t> ids;
 			uint64_t fp = kServeBatchOverhead;
+			uint64_t hist_entries = 0;
 			while (!s->queue.empty() && ids.size() < s->cfg.max_batch) {
 				hj_sreq &r = s->reqs[s->queue.front()];
 				if (fp + r.footprint + 64 * 256 > sl.arena_bytes) break;  // FIFO: wait for memory
+				const serve_req_plan p = serve_plan_req(s, r);
+				if (!ids.empty() && hist_entries + p.hist_entries + p.npairs > (uint64_t)SCAN_CH * 60000) break;
+				hist_entries += p.hist_entries + p.npairs;
 				fp += r.footprint + 64 * 256;
 				ids.push_back(s->queue.front());
 				s->queue.pop_front();
@@ -2128,20 +2143,65 @@ static void serve_loop(hj_server *s) {
 	}
 }
 
+// A solo lane runs one large request at a time through the single-join
+// engine on its own context (stream + scratch); the device overlaps it with
+// the batch slots.
+static void solo_loop(hj_server *s, hj_solo_lane *lane) {
+	(void)hipSetDevice(s->ctx->device);
+	std::unique_lock<std::mutex> lk(s->mu);
+	for (;;) {
+		s->cv.wait(lk, [&] { return s->stop || !s->solo_queue.empty(); });
+		if (s->solo_queue.empty()) {
+			if (s->stop) break;
+			continue;
+		}
+		const uint64_t id = s->solo_queue.front();
+		s->solo_queue.pop_front();
+		hj_sreq &r = s->reqs[id];  // stable: unordered_map nodes do not move
+		r.state = 1;
+		const hj_relation_t R = r.R, S = r.S;
+		lk.unlock();
+		hj_params_t p{};
+		hj_result_t out{};
+		const int rc = hj_join(lane->ctx, s->algo.c_str(), &R, &S, &p, &out);
+		lk.lock();
+		r.err = rc;
+		r.res.matched_cnt = out.matched_cnt;
+		r.res.check_sum = out.check_sum;
+		r.t_done = hj_clock::now();
+		r.state = 2;
+		s->st.completed++;
+		s->st.solo_completed++;
+		s->cv.notify_all();
+	}
+}
+
+// scratch of the single-join engine for one request (two passes worst case)
+static uint64_t solo_footprint(const hj_sreq &r, uint32_t W) {
+	const uint64_t n = r.R.tuple_num + r.S.tuple_num;
+	return 2 * n * W + n / 64 + (64u << 20);
+}
+
 extern "C" int hj_serve_open(hj_ctx_t *ctx, const char *algo, const hj_serve_config_t *cfg, hj_server_t **out) {
 	if (!ctx || !algo || !cfg || !out) return hj_fail(ctx, HJ_EINVAL, "null argument");
 	*out = nullptr;
 	bool radix = false;
 	const bool lp = hj_algo_is_lp(algo, &radix);
 	if (!radix) return hj_fail(ctx, HJ_EINVAL, "serving runs the radix joins (phj_radix_bc/lp), not '%s'", algo);
-	if (cfg->slots < 1 || cfg->slots > 8 || cfg->max_batch < 1 || cfg->hbm_cap_bytes < ((uint64_t)cfg->slots << 24))
-		return hj_fail(ctx, HJ_EINVAL, "serve config: 1..8 slots, max_batch >= 1, cap >= 16 MiB per slot");
+	if (cfg->slots < 1 || cfg->slots > 8 || cfg->max_batch < 1 || cfg->solo_lanes > 4 ||
+	    cfg->hbm_cap_bytes < ((uint64_t)(cfg->slots + cfg->solo_lanes) << 24))
+		return hj_fail(ctx, HJ_EINVAL, "serve config: 1..8 slots, 0..4 solo lanes, max_batch >= 1, cap >= 16 MiB per lane");
 	hj_server *s = new hj_server();
 	s->ctx = ctx;
 	s->lp = lp;
+	s->algo = algo;
 	s->cfg = *cfg;
+	s->solo_min = 1ull << (cfg->solo_min_log2 ? cfg->solo_min_log2 : 16);
 	s->slots.resize(cfg->slots);
-	const size_t per = (size_t)(cfg->hbm_cap_bytes / cfg->slots) & ~(size_t)4095;
+	// with solo lanes, half of the cap backs the batch slots, half the lanes
+	const uint64_t batch_cap = cfg->solo_lanes ? cfg->hbm_cap_bytes / 2 : cfg->hbm_cap_bytes;
+	if (cfg->solo_lanes) s->solo_budget = (cfg->hbm_cap_bytes - batch_cap) / cfg->solo_lanes;
+	const size_t per = (size_t)(batch_cap / cfg->slots) & ~(size_t)4095;
 	int rc = HJ_OK;
 	{
 		std::lock_guard<std::mutex> lk(ctx->mu);
@@ -2172,7 +2232,27 @@ extern "C" int hj_serve_open(hj_ctx_t *ctx, const char *algo, const hj_serve_con
 		delete s;
 		return rc;
 	}
+	s->solo.resize(cfg->solo_lanes);
+	for (hj_solo_lane &l : s->solo) {
+		if (hj_ctx_create(ctx->device, &l.ctx) != HJ_OK) {
+			for (hj_solo_lane &m : s->solo)
+				if (m.ctx) hj_ctx_destroy(m.ctx);
+			for (hj_slot &sl : s->slots) {
+				(void)hipFree(sl.arena);
+				(void)hipEventDestroy(sl.done);
+				(void)hipStreamDestroy(sl.stream);
+			}
+			delete s;
+			return hj_fail(ctx, HJ_EHIP, "solo lane context");
+		}
+		if (ctx->tuned) {  // lanes use the parent's size-class tuning
+			memcpy(l.ctx->tune_bits, ctx->tune_bits, sizeof ctx->tune_bits);
+			memcpy(l.ctx->tune_passes, ctx->tune_passes, sizeof ctx->tune_passes);
+			l.ctx->tuned = true;
+		}
+	}
 	s->th = std::thread(serve_loop, s);
+	for (hj_solo_lane &l : s->solo) l.th = std::thread(solo_loop, s, &l);
 	*out = s;
 	return HJ_OK;
 }
@@ -2265,17 +2345,19 @@ extern "C" int hj_serve_submit(hj_server_t *s, int32_t build_id, const hj_relati
 	if (r.R.location != HJ_LOC_DEVICE || r.S.location != HJ_LOC_DEVICE || (r.R.tuple_num && !r.R.tuples) ||
 	    (r.S.tuple_num && !r.S.tuples))
 		return hj_fail(ctx, HJ_EINVAL, "device relations required");
-	r.footprint = serve_footprint(serve_plan_req(s, r), s->W);
+	const bool solo = !s->solo.empty() && r.R.tuple_num >= s->solo_min;
+	r.footprint = solo ? solo_footprint(r, s->W) : serve_footprint(serve_plan_req(s, r), s->W);
 	s->st.submitted++;
-	if (r.footprint + kServeBatchOverhead + 64 * 256 > s->slots[0].arena_bytes) {
+	const uint64_t room = solo ? s->solo_budget : s->slots[0].arena_bytes - kServeBatchOverhead - 64 * 256;
+	if (r.footprint > room) {
 		s->st.rejected++;
-		return hj_fail(ctx, HJ_EREJECT, "request needs %llu bytes of scratch, a slot holds %zu",
-		               (unsigned long long)r.footprint, s->slots[0].arena_bytes);
+		return hj_fail(ctx, HJ_EREJECT, "request needs %llu bytes of scratch, its lane holds %llu",
+		               (unsigned long long)r.footprint, (unsigned long long)room);
 	}
 	r.t_arr = hj_clock::now();
 	const uint64_t id = s->next_id++;
 	s->reqs.emplace(id, r);
-	s->queue.push_back(id);
+	(solo ? s->solo_queue : s->queue).push_back(id);
 	*req_id = id;
 	lk.unlock();
 	s->cv.notify_all();
@@ -2317,6 +2399,10 @@ extern "C" int hj_serve_close(hj_server_t *s) {
 	}
 	s->cv.notify_all();
 	if (s->th.joinable()) s->th.join();
+	for (hj_solo_lane &l : s->solo) {
+		if (l.th.joinable()) l.th.join();
+		if (l.ctx) hj_ctx_destroy(l.ctx);
+	}
 	(void)hipSetDevice(s->ctx->device);
 	for (hj_slot &sl : s->slots) {
 		if (sl.stream) (void)hipStreamSynchronize(sl.stream);

@@ -75,14 +75,14 @@ class BatchInfo(C.Structure):
 
 class ServeConfig(C.Structure):
     _fields_ = [("hbm_cap_bytes", C.c_uint64), ("slots", C.c_uint32), ("max_batch", C.c_uint32),
-                ("reserved", C.c_uint32 * 4)]
+                ("solo_lanes", C.c_uint32), ("solo_min_log2", C.c_uint32), ("reserved", C.c_uint32 * 2)]
 
 
 class ServeStats(C.Structure):
     _fields_ = [("submitted", C.c_uint64), ("completed", C.c_uint64), ("rejected", C.c_uint64),
                 ("batches", C.c_uint64), ("max_batch_joins", C.c_uint64), ("max_busy_slots", C.c_uint32),
                 ("n_builds", C.c_uint32), ("resident_bytes", C.c_uint64), ("arena_bytes", C.c_uint64),
-                ("peak_admitted_bytes", C.c_uint64)]
+                ("peak_admitted_bytes", C.c_uint64), ("solo_completed", C.c_uint64)]
 
 
 class HJError(RuntimeError):
@@ -374,9 +374,9 @@ class Server:
     """Continuous-batching join server (hj_serve_*)."""
 
     def __init__(self, ctx: Context, algo: str = "phj_radix_bc", hbm_cap_bytes: int = 8 << 30, slots: int = 3,
-                 max_batch: int = 64):
+                 max_batch: int = 64, solo_lanes: int = 0, solo_min_log2: int = 0):
         self.ctx = ctx
-        cfg = ServeConfig(hbm_cap_bytes, slots, max_batch)
+        cfg = ServeConfig(hbm_cap_bytes, slots, max_batch, solo_lanes, solo_min_log2)
         h = C.c_void_p()
         ctx._check(ctx.lib.hj_serve_open(ctx.h, algo.encode(), C.byref(cfg), C.byref(h)), "hj_serve_open")
         self.h = h

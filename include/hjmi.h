@@ -160,17 +160,22 @@ int hj_tuned_config(hj_ctx_t *ctx, uint64_t r_num, uint32_t *radix_bits, uint32_
  * whenever a slot is free, requests from the head of the queue are admitted
  * while their exact scratch footprint fits that slot's arena (at most
  * max_batch per launch), planned as one batch and launched asynchronously,
- * so batches on different slots overlap on the device.  Registered build
+ * so batches on different slots overlap on the device.  Requests with a
+ * large build side instead go to solo lanes: worker threads with their own
+ * context and stream that run the full single-join engine (two radix passes)
+ * one request at a time, overlapping with the batches.  Registered build
  * relations are partitioned once and stay resident in HBM; requests naming
  * them only partition their probe side.  A request whose footprint exceeds
  * one arena is rejected (HJ_EREJECT).  Results are bit-identical to hj_join
  * on the same relations.  All relations are borrowed until their request
  * (or, for builds, the server) is done. */
 typedef struct {
-	uint64_t hbm_cap_bytes;  /* scratch cap over all slots              */
-	uint32_t slots;          /* concurrent batches (streams), 1..8      */
-	uint32_t max_batch;      /* joins per launch                        */
-	uint32_t reserved[4];
+	uint64_t hbm_cap_bytes;  /* scratch cap over batch slots and solo lanes */
+	uint32_t slots;          /* concurrent batches (streams), 1..8          */
+	uint32_t max_batch;      /* joins per launch                            */
+	uint32_t solo_lanes;     /* lanes for large joins, 0..4 (0: none)       */
+	uint32_t solo_min_log2;  /* |R| >= 2^x runs in a solo lane (0: 16)      */
+	uint32_t reserved[2];
 } hj_serve_config_t;
 
 typedef struct {
@@ -181,6 +186,7 @@ typedef struct {
 	uint64_t resident_bytes;       /* HBM held by resident build tables    */
 	uint64_t arena_bytes;          /* HBM held by the slot arenas          */
 	uint64_t peak_admitted_bytes;  /* largest footprint in flight at once  */
+	uint64_t solo_completed;       /* requests run by the solo lanes        */
 } hj_serve_stats_t;
 
 typedef struct hj_server hj_server_t;

@@ -62,6 +62,37 @@ def test_serve_results_equal_alone_and_in_mix(ctx, algo):
         ctx.free_relation(S)
 
 
+@pytest.mark.parametrize("algo", ["phj_radix_bc", "phj_radix_lp"])
+def test_serve_solo_lanes_in_mix(ctx, algo):
+    """Requests with |R| >= 2^14 run in solo lanes (full two-pass engine on
+    their own streams) while small ones are batched; every result equals the
+    join run alone."""
+    pool = _pool(ctx, 10, 8, 13, 21) + _pool(ctx, 6, 14, 18, 22)
+    builds = _pool(ctx, 2, 12, 16, 23)
+    alone = [ctx.join(algo, R, S) for R, S in pool] + [ctx.join(algo, R, S) for R, S in builds]
+    want = [(a["matched_cnt"], a["check_sum"]) for a in alone]
+    srv = hjmi.Server(ctx, algo, hbm_cap_bytes=8 << 30, slots=2, max_batch=8, solo_lanes=2, solo_min_log2=14)
+    bids = [srv.register_build(R) for R, _ in builds]
+    rng = np.random.default_rng(3)
+    reqs = []
+    for _ in range(60):
+        i = int(rng.integers(0, len(pool) + len(builds)))
+        if i < len(pool):
+            rid = srv.submit(pool[i][1], R=pool[i][0])
+        else:
+            rid = srv.submit(builds[i - len(pool)][1], build=bids[i - len(pool)])
+        reqs.append((rid, i))
+    srv.drain()
+    for rid, i in reqs:
+        assert srv.poll(rid)[0] == want[i], (rid, i)
+    st = srv.stats()
+    assert st["completed"] == 60 and st["solo_completed"] > 0 and st["batches"] > 0
+    srv.close()
+    for R, S in pool + builds:
+        ctx.free_relation(R)
+        ctx.free_relation(S)
+
+
 def test_serve_lp_duplicates_exact(ctx):
     hR, hS = O.generate("random", 5000, 40000, 16, seed=5, key_range=1500)
     want = O.aggregate(hR, hS, 16, lp=True)
@@ -99,3 +130,5 @@ def test_serve_rejects_bad_config(ctx):
         hjmi.Server(ctx, "nphj_sc")
     with pytest.raises(hjmi.HJError):
         hjmi.Server(ctx, "phj_radix_bc", slots=0)
+    with pytest.raises(hjmi.HJError):
+        hjmi.Server(ctx, "phj_radix_bc", solo_lanes=5)
