@@ -234,7 +234,7 @@ __global__ __launch_bounds__(512) void k_hist(const hj_seg *__restrict__ segs, c
 // LDS: tile[TILE*WV] uint4 | tcnt[F] u32 | tbase[F] u32 | cursor[F] u64 |
 //      pid[TILE] u16 | scan scratch
 template <int WV, int NT, int ITEMS>
-__global__ __launch_bounds__(NT) void k_scatter(const hj_seg *__restrict__ segs,
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT == 1024 ? 8 : 1))) void k_scatter(const hj_seg *__restrict__ segs,
                                                 const uint32_t *__restrict__ chunk_seg,
                                                 const hj_stats *__restrict__ st, uint64_t ch, uint32_t shift,
                                                 uint32_t bits, const uint64_t *__restrict__ scan,
@@ -595,37 +595,139 @@ __global__ void k_plan2(const hj_seg *__restrict__ segs2, uint32_t Q, uint32_t b
 	}
 }
 
-// Join over chunk runs.  Lanes work in groups of G = 2^glog; a group copies
-// or probes one run at a time.  The build side of a partition is gathered
-// run by run into the LDS table (bc: chained, inserted while copying; lp:
-// inserted after the gather).  A bc build side larger than `cap` is joined
-// in pieces; an lp one sets the oversize flag (exact path on the host).
+// Join over chunk runs.  A work item is (final partition q = (p2 << b1) | p1,
+// probe tile = a range of pass-1 chunks of S).  Descriptors of up to NT
+// chunk runs (absolute start, flat offset) are staged in LDS; the staged runs
+// form one flat sequence of tuples that the waves split into contiguous
+// ranges, so the lanes of a wave read consecutive tuples whatever the run
+// lengths.  Build side: runs are copied into the LDS table until it holds
+// `cap` tuples, and a run may be split between two fills (bc joins every fill
+// against the whole probe tile; lp flags a partition that needs more than one
+// fill, and the host recomputes that join exactly).  Probe side: each wave
+// loads the next RU tuples per lane while it probes the current ones.
 // LDS: keys u64[cap] | pays u64[cap] | heads u32[np2(cap)] | next u16[cap]
 template <int WV, int NT, bool LP>
-__global__ __launch_bounds__(NT) void k_join2(const uint4 *__restrict__ P, const uint64_t *__restrict__ cstart,
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_join2(const uint4 *__restrict__ P, const uint64_t *__restrict__ cstart,
                                               const uint16_t *__restrict__ run, uint32_t F2, uint32_t b1,
                                               const hj_pair *__restrict__ pairs,
                                               const uint64_t *__restrict__ tprefix, uint32_t npairs, uint32_t B,
-                                              uint32_t cpt, uint32_t cap, uint32_t glog, hj_stats *st) {
+                                              uint32_t cpt, uint32_t cap, hj_stats *st) {
+	constexpr uint32_t NW = NT / HJ_WAVE;
+	constexpr uint32_t RU = 6;                  // tuples per lane per step
+	constexpr uint32_t STEP = HJ_WAVE * RU;     // tuples per wave per step
 	extern __shared__ uint64_t lds_u64[];
-	__shared__ unsigned long long red[2 * (NT / HJ_WAVE)];
-	__shared__ uint32_t sred[NT / HJ_WAVE + 1];
+	__shared__ unsigned long long red[2 * NW];
+	__shared__ uint32_t sred[NW + 1];
 	__shared__ uint64_t s_item[4];
-	__shared__ uint64_t r_src[NT];
-	__shared__ uint32_t r_dst[NT];
-	__shared__ uint16_t r_len[NT];
+	__shared__ uint64_t r_src[NT];      // absolute position of a staged run's first tuple
+	__shared__ uint32_t r_off[NT + 1];  // flat offset of a staged run; r_off[NT] = total
 	uint64_t *keys = lds_u64;
 	uint64_t *pays = keys + cap;
 	uint32_t *heads = (uint32_t *)(pays + cap);
 	const uint32_t nb = 1u << (32 - __clz(cap - 1));
 	const uint32_t hmask = nb - 1;
 	uint16_t *next = (uint16_t *)(heads + nb);
-	const uint32_t G = 1u << glog, gl = threadIdx.x & (G - 1), grp = threadIdx.x >> glog, ngrp = NT >> glog;
+	const uint32_t lane = threadIdx.x & (HJ_WAVE - 1), wid = threadIdx.x / HJ_WAVE;
 	const uint64_t total = tprefix[npairs];
 	const uint32_t stride = F2 + 1;
-	constexpr int RU = 8;  // runs in flight per lane group
 	unsigned long long cnt = 0, sum = 0;
 	bool dup = false, oversize = false;
+
+	// Stage the digit-p2 runs of chunks [c0, c1), c1 - c0 <= NT; the first run
+	// starts `skip` tuples in.  Returns the flat total; the caller syncs
+	// before reading r_src / r_off.
+	auto stage = [&](uint64_t c0, uint64_t c1, uint32_t p2, uint32_t skip) -> uint32_t {
+		const uint64_t c = c0 + threadIdx.x;
+		uint32_t len = 0;
+		uint64_t src = 0;
+		if (c < c1) {
+			uint32_t a = run[c * stride + p2];
+			len = run[c * stride + p2 + 1] - a;
+			if (threadIdx.x == 0) {
+				a += skip;
+				len -= skip;
+			}
+			src = cstart[c] + a;
+		}
+		uint32_t tot;
+		const uint32_t ex = hj_block_exscan<uint32_t>(len, sred, &tot);
+		r_src[threadIdx.x] = src;
+		r_off[threadIdx.x] = ex;
+		if (threadIdx.x == NT - 1) r_off[NT] = tot;
+		return tot;
+	};
+	// staged run holding flat index f (largest k with r_off[k] <= f)
+	auto find_run = [&](uint32_t f) -> uint32_t {
+		uint32_t lo = 0, hi = NT;
+		while (hi - lo > 1) {
+			const uint32_t mid = (lo + hi) >> 1;
+			if (r_off[mid] <= f) lo = mid; else hi = mid;
+		}
+		return lo;
+	};
+	// load the step of flat indices f0 + u*64 + lane (< fe); k only moves forward
+	auto load_step = [&](uint32_t f0, uint32_t fe, uint32_t &k, uint4 (&t)[RU], bool stream) {
+#pragma unroll
+		for (uint32_t u = 0; u < RU; u++) {
+			const uint32_t f = f0 + u * HJ_WAVE + lane;
+			if (f < fe) {
+				while (r_off[k + 1] <= f) k++;
+				const uint4 *src = &P[(r_src[k] + (f - r_off[k])) * WV];
+				t[u] = stream ? hj_load_stream(src) : *src;
+			}
+		}
+	};
+	auto probe_step = [&](uint32_t f0, uint32_t fe, const uint4 (&t)[RU]) {
+		uint64_t kk[RU];
+		uint32_t h[RU];
+		bool live[RU];
+#pragma unroll
+		for (uint32_t u = 0; u < RU; u++) {
+			live[u] = f0 + u * HJ_WAVE + lane < fe;
+			kk[u] = hj_key_of(t[u]);
+			h[u] = (uint32_t)(kk[u] >> B) & hmask;
+		}
+		if (!LP) {
+			// h becomes the chain cursor (1-based entry, 0 = end)
+#pragma unroll
+			for (uint32_t u = 0; u < RU; u++) h[u] = live[u] ? heads[h[u]] : 0;
+			bool any = true;
+			while (any) {
+				any = false;
+#pragma unroll
+				for (uint32_t u = 0; u < RU; u++) {
+					if (!h[u]) continue;
+					const uint32_t i = h[u] - 1;
+					if (keys[i] == kk[u]) {
+						cnt++;
+						sum += pays[i] + hj_pay_of(t[u]);
+					}
+					h[u] = next[i];
+					any |= h[u] != 0;
+				}
+			}
+		} else {
+			for (uint32_t step = 0; step < nb; step++) {
+				bool any = false;
+#pragma unroll
+				for (uint32_t u = 0; u < RU; u++) {
+					if (!live[u]) continue;
+					const uint32_t o = heads[h[u]];
+					if (!o) {
+						live[u] = false;
+					} else if (keys[o - 1] == kk[u]) {
+						cnt++;
+						sum += pays[o - 1] + hj_pay_of(t[u]);
+						live[u] = false;
+					} else {
+						h[u] = (h[u] + 1) & hmask;
+					}
+					any |= live[u];
+				}
+				if (!any) break;
+			}
+		}
+	};
 
 	for (;;) {
 		if (threadIdx.x == 0) {
@@ -649,43 +751,31 @@ __global__ __launch_bounds__(NT) void k_join2(const uint4 *__restrict__ P, const
 		const uint32_t p2 = q >> b1;
 		const uint64_t sb = pr.s0 + tix * cpt, se = min(pr.s1, sb + cpt);
 		uint64_t rc = pr.r0;
+		uint32_t roff = 0;  // tuples of run rc already taken by an earlier fill
 		while (rc < pr.r1) {
-			// ---- gather build runs into the table until it is full
+			// ---- fill the table with up to `cap` build tuples
 			for (uint32_t i = threadIdx.x; i < nb; i += NT) heads[i] = 0;
 			uint32_t m = 0;
-			__syncthreads();
-			while (rc < pr.r1) {
-				const uint64_t c = rc + threadIdx.x;
-				uint32_t len = 0, a = 0;
-				if (c < pr.r1) {
-					a = run[c * stride + p2];
-					len = run[c * stride + p2 + 1] - a;
-				}
-				uint32_t tot;
-				const uint32_t ex = hj_block_exscan<uint32_t>(len, sred, &tot);
-				const bool fits = c < pr.r1 && m + ex + len <= cap;
-				const uint32_t take = __syncthreads_count(fits);
-				if (fits) {
-					r_src[threadIdx.x] = cstart[c] + a;
-					r_dst[threadIdx.x] = m + ex;
-					r_len[threadIdx.x] = (uint16_t)len;
-				}
+			for (;;) {
+				const uint64_t c1 = min(pr.r1, rc + NT);
+				const uint32_t n = (uint32_t)(c1 - rc);
+				const uint32_t tot = stage(rc, c1, p2, roff);
 				__syncthreads();
-				// a group copies RU runs at once so that RU loads are in flight
-				for (uint32_t k0 = grp * RU; k0 < take; k0 += ngrp * RU) {
-					uint32_t Lmax = 0;
-#pragma unroll
-					for (int u = 0; u < RU; u++)
-						if (k0 + u < take) Lmax = max(Lmax, (uint32_t)r_len[k0 + u]);
-					for (uint32_t j = gl; j < Lmax; j += G) {
+				const uint32_t T = min(tot, cap - m);
+				// runs taken whole form a prefix; the next one may be taken in part
+				const uint32_t nfull = __syncthreads_count(threadIdx.x < n && r_off[threadIdx.x + 1] <= T);
+				const uint32_t roff_new = nfull < n ? (nfull == 0 ? roff : 0) + (T - r_off[nfull]) : 0;
+				const uint32_t fb = (uint32_t)((uint64_t)T * wid / NW), fe = (uint32_t)((uint64_t)T * (wid + 1) / NW);
+				if (fb < fe) {
+					uint32_t k = find_run(fb);
+					for (uint32_t f0 = fb; f0 < fe; f0 += STEP) {
 						uint4 t[RU];
+						load_step(f0, fe, k, t, false);
 #pragma unroll
-						for (int u = 0; u < RU; u++)
-							if (k0 + u < take && j < r_len[k0 + u]) t[u] = P[(r_src[k0 + u] + j) * WV];
-#pragma unroll
-						for (int u = 0; u < RU; u++) {
-							if (!(k0 + u < take && j < r_len[k0 + u])) continue;
-							const uint32_t i = r_dst[k0 + u] + j;
+						for (uint32_t u = 0; u < RU; u++) {
+							const uint32_t f = f0 + u * HJ_WAVE + lane;
+							if (f >= fe) continue;
+							const uint32_t i = m + f;
 							const uint64_t key = hj_key_of(t[u]);
 							keys[i] = key;
 							pays[i] = hj_pay_of(t[u]);
@@ -693,10 +783,11 @@ __global__ __launch_bounds__(NT) void k_join2(const uint4 *__restrict__ P, const
 						}
 					}
 				}
-				if (take) m = r_dst[take - 1] + r_len[take - 1];
-				rc += take;
+				m += T;
+				rc += nfull;
+				roff = roff_new;
 				__syncthreads();
-				if (take < NT) break;  // table full, or all runs taken
+				if (m >= cap || rc >= pr.r1) break;
 			}
 			if (LP) {
 				if (rc < pr.r1) {
@@ -718,59 +809,31 @@ __global__ __launch_bounds__(NT) void k_join2(const uint4 *__restrict__ P, const
 				}
 				__syncthreads();
 			}
-			// ---- probe the tile's runs of the probe side
-			// run descriptors of the probe tile are staged in LDS NT at a time;
-			// a group then probes RU runs at once
+			// ---- probe the tile's runs, NT chunks at a time
 			for (uint64_t kb = sb; m && kb < se; kb += NT) {
-				const uint32_t nrun = (uint32_t)min((uint64_t)NT, se - kb);
-				if (threadIdx.x < nrun) {
-					const uint64_t c = kb + threadIdx.x;
-					const uint32_t a = run[c * stride + p2];
-					r_src[threadIdx.x] = cstart[c] + a;
-					r_len[threadIdx.x] = (uint16_t)(run[c * stride + p2 + 1] - a);
-				}
+				const uint32_t T = stage(kb, min(se, kb + NT), p2, 0);
 				__syncthreads();
-				for (uint32_t k0 = grp * RU; k0 < nrun; k0 += ngrp * RU) {
-					uint32_t Lmax = 0;
-#pragma unroll
-					for (int u = 0; u < RU; u++)
-						if (k0 + u < nrun) Lmax = max(Lmax, (uint32_t)r_len[k0 + u]);
-					for (uint32_t j = gl; j < Lmax; j += G) {
-						uint4 tt[RU];
-#pragma unroll
-						for (int u = 0; u < RU; u++)
-							if (k0 + u < nrun && j < r_len[k0 + u]) tt[u] = hj_load_stream(&P[(r_src[k0 + u] + j) * WV]);
-#pragma unroll
-						for (int u = 0; u < RU; u++) {
-						if (!(k0 + u < nrun && j < r_len[k0 + u])) continue;
-						const uint4 t = tt[u];
-						const uint64_t key = hj_key_of(t), pay = hj_pay_of(t);
-						uint32_t h = (uint32_t)(key >> B) & hmask;
-						if (!LP) {
-							for (uint32_t hit = heads[h]; hit; hit = next[hit - 1])
-								if (keys[hit - 1] == key) {
-									cnt++;
-									sum += pays[hit - 1] + pay;
-								}
-						} else {
-							for (uint32_t step = 0; step < nb; step++) {
-								const uint32_t o = heads[h];
-								if (!o) break;
-								if (keys[o - 1] == key) {
-									cnt++;
-									sum += pays[o - 1] + pay;
-									break;
-								}
-								h = (h + 1) & hmask;
-							}
-						}
-						}
+				const uint32_t fb = (uint32_t)((uint64_t)T * wid / NW), fe = (uint32_t)((uint64_t)T * (wid + 1) / NW);
+				if (fb < fe) {
+					uint32_t k = find_run(fb);
+					uint4 ta[RU] = {}, tb[RU] = {};
+					load_step(fb, fe, k, ta, true);
+					for (uint32_t f0 = fb;;) {
+						const uint32_t f1 = f0 + STEP;
+						if (f1 < fe) load_step(f1, fe, k, tb, true);
+						probe_step(f0, fe, ta);
+						if (f1 >= fe) break;
+						const uint32_t f2 = f1 + STEP;
+						if (f2 < fe) load_step(f2, fe, k, ta, true);
+						probe_step(f1, fe, tb);
+						if (f2 >= fe) break;
+						f0 = f2;
 					}
 				}
 				__syncthreads();
 			}
-			__syncthreads();
 		}
+		__syncthreads();
 	}
 	const unsigned fl = (__any(dup) ? HJ_DF_DUPKEY : 0u) | (__any(oversize) ? HJ_DF_OVERSIZE : 0u);
 	if (fl && (threadIdx.x & (HJ_WAVE - 1)) == 0) atomicOr(&st->flags, fl);
@@ -967,19 +1030,19 @@ static void launch_sortchunk(hj_ctx_t *ctx, uint32_t wv, unsigned grid, size_t l
 template <int WV, bool LP>
 static void launch_join2_t(hj_ctx_t *ctx, unsigned grid, size_t lds, const void *P, const uint64_t *cst,
                            const uint16_t *run, uint32_t F2, uint32_t b1, const hj_pair *pairs, const uint64_t *tpre,
-                           uint32_t npairs, uint32_t B, uint32_t cpt, uint32_t cap, uint32_t glog) {
+                           uint32_t npairs, uint32_t B, uint32_t cpt, uint32_t cap) {
 	auto kfn = k_join2<WV, 512, LP>;
 	allow_lds(ctx, (const void *)kfn);
 	hipLaunchKernelGGL(kfn, dim3(grid), dim3(512), lds, ctx->stream, (const uint4 *)P, cst, run, F2, b1, pairs, tpre,
-	                   npairs, B, cpt, cap, glog, (hj_stats *)ctx->stats.p);
+	                   npairs, B, cpt, cap, (hj_stats *)ctx->stats.p);
 }
 
 static void launch_join2(hj_ctx_t *ctx, uint32_t wv, bool lp, unsigned grid, size_t lds, const void *P,
                          const uint64_t *cst, const uint16_t *run, uint32_t F2, uint32_t b1, const hj_pair *pairs,
-                         const uint64_t *tpre, uint32_t npairs, uint32_t B, uint32_t cpt, uint32_t cap, uint32_t glog) {
-#define J(W)                                                                                                   \
-	if (lp) launch_join2_t<W, true>(ctx, grid, lds, P, cst, run, F2, b1, pairs, tpre, npairs, B, cpt, cap, glog); \
-	else launch_join2_t<W, false>(ctx, grid, lds, P, cst, run, F2, b1, pairs, tpre, npairs, B, cpt, cap, glog)
+                         const uint64_t *tpre, uint32_t npairs, uint32_t B, uint32_t cpt, uint32_t cap) {
+#define J(W)                                                                                             \
+	if (lp) launch_join2_t<W, true>(ctx, grid, lds, P, cst, run, F2, b1, pairs, tpre, npairs, B, cpt, cap); \
+	else launch_join2_t<W, false>(ctx, grid, lds, P, cst, run, F2, b1, pairs, tpre, npairs, B, cpt, cap)
 	switch (wv) {
 	case 1: J(1); break;
 	case 2: J(2); break;
@@ -1116,8 +1179,7 @@ int hj_run_radix(hj_ctx_t *ctx, const hj_relation_t *R, const hj_relation_t *S, 
 	const uint64_t avg_r = (nr + Q - 1) / Q;
 	const size_t lds_max = std::min<size_t>(ctx->prop.sharedMemPerBlock, 160 * 1024) - join_static_lds(wv, lp, P == 2) - 256;
 	uint32_t cap = (uint32_t)std::min<uint64_t>(65535, std::max<uint64_t>(256, avg_r + avg_r / 4 + 64));
-	if (P == 2 && cap < (uint32_t)pc2.tile) cap = (uint32_t)pc2.tile;  // a chunk run always fits
-	while (cap > (P == 2 ? (uint32_t)pc2.tile : 256u) && join_lds(cap, lp) > lds_max) cap -= 256;
+	while (cap > 256u && join_lds(cap, lp) > lds_max) cap -= 256;
 	const size_t jlds = join_lds(cap, lp);
 	if (jlds > lds_max) return hj_fail(ctx, HJ_EINVAL, "join table of %u tuples does not fit LDS", cap);
 	hj_kt_begin(ctx, HJ_K_PLAN);
@@ -1133,13 +1195,12 @@ int hj_run_radix(hj_ctx_t *ctx, const hj_relation_t *R, const hj_relation_t *S, 
 	rc = scan_u64(ctx, tiles, Q, (uint64_t *)ctx->scan_tmp.p);
 	hj_kt_end(ctx);
 	if (rc) return rc;
-	const int per_cu = jlds <= 40 * 1024 ? 4 : (jlds <= 76 * 1024 ? 2 : 1);
+	const size_t jall = jlds + join_static_lds(wv, lp, P == 2);
+	const int per_cu = (int)std::max<size_t>(1, std::min<size_t>(4, (size_t)(160 * 1024) / jall));
 	const unsigned jgrid = (unsigned)std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)ctx->num_cu * per_cu, (uint64_t)Q * 64));
 	hj_kt_begin(ctx, HJ_K_JOIN);
 	if (P == 2) {
-		uint32_t glog = 4;
-		while (glog < 6 && (pc2.tile >> b2) > (1u << glog)) glog++;
-		launch_join2(ctx, wv, lp, jgrid, jlds, Rb, cst, runs, F2, b1, pairs, tiles, Q, B, cpt, cap, glog);
+		launch_join2(ctx, wv, lp, jgrid, jlds, Rb, cst, runs, F2, b1, pairs, tiles, Q, B, cpt, cap);
 	} else {
 		launch_join(ctx, wv, lp, jgrid, jlds, Rb, Sb, pairs, tiles, Q, B, s_tile, cap);
 	}
