@@ -27,14 +27,35 @@ __device__ __forceinline__ uint4 hj_load_head(const uint4 *__restrict__ base, ui
 
 typedef unsigned int hj_u32x4 __attribute__((ext_vector_type(4)));
 
+// Device memory seen as the global address space.  Pointers read from
+// descriptors in memory are generic to the compiler, which then emits FLAT
+// instructions; those count on both the vector-memory and the LDS counters
+// and may complete out of order with LDS traffic, so every use must wait for
+// all outstanding memory operations.  Global instructions keep the counters
+// exact, so prefetched loads stay in flight across unrelated work.
+#define HJ_GLOBAL __attribute__((address_space(1)))
+template <typename T>
+__device__ __forceinline__ const HJ_GLOBAL T *hj_g(const T *p) {
+	return (const HJ_GLOBAL T *)p;
+}
+template <typename T>
+__device__ __forceinline__ HJ_GLOBAL T *hj_g(T *p) {
+	return (HJ_GLOBAL T *)p;
+}
+
 // Streaming load hint for data read exactly once.
 __device__ __forceinline__ uint4 hj_load_stream(const uint4 *p) {
-	const hj_u32x4 v = __builtin_nontemporal_load((const hj_u32x4 *)p);
+	const hj_u32x4 v = __builtin_nontemporal_load((const HJ_GLOBAL hj_u32x4 *)p);
 	return make_uint4(v.x, v.y, v.z, v.w);
 }
 __device__ __forceinline__ void hj_store_stream(uint4 *p, const uint4 &v) {
 	hj_u32x4 t = {v.x, v.y, v.z, v.w};
-	__builtin_nontemporal_store(t, (hj_u32x4 *)p);
+	__builtin_nontemporal_store(t, (HJ_GLOBAL hj_u32x4 *)p);
+}
+// Plain (cached) global load.
+__device__ __forceinline__ uint4 hj_load(const uint4 *p) {
+	const hj_u32x4 v = *(const HJ_GLOBAL hj_u32x4 *)p;
+	return make_uint4(v.x, v.y, v.z, v.w);
 }
 
 __device__ __forceinline__ unsigned long long hj_wave_sum(unsigned long long v) {

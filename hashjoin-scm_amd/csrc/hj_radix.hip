@@ -218,7 +218,7 @@ __global__ __launch_bounds__(512) void k_hist(const hj_seg *__restrict__ segs, c
 #pragma unroll
 		for (int u = 0; u < U; u++) {
 			const uint64_t i = i0 + (uint64_t)u * blockDim.x;
-			k[u] = i < e ? __builtin_nontemporal_load(&keys[i * 2 * WV]) : 0;
+			k[u] = i < e ? __builtin_nontemporal_load(hj_g(&keys[i * 2 * WV])) : 0;
 		}
 #pragma unroll
 		for (int u = 0; u < U; u++) {
@@ -231,23 +231,35 @@ __global__ __launch_bounds__(512) void k_hist(const hj_seg *__restrict__ segs, c
 }
 
 // ------------------------------------------------------------ scatter
-// LDS: tile[TILE*WV] uint4 | tcnt[F] u32 | tbase[F] u32 | cursor[F] u64 |
-//      pid[TILE] u16 | scan scratch
+// Each tile is counting-sorted by digit in LDS and written out as runs.  A
+// run's tail that does not fill a `lt`-tuple line (lt * W = 64 or 128 bytes)
+// is held back in LDS (`carry`) and written with the next tile's run of the
+// same digit, so every store of a digit's stream after its first covers whole
+// lines; the carries are flushed when the chunk ends.  This is the GPU form
+// of software write-combining buffers: partial-line writes cost as much HBM
+// time as full ones.  lt = 1 disables combining.
+// LDS: tile[XT*WV] uint4 | carry[FL*(lt-1)*WV] uint4 | cursor[FL] u64 |
+//      tcnt, tbase, cf [FL] u32 | pid[XT] u16 | scan scratch,
+//      XT = TILE + FL*(lt-1) (the tile plus every digit's carry),
+//      cf = carried tuples << 16 | tuples stored this tile
 template <int WV, int NT, int ITEMS>
-__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT == 1024 ? 8 : 1))) void k_scatter(const hj_seg *__restrict__ segs,
-                                                const uint32_t *__restrict__ chunk_seg,
-                                                const hj_stats *__restrict__ st, uint64_t ch, uint32_t shift,
-                                                uint32_t bits, const uint64_t *__restrict__ scan,
-                                                uint4 *__restrict__ out) {
+__global__ __launch_bounds__(NT) void k_scatter(
+    const hj_seg *__restrict__ segs, const uint32_t *__restrict__ chunk_seg, const hj_stats *__restrict__ st,
+    uint64_t ch, uint32_t shift, uint32_t bits, uint32_t lt, const uint64_t *__restrict__ scan,
+    uint4 *__restrict__ out) {
 	constexpr int TILE = NT * ITEMS;
 	extern __shared__ uint4 lds_u4[];
 	const uint32_t FL = 1u << bits;  // launch maximum: LDS layout
+	const uint32_t CW = lt - 1;      // carry slots per digit
+	const uint32_t XT = TILE + FL * CW;
 	uint4 *tile = lds_u4;
-	uint64_t *cursor = (uint64_t *)(tile + TILE * WV);
+	uint4 *carry = tile + (size_t)XT * WV;
+	uint64_t *cursor = (uint64_t *)(carry + (size_t)FL * CW * WV);
 	uint32_t *tcnt = (uint32_t *)(cursor + FL);
 	uint32_t *tbase = tcnt + FL;
-	uint16_t *pid = (uint16_t *)(tbase + FL);
-	uint32_t *red = (uint32_t *)(pid + TILE + 2);
+	uint32_t *cf = tbase + FL;
+	uint16_t *pid = (uint16_t *)(cf + FL);
+	uint32_t *red = (uint32_t *)(pid + ((XT + 3) & ~1u));
 
 	const uint32_t c = blockIdx.x;
 	if (c >= st->total_chunks) return;
@@ -255,30 +267,31 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT == 1024 ?
 	const uint32_t F = 1u << min(bits, g.bits), fmask = F - 1;
 	const uint32_t cl = c - g.chunk_base;
 	const uint64_t b = (uint64_t)cl * ch, e = min(g.len, b + ch);
+	if (b >= e) return;  // whole workgroup: nothing to scatter
 	for (uint32_t p = threadIdx.x; p < F; p += NT) {
 		cursor[p] = scan[g.hist_base + (uint64_t)p * g.nchunks + cl];
 		tcnt[p] = 0;
+		cf[p] = 0;
 	}
 	__syncthreads();
 	const uint32_t per = (F + NT - 1) / NT;  // digits per thread in the tile scan
-	// the next tile's tuples are loaded into registers while the current tile
-	// is written out (barriers on gfx950 wait only for LDS, not for loads)
-	uint4 v[ITEMS][WV];
-	auto load_tile = [&](uint64_t t0) {
-		const uint32_t n = (uint32_t)min((uint64_t)TILE, e - t0);
+	// Two register buffers: while one tile is sorted and written out, the
+	// next one is in flight, and a buffer is refilled two tiles ahead right
+	// after its tile's stores.  Every thread issues exactly ITEMS*WV loads per
+	// refill (past the chunk end they re-read its last tuple) and both buffers
+	// are processed on every trip, so a tile start waits only for its own
+	// loads and the stores before them, never for the other buffer's loads.
+	uint4 va[ITEMS][WV], vb[ITEMS][WV];
+	auto load_tile = [&](uint4 (&v)[ITEMS][WV], uint64_t t0) {
 #pragma unroll
 		for (int k = 0; k < ITEMS; k++) {
-			const uint32_t idx = k * NT + threadIdx.x;
-			if (idx < n) {
-				const uint4 *src = g.in + (t0 + idx) * WV;
+			const uint64_t i = min(t0 + (uint64_t)(k * NT + threadIdx.x), e - 1);
 #pragma unroll
-				for (int w = 0; w < WV; w++) v[k][w] = hj_load_stream(src + w);
-			}
+			for (int w = 0; w < WV; w++) v[k][w] = hj_load_stream(g.in + i * WV + w);
 		}
 	};
-	if (b < e) load_tile(b);
-	for (uint64_t t0 = b; t0 < e; t0 += TILE) {
-		const uint32_t n = (uint32_t)min((uint64_t)TILE, e - t0);
+	auto do_tile = [&](uint4 (&v)[ITEMS][WV], uint64_t t0) {
+		const uint32_t n = t0 < e ? (uint32_t)min((uint64_t)TILE, e - t0) : 0u;
 		uint32_t dig[ITEMS], rank[ITEMS];
 #pragma unroll
 		for (int k = 0; k < ITEMS; k++) {
@@ -289,48 +302,72 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT == 1024 ?
 			}
 		}
 		__syncthreads();
-		// exclusive scan of tcnt -> tbase; cursor becomes cursor - tbase so
-		// that a sorted position i maps to cursor[p] + i
+		// digit p's stream this tile = its carry, then its tuples of the tile;
+		// the first (cf[p] & 0xffff) tuples of it end on a line boundary and
+		// are stored now
+		uint32_t tot;
 		{
 			const uint32_t p0 = threadIdx.x * per;
-			uint32_t s = 0;
+			uint32_t sum = 0;
 			for (uint32_t q = 0; q < per; q++)
-				if (p0 + q < F) s += tcnt[p0 + q];
-			uint32_t tot;
-			uint32_t run = hj_block_exscan<uint32_t>(s, red, &tot);
+				if (p0 + q < F) sum += (cf[p0 + q] >> 16) + tcnt[p0 + q];
+			uint32_t run = hj_block_exscan<uint32_t>(sum, red, &tot);
 			for (uint32_t q = 0; q < per; q++)
 				if (p0 + q < F) {
-					tbase[p0 + q] = run;
-					cursor[p0 + q] -= run;
-					run += tcnt[p0 + q];
+					const uint32_t p = p0 + q, len = (cf[p] >> 16) + tcnt[p];
+					tbase[p] = run;
+					run += len;
+					const uint64_t end = cursor[p] + len, aend = end - end % lt;
+					cf[p] = (cf[p] & 0xffff0000u) | (aend > cursor[p] ? (uint32_t)(aend - cursor[p]) : 0u);
 				}
 		}
 		__syncthreads();
+		for (uint32_t x = threadIdx.x; x < F * CW; x += NT) {
+			const uint32_t p = x / CW, j = x - p * CW;
+			if (j < (cf[p] >> 16)) {
+				const uint32_t q = tbase[p] + j;
+#pragma unroll
+				for (int w = 0; w < WV; w++) tile[q * WV + w] = carry[x * WV + w];
+				pid[q] = (uint16_t)p;
+			}
+		}
 #pragma unroll
 		for (int k = 0; k < ITEMS; k++) {
 			const uint32_t idx = k * NT + threadIdx.x;
 			if (idx < n) {
-				const uint32_t pos = tbase[dig[k]] + rank[k];
+				const uint32_t q = tbase[dig[k]] + (cf[dig[k]] >> 16) + rank[k];
 #pragma unroll
-				for (int w = 0; w < WV; w++) tile[pos * WV + w] = v[k][w];
-				pid[pos] = (uint16_t)dig[k];
+				for (int w = 0; w < WV; w++) tile[q * WV + w] = v[k][w];
+				pid[q] = (uint16_t)dig[k];
 			}
 		}
 		__syncthreads();
-		if (t0 + TILE < e) load_tile(t0 + TILE);
-		// write the sorted tile out with streaming stores: consecutive lanes
-		// store consecutive 16-byte words of a run
-		for (uint32_t u = threadIdx.x; u < n * WV; u += NT) {
-			const uint32_t i = u / WV, w = u % WV;
-			const uint64_t dest = cursor[pid[i]] + i;
-			hj_store_stream(&out[dest * WV + w], tile[u]);
+		// consecutive lanes store consecutive 16-byte words of a run
+		for (uint32_t u = threadIdx.x; u < tot * WV; u += NT) {
+			const uint32_t i = u / WV, w = u % WV, p = pid[i], j = i - tbase[p], fl = cf[p] & 0xffffu;
+			if (j < fl) hj_store_stream(&out[(cursor[p] + j) * WV + w], tile[u]);
+			else carry[(p * CW + j - fl) * WV + w] = tile[u];
 		}
+		load_tile(v, t0 + 2 * TILE);
 		__syncthreads();
 		for (uint32_t p = threadIdx.x; p < F; p += NT) {
-			cursor[p] += tbase[p] + tcnt[p];
+			const uint32_t fl = cf[p] & 0xffffu;
+			cursor[p] += fl;
+			cf[p] = ((cf[p] >> 16) + tcnt[p] - fl) << 16;
 			tcnt[p] = 0;
 		}
 		__syncthreads();
+	};
+	load_tile(va, b);
+	load_tile(vb, b + TILE);
+	for (uint64_t t0 = b; t0 < e; t0 += 2 * TILE) {
+		do_tile(va, t0);
+		do_tile(vb, t0 + TILE);  // an empty tile past the end changes nothing
+	}
+	// the chunk's last partial lines
+	for (uint32_t x = threadIdx.x; x < F * CW * WV; x += NT) {
+		const uint32_t s = x / WV, w = x % WV, p = s / CW, j = s - p * CW;
+		if (j < (cf[p] >> 16)) hj_store_stream(&out[(cursor[p] + j) * WV + w], carry[x]);
 	}
 }
 
@@ -847,10 +884,11 @@ struct part_cfg {
 
 // Tile shapes (threads x tuples per thread).  16-byte tuples have several
 // shapes for tuning sweeps; wider tuples use one 32 KB tile.
-static const part_cfg k_shapes16[] = {{1024, 4, 4096}, {512, 4, 2048}, {512, 8, 4096},
-                                      {256, 8, 2048}, {1024, 4, 4096}, {256, 16, 4096}};
+static const part_cfg k_shapes16[] = {{512, 4, 2048}, {1024, 4, 4096}, {512, 8, 4096},
+                                      {256, 8, 2048}, {1024, 2, 2048}, {256, 16, 4096}};
 
 static part_cfg cfg_for(uint32_t wv, uint32_t variant = 0) {
+	variant &= 15;  // shape field
 	if (wv == 1) return k_shapes16[variant < 6 ? variant : 0];
 	const int nt = wv >= 8 ? 256 : 512;
 	int items = (int)(32768 / (16 * wv) / nt);
@@ -858,8 +896,20 @@ static part_cfg cfg_for(uint32_t wv, uint32_t variant = 0) {
 	return part_cfg{nt, items, nt * items};
 }
 
-static size_t scatter_lds(const part_cfg &c, uint32_t wv, uint32_t F) {
-	return (size_t)c.tile * wv * 16 + (size_t)F * 16 + (size_t)(c.tile + 2) * 2 + 64 * 4;
+static size_t scatter_lds(const part_cfg &c, uint32_t wv, uint32_t F, uint32_t lt) {
+	const size_t xt = (size_t)c.tile + (size_t)F * (lt - 1);
+	return xt * wv * 16 + (size_t)F * (lt - 1) * wv * 16 + (size_t)F * 20 + ((xt + 3) & ~(size_t)1) * 2 + 64 * 4;
+}
+
+// Write-combining line of the scatter (tuples): the largest of 128 / W, 64 / W
+// (at least one tuple) whose carries still let two workgroups share a CU;
+// 1 when neither does.
+static uint32_t scatter_lt(const part_cfg &c, uint32_t wv, uint32_t F, size_t lds_cu) {
+	for (uint32_t line = 128; line >= 64; line >>= 1) {
+		const uint32_t lt = line / (16 * wv);
+		if (lt >= 2 && scatter_lds(c, wv, F, lt) + 1024 <= lds_cu / 2) return lt;
+	}
+	return 1;
 }
 
 // Static LDS of a kernel (its __shared__ arrays), so that static + dynamic
@@ -901,11 +951,11 @@ static void launch_hist(hj_ctx_t *ctx, unsigned grid, const hj_seg *segs, const 
 
 template <int WV, int NT, int ITEMS>
 static void launch_scatter_t(hj_ctx_t *ctx, unsigned grid, size_t lds, const hj_seg *segs, const uint32_t *cmap,
-                             const hj_stats *st, uint64_t ch, uint32_t shift, uint32_t bits, const uint64_t *scan,
-                             void *out) {
+                             const hj_stats *st, uint64_t ch, uint32_t shift, uint32_t bits, uint32_t lt,
+                             const uint64_t *scan, void *out) {
 	auto kfn = k_scatter<WV, NT, ITEMS>;
 	if (!capturing(ctx)) allow_lds(ctx, (const void *)kfn);
-	hipLaunchKernelGGL(kfn, dim3(grid), dim3(NT), lds, ctx->stream, segs, cmap, st, ch, shift, bits, scan,
+	hipLaunchKernelGGL(kfn, dim3(grid), dim3(NT), lds, ctx->stream, segs, cmap, st, ch, shift, bits, lt, scan,
 	                   (uint4 *)out);
 }
 
@@ -914,7 +964,9 @@ static int launch_pass(hj_ctx_t *ctx, uint32_t wv, uint32_t variant, unsigned gr
                        uint64_t hist_len, void *out, const hj_stats *st, uint64_t *scan_tmp) {
 	const part_cfg c = cfg_for(wv, variant);
 	const uint32_t F = 1u << bits;
-	const size_t lds = scatter_lds(c, wv, F);
+	uint32_t lt = scatter_lt(c, wv, F, ctx->prop.sharedMemPerBlock);
+	if (variant & 0x70) lt = 1u << (((variant >> 4) & 7) - 1);  // forced line for tuning sweeps
+	const size_t lds = scatter_lds(c, wv, F, lt);
 	if (lds + 1024 > ctx->prop.sharedMemPerBlock)
 		return hj_fail(ctx, HJ_EINVAL, "%u radix bits in one pass need %zu B of LDS (device: %zu)", bits, lds,
 		               (size_t)ctx->prop.sharedMemPerBlock);
@@ -935,19 +987,19 @@ static int launch_pass(hj_ctx_t *ctx, uint32_t wv, uint32_t variant, unsigned gr
 	hj_kt_begin(ctx, HJ_K_SCATTER);
 	switch (wv) {
 	case 1:
-		switch (variant < 6 ? variant : 0) {
-		case 2: launch_scatter_t<1, 512, 8>(ctx, grid, lds, segs, cmap, st, ch, shift, bits, hist, out); break;
-		case 3: launch_scatter_t<1, 256, 8>(ctx, grid, lds, segs, cmap, st, ch, shift, bits, hist, out); break;
-		case 4: launch_scatter_t<1, 1024, 4>(ctx, grid, lds, segs, cmap, st, ch, shift, bits, hist, out); break;
-		case 5: launch_scatter_t<1, 256, 16>(ctx, grid, lds, segs, cmap, st, ch, shift, bits, hist, out); break;
-		case 1: launch_scatter_t<1, 512, 4>(ctx, grid, lds, segs, cmap, st, ch, shift, bits, hist, out); break;
-		default: launch_scatter_t<1, 1024, 4>(ctx, grid, lds, segs, cmap, st, ch, shift, bits, hist, out); break;
+		switch ((variant & 15) < 6 ? (variant & 15) : 0) {
+		case 2: launch_scatter_t<1, 512, 8>(ctx, grid, lds, segs, cmap, st, ch, shift, bits, lt, hist, out); break;
+		case 3: launch_scatter_t<1, 256, 8>(ctx, grid, lds, segs, cmap, st, ch, shift, bits, lt, hist, out); break;
+		case 4: launch_scatter_t<1, 1024, 2>(ctx, grid, lds, segs, cmap, st, ch, shift, bits, lt, hist, out); break;
+		case 5: launch_scatter_t<1, 256, 16>(ctx, grid, lds, segs, cmap, st, ch, shift, bits, lt, hist, out); break;
+		case 1: launch_scatter_t<1, 1024, 4>(ctx, grid, lds, segs, cmap, st, ch, shift, bits, lt, hist, out); break;
+		default: launch_scatter_t<1, 512, 4>(ctx, grid, lds, segs, cmap, st, ch, shift, bits, lt, hist, out); break;
 		}
 		break;
-	case 2: launch_scatter_t<2, 512, 2>(ctx, grid, lds, segs, cmap, st, ch, shift, bits, hist, out); break;
-	case 4: launch_scatter_t<4, 512, 1>(ctx, grid, lds, segs, cmap, st, ch, shift, bits, hist, out); break;
-	case 8: launch_scatter_t<8, 256, 1>(ctx, grid, lds, segs, cmap, st, ch, shift, bits, hist, out); break;
-	default: launch_scatter_t<16, 256, 1>(ctx, grid, lds, segs, cmap, st, ch, shift, bits, hist, out); break;
+	case 2: launch_scatter_t<2, 512, 2>(ctx, grid, lds, segs, cmap, st, ch, shift, bits, lt, hist, out); break;
+	case 4: launch_scatter_t<4, 512, 1>(ctx, grid, lds, segs, cmap, st, ch, shift, bits, lt, hist, out); break;
+	case 8: launch_scatter_t<8, 256, 1>(ctx, grid, lds, segs, cmap, st, ch, shift, bits, lt, hist, out); break;
+	default: launch_scatter_t<16, 256, 1>(ctx, grid, lds, segs, cmap, st, ch, shift, bits, lt, hist, out); break;
 	}
 	hj_kt_end(ctx);
 	HJ_HIP(ctx, hipGetLastError());
@@ -965,7 +1017,7 @@ static size_t join_static_lds(uint32_t wv, bool lp, bool two_pass) {
 static void set_scatter_attr(hj_ctx_t *ctx, uint32_t wv) {
 	const void *k = nullptr;
 	switch (wv) {
-	case 1: k = (const void *)k_scatter<1, 1024, 4>; break;
+	case 1: k = (const void *)k_scatter<1, 512, 4>; break;
 	case 2: k = (const void *)k_scatter<2, 512, 2>; break;
 	case 4: k = (const void *)k_scatter<4, 512, 1>; break;
 	case 8: k = (const void *)k_scatter<8, 256, 1>; break;
@@ -1017,7 +1069,7 @@ static void launch_sortchunk_t(hj_ctx_t *ctx, unsigned grid, size_t lds, const h
 static void launch_sortchunk(hj_ctx_t *ctx, uint32_t wv, unsigned grid, size_t lds, const hj_seg *segs,
                              const uint32_t *cmap, uint32_t shift, uint32_t bits, void *out, uint16_t *run,
                              uint64_t *cst) {
-	// shapes must match cfg_for(wv, 1)
+	// shapes must match cfg_for(wv, 0)
 	switch (wv) {
 	case 1: launch_sortchunk_t<1, 512, 4>(ctx, grid, lds, segs, cmap, shift, bits, out, run, cst); break;
 	case 2: launch_sortchunk_t<2, 512, 2>(ctx, grid, lds, segs, cmap, shift, bits, out, run, cst); break;
@@ -1101,7 +1153,7 @@ int hj_run_radix(hj_ctx_t *ctx, const hj_relation_t *R, const hj_relation_t *S, 
 		if (ch < (uint64_t)pc.tile) ch = pc.tile;
 		return ch;
 	};
-	const part_cfg pc2 = cfg_for(wv, 1);  // pass-2 chunk = one 32 KB sort tile
+	const part_cfg pc2 = cfg_for(wv, 0);  // pass-2 chunk = one 32 KB sort tile
 	const uint64_t ch1 = chunk_for(N), ch2 = pc2.tile;
 	const uint64_t chunks1 = (nr + ch1 - 1) / ch1 + (ns + ch1 - 1) / ch1;
 	const uint32_t nseg2 = 2 * F1;
@@ -1312,8 +1364,8 @@ __global__ void k_plan_b(const hj_bjoin *__restrict__ joins, const uint32_t *__r
 			const uint64_t sb = segs[J.seg].pb_base + q;
 			pr.rb = J.rpart;
 			pr.sb = part;
-			pr.r0 = J.rpb[q];
-			pr.r1 = J.rpb[q + 1];
+			pr.r0 = *hj_g(&J.rpb[q]);
+			pr.r1 = *hj_g(&J.rpb[q + 1]);
 			pr.s0 = pb[sb];
 			pr.s1 = pb[sb + 1];
 		} else {
@@ -1383,7 +1435,7 @@ __global__ __launch_bounds__(NT) void k_join_b(const hj_bpair *__restrict__ pair
 			for (uint32_t i = threadIdx.x; i < nb; i += NT) heads[i] = 0;
 			__syncthreads();
 			for (uint32_t i = threadIdx.x; i < m; i += NT) {
-				const uint4 t = pr.rb[(rc + i) * WV];
+				const uint4 t = hj_load(&pr.rb[(rc + i) * WV]);
 				const uint64_t key = hj_key_of(t);
 				keys[i] = key;
 				pays[i] = hj_pay_of(t);
@@ -1618,7 +1670,7 @@ extern "C" int hj_batch_create(hj_ctx_t *ctx, const char *algo, uint32_t njoins,
 	const uint64_t nchunks = cmap.size();
 	const uint32_t npairs = (uint32_t)pair_join.size();
 	const uint32_t nseg = (uint32_t)segs.size();
-	const size_t slds = scatter_lds(pc, wv, 1u << maxbits);
+	const size_t slds = scatter_lds(pc, wv, 1u << maxbits, scatter_lt(pc, wv, 1u << maxbits, ctx->prop.sharedMemPerBlock));
 	// LDS table: expected partition size with headroom; bigger build sides
 	// are handled by the kernels (bc: in pieces, lp: exact host path)
 	const size_t lds_max = std::min<size_t>(ctx->prop.sharedMemPerBlock, 160 * 1024) - 1024;

@@ -59,8 +59,10 @@ typedef struct {
 	uint32_t lp_slots_log2;  /* nphj_lp slots = 2^x; 0 = next_pow_2(|R|) as
 	                            src/algo/nphj.c:1556-1558                      */
 	uint32_t flags;          /* HJ_F_*                                        */
-	uint32_t part_variant;   /* partition tile shape for 16-byte tuples, for
-	                            tuning sweeps: 0 default, 1..5 fixed shapes  */
+	uint32_t part_variant;   /* tuning sweeps: bits 0-3 partition tile shape
+	                            for 16-byte tuples (0 default, 1..5 fixed);
+	                            bits 4-6 scatter write-combining line:
+	                            0 auto, k = 2^(k-1) tuples                    */
 	uint32_t reserved[3];
 } hj_params_t;
 
