@@ -51,6 +51,8 @@ def kernel_bytes(name, W, nr, ns):
     return {
         "radix_scatter": 2 * W * N,
         "radix_sortchunk": 2 * W * N,
+        "radix_tilesort": 2 * W * N,
+        "radix_gathersort": 2 * W * N,
         "radix_join": W * N,
         "lp_build": (W + 16) * nr,
         "sc_build": (W + 16) * nr,

@@ -23,6 +23,8 @@ enum hj_kid : uint32_t {
 	HJ_K_PLAN,
 	HJ_K_JOIN,
 	HJ_K_SORTCHUNK,
+	HJ_K_TILESORT,
+	HJ_K_GATHERSORT,
 	HJ_K_COUNT
 };
 

@@ -118,14 +118,24 @@ __global__ __launch_bounds__(SCAN_NT) void k_scan_down(uint64_t *__restrict__ a,
 	}
 }
 
-// Exclusive scan of a[0..L) in place, a[L] = total.  `tmp` holds
-// ceil(L / SCAN_CH) partials (<= 16384 * 1024 entries supported).
+__global__ void k_copy_u64_1(const uint64_t *__restrict__ src, uint64_t *__restrict__ dst) { *dst = *src; }
+
+// Exclusive scan of a[0..L) in place, a[L] = total.  `tmp` holds the
+// ceil(L / SCAN_CH) partials; beyond 65536 partials they are scanned the
+// same way one level up, which needs ceil(nb / SCAN_CH) + 2 more entries
+// (callers reserve partials + 2 + 512 entries: L up to 2^33).
 static int scan_u64(hj_ctx_t *ctx, uint64_t *a, uint64_t L, uint64_t *tmp) {
 	uint64_t nb = (L + SCAN_CH - 1) / SCAN_CH;
 	if (nb == 0) nb = 1;
-	if (nb > 1024 * 64) return hj_fail(ctx, HJ_EINVAL, "scan too long (%llu)", (unsigned long long)L);
+	if (nb > ((uint64_t)1 << 31) - 1) return hj_fail(ctx, HJ_EINVAL, "scan too long (%llu)", (unsigned long long)L);
 	hipLaunchKernelGGL(k_scan_reduce, dim3((unsigned)nb), dim3(SCAN_NT), 0, ctx->stream, a, L, tmp);
-	hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, ctx->stream, tmp, (uint32_t)nb, a + L);
+	if (nb <= 1024 * 64) {
+		hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, ctx->stream, tmp, (uint32_t)nb, a + L);
+	} else {
+		const int rc = scan_u64(ctx, tmp, nb, tmp + nb + 1);
+		if (rc) return rc;
+		hipLaunchKernelGGL(k_copy_u64_1, dim3(1), dim3(1), 0, ctx->stream, tmp + nb, a + L);
+	}
 	hipLaunchKernelGGL(k_scan_down, dim3((unsigned)nb), dim3(SCAN_NT), 0, ctx->stream, a, L, tmp);
 	return HJ_OK;
 }
@@ -619,6 +629,206 @@ __global__ __launch_bounds__(NT) void k_sortchunk(const hj_seg *__restrict__ seg
 	}
 }
 
+// Two-pass joins partition without a histogram pass.  Pass A sorts every
+// input tile by the first digit in place (k_sortchunk over the input
+// segments); its run tables are the histogram: k_runs_to_hist turns them into
+// the [segment][digit][tile] count matrix whose exclusive scan gives each
+// tile's run of each digit its position in the fully partitioned order.
+// Pass B (k_sortchunk_g) then cuts every first-level partition into tiles of
+// that order, gathers each tile from the pass-A runs that cover it, and
+// sorts it by the second digit, which gives the same layout as a
+// histogram + scatter pass followed by the chunk-local second pass.
+
+// Count matrix from pass-A run tables: hist[g.hist_base + d*g.nchunks + cl]
+// = tuples of digit d in tile cl of segment g.  A workgroup transposes 64
+// tiles x 64 digits through LDS so that reads and writes are both coalesced.
+__global__ __launch_bounds__(256) void k_runs_to_hist(const uint16_t *__restrict__ run, const uint64_t *__restrict__ cst,
+                                                      const hj_seg *__restrict__ segs,
+                                                      const uint32_t *__restrict__ chunk_seg, uint32_t total,
+                                                      uint32_t F, uint64_t *__restrict__ hist,
+                                                      uint64_t *__restrict__ srcp) {
+	__shared__ uint32_t cnt[64][65];
+	__shared__ uint32_t beg[64][65];
+	const uint32_t c0 = blockIdx.x * 64, d0 = blockIdx.y * 64;
+	for (uint32_t e = threadIdx.x; e < 64 * 64; e += 256) {
+		const uint32_t ci = e / 64, dj = e % 64, c = c0 + ci, d = d0 + dj;
+		uint32_t v = 0, a = 0;
+		if (c < total && d < F) {
+			const uint16_t *r = run + (uint64_t)c * (F + 1) + d;
+			a = r[0];
+			v = (uint32_t)r[1] - a;
+		}
+		cnt[ci][dj] = v;
+		beg[ci][dj] = a;
+	}
+	__syncthreads();
+	for (uint32_t e = threadIdx.x; e < 64 * 64; e += 256) {
+		const uint32_t dj = e / 64, ci = e % 64, c = c0 + ci, d = d0 + dj;
+		if (c < total && d < F) {
+			const hj_seg &g = segs[chunk_seg[c]];
+			const uint64_t at = g.hist_base + (uint64_t)d * g.nchunks + (c - g.chunk_base);
+			hist[at] = cnt[ci][dj];
+			srcp[at] = cst[c] + beg[ci][dj];
+		}
+	}
+}
+
+// first[c2] = pass-A tile whose run holds the first tuple of pass-B tile c2
+// (largest i with row[i] <= tile start; row = scanned counts of its digit).
+__global__ void k_first_run(const hj_seg *__restrict__ segs1, const hj_seg *__restrict__ segs2,
+                            const uint32_t *__restrict__ chunk_seg2, const hj_stats *__restrict__ st, uint32_t b1,
+                            uint32_t tile, const uint64_t *__restrict__ hist, uint32_t *__restrict__ first) {
+	const uint32_t total = (uint32_t)st->total_chunks;
+	for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < total; c += gridDim.x * blockDim.x) {
+		const uint32_t s2 = chunk_seg2[c];
+		const hj_seg &g2 = segs2[s2];
+		const uint64_t ts = g2.out_start + (uint64_t)(c - g2.chunk_base) * tile;
+		const hj_seg &g1 = segs1[s2 >> b1];
+		const uint64_t *row = hist + g1.hist_base + (uint64_t)(s2 & ((1u << b1) - 1)) * g1.nchunks;
+		uint32_t lo = 0, hi = g1.nchunks;
+		while (hi - lo > 1) {
+			const uint32_t mid = (lo + hi) >> 1;
+			if (row[mid] <= ts) lo = mid; else hi = mid;
+		}
+		first[c] = lo;
+	}
+}
+
+// Pass B: every tile of a first-level partition is gathered from the pass-A
+// runs that cover it, sorted by the second digit in LDS and written to `out`
+// at its place, with the run table of k_sortchunk.  Staging writes each tile
+// position's source into LDS (two buffers); the next tile's loads are issued
+// before the current tile is sorted, and every thread issues exactly
+// ITEMS*WV of them (past the tile end they re-read its last tuple) so the
+// waits stay exact.  row = scanned counts of one digit over the pass-A tiles
+// of one segment (row[i+1] - row[i] = tile i's run length), srow = the runs'
+// positions in `in`.
+template <int WV, int NT, int ITEMS>
+__global__ __launch_bounds__(NT) void k_sortchunk_g(
+    const hj_seg *__restrict__ segs2, const uint32_t *__restrict__ chunk_seg2, const hj_seg *__restrict__ segs1,
+    const uint64_t *__restrict__ hist, const uint64_t *__restrict__ srcp, const uint32_t *__restrict__ first,
+    const hj_stats *__restrict__ st, uint32_t b1, uint32_t bits, const uint4 *__restrict__ in,
+    uint4 *__restrict__ out, uint16_t *__restrict__ run, uint64_t *__restrict__ cstart) {
+	constexpr int TILE = NT * ITEMS;
+	extern __shared__ uint4 lds_u4[];
+	const uint32_t F = 1u << bits, fmask = F - 1, F1 = 1u << b1;
+	uint4 *tile = lds_u4;
+	uint64_t *src0 = (uint64_t *)(tile + TILE * WV), *src1 = src0 + TILE;
+	uint32_t *tcnt = (uint32_t *)(src1 + TILE);
+	uint32_t *tbase = tcnt + F;
+	uint32_t *red = tbase + F + 1;
+	const uint32_t total = (uint32_t)st->total_chunks;
+	const uint32_t per = (F + NT - 1) / NT;
+	const uint32_t grid = gridDim.x;
+	for (uint32_t p = threadIdx.x; p < F; p += NT) tcnt[p] = 0;
+
+	auto tile_of = [&](uint32_t c, uint64_t &pos, uint32_t &n) {
+		if (c >= total) {
+			pos = 0;
+			n = 0;
+			return;
+		}
+		const hj_seg &g2 = segs2[chunk_seg2[c]];
+		pos = g2.out_start + (uint64_t)(c - g2.chunk_base) * TILE;
+		n = (uint32_t)min((uint64_t)TILE, g2.out_start + g2.len - pos);
+	};
+	// source position of every tuple of tile c into srcb[0..n)
+	auto stage = [&](uint32_t c, uint64_t *srcb) {
+		uint64_t pos;
+		uint32_t n;
+		tile_of(c, pos, n);
+		if (n == 0) return;
+		const uint32_t s2 = chunk_seg2[c];
+		const hj_seg &g1 = segs1[s2 >> b1];
+		const uint64_t at = g1.hist_base + (uint64_t)(s2 & (F1 - 1)) * g1.nchunks;
+		const uint64_t *row = hist + at, *srow = srcp + at;
+		for (uint32_t i = first[c];; i += NT) {
+			const uint32_t ii = i + threadIdx.x;
+			if (ii < g1.nchunks) {
+				const uint64_t H = row[ii], lo = max(H, pos), hi = min(row[ii + 1], pos + n);
+				const uint64_t sp = srow[ii] + (lo - H);
+				for (uint64_t q = lo; q < hi; q++) srcb[q - pos] = sp + (q - lo);
+			}
+			if (i + NT >= g1.nchunks || row[i + NT] >= pos + n) break;
+		}
+	};
+	auto load = [&](uint4 (&v)[ITEMS][WV], uint32_t c, const uint64_t *srcb) {
+		uint64_t pos;
+		uint32_t n;
+		tile_of(c, pos, n);
+#pragma unroll
+		for (int k = 0; k < ITEMS; k++) {
+			const uint32_t f = min((uint32_t)(k * NT) + threadIdx.x, n ? n - 1 : 0u);
+			const uint64_t sp = n ? srcb[f] : 0;
+#pragma unroll
+			for (int w = 0; w < WV; w++) v[k][w] = hj_load_stream(in + sp * WV + w);
+		}
+	};
+	auto sort_store = [&](uint4 (&v)[ITEMS][WV], uint32_t c) {
+		uint64_t pos;
+		uint32_t n;
+		tile_of(c, pos, n);
+		uint32_t dig[ITEMS], rank[ITEMS];
+#pragma unroll
+		for (int k = 0; k < ITEMS; k++) {
+			const uint32_t idx = k * NT + threadIdx.x;
+			if (idx < n) {
+				dig[k] = (uint32_t)(hj_key_of(v[k][0]) >> b1) & fmask;
+				rank[k] = atomicAdd(&tcnt[dig[k]], 1u);
+			}
+		}
+		__syncthreads();
+		{
+			const uint32_t p0 = threadIdx.x * per;
+			uint32_t sum = 0;
+			for (uint32_t q = 0; q < per; q++)
+				if (p0 + q < F) sum += tcnt[p0 + q];
+			uint32_t tot;
+			uint32_t r = hj_block_exscan<uint32_t>(sum, red, &tot);
+			if (n)
+				for (uint32_t q = 0; q < per; q++)
+					if (p0 + q < F) {
+						tbase[p0 + q] = r;
+						run[(uint64_t)c * (F + 1) + p0 + q] = (uint16_t)r;
+						r += tcnt[p0 + q];
+					}
+			if (threadIdx.x == 0 && n) {
+				run[(uint64_t)c * (F + 1) + F] = (uint16_t)n;
+				cstart[c] = pos;
+			}
+		}
+		__syncthreads();
+#pragma unroll
+		for (int k = 0; k < ITEMS; k++) {
+			const uint32_t idx = k * NT + threadIdx.x;
+			if (idx < n) {
+				const uint32_t q = tbase[dig[k]] + rank[k];
+#pragma unroll
+				for (int w = 0; w < WV; w++) tile[q * WV + w] = v[k][w];
+			}
+		}
+		__syncthreads();
+		uint4 *dst = out + pos * WV;
+		for (uint32_t u = threadIdx.x; u < n * WV; u += NT) hj_store_stream(&dst[u], tile[u]);
+		for (uint32_t p = threadIdx.x; p < F; p += NT) tcnt[p] = 0;
+	};
+
+	uint4 va[ITEMS][WV], vb[ITEMS][WV];
+	stage(blockIdx.x, src0);
+	__syncthreads();
+	load(va, blockIdx.x, src0);
+	for (uint32_t c = blockIdx.x; c < total; c += 2 * grid) {
+		stage(c + grid, src1);
+		__syncthreads();
+		load(vb, c + grid, src1);
+		sort_store(va, c);
+		stage(c + 2 * grid, src0);
+		__syncthreads();
+		load(va, c + 2 * grid, src0);
+		sort_store(vb, c + grid);  // an empty tile past the end changes nothing
+	}
+}
+
 // Work list of the two-pass join: pair q = (p2 << b1) | p1 holds the chunk id
 // ranges of pass-1 partition p1 of R (segment p1) and of S (segment F1+p1).
 __global__ void k_plan2(const hj_seg *__restrict__ segs2, uint32_t Q, uint32_t b1, uint32_t cpt,
@@ -1079,6 +1289,38 @@ static void launch_sortchunk(hj_ctx_t *ctx, uint32_t wv, unsigned grid, size_t l
 	}
 }
 
+template <int WV, int NT, int ITEMS>
+static void launch_sortchunk_g_t(hj_ctx_t *ctx, unsigned grid, size_t lds, const hj_seg *segs2, const uint32_t *cmap2,
+                                 const hj_seg *segs1, const uint64_t *hist, const uint64_t *srcp, const uint32_t *first,
+                                 uint32_t b1, uint32_t bits, const void *in, void *out, uint16_t *run, uint64_t *cst) {
+	auto kfn = k_sortchunk_g<WV, NT, ITEMS>;
+	allow_lds(ctx, (const void *)kfn);
+	hipLaunchKernelGGL(kfn, dim3(grid), dim3(NT), lds, ctx->stream, segs2, cmap2, segs1, hist, srcp, first,
+	                   (const hj_stats *)ctx->stats.p, b1, bits, (const uint4 *)in, (uint4 *)out, run, cst);
+}
+
+// LDS of k_sortchunk_g: tile, two source-position buffers, counters
+static size_t gathersort_lds(const part_cfg &c, uint32_t wv, uint32_t F) {
+	return (size_t)c.tile * wv * 16 + (size_t)c.tile * 16 + (2 * (size_t)F + 1) * 4 + 64 * 4;
+}
+
+static void launch_sortchunk_g(hj_ctx_t *ctx, uint32_t wv, unsigned grid, size_t lds, const hj_seg *segs2,
+                               const uint32_t *cmap2, const hj_seg *segs1, const uint64_t *hist, const uint64_t *srcp,
+                               const uint32_t *first, uint32_t b1, uint32_t bits, const void *in, void *out,
+                               uint16_t *run, uint64_t *cst) {
+	// shapes must match cfg_for(wv, 0)
+#define G(W, N, I) \
+	launch_sortchunk_g_t<W, N, I>(ctx, grid, lds, segs2, cmap2, segs1, hist, srcp, first, b1, bits, in, out, run, cst)
+	switch (wv) {
+	case 1: G(1, 512, 4); break;
+	case 2: G(2, 512, 2); break;
+	case 4: G(4, 512, 1); break;
+	case 8: G(8, 256, 1); break;
+	default: G(16, 256, 1); break;
+	}
+#undef G
+}
+
 template <int WV, bool LP>
 static void launch_join2_t(hj_ctx_t *ctx, unsigned grid, size_t lds, const void *P, const uint64_t *cst,
                            const uint16_t *run, uint32_t F2, uint32_t b1, const hj_pair *pairs, const uint64_t *tpre,
@@ -1153,8 +1395,10 @@ int hj_run_radix(hj_ctx_t *ctx, const hj_relation_t *R, const hj_relation_t *S, 
 		if (ch < (uint64_t)pc.tile) ch = pc.tile;
 		return ch;
 	};
-	const part_cfg pc2 = cfg_for(wv, 0);  // pass-2 chunk = one 32 KB sort tile
-	const uint64_t ch1 = chunk_for(N), ch2 = pc2.tile;
+	const part_cfg pc2 = cfg_for(wv, 0);  // pass-A / pass-B tile = one 32 KB sort tile
+	// one pass: histogram + scatter over large chunks; two passes: pass A
+	// sorts sort-tile-sized chunks in place
+	const uint64_t ch1 = P == 2 ? (uint64_t)pc2.tile : chunk_for(N), ch2 = pc2.tile;
 	const uint64_t chunks1 = (nr + ch1 - 1) / ch1 + (ns + ch1 - 1) / ch1;
 	const uint32_t nseg2 = 2 * F1;
 	const uint64_t chunks2_max = N / ch2 + nseg2 + 1;
@@ -1162,8 +1406,9 @@ int hj_run_radix(hj_ctx_t *ctx, const hj_relation_t *R, const hj_relation_t *S, 
 
 	// scratch
 	int rc;
-	if (P >= 1 && (rc = hj_reserve(ctx, ctx->part[0], N * W + 64))) return rc;
-	if (P == 2 && (rc = hj_reserve(ctx, ctx->part[1], N * W + 64))) return rc;
+	// slack: a gathering kernel re-reads the first tuple for an empty tile
+	if (P >= 1 && (rc = hj_reserve(ctx, ctx->part[0], N * W + 4096))) return rc;
+	if (P == 2 && (rc = hj_reserve(ctx, ctx->part[1], N * W + 4096))) return rc;
 	const uint64_t hist_len = std::max(hist1_len, hist2_len) + 1;
 	if ((rc = hj_reserve(ctx, ctx->hist, hist_len * 8))) return rc;
 	if ((rc = hj_reserve(ctx, ctx->scan_tmp, ((hist_len + Q + nseg2) / SCAN_CH + 2) * 8 + 4096))) return rc;
@@ -1181,6 +1426,9 @@ int hj_run_radix(hj_ctx_t *ctx, const hj_relation_t *R, const hj_relation_t *S, 
 	const size_t o_pairs = take((size_t)Q * sizeof(hj_pair)), o_tiles = take(((size_t)Q + 1) * 8);
 	const size_t o_run = take(P == 2 ? chunks2_max * (F2 + 1) * 2 : 8);
 	const size_t o_cst = take(P == 2 ? chunks2_max * 8 : 8);
+	const size_t o_runA = take(P == 2 ? chunks1 * (F1 + 1) * 2 : 8), o_cstA = take(P == 2 ? chunks1 * 8 : 8);
+	const size_t o_srcp = take(P == 2 ? (hist1_len + 1) * 8 : 8);
+	const size_t o_first = take(P == 2 ? chunks2_max * 4 : 8);
 	if ((rc = hj_reserve(ctx, ctx->meta, off))) return rc;
 	char *meta = (char *)ctx->meta.p;
 	hj_seg *segs1 = (hj_seg *)(meta + o_seg1), *segs2 = (hj_seg *)(meta + o_seg2);
@@ -1188,21 +1436,43 @@ int hj_run_radix(hj_ctx_t *ctx, const hj_relation_t *R, const hj_relation_t *S, 
 	uint64_t *pb1 = (uint64_t *)(meta + o_pb1);
 	uint64_t *nch = (uint64_t *)(meta + o_nch), *tiles = (uint64_t *)(meta + o_tiles);
 	hj_pair *pairs = (hj_pair *)(meta + o_pairs);
-	uint16_t *runs = (uint16_t *)(meta + o_run);
-	uint64_t *cst = (uint64_t *)(meta + o_cst);
+	uint16_t *runs = (uint16_t *)(meta + o_run), *runsA = (uint16_t *)(meta + o_runA);
+	uint64_t *cst = (uint64_t *)(meta + o_cst), *cstA = (uint64_t *)(meta + o_cstA);
+	uint32_t *first = (uint32_t *)(meta + o_first);
+	uint64_t *srcp = (uint64_t *)(meta + o_srcp);
 	uint64_t *hist = (uint64_t *)ctx->hist.p;
 
 	HJ_HIP(ctx, hipEventRecord(ctx->ev[0], ctx->stream));
 	hj_stats_reset(ctx);
 	const void *Rb = R->tuples, *Sb = S->tuples;
 	const uint64_t *pbF = nullptr;
+	const size_t sldsA = (size_t)pc2.tile * wv * 16 + (2 * (size_t)F1 + 1) * 4 + 64 * 4;
+	const size_t sldsB = gathersort_lds(pc2, wv, F2);
 	if (P >= 1) {
 		hipLaunchKernelGGL(k_seg_pair, dim3(std::max<unsigned>(1, (unsigned)((chunks1 + 255) / 256))), dim3(256), 0,
 		                   ctx->stream, (const uint4 *)R->tuples, nr, (const uint4 *)S->tuples, ns, ch1, F1, segs1,
 		                   cmap, st);
-		rc = launch_pass(ctx, wv, variant, (unsigned)chunks1, segs1, cmap, ch1, 0, b1, hist, hist1_len, ctx->part[0].p,
-		                 st, (uint64_t *)ctx->scan_tmp.p);
-		if (rc) return rc;
+		if (P == 1) {
+			rc = launch_pass(ctx, wv, variant, (unsigned)chunks1, segs1, cmap, ch1, 0, b1, hist, hist1_len,
+			                 ctx->part[0].p, st, (uint64_t *)ctx->scan_tmp.p);
+			if (rc) return rc;
+		} else {
+			// pass A: every input tile sorted by the first digit in place; its
+			// run tables give the histogram
+			if (sldsA + 1024 > ctx->prop.sharedMemPerBlock || sldsB + 1024 > ctx->prop.sharedMemPerBlock)
+				return hj_fail(ctx, HJ_EINVAL, "radix bits per pass too large for LDS");
+			const unsigned gA = (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(chunks1, (uint64_t)ctx->num_cu * 4));
+			hj_kt_begin(ctx, HJ_K_TILESORT);
+			launch_sortchunk(ctx, wv, gA, sldsA, segs1, cmap, 0, b1, ctx->part[0].p, runsA, cstA);
+			hj_kt_end(ctx);
+			hj_kt_begin(ctx, HJ_K_SCAN);
+			hipLaunchKernelGGL(k_runs_to_hist, dim3((unsigned)std::max<uint64_t>(1, (chunks1 + 63) / 64), (F1 + 63) / 64),
+			                   dim3(256), 0,
+			                   ctx->stream, runsA, cstA, segs1, cmap, (uint32_t)chunks1, F1, hist, srcp);
+			rc = scan_u64(ctx, hist, hist1_len, (uint64_t *)ctx->scan_tmp.p);
+			hj_kt_end(ctx);
+			if (rc) return rc;
+		}
 		hipLaunchKernelGGL(k_bounds, dim3((unsigned)std::min<uint64_t>(1024, (2ull * F1 + 256) / 256)), dim3(256), 0,
 		                   ctx->stream, segs1, 2u, F1, hist, pb1);
 		pbF = pb1;
@@ -1213,11 +1483,14 @@ int hj_run_radix(hj_ctx_t *ctx, const hj_relation_t *R, const hj_relation_t *S, 
 			if ((rc = scan_u64(ctx, nch, nseg2, (uint64_t *)ctx->scan_tmp.p))) return rc;
 			hipLaunchKernelGGL(k_seg2_fin, dim3(std::min<uint32_t>(nseg2, 4096)), dim3(256), 0, ctx->stream, nch,
 			                   nseg2, F2, segs2, cmap, st);
-			const size_t slds = (size_t)pc2.tile * wv * 16 + (2 * (size_t)F2 + 1) * 4 + 64 * 4;
-			if (slds + 1024 > ctx->prop.sharedMemPerBlock) return hj_fail(ctx, HJ_EINVAL, "pass-2 bits too large for LDS");
+			hipLaunchKernelGGL(k_first_run, dim3((unsigned)std::min<uint64_t>(4096, (chunks2_max + 255) / 256)),
+			                   dim3(256), 0, ctx->stream, segs1, segs2, cmap, st, b1, (uint32_t)ch2, hist, first);
+			// pass B: each first-level partition's tiles gathered from the
+			// pass-A runs and sorted by the second digit
 			const unsigned sgrid = (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(chunks2_max, (uint64_t)ctx->num_cu * 4));
-			hj_kt_begin(ctx, HJ_K_SORTCHUNK);
-			launch_sortchunk(ctx, wv, sgrid, slds, segs2, cmap, b1, b2, ctx->part[1].p, runs, cst);
+			hj_kt_begin(ctx, HJ_K_GATHERSORT);
+			launch_sortchunk_g(ctx, wv, sgrid, sldsB, segs2, cmap, segs1, hist, srcp, first, b1, b2, ctx->part[0].p,
+			                   ctx->part[1].p, runs, cst);
 			hj_kt_end(ctx);
 			Rb = Sb = ctx->part[1].p;
 		}

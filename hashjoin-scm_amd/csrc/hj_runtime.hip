@@ -70,6 +70,7 @@ double hj_event_seconds(hj_ctx_t *ctx, int a, int b) {
 static const char *k_names[HJ_K_COUNT] = {
     "fill", "lp_build", "lp_probe", "lp_fixup", "sc_build", "sc_probe",
     "radix_hist", "radix_scan", "radix_scatter", "radix_plan", "radix_join", "radix_sortchunk",
+    "radix_tilesort", "radix_gathersort",
 };
 
 extern "C" {
