@@ -529,12 +529,13 @@ __global__ __launch_bounds__(NT) void k_join(const uint4 *__restrict__ Rb, const
 }
 
 // ------------------------------------------------------------ pass 2
-// Chunk-local second pass: every chunk (one LDS tile) of a pass-1 partition
-// is counting-sorted by the pass-2 digit and written back to the same range
-// of the output buffer, so the stores of a chunk are one contiguous stream.
-// run[c*(F+1) + p] = start of digit p's run inside chunk c (u16), run[..F] =
-// chunk length; cstart[c] = absolute position of chunk c.  A final
-// partition (p1, p2) is then the union of run p2 of every chunk of p1.
+// Tile sort: every chunk (one LDS tile) of a segment is counting-sorted by
+// one digit and written to the same range of `out`, so the stores of a chunk
+// are one contiguous stream.  run[c*(F+1) + p] = start of digit p's run
+// inside chunk c (u16), run[..F] = chunk length; cstart[c] = absolute
+// position of chunk c.  Pass A of a two-pass join runs it over the input.
+// The next tiles are loaded while a tile is sorted (two register buffers,
+// as in k_scatter).
 template <int WV, int NT, int ITEMS>
 __global__ __launch_bounds__(NT) void k_sortchunk(const hj_seg *__restrict__ segs, const uint32_t *__restrict__ cmap,
                                                   const hj_stats *__restrict__ st, uint32_t shift, uint32_t bits,
@@ -549,33 +550,40 @@ __global__ __launch_bounds__(NT) void k_sortchunk(const hj_seg *__restrict__ seg
 	uint32_t *red = tbase + F + 1;
 	const uint32_t total = (uint32_t)st->total_chunks;
 	const uint32_t per = (F + NT - 1) / NT;
-	uint4 v[ITEMS][WV];
-	auto chunk_of = [&](uint32_t c, const uint4 *&src, uint64_t &pos, uint32_t &n) {
-		const hj_seg g = segs[cmap[c]];
+	const uint32_t grid = gridDim.x;
+	// tile c: source, output position, length (an empty tile past the end
+	// reads `out`, whose first tuple always exists)
+	auto tile_of = [&](uint32_t c, const uint4 *&src, uint64_t &pos, uint32_t &n) {
+		if (c >= total) {
+			src = out;
+			pos = 0;
+			n = 0;
+			return;
+		}
+		const hj_seg &g = segs[cmap[c]];
 		const uint64_t cl = c - g.chunk_base;
 		pos = g.out_start + cl * TILE;
 		src = g.in + cl * TILE * WV;
 		n = (uint32_t)min((uint64_t)TILE, g.len - cl * TILE);
 	};
-	auto load = [&](const uint4 *src, uint32_t n) {
+	// every thread issues exactly ITEMS*WV loads (see k_scatter)
+	auto load = [&](uint4 (&v)[ITEMS][WV], uint32_t c) {
+		const uint4 *src;
+		uint64_t pos;
+		uint32_t n;
+		tile_of(c, src, pos, n);
 #pragma unroll
 		for (int k = 0; k < ITEMS; k++) {
-			const uint32_t idx = k * NT + threadIdx.x;
-			if (idx < n)
+			const uint32_t idx = min((uint32_t)(k * NT) + threadIdx.x, n ? n - 1 : 0u);
 #pragma unroll
-				for (int w = 0; w < WV; w++) v[k][w] = hj_load_stream(src + (uint64_t)idx * WV + w);
+			for (int w = 0; w < WV; w++) v[k][w] = hj_load_stream(src + (uint64_t)idx * WV + w);
 		}
 	};
-	for (uint32_t p = threadIdx.x; p < F; p += NT) tcnt[p] = 0;
-	const uint4 *src;
-	uint64_t pos;
-	uint32_t n;
-	if (blockIdx.x < total) {
-		chunk_of(blockIdx.x, src, pos, n);
-		load(src, n);
-	}
-	__syncthreads();
-	for (uint32_t c = blockIdx.x; c < total; c += gridDim.x) {
+	auto do_tile = [&](uint4 (&v)[ITEMS][WV], uint32_t c) {
+		const uint4 *src;
+		uint64_t pos;
+		uint32_t n;
+		tile_of(c, src, pos, n);
 		uint32_t dig[ITEMS], rank[ITEMS];
 #pragma unroll
 		for (int k = 0; k < ITEMS; k++) {
@@ -593,13 +601,14 @@ __global__ __launch_bounds__(NT) void k_sortchunk(const hj_seg *__restrict__ seg
 				if (p0 + q < F) sum += tcnt[p0 + q];
 			uint32_t tot;
 			uint32_t r = hj_block_exscan<uint32_t>(sum, red, &tot);
-			for (uint32_t q = 0; q < per; q++)
-				if (p0 + q < F) {
-					tbase[p0 + q] = r;
-					run[(uint64_t)c * (F + 1) + p0 + q] = (uint16_t)r;
-					r += tcnt[p0 + q];
-				}
-			if (threadIdx.x == 0) {
+			if (n)
+				for (uint32_t q = 0; q < per; q++)
+					if (p0 + q < F) {
+						tbase[p0 + q] = r;
+						run[(uint64_t)c * (F + 1) + p0 + q] = (uint16_t)r;
+						r += tcnt[p0 + q];
+					}
+			if (threadIdx.x == 0 && n) {
 				run[(uint64_t)c * (F + 1) + F] = (uint16_t)n;
 				cstart[c] = pos;
 			}
@@ -615,17 +624,20 @@ __global__ __launch_bounds__(NT) void k_sortchunk(const hj_seg *__restrict__ seg
 			}
 		}
 		__syncthreads();
-		const uint32_t n_cur = n;
-		const uint64_t pos_cur = pos;
-		const uint32_t cn = c + gridDim.x;
-		if (cn < total) {
-			chunk_of(cn, src, pos, n);
-			load(src, n);
-		}
-		uint4 *dst = out + pos_cur * WV;
-		for (uint32_t u = threadIdx.x; u < n_cur * WV; u += NT) hj_store_stream(&dst[u], tile[u]);
+		uint4 *dst = out + pos * WV;
+		for (uint32_t u = threadIdx.x; u < n * WV; u += NT) hj_store_stream(&dst[u], tile[u]);
+		load(v, c + 2 * grid);
 		for (uint32_t p = threadIdx.x; p < F; p += NT) tcnt[p] = 0;
 		__syncthreads();
+	};
+	for (uint32_t p = threadIdx.x; p < F; p += NT) tcnt[p] = 0;
+	__syncthreads();
+	uint4 va[ITEMS][WV], vb[ITEMS][WV];
+	load(va, blockIdx.x);
+	load(vb, blockIdx.x + grid);
+	for (uint32_t c = blockIdx.x; c < total; c += 2 * grid) {
+		do_tile(va, c);
+		do_tile(vb, c + grid);  // an empty tile past the end changes nothing
 	}
 }
 
@@ -640,13 +652,13 @@ __global__ __launch_bounds__(NT) void k_sortchunk(const hj_seg *__restrict__ seg
 // histogram + scatter pass followed by the chunk-local second pass.
 
 // Count matrix from pass-A run tables: hist[g.hist_base + d*g.nchunks + cl]
-// = tuples of digit d in tile cl of segment g.  A workgroup transposes 64
-// tiles x 64 digits through LDS so that reads and writes are both coalesced.
-__global__ __launch_bounds__(256) void k_runs_to_hist(const uint16_t *__restrict__ run, const uint64_t *__restrict__ cst,
-                                                      const hj_seg *__restrict__ segs,
+// = tuples of digit d in tile cl of segment g, and beg_t at the same index =
+// where that run starts inside the tile.  A workgroup transposes 64 tiles x
+// 64 digits through LDS so that reads and writes are both coalesced.
+__global__ __launch_bounds__(256) void k_runs_to_hist(const uint16_t *__restrict__ run, const hj_seg *__restrict__ segs,
                                                       const uint32_t *__restrict__ chunk_seg, uint32_t total,
                                                       uint32_t F, uint64_t *__restrict__ hist,
-                                                      uint64_t *__restrict__ srcp) {
+                                                      uint16_t *__restrict__ beg_t) {
 	__shared__ uint32_t cnt[64][65];
 	__shared__ uint32_t beg[64][65];
 	const uint32_t c0 = blockIdx.x * 64, d0 = blockIdx.y * 64;
@@ -668,7 +680,7 @@ __global__ __launch_bounds__(256) void k_runs_to_hist(const uint16_t *__restrict
 			const hj_seg &g = segs[chunk_seg[c]];
 			const uint64_t at = g.hist_base + (uint64_t)d * g.nchunks + (c - g.chunk_base);
 			hist[at] = cnt[ci][dj];
-			srcp[at] = cst[c] + beg[ci][dj];
+			beg_t[at] = (uint16_t)beg[ci][dj];
 		}
 	}
 }
@@ -697,24 +709,24 @@ __global__ void k_first_run(const hj_seg *__restrict__ segs1, const hj_seg *__re
 // Pass B: every tile of a first-level partition is gathered from the pass-A
 // runs that cover it, sorted by the second digit in LDS and written to `out`
 // at its place, with the run table of k_sortchunk.  Staging writes each tile
-// position's source into LDS (two buffers); the next tile's loads are issued
-// before the current tile is sorted, and every thread issues exactly
-// ITEMS*WV of them (past the tile end they re-read its last tuple) so the
-// waits stay exact.  row = scanned counts of one digit over the pass-A tiles
-// of one segment (row[i+1] - row[i] = tile i's run length), srow = the runs'
-// positions in `in`.
+// position's source (offset in its segment) into LDS (two buffers); the
+// next tile's loads are issued before the current tile is sorted, and every
+// thread issues exactly ITEMS*WV of them (past the tile end they re-read its
+// last tuple) so the waits stay exact.  row = scanned counts of one digit
+// over the pass-A tiles of one segment (row[i+1] - row[i] = tile i's run
+// length), brow = where those runs start inside their tiles.
 template <int WV, int NT, int ITEMS>
 __global__ __launch_bounds__(NT) void k_sortchunk_g(
     const hj_seg *__restrict__ segs2, const uint32_t *__restrict__ chunk_seg2, const hj_seg *__restrict__ segs1,
-    const uint64_t *__restrict__ hist, const uint64_t *__restrict__ srcp, const uint32_t *__restrict__ first,
+    const uint64_t *__restrict__ hist, const uint16_t *__restrict__ beg_t, const uint32_t *__restrict__ first,
     const hj_stats *__restrict__ st, uint32_t b1, uint32_t bits, const uint4 *__restrict__ in,
     uint4 *__restrict__ out, uint16_t *__restrict__ run, uint64_t *__restrict__ cstart) {
 	constexpr int TILE = NT * ITEMS;
 	extern __shared__ uint4 lds_u4[];
 	const uint32_t F = 1u << bits, fmask = F - 1, F1 = 1u << b1;
 	uint4 *tile = lds_u4;
-	uint64_t *src0 = (uint64_t *)(tile + TILE * WV), *src1 = src0 + TILE;
-	uint32_t *tcnt = (uint32_t *)(src1 + TILE);
+	uint32_t *src0 = (uint32_t *)(tile + TILE * WV), *src1 = src0 + TILE;
+	uint32_t *tcnt = src1 + TILE;
 	uint32_t *tbase = tcnt + F;
 	uint32_t *red = tbase + F + 1;
 	const uint32_t total = (uint32_t)st->total_chunks;
@@ -732,8 +744,12 @@ __global__ __launch_bounds__(NT) void k_sortchunk_g(
 		pos = g2.out_start + (uint64_t)(c - g2.chunk_base) * TILE;
 		n = (uint32_t)min((uint64_t)TILE, g2.out_start + g2.len - pos);
 	};
-	// source position of every tuple of tile c into srcb[0..n)
-	auto stage = [&](uint32_t c, uint64_t *srcb) {
+	// the segment base that source offsets of tile c are relative to
+	auto base_of = [&](uint32_t c) -> const uint4 * {
+		return c < total ? in + segs1[chunk_seg2[c] >> b1].out_start * WV : in;
+	};
+	// source offset of every tuple of tile c into srcb[0..n)
+	auto stage = [&](uint32_t c, uint32_t *srcb) {
 		uint64_t pos;
 		uint32_t n;
 		tile_of(c, pos, n);
@@ -741,27 +757,29 @@ __global__ __launch_bounds__(NT) void k_sortchunk_g(
 		const uint32_t s2 = chunk_seg2[c];
 		const hj_seg &g1 = segs1[s2 >> b1];
 		const uint64_t at = g1.hist_base + (uint64_t)(s2 & (F1 - 1)) * g1.nchunks;
-		const uint64_t *row = hist + at, *srow = srcp + at;
+		const uint64_t *row = hist + at;
+		const uint16_t *brow = beg_t + at;
 		for (uint32_t i = first[c];; i += NT) {
 			const uint32_t ii = i + threadIdx.x;
 			if (ii < g1.nchunks) {
 				const uint64_t H = row[ii], lo = max(H, pos), hi = min(row[ii + 1], pos + n);
-				const uint64_t sp = srow[ii] + (lo - H);
-				for (uint64_t q = lo; q < hi; q++) srcb[q - pos] = sp + (q - lo);
+				const uint32_t sp = ii * (uint32_t)TILE + brow[ii] + (uint32_t)(lo - H);
+				for (uint64_t q = lo; q < hi; q++) srcb[q - pos] = sp + (uint32_t)(q - lo);
 			}
 			if (i + NT >= g1.nchunks || row[i + NT] >= pos + n) break;
 		}
 	};
-	auto load = [&](uint4 (&v)[ITEMS][WV], uint32_t c, const uint64_t *srcb) {
+	auto load = [&](uint4 (&v)[ITEMS][WV], uint32_t c, const uint32_t *srcb) {
 		uint64_t pos;
 		uint32_t n;
 		tile_of(c, pos, n);
+		const uint4 *base = base_of(c);
 #pragma unroll
 		for (int k = 0; k < ITEMS; k++) {
 			const uint32_t f = min((uint32_t)(k * NT) + threadIdx.x, n ? n - 1 : 0u);
 			const uint64_t sp = n ? srcb[f] : 0;
 #pragma unroll
-			for (int w = 0; w < WV; w++) v[k][w] = hj_load_stream(in + sp * WV + w);
+			for (int w = 0; w < WV; w++) v[k][w] = hj_load_stream(base + sp * WV + w);
 		}
 	};
 	auto sort_store = [&](uint4 (&v)[ITEMS][WV], uint32_t c) {
@@ -1144,6 +1162,18 @@ static void allow_lds(hj_ctx_t *ctx, const void *kfn) {
 		(void)hipGetLastError();
 }
 
+// Workgroups of `kfn` that are resident at once on the device: the grid of a
+// persistent kernel, so that no workgroup waits for another to finish before
+// it starts its share.
+static unsigned resident_grid(hj_ctx_t *ctx, const void *kfn, int nt, size_t lds) {
+	int nb = 0;
+	if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kfn, nt, lds) != hipSuccess || nb < 1) {
+		(void)hipGetLastError();
+		nb = 1;
+	}
+	return (unsigned)nb * (unsigned)ctx->num_cu;
+}
+
 static bool capturing(hj_ctx_t *ctx) {
 	hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
 	if (hipStreamIsCapturing(ctx->stream, &cs) != hipSuccess) (void)hipGetLastError();
@@ -1272,6 +1302,7 @@ static void launch_sortchunk_t(hj_ctx_t *ctx, unsigned grid, size_t lds, const h
                                uint32_t shift, uint32_t bits, void *out, uint16_t *run, uint64_t *cst) {
 	auto kfn = k_sortchunk<WV, NT, ITEMS>;
 	allow_lds(ctx, (const void *)kfn);
+	grid = std::max(1u, std::min(grid, resident_grid(ctx, (const void *)kfn, NT, lds)));
 	hipLaunchKernelGGL(kfn, dim3(grid), dim3(NT), lds, ctx->stream, segs, cmap, (const hj_stats *)ctx->stats.p, shift,
 	                   bits, (uint4 *)out, run, cst);
 }
@@ -1291,26 +1322,27 @@ static void launch_sortchunk(hj_ctx_t *ctx, uint32_t wv, unsigned grid, size_t l
 
 template <int WV, int NT, int ITEMS>
 static void launch_sortchunk_g_t(hj_ctx_t *ctx, unsigned grid, size_t lds, const hj_seg *segs2, const uint32_t *cmap2,
-                                 const hj_seg *segs1, const uint64_t *hist, const uint64_t *srcp, const uint32_t *first,
+                                 const hj_seg *segs1, const uint64_t *hist, const uint16_t *beg_t, const uint32_t *first,
                                  uint32_t b1, uint32_t bits, const void *in, void *out, uint16_t *run, uint64_t *cst) {
 	auto kfn = k_sortchunk_g<WV, NT, ITEMS>;
 	allow_lds(ctx, (const void *)kfn);
-	hipLaunchKernelGGL(kfn, dim3(grid), dim3(NT), lds, ctx->stream, segs2, cmap2, segs1, hist, srcp, first,
+	grid = std::max(1u, std::min(grid, resident_grid(ctx, (const void *)kfn, NT, lds)));
+	hipLaunchKernelGGL(kfn, dim3(grid), dim3(NT), lds, ctx->stream, segs2, cmap2, segs1, hist, beg_t, first,
 	                   (const hj_stats *)ctx->stats.p, b1, bits, (const uint4 *)in, (uint4 *)out, run, cst);
 }
 
-// LDS of k_sortchunk_g: tile, two source-position buffers, counters
+// LDS of k_sortchunk_g: tile, two source-offset buffers, counters
 static size_t gathersort_lds(const part_cfg &c, uint32_t wv, uint32_t F) {
-	return (size_t)c.tile * wv * 16 + (size_t)c.tile * 16 + (2 * (size_t)F + 1) * 4 + 64 * 4;
+	return (size_t)c.tile * wv * 16 + (size_t)c.tile * 8 + (2 * (size_t)F + 1) * 4 + 64 * 4;
 }
 
 static void launch_sortchunk_g(hj_ctx_t *ctx, uint32_t wv, unsigned grid, size_t lds, const hj_seg *segs2,
-                               const uint32_t *cmap2, const hj_seg *segs1, const uint64_t *hist, const uint64_t *srcp,
+                               const uint32_t *cmap2, const hj_seg *segs1, const uint64_t *hist, const uint16_t *beg_t,
                                const uint32_t *first, uint32_t b1, uint32_t bits, const void *in, void *out,
                                uint16_t *run, uint64_t *cst) {
 	// shapes must match cfg_for(wv, 0)
 #define G(W, N, I) \
-	launch_sortchunk_g_t<W, N, I>(ctx, grid, lds, segs2, cmap2, segs1, hist, srcp, first, b1, bits, in, out, run, cst)
+	launch_sortchunk_g_t<W, N, I>(ctx, grid, lds, segs2, cmap2, segs1, hist, beg_t, first, b1, bits, in, out, run, cst)
 	switch (wv) {
 	case 1: G(1, 512, 4); break;
 	case 2: G(2, 512, 2); break;
@@ -1378,9 +1410,14 @@ int hj_run_radix(hj_ctx_t *ctx, const hj_relation_t *R, const hj_relation_t *S, 
 	if (P == 2 && B < 2) P = 1;
 	// two passes: the first takes the larger half of the bits, so the chunk-
 	// local second pass sorts each tile into at most as many runs as pass 1
-	const uint32_t b1 = P == 2 ? (B + 1) / 2 : (P ? B : 0), b2 = B - b1;
+	uint32_t b1 = P == 2 ? (B + 1) / 2 : (P ? B : 0);
+	if (P == 2 && (prm->part_variant >> 8 & 15) && (prm->part_variant >> 8 & 15) < B)
+		b1 = prm->part_variant >> 8 & 15;  // first-pass width forced for tuning sweeps
+	const uint32_t b2 = B - b1;
 	const uint32_t F1 = 1u << b1, F2 = 1u << b2, Q = 1u << B;
 	if (b1 > 12 || (P == 2 && b2 > 12)) return hj_fail(ctx, HJ_EINVAL, "at most 12 radix bits per pass");
+	if (P == 2 && (nr >> 32 || ns >> 32))
+		return hj_fail(ctx, HJ_EINVAL, "two radix passes take relations of fewer than 2^32 tuples");
 	out->radix_bits = B;
 	out->passes = P;
 	hj_stats *st = (hj_stats *)ctx->stats.p;
@@ -1427,7 +1464,7 @@ int hj_run_radix(hj_ctx_t *ctx, const hj_relation_t *R, const hj_relation_t *S, 
 	const size_t o_run = take(P == 2 ? chunks2_max * (F2 + 1) * 2 : 8);
 	const size_t o_cst = take(P == 2 ? chunks2_max * 8 : 8);
 	const size_t o_runA = take(P == 2 ? chunks1 * (F1 + 1) * 2 : 8), o_cstA = take(P == 2 ? chunks1 * 8 : 8);
-	const size_t o_srcp = take(P == 2 ? (hist1_len + 1) * 8 : 8);
+	const size_t o_begt = take(P == 2 ? (hist1_len + 1) * 2 : 8);
 	const size_t o_first = take(P == 2 ? chunks2_max * 4 : 8);
 	if ((rc = hj_reserve(ctx, ctx->meta, off))) return rc;
 	char *meta = (char *)ctx->meta.p;
@@ -1439,7 +1476,7 @@ int hj_run_radix(hj_ctx_t *ctx, const hj_relation_t *R, const hj_relation_t *S, 
 	uint16_t *runs = (uint16_t *)(meta + o_run), *runsA = (uint16_t *)(meta + o_runA);
 	uint64_t *cst = (uint64_t *)(meta + o_cst), *cstA = (uint64_t *)(meta + o_cstA);
 	uint32_t *first = (uint32_t *)(meta + o_first);
-	uint64_t *srcp = (uint64_t *)(meta + o_srcp);
+	uint16_t *beg_t = (uint16_t *)(meta + o_begt);
 	uint64_t *hist = (uint64_t *)ctx->hist.p;
 
 	HJ_HIP(ctx, hipEventRecord(ctx->ev[0], ctx->stream));
@@ -1468,7 +1505,7 @@ int hj_run_radix(hj_ctx_t *ctx, const hj_relation_t *R, const hj_relation_t *S, 
 			hj_kt_begin(ctx, HJ_K_SCAN);
 			hipLaunchKernelGGL(k_runs_to_hist, dim3((unsigned)std::max<uint64_t>(1, (chunks1 + 63) / 64), (F1 + 63) / 64),
 			                   dim3(256), 0,
-			                   ctx->stream, runsA, cstA, segs1, cmap, (uint32_t)chunks1, F1, hist, srcp);
+			                   ctx->stream, runsA, segs1, cmap, (uint32_t)chunks1, F1, hist, beg_t);
 			rc = scan_u64(ctx, hist, hist1_len, (uint64_t *)ctx->scan_tmp.p);
 			hj_kt_end(ctx);
 			if (rc) return rc;
@@ -1489,7 +1526,7 @@ int hj_run_radix(hj_ctx_t *ctx, const hj_relation_t *R, const hj_relation_t *S, 
 			// pass-A runs and sorted by the second digit
 			const unsigned sgrid = (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(chunks2_max, (uint64_t)ctx->num_cu * 4));
 			hj_kt_begin(ctx, HJ_K_GATHERSORT);
-			launch_sortchunk_g(ctx, wv, sgrid, sldsB, segs2, cmap, segs1, hist, srcp, first, b1, b2, ctx->part[0].p,
+			launch_sortchunk_g(ctx, wv, sgrid, sldsB, segs2, cmap, segs1, hist, beg_t, first, b1, b2, ctx->part[0].p,
 			                   ctx->part[1].p, runs, cst);
 			hj_kt_end(ctx);
 			Rb = Sb = ctx->part[1].p;

@@ -62,7 +62,8 @@ typedef struct {
 	uint32_t part_variant;   /* tuning sweeps: bits 0-3 partition tile shape
 	                            for 16-byte tuples (0 default, 1..5 fixed);
 	                            bits 4-6 scatter write-combining line:
-	                            0 auto, k = 2^(k-1) tuples                    */
+	                            0 auto, k = 2^(k-1) tuples; bits 8-11 the
+	                            first-pass width of two passes (0 auto)       */
 	uint32_t reserved[3];
 } hj_params_t;
 
