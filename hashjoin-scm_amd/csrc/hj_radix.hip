@@ -860,6 +860,26 @@ __global__ void k_plan2(const hj_seg *__restrict__ segs2, uint32_t Q, uint32_t b
 	}
 }
 
+// Digit-major copy of a chunk run table: run_t[p * stride + c] =
+// run[c * (F + 1) + p] for p in [0, F], so that a partition's runs over
+// consecutive chunks are read as consecutive u16s.
+__global__ __launch_bounds__(256) void k_run_transpose(const uint16_t *__restrict__ run, const hj_stats *__restrict__ st,
+                                                       uint32_t F, uint32_t stride, uint16_t *__restrict__ run_t) {
+	__shared__ uint16_t t[64][66];
+	const uint32_t total = (uint32_t)st->total_chunks;
+	const uint32_t c0 = blockIdx.x * 64, p0 = blockIdx.y * 64;
+	if (c0 >= total) return;
+	for (uint32_t e = threadIdx.x; e < 64 * 64; e += 256) {
+		const uint32_t ci = e / 64, pj = e % 64, c = c0 + ci, p = p0 + pj;
+		t[ci][pj] = (c < total && p <= F) ? run[(uint64_t)c * (F + 1) + p] : (uint16_t)0;
+	}
+	__syncthreads();
+	for (uint32_t e = threadIdx.x; e < 64 * 64; e += 256) {
+		const uint32_t pj = e / 64, ci = e % 64, c = c0 + ci, p = p0 + pj;
+		if (c < total && p <= F) run_t[(uint64_t)p * stride + c] = t[ci][pj];
+	}
+}
+
 // Join over chunk runs.  A work item is (final partition q = (p2 << b1) | p1,
 // probe tile = a range of pass-1 chunks of S).  Descriptors of up to NT
 // chunk runs (absolute start, flat offset) are staged in LDS; the staged runs
@@ -873,7 +893,7 @@ __global__ void k_plan2(const hj_seg *__restrict__ segs2, uint32_t Q, uint32_t b
 // LDS: keys u64[cap] | pays u64[cap] | heads u32[np2(cap)] | next u16[cap]
 template <int WV, int NT, bool LP>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_join2(const uint4 *__restrict__ P, const uint64_t *__restrict__ cstart,
-                                              const uint16_t *__restrict__ run, uint32_t F2, uint32_t b1,
+                                              const uint16_t *__restrict__ run_t, uint32_t stride, uint32_t b1,
                                               const hj_pair *__restrict__ pairs,
                                               const uint64_t *__restrict__ tprefix, uint32_t npairs, uint32_t B,
                                               uint32_t cpt, uint32_t cap, hj_stats *st) {
@@ -894,7 +914,6 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
 	uint16_t *next = (uint16_t *)(heads + nb);
 	const uint32_t lane = threadIdx.x & (HJ_WAVE - 1), wid = threadIdx.x / HJ_WAVE;
 	const uint64_t total = tprefix[npairs];
-	const uint32_t stride = F2 + 1;
 	unsigned long long cnt = 0, sum = 0;
 	bool dup = false, oversize = false;
 
@@ -906,8 +925,8 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
 		uint32_t len = 0;
 		uint64_t src = 0;
 		if (c < c1) {
-			uint32_t a = run[c * stride + p2];
-			len = run[c * stride + p2 + 1] - a;
+			uint32_t a = run_t[(uint64_t)p2 * stride + c];
+			len = run_t[(uint64_t)(p2 + 1) * stride + c] - a;
 			if (threadIdx.x == 0) {
 				a += skip;
 				len -= skip;
@@ -1355,20 +1374,21 @@ static void launch_sortchunk_g(hj_ctx_t *ctx, uint32_t wv, unsigned grid, size_t
 
 template <int WV, bool LP>
 static void launch_join2_t(hj_ctx_t *ctx, unsigned grid, size_t lds, const void *P, const uint64_t *cst,
-                           const uint16_t *run, uint32_t F2, uint32_t b1, const hj_pair *pairs, const uint64_t *tpre,
-                           uint32_t npairs, uint32_t B, uint32_t cpt, uint32_t cap) {
+                           const uint16_t *run_t, uint32_t stride, uint32_t b1, const hj_pair *pairs,
+                           const uint64_t *tpre, uint32_t npairs, uint32_t B, uint32_t cpt, uint32_t cap) {
 	auto kfn = k_join2<WV, 512, LP>;
 	allow_lds(ctx, (const void *)kfn);
-	hipLaunchKernelGGL(kfn, dim3(grid), dim3(512), lds, ctx->stream, (const uint4 *)P, cst, run, F2, b1, pairs, tpre,
-	                   npairs, B, cpt, cap, (hj_stats *)ctx->stats.p);
+	hipLaunchKernelGGL(kfn, dim3(grid), dim3(512), lds, ctx->stream, (const uint4 *)P, cst, run_t, stride, b1, pairs,
+	                   tpre, npairs, B, cpt, cap, (hj_stats *)ctx->stats.p);
 }
 
 static void launch_join2(hj_ctx_t *ctx, uint32_t wv, bool lp, unsigned grid, size_t lds, const void *P,
-                         const uint64_t *cst, const uint16_t *run, uint32_t F2, uint32_t b1, const hj_pair *pairs,
-                         const uint64_t *tpre, uint32_t npairs, uint32_t B, uint32_t cpt, uint32_t cap) {
-#define J(W)                                                                                             \
-	if (lp) launch_join2_t<W, true>(ctx, grid, lds, P, cst, run, F2, b1, pairs, tpre, npairs, B, cpt, cap); \
-	else launch_join2_t<W, false>(ctx, grid, lds, P, cst, run, F2, b1, pairs, tpre, npairs, B, cpt, cap)
+                         const uint64_t *cst, const uint16_t *run_t, uint32_t stride, uint32_t b1,
+                         const hj_pair *pairs, const uint64_t *tpre, uint32_t npairs, uint32_t B, uint32_t cpt,
+                         uint32_t cap) {
+#define J(W)                                                                                                   \
+	if (lp) launch_join2_t<W, true>(ctx, grid, lds, P, cst, run_t, stride, b1, pairs, tpre, npairs, B, cpt, cap); \
+	else launch_join2_t<W, false>(ctx, grid, lds, P, cst, run_t, stride, b1, pairs, tpre, npairs, B, cpt, cap)
 	switch (wv) {
 	case 1: J(1); break;
 	case 2: J(2); break;
@@ -1466,6 +1486,7 @@ int hj_run_radix(hj_ctx_t *ctx, const hj_relation_t *R, const hj_relation_t *S, 
 	const size_t o_runA = take(P == 2 ? chunks1 * (F1 + 1) * 2 : 8), o_cstA = take(P == 2 ? chunks1 * 8 : 8);
 	const size_t o_begt = take(P == 2 ? (hist1_len + 1) * 2 : 8);
 	const size_t o_first = take(P == 2 ? chunks2_max * 4 : 8);
+	const size_t o_runt = take(P == 2 ? chunks2_max * (F2 + 1) * 2 : 8);
 	if ((rc = hj_reserve(ctx, ctx->meta, off))) return rc;
 	char *meta = (char *)ctx->meta.p;
 	hj_seg *segs1 = (hj_seg *)(meta + o_seg1), *segs2 = (hj_seg *)(meta + o_seg2);
@@ -1476,7 +1497,7 @@ int hj_run_radix(hj_ctx_t *ctx, const hj_relation_t *R, const hj_relation_t *S, 
 	uint16_t *runs = (uint16_t *)(meta + o_run), *runsA = (uint16_t *)(meta + o_runA);
 	uint64_t *cst = (uint64_t *)(meta + o_cst), *cstA = (uint64_t *)(meta + o_cstA);
 	uint32_t *first = (uint32_t *)(meta + o_first);
-	uint16_t *beg_t = (uint16_t *)(meta + o_begt);
+	uint16_t *beg_t = (uint16_t *)(meta + o_begt), *run_t = (uint16_t *)(meta + o_runt);
 	uint64_t *hist = (uint64_t *)ctx->hist.p;
 
 	HJ_HIP(ctx, hipEventRecord(ctx->ev[0], ctx->stream));
@@ -1529,6 +1550,8 @@ int hj_run_radix(hj_ctx_t *ctx, const hj_relation_t *R, const hj_relation_t *S, 
 			launch_sortchunk_g(ctx, wv, sgrid, sldsB, segs2, cmap, segs1, hist, beg_t, first, b1, b2, ctx->part[0].p,
 			                   ctx->part[1].p, runs, cst);
 			hj_kt_end(ctx);
+			hipLaunchKernelGGL(k_run_transpose, dim3((unsigned)((chunks2_max + 63) / 64), (F2 + 1 + 63) / 64),
+			                   dim3(256), 0, ctx->stream, runs, st, F2, (uint32_t)chunks2_max, run_t);
 			Rb = Sb = ctx->part[1].p;
 		}
 	}
@@ -1562,7 +1585,7 @@ int hj_run_radix(hj_ctx_t *ctx, const hj_relation_t *R, const hj_relation_t *S, 
 	const unsigned jgrid = (unsigned)std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)ctx->num_cu * per_cu, (uint64_t)Q * 64));
 	hj_kt_begin(ctx, HJ_K_JOIN);
 	if (P == 2) {
-		launch_join2(ctx, wv, lp, jgrid, jlds, Rb, cst, runs, F2, b1, pairs, tiles, Q, B, cpt, cap);
+		launch_join2(ctx, wv, lp, jgrid, jlds, Rb, cst, run_t, (uint32_t)chunks2_max, b1, pairs, tiles, Q, B, cpt, cap);
 	} else {
 		launch_join(ctx, wv, lp, jgrid, jlds, Rb, Sb, pairs, tiles, Q, B, s_tile, cap);
 	}
