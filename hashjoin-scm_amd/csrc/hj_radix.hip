@@ -686,14 +686,27 @@ __global__ __launch_bounds__(256) void k_runs_to_hist(const uint16_t *__restrict
 }
 
 // first[c2] = pass-A tile whose run holds the first tuple of pass-B tile c2
-// (largest i with row[i] <= tile start; row = scanned counts of its digit).
+// (largest i with row[i] <= tile start; row = scanned counts of its digit),
+// and order[] = the pass-B tiles sorted by (segment, tile index k, digit):
+// tiles of neighbouring digits at the same k read the same pass-A tiles, so
+// processing them together lets them share those lines in L2.
 __global__ void k_first_run(const hj_seg *__restrict__ segs1, const hj_seg *__restrict__ segs2,
                             const uint32_t *__restrict__ chunk_seg2, const hj_stats *__restrict__ st, uint32_t b1,
-                            uint32_t tile, const uint64_t *__restrict__ hist, uint32_t *__restrict__ first) {
-	const uint32_t total = (uint32_t)st->total_chunks;
+                            uint32_t tile, const uint64_t *__restrict__ hist, uint32_t *__restrict__ first,
+                            uint32_t *__restrict__ order) {
+	const uint32_t total = (uint32_t)st->total_chunks, F1 = 1u << b1;
 	for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < total; c += gridDim.x * blockDim.x) {
 		const uint32_t s2 = chunk_seg2[c];
 		const hj_seg &g2 = segs2[s2];
+		{
+			const uint32_t seg = s2 >> b1, d1 = s2 & (F1 - 1), k = c - g2.chunk_base;
+			uint32_t rank = seg ? segs2[F1].chunk_base : 0u;  // segment 1 follows all of segment 0
+			for (uint32_t d = 0; d < F1; d++) {
+				const uint32_t n = segs2[seg * F1 + d].nchunks;
+				rank += min(n, k) + (d < d1 && n > k ? 1u : 0u);
+			}
+			order[rank] = c;
+		}
 		const uint64_t ts = g2.out_start + (uint64_t)(c - g2.chunk_base) * tile;
 		const hj_seg &g1 = segs1[s2 >> b1];
 		const uint64_t *row = hist + g1.hist_base + (uint64_t)(s2 & ((1u << b1) - 1)) * g1.nchunks;
@@ -719,8 +732,8 @@ template <int WV, int NT, int ITEMS>
 __global__ __launch_bounds__(NT) void k_sortchunk_g(
     const hj_seg *__restrict__ segs2, const uint32_t *__restrict__ chunk_seg2, const hj_seg *__restrict__ segs1,
     const uint64_t *__restrict__ hist, const uint16_t *__restrict__ beg_t, const uint32_t *__restrict__ first,
-    const hj_stats *__restrict__ st, uint32_t b1, uint32_t bits, const uint4 *__restrict__ in,
-    uint4 *__restrict__ out, uint16_t *__restrict__ run, uint64_t *__restrict__ cstart) {
+    const uint32_t *__restrict__ order, const hj_stats *__restrict__ st, uint32_t b1, uint32_t bits,
+    const uint4 *__restrict__ in, uint4 *__restrict__ out, uint16_t *__restrict__ run, uint64_t *__restrict__ cstart) {
 	constexpr int TILE = NT * ITEMS;
 	extern __shared__ uint4 lds_u4[];
 	const uint32_t F = 1u << bits, fmask = F - 1, F1 = 1u << b1;
@@ -778,8 +791,10 @@ __global__ __launch_bounds__(NT) void k_sortchunk_g(
 		for (int k = 0; k < ITEMS; k++) {
 			const uint32_t f = min((uint32_t)(k * NT) + threadIdx.x, n ? n - 1 : 0u);
 			const uint64_t sp = n ? srcb[f] : 0;
+			// cached loads: a run's neighbours belong to tiles of nearby digits
+			// that the same XCD works on at the same time
 #pragma unroll
-			for (int w = 0; w < WV; w++) v[k][w] = hj_load_stream(base + sp * WV + w);
+			for (int w = 0; w < WV; w++) v[k][w] = hj_load(base + sp * WV + w);
 		}
 	};
 	auto sort_store = [&](uint4 (&v)[ITEMS][WV], uint32_t c) {
@@ -831,19 +846,32 @@ __global__ __launch_bounds__(NT) void k_sortchunk_g(
 		for (uint32_t p = threadIdx.x; p < F; p += NT) tcnt[p] = 0;
 	};
 
+	// Step j of this workgroup takes ordered tile j*grid + xcd*(grid/8) + lane
+	// where workgroups are dispatched round-robin over the 8 XCDs (xcd =
+	// blockIdx % 8): at each step an XCD works on a contiguous run of the
+	// order, i.e. neighbouring digits at the same k, in its own L2.  grid is
+	// a multiple of 8; past the end a step takes an empty tile.
+	const uint32_t lane = blockIdx.x >> 3, xslot = (blockIdx.x & 7) * (grid >> 3);
+	auto tile_at = [&](uint32_t j) -> uint32_t {
+		const uint32_t w = j * grid + xslot + lane;
+		return w < total ? order[w] : total;
+	};
 	uint4 va[ITEMS][WV], vb[ITEMS][WV];
-	stage(blockIdx.x, src0);
+	uint32_t ca = tile_at(0), cb;
+	stage(ca, src0);
 	__syncthreads();
-	load(va, blockIdx.x, src0);
-	for (uint32_t c = blockIdx.x; c < total; c += 2 * grid) {
-		stage(c + grid, src1);
+	load(va, ca, src0);
+	for (uint32_t j = 0; j * grid < total; j += 2) {
+		cb = tile_at(j + 1);
+		stage(cb, src1);
 		__syncthreads();
-		load(vb, c + grid, src1);
-		sort_store(va, c);
-		stage(c + 2 * grid, src0);
+		load(vb, cb, src1);
+		sort_store(va, ca);
+		ca = tile_at(j + 2);
+		stage(ca, src0);
 		__syncthreads();
-		load(va, c + 2 * grid, src0);
-		sort_store(vb, c + grid);  // an empty tile past the end changes nothing
+		load(va, ca, src0);
+		sort_store(vb, cb);  // an empty tile past the end changes nothing
 	}
 }
 
@@ -1342,11 +1370,13 @@ static void launch_sortchunk(hj_ctx_t *ctx, uint32_t wv, unsigned grid, size_t l
 template <int WV, int NT, int ITEMS>
 static void launch_sortchunk_g_t(hj_ctx_t *ctx, unsigned grid, size_t lds, const hj_seg *segs2, const uint32_t *cmap2,
                                  const hj_seg *segs1, const uint64_t *hist, const uint16_t *beg_t, const uint32_t *first,
-                                 uint32_t b1, uint32_t bits, const void *in, void *out, uint16_t *run, uint64_t *cst) {
+                                 const uint32_t *order, uint32_t b1, uint32_t bits, const void *in, void *out,
+                                 uint16_t *run, uint64_t *cst) {
 	auto kfn = k_sortchunk_g<WV, NT, ITEMS>;
 	allow_lds(ctx, (const void *)kfn);
-	grid = std::max(1u, std::min(grid, resident_grid(ctx, (const void *)kfn, NT, lds)));
-	hipLaunchKernelGGL(kfn, dim3(grid), dim3(NT), lds, ctx->stream, segs2, cmap2, segs1, hist, beg_t, first,
+	// all resident workgroups, a multiple of the 8 XCDs (the kernel's mapping)
+	grid = std::max(8u, resident_grid(ctx, (const void *)kfn, NT, lds) & ~7u);
+	hipLaunchKernelGGL(kfn, dim3(grid), dim3(NT), lds, ctx->stream, segs2, cmap2, segs1, hist, beg_t, first, order,
 	                   (const hj_stats *)ctx->stats.p, b1, bits, (const uint4 *)in, (uint4 *)out, run, cst);
 }
 
@@ -1357,11 +1387,12 @@ static size_t gathersort_lds(const part_cfg &c, uint32_t wv, uint32_t F) {
 
 static void launch_sortchunk_g(hj_ctx_t *ctx, uint32_t wv, unsigned grid, size_t lds, const hj_seg *segs2,
                                const uint32_t *cmap2, const hj_seg *segs1, const uint64_t *hist, const uint16_t *beg_t,
-                               const uint32_t *first, uint32_t b1, uint32_t bits, const void *in, void *out,
-                               uint16_t *run, uint64_t *cst) {
+                               const uint32_t *first, const uint32_t *order, uint32_t b1, uint32_t bits, const void *in,
+                               void *out, uint16_t *run, uint64_t *cst) {
 	// shapes must match cfg_for(wv, 0)
 #define G(W, N, I) \
-	launch_sortchunk_g_t<W, N, I>(ctx, grid, lds, segs2, cmap2, segs1, hist, beg_t, first, b1, bits, in, out, run, cst)
+	launch_sortchunk_g_t<W, N, I>(ctx, grid, lds, segs2, cmap2, segs1, hist, beg_t, first, order, b1, bits, in, out, run, \
+	                              cst)
 	switch (wv) {
 	case 1: G(1, 512, 4); break;
 	case 2: G(2, 512, 2); break;
@@ -1487,6 +1518,7 @@ int hj_run_radix(hj_ctx_t *ctx, const hj_relation_t *R, const hj_relation_t *S, 
 	const size_t o_begt = take(P == 2 ? (hist1_len + 1) * 2 : 8);
 	const size_t o_first = take(P == 2 ? chunks2_max * 4 : 8);
 	const size_t o_runt = take(P == 2 ? chunks2_max * (F2 + 1) * 2 : 8);
+	const size_t o_order = take(P == 2 ? chunks2_max * 4 : 8);
 	if ((rc = hj_reserve(ctx, ctx->meta, off))) return rc;
 	char *meta = (char *)ctx->meta.p;
 	hj_seg *segs1 = (hj_seg *)(meta + o_seg1), *segs2 = (hj_seg *)(meta + o_seg2);
@@ -1498,6 +1530,7 @@ int hj_run_radix(hj_ctx_t *ctx, const hj_relation_t *R, const hj_relation_t *S, 
 	uint64_t *cst = (uint64_t *)(meta + o_cst), *cstA = (uint64_t *)(meta + o_cstA);
 	uint32_t *first = (uint32_t *)(meta + o_first);
 	uint16_t *beg_t = (uint16_t *)(meta + o_begt), *run_t = (uint16_t *)(meta + o_runt);
+	uint32_t *order = (uint32_t *)(meta + o_order);
 	uint64_t *hist = (uint64_t *)ctx->hist.p;
 
 	HJ_HIP(ctx, hipEventRecord(ctx->ev[0], ctx->stream));
@@ -1542,13 +1575,13 @@ int hj_run_radix(hj_ctx_t *ctx, const hj_relation_t *R, const hj_relation_t *S, 
 			hipLaunchKernelGGL(k_seg2_fin, dim3(std::min<uint32_t>(nseg2, 4096)), dim3(256), 0, ctx->stream, nch,
 			                   nseg2, F2, segs2, cmap, st);
 			hipLaunchKernelGGL(k_first_run, dim3((unsigned)std::min<uint64_t>(4096, (chunks2_max + 255) / 256)),
-			                   dim3(256), 0, ctx->stream, segs1, segs2, cmap, st, b1, (uint32_t)ch2, hist, first);
+			                   dim3(256), 0, ctx->stream, segs1, segs2, cmap, st, b1, (uint32_t)ch2, hist, first, order);
 			// pass B: each first-level partition's tiles gathered from the
 			// pass-A runs and sorted by the second digit
 			const unsigned sgrid = (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(chunks2_max, (uint64_t)ctx->num_cu * 4));
 			hj_kt_begin(ctx, HJ_K_GATHERSORT);
-			launch_sortchunk_g(ctx, wv, sgrid, sldsB, segs2, cmap, segs1, hist, beg_t, first, b1, b2, ctx->part[0].p,
-			                   ctx->part[1].p, runs, cst);
+			launch_sortchunk_g(ctx, wv, sgrid, sldsB, segs2, cmap, segs1, hist, beg_t, first, order, b1, b2,
+			                   ctx->part[0].p, ctx->part[1].p, runs, cst);
 			hj_kt_end(ctx);
 			hipLaunchKernelGGL(k_run_transpose, dim3((unsigned)((chunks2_max + 63) / 64), (F2 + 1 + 63) / 64),
 			                   dim3(256), 0, ctx->stream, runs, st, F2, (uint32_t)chunks2_max, run_t);
