@@ -159,7 +159,8 @@ int hj_tuned_config(hj_ctx_t *ctx, uint64_t r_num, uint32_t *radix_bits, uint32_
 
 /* ---- serving: continuous batching of arriving joins (no reference
  * counterpart).  A scheduler thread owns `slots` HIP streams, each with an
- * HBM arena of hbm_cap_bytes / slots.  Requests queue in arrival order;
+ * HBM arena of hbm_cap_bytes / slots (half the cap when solo lanes exist;
+ * the other half bounds the lanes).  Requests queue in arrival order;
  * whenever a slot is free, requests from the head of the queue are admitted
  * while their exact scratch footprint fits that slot's arena (at most
  * max_batch per launch), planned as one batch and launched asynchronously,
