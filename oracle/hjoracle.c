@@ -575,7 +575,101 @@ static void lp_join(const uint8_t *R, uint64_t nr, const uint8_t *S, uint64_t ns
 	*sum += s;
 }
 
-static void *radix_thread(void *p) {
+/* sc_join, phj.c:683-781: np2(|R_p|)/2 buckets of PHJ_SC_BUCKET_CAPACITY = 2
+ * tuples (CMakeLists.txt:330-333) with a count and a next pointer; a full
+ * bucket gets an overflow bucket inserted right after it (or the existing
+ * one is filled); the probe compares every tuple of the chain.  Departure: at
+ * least one bucket (for |R_p| = 1 the reference's mask underflows). */
+typedef struct or_sc_bkt {
+	uint32_t count;
+	struct or_sc_bkt *next;
+	uint64_t key[2], rid[2];
+} or_sc_bkt_t;
+
+static void sc_join(const uint8_t *R, uint64_t nr, const uint8_t *S, uint64_t ns, uint32_t W, uint32_t B,
+                    uint64_t *cnt, uint64_t *sum) {
+	uint64_t nbk = np2(nr) / 2;
+	if (nbk == 0) nbk = 1;
+	const uint64_t mask = nbk - 1;
+	or_sc_bkt_t *bk = (or_sc_bkt_t *)calloc(nbk * 2, sizeof(or_sc_bkt_t));
+	or_sc_bkt_t *pool = bk + nbk;
+	uint64_t free_idx = 0;
+	for (uint64_t i = 0; i < nr; i++) {
+		or_sc_bkt_t *curr = bk + ((key_at(R, i, W) >> B) & mask), *nxt = curr->next, *dest;
+		if (curr->count == 2) {
+			if (!nxt || nxt->count == 2) {
+				dest = pool + free_idx++;
+				curr->next = dest;
+				dest->next = nxt;
+			} else {
+				dest = nxt;
+			}
+		} else {
+			dest = curr;
+		}
+		dest->key[dest->count] = key_at(R, i, W);
+		dest->rid[dest->count] = rid_at(R, i, W);
+		dest->count++;
+	}
+	uint64_t c = 0, s = 0;
+	for (uint64_t j = 0; j < ns; j++) {
+		const uint64_t sk = key_at(S, j, W), sr = rid_at(S, j, W);
+		for (const or_sc_bkt_t *t = bk + ((sk >> B) & mask); t; t = t->next)
+			for (uint32_t k = 0; k < t->count; k++) {
+				const uint64_t f = t->key[k] == sk;
+				c += f;
+				s += (t->rid[k] + sr) * f;
+			}
+	}
+	free(bk);
+	*cnt += c;
+	*sum += s;
+}
+
+/* hr_join (phj.c:784-877, shrink = 0) and hro_join (phj.c:881-1037, shrink =
+ * log2(64 / W), at least 2^shrink buckets): count R_p per bucket, exclusive
+ * prefix, scatter R_p into bucket order (the running counters end as the
+ * bucket ends), probe the bucket's range.  hro reads the range in blocks of
+ * 64 / W tuples as its 512-bit compare does; tuples past the range belong to
+ * other buckets and never match.  Departure: the zero padding past |R_p| is
+ * not compared (in the reference a probe key 0 can match it). */
+static void hr_join(const uint8_t *R, uint64_t nr, const uint8_t *S, uint64_t ns, uint32_t W, uint32_t B,
+                    int shrink, uint64_t *cnt, uint64_t *sum) {
+	uint64_t nb = np2(nr) >> shrink;
+	if (nb < (1ull << shrink)) nb = 1ull << shrink;
+	const uint64_t mask = nb - 1;
+	uint64_t *hist = (uint64_t *)calloc(nb, 8);
+	uint8_t *tmp = (uint8_t *)malloc(nr * W + 1);
+	for (uint64_t i = 0; i < nr; i++) hist[(key_at(R, i, W) >> B) & mask]++;
+	uint64_t acc = 0;
+	for (uint64_t h = 0; h < nb; h++) {
+		acc += hist[h];
+		hist[h] = acc - hist[h];
+	}
+	for (uint64_t i = 0; i < nr; i++) {
+		const uint64_t h = (key_at(R, i, W) >> B) & mask;
+		memcpy(tmp + hist[h] * W, R + i * W, W);
+		hist[h]++;
+	}
+	const uint64_t blk = shrink ? 64 / W : 1;
+	uint64_t c = 0, s = 0;
+	for (uint64_t j = 0; j < ns; j++) {
+		const uint64_t sk = key_at(S, j, W), sr = rid_at(S, j, W), h = (sk >> B) & mask;
+		const uint64_t st = h ? hist[h - 1] : 0, en = hist[h];
+		for (uint64_t t0 = st; t0 < en; t0 += blk)
+			for (uint64_t t = t0; t < t0 + blk && t < nr; t++) {
+				const uint64_t f = key_at(tmp, t, W) == sk;
+				c += f;
+				s += (rid_at(tmp, t, W) + sr) * f;
+			}
+	}
+	free(hist);
+	free(tmp);
+	*cnt += c;
+	*sum += s;
+}
+
+static void *radix_thread(void *p) {static void *radix_thread(void *p) {
 	radix_arg_t *a = (radix_arg_t *)p;
 	radix_shared_t *sh = a->sh;
 	const uint32_t W = sh->W;
@@ -638,10 +732,15 @@ static void *radix_thread(void *p) {
 			scratch = (uint64_t *)malloc(need);
 			scap = need;
 		}
-		if (sh->lp)
-			lp_join(RB + t.r0 * W, nr, SB + t.s0 * W, t.s1 - t.s0, W, sh->B, scratch, &a->cnt, &a->sum);
-		else
-			bc_join(RB + t.r0 * W, nr, SB + t.s0 * W, t.s1 - t.s0, W, sh->B, scratch, &a->cnt, &a->sum);
+		const uint8_t *RP = RB + t.r0 * W, *SP = SB + t.s0 * W;
+		const uint64_t ns = t.s1 - t.s0;
+		switch (sh->lp) {  /* join kind: 0 bc, 1 lp, 2 sc, 3 hr, 4 hro */
+		case 1: lp_join(RP, nr, SP, ns, W, sh->B, scratch, &a->cnt, &a->sum); break;
+		case 2: sc_join(RP, nr, SP, ns, W, sh->B, &a->cnt, &a->sum); break;
+		case 3: hr_join(RP, nr, SP, ns, W, sh->B, 0, &a->cnt, &a->sum); break;
+		case 4: hr_join(RP, nr, SP, ns, W, sh->B, W == 16 ? 2 : (W == 32 ? 1 : 0), &a->cnt, &a->sum); break;
+		default: bc_join(RP, nr, SP, ns, W, sh->B, scratch, &a->cnt, &a->sum); break;
+		}
 	}
 	free(scratch);
 	pthread_barrier_wait(sh->bar);
