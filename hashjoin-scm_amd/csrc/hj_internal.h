@@ -112,8 +112,10 @@ int hj_run_nphj_lp(hj_ctx_t *ctx, const hj_relation_t *R, const hj_relation_t *S
                    const hj_params_t *prm, hj_result_t *out);
 int hj_run_nphj_sc(hj_ctx_t *ctx, const hj_relation_t *R, const hj_relation_t *S,
                    const hj_params_t *prm, hj_result_t *out);
+// tk = per-partition table kind (HJ_TK_* of hj_tables.h: 0 bc, 1 lp, 2 sc,
+// 3 hr, 4 hro)
 int hj_run_radix(hj_ctx_t *ctx, const hj_relation_t *R, const hj_relation_t *S,
-                 const hj_params_t *prm, bool lp, hj_result_t *out);
+                 const hj_params_t *prm, int tk, hj_result_t *out);
 // radix configuration for |R|: the tuned class if the context has one, else
 // the default rule
 void hj_radix_config(hj_ctx_t *ctx, uint64_t nr, uint32_t *bits, uint32_t *passes);

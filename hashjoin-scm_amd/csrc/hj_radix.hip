@@ -34,6 +34,7 @@
 
 #include "hj_device.h"
 #include "hj_internal.h"
+#include "hj_tables.h"
 
 #include <algorithm>
 #include <string.h>
@@ -411,19 +412,20 @@ __global__ void k_plan(const uint64_t *__restrict__ pb, uint32_t passes, uint32_
 // ------------------------------------------------------------ join
 // LDS (bc): keys u64[cap] | pays u64[cap] | heads u32[np2(cap)] | next u16[cap]
 // LDS (lp): keys u64[cap] | pays u64[cap] | occ   u32[np2(cap)]
-template <int WV, int NT, bool LP>
+template <int WV, int NT, int TK>
 __global__ __launch_bounds__(NT) void k_join(const uint4 *__restrict__ Rb, const uint4 *__restrict__ Sb,
                                              const hj_pair *__restrict__ pairs,
                                              const uint64_t *__restrict__ tprefix, uint32_t npairs, uint32_t B,
                                              uint64_t s_tile, uint32_t cap, hj_stats *st) {
+	constexpr bool LP = TK == HJ_TK_LP;
+	constexpr bool TWO = TK == HJ_TK_HR || TK == HJ_TK_HRO;  // count + place builds
 	extern __shared__ uint64_t lds_u64[];
 	__shared__ unsigned long long red[2 * (NT / HJ_WAVE)];
+	__shared__ uint32_t sred[NT / HJ_WAVE + 1];
 	__shared__ uint64_t s_item[4];
-	uint64_t *keys = lds_u64;
-	uint64_t *pays = keys + cap;
-	uint32_t *heads = (uint32_t *)(pays + cap);
-	const uint32_t nbmax = 1u << (32 - __clz(cap - 1));
-	uint16_t *next = (uint16_t *)(heads + nbmax);
+	__shared__ uint32_t s_pool;
+	hj_tab<TK, WV> tab;
+	tab.map(lds_u64, cap, &s_pool);
 	const uint64_t total = tprefix[npairs];
 	unsigned long long cnt = 0, sum = 0;
 	bool dup = false, oversize = false;
@@ -454,32 +456,26 @@ __global__ __launch_bounds__(NT) void k_join(const uint4 *__restrict__ Rb, const
 		}
 		for (uint64_t rc = pr.r0; rc < pr.r1; rc += cap) {
 			const uint32_t m = (uint32_t)min((uint64_t)cap, pr.r1 - rc);
-			const uint32_t nb = m <= 1 ? 1u : 1u << (32 - __clz(m - 1));
-			const uint32_t hmask = nb - 1;
-			for (uint32_t i = threadIdx.x; i < nb; i += NT) heads[i] = 0;
+			tab.reset(m, NT);
 			__syncthreads();
-			for (uint32_t i = threadIdx.x; i < m; i += NT) {
-				const uint4 t = Rb[(rc + i) * WV];
-				const uint64_t key = hj_key_of(t);
-				keys[i] = key;
-				pays[i] = hj_pay_of(t);
-				if (!LP) next[i] = (uint16_t)atomicExch(&heads[(uint32_t)(key >> B) & hmask], i + 1);
+			if (TWO) {
+				for (uint32_t i = threadIdx.x; i < m; i += NT) tab.count(hj_key_of(hj_load(&Rb[(rc + i) * WV])), B);
+				__syncthreads();
+				tab.starts(sred, NT);
+				__syncthreads();
+				for (uint32_t i = threadIdx.x; i < m; i += NT) {
+					const uint4 t = hj_load(&Rb[(rc + i) * WV]);
+					tab.place(hj_key_of(t), hj_pay_of(t), B);
+				}
+			} else {
+				for (uint32_t i = threadIdx.x; i < m; i += NT) {
+					const uint4 t = hj_load(&Rb[(rc + i) * WV]);
+					tab.put(i, hj_key_of(t), hj_pay_of(t), B);
+				}
 			}
 			__syncthreads();
 			if (LP) {
-				for (uint32_t i = threadIdx.x; i < m; i += NT) {
-					const uint64_t key = keys[i];
-					uint32_t pos = (uint32_t)(key >> B) & hmask;
-					for (uint32_t step = 0; step < nb; step++) {
-						const uint32_t old = atomicCAS(&heads[pos], 0u, i + 1);
-						if (old == 0) break;
-						if (keys[old - 1] == key) {
-							dup = true;
-							break;
-						}
-						pos = (pos + 1) & hmask;
-					}
-				}
+				tab.lp_insert(m, B, NT, dup);
 				__syncthreads();
 			}
 			constexpr int U = 4;
@@ -494,31 +490,7 @@ __global__ __launch_bounds__(NT) void k_join(const uint4 *__restrict__ Rb, const
 					sk[u] = hj_key_of(t);
 					sp[u] = hj_pay_of(t);
 				}
-#pragma unroll
-				for (int u = 0; u < U; u++) {
-					if (!live[u]) continue;
-					const uint64_t key = sk[u];
-					uint32_t h = (uint32_t)(key >> B) & hmask;
-					if (!LP) {
-						for (uint32_t hit = heads[h]; hit; hit = next[hit - 1]) {
-							if (keys[hit - 1] == key) {
-								cnt++;
-								sum += pays[hit - 1] + sp[u];
-							}
-						}
-					} else {
-						for (uint32_t step = 0; step < nb; step++) {
-							const uint32_t o = heads[h];
-							if (!o) break;
-							if (keys[o - 1] == key) {
-								cnt++;
-								sum += pays[o - 1] + sp[u];
-								break;
-							}
-							h = (h + 1) & hmask;
-						}
-					}
-				}
+				tab.template probe<U>(sk, sp, live, B, cnt, sum);
 			}
 			__syncthreads();
 		}
@@ -919,12 +891,14 @@ __global__ __launch_bounds__(256) void k_run_transpose(const uint16_t *__restric
 // fill, and the host recomputes that join exactly).  Probe side: each wave
 // loads the next RU tuples per lane while it probes the current ones.
 // LDS: keys u64[cap] | pays u64[cap] | heads u32[np2(cap)] | next u16[cap]
-template <int WV, int NT, bool LP>
+template <int WV, int NT, int TK>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_join2(const uint4 *__restrict__ P, const uint64_t *__restrict__ cstart,
                                               const uint16_t *__restrict__ run_t, uint32_t stride, uint32_t b1,
                                               const hj_pair *__restrict__ pairs,
                                               const uint64_t *__restrict__ tprefix, uint32_t npairs, uint32_t B,
                                               uint32_t cpt, uint32_t cap, hj_stats *st) {
+	constexpr bool LP = TK == HJ_TK_LP;
+	constexpr bool TWO = TK == HJ_TK_HR || TK == HJ_TK_HRO;  // count + place builds
 	constexpr uint32_t NW = NT / HJ_WAVE;
 	constexpr uint32_t RU = 6;                  // tuples per lane per step
 	constexpr uint32_t STEP = HJ_WAVE * RU;     // tuples per wave per step
@@ -934,12 +908,9 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
 	__shared__ uint64_t s_item[4];
 	__shared__ uint64_t r_src[NT];      // absolute position of a staged run's first tuple
 	__shared__ uint32_t r_off[NT + 1];  // flat offset of a staged run; r_off[NT] = total
-	uint64_t *keys = lds_u64;
-	uint64_t *pays = keys + cap;
-	uint32_t *heads = (uint32_t *)(pays + cap);
-	const uint32_t nb = 1u << (32 - __clz(cap - 1));
-	const uint32_t hmask = nb - 1;
-	uint16_t *next = (uint16_t *)(heads + nb);
+	__shared__ uint32_t s_pool;
+	hj_tab<TK, WV> tab;
+	tab.map(lds_u64, cap, &s_pool);
 	const uint32_t lane = threadIdx.x & (HJ_WAVE - 1), wid = threadIdx.x / HJ_WAVE;
 	const uint64_t total = tprefix[npairs];
 	unsigned long long cnt = 0, sum = 0;
@@ -990,55 +961,15 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
 		}
 	};
 	auto probe_step = [&](uint32_t f0, uint32_t fe, const uint4 (&t)[RU]) {
-		uint64_t kk[RU];
-		uint32_t h[RU];
+		uint64_t kk[RU], pp[RU];
 		bool live[RU];
 #pragma unroll
 		for (uint32_t u = 0; u < RU; u++) {
 			live[u] = f0 + u * HJ_WAVE + lane < fe;
 			kk[u] = hj_key_of(t[u]);
-			h[u] = (uint32_t)(kk[u] >> B) & hmask;
+			pp[u] = hj_pay_of(t[u]);
 		}
-		if (!LP) {
-			// h becomes the chain cursor (1-based entry, 0 = end)
-#pragma unroll
-			for (uint32_t u = 0; u < RU; u++) h[u] = live[u] ? heads[h[u]] : 0;
-			bool any = true;
-			while (any) {
-				any = false;
-#pragma unroll
-				for (uint32_t u = 0; u < RU; u++) {
-					if (!h[u]) continue;
-					const uint32_t i = h[u] - 1;
-					if (keys[i] == kk[u]) {
-						cnt++;
-						sum += pays[i] + hj_pay_of(t[u]);
-					}
-					h[u] = next[i];
-					any |= h[u] != 0;
-				}
-			}
-		} else {
-			for (uint32_t step = 0; step < nb; step++) {
-				bool any = false;
-#pragma unroll
-				for (uint32_t u = 0; u < RU; u++) {
-					if (!live[u]) continue;
-					const uint32_t o = heads[h[u]];
-					if (!o) {
-						live[u] = false;
-					} else if (keys[o - 1] == kk[u]) {
-						cnt++;
-						sum += pays[o - 1] + hj_pay_of(t[u]);
-						live[u] = false;
-					} else {
-						h[u] = (h[u] + 1) & hmask;
-					}
-					any |= live[u];
-				}
-				if (!any) break;
-			}
-		}
+		tab.template probe<(int)RU>(kk, pp, live, B, cnt, sum);
 	};
 
 	for (;;) {
@@ -1064,19 +995,19 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
 		const uint64_t sb = pr.s0 + tix * cpt, se = min(pr.s1, sb + cpt);
 		uint64_t rc = pr.r0;
 		uint32_t roff = 0;  // tuples of run rc already taken by an earlier fill
-		while (rc < pr.r1) {
-			// ---- fill the table with up to `cap` build tuples
-			for (uint32_t i = threadIdx.x; i < nb; i += NT) heads[i] = 0;
-			uint32_t m = 0;
+		// Gather the next fill of up to `cap` build tuples from (rc, roff) on,
+		// advancing (rc, roff, m).  mode 0 inserts (bc, sc, lp), 1 counts and
+		// 2 places (hr, hro: two gathers over the same runs).
+		auto fill = [&](int mode, uint64_t &frc, uint32_t &froff, uint32_t &m) {
 			for (;;) {
-				const uint64_t c1 = min(pr.r1, rc + NT);
-				const uint32_t n = (uint32_t)(c1 - rc);
-				const uint32_t tot = stage(rc, c1, p2, roff);
+				const uint64_t c1 = min(pr.r1, frc + NT);
+				const uint32_t n = (uint32_t)(c1 - frc);
+				const uint32_t tot = stage(frc, c1, p2, froff);
 				__syncthreads();
 				const uint32_t T = min(tot, cap - m);
 				// runs taken whole form a prefix; the next one may be taken in part
 				const uint32_t nfull = __syncthreads_count(threadIdx.x < n && r_off[threadIdx.x + 1] <= T);
-				const uint32_t roff_new = nfull < n ? (nfull == 0 ? roff : 0) + (T - r_off[nfull]) : 0;
+				const uint32_t roff_new = nfull < n ? (nfull == 0 ? froff : 0) + (T - r_off[nfull]) : 0;
 				const uint32_t fb = (uint32_t)((uint64_t)T * wid / NW), fe = (uint32_t)((uint64_t)T * (wid + 1) / NW);
 				if (fb < fe) {
 					uint32_t k = find_run(fb);
@@ -1087,38 +1018,39 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
 						for (uint32_t u = 0; u < RU; u++) {
 							const uint32_t f = f0 + u * HJ_WAVE + lane;
 							if (f >= fe) continue;
-							const uint32_t i = m + f;
 							const uint64_t key = hj_key_of(t[u]);
-							keys[i] = key;
-							pays[i] = hj_pay_of(t[u]);
-							if (!LP) next[i] = (uint16_t)atomicExch(&heads[(uint32_t)(key >> B) & hmask], i + 1);
+							if (mode == 1) tab.count(key, B);
+							else if (mode == 2) tab.place(key, hj_pay_of(t[u]), B);
+							else tab.put(m + f, key, hj_pay_of(t[u]), B);
 						}
 					}
 				}
 				m += T;
-				rc += nfull;
-				roff = roff_new;
+				frc += nfull;
+				froff = roff_new;
 				__syncthreads();
-				if (m >= cap || rc >= pr.r1) break;
+				if (m >= cap || frc >= pr.r1) break;
 			}
+		};
+		while (rc < pr.r1) {
+			// ---- fill the table with up to `cap` build tuples
+			tab.reset(cap, NT);
+			__syncthreads();
+			uint32_t m = 0;
+			if (TWO) {
+				uint64_t rc0 = rc;
+				uint32_t roff0 = roff, m0 = 0;
+				fill(1, rc0, roff0, m0);
+				tab.starts(sred, NT);
+				__syncthreads();
+			}
+			fill(TWO ? 2 : 0, rc, roff, m);
 			if (LP) {
 				if (rc < pr.r1) {
 					oversize = true;  // the host recomputes this join exactly
 					break;
 				}
-				for (uint32_t i = threadIdx.x; i < m; i += NT) {
-					const uint64_t key = keys[i];
-					uint32_t p = (uint32_t)(key >> B) & hmask;
-					for (uint32_t step = 0; step < nb; step++) {
-						const uint32_t old = atomicCAS(&heads[p], 0u, i + 1);
-						if (old == 0) break;
-						if (keys[old - 1] == key) {
-							dup = true;
-							break;
-						}
-						p = (p + 1) & hmask;
-					}
-				}
+				tab.lp_insert(m, B, NT, dup);
 				__syncthreads();
 			}
 			// ---- probe the tile's runs, NT chunks at a time
@@ -1293,11 +1225,23 @@ static int launch_pass(hj_ctx_t *ctx, uint32_t wv, uint32_t variant, unsigned gr
 	return HJ_OK;
 }
 
-static size_t join_static_lds(uint32_t wv, bool lp, bool two_pass) {
-	// every instance of one kernel template has the same __shared__ arrays
-	(void)wv;
-	if (two_pass) return lp ? static_lds((const void *)k_join2<1, 512, true>) : static_lds((const void *)k_join2<1, 512, false>);
-	return lp ? static_lds((const void *)k_join<1, 256, true>) : static_lds((const void *)k_join<1, 256, false>);
+// instances of one kernel template for every tuple width and table kind
+#define HJ_TK_SWITCH(tk, X)              \
+	switch (tk) {                        \
+	case HJ_TK_LP: X(HJ_TK_LP); break;   \
+	case HJ_TK_SC: X(HJ_TK_SC); break;   \
+	case HJ_TK_HR: X(HJ_TK_HR); break;   \
+	case HJ_TK_HRO: X(HJ_TK_HRO); break; \
+	default: X(HJ_TK_BC); break;         \
+	}
+
+static size_t join_static_lds(int tk, bool two_pass) {
+	// every width of one kernel template has the same __shared__ arrays
+	size_t r = 16384;
+#define S(K) r = two_pass ? static_lds((const void *)k_join2<1, 512, K>) : static_lds((const void *)k_join<1, 256, K>)
+	HJ_TK_SWITCH(tk, S)
+#undef S
+	return r;
 }
 
 // Dynamic-LDS attribute of the default-shape scatter kernel for width wv.
@@ -1313,34 +1257,40 @@ static void set_scatter_attr(hj_ctx_t *ctx, uint32_t wv) {
 	allow_lds(ctx, k);
 }
 
-// LDS bytes of a join table holding `cap` build tuples.
-static size_t join_lds(uint32_t cap, bool lp) {
-	const size_t nb = hj_next_pow2(cap);
-	return (size_t)cap * 16 + nb * 4 + (lp ? 0 : (size_t)cap * 2 + 8);
-}
+// LDS bytes of a bc or lp join table holding `cap` build tuples (batches).
+static size_t join_lds(uint32_t cap, bool lp) { return hj_tab_lds(lp ? HJ_TK_LP : HJ_TK_BC, cap, 1); }
 
-template <int WV, bool LP>
+template <int WV, int TK>
 static void launch_join_t(hj_ctx_t *ctx, unsigned grid, size_t lds, const void *Rb, const void *Sb, const hj_pair *pairs,
                           const uint64_t *tpre, uint32_t npairs, uint32_t B, uint64_t s_tile, uint32_t cap) {
-	auto kfn = k_join<WV, 256, LP>;
+	auto kfn = k_join<WV, 256, TK>;
 	allow_lds(ctx, (const void *)kfn);
 	hipLaunchKernelGGL(kfn, dim3(grid), dim3(256), lds, ctx->stream, (const uint4 *)Rb, (const uint4 *)Sb, pairs,
 	                   tpre, npairs, B, s_tile, cap, (hj_stats *)ctx->stats.p);
 }
 
-static void launch_join(hj_ctx_t *ctx, uint32_t wv, bool lp, unsigned grid, size_t lds, const void *Rb,
+// hro joins blocks of tuples of at most 64 bytes (phj.c:880), so it has no
+// instances for wider tuples
+template <int W, int TK>
+struct hj_tk_ok {
+	static constexpr bool value = TK != HJ_TK_HRO || W <= 4;
+};
+
+static void launch_join(hj_ctx_t *ctx, uint32_t wv, int tk, unsigned grid, size_t lds, const void *Rb,
                         const void *Sb, const hj_pair *pairs, const uint64_t *tpre, uint32_t npairs, uint32_t B,
                         uint64_t s_tile, uint32_t cap) {
-#define J(W)                                                                                      \
-	if (lp) launch_join_t<W, true>(ctx, grid, lds, Rb, Sb, pairs, tpre, npairs, B, s_tile, cap); \
-	else launch_join_t<W, false>(ctx, grid, lds, Rb, Sb, pairs, tpre, npairs, B, s_tile, cap)
-	switch (wv) {
-	case 1: J(1); break;
-	case 2: J(2); break;
-	case 4: J(4); break;
-	case 8: J(8); break;
-	default: J(16); break;
+#define J(W, K) \
+	if constexpr (hj_tk_ok<W, K>::value) launch_join_t<W, K>(ctx, grid, lds, Rb, Sb, pairs, tpre, npairs, B, s_tile, cap)
+#define JW(K)                    \
+	switch (wv) {                \
+	case 1: J(1, K); break;      \
+	case 2: J(2, K); break;      \
+	case 4: J(4, K); break;      \
+	case 8: J(8, K); break;      \
+	default: J(16, K); break;    \
 	}
+	HJ_TK_SWITCH(tk, JW)
+#undef JW
 #undef J
 }
 
@@ -1403,30 +1353,33 @@ static void launch_sortchunk_g(hj_ctx_t *ctx, uint32_t wv, unsigned grid, size_t
 #undef G
 }
 
-template <int WV, bool LP>
+template <int WV, int TK>
 static void launch_join2_t(hj_ctx_t *ctx, unsigned grid, size_t lds, const void *P, const uint64_t *cst,
                            const uint16_t *run_t, uint32_t stride, uint32_t b1, const hj_pair *pairs,
                            const uint64_t *tpre, uint32_t npairs, uint32_t B, uint32_t cpt, uint32_t cap) {
-	auto kfn = k_join2<WV, 512, LP>;
+	auto kfn = k_join2<WV, 512, TK>;
 	allow_lds(ctx, (const void *)kfn);
 	hipLaunchKernelGGL(kfn, dim3(grid), dim3(512), lds, ctx->stream, (const uint4 *)P, cst, run_t, stride, b1, pairs,
 	                   tpre, npairs, B, cpt, cap, (hj_stats *)ctx->stats.p);
 }
 
-static void launch_join2(hj_ctx_t *ctx, uint32_t wv, bool lp, unsigned grid, size_t lds, const void *P,
+static void launch_join2(hj_ctx_t *ctx, uint32_t wv, int tk, unsigned grid, size_t lds, const void *P,
                          const uint64_t *cst, const uint16_t *run_t, uint32_t stride, uint32_t b1,
                          const hj_pair *pairs, const uint64_t *tpre, uint32_t npairs, uint32_t B, uint32_t cpt,
                          uint32_t cap) {
-#define J(W)                                                                                                   \
-	if (lp) launch_join2_t<W, true>(ctx, grid, lds, P, cst, run_t, stride, b1, pairs, tpre, npairs, B, cpt, cap); \
-	else launch_join2_t<W, false>(ctx, grid, lds, P, cst, run_t, stride, b1, pairs, tpre, npairs, B, cpt, cap)
-	switch (wv) {
-	case 1: J(1); break;
-	case 2: J(2); break;
-	case 4: J(4); break;
-	case 8: J(8); break;
-	default: J(16); break;
+#define J(W, K)                                                                                                    \
+	if constexpr (hj_tk_ok<W, K>::value)                                                                           \
+	launch_join2_t<W, K>(ctx, grid, lds, P, cst, run_t, stride, b1, pairs, tpre, npairs, B, cpt, cap)
+#define JW(K)                    \
+	switch (wv) {                \
+	case 1: J(1, K); break;      \
+	case 2: J(2, K); break;      \
+	case 4: J(4, K); break;      \
+	case 8: J(8, K); break;      \
+	default: J(16, K); break;    \
 	}
+	HJ_TK_SWITCH(tk, JW)
+#undef JW
 #undef J
 }
 
@@ -1450,8 +1403,11 @@ void hj_radix_config(hj_ctx_t *ctx, uint64_t nr, uint32_t *bits, uint32_t *passe
 	hj_radix_default(nr, bits, passes);
 }
 
-int hj_run_radix(hj_ctx_t *ctx, const hj_relation_t *R, const hj_relation_t *S, const hj_params_t *prm, bool lp,
+int hj_run_radix(hj_ctx_t *ctx, const hj_relation_t *R, const hj_relation_t *S, const hj_params_t *prm, int tk,
                  hj_result_t *out) {
+	const bool lp = tk == HJ_TK_LP;
+	if (tk == HJ_TK_HRO && R->tuple_size > 64)
+		return hj_fail(ctx, HJ_EINVAL, "hro joins tuples of at most 64 bytes (one 512-bit block holds a tuple)");
 	const uint32_t W = R->tuple_size, wv = W / 16;
 	const uint64_t nr = R->tuple_num, ns = S->tuple_num, N = nr + ns;
 	uint32_t B = prm->radix_bits, P = prm->passes;
@@ -1595,10 +1551,10 @@ int hj_run_radix(hj_ctx_t *ctx, const hj_relation_t *R, const hj_relation_t *S, 
 	const uint64_t avg_s = (ns >> B) + 1;
 	uint64_t s_tile = std::max<uint64_t>(8192, 2 * avg_s);
 	const uint64_t avg_r = (nr + Q - 1) / Q;
-	const size_t lds_max = std::min<size_t>(ctx->prop.sharedMemPerBlock, 160 * 1024) - join_static_lds(wv, lp, P == 2) - 256;
+	const size_t lds_max = std::min<size_t>(ctx->prop.sharedMemPerBlock, 160 * 1024) - join_static_lds(tk, P == 2) - 256;
 	uint32_t cap = (uint32_t)std::min<uint64_t>(65535, std::max<uint64_t>(256, avg_r + avg_r / 4 + 64));
-	while (cap > 256u && join_lds(cap, lp) > lds_max) cap -= 256;
-	const size_t jlds = join_lds(cap, lp);
+	while (cap > 256u && hj_tab_lds(tk, cap, wv) > lds_max) cap -= 256;
+	const size_t jlds = hj_tab_lds(tk, cap, wv);
 	if (jlds > lds_max) return hj_fail(ctx, HJ_EINVAL, "join table of %u tuples does not fit LDS", cap);
 	hj_kt_begin(ctx, HJ_K_PLAN);
 	uint32_t cpt = 0;
@@ -1613,14 +1569,14 @@ int hj_run_radix(hj_ctx_t *ctx, const hj_relation_t *R, const hj_relation_t *S, 
 	rc = scan_u64(ctx, tiles, Q, (uint64_t *)ctx->scan_tmp.p);
 	hj_kt_end(ctx);
 	if (rc) return rc;
-	const size_t jall = jlds + join_static_lds(wv, lp, P == 2);
+	const size_t jall = jlds + join_static_lds(tk, P == 2);
 	const int per_cu = (int)std::max<size_t>(1, std::min<size_t>(4, (size_t)(160 * 1024) / jall));
 	const unsigned jgrid = (unsigned)std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)ctx->num_cu * per_cu, (uint64_t)Q * 64));
 	hj_kt_begin(ctx, HJ_K_JOIN);
 	if (P == 2) {
-		launch_join2(ctx, wv, lp, jgrid, jlds, Rb, cst, run_t, (uint32_t)chunks2_max, b1, pairs, tiles, Q, B, cpt, cap);
+		launch_join2(ctx, wv, tk, jgrid, jlds, Rb, cst, run_t, (uint32_t)chunks2_max, b1, pairs, tiles, Q, B, cpt, cap);
 	} else {
-		launch_join(ctx, wv, lp, jgrid, jlds, Rb, Sb, pairs, tiles, Q, B, s_tile, cap);
+		launch_join(ctx, wv, tk, jgrid, jlds, Rb, Sb, pairs, tiles, Q, B, s_tile, cap);
 	}
 	hj_kt_end(ctx);
 	HJ_HIP(ctx, hipGetLastError());

@@ -145,14 +145,18 @@ int hj_ctx_destroy(hj_ctx_t *ctx) {
 typedef int (*hj_algo_fn)(hj_ctx_t *, const hj_relation_t *, const hj_relation_t *, const hj_params_t *,
                           hj_result_t *);
 
-static int run_radix_bc(hj_ctx_t *c, const hj_relation_t *R, const hj_relation_t *S, const hj_params_t *p,
-                        hj_result_t *o) {
-	return hj_run_radix(c, R, S, p, false, o);
-}
-static int run_radix_lp(hj_ctx_t *c, const hj_relation_t *R, const hj_relation_t *S, const hj_params_t *p,
-                        hj_result_t *o) {
-	return hj_run_radix(c, R, S, p, true, o);
-}
+// radix joins by per-partition table (src/algo/phj.c:2515-2535)
+#define HJ_RADIX_FN(NAME, TK)                                                                           \
+	static int NAME(hj_ctx_t *c, const hj_relation_t *R, const hj_relation_t *S, const hj_params_t *p, \
+	                hj_result_t *o) {                                                                   \
+		return hj_run_radix(c, R, S, p, TK, o);                                                         \
+	}
+HJ_RADIX_FN(run_radix_bc, 0)
+HJ_RADIX_FN(run_radix_lp, 1)
+HJ_RADIX_FN(run_radix_sc, 2)
+HJ_RADIX_FN(run_radix_hr, 3)
+HJ_RADIX_FN(run_radix_hro, 4)
+#undef HJ_RADIX_FN
 
 struct hj_algo_entry {
 	const char *name;
@@ -164,8 +168,14 @@ static const hj_algo_entry k_algos[] = {
     {"nphj_lp", hj_run_nphj_lp},
     {"phj_radix_bc", run_radix_bc},
     {"phj_radix_lp", run_radix_lp},
+    {"phj_radix_sc", run_radix_sc},
+    {"phj_radix_hr", run_radix_hr},
+    {"phj_radix_hro", run_radix_hro},
     {"phj_radix_bw_reg_bc", run_radix_bc},
     {"phj_radix_bw_reg_lp", run_radix_lp},
+    {"phj_radix_bw_reg_sc", run_radix_sc},
+    {"phj_radix_bw_reg_hr", run_radix_hr},
+    {"phj_radix_bw_reg_hro", run_radix_hro},
     {nullptr, nullptr},
 };
 

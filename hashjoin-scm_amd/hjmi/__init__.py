@@ -30,7 +30,9 @@ WL_PKFK_UNIFORM, WL_PKFK_REF, WL_PAYLOAD_REF, WL_ZIPF, WL_RANDOM = 0, 1, 2, 3, 4
 WORKLOADS = {"pkfk": WL_PKFK_UNIFORM, "pkfk_ref": WL_PKFK_REF, "payload_ref": WL_PAYLOAD_REF,
              "zipf": WL_ZIPF, "random": WL_RANDOM}
 
-ALGOS = ("nphj_sc", "nphj_lp", "phj_radix_bc", "phj_radix_lp", "phj_radix_bw_reg_bc", "phj_radix_bw_reg_lp")
+ALGOS = ("nphj_sc", "nphj_lp", "phj_radix_bc", "phj_radix_lp", "phj_radix_sc", "phj_radix_hr", "phj_radix_hro",
+         "phj_radix_bw_reg_bc", "phj_radix_bw_reg_lp", "phj_radix_bw_reg_sc", "phj_radix_bw_reg_hr",
+         "phj_radix_bw_reg_hro")
 MAX_KTIMES = 16
 
 

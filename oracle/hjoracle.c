@@ -503,7 +503,7 @@ static int tq_get(task_queue_t *q, task_t *t) {
 }
 
 typedef struct {
-	int nt, lp;
+	int nt, lp;  /* lp: per-partition join kind (0 bc, 1 lp, 2 sc, 3 hr, 4 hro) */
 	pthread_barrier_t *bar;
 	part_shared_t pr, ps;        /* pass-1 partition of R and of S */
 	uint32_t W, B, b1, b2, passes;
@@ -669,7 +669,7 @@ static void hr_join(const uint8_t *R, uint64_t nr, const uint8_t *S, uint64_t ns
 	*sum += s;
 }
 
-static void *radix_thread(void *p) {static void *radix_thread(void *p) {
+static void *radix_thread(void *p) {
 	radix_arg_t *a = (radix_arg_t *)p;
 	radix_shared_t *sh = a->sh;
 	const uint32_t W = sh->W;
@@ -749,14 +749,15 @@ static void *radix_thread(void *p) {static void *radix_thread(void *p) {
 }
 
 int or_phj_radix(const void *R, uint64_t nr, const void *S, uint64_t ns, uint32_t width, uint32_t radix_bits,
-                 uint32_t passes, int lp, int threads, or_result_t *out) {
+                 uint32_t passes, int kind, int threads, or_result_t *out) {
 	if (threads < 1) threads = 1;
 	if (threads > 256) threads = 256;
 	if (passes < 1 || passes > 2 || radix_bits > 20 || (passes == 2 && radix_bits < 2)) return -1;
 	radix_shared_t sh;
 	memset(&sh, 0, sizeof sh);
 	sh.nt = threads;
-	sh.lp = lp;
+	if (kind < 0 || kind > 4) return -1;
+	sh.lp = kind;
 	sh.W = width;
 	sh.B = radix_bits;
 	sh.passes = passes;

@@ -46,8 +46,10 @@ void or_generate(const hj_gen_spec_t *spec, void *r, void *s, int threads);
 /* restatements of the reference joins; width in bytes (16..256) */
 int or_nphj_sc(const void *R, uint64_t nr, const void *S, uint64_t ns, uint32_t width, int threads, or_result_t *out);
 int or_nphj_lp(const void *R, uint64_t nr, const void *S, uint64_t ns, uint32_t width, int threads, or_result_t *out);
+/* kind = per-partition join: 0 bc_join, 1 lp_join, 2 sc_join, 3 hr_join,
+ * 4 hro_join (phj.c:496-1037) */
 int or_phj_radix(const void *R, uint64_t nr, const void *S, uint64_t ns, uint32_t width, uint32_t radix_bits,
-                 uint32_t passes, int lp, int threads, or_result_t *out);
+                 uint32_t passes, int kind, int threads, or_result_t *out);
 
 /* stable radix partition of one relation by key bits [shift, shift+bits)
  * (the reference's parallel_radix_partition_optimized, phj.c:257-359, on

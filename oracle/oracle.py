@@ -84,6 +84,10 @@ def _res(r: OrResult) -> dict:
             "probejoin_time": r.probejoin_time, "total_time": r.total_time}
 
 
+# per-partition join of the radix joins (phj.c:2515-2535): bc, lp, sc, hr, hro
+RADIX_KINDS = {"phj_radix_bc": 0, "phj_radix_lp": 1, "phj_radix_sc": 2, "phj_radix_hr": 3, "phj_radix_hro": 4}
+
+
 def join(algo: str, R: np.ndarray, S: np.ndarray, width: int, threads: int = 8, radix_bits: int = 11,
          passes: int = 1) -> dict:
     """Run the restated reference join `algo` on host tuples."""
@@ -95,8 +99,8 @@ def join(algo: str, R: np.ndarray, S: np.ndarray, width: int, threads: int = 8, 
         rc = L.or_nphj_sc(_ptr(R), nr, _ptr(S), ns, width, threads, C.byref(out))
     elif algo == "nphj_lp":
         rc = L.or_nphj_lp(_ptr(R), nr, _ptr(S), ns, width, threads, C.byref(out))
-    elif algo in ("phj_radix_bc", "phj_radix_lp"):
-        rc = L.or_phj_radix(_ptr(R), nr, _ptr(S), ns, width, radix_bits, passes, int(algo.endswith("lp")),
+    elif algo in RADIX_KINDS:
+        rc = L.or_phj_radix(_ptr(R), nr, _ptr(S), ns, width, radix_bits, passes, RADIX_KINDS[algo],
                             threads, C.byref(out))
     else:
         raise ValueError(algo)

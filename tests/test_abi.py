@@ -11,8 +11,9 @@ import pytest
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIB = os.path.join(ROOT, "hashjoin-scm_amd", "libhjmi.so")
 CLI = os.path.join(ROOT, "hashjoin-scm_amd", "bin", "hjmain")
-REFERENCE_ALGOS = ["nphj_sc", "nphj_lp", "phj_radix_bc", "phj_radix_lp", "phj_radix_bw_reg_bc",
-                   "phj_radix_bw_reg_lp"]
+REFERENCE_ALGOS = ["nphj_sc", "nphj_lp", "phj_radix_bc", "phj_radix_lp", "phj_radix_sc", "phj_radix_hr",
+                   "phj_radix_hro", "phj_radix_bw_reg_bc", "phj_radix_bw_reg_lp", "phj_radix_bw_reg_sc",
+                   "phj_radix_bw_reg_hr", "phj_radix_bw_reg_hro"]
 
 
 def declared_functions():

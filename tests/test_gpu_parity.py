@@ -50,7 +50,10 @@ def test_join_matches_oracle(ctx, algo, wl, nr, ns, width):
     ctx.free_relation(dS)
 
 
-@pytest.mark.parametrize("algo", ["phj_radix_bc", "phj_radix_lp"])
+RADIX = ["phj_radix_bc", "phj_radix_lp", "phj_radix_sc", "phj_radix_hr", "phj_radix_hro"]
+
+
+@pytest.mark.parametrize("algo", RADIX)
 @pytest.mark.parametrize("bits,passes", [(0, 0), (1, 1), (4, 1), (8, 1), (11, 1), (12, 1), (2, 2), (9, 2),
                                          (12, 2), (14, 2), (16, 2), (20, 2)])
 def test_radix_configurations(ctx, algo, bits, passes):
@@ -124,7 +127,7 @@ def test_empty_and_tiny_relations(ctx, algo):
         ctx.free_relation(dB)
 
 
-@pytest.mark.parametrize("algo", ["nphj_sc", "phj_radix_bc", "phj_radix_lp", "nphj_lp"])
+@pytest.mark.parametrize("algo", ["nphj_sc", "nphj_lp"] + RADIX)
 def test_one_hot_build_key(ctx, algo):
     """Every build tuple has the same key: one bucket chain of |R| entries
     (SC overflow pool grows and rebuilds; radix bc walks an LDS chain over
@@ -144,6 +147,37 @@ def test_host_resident_relations(ctx):
     want = O.pkfk_closed_form(S, 16)
     for algo in ALGOS:
         _check(ctx.join_host(algo, R, S, 16), want, algo)
+
+
+@pytest.mark.parametrize("algo", ["phj_radix_sc", "phj_radix_hr", "phj_radix_hro"])
+@pytest.mark.parametrize("width", [16, 32, 64])
+def test_partition_tables_on_collisions(ctx, algo, width):
+    """Keys sharing their low 16 bits fill one partition and a few buckets:
+    sc overflow chains, hr/hro ranges over many tuples, several LDS fills."""
+    rng = np.random.default_rng(width)
+    base = rng.integers(1, 1 << 40, 300, dtype=np.uint64) << np.uint64(16)
+    rk = np.concatenate([base, base[:100], rng.integers(1, 1 << 50, 5000, dtype=np.uint64)])
+    sk = np.concatenate([rng.choice(rk, 60000), rng.integers(1, 1 << 50, 5000, dtype=np.uint64)])
+    R = np.zeros((len(rk), width // 8), dtype=np.uint64)
+    S = np.zeros((len(sk), width // 8), dtype=np.uint64)
+    R[:, 0], R[:, 1] = rk, rng.integers(0, 1 << 63, len(rk), dtype=np.uint64)
+    S[:, 0], S[:, 1] = sk, rng.integers(0, 1 << 63, len(sk), dtype=np.uint64)
+    R, S = R.view(np.uint8).reshape(-1), S.view(np.uint8).reshape(-1)
+    want = O.aggregate(R, S, width)
+    dR, dS = _dev(ctx, R, width), _dev(ctx, S, width)
+    for bits, passes in ((0, 0), (4, 1), (12, 1), (8, 2), (16, 2)):
+        _check(ctx.join(algo, dR, dS, radix_bits=bits, passes=passes), want, (algo, bits, passes))
+    ctx.free_relation(dR)
+    ctx.free_relation(dS)
+
+
+def test_hro_rejects_wide_tuples(ctx):
+    R, S = ctx.generate("pkfk", 64, 256, 128)
+    with pytest.raises(hjmi.HJError) as e:
+        ctx.join("phj_radix_hro", R, S, radix_bits=2, passes=1)
+    assert e.value.code == hjmi.HJ_EINVAL
+    ctx.free_relation(R)
+    ctx.free_relation(S)
 
 
 def test_errors_are_status_codes(ctx):

@@ -66,7 +66,7 @@ def test_benchmark_config_c2(ctx):
     nr, ns = 1 << 24, 1 << 28
     want = _s_closed_form(nr, ns)
     R, S = ctx.generate("pkfk", nr, ns, 16)
-    for algo in ("phj_radix_bc", "phj_radix_lp", "nphj_lp", "nphj_sc"):
+    for algo in ("phj_radix_bc", "phj_radix_lp", "nphj_lp", "nphj_sc", "phj_radix_sc", "phj_radix_hr", "phj_radix_hro"):
         _check(ctx.join(algo, R, S), want["matched_cnt"], want["check_sum"], algo)
     for bits, passes in ((10, 1), (11, 1), (12, 1), (12, 2), (14, 2), (16, 2)):
         _check(ctx.join("phj_radix_bc", R, S, radix_bits=bits, passes=passes), want["matched_cnt"],

@@ -17,7 +17,8 @@ import pytest
 import oracle as O
 
 GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "reference_logs.json")
-ALGOS = ["nphj_sc", "nphj_lp", "phj_radix_bc", "phj_radix_lp"]
+ALGOS = ["nphj_sc", "nphj_lp", "phj_radix_bc", "phj_radix_lp", "phj_radix_sc", "phj_radix_hr", "phj_radix_hro"]
+ALL_PAIRS = ["phj_radix_bc", "phj_radix_sc", "phj_radix_hr", "phj_radix_hro"]
 
 
 def _records():
@@ -76,7 +77,7 @@ def test_restated_joins_match_reference_closed_forms(wl, cf):
         got = O.join(algo, R, S, 16, threads=4)
         assert (got["matched_cnt"], got["check_sum"]) == (want["matched_cnt"], want["check_sum"]), algo
     for bits, passes in [(4, 1), (8, 2), (11, 1), (12, 2), (13, 2)]:
-        for algo in ("phj_radix_bc", "phj_radix_lp"):
+        for algo in ALL_PAIRS + ["phj_radix_lp"]:
             got = O.join(algo, R, S, 16, threads=4, radix_bits=bits, passes=passes)
             assert (got["matched_cnt"], got["check_sum"]) == (want["matched_cnt"], want["check_sum"])
 
@@ -99,7 +100,7 @@ def test_restated_joins_vs_bruteforce_with_duplicates(width):
     assert bc["matched_cnt"] > lp["matched_cnt"] > 0  # duplicates and misses present
     assert O.aggregate(R, S, width, lp=False) == {**bc}
     assert O.aggregate(R, S, width, lp=True) == {**lp}
-    for algo in ("nphj_sc", "phj_radix_bc"):
+    for algo in ["nphj_sc"] + ALL_PAIRS:
         g = O.join(algo, R, S, width, threads=3, radix_bits=5, passes=1)
         assert (g["matched_cnt"], g["check_sum"]) == (bc["matched_cnt"], bc["check_sum"]), algo
     # LP keeps the first build tuple per key: deterministic with one build thread
@@ -108,6 +109,29 @@ def test_restated_joins_vs_bruteforce_with_duplicates(width):
     for bits, passes in [(3, 1), (6, 2)]:
         g = O.join("phj_radix_lp", R, S, width, threads=3, radix_bits=bits, passes=passes)
         assert (g["matched_cnt"], g["check_sum"]) == (lp["matched_cnt"], lp["check_sum"])
+
+
+@pytest.mark.parametrize("width", [16, 32, 64])
+def test_restated_partition_joins_on_collisions_and_chains(width):
+    """sc overflow chains, hr/hro buckets holding whole partitions, one-tuple
+    partitions: all-pairs joins equal the brute force.  Keys that share
+    their low 12 bits land in one partition and (above the radix bits) in a
+    few hash buckets, so sc chains grow past their two inline slots and hr
+    ranges hold many tuples."""
+    rng = np.random.default_rng(11)
+    base = rng.integers(1, 1 << 40, 40, dtype=np.uint64) << np.uint64(12)
+    rk = np.concatenate([base, base[:10], rng.integers(1, 1 << 50, 200, dtype=np.uint64), [7]])
+    sk = np.concatenate([rng.choice(rk, 3000), rng.integers(1, 1 << 50, 500, dtype=np.uint64)])
+    R = np.zeros((len(rk), width // 8), dtype=np.uint64)
+    S = np.zeros((len(sk), width // 8), dtype=np.uint64)
+    R[:, 0], R[:, 1] = rk, rng.integers(0, 1 << 63, len(rk), dtype=np.uint64)
+    S[:, 0], S[:, 1] = sk, rng.integers(0, 1 << 63, len(sk), dtype=np.uint64)
+    R, S = R.view(np.uint8).reshape(-1), S.view(np.uint8).reshape(-1)
+    want = O.bruteforce(R, S, width)
+    for algo in ALL_PAIRS:
+        for bits, passes in [(1, 1), (12, 1), (12, 2), (16, 2)]:
+            g = O.join(algo, R, S, width, threads=3, radix_bits=bits, passes=passes)
+            assert (g["matched_cnt"], g["check_sum"]) == (want["matched_cnt"], want["check_sum"]), (algo, bits)
 
 
 def _tuples(keys, pays, width=16):
