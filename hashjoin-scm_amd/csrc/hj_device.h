@@ -4,6 +4,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 #define HJ_WAVE 64
 
 // A tuple is WV 16-byte words; word 0 holds {key, payload}.
@@ -56,6 +58,102 @@ __device__ __forceinline__ void hj_store_stream(uint4 *p, const uint4 &v) {
 __device__ __forceinline__ uint4 hj_load(const uint4 *p) {
 	const hj_u32x4 v = *(const HJ_GLOBAL hj_u32x4 *)p;
 	return make_uint4(v.x, v.y, v.z, v.w);
+}
+
+// Match sinks of the probe loops.  hj_sum_sink counts matches and adds the
+// two payloads (the reference's matched_cnt / check_sum).  hj_pair_sink also
+// appends the pair (build payload, probe payload) -- the row-id pairs of late
+// materialisation -- to a device array.
+//
+// Pair slots come in per-wave chunks of `chunk` slots claimed with one
+// global atomic on `cursor`, so a join of M pairs issues about M / chunk
+// atomics (one per match call on a single address serialises the device).
+// A wave's chunk (pos, end) lives in LDS (`ws`, two words per wave) because
+// matches arrive in divergent code, where lane registers of inactive lanes
+// would go stale.  At the end each wave fills the unused rest of its chunk
+// with HJ_PAIR_HOLE, which readers skip.  Slots at or past `cap` are not
+// written; the cursor then exceeds cap and the caller re-runs with more.
+#define HJ_PAIR_HOLE 0xFFFFFFFFFFFFFFFFULL
+
+struct hj_sum_sink {
+	unsigned long long cnt = 0, sum = 0;
+	__device__ __forceinline__ void match(uint64_t r, uint64_t s) {
+		cnt++;
+		sum += r + s;
+	}
+	__device__ __forceinline__ void finish() {}
+};
+
+struct hj_pair_sink {
+	unsigned long long cnt = 0, sum = 0;
+	uint4 *out = nullptr;
+	unsigned long long cap = 0;
+	unsigned long long *cursor = nullptr;
+	volatile unsigned long long *ws = nullptr;  // LDS: pos, end per wave
+	unsigned chunk = 0;                          // >= 64
+
+	__device__ __forceinline__ void put(unsigned long long k, uint64_t r, uint64_t s) const {
+		if (k < cap) {
+			hj_u32x4 v = {(uint32_t)r, (uint32_t)(r >> 32), (uint32_t)s, (uint32_t)(s >> 32)};
+			__builtin_nontemporal_store(v, (HJ_GLOBAL hj_u32x4 *)(out + k));
+		}
+	}
+	__device__ __forceinline__ void match(uint64_t r, uint64_t s) {
+		cnt++;
+		sum += r + s;
+		const unsigned long long act = __ballot(1);  // lanes storing a pair now
+		const int lane = threadIdx.x & (HJ_WAVE - 1), lead = __ffsll((long long)act) - 1;
+		const unsigned c = __popcll(act), rank = __popcll(act & ((1ull << lane) - 1));
+		volatile unsigned long long *w = ws + 2 * (threadIdx.x / HJ_WAVE);
+		const unsigned long long pos = w[0], end = w[1], left = end - pos;
+		unsigned long long slot, npos, nend;
+		if (c > left) {  // wave-uniform: chunk exhausted
+			unsigned long long base = 0;
+			if (lane == lead) base = atomicAdd(cursor, (unsigned long long)chunk);
+			base = __shfl(base, lead, HJ_WAVE);
+			slot = rank < left ? pos + rank : base + (rank - left);
+			npos = base + (c - left);
+			nend = base + chunk;
+		} else {
+			slot = pos + rank;
+			npos = pos + c;
+			nend = end;
+		}
+		if (lane == lead) {
+			w[0] = npos;
+			w[1] = nend;
+		}
+		put(slot, r, s);
+	}
+	// every lane of the wave, converged: mark the rest of the chunk
+	__device__ __forceinline__ void finish() {
+		volatile unsigned long long *w = ws + 2 * (threadIdx.x / HJ_WAVE);
+		const unsigned long long pos = w[0], end = w[1];
+		for (unsigned long long k = pos + (threadIdx.x & (HJ_WAVE - 1)); k < end; k += HJ_WAVE)
+			put(k, HJ_PAIR_HOLE, HJ_PAIR_HOLE);
+	}
+};
+
+// The sink of a probe kernel: hj_pair_sink when EMIT, else hj_sum_sink.
+// `ws` is the kernel's LDS array of two words per wave; every lane calls
+// this before any match (lane 0 of each wave clears its chunk).
+template <bool EMIT>
+__device__ __forceinline__ typename std::conditional<EMIT, hj_pair_sink, hj_sum_sink>::type
+hj_make_sink(uint4 *out, unsigned long long cap, unsigned long long *cursor, unsigned chunk,
+             unsigned long long *ws) {
+	typename std::conditional<EMIT, hj_pair_sink, hj_sum_sink>::type acc;
+	if constexpr (EMIT) {
+		acc.out = out;
+		acc.cap = cap;
+		acc.cursor = cursor;
+		acc.chunk = chunk;
+		acc.ws = ws;
+		if ((threadIdx.x & (HJ_WAVE - 1)) == 0) {
+			acc.ws[2 * (threadIdx.x / HJ_WAVE)] = 0;
+			acc.ws[2 * (threadIdx.x / HJ_WAVE) + 1] = 0;
+		}
+	}
+	return acc;
 }
 
 __device__ __forceinline__ unsigned long long hj_wave_sum(unsigned long long v) {

@@ -64,7 +64,8 @@ int hj_launch_generate(hj_ctx_t *ctx, const hj_gen_spec_t *spec, void *r, void *
 		case 32: launch_gen<4>(ctx, g, side, cdf, n, out); break;
 		case 64: launch_gen<8>(ctx, g, side, cdf, n, out); break;
 		case 128: launch_gen<16>(ctx, g, side, cdf, n, out); break;
-		default: launch_gen<32>(ctx, g, side, cdf, n, out); break;
+		case 256: launch_gen<32>(ctx, g, side, cdf, n, out); break;
+		default: launch_gen<64>(ctx, g, side, cdf, n, out); break;
 		}
 	}
 	hipError_t e = hipGetLastError();

@@ -25,6 +25,8 @@ enum hj_kid : uint32_t {
 	HJ_K_SORTCHUNK,
 	HJ_K_TILESORT,
 	HJ_K_GATHERSORT,
+	HJ_K_KEYRID,
+	HJ_K_RETRIEVE,
 	HJ_K_COUNT
 };
 
@@ -54,13 +56,25 @@ struct hj_ctx {
 	hj_buf stats;      // device counters and results
 	hj_buf hostcopy[2];// device copies of host-resident relations
 
+	// late materialisation: while set, joins on 16-byte tuples also append
+	// their (build payload, probe payload) pairs here (emit_cap slots at
+	// most, claimed in per-wave chunks of emit_chunk slots)
+	uint4 *emit_pairs = nullptr;
+	uint64_t emit_cap = 0;
+	unsigned emit_chunk = 0;
+	hj_buf pairs;      // the pair buffer of late-materialising joins
+	hj_buf keyrid[2];  // {key, row_id} copies of wide R and S
+
 	// size-class autotuner: radix bits / passes per log2 |R| (0xff = unset)
 	uint8_t tune_bits[40];
 	uint8_t tune_passes[40];
 	bool tuned = false;
 
 	// timing
-	static const int NEV = 2 * HJ_MAX_KTIMES + 4;
+	// ev[0..3] phases of a join, ev[4..] kernel pairs, the last three the
+	// outer phases of a late-materialising join
+	static const int NEV = 2 * HJ_MAX_KTIMES + 7;
+	static const int EV_PTR = 2 * HJ_MAX_KTIMES + 4;
 	hipEvent_t ev[NEV] = {};
 	bool kernel_times = false;
 	int n_kt = 0;
@@ -97,7 +111,10 @@ struct hj_stats {
 	unsigned long long sent_sum;   // payload of that row
 	unsigned long long total_work; // work items produced by the planner
 	unsigned long long total_chunks;
-	unsigned long long pad1[6];
+	unsigned long long pair_next;  // late materialisation: row-id pairs emitted
+	unsigned long long retrieved;  // late materialisation: retrieve checksum
+	unsigned long long retrieved_ok; // pairs whose row ids lie inside R and S
+	unsigned long long pad1[3];
 };
 
 #define HJ_DF_DUPKEY 1u       // duplicate build key met in an LP table
@@ -106,6 +123,10 @@ struct hj_stats {
 #define HJ_DF_TABLE_FULL 8u   // LP global table full on insert
 
 // ---- launchers (each returns HJ_OK or an error code) -----------------------
+int hj_fill16(hj_ctx_t *ctx, void *p, uint64_t n16, uint4 v);
+int hj_stats_reset(hj_ctx_t *ctx);
+// copy the device counters to *h (synchronises the stream)
+int hj_stats_read(hj_ctx_t *ctx, hj_stats *h);
 int hj_launch_generate(hj_ctx_t *ctx, const hj_gen_spec_t *spec, void *r, void *s);
 
 int hj_run_nphj_lp(hj_ctx_t *ctx, const hj_relation_t *R, const hj_relation_t *S,
@@ -116,6 +137,12 @@ int hj_run_nphj_sc(hj_ctx_t *ctx, const hj_relation_t *R, const hj_relation_t *S
 // 3 hr, 4 hro)
 int hj_run_radix(hj_ctx_t *ctx, const hj_relation_t *R, const hj_relation_t *S,
                  const hj_params_t *prm, int tk, hj_result_t *out);
+typedef int (*hj_algo_fn)(hj_ctx_t *, const hj_relation_t *, const hj_relation_t *, const hj_params_t *,
+                          hj_result_t *);
+// late materialisation around `inner`: join the {key, row_id} words, emit
+// row-id pairs, then retrieve the checksum through them
+int hj_run_ptr(hj_ctx_t *ctx, const hj_relation_t *R, const hj_relation_t *S, const hj_params_t *prm,
+               hj_algo_fn inner, hj_result_t *out);
 // radix configuration for |R|: the tuned class if the context has one, else
 // the default rule
 void hj_radix_config(hj_ctx_t *ctx, uint64_t nr, uint32_t *bits, uint32_t *passes);

@@ -165,12 +165,14 @@ __global__ __launch_bounds__(NPHJ_THREADS) void k_lp_setpay(const uint4 *__restr
 	}
 }
 
-template <int WV>
+template <int WV, bool EMIT>
 __global__ __launch_bounds__(NPHJ_THREADS) void k_lp_probe(const uint4 *__restrict__ S, uint64_t n,
                                                            const ulonglong2 *__restrict__ tab, uint64_t mask,
-                                                           const uint4 *__restrict__ R, hj_stats *st) {
+                                                           const uint4 *__restrict__ R, hj_stats *st, uint4 *pout,
+                                                           unsigned long long pcap, unsigned pchunk) {
 	__shared__ unsigned long long red[2 * (NPHJ_THREADS / HJ_WAVE)];
-	unsigned long long cnt = 0, sum = 0;
+	__shared__ unsigned long long emit_ws[EMIT ? 2 * (NPHJ_THREADS / HJ_WAVE) : 2];
+	auto acc = hj_make_sink<EMIT>(pout, pcap, &st->pair_next, pchunk, emit_ws);
 	const uint64_t sent_cnt = st->sent_cnt;
 	const uint64_t sent_pay = sent_cnt ? hj_pay_of(R[st->sent_min * WV]) : 0;
 	const uint64_t tile = (uint64_t)blockDim.x * PROBE_U;
@@ -195,16 +197,12 @@ __global__ __launch_bounds__(NPHJ_THREADS) void k_lp_probe(const uint4 *__restri
 		for (int u = 0; u < PROBE_U; u++) {
 			if (!live[u]) continue;
 			if (key[u] == LP_EMPTY) {
-				if (sent_cnt) {
-					cnt++;
-					sum += sent_pay + pay[u];
-				}
+				if (sent_cnt) acc.match(sent_pay, pay[u]);
 				continue;
 			}
 			for (uint64_t step = 0; step <= mask; step++) {
 				if (e[u].x == key[u]) {
-					cnt++;
-					sum += e[u].y + pay[u];
+					acc.match(e[u].y, pay[u]);
 					break;
 				}
 				if (e[u].x == LP_EMPTY) break;
@@ -213,7 +211,8 @@ __global__ __launch_bounds__(NPHJ_THREADS) void k_lp_probe(const uint4 *__restri
 			}
 		}
 	}
-	hj_block_add2(cnt, sum, red, &st->matched, &st->checksum);
+	acc.finish();
+	hj_block_add2(acc.cnt, acc.sum, red, &st->matched, &st->checksum);
 }
 
 // Build (and, if duplicates were seen, fix up) the LP table for R.
@@ -291,15 +290,21 @@ int hj_run_nphj_lp(hj_ctx_t *ctx, const hj_relation_t *R, const hj_relation_t *S
 	const uint64_t n = S->tuple_num;
 	const unsigned g = grid_for(ctx, (n + PROBE_U - 1) / PROBE_U, NPHJ_THREADS, 8);
 	hj_kt_begin(ctx, HJ_K_LP_PROBE);
-#define LP_P(W)                                                                                                  \
-	hipLaunchKernelGGL(k_lp_probe<W>, dim3(g), dim3(NPHJ_THREADS), 0, ctx->stream, (const uint4 *)S->tuples, n, \
-	                   (const ulonglong2 *)tab, slots - 1, (const uint4 *)R->tuples, st)
+#define LP_P(W, E)                                                                                                  \
+	hipLaunchKernelGGL((k_lp_probe<W, E>), dim3(g), dim3(NPHJ_THREADS), 0, ctx->stream, (const uint4 *)S->tuples, n, \
+	                   (const ulonglong2 *)tab, slots - 1, (const uint4 *)R->tuples, st, ctx->emit_pairs,           \
+	                   (unsigned long long)ctx->emit_cap, ctx->emit_chunk)
 	switch (wv) {
-	case 1: LP_P(1); break;
-	case 2: LP_P(2); break;
-	case 4: LP_P(4); break;
-	case 8: LP_P(8); break;
-	default: LP_P(16); break;
+	case 1:
+		if (ctx->emit_pairs)
+			LP_P(1, true);
+		else
+			LP_P(1, false);
+		break;
+	case 2: LP_P(2, false); break;
+	case 4: LP_P(4, false); break;
+	case 8: LP_P(8, false); break;
+	default: LP_P(16, false); break;
 	}
 #undef LP_P
 	hj_kt_end(ctx);
@@ -371,23 +376,26 @@ __global__ __launch_bounds__(NPHJ_THREADS) void k_sc_build(const uint4 *__restri
 	if (__any(pool_full) && (threadIdx.x & (HJ_WAVE - 1)) == 0) atomicOr(&st->flags, HJ_DF_POOL_FULL);
 }
 
-__device__ __forceinline__ void sc_match_node(const uint4 *nd, uint64_t key, uint64_t pay, unsigned long long &cnt,
-                                              unsigned long long &sum, unsigned &next) {
+template <class Sink>
+__device__ __forceinline__ void sc_match_node(const uint4 *nd, uint64_t key, uint64_t pay, Sink &sink,
+                                              unsigned &next) {
 	const uint4 w0 = nd[0], w1 = nd[1], w2 = nd[2], w3 = nd[3];
 	const unsigned c = w3.x < SC_SLOTS ? w3.x : SC_SLOTS;
 	next = w3.y;
-	if (c > 0 && hj_key_of(w0) == key) { cnt++; sum += hj_pay_of(w0) + pay; }
-	if (c > 1 && hj_key_of(w1) == key) { cnt++; sum += hj_pay_of(w1) + pay; }
-	if (c > 2 && hj_key_of(w2) == key) { cnt++; sum += hj_pay_of(w2) + pay; }
+	if (c > 0 && hj_key_of(w0) == key) sink.match(hj_pay_of(w0), pay);
+	if (c > 1 && hj_key_of(w1) == key) sink.match(hj_pay_of(w1), pay);
+	if (c > 2 && hj_key_of(w2) == key) sink.match(hj_pay_of(w2), pay);
 }
 
-template <int WV>
+template <int WV, bool EMIT>
 __global__ __launch_bounds__(NPHJ_THREADS) void k_sc_probe(const uint4 *__restrict__ S, uint64_t n,
                                                            const sc_node *__restrict__ tab, uint64_t mask,
                                                            const sc_node *__restrict__ pool, uint64_t pool_cap,
-                                                           hj_stats *st) {
+                                                           hj_stats *st, uint4 *pout, unsigned long long pcap,
+                                                           unsigned pchunk) {
 	__shared__ unsigned long long red[2 * (NPHJ_THREADS / HJ_WAVE)];
-	unsigned long long cnt = 0, sum = 0;
+	__shared__ unsigned long long emit_ws[EMIT ? 2 * (NPHJ_THREADS / HJ_WAVE) : 2];
+	auto acc = hj_make_sink<EMIT>(pout, pcap, &st->pair_next, pchunk, emit_ws);
 	const uint64_t tile = (uint64_t)blockDim.x * PROBE_U;
 	for (uint64_t base = (uint64_t)blockIdx.x * tile; base < n; base += (uint64_t)gridDim.x * tile) {
 		uint64_t key[PROBE_U], pay[PROBE_U];
@@ -404,19 +412,20 @@ __global__ __launch_bounds__(NPHJ_THREADS) void k_sc_probe(const uint4 *__restri
 #pragma unroll
 		for (int u = 0; u < PROBE_U; u++) {
 			next[u] = 0;
-			if (live[u]) sc_match_node((const uint4 *)&tab[key[u] & mask], key[u], pay[u], cnt, sum, next[u]);
+			if (live[u]) sc_match_node((const uint4 *)&tab[key[u] & mask], key[u], pay[u], acc, next[u]);
 		}
 #pragma unroll
 		for (int u = 0; u < PROBE_U; u++) {
 			uint64_t hops = 0;
 			while (next[u] && hops++ < pool_cap) {
 				unsigned nx;
-				sc_match_node((const uint4 *)&pool[next[u] - 1], key[u], pay[u], cnt, sum, nx);
+				sc_match_node((const uint4 *)&pool[next[u] - 1], key[u], pay[u], acc, nx);
 				next[u] = nx;
 			}
 		}
 	}
-	hj_block_add2(cnt, sum, red, &st->matched, &st->checksum);
+	acc.finish();
+	hj_block_add2(acc.cnt, acc.sum, red, &st->matched, &st->checksum);
 }
 
 int hj_run_nphj_sc(hj_ctx_t *ctx, const hj_relation_t *R, const hj_relation_t *S, const hj_params_t *prm,
@@ -460,15 +469,21 @@ int hj_run_nphj_sc(hj_ctx_t *ctx, const hj_relation_t *R, const hj_relation_t *S
 		const uint64_t n = S->tuple_num;
 		const unsigned gp = grid_for(ctx, (n + PROBE_U - 1) / PROBE_U, NPHJ_THREADS, 8);
 		hj_kt_begin(ctx, HJ_K_SC_PROBE);
-#define SC_P(W)                                                                                                   \
-	hipLaunchKernelGGL(k_sc_probe<W>, dim3(gp), dim3(NPHJ_THREADS), 0, ctx->stream, (const uint4 *)S->tuples, n, \
-	                   (const sc_node *)tab, nb - 1, (const sc_node *)pool, pool_cap, st)
+#define SC_P(W, E)                                                                                                   \
+	hipLaunchKernelGGL((k_sc_probe<W, E>), dim3(gp), dim3(NPHJ_THREADS), 0, ctx->stream, (const uint4 *)S->tuples, n, \
+	                   (const sc_node *)tab, nb - 1, (const sc_node *)pool, pool_cap, st, ctx->emit_pairs,          \
+	                   (unsigned long long)ctx->emit_cap, ctx->emit_chunk)
 		switch (wv) {
-		case 1: SC_P(1); break;
-		case 2: SC_P(2); break;
-		case 4: SC_P(4); break;
-		case 8: SC_P(8); break;
-		default: SC_P(16); break;
+		case 1:
+			if (ctx->emit_pairs)
+				SC_P(1, true);
+			else
+				SC_P(1, false);
+			break;
+		case 2: SC_P(2, false); break;
+		case 4: SC_P(4, false); break;
+		case 8: SC_P(8, false); break;
+		default: SC_P(16, false); break;
 		}
 #undef SC_P
 		hj_kt_end(ctx);

@@ -40,9 +40,6 @@
 #include <string.h>
 #include <vector>
 
-int hj_fill16(hj_ctx_t *ctx, void *p, uint64_t n16, uint4 v);
-int hj_stats_reset(hj_ctx_t *ctx);
-int hj_stats_read(hj_ctx_t *ctx, hj_stats *h);
 
 struct hj_seg {
 	const uint4 *in;     // first input tuple
@@ -412,11 +409,12 @@ __global__ void k_plan(const uint64_t *__restrict__ pb, uint32_t passes, uint32_
 // ------------------------------------------------------------ join
 // LDS (bc): keys u64[cap] | pays u64[cap] | heads u32[np2(cap)] | next u16[cap]
 // LDS (lp): keys u64[cap] | pays u64[cap] | occ   u32[np2(cap)]
-template <int WV, int NT, int TK>
+template <int WV, int NT, int TK, bool EMIT>
 __global__ __launch_bounds__(NT) void k_join(const uint4 *__restrict__ Rb, const uint4 *__restrict__ Sb,
                                              const hj_pair *__restrict__ pairs,
                                              const uint64_t *__restrict__ tprefix, uint32_t npairs, uint32_t B,
-                                             uint64_t s_tile, uint32_t cap, hj_stats *st) {
+                                             uint64_t s_tile, uint32_t cap, hj_stats *st, uint4 *pout,
+                                             unsigned long long pcap, unsigned pchunk) {
 	constexpr bool LP = TK == HJ_TK_LP;
 	constexpr bool TWO = TK == HJ_TK_HR || TK == HJ_TK_HRO;  // count + place builds
 	extern __shared__ uint64_t lds_u64[];
@@ -427,7 +425,8 @@ __global__ __launch_bounds__(NT) void k_join(const uint4 *__restrict__ Rb, const
 	hj_tab<TK, WV> tab;
 	tab.map(lds_u64, cap, &s_pool);
 	const uint64_t total = tprefix[npairs];
-	unsigned long long cnt = 0, sum = 0;
+	__shared__ unsigned long long emit_ws[EMIT ? 2 * (NT / HJ_WAVE) : 2];
+	auto acc = hj_make_sink<EMIT>(pout, pcap, &st->pair_next, pchunk, emit_ws);
 	bool dup = false, oversize = false;
 
 	for (;;) {
@@ -490,14 +489,15 @@ __global__ __launch_bounds__(NT) void k_join(const uint4 *__restrict__ Rb, const
 					sk[u] = hj_key_of(t);
 					sp[u] = hj_pay_of(t);
 				}
-				tab.template probe<U>(sk, sp, live, B, cnt, sum);
+				tab.template probe<U>(sk, sp, live, B, acc);
 			}
 			__syncthreads();
 		}
 	}
+	acc.finish();
 	const unsigned fl = (__any(dup) ? HJ_DF_DUPKEY : 0u) | (__any(oversize) ? HJ_DF_OVERSIZE : 0u);
 	if (fl && (threadIdx.x & (HJ_WAVE - 1)) == 0) atomicOr(&st->flags, fl);
-	hj_block_add2(cnt, sum, red, &st->matched, &st->checksum);
+	hj_block_add2(acc.cnt, acc.sum, red, &st->matched, &st->checksum);
 }
 
 // ------------------------------------------------------------ pass 2
@@ -891,12 +891,13 @@ __global__ __launch_bounds__(256) void k_run_transpose(const uint16_t *__restric
 // fill, and the host recomputes that join exactly).  Probe side: each wave
 // loads the next RU tuples per lane while it probes the current ones.
 // LDS: keys u64[cap] | pays u64[cap] | heads u32[np2(cap)] | next u16[cap]
-template <int WV, int NT, int TK>
+template <int WV, int NT, int TK, bool EMIT>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_join2(const uint4 *__restrict__ P, const uint64_t *__restrict__ cstart,
                                               const uint16_t *__restrict__ run_t, uint32_t stride, uint32_t b1,
                                               const hj_pair *__restrict__ pairs,
                                               const uint64_t *__restrict__ tprefix, uint32_t npairs, uint32_t B,
-                                              uint32_t cpt, uint32_t cap, hj_stats *st) {
+                                              uint32_t cpt, uint32_t cap, hj_stats *st, uint4 *pout,
+                                              unsigned long long pcap, unsigned pchunk) {
 	constexpr bool LP = TK == HJ_TK_LP;
 	constexpr bool TWO = TK == HJ_TK_HR || TK == HJ_TK_HRO;  // count + place builds
 	constexpr uint32_t NW = NT / HJ_WAVE;
@@ -913,7 +914,8 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
 	tab.map(lds_u64, cap, &s_pool);
 	const uint32_t lane = threadIdx.x & (HJ_WAVE - 1), wid = threadIdx.x / HJ_WAVE;
 	const uint64_t total = tprefix[npairs];
-	unsigned long long cnt = 0, sum = 0;
+	__shared__ unsigned long long emit_ws[EMIT ? 2 * (NT / HJ_WAVE) : 2];
+	auto acc = hj_make_sink<EMIT>(pout, pcap, &st->pair_next, pchunk, emit_ws);
 	bool dup = false, oversize = false;
 
 	// Stage the digit-p2 runs of chunks [c0, c1), c1 - c0 <= NT; the first run
@@ -969,7 +971,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
 			kk[u] = hj_key_of(t[u]);
 			pp[u] = hj_pay_of(t[u]);
 		}
-		tab.template probe<(int)RU>(kk, pp, live, B, cnt, sum);
+		tab.template probe<(int)RU>(kk, pp, live, B, acc);
 	};
 
 	for (;;) {
@@ -1079,9 +1081,10 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
 		}
 		__syncthreads();
 	}
+	acc.finish();
 	const unsigned fl = (__any(dup) ? HJ_DF_DUPKEY : 0u) | (__any(oversize) ? HJ_DF_OVERSIZE : 0u);
 	if (fl && (threadIdx.x & (HJ_WAVE - 1)) == 0) atomicOr(&st->flags, fl);
-	hj_block_add2(cnt, sum, red, &st->matched, &st->checksum);
+	hj_block_add2(acc.cnt, acc.sum, red, &st->matched, &st->checksum);
 }
 
 // ------------------------------------------------------------ host side
@@ -1238,7 +1241,8 @@ static int launch_pass(hj_ctx_t *ctx, uint32_t wv, uint32_t variant, unsigned gr
 static size_t join_static_lds(int tk, bool two_pass) {
 	// every width of one kernel template has the same __shared__ arrays
 	size_t r = 16384;
-#define S(K) r = two_pass ? static_lds((const void *)k_join2<1, 512, K>) : static_lds((const void *)k_join<1, 256, K>)
+#define S(K) \
+	r = two_pass ? static_lds((const void *)k_join2<1, 512, K, false>) : static_lds((const void *)k_join<1, 256, K, false>)
 	HJ_TK_SWITCH(tk, S)
 #undef S
 	return r;
@@ -1260,13 +1264,18 @@ static void set_scatter_attr(hj_ctx_t *ctx, uint32_t wv) {
 // LDS bytes of a bc or lp join table holding `cap` build tuples (batches).
 static size_t join_lds(uint32_t cap, bool lp) { return hj_tab_lds(lp ? HJ_TK_LP : HJ_TK_BC, cap, 1); }
 
+// Joins that emit row-id pairs (late materialisation) run on 16-byte
+// key/row-id tuples only; every other instance counts and sums.
 template <int WV, int TK>
 static void launch_join_t(hj_ctx_t *ctx, unsigned grid, size_t lds, const void *Rb, const void *Sb, const hj_pair *pairs,
                           const uint64_t *tpre, uint32_t npairs, uint32_t B, uint64_t s_tile, uint32_t cap) {
-	auto kfn = k_join<WV, 256, TK>;
+	auto kfn = k_join<WV, 256, TK, false>;
+	if constexpr (WV == 1 && TK != HJ_TK_HRO)
+		if (ctx->emit_pairs) kfn = k_join<1, 256, TK, true>;
 	allow_lds(ctx, (const void *)kfn);
 	hipLaunchKernelGGL(kfn, dim3(grid), dim3(256), lds, ctx->stream, (const uint4 *)Rb, (const uint4 *)Sb, pairs,
-	                   tpre, npairs, B, s_tile, cap, (hj_stats *)ctx->stats.p);
+	                   tpre, npairs, B, s_tile, cap, (hj_stats *)ctx->stats.p, ctx->emit_pairs,
+	                   (unsigned long long)ctx->emit_cap, ctx->emit_chunk);
 }
 
 // hro joins blocks of tuples of at most 64 bytes (phj.c:880), so it has no
@@ -1357,10 +1366,13 @@ template <int WV, int TK>
 static void launch_join2_t(hj_ctx_t *ctx, unsigned grid, size_t lds, const void *P, const uint64_t *cst,
                            const uint16_t *run_t, uint32_t stride, uint32_t b1, const hj_pair *pairs,
                            const uint64_t *tpre, uint32_t npairs, uint32_t B, uint32_t cpt, uint32_t cap) {
-	auto kfn = k_join2<WV, 512, TK>;
+	auto kfn = k_join2<WV, 512, TK, false>;
+	if constexpr (WV == 1 && TK != HJ_TK_HRO)
+		if (ctx->emit_pairs) kfn = k_join2<1, 512, TK, true>;
 	allow_lds(ctx, (const void *)kfn);
 	hipLaunchKernelGGL(kfn, dim3(grid), dim3(512), lds, ctx->stream, (const uint4 *)P, cst, run_t, stride, b1, pairs,
-	                   tpre, npairs, B, cpt, cap, (hj_stats *)ctx->stats.p);
+	                   tpre, npairs, B, cpt, cap, (hj_stats *)ctx->stats.p, ctx->emit_pairs,
+	                   (unsigned long long)ctx->emit_cap, ctx->emit_chunk);
 }
 
 static void launch_join2(hj_ctx_t *ctx, uint32_t wv, int tk, unsigned grid, size_t lds, const void *P,

@@ -70,7 +70,7 @@ double hj_event_seconds(hj_ctx_t *ctx, int a, int b) {
 static const char *k_names[HJ_K_COUNT] = {
     "fill", "lp_build", "lp_probe", "lp_fixup", "sc_build", "sc_probe",
     "radix_hist", "radix_scan", "radix_scatter", "radix_plan", "radix_join", "radix_sortchunk",
-    "radix_tilesort", "radix_gathersort",
+    "radix_tilesort", "radix_gathersort", "ptr_keyrid", "ptr_retrieve",
 };
 
 extern "C" {
@@ -132,7 +132,8 @@ int hj_ctx_destroy(hj_ctx_t *ctx) {
 	(void)hipSetDevice(ctx->device);
 	if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
 	hj_buf *bufs[] = {&ctx->table, &ctx->pool, &ctx->aux, &ctx->part[0], &ctx->part[1], &ctx->hist,
-	                  &ctx->meta, &ctx->scan_tmp, &ctx->stats, &ctx->hostcopy[0], &ctx->hostcopy[1]};
+	                  &ctx->meta, &ctx->scan_tmp, &ctx->stats, &ctx->hostcopy[0], &ctx->hostcopy[1],
+	                  &ctx->pairs, &ctx->keyrid[0], &ctx->keyrid[1]};
 	for (hj_buf *b : bufs)
 		if (b->p) (void)hipFree(b->p);
 	for (int i = 0; i < hj_ctx::NEV; i++)
@@ -141,9 +142,6 @@ int hj_ctx_destroy(hj_ctx_t *ctx) {
 	delete ctx;
 	return HJ_OK;
 }
-
-typedef int (*hj_algo_fn)(hj_ctx_t *, const hj_relation_t *, const hj_relation_t *, const hj_params_t *,
-                          hj_result_t *);
 
 // radix joins by per-partition table (src/algo/phj.c:2515-2535)
 #define HJ_RADIX_FN(NAME, TK)                                                                           \
@@ -157,6 +155,21 @@ HJ_RADIX_FN(run_radix_sc, 2)
 HJ_RADIX_FN(run_radix_hr, 3)
 HJ_RADIX_FN(run_radix_hro, 4)
 #undef HJ_RADIX_FN
+
+// late-materialising joins: the same engines on {key, row_id} heads
+// (src/algo/ptr_nphj.c, ptr_phj.c:3100-3115, ptr_phj_bw_reg.c)
+#define HJ_PTR_FN(NAME, INNER)                                                                          \
+	static int NAME(hj_ctx_t *c, const hj_relation_t *R, const hj_relation_t *S, const hj_params_t *p, \
+	                hj_result_t *o) {                                                                   \
+		return hj_run_ptr(c, R, S, p, INNER, o);                                                        \
+	}
+HJ_PTR_FN(ptr_nphj_sc, hj_run_nphj_sc)
+HJ_PTR_FN(ptr_nphj_lp, hj_run_nphj_lp)
+HJ_PTR_FN(ptr_radix_bc, run_radix_bc)
+HJ_PTR_FN(ptr_radix_lp, run_radix_lp)
+HJ_PTR_FN(ptr_radix_sc, run_radix_sc)
+HJ_PTR_FN(ptr_radix_hr, run_radix_hr)
+#undef HJ_PTR_FN
 
 struct hj_algo_entry {
 	const char *name;
@@ -176,6 +189,16 @@ static const hj_algo_entry k_algos[] = {
     {"phj_radix_bw_reg_sc", run_radix_sc},
     {"phj_radix_bw_reg_hr", run_radix_hr},
     {"phj_radix_bw_reg_hro", run_radix_hro},
+    {"ptr_nphj_sc", ptr_nphj_sc},
+    {"ptr_nphj_lp", ptr_nphj_lp},
+    {"ptr_phj_radix_bc", ptr_radix_bc},
+    {"ptr_phj_radix_sc", ptr_radix_sc},
+    {"ptr_phj_radix_lp", ptr_radix_lp},
+    {"ptr_phj_radix_hr", ptr_radix_hr},
+    {"ptr_phj_radix_bw_reg_bc", ptr_radix_bc},
+    {"ptr_phj_radix_bw_reg_sc", ptr_radix_sc},
+    {"ptr_phj_radix_bw_reg_lp", ptr_radix_lp},
+    {"ptr_phj_radix_bw_reg_hr", ptr_radix_hr},
     {nullptr, nullptr},
 };
 
@@ -197,6 +220,9 @@ bool hj_algo_is_lp(const char *algo, bool *radix) {
 }
 
 static bool width_ok(uint32_t w) { return w == 16 || w == 32 || w == 64 || w == 128 || w == 256; }
+// late materialisation reads only the 16-byte head and one row id of a
+// tuple, so it also takes the reference's widest payload tuples
+static bool ptr_width_ok(uint32_t w) { return width_ok(w) || w == 512; }
 
 int hj_join(hj_ctx_t *ctx, const char *algo, const hj_relation_t *R, const hj_relation_t *S,
             const hj_params_t *params, hj_result_t *out) {
@@ -206,9 +232,10 @@ int hj_join(hj_ctx_t *ctx, const char *algo, const hj_relation_t *R, const hj_re
 	for (const hj_algo_entry *a = k_algos; a->name; a++)
 		if (strcmp(a->name, algo) == 0) e = a;
 	if (!e) return hj_fail(ctx, HJ_EINVAL, "unknown algo '%s'", algo);
-	if (!width_ok(R->tuple_size) || R->tuple_size != S->tuple_size)
-		return hj_fail(ctx, HJ_EINVAL, "tuple sizes %u/%u unsupported (16..256, equal)", R->tuple_size,
-		               S->tuple_size);
+	const bool ptr = strncmp(algo, "ptr_", 4) == 0;
+	if (!(ptr ? ptr_width_ok(R->tuple_size) : width_ok(R->tuple_size)) || R->tuple_size != S->tuple_size)
+		return hj_fail(ctx, HJ_EINVAL, "tuple sizes %u/%u unsupported (16..%u, equal)", R->tuple_size,
+		               S->tuple_size, ptr ? 512u : 256u);
 	if ((R->tuple_num && !R->tuples) || (S->tuple_num && !S->tuples))
 		return hj_fail(ctx, HJ_EINVAL, "null tuples pointer");
 	if (R->location > HJ_LOC_HOST || S->location > HJ_LOC_HOST) return hj_fail(ctx, HJ_EINVAL, "bad location");
@@ -290,7 +317,7 @@ int hj_sync(hj_ctx_t *ctx) {
 
 int hj_generate(hj_ctx_t *ctx, const hj_gen_spec_t *spec, void *r_dev, void *s_dev) {
 	if (!ctx || !spec) return HJ_EINVAL;
-	if (!width_ok(spec->width)) return hj_fail(ctx, HJ_EINVAL, "width %u unsupported", spec->width);
+	if (!ptr_width_ok(spec->width)) return hj_fail(ctx, HJ_EINVAL, "width %u unsupported", spec->width);
 	if ((spec->r_num && !r_dev) || (spec->s_num && !s_dev)) return hj_fail(ctx, HJ_EINVAL, "null buffer");
 	if (spec->workload > HJ_WL_RANDOM) return hj_fail(ctx, HJ_EINVAL, "unknown workload %u", spec->workload);
 	if (spec->workload == HJ_WL_RANDOM && spec->key_range == 0)

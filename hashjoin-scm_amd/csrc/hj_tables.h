@@ -165,9 +165,9 @@ struct hj_tab {
 	}
 
 	// Probe RU keys at once (independent LDS reads overlap); adds matches.
-	template <int RU>
+	template <int RU, class Sink>
 	__device__ void probe(const uint64_t (&kk)[RU], const uint64_t (&pp)[RU], const bool (&live)[RU], uint32_t B,
-	                      unsigned long long &cnt, unsigned long long &sum) const {
+	                      Sink &sink) const {
 		uint32_t cur[RU], end[RU];
 		if (TK == HJ_TK_BC || TK == HJ_TK_SC) {
 			if (TK == HJ_TK_SC) {
@@ -176,10 +176,7 @@ struct hj_tab {
 					if (!live[u]) continue;
 					const uint32_t h = bucket(kk[u], B), n = min(heads[h], 2u);
 					for (uint32_t s = 0; s < n; s++)
-						if (bkey[2 * h + s] == kk[u]) {
-							cnt++;
-							sum += bpay[2 * h + s] + pp[u];
-						}
+						if (bkey[2 * h + s] == kk[u]) sink.match(bpay[2 * h + s], pp[u]);
 				}
 			}
 			// chain cursors (1-based entries of keys/pays, 0 = end)
@@ -193,10 +190,7 @@ struct hj_tab {
 				for (int u = 0; u < RU; u++) {
 					if (!cur[u]) continue;
 					const uint32_t i = cur[u] - 1;
-					if (keys[i] == kk[u]) {
-						cnt++;
-						sum += pays[i] + pp[u];
-					}
+					if (keys[i] == kk[u]) sink.match(pays[i], pp[u]);
 					cur[u] = next[i];
 					any |= cur[u] != 0;
 				}
@@ -217,8 +211,7 @@ struct hj_tab {
 					if (!o) {
 						act[u] = false;
 					} else if (keys[o - 1] == kk[u]) {
-						cnt++;
-						sum += pays[o - 1] + pp[u];
+						sink.match(pays[o - 1], pp[u]);
 						act[u] = false;
 					} else {
 						cur[u] = (cur[u] + 1) & hmask;
@@ -243,24 +236,15 @@ struct hj_tab {
 				for (int u = 0; u < RU; u++) {
 					if (cur[u] >= end[u]) continue;
 					if (TK == HJ_TK_HR) {
-						if (keys[cur[u]] == kk[u]) {
-							cnt++;
-							sum += pays[cur[u]] + pp[u];
-						}
+						if (keys[cur[u]] == kk[u]) sink.match(pays[cur[u]], pp[u]);
 						cur[u]++;
 					} else {
 						// keys[lo] and keys[lo + 1] in one read; a key before
 						// the range (odd start) or past its end is skipped
 						const uint32_t lo = cur[u];
 						const uint4 two = *(const uint4 *)&keys[lo];
-						if (lo >= beg[u] && hj_key_of(two) == kk[u]) {
-							cnt++;
-							sum += pays[lo] + pp[u];
-						}
-						if (lo + 1 < end[u] && hj_pay_of(two) == kk[u]) {
-							cnt++;
-							sum += pays[lo + 1] + pp[u];
-						}
+						if (lo >= beg[u] && hj_key_of(two) == kk[u]) sink.match(pays[lo], pp[u]);
+						if (lo + 1 < end[u] && hj_pay_of(two) == kk[u]) sink.match(pays[lo + 1], pp[u]);
 						cur[u] += 2;
 					}
 					any |= cur[u] < end[u];

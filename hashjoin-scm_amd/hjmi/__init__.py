@@ -33,6 +33,11 @@ WORKLOADS = {"pkfk": WL_PKFK_UNIFORM, "pkfk_ref": WL_PKFK_REF, "payload_ref": WL
 ALGOS = ("nphj_sc", "nphj_lp", "phj_radix_bc", "phj_radix_lp", "phj_radix_sc", "phj_radix_hr", "phj_radix_hro",
          "phj_radix_bw_reg_bc", "phj_radix_bw_reg_lp", "phj_radix_bw_reg_sc", "phj_radix_bw_reg_hr",
          "phj_radix_bw_reg_hro")
+# late materialisation (src/algo/ptr_*.c): the join emits row-id pairs and a
+# retrieve pass computes check_sum = sum R[r_row_id-1].row_id + S[s_row_id-1].row_id
+PTR_ALGOS = ("ptr_nphj_sc", "ptr_nphj_lp", "ptr_phj_radix_bc", "ptr_phj_radix_sc", "ptr_phj_radix_lp",
+             "ptr_phj_radix_hr", "ptr_phj_radix_bw_reg_bc", "ptr_phj_radix_bw_reg_sc", "ptr_phj_radix_bw_reg_lp",
+             "ptr_phj_radix_bw_reg_hr")
 MAX_KTIMES = 16
 
 

@@ -44,6 +44,7 @@ static const wl_t k_workloads[] = {
     {"payload", "pkfk", "64", 1ull << 24, 1ull << 28, 64, HJ_WL_PAYLOAD_REF},
     {"payload", "pkfk", "128", 1ull << 24, 1ull << 28, 128, HJ_WL_PAYLOAD_REF},
     {"payload", "pkfk", "256", 1ull << 24, 1ull << 28, 256, HJ_WL_PAYLOAD_REF},
+    {"payload", "pkfk", "512", 1ull << 24, 1ull << 28, 512, HJ_WL_PAYLOAD_REF},  /* ptr_* joins only */
     {"fndly", "pkfk", "Q", 1ull << 28, 1ull << 30, 16, HJ_WL_PKFK_REF},
     {"fndly", "pkfk", "R", 1ull << 28, 1ull << 31, 16, HJ_WL_PKFK_REF},
     {"fndly", "pkfk", "H", 1ull << 28, 1ull << 32, 16, HJ_WL_PKFK_REF},

@@ -46,7 +46,8 @@ extern "C" {
 typedef struct {
 	void *tuples;         /* packed AoS records {u64 key; u64 row_id; u8 rest[]} */
 	uint64_t tuple_num;
-	uint32_t tuple_size;  /* bytes per tuple: 16, 32, 64, 128 or 256 */
+	uint32_t tuple_size;  /* bytes per tuple: 16, 32, 64, 128 or 256 (and 512
+	                         for the ptr_* joins and the generator)      */
 	uint32_t location;    /* HJ_LOC_* */
 } hj_relation_t;
 
@@ -103,9 +104,16 @@ const char *hj_strerror(int status);
 const char *hj_last_error(hj_ctx_t *ctx);   /* detail of the last failure */
 
 /* Algorithm registry, same names as the reference's algos[] table
- * (src/main.c:72-143): nphj_sc, nphj_lp, phj_radix_bc, phj_radix_lp,
- * phj_radix_bw_reg_bc, phj_radix_bw_reg_lp.  Index past the end -> NULL
- * (the reference table has no sentinel, src/main.c:206). */
+ * (src/main.c:72-143): nphj_sc, nphj_lp, phj_radix_{bc,lp,sc,hr,hro},
+ * phj_radix_bw_reg_{bc,lp,sc,hr,hro}, and the late-materialising
+ * ptr_nphj_{sc,lp}, ptr_phj_radix_{bc,sc,lp,hr},
+ * ptr_phj_radix_bw_reg_{bc,sc,lp,hr} (src/main.c:87-92, :119-122).  A ptr_
+ * join matches the 16-byte {key, row_id} heads, writes one row-id pair per
+ * match and returns check_sum = sum of R[r_row_id-1].row_id +
+ * S[s_row_id-1].row_id over the pairs (src/inc/utils.c:712-723); row ids
+ * must be 1-based positions (otherwise HJ_EINVAL, counting the bad pairs).
+ * Index past the end -> NULL (the reference table has no sentinel,
+ * src/main.c:206). */
 int hj_algo_count(void);
 const char *hj_algo_name(int idx);
 const char *hj_kernel_name(uint32_t id);

@@ -904,3 +904,106 @@ int or_bruteforce(const void *Rv, uint64_t nr, const void *Sv, uint64_t ns, uint
 	out->buildpart_time = out->probejoin_time = out->total_time = 0;
 	return 0;
 }
+
+/* ---- late materialisation (src/algo/ptr_nphj.c, ptr_phj.c) ---------------
+ * The reference joins {key, row_id} heads, writes one row_id_pair_t
+ * {r.row_id, s.row_id} per match, then ptr_retrieve (src/inc/utils.c:
+ * 712-723) adds R[r_row_id - 1].row_id + S[s_row_id - 1].row_id over the
+ * pairs.  A row id outside 1..n is an error here (the reference reads out of
+ * bounds). */
+
+int or_ptr_retrieve(const uint64_t *pairs, uint64_t npairs, const void *Rv, uint64_t nr, const void *Sv,
+                    uint64_t ns, uint32_t W, uint64_t *checksum) {
+	const uint8_t *R = (const uint8_t *)Rv, *S = (const uint8_t *)Sv;
+	uint64_t sum = 0;
+	for (uint64_t k = 0; k < npairs; k++) {
+		const uint64_t r = pairs[2 * k], s = pairs[2 * k + 1];
+		if (r - 1 >= nr || s - 1 >= ns) return -1;
+		sum += rid_at(R, r - 1, W) + rid_at(S, s - 1, W);
+	}
+	*checksum = sum;
+	return 0;
+}
+
+/* Literal pipeline on small inputs: nested loop emitting the pairs (lp: the
+ * first R row of the key), then the retrieve. */
+int or_ptr_bruteforce(const void *Rv, uint64_t nr, const void *Sv, uint64_t ns, uint32_t W, int lp,
+                      or_result_t *out) {
+	const uint8_t *R = (const uint8_t *)Rv, *S = (const uint8_t *)Sv;
+	uint64_t cap = 1024, n = 0;
+	uint64_t *pairs = (uint64_t *)malloc(cap * 16);
+	if (!pairs) return -3;
+	for (uint64_t j = 0; j < ns; j++) {
+		const uint64_t sk = key_at(S, j, W);
+		for (uint64_t i = 0; i < nr; i++) {
+			if (key_at(R, i, W) != sk) continue;
+			if (n == cap) {
+				uint64_t *p = (uint64_t *)realloc(pairs, cap * 32);
+				if (!p) {
+					free(pairs);
+					return -3;
+				}
+				pairs = p;
+				cap *= 2;
+			}
+			pairs[2 * n] = rid_at(R, i, W);
+			pairs[2 * n + 1] = rid_at(S, j, W);
+			n++;
+			if (lp) break;
+		}
+	}
+	uint64_t sum = 0;
+	const int rc = or_ptr_retrieve(pairs, n, Rv, nr, Sv, ns, W, &sum);
+	free(pairs);
+	if (rc) return rc;
+	out->matched_cnt = n;
+	out->check_sum = sum;
+	out->buildpart_time = out->probejoin_time = out->total_time = 0;
+	return 0;
+}
+
+/* The restated joins on the heads.  The retrieve sum is additive over pairs,
+ * so it equals the plain join checksum of heads whose row_id is replaced by
+ * the row id it points at: {key, rel[row_id - 1].row_id}.  A second join of
+ * heads carrying 1 for an out-of-range row id (else 0) sums the pairs'
+ * out-of-range ends, so a non-zero sum is the reference's out-of-bounds read.
+ * algo: 0 nphj_sc, 1 nphj_lp, 2 phj_radix with per-partition `kind`. */
+static int ptr_heads(const uint8_t *rel, uint64_t n, uint32_t W, int flag_only, uint64_t *out) {
+	for (uint64_t i = 0; i < n; i++) {
+		const uint64_t r = rid_at(rel, i, W);
+		const int ok = r - 1 < n;
+		out[2 * i] = key_at(rel, i, W);
+		out[2 * i + 1] = flag_only ? (uint64_t)!ok : (ok ? rid_at(rel, r - 1, W) : 0);
+	}
+	return 0;
+}
+
+static int ptr_inner(int algo, int kind, uint32_t bits, uint32_t passes, const uint64_t *r, uint64_t nr,
+                     const uint64_t *s, uint64_t ns, int threads, or_result_t *out) {
+	if (algo == 0) return or_nphj_sc(r, nr, s, ns, 16, threads, out);
+	if (algo == 1) return or_nphj_lp(r, nr, s, ns, 16, threads, out);
+	return or_phj_radix(r, nr, s, ns, 16, bits, passes, kind, threads, out);
+}
+
+int or_ptr_join(int algo, int kind, uint32_t radix_bits, uint32_t passes, const void *Rv, uint64_t nr,
+                const void *Sv, uint64_t ns, uint32_t W, int threads, or_result_t *out) {
+	uint64_t *r = (uint64_t *)malloc((nr ? nr : 1) * 16), *s = (uint64_t *)malloc((ns ? ns : 1) * 16);
+	if (!r || !s) {
+		free(r);
+		free(s);
+		return -3;
+	}
+	or_result_t flag;
+	ptr_heads((const uint8_t *)Rv, nr, W, 1, r);
+	ptr_heads((const uint8_t *)Sv, ns, W, 1, s);
+	int rc = ptr_inner(algo, kind, radix_bits, passes, r, nr, s, ns, threads, &flag);
+	if (!rc && flag.check_sum) rc = -1;
+	if (!rc) {
+		ptr_heads((const uint8_t *)Rv, nr, W, 0, r);
+		ptr_heads((const uint8_t *)Sv, ns, W, 0, s);
+		rc = ptr_inner(algo, kind, radix_bits, passes, r, nr, s, ns, threads, out);
+	}
+	free(r);
+	free(s);
+	return rc;
+}

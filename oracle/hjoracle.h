@@ -57,6 +57,18 @@ int or_phj_radix(const void *R, uint64_t nr, const void *S, uint64_t ns, uint32_
 int or_radix_partition(const void *in, uint64_t n, uint32_t width, uint32_t shift, uint32_t bits, int threads,
                        void *out, uint64_t *hist);
 
+/* late materialisation (src/algo/ptr_*.c): count of pairs and the retrieve
+ * checksum sum(R[r_row_id-1].row_id + S[s_row_id-1].row_id) of
+ * src/inc/utils.c:712-723.  algo 0 nphj_sc, 1 nphj_lp, 2 phj_radix (kind as
+ * above).  Returns -1 when a matched row id lies outside its relation. */
+int or_ptr_join(int algo, int kind, uint32_t radix_bits, uint32_t passes, const void *R, uint64_t nr,
+                const void *S, uint64_t ns, uint32_t width, int threads, or_result_t *out);
+/* the literal pipeline for small inputs: emit pairs by nested loop, retrieve */
+int or_ptr_bruteforce(const void *R, uint64_t nr, const void *S, uint64_t ns, uint32_t width, int lp,
+                      or_result_t *out);
+int or_ptr_retrieve(const uint64_t *pairs, uint64_t npairs, const void *R, uint64_t nr, const void *S,
+                    uint64_t ns, uint32_t width, uint64_t *checksum);
+
 /* independent checkers */
 int or_aggregate(const void *R, uint64_t nr, const void *S, uint64_t ns, uint32_t width, int lp, or_result_t *out);
 int or_bruteforce(const void *R, uint64_t nr, const void *S, uint64_t ns, uint32_t width, int lp, or_result_t *out);

@@ -57,9 +57,11 @@ def lib():
         L.or_phj_radix.restype = I
         L.or_radix_partition.argtypes = [P, U64, U32, U32, U32, I, P, P]
         L.or_radix_partition.restype = I
-        for name in ("or_aggregate", "or_bruteforce"):
+        for name in ("or_aggregate", "or_bruteforce", "or_ptr_bruteforce"):
             getattr(L, name).argtypes = [P, U64, P, U64, U32, I, C.POINTER(OrResult)]
             getattr(L, name).restype = I
+        L.or_ptr_join.argtypes = [I, I, U32, U32, P, U64, P, U64, U32, I, C.POINTER(OrResult)]
+        L.or_ptr_join.restype = I
         _lib = L
     return _lib
 
@@ -106,6 +108,56 @@ def join(algo: str, R: np.ndarray, S: np.ndarray, width: int, threads: int = 8, 
         raise ValueError(algo)
     if rc:
         raise RuntimeError(f"oracle {algo} failed: {rc}")
+    return _res(out)
+
+
+class RowIdError(ValueError):
+    """A matched row id lies outside 1..|relation| (the reference's retrieve
+    would read out of bounds, src/inc/utils.c:712-723)."""
+
+
+# late-materialising joins (src/main.c:87-122): inner engine and partition kind
+PTR_ALGOS = ["ptr_nphj_sc", "ptr_nphj_lp", "ptr_phj_radix_bc", "ptr_phj_radix_sc", "ptr_phj_radix_lp",
+             "ptr_phj_radix_hr", "ptr_phj_radix_bw_reg_bc", "ptr_phj_radix_bw_reg_sc", "ptr_phj_radix_bw_reg_lp",
+             "ptr_phj_radix_bw_reg_hr"]
+
+
+def ptr_join(algo: str, R: np.ndarray, S: np.ndarray, width: int, threads: int = 8, radix_bits: int = 11,
+             passes: int = 1) -> dict:
+    """Late materialisation: pair count and retrieve checksum of `algo`
+    (a ptr_* name) restated on host tuples."""
+    inner = algo.replace("_bw_reg", "")
+    if not inner.startswith("ptr_"):
+        raise ValueError(algo)
+    inner = inner[4:]
+    if inner == "nphj_sc":
+        a, kind = 0, 0
+    elif inner == "nphj_lp":
+        a, kind = 1, 0
+    elif inner in RADIX_KINDS and inner != "phj_radix_hro":
+        a, kind = 2, RADIX_KINDS[inner]
+    else:
+        raise ValueError(algo)
+    out = OrResult()
+    rc = lib().or_ptr_join(a, kind, radix_bits, passes, _ptr(R), R.nbytes // width, _ptr(S), S.nbytes // width,
+                           width, threads, C.byref(out))
+    if rc == -1:
+        raise RowIdError(f"{algo}: a matched row id lies outside its relation")
+    if rc:
+        raise RuntimeError(f"oracle {algo} failed: {rc}")
+    return _res(out)
+
+
+def ptr_bruteforce(R: np.ndarray, S: np.ndarray, width: int, lp: bool = False) -> dict:
+    """The literal late-materialisation pipeline on small inputs: nested-loop
+    pair emission, then the retrieve of src/inc/utils.c:712-723."""
+    out = OrResult()
+    rc = lib().or_ptr_bruteforce(_ptr(R), R.nbytes // width, _ptr(S), S.nbytes // width, width, int(lp),
+                                 C.byref(out))
+    if rc == -1:
+        raise RowIdError("a matched row id lies outside its relation")
+    if rc:
+        raise RuntimeError(f"ptr_bruteforce failed: {rc}")
     return _res(out)
 
 
@@ -165,4 +217,24 @@ def direct_address(R: np.ndarray, S: np.ndarray, width: int) -> dict:
     P[rk.astype(np.int64)] = rp
     with np.errstate(over="ignore"):
         s = int(P[sk.astype(np.int64)].sum(dtype=np.uint64)) + int(sp.sum(dtype=np.uint64))
+    return {"matched_cnt": int(sk.size), "check_sum": s & M64}
+
+
+def ptr_direct_address(R: np.ndarray, S: np.ndarray, width: int) -> dict:
+    """Late materialisation when R keys are unique and every S key is in R:
+    the pair of s is (r.row_id, s.row_id) with r the R row of s.key, and the
+    retrieve adds R[r.row_id - 1].row_id + S[s.row_id - 1].row_id.  O(N)."""
+    rk, rp = keys_payloads(R, width)
+    sk, sp = keys_payloads(S, width)
+    order = np.argsort(rk, kind="stable")
+    ks = rk[order]
+    idx = np.searchsorted(ks, sk)
+    if np.any(idx >= ks.size) or np.any(ks[np.minimum(idx, ks.size - 1)] != sk):
+        raise ValueError("an S key is missing from R")
+    r_rid = rp[order[idx]]
+    if np.any(r_rid - 1 >= rp.size) or np.any(sp - 1 >= sp.size):
+        raise RowIdError("a matched row id lies outside its relation")
+    with np.errstate(over="ignore"):
+        s = int(rp[(r_rid - 1).astype(np.int64)].sum(dtype=np.uint64)) + int(
+            sp[(sp - 1).astype(np.int64)].sum(dtype=np.uint64))
     return {"matched_cnt": int(sk.size), "check_sum": s & M64}

@@ -242,3 +242,109 @@ def test_generator_is_seeded_and_threads_do_not_matter():
     c = O.generate("zipf", 3000, 9000, 64, theta=0.5, seed=99)
     assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
     assert not np.array_equal(a[1], c[1])
+
+
+# ------------------------------------------------------- late materialisation
+PTR = O.PTR_ALGOS
+
+
+def test_logged_ptr_vectors_are_the_payload_closed_form():
+    """All twelve logged ptr_* results (ptr_nphj_sc and ptr_phj_radix_bc over
+    the payload sweep 16..512 B, 2^24 x 2^28)
+    equal |S|(|S|+|R|+2)/2: with row_id = position the retrieve returns the
+    plain checksum."""
+    n = 0
+    for r in _records():
+        if r["algo"].startswith("ptr_"):
+            c = r["configs"][0]
+            cf = O.payload_ref_closed_form(c["r_tuple_num"], c["s_tuple_num"])
+            assert (r["matched_cnt"], r["check_sum"]) == (cf["matched_cnt"], cf["check_sum"]), r
+            n += 1
+    assert n == 12
+
+
+def test_ptr_retrieve_known_answer():
+    """Hand-worked: R = {(5, rid 2), (7, rid 1)}, S = {(5, 3), (7, 1), (5, 2),
+    (9, 1)}.  Pairs (2,3), (1,1), (2,2); retrieve R[1].rid + S[2].rid = 1+2,
+    R[0].rid + S[0].rid = 2+3, R[1].rid + S[1].rid = 1+1: count 3, sum 10."""
+    R = np.array([5, 2, 7, 1], dtype=np.uint64).view(np.uint8)
+    S = np.array([5, 3, 7, 1, 5, 2, 9, 1], dtype=np.uint64).view(np.uint8)
+    want = {"matched_cnt": 3, "check_sum": 10}
+    for lp in (False, True):
+        got = O.ptr_bruteforce(R, S, 16, lp=lp)
+        assert (got["matched_cnt"], got["check_sum"]) == (3, 10)
+    for algo in PTR:
+        got = O.ptr_join(algo, R, S, 16, threads=2, radix_bits=1, passes=1)
+        assert (got["matched_cnt"], got["check_sum"]) == (want["matched_cnt"], want["check_sum"]), algo
+
+
+@pytest.mark.parametrize("width", [16, 64, 512])
+def test_ptr_restated_joins_on_payload_data(width):
+    nr, ns = 1 << 9, 1 << 13
+    R, S = O.generate("payload_ref", nr, ns, width)
+    want = O.payload_ref_closed_form(nr, ns)
+    got = O.ptr_bruteforce(R, S, width)
+    assert (got["matched_cnt"], got["check_sum"]) == (want["matched_cnt"], want["check_sum"])
+    for algo in PTR:
+        for bits, passes in ((5, 1), (8, 2)):
+            got = O.ptr_join(algo, R, S, width, threads=4, radix_bits=bits, passes=passes)
+            assert (got["matched_cnt"], got["check_sum"]) == (want["matched_cnt"], want["check_sum"]), algo
+
+
+def pointer_relations(seed, nr, ns, width, key_range):
+    """Random keys (duplicates and misses on both sides) whose row ids point
+    at random rows of their own relation, so the retrieve differs from the
+    plain checksum."""
+    R, S = O.generate("random", nr, ns, width, seed=seed, key_range=key_range)
+    rng = np.random.default_rng(seed)
+    for rel, n in ((R, nr), (S, ns)):
+        w = rel.view(np.uint64).reshape(n, width // 8)
+        w[:, 1] = rng.integers(1, n + 1, size=n, dtype=np.uint64)
+    return R, S
+
+
+@pytest.mark.parametrize("seed,nr,ns,kr,width", [(1, 300, 2000, 100, 16), (2, 1000, 3000, 5000, 32),
+                                                  (3, 64, 64, 8, 16), (4, 700, 900, 700, 64)])
+def test_ptr_restated_joins_match_literal_pipeline(seed, nr, ns, kr, width):
+    R, S = pointer_relations(seed, nr, ns, width, kr)
+    want_all = O.ptr_bruteforce(R, S, width)
+    want_lp = O.ptr_bruteforce(R, S, width, lp=True)
+    for algo in PTR:
+        want = want_lp if algo.endswith("_lp") else want_all
+        # a concurrent nphj_lp build keeps an arbitrary duplicate; one thread
+        # keeps the first in R order
+        th = 1 if algo == "ptr_nphj_lp" else 3
+        for bits, passes in ((3, 1), (6, 2)):
+            got = O.ptr_join(algo, R, S, width, threads=th, radix_bits=bits, passes=passes)
+            assert (got["matched_cnt"], got["check_sum"]) == (want["matched_cnt"], want["check_sum"]), algo
+
+
+def test_ptr_retrieve_dereferences_row_ids():
+    """Planted mistake: a retrieve that added the pair's row ids themselves
+    would give the plain join checksum.  With row ids that are not positions
+    the two differ, so the parity tests above catch it."""
+    R, S = pointer_relations(5, 500, 3000, 16, 200)
+    plain, ptr = O.bruteforce(R, S, 16), O.ptr_bruteforce(R, S, 16)
+    assert plain["matched_cnt"] == ptr["matched_cnt"] and plain["check_sum"] != ptr["check_sum"]
+    assert O.ptr_join("ptr_phj_radix_bc", R, S, 16, radix_bits=4)["check_sum"] == ptr["check_sum"]
+
+
+def test_ptr_direct_address_matches_literal_pipeline():
+    R, S = O.generate("pkfk", 1 << 10, 1 << 13, 32, seed=9)
+    want = O.ptr_bruteforce(R, S, 32)
+    assert O.ptr_direct_address(R, S, 32) == {"matched_cnt": want["matched_cnt"], "check_sum": want["check_sum"]}
+    assert O.ptr_direct_address(R, S, 32)["check_sum"] != O.pkfk_closed_form(S, 32)["check_sum"]
+
+
+def test_ptr_rejects_only_matched_out_of_range_row_ids():
+    R = np.array([5, 2, 7, 1], dtype=np.uint64)
+    S = np.array([5, 1, 9, 99], dtype=np.uint64)  # row 1 (key 9) misses: its bad id is never read
+    got = O.ptr_join("ptr_nphj_sc", R.view(np.uint8), S.view(np.uint8), 16)
+    assert (got["matched_cnt"], got["check_sum"]) == (1, 1 + 1)  # pair (2, 1): R[1].rid + S[0].rid
+    for bad in (0, 3):
+        S2 = S.copy()
+        S2[1] = bad  # matched row id 0 or past |S|
+        with pytest.raises(O.RowIdError):
+            O.ptr_join("ptr_phj_radix_bc", R.view(np.uint8), S2.view(np.uint8), 16, radix_bits=1)
+        with pytest.raises(O.RowIdError):
+            O.ptr_bruteforce(R.view(np.uint8), S2.view(np.uint8), 16)
