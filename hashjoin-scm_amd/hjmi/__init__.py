@@ -92,6 +92,22 @@ class ServeStats(C.Structure):
                 ("peak_admitted_bytes", C.c_uint64), ("solo_completed", C.c_uint64)]
 
 
+class RefSpec(C.Structure):
+    _fields_ = [("kind", C.c_uint32), ("tuple_size", C.c_uint32), ("r_num", C.c_uint64), ("s_num", C.c_uint64),
+                ("zipf_factor", C.c_double), ("seed", C.c_int32), ("exact_order", C.c_uint32)]
+
+
+class DataCfg(C.Structure):
+    _fields_ = [("r_path_suffix", C.c_char * 256), ("s_path_suffix", C.c_char * 256),
+                ("workload_name", C.c_char * 128), ("r_tuple_num", C.c_uint64), ("s_tuple_num", C.c_uint64),
+                ("r_tuple_size", C.c_uint32), ("s_tuple_size", C.c_uint32), ("theta", C.c_double),
+                ("selectivity", C.c_double), ("density", C.c_double), ("sparsity", C.c_double),
+                ("min_key", C.c_int64), ("max_key", C.c_int64)]
+
+
+HJ_REF_PKFK, HJ_REF_DUPFK, HJ_REF_ZIPF, HJ_REF_PAYLOAD = 0, 1, 2, 3
+
+
 class HJError(RuntimeError):
     def __init__(self, code: int, msg: str):
         super().__init__(f"{msg} (status {code})")
@@ -142,6 +158,12 @@ def load_library(path: str = LIB_PATH):
         "hj_serve_drain": (I, [P]),
         "hj_serve_stats": (I, [P, C.POINTER(ServeStats)]),
         "hj_serve_close": (I, [P]),
+        "hj_ref_spec_for": (I, [C.c_char_p, U32, C.POINTER(RefSpec)]),
+        "hj_ref_generate": (I, [C.POINTER(RefSpec), P, P]),
+        "hj_ref_rand": (I, [C.c_int32, U64, P]),
+        "hj_data_cfg_read": (I, [C.c_char_p, C.POINTER(DataCfg)]),
+        "hj_bin_read": (I, [P, C.c_char_p, U64, U32, P]),
+        "hj_bin_write": (I, [P, C.c_char_p, P, U64, U32]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -157,7 +179,8 @@ def exported_symbols():
             "hj_copy_d2h", "hj_sync", "hj_generate", "hj_device_info", "hj_copy_d2d", "hj_batch_create",
             "hj_batch_run", "hj_batch_destroy", "hj_autotune", "hj_tuned_config", "hj_serve_open",
             "hj_serve_register_build", "hj_serve_submit", "hj_serve_poll", "hj_serve_drain", "hj_serve_stats",
-            "hj_serve_close"]
+            "hj_serve_close", "hj_ref_spec_for", "hj_ref_generate", "hj_ref_rand", "hj_data_cfg_read",
+            "hj_bin_read", "hj_bin_write"]
 
 
 @dataclass
@@ -301,6 +324,17 @@ class Context:
                     f"hj_join({algo})")
         return result_dict(res)
 
+    def bin_read(self, path: str, tuple_num: int, tuple_size: int) -> DeviceRelation:
+        """Load a raw tuple file (the reference's .bin layout) into HBM."""
+        ptr = self.alloc(max(tuple_num * tuple_size, 1))
+        self._check(self.lib.hj_bin_read(self.h, path.encode(), tuple_num, tuple_size, C.c_void_p(ptr)),
+                    "hj_bin_read")
+        return DeviceRelation(ptr, tuple_num, tuple_size, self)
+
+    def bin_write(self, path: str, rel: DeviceRelation):
+        self._check(self.lib.hj_bin_write(self.h, path.encode(), C.c_void_p(rel.ptr), rel.tuple_num,
+                                          rel.tuple_size), "hj_bin_write")
+
     def batch(self, algo: str, Rs, Ss) -> "Batch":
         return Batch(self, algo, Rs, Ss)
 
@@ -336,6 +370,48 @@ def result_dict(res: Result) -> dict:
         "total_time": res.total_time, "algo_bytes": res.algo_bytes, "radix_bits": res.radix_bits,
         "passes": res.passes, "fallback": res.fallback, "kernel_ms": kt,
     }
+
+
+# ------------------------------------------------- the reference's datasets
+def ref_spec(workload_name: str, tuple_size: int = 16, exact_order: bool = False) -> RefSpec:
+    """Generator spec of a reference workload ("uniform_dupfk_D",
+    "zipfian_1.250_A", "payload_pkfk_64", ...)."""
+    sp = RefSpec()
+    rc = load_library().hj_ref_spec_for(workload_name.encode(), tuple_size, C.byref(sp))
+    if rc != HJ_OK:
+        raise HJError(rc, f"hj_ref_spec_for({workload_name})")
+    sp.exact_order = int(exact_order)
+    return sp
+
+
+def ref_generate(spec: RefSpec):
+    """Host tuples (uint8 arrays R, S) drawn as the reference's generator
+    draws them."""
+    r = np.empty(spec.r_num * spec.tuple_size, dtype=np.uint8)
+    s = np.empty(spec.s_num * spec.tuple_size, dtype=np.uint8)
+    rc = load_library().hj_ref_generate(C.byref(spec), r.ctypes.data_as(C.c_void_p), s.ctypes.data_as(C.c_void_p))
+    if rc != HJ_OK:
+        raise HJError(rc, "hj_ref_generate")
+    return r, s
+
+
+def ref_rand(seed: int, n: int) -> np.ndarray:
+    out = np.empty(n, dtype=np.int32)
+    load_library().hj_ref_rand(seed, n, out.ctypes.data_as(C.c_void_p))
+    return out
+
+
+def read_data_cfg(path: str) -> dict:
+    """A reference data configuration file as a dict of its settings."""
+    c = DataCfg()
+    rc = load_library().hj_data_cfg_read(path.encode(), C.byref(c))
+    if rc != HJ_OK:
+        raise HJError(rc, f"hj_data_cfg_read({path})")
+    out = {}
+    for name, _ in DataCfg._fields_:
+        v = getattr(c, name)
+        out[name] = v.decode() if isinstance(v, bytes) else v
+    return out
 
 
 def algo_names():

@@ -8,15 +8,21 @@
  * up by name in the library's registry (src/main.c:204-218, unknown name ->
  * error exit), and (workload, param, subtype) name a workload of the
  * reference's config tree (src/main.c:244-255, config/data/<w>/<p>_<s>.cfg).
- * Instead of reading pmem files, the relations are generated on the GPU with
- * the same layout and closed-form results as the reference's generator.
+ * The relations are generated on the GPU with the same layout and
+ * closed-form results as the reference's generator; the dupfk and Zipf
+ * workloads, whose results depend on glibc's draws, come from the library's
+ * restatement of the reference generator (hj_ref_generate) on the host, as
+ * does any reference workload with --exact (the reference's tuple order
+ * too).  --cfg=FILE --data-prefix=DIR instead reads a reference data
+ * configuration and its raw .bin files (src/algo/phj.c:1885-1915), and
+ * --write-bin=PREFIX writes the relations as PREFIX_R.bin / PREFIX_S.bin.
  * The trial protocol is src/main.c:429-459: TRIALS runs, each printing
  * matched_cnt/check_sum and the phase times, then the mean of trials 2..N.
  *
  * Extra options (all optional):
  *   --r=N --s=N --width=W --gen=pkfk|pkfk_ref|payload_ref|zipf|random
  *   --theta=T --key-range=K --seed=S --bits=B --passes=P --trials=T
- *   --device=D --list
+ *   --device=D --list --cfg=FILE --data-prefix=DIR --write-bin=PREFIX --exact
  * Workload "baseline" with subtype C1, C2, C3 selects the benchmark
  * configurations: C1 2^20 x 2^24 uniform PK-FK, C2 2^24 x 2^28, C3 64-byte
  * Zipf 2^24 x 2^27 (theta from --theta, default 1.0).
@@ -32,27 +38,36 @@ typedef struct {
 	const char *workload, *param, *subtype;
 	uint64_t r_num, s_num;
 	uint32_t width, gen;
+	const char *ref;  /* reference workload_name: host restatement of gen.c */
+	int ref_only;     /* no closed-form device generator for it */
 } wl_t;
 
 /* Sizes from the reference's config/data and config/fndly-data files. */
 static const wl_t k_workloads[] = {
-    {"uniform", "pkfk", "A", 1ull << 24, 1ull << 28, 16, HJ_WL_PKFK_REF},
-    {"uniform", "pkfk", "B", 1ull << 26, 1ull << 28, 16, HJ_WL_PKFK_REF},
-    {"uniform", "pkfk", "C", 1ull << 28, 1ull << 28, 16, HJ_WL_PKFK_REF},
-    {"payload", "pkfk", "16", 1ull << 24, 1ull << 28, 16, HJ_WL_PAYLOAD_REF},
-    {"payload", "pkfk", "32", 1ull << 24, 1ull << 28, 32, HJ_WL_PAYLOAD_REF},
-    {"payload", "pkfk", "64", 1ull << 24, 1ull << 28, 64, HJ_WL_PAYLOAD_REF},
-    {"payload", "pkfk", "128", 1ull << 24, 1ull << 28, 128, HJ_WL_PAYLOAD_REF},
-    {"payload", "pkfk", "256", 1ull << 24, 1ull << 28, 256, HJ_WL_PAYLOAD_REF},
-    {"payload", "pkfk", "512", 1ull << 24, 1ull << 28, 512, HJ_WL_PAYLOAD_REF},  /* ptr_* joins only */
-    {"fndly", "pkfk", "Q", 1ull << 28, 1ull << 30, 16, HJ_WL_PKFK_REF},
-    {"fndly", "pkfk", "R", 1ull << 28, 1ull << 31, 16, HJ_WL_PKFK_REF},
-    {"fndly", "pkfk", "H", 1ull << 28, 1ull << 32, 16, HJ_WL_PKFK_REF},
-    {"fndly", "pkfk", "T", 1ull << 29, 1ull << 32, 16, HJ_WL_PKFK_REF},
-    {"baseline", "pkfk", "C1", 1ull << 20, 1ull << 24, 16, HJ_WL_PKFK_UNIFORM},
-    {"baseline", "pkfk", "C2", 1ull << 24, 1ull << 28, 16, HJ_WL_PKFK_UNIFORM},
-    {"baseline", "zipf", "C3", 1ull << 24, 1ull << 27, 64, HJ_WL_ZIPF},
-    {NULL, NULL, NULL, 0, 0, 0, 0},
+    {"uniform", "pkfk", "A", 1ull << 24, 1ull << 28, 16, HJ_WL_PKFK_REF, "uniform_pkfk_A", 0},
+    {"uniform", "pkfk", "B", 1ull << 26, 1ull << 28, 16, HJ_WL_PKFK_REF, "uniform_pkfk_B", 0},
+    {"uniform", "pkfk", "C", 1ull << 28, 1ull << 28, 16, HJ_WL_PKFK_REF, "uniform_pkfk_C", 0},
+    {"uniform", "dupfk", "D", 1ull << 24, 1ull << 28, 16, 0, "uniform_dupfk_D", 1},
+    {"uniform", "dupfk", "E", 1ull << 26, 1ull << 28, 16, 0, "uniform_dupfk_E", 1},
+    {"uniform", "dupfk", "F", 1ull << 28, 1ull << 28, 16, 0, "uniform_dupfk_F", 1},
+    {"zipfian", "fk", "1.050_A", 1ull << 24, 1ull << 28, 16, 0, "zipfian_1.050_A", 1},
+    {"zipfian", "fk", "1.250_A", 1ull << 24, 1ull << 28, 16, 0, "zipfian_1.250_A", 1},
+    {"zipfian", "fk", "1.500_A", 1ull << 24, 1ull << 28, 16, 0, "zipfian_1.500_A", 1},
+    {"zipfian", "fk", "1.750_A", 1ull << 24, 1ull << 28, 16, 0, "zipfian_1.750_A", 1},
+    {"payload", "pkfk", "16", 1ull << 24, 1ull << 28, 16, HJ_WL_PAYLOAD_REF, "payload_pkfk_16", 0},
+    {"payload", "pkfk", "32", 1ull << 24, 1ull << 28, 32, HJ_WL_PAYLOAD_REF, "payload_pkfk_32", 0},
+    {"payload", "pkfk", "64", 1ull << 24, 1ull << 28, 64, HJ_WL_PAYLOAD_REF, "payload_pkfk_64", 0},
+    {"payload", "pkfk", "128", 1ull << 24, 1ull << 28, 128, HJ_WL_PAYLOAD_REF, "payload_pkfk_128", 0},
+    {"payload", "pkfk", "256", 1ull << 24, 1ull << 28, 256, HJ_WL_PAYLOAD_REF, "payload_pkfk_256", 0},
+    {"payload", "pkfk", "512", 1ull << 24, 1ull << 28, 512, HJ_WL_PAYLOAD_REF, "payload_pkfk_512", 0},  /* ptr_* joins only */
+    {"fndly", "pkfk", "Q", 1ull << 28, 1ull << 30, 16, HJ_WL_PKFK_REF, NULL, 0},
+    {"fndly", "pkfk", "R", 1ull << 28, 1ull << 31, 16, HJ_WL_PKFK_REF, NULL, 0},
+    {"fndly", "pkfk", "H", 1ull << 28, 1ull << 32, 16, HJ_WL_PKFK_REF, NULL, 0},
+    {"fndly", "pkfk", "T", 1ull << 29, 1ull << 32, 16, HJ_WL_PKFK_REF, NULL, 0},
+    {"baseline", "pkfk", "C1", 1ull << 20, 1ull << 24, 16, HJ_WL_PKFK_UNIFORM, NULL, 0},
+    {"baseline", "pkfk", "C2", 1ull << 24, 1ull << 28, 16, HJ_WL_PKFK_UNIFORM, NULL, 0},
+    {"baseline", "zipf", "C3", 1ull << 24, 1ull << 27, 64, HJ_WL_ZIPF, NULL, 0},
+    {NULL, NULL, NULL, 0, 0, 0, 0, NULL, 0},
 };
 
 static int die(hj_ctx_t *ctx, int rc, const char *what) {
@@ -60,8 +75,35 @@ static int die(hj_ctx_t *ctx, int rc, const char *what) {
 	return 1;
 }
 
+/* host tuples of a reference workload, copied into device memory */
+static int ref_upload(hj_ctx_t *ctx, const char *name, uint32_t width, int exact, void *r_dev, void *s_dev,
+                      uint64_t *nr, uint64_t *ns) {
+	hj_ref_spec_t sp;
+	int rc = hj_ref_spec_for(name, width, &sp);
+	if (rc) return rc;
+	sp.exact_order = (uint32_t)exact;
+	if (*nr) sp.r_num = *nr;
+	if (*ns) sp.s_num = *ns;
+	void *r = malloc(sp.r_num * sp.tuple_size), *s = malloc(sp.s_num * sp.tuple_size + 1);
+	if (!r || !s) {
+		free(r);
+		free(s);
+		return HJ_ENOMEM;
+	}
+	rc = hj_ref_generate(&sp, r, s);
+	if (!rc) rc = hj_copy_h2d(ctx, r_dev, r, sp.r_num * sp.tuple_size);
+	if (!rc) rc = hj_copy_h2d(ctx, s_dev, s, sp.s_num * sp.tuple_size);
+	free(r);
+	free(s);
+	*nr = sp.r_num;
+	*ns = sp.s_num;
+	return rc;
+}
+
 int main(int argc, char **argv) {
 	const char *algo = NULL, *workload = "baseline", *subtype = "C2", *param = "pkfk", *gen = NULL;
+	const char *cfg = NULL, *data_prefix = ".", *write_bin = NULL;
+	int exact = 0;
 	uint64_t r_num = 0, s_num = 0, key_range = 0, seed = 12345;
 	uint32_t width = 0, bits = 0, passes = 0;
 	int trials = 6, device = 0, list = 0;
@@ -75,6 +117,8 @@ int main(int argc, char **argv) {
 	    {"seed", required_argument, 0, 'S'},    {"bits", required_argument, 0, 'b'},
 	    {"passes", required_argument, 0, 'P'},  {"trials", required_argument, 0, 'T'},
 	    {"device", required_argument, 0, 'd'},  {"list", no_argument, 0, 'l'},
+	    {"cfg", required_argument, 0, 'c'},     {"data-prefix", required_argument, 0, 'D'},
+	    {"write-bin", required_argument, 0, 'o'}, {"exact", no_argument, 0, 'x'},
 	    {0, 0, 0, 0}};
 	int c;
 	while ((c = getopt_long(argc, argv, "", opts, NULL)) != -1) {
@@ -95,6 +139,10 @@ int main(int argc, char **argv) {
 		case 'T': trials = atoi(optarg); break;
 		case 'd': device = atoi(optarg); break;
 		case 'l': list = 1; break;
+		case 'c': cfg = optarg; break;
+		case 'D': data_prefix = optarg; break;
+		case 'o': write_bin = optarg; break;
+		case 'x': exact = 1; break;
 		default: fprintf(stderr, "unknown option\n"); return 2;
 		}
 	}
@@ -118,9 +166,25 @@ int main(int argc, char **argv) {
 	spec.seed = seed;
 	spec.theta = theta;
 	const wl_t *wl = NULL;
-	for (const wl_t *w = k_workloads; w->workload; w++)
+	hj_data_cfg_t dc;
+	char wname[256];
+	if (cfg) {
+		int rc = hj_data_cfg_read(cfg, &dc);
+		if (rc || dc.r_tuple_size != dc.s_tuple_size || !dc.r_tuple_size) {
+			fprintf(stderr, "[ERROR] cannot use data configuration %s\n", cfg);
+			return 1;
+		}
+		spec.r_num = dc.r_tuple_num;
+		spec.s_num = dc.s_tuple_num;
+		spec.width = dc.r_tuple_size;
+		snprintf(wname, sizeof wname, "%s", dc.workload_name);
+	} else {
+		snprintf(wname, sizeof wname, "%s_%s_%s", workload, param, subtype);
+	}
+	for (const wl_t *w = k_workloads; !cfg && w->workload; w++)
 		if (!strcmp(w->workload, workload) && !strcmp(w->param, param) && !strcmp(w->subtype, subtype)) wl = w;
-	if (wl) {
+	if (cfg) {
+	} else if (wl) {
 		spec.r_num = wl->r_num;
 		spec.s_num = wl->s_num;
 		spec.width = wl->width;
@@ -132,9 +196,9 @@ int main(int argc, char **argv) {
 		spec.width = 16;
 		spec.workload = HJ_WL_PKFK_UNIFORM;
 	}
-	if (r_num) spec.r_num = r_num;
-	if (s_num) spec.s_num = s_num;
-	if (width) spec.width = width;
+	if (r_num && !cfg) spec.r_num = r_num;
+	if (s_num && !cfg) spec.s_num = s_num;
+	if (width && !cfg) spec.width = width;
 	if (gen) {
 		if (!strcmp(gen, "pkfk")) spec.workload = HJ_WL_PKFK_UNIFORM;
 		else if (!strcmp(gen, "pkfk_ref")) spec.workload = HJ_WL_PKFK_REF;
@@ -154,7 +218,26 @@ int main(int argc, char **argv) {
 	void *r_dev = NULL, *s_dev = NULL;
 	if ((rc = hj_dev_alloc(ctx, spec.r_num * spec.width, &r_dev))) return die(ctx, rc, "alloc R");
 	if ((rc = hj_dev_alloc(ctx, spec.s_num * spec.width, &s_dev))) return die(ctx, rc, "alloc S");
-	if ((rc = hj_generate(ctx, &spec, r_dev, s_dev))) return die(ctx, rc, "generate");
+	if (cfg) {
+		char path[1024];
+		snprintf(path, sizeof path, "%s/%s", data_prefix, dc.r_path_suffix);
+		if ((rc = hj_bin_read(ctx, path, spec.r_num, spec.width, r_dev))) return die(ctx, rc, "read R");
+		snprintf(path, sizeof path, "%s/%s", data_prefix, dc.s_path_suffix);
+		if ((rc = hj_bin_read(ctx, path, spec.s_num, spec.width, s_dev))) return die(ctx, rc, "read S");
+	} else if (wl && wl->ref && (wl->ref_only || exact)) {
+		uint64_t nr = r_num, ns = s_num;
+		if ((rc = ref_upload(ctx, wl->ref, spec.width, exact, r_dev, s_dev, &nr, &ns)))
+			return die(ctx, rc, "reference generator");
+	} else if ((rc = hj_generate(ctx, &spec, r_dev, s_dev))) {
+		return die(ctx, rc, "generate");
+	}
+	if (write_bin) {
+		char path[1024];
+		snprintf(path, sizeof path, "%s_R.bin", write_bin);
+		if ((rc = hj_bin_write(ctx, path, r_dev, spec.r_num, spec.width))) return die(ctx, rc, "write R");
+		snprintf(path, sizeof path, "%s_S.bin", write_bin);
+		if ((rc = hj_bin_write(ctx, path, s_dev, spec.s_num, spec.width))) return die(ctx, rc, "write S");
+	}
 
 	printf("CONFIGURATION:\tTUPLE_T_SIZE: %u\tNUM_RADIX_BITS: %u\tNUM_PASSES: %u\tTRIALS: %d\n", spec.width, bits,
 	       passes, trials);
@@ -166,8 +249,6 @@ int main(int argc, char **argv) {
 	prm.passes = passes;
 	double sum_b = 0, sum_p = 0, sum_t = 0;
 	int counted = 0;
-	char wname[128];
-	snprintf(wname, sizeof wname, "%s_%s_%s", workload, param, subtype);
 	for (int t = 0; t < trials; t++) {
 		hj_result_t res;
 		if ((rc = hj_join(ctx, algo, &R, &S, &prm, &res))) return die(ctx, rc, "hj_join");

@@ -233,6 +233,51 @@ int hj_copy_d2d(hj_ctx_t *ctx, void *dst, const void *src, uint64_t bytes, doubl
  * the host functions in hjgen.h. */
 int hj_generate(hj_ctx_t *ctx, const hj_gen_spec_t *spec, void *r_dev, void *s_dev);
 
+/* ---- the reference's own datasets -----------------------------------------
+ * Generator with the reference's draws (src/gen.c:29-377 and
+ * src/inc/eth_genzipf.c:30-148 through glibc rand() after srand(seed)):
+ * fills host buffers of r_num / s_num tuples.  With exact_order the key and
+ * row_id words equal the reference's .bin files byte for byte (its shuffles
+ * included; the rest of each tuple is zero); without it the same tuples come
+ * unshuffled, which no join result depends on. */
+#define HJ_REF_PKFK 0     /* uniform_pkfk_{A,B,C}: gen_pkfk_uniform            */
+#define HJ_REF_DUPFK 1    /* uniform_dupfk_{D,E,F}: gen_dup_uniform            */
+#define HJ_REF_ZIPF 2     /* zipfian_<1+theta>_A: pk_AR + gen_fk_zipfian_eth   */
+#define HJ_REF_PAYLOAD 3  /* payload_pkfk_<bytes>: payload_gen_pkfk_uniform    */
+typedef struct {
+	uint32_t kind;         /* HJ_REF_*                                      */
+	uint32_t tuple_size;   /* bytes, multiple of 16                          */
+	uint64_t r_num, s_num; /* pkfk/dupfk/payload: s_num a multiple of r_num,
+	                          r_num * tuple_size a multiple of 4096 (whole
+	                          blocks, as gen_fk_from_pk copies)              */
+	double zipf_factor;    /* zipf: 1 + the configuration's theta            */
+	int32_t seed;          /* 12345 in the reference (src/gen.c:144)         */
+	uint32_t exact_order;  /* 1: the reference's tuple order too             */
+} hj_ref_spec_t;
+/* spec of a workload named as the reference's configurations name it
+ * (workload_name of the config/data .cfg files: "uniform_dupfk_D",
+ * "zipfian_1.250_A", "payload_pkfk_64", ...); exact_order left 0 */
+int hj_ref_spec_for(const char *workload_name, uint32_t tuple_size, hj_ref_spec_t *out);
+int hj_ref_generate(const hj_ref_spec_t *spec, void *r_host, void *s_host);
+/* the first n values of glibc rand() after srand(seed) */
+int hj_ref_rand(int32_t seed, uint64_t n, int32_t *out);
+
+/* A data configuration file of the reference (the .cfg files under config/data, read by
+ * data_cfg, src/inc/utils.c:236-290). */
+typedef struct {
+	char r_path_suffix[256], s_path_suffix[256], workload_name[128];
+	uint64_t r_tuple_num, s_tuple_num;
+	uint32_t r_tuple_size, s_tuple_size;
+	double theta, selectivity, density, sparsity;
+	int64_t min_key, max_key;
+} hj_data_cfg_t;
+int hj_data_cfg_read(const char *cfg_path, hj_data_cfg_t *out);
+/* Raw tuple files (packed tuple_t[], no header): the reference's data files
+ * (read at src/algo/phj.c:1885-1915) into / out of device memory through
+ * pinned staging buffers. */
+int hj_bin_read(hj_ctx_t *ctx, const char *path, uint64_t tuple_num, uint32_t tuple_size, void *dev);
+int hj_bin_write(hj_ctx_t *ctx, const char *path, const void *dev, uint64_t tuple_num, uint32_t tuple_size);
+
 /* ---- device properties used by the autotuner and the benchmark ---------- */
 typedef struct {
 	int cu_count;

@@ -18,6 +18,7 @@
 #define _GNU_SOURCE
 #include "hjoracle.h"
 
+#include <math.h>
 #include <pthread.h>
 #include <stdlib.h>
 #include <string.h>
@@ -1006,4 +1007,136 @@ int or_ptr_join(int algo, int kind, uint32_t radix_bits, uint32_t passes, const 
 	free(r);
 	free(s);
 	return rc;
+}
+
+/* ---- the reference's generator (src/gen.c, src/inc/eth_genzipf.c) --------
+ * Restated on this C library's own srand()/rand(), the generator the
+ * reference calls, so the draws are glibc's by construction.  kind: 0 pkfk
+ * (gen_pkfk_uniform :192-214 + gen_fk_from_pk :174-189), 1 dupfk
+ * (gen_dup_uniform :286-331), 2 zipf (pk_AR + gen_fk_zipfian_eth :361-377),
+ * 3 payload (payload_gen_pkfk_uniform :246-283).  exact: also the
+ * knuth_shuffle calls (:161-171).  Tuple bytes past the row id are zero.
+ * Zipf table lookups run on `threads` threads (the draws stay sequential). */
+static void ref_put(uint8_t *b, uint64_t i, uint32_t W, uint64_t key, uint64_t rid) {
+	uint64_t *t = (uint64_t *)(b + i * W);
+	t[0] = key;
+	t[1] = rid;
+	memset(t + 2, 0, W - 16);
+}
+
+static void ref_shuffle(uint8_t *b, uint64_t n, uint32_t W) {
+	uint8_t tmp[4096];
+	for (uint64_t i = 0; i < n; i++) {
+		const uint64_t j = (uint64_t)rand() % n;
+		memcpy(tmp, b + j * W, W);
+		memcpy(b + j * W, b + i * W, W);
+		memcpy(b + i * W, tmp, W);
+	}
+}
+
+typedef struct {
+	const int *draws;
+	const double *lut;
+	const uint64_t *alpha;
+	uint64_t n, lo, hi;
+	uint8_t *S;
+	uint32_t W;
+} zipf_arg_t;
+
+static void *zipf_thread(void *p) {
+	zipf_arg_t *a = (zipf_arg_t *)p;
+	for (uint64_t j = a->lo; j < a->hi; j++) {
+		const double r = (double)a->draws[j] / RAND_MAX;
+		uint64_t pos = 0;
+		if (a->lut[0] < r) {
+			uint64_t l = 1, h = a->n;
+			while (h - l > 1) {
+				const uint64_t m = (l + h) / 2;
+				if (a->lut[m] < r)
+					l = m;
+				else
+					h = m;
+			}
+			pos = h;
+		}
+		ref_put(a->S, j, a->W, a->alpha[pos], j + 1);
+	}
+	return NULL;
+}
+
+int or_ref_generate(int kind, uint64_t nr, uint64_t ns, uint32_t W, double zf, int seed, int exact, int threads,
+                    void *Rv, void *Sv) {
+	uint8_t *R = (uint8_t *)Rv, *S = (uint8_t *)Sv;
+	if (W < 16 || W % 16 || W > 4096 || nr == 0) return -2;
+	if (kind != 2 && (ns % nr || (nr * W) % 4096)) return -2;
+	srand((unsigned)seed);
+	if (kind == 1) {
+		int u = rand() % RAND_MAX;
+		for (uint64_t i = 0; i < nr; i++) {
+			const int k = u / 127773;
+			u = 16807 * (u - k * 127773) - k * 2836;
+			if (u < 0) u += 2147483647;
+			const double r = (double)u * 4.656612875E-10;
+			volatile double lo = (1.0 - r) * (1.0 - 0.5), hi = r * ((double)nr + 0.5);
+			long long key = (int)round(lo + hi);
+			if (key < 1) key = 1;
+			if (key > (long long)nr) key = (long long)nr;
+			ref_put(R, i, W, (uint64_t)key, i + 1);
+		}
+	} else {
+		for (uint64_t i = 0; i < nr; i++) ref_put(R, i, W, i + 1, i + 1);
+	}
+	if (exact) ref_shuffle(R, nr, W);
+	if (kind != 2) {
+		for (uint64_t c = 0; c < ns / nr; c++) memcpy(S + c * nr * W, R, nr * W);
+		if (exact) ref_shuffle(S, ns, W);
+		if (kind == 3) {
+			for (uint64_t j = 0; j < ns; j++) ((uint64_t *)(S + j * W))[1] = j + 1;
+			for (uint64_t i = 0; i < nr; i++) ((uint64_t *)(R + i * W))[1] = i + 1;
+		}
+		return 0;
+	}
+	/* zipf: a fresh srand(seed), then alphabet, table and stream */
+	if (!ns) return 0;
+	srand((unsigned)seed);
+	uint64_t *alpha = (uint64_t *)malloc(nr * 8);
+	double *lut = (double *)malloc(nr * 8);
+	int *draws = (int *)malloc(ns * sizeof(int));
+	if (!alpha || !lut || !draws) {
+		free(alpha);
+		free(lut);
+		free(draws);
+		return -3;
+	}
+	for (uint64_t i = 0; i < nr; i++) alpha[i] = i + 1;
+	for (uint64_t i = nr - 1; i > 0; i--) {
+		const uint64_t k = i * (uint64_t)rand() / RAND_MAX;
+		const uint64_t t = alpha[i];
+		alpha[i] = alpha[k];
+		alpha[k] = t;
+	}
+	double scale = 0, sum = 0;
+	for (uint64_t i = 1; i <= nr; i++) scale += 1.0 / pow((double)i, zf);
+	for (uint64_t i = 1; i <= nr; i++) {
+		sum += 1.0 / pow((double)i, zf);
+		lut[i - 1] = sum / scale;
+	}
+	for (uint64_t j = 0; j < ns; j++) draws[j] = rand();
+	if (threads < 1) threads = 1;
+	pthread_t *th = (pthread_t *)malloc(sizeof(pthread_t) * threads);
+	zipf_arg_t *args = (zipf_arg_t *)malloc(sizeof(zipf_arg_t) * threads);
+	const uint64_t per = (ns + threads - 1) / threads;
+	for (int t = 0; t < threads; t++) {
+		const uint64_t lo = (uint64_t)t * per < ns ? (uint64_t)t * per : ns;
+		const uint64_t hi = lo + per < ns ? lo + per : ns;
+		args[t] = (zipf_arg_t){draws, lut, alpha, nr, lo, hi, S, W};
+		pthread_create(&th[t], NULL, zipf_thread, &args[t]);
+	}
+	for (int t = 0; t < threads; t++) pthread_join(th[t], NULL);
+	free(th);
+	free(args);
+	free(alpha);
+	free(lut);
+	free(draws);
+	return 0;
 }

@@ -69,6 +69,12 @@ int or_ptr_bruteforce(const void *R, uint64_t nr, const void *S, uint64_t ns, ui
 int or_ptr_retrieve(const uint64_t *pairs, uint64_t npairs, const void *R, uint64_t nr, const void *S,
                     uint64_t ns, uint32_t width, uint64_t *checksum);
 
+/* the reference's generator on this C library's srand()/rand() (src/gen.c);
+ * kind 0 pkfk, 1 dupfk, 2 zipf (zf = 1 + theta), 3 payload; exact = with the
+ * reference's shuffles */
+int or_ref_generate(int kind, uint64_t nr, uint64_t ns, uint32_t width, double zf, int seed, int exact, int threads,
+                    void *R, void *S);
+
 /* independent checkers */
 int or_aggregate(const void *R, uint64_t nr, const void *S, uint64_t ns, uint32_t width, int lp, or_result_t *out);
 int or_bruteforce(const void *R, uint64_t nr, const void *S, uint64_t ns, uint32_t width, int lp, or_result_t *out);

@@ -62,6 +62,8 @@ def lib():
             getattr(L, name).restype = I
         L.or_ptr_join.argtypes = [I, I, U32, U32, P, U64, P, U64, U32, I, C.POINTER(OrResult)]
         L.or_ptr_join.restype = I
+        L.or_ref_generate.argtypes = [I, U64, U64, U32, C.c_double, I, I, I, P, P]
+        L.or_ref_generate.restype = I
         _lib = L
     return _lib
 
@@ -238,3 +240,19 @@ def ptr_direct_address(R: np.ndarray, S: np.ndarray, width: int) -> dict:
         s = int(rp[(r_rid - 1).astype(np.int64)].sum(dtype=np.uint64)) + int(
             sp[(sp - 1).astype(np.int64)].sum(dtype=np.uint64))
     return {"matched_cnt": int(sk.size), "check_sum": s & M64}
+
+
+REF_KINDS = {"pkfk": 0, "dupfk": 1, "zipf": 2, "payload": 3}
+
+
+def ref_generate(kind: str, r_num: int, s_num: int, width: int = 16, zipf_factor: float = 0.0, seed: int = 12345,
+                 exact: bool = True, threads: int = 8):
+    """The reference's generator (src/gen.c) driven by this C library's own
+    srand()/rand(): host tuples R, S."""
+    r = np.empty(r_num * width, dtype=np.uint8)
+    s = np.empty(s_num * width, dtype=np.uint8)
+    rc = lib().or_ref_generate(REF_KINDS[kind], r_num, s_num, width, zipf_factor, seed, int(exact), threads,
+                               _ptr(r), _ptr(s))
+    if rc:
+        raise ValueError(f"or_ref_generate({kind}, {r_num}, {s_num}, {width}) failed: {rc}")
+    return r, s

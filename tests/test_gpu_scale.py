@@ -47,6 +47,41 @@ def test_reference_logged_payload(ctx, width):
     ctx.free_relation(S)
 
 
+@pytest.mark.parametrize("width", [16, 64, 512])
+def test_reference_logged_payload_late_materialisation(ctx, width):
+    """ptr_phj_radix_bc / ptr_nphj_sc on payload_pkfk_16 / _64 / _512 (logged
+    268435456 / 38280597101084672, logs/payload-logs/ptr_*_payload_nvm.log),
+    plus the other ptr joins at 16 bytes."""
+    R, S = ctx.generate("payload_ref", 1 << 24, 1 << 28, width)
+    algos = list(hjmi.PTR_ALGOS) if width == 16 else ["ptr_phj_radix_bc", "ptr_nphj_sc"]
+    for algo in algos:
+        _check(ctx.join(algo, R, S), 268435456, 38280597101084672, (width, algo))
+    ctx.free_relation(R)
+    ctx.free_relation(S)
+
+
+LOGGED_REF = {"uniform_dupfk_D": (534841984, 8973526820846944), "uniform_dupfk_E": (528480568, 35467655496411728),
+              "uniform_dupfk_F": (503316966, 135111353818793896),
+              "zipfian_1.050_A": (268435456, 38229679500325349), "zipfian_1.250_A": (268435456, 38182251103361772),
+              "zipfian_1.500_A": (268435456, 38191112124779249), "zipfian_1.750_A": (268435456, 38252395714941475)}
+
+
+@pytest.mark.parametrize("name", list(LOGGED_REF))
+def test_reference_logged_generator_dependent(ctx, name):
+    """dupfk D/E/F and Zipf 1.050..1.750 (logs/dupfk-logs, logs/skew-logs;
+    tests/golden/reference_logs.json): results fixed by glibc's draws,
+    reproduced by the restated reference generator (unshuffled: no join
+    result depends on the order)."""
+    hR, hS = hjmi.ref_generate(hjmi.ref_spec(name))
+    R, S = ctx.upload(hR, 16), ctx.upload(hS, 16)
+    del hR, hS
+    algos = ["phj_radix_bc", "nphj_sc", "phj_radix_hr", "phj_radix_sc", "phj_radix_hro"]
+    for algo in algos:
+        _check(ctx.join(algo, R, S), *LOGGED_REF[name], (name, algo))
+    ctx.free_relation(R)
+    ctx.free_relation(S)
+
+
 def _s_closed_form(nr, ns, width=16, seed=12345):
     _, S = O.generate("pkfk", nr, ns, width, seed=seed, threads=16)
     return O.pkfk_closed_form(S, width)
