@@ -68,6 +68,7 @@ struct hj_ctx {
 	// size-class autotuner: radix bits / passes per log2 |R| (0xff = unset)
 	uint8_t tune_bits[40];
 	uint8_t tune_passes[40];
+	uint8_t tune_pass1[40];  // first-pass bits of a two-pass class (0 = even split)
 	bool tuned = false;
 
 	// timing
@@ -145,7 +146,7 @@ int hj_run_ptr(hj_ctx_t *ctx, const hj_relation_t *R, const hj_relation_t *S, co
                hj_algo_fn inner, hj_result_t *out);
 // radix configuration for |R|: the tuned class if the context has one, else
 // the default rule
-void hj_radix_config(hj_ctx_t *ctx, uint64_t nr, uint32_t *bits, uint32_t *passes);
+void hj_radix_config(hj_ctx_t *ctx, uint64_t nr, uint32_t *bits, uint32_t *passes, uint32_t *pass1 = nullptr);
 extern "C" bool hj_algo_is_lp(const char *algo, bool *radix);
 
 // Time between two recorded events of ctx->ev, in seconds.

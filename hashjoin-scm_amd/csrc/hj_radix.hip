@@ -1405,11 +1405,13 @@ void hj_radix_default(uint64_t nr, uint32_t *bits, uint32_t *passes) {
 	*passes = b == 0 ? 0 : (b > 11 ? 2 : 1);
 }
 
-void hj_radix_config(hj_ctx_t *ctx, uint64_t nr, uint32_t *bits, uint32_t *passes) {
+void hj_radix_config(hj_ctx_t *ctx, uint64_t nr, uint32_t *bits, uint32_t *passes, uint32_t *pass1) {
 	const uint32_t c = hj_log2_ceil(nr ? nr : 1);
+	if (pass1) *pass1 = 0;
 	if (ctx && ctx->tuned && c < 40 && ctx->tune_bits[c] != 0xff) {
 		*bits = ctx->tune_bits[c];
 		*passes = ctx->tune_passes[c];
+		if (pass1) *pass1 = ctx->tune_pass1[c];
 		return;
 	}
 	hj_radix_default(nr, bits, passes);
@@ -1422,16 +1424,24 @@ int hj_run_radix(hj_ctx_t *ctx, const hj_relation_t *R, const hj_relation_t *S, 
 		return hj_fail(ctx, HJ_EINVAL, "hro joins tuples of at most 64 bytes (one 512-bit block holds a tuple)");
 	const uint32_t W = R->tuple_size, wv = W / 16;
 	const uint64_t nr = R->tuple_num, ns = S->tuple_num, N = nr + ns;
-	uint32_t B = prm->radix_bits, P = prm->passes;
-	if (B == 0 && P == 0) hj_radix_config(ctx, nr, &B, &P);
+	uint32_t B = prm->radix_bits, P = prm->passes, tuned_b1 = 0;
+	if (B == 0 && P == 0) hj_radix_config(ctx, nr, &B, &P, &tuned_b1);
 	if (P == 0) P = B == 0 ? 0 : (B > 11 ? 2 : 1);
 	if (B == 0) P = 0;
 	if (P == 2 && B < 2) P = 1;
-	// two passes: the first takes the larger half of the bits, so the chunk-
-	// local second pass sorts each tile into at most as many runs as pass 1
+	// two passes: by default the first takes the larger half of the bits, so
+	// the chunk-local second pass sorts each tile into at most as many runs
+	// as pass 1; the tuner or the caller (NUM_RADIX_BITS_PASS1) may choose
+	// another split
 	uint32_t b1 = P == 2 ? (B + 1) / 2 : (P ? B : 0);
+	if (P == 2 && tuned_b1 && tuned_b1 < B) b1 = tuned_b1;
 	if (P == 2 && (prm->part_variant >> 8 & 15) && (prm->part_variant >> 8 & 15) < B)
 		b1 = prm->part_variant >> 8 & 15;  // first-pass width forced for tuning sweeps
+	if (P == 2 && prm->radix_bits_pass1) {
+		if (prm->radix_bits_pass1 >= B)
+			return hj_fail(ctx, HJ_EINVAL, "radix_bits_pass1 %u must be below radix_bits %u", prm->radix_bits_pass1, B);
+		b1 = prm->radix_bits_pass1;
+	}
 	const uint32_t b2 = B - b1;
 	const uint32_t F1 = 1u << b1, F2 = 1u << b2, Q = 1u << B;
 	if (b1 > 12 || (P == 2 && b2 > 12)) return hj_fail(ctx, HJ_EINVAL, "at most 12 radix bits per pass");
@@ -2183,15 +2193,23 @@ extern "C" int hj_autotune(hj_ctx_t *ctx, uint32_t width, uint32_t s_ratio, uint
 		if ((rc = hj_launch_generate(ctx, &spec, r, s))) break;
 		hj_relation_t R{r, spec.r_num, width, HJ_LOC_DEVICE}, S{s, spec.s_num, width, HJ_LOC_DEVICE};
 		double best = 1e30;
-		uint32_t bb = 0, bp = 0;
-		// candidates: partitions of 512..8192 build tuples, one or two passes
+		uint32_t bb = 0, bp = 0, bs = 0;
+		// candidates: partitions of 512..8192 build tuples, one or two passes,
+		// and for two passes the first-pass width around the even split
+		// (asymmetric bits, phj_asymm_radix.h:10-22)
 		for (int b = (int)c - 13; b <= (int)c - 9; b++) {
 			if (b < 0) continue;
-			for (uint32_t p = (b == 0 ? 0 : 1); p <= (b >= 12 ? 2u : (b == 0 ? 0u : 1u)); p++) {
+			for (uint32_t q = 0; q < 4; q++) {
+				const uint32_t p = q == 0 ? (b == 0 ? 0u : 1u) : 2u;
+				if (p == 2 && b < 12) break;
 				if (b > 12 && p == 1) continue;
+				// q = 1, 2, 3: first pass takes the even split -1, +0, +1
+				const uint32_t s1 = p == 2 ? (uint32_t)(b + 1) / 2 + q - 2 : 0;
+				if (p == 2 && (s1 < 1 || s1 >= (uint32_t)b || s1 > 12 || b - s1 > 12)) continue;
 				hj_params_t prm{};
 				prm.radix_bits = (uint32_t)b;
 				prm.passes = p;
+				prm.radix_bits_pass1 = s1;
 				if (b == 0) prm.passes = 0;
 				double t = 1e30;
 				for (int rep = 0; rep < 3; rep++) {
@@ -2209,12 +2227,14 @@ extern "C" int hj_autotune(hj_ctx_t *ctx, uint32_t width, uint32_t s_ratio, uint
 					best = t;
 					bb = (uint32_t)b;
 					bp = b ? p : 0;
+					bs = s1;
 				}
 			}
 			if (rc) break;
 		}
 		ctx->tune_bits[c] = (uint8_t)bb;
 		ctx->tune_passes[c] = (uint8_t)bp;
+		ctx->tune_pass1[c] = (uint8_t)bs;
 	}
 	ctx->kernel_times = kt;
 	(void)hipStreamSynchronize(ctx->stream);
@@ -2227,6 +2247,13 @@ extern "C" int hj_autotune(hj_ctx_t *ctx, uint32_t width, uint32_t s_ratio, uint
 extern "C" int hj_tuned_config(hj_ctx_t *ctx, uint64_t r_num, uint32_t *radix_bits, uint32_t *passes) {
 	if (!ctx || !radix_bits || !passes) return HJ_EINVAL;
 	hj_radix_config(ctx, r_num, radix_bits, passes);
+	return HJ_OK;
+}
+
+extern "C" int hj_tuned_pass1_bits(hj_ctx_t *ctx, uint64_t r_num, uint32_t *radix_bits_pass1) {
+	if (!ctx || !radix_bits_pass1) return HJ_EINVAL;
+	uint32_t b, p;
+	hj_radix_config(ctx, r_num, &b, &p, radix_bits_pass1);
 	return HJ_OK;
 }
 
@@ -2695,6 +2722,7 @@ extern "C" int hj_serve_open(hj_ctx_t *ctx, const char *algo, const hj_serve_con
 		if (ctx->tuned) {  // lanes use the parent's size-class tuning
 			memcpy(l.ctx->tune_bits, ctx->tune_bits, sizeof ctx->tune_bits);
 			memcpy(l.ctx->tune_passes, ctx->tune_passes, sizeof ctx->tune_passes);
+			memcpy(l.ctx->tune_pass1, ctx->tune_pass1, sizeof ctx->tune_pass1);
 			l.ctx->tuned = true;
 		}
 	}

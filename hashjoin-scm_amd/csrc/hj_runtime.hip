@@ -13,6 +13,8 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <algorithm>
+
 int hj_fail(hj_ctx_t *ctx, int code, const char *fmt, ...) {
 	if (ctx) {
 		char buf[512];
@@ -112,6 +114,7 @@ int hj_ctx_create(int device, hj_ctx_t **out) {
 	ctx->num_cu = ctx->prop.multiProcessorCount;
 	memset(ctx->tune_bits, 0xff, sizeof ctx->tune_bits);
 	memset(ctx->tune_passes, 0xff, sizeof ctx->tune_passes);
+	memset(ctx->tune_pass1, 0, sizeof ctx->tune_pass1);
 	for (int i = 0; i < hj_ctx::NEV; i++) {
 		if (hipEventCreate(&ctx->ev[i]) != hipSuccess) {
 			(void)hipGetLastError();
@@ -156,6 +159,40 @@ HJ_RADIX_FN(run_radix_hr, 3)
 HJ_RADIX_FN(run_radix_hro, 4)
 #undef HJ_RADIX_FN
 
+// asymmetric two-pass radix joins (src/algo/phj_asymm_radix.c,
+// phj_asymm_radix_bw_reg.c): NUM_RADIX_BITS_PASS1 bits in the first pass,
+// the rest in the second.  The reference's default first pass is 11 bits,
+// lowered to NUM_RADIX_BITS when that is smaller (phj_asymm_radix.h:10-17),
+// which leaves no second pass; here that case runs one pass of the same
+// bits.  The first pass is capped at 12 bits (one LDS sort tile per pass).
+static int run_asymm(hj_ctx_t *c, const hj_relation_t *R, const hj_relation_t *S, const hj_params_t *p, int tk,
+                     hj_result_t *o) {
+	hj_params_t q = *p;
+	if (q.radix_bits == 0) {
+		uint32_t passes;
+		hj_radix_config(c, R->tuple_num, &q.radix_bits, &passes);
+	}
+	uint32_t b1 = q.radix_bits_pass1 ? q.radix_bits_pass1 : 11;
+	if (b1 >= q.radix_bits) {
+		q.passes = q.radix_bits ? 1 : 0;
+		q.radix_bits_pass1 = 0;
+	} else {
+		q.passes = 2;
+		q.radix_bits_pass1 = std::min(b1, 12u);
+	}
+	return hj_run_radix(c, R, S, &q, tk, o);
+}
+#define HJ_ASYMM_FN(NAME, TK)                                                                           \
+	static int NAME(hj_ctx_t *c, const hj_relation_t *R, const hj_relation_t *S, const hj_params_t *p, \
+	                hj_result_t *o) {                                                                   \
+		return run_asymm(c, R, S, p, TK, o);                                                            \
+	}
+HJ_ASYMM_FN(asymm_bc, 0)
+HJ_ASYMM_FN(asymm_sc, 2)
+HJ_ASYMM_FN(asymm_hr, 3)
+HJ_ASYMM_FN(asymm_hro, 4)
+#undef HJ_ASYMM_FN
+
 // late-materialising joins: the same engines on {key, row_id} heads
 // (src/algo/ptr_nphj.c, ptr_phj.c:3100-3115, ptr_phj_bw_reg.c)
 #define HJ_PTR_FN(NAME, INNER)                                                                          \
@@ -189,6 +226,14 @@ static const hj_algo_entry k_algos[] = {
     {"phj_radix_bw_reg_sc", run_radix_sc},
     {"phj_radix_bw_reg_hr", run_radix_hr},
     {"phj_radix_bw_reg_hro", run_radix_hro},
+    {"phj_asymm_radix_bc", asymm_bc},
+    {"phj_asymm_radix_sc", asymm_sc},
+    {"phj_asymm_radix_hr", asymm_hr},
+    {"phj_asymm_radix_hro", asymm_hro},
+    {"phj_asymm_radix_bw_reg_bc", asymm_bc},
+    {"phj_asymm_radix_bw_reg_sc", asymm_sc},
+    {"phj_asymm_radix_bw_reg_hr", asymm_hr},
+    {"phj_asymm_radix_bw_reg_hro", asymm_hro},
     {"ptr_nphj_sc", ptr_nphj_sc},
     {"ptr_nphj_lp", ptr_nphj_lp},
     {"ptr_phj_radix_bc", ptr_radix_bc},

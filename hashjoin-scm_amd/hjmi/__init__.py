@@ -32,7 +32,9 @@ WORKLOADS = {"pkfk": WL_PKFK_UNIFORM, "pkfk_ref": WL_PKFK_REF, "payload_ref": WL
 
 ALGOS = ("nphj_sc", "nphj_lp", "phj_radix_bc", "phj_radix_lp", "phj_radix_sc", "phj_radix_hr", "phj_radix_hro",
          "phj_radix_bw_reg_bc", "phj_radix_bw_reg_lp", "phj_radix_bw_reg_sc", "phj_radix_bw_reg_hr",
-         "phj_radix_bw_reg_hro")
+         "phj_radix_bw_reg_hro", "phj_asymm_radix_bc", "phj_asymm_radix_sc", "phj_asymm_radix_hr",
+         "phj_asymm_radix_hro", "phj_asymm_radix_bw_reg_bc", "phj_asymm_radix_bw_reg_sc",
+         "phj_asymm_radix_bw_reg_hr", "phj_asymm_radix_bw_reg_hro")
 # late materialisation (src/algo/ptr_*.c): the join emits row-id pairs and a
 # retrieve pass computes check_sum = sum R[r_row_id-1].row_id + S[s_row_id-1].row_id
 PTR_ALGOS = ("ptr_nphj_sc", "ptr_nphj_lp", "ptr_phj_radix_bc", "ptr_phj_radix_sc", "ptr_phj_radix_lp",
@@ -48,7 +50,8 @@ class Relation(C.Structure):
 
 class Params(C.Structure):
     _fields_ = [("radix_bits", C.c_uint32), ("passes", C.c_uint32), ("lp_slots_log2", C.c_uint32),
-                ("flags", C.c_uint32), ("part_variant", C.c_uint32), ("reserved", C.c_uint32 * 3)]
+                ("flags", C.c_uint32), ("part_variant", C.c_uint32), ("radix_bits_pass1", C.c_uint32),
+                ("reserved", C.c_uint32 * 2)]
 
 
 class Result(C.Structure):
@@ -151,6 +154,7 @@ def load_library(path: str = LIB_PATH):
         "hj_batch_destroy": (I, [P]),
         "hj_autotune": (I, [P, U32, U32, U32, U32]),
         "hj_tuned_config": (I, [P, U64, C.POINTER(U32), C.POINTER(U32)]),
+        "hj_tuned_pass1_bits": (I, [P, U64, C.POINTER(U32)]),
         "hj_serve_open": (I, [P, C.c_char_p, C.POINTER(ServeConfig), C.POINTER(P)]),
         "hj_serve_register_build": (I, [P, C.POINTER(Relation), C.POINTER(U32)]),
         "hj_serve_submit": (I, [P, C.c_int32, C.POINTER(Relation), C.POINTER(Relation), C.POINTER(U64)]),
@@ -177,7 +181,8 @@ def exported_symbols():
     return ["hj_ctx_create", "hj_ctx_destroy", "hj_strerror", "hj_last_error", "hj_algo_count",
             "hj_algo_name", "hj_kernel_name", "hj_join", "hj_dev_alloc", "hj_dev_free", "hj_copy_h2d",
             "hj_copy_d2h", "hj_sync", "hj_generate", "hj_device_info", "hj_copy_d2d", "hj_batch_create",
-            "hj_batch_run", "hj_batch_destroy", "hj_autotune", "hj_tuned_config", "hj_serve_open",
+            "hj_batch_run", "hj_batch_destroy", "hj_autotune", "hj_tuned_config", "hj_tuned_pass1_bits",
+            "hj_serve_open",
             "hj_serve_register_build", "hj_serve_submit", "hj_serve_poll", "hj_serve_drain", "hj_serve_stats",
             "hj_serve_close", "hj_ref_spec_for", "hj_ref_generate", "hj_ref_rand", "hj_data_cfg_read",
             "hj_bin_read", "hj_bin_write"]
@@ -299,7 +304,7 @@ class Context:
 
     # -- join
     def join(self, algo: str, R, S, radix_bits: int = 0, passes: int = 0, lp_slots_log2: int = 0,
-             kernel_times: bool = False, part_variant: int = 0) -> dict:
+             kernel_times: bool = False, part_variant: int = 0, radix_bits_pass1: int = 0) -> dict:
         """Run one join; R/S are DeviceRelation or host numpy uint8 arrays."""
         rr, keep = [], []
         for rel, ts in ((R, None), (S, None)):
@@ -307,7 +312,8 @@ class Context:
                 rr.append(rel.as_c())
             else:
                 raise TypeError("pass DeviceRelation (use Context.upload or Context.host_relation)")
-        prm = Params(radix_bits, passes, lp_slots_log2, HJ_F_KERNEL_TIMES if kernel_times else 0, part_variant)
+        prm = Params(radix_bits, passes, lp_slots_log2, HJ_F_KERNEL_TIMES if kernel_times else 0, part_variant,
+                     radix_bits_pass1)
         res = Result()
         self._check(self.lib.hj_join(self.h, algo.encode(), C.byref(rr[0]), C.byref(rr[1]), C.byref(prm),
                                      C.byref(res)), f"hj_join({algo})")
@@ -345,6 +351,11 @@ class Context:
         b, p = C.c_uint32(), C.c_uint32()
         self._check(self.lib.hj_tuned_config(self.h, r_num, C.byref(b), C.byref(p)), "hj_tuned_config")
         return b.value, p.value
+
+    def tuned_pass1_bits(self, r_num: int) -> int:
+        b1 = C.c_uint32()
+        self._check(self.lib.hj_tuned_pass1_bits(self.h, r_num, C.byref(b1)), "hj_tuned_pass1_bits")
+        return b1.value
 
     def device_info(self) -> dict:
         d = DeviceInfo()

@@ -21,7 +21,7 @@
  *
  * Extra options (all optional):
  *   --r=N --s=N --width=W --gen=pkfk|pkfk_ref|payload_ref|zipf|random
- *   --theta=T --key-range=K --seed=S --bits=B --passes=P --trials=T
+ *   --theta=T --key-range=K --seed=S --bits=B --passes=P --bits-pass1=B1 --trials=T
  *   --device=D --list --cfg=FILE --data-prefix=DIR --write-bin=PREFIX --exact
  * Workload "baseline" with subtype C1, C2, C3 selects the benchmark
  * configurations: C1 2^20 x 2^24 uniform PK-FK, C2 2^24 x 2^28, C3 64-byte
@@ -105,7 +105,7 @@ int main(int argc, char **argv) {
 	const char *cfg = NULL, *data_prefix = ".", *write_bin = NULL;
 	int exact = 0;
 	uint64_t r_num = 0, s_num = 0, key_range = 0, seed = 12345;
-	uint32_t width = 0, bits = 0, passes = 0;
+	uint32_t width = 0, bits = 0, passes = 0, bits_pass1 = 0;
 	int trials = 6, device = 0, list = 0;
 	double theta = 1.0;
 	static struct option opts[] = {
@@ -119,6 +119,7 @@ int main(int argc, char **argv) {
 	    {"device", required_argument, 0, 'd'},  {"list", no_argument, 0, 'l'},
 	    {"cfg", required_argument, 0, 'c'},     {"data-prefix", required_argument, 0, 'D'},
 	    {"write-bin", required_argument, 0, 'o'}, {"exact", no_argument, 0, 'x'},
+	    {"bits-pass1", required_argument, 0, '1'},
 	    {0, 0, 0, 0}};
 	int c;
 	while ((c = getopt_long(argc, argv, "", opts, NULL)) != -1) {
@@ -143,6 +144,7 @@ int main(int argc, char **argv) {
 		case 'D': data_prefix = optarg; break;
 		case 'o': write_bin = optarg; break;
 		case 'x': exact = 1; break;
+		case '1': bits_pass1 = (uint32_t)strtoul(optarg, NULL, 0); break;
 		default: fprintf(stderr, "unknown option\n"); return 2;
 		}
 	}
@@ -247,6 +249,7 @@ int main(int argc, char **argv) {
 	memset(&prm, 0, sizeof prm);
 	prm.radix_bits = bits;
 	prm.passes = passes;
+	prm.radix_bits_pass1 = bits_pass1;
 	double sum_b = 0, sum_p = 0, sum_t = 0;
 	int counted = 0;
 	for (int t = 0; t < trials; t++) {

@@ -749,11 +749,31 @@ static void *radix_thread(void *p) {
 	return NULL;
 }
 
+static int radix_join(const void *R, uint64_t nr, const void *S, uint64_t ns, uint32_t width, uint32_t radix_bits,
+                      uint32_t passes, uint32_t b1, int kind, int threads, or_result_t *out);
+
 int or_phj_radix(const void *R, uint64_t nr, const void *S, uint64_t ns, uint32_t width, uint32_t radix_bits,
                  uint32_t passes, int kind, int threads, or_result_t *out) {
+	if (passes < 1 || passes > 2) return -1;
+	return radix_join(R, nr, S, ns, width, radix_bits, passes, radix_bits / passes, kind, threads, out);
+}
+
+/* phj_asymm_radix_* (src/algo/phj_asymm_radix.c): the first pass takes
+ * NUM_RADIX_BITS_PASS1 bits, the second the rest (phj_asymm_radix.h:10-22);
+ * a first pass of all the bits leaves a single pass */
+int or_phj_asymm_radix(const void *R, uint64_t nr, const void *S, uint64_t ns, uint32_t width, uint32_t radix_bits,
+                       uint32_t radix_bits_pass1, int kind, int threads, or_result_t *out) {
+	if (radix_bits_pass1 >= radix_bits)
+		return radix_join(R, nr, S, ns, width, radix_bits, 1, radix_bits, kind, threads, out);
+	return radix_join(R, nr, S, ns, width, radix_bits, 2, radix_bits_pass1, kind, threads, out);
+}
+
+static int radix_join(const void *R, uint64_t nr, const void *S, uint64_t ns, uint32_t width, uint32_t radix_bits,
+                      uint32_t passes, uint32_t b1, int kind, int threads, or_result_t *out) {
 	if (threads < 1) threads = 1;
 	if (threads > 256) threads = 256;
 	if (passes < 1 || passes > 2 || radix_bits > 20 || (passes == 2 && radix_bits < 2)) return -1;
+	if (passes == 2 && (b1 < 1 || b1 >= radix_bits)) return -1;
 	radix_shared_t sh;
 	memset(&sh, 0, sizeof sh);
 	sh.nt = threads;
@@ -762,7 +782,7 @@ int or_phj_radix(const void *R, uint64_t nr, const void *S, uint64_t ns, uint32_
 	sh.W = width;
 	sh.B = radix_bits;
 	sh.passes = passes;
-	sh.b1 = radix_bits / passes;
+	sh.b1 = passes == 2 ? b1 : radix_bits;
 	sh.b2 = radix_bits - sh.b1;
 	sh.nr = nr;
 	sh.ns = ns;

@@ -51,6 +51,11 @@ int or_nphj_lp(const void *R, uint64_t nr, const void *S, uint64_t ns, uint32_t 
 int or_phj_radix(const void *R, uint64_t nr, const void *S, uint64_t ns, uint32_t width, uint32_t radix_bits,
                  uint32_t passes, int kind, int threads, or_result_t *out);
 
+/* asymmetric two passes (src/algo/phj_asymm_radix.c): radix_bits_pass1 bits
+ * in the first pass; radix_bits_pass1 >= radix_bits runs one pass */
+int or_phj_asymm_radix(const void *R, uint64_t nr, const void *S, uint64_t ns, uint32_t width, uint32_t radix_bits,
+                       uint32_t radix_bits_pass1, int kind, int threads, or_result_t *out);
+
 /* stable radix partition of one relation by key bits [shift, shift+bits)
  * (the reference's parallel_radix_partition_optimized, phj.c:257-359, on
  * `threads` slices); hist receives 2^bits partition sizes */

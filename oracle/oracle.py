@@ -55,6 +55,8 @@ def lib():
             getattr(L, name).restype = I
         L.or_phj_radix.argtypes = [P, U64, P, U64, U32, U32, U32, I, I, C.POINTER(OrResult)]
         L.or_phj_radix.restype = I
+        L.or_phj_asymm_radix.argtypes = [P, U64, P, U64, U32, U32, U32, I, I, C.POINTER(OrResult)]
+        L.or_phj_asymm_radix.restype = I
         L.or_radix_partition.argtypes = [P, U64, U32, U32, U32, I, P, P]
         L.or_radix_partition.restype = I
         for name in ("or_aggregate", "or_bruteforce", "or_ptr_bruteforce"):
@@ -93,12 +95,20 @@ RADIX_KINDS = {"phj_radix_bc": 0, "phj_radix_lp": 1, "phj_radix_sc": 2, "phj_rad
 
 
 def join(algo: str, R: np.ndarray, S: np.ndarray, width: int, threads: int = 8, radix_bits: int = 11,
-         passes: int = 1) -> dict:
-    """Run the restated reference join `algo` on host tuples."""
+         passes: int = 1, radix_bits_pass1: int = 11) -> dict:
+    """Run the restated reference join `algo` on host tuples (asymmetric
+    names take radix_bits_pass1 first-pass bits of radix_bits)."""
     out = OrResult()
     nr, ns = R.nbytes // width, S.nbytes // width
     L = lib()
     algo = algo.replace("_bw_reg", "")
+    if algo.startswith("phj_asymm_radix_"):
+        kind = RADIX_KINDS["phj_radix_" + algo[len("phj_asymm_radix_"):]]
+        rc = L.or_phj_asymm_radix(_ptr(R), nr, _ptr(S), ns, width, radix_bits, radix_bits_pass1, kind, threads,
+                                  C.byref(out))
+        if rc:
+            raise RuntimeError(f"oracle {algo} failed: {rc}")
+        return _res(out)
     if algo == "nphj_sc":
         rc = L.or_nphj_sc(_ptr(R), nr, _ptr(S), ns, width, threads, C.byref(out))
     elif algo == "nphj_lp":

@@ -13,7 +13,9 @@ LIB = os.path.join(ROOT, "hashjoin-scm_amd", "libhjmi.so")
 CLI = os.path.join(ROOT, "hashjoin-scm_amd", "bin", "hjmain")
 REFERENCE_ALGOS = ["nphj_sc", "nphj_lp", "phj_radix_bc", "phj_radix_lp", "phj_radix_sc", "phj_radix_hr",
                    "phj_radix_hro", "phj_radix_bw_reg_bc", "phj_radix_bw_reg_lp", "phj_radix_bw_reg_sc",
-                   "phj_radix_bw_reg_hr", "phj_radix_bw_reg_hro", "ptr_nphj_sc", "ptr_nphj_lp", "ptr_phj_radix_bc",
+                   "phj_radix_bw_reg_hr", "phj_radix_bw_reg_hro", "phj_asymm_radix_bc", "phj_asymm_radix_sc",
+                   "phj_asymm_radix_hr", "phj_asymm_radix_hro", "phj_asymm_radix_bw_reg_bc",
+                   "phj_asymm_radix_bw_reg_sc", "phj_asymm_radix_bw_reg_hr", "phj_asymm_radix_bw_reg_hro", "ptr_nphj_sc", "ptr_nphj_lp", "ptr_phj_radix_bc",
                    "ptr_phj_radix_sc", "ptr_phj_radix_lp", "ptr_phj_radix_hr", "ptr_phj_radix_bw_reg_bc",
                    "ptr_phj_radix_bw_reg_sc", "ptr_phj_radix_bw_reg_lp", "ptr_phj_radix_bw_reg_hr"]
 

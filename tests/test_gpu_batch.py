@@ -116,6 +116,8 @@ def test_autotune_classes_and_parity(ctx):
     for c in range(10, 17):
         bits, passes = ctx.tuned_config(1 << c)
         assert bits <= c and passes in (0, 1, 2)
+        b1 = ctx.tuned_pass1_bits(1 << c)
+        assert b1 == 0 or (passes == 2 and 1 <= b1 < bits)
     R, S = ctx.generate("pkfk", 1 << 16, 1 << 20, 16)
     hR, hS = O.generate("pkfk", 1 << 16, 1 << 20, 16)
     want = O.pkfk_closed_form(hS, 16)

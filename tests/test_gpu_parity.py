@@ -205,3 +205,42 @@ def test_repeated_joins_reuse_scratch(ctx):
         _check(got, first, algo)
     ctx.free_relation(R)
     ctx.free_relation(S)
+
+
+ASYMM = ["phj_asymm_radix_bc", "phj_asymm_radix_sc", "phj_asymm_radix_hr", "phj_asymm_radix_hro"]
+
+
+@pytest.mark.parametrize("algo", ASYMM)
+def test_asymmetric_first_pass_bits(ctx, algo):
+    """phj_asymm_radix_*: NUM_RADIX_BITS_PASS1 first-pass bits (default 11),
+    the rest in the second pass; a first pass of all the bits is one pass.
+    Every split gives the oracle's result on duplicates and misses."""
+    R, S = O.generate("random", 30000, 200000, 16, key_range=20000, seed=9)
+    want = O.aggregate(R, S, 16)
+    _check(O.join(algo, R, S, 16, threads=4, radix_bits=14, radix_bits_pass1=9), want, "oracle split")
+    dR, dS = _dev(ctx, R, 16), _dev(ctx, S, 16)
+    for b1 in (2, 5, 7, 9, 11, 12):
+        got = ctx.join(algo, dR, dS, radix_bits=14, radix_bits_pass1=b1)
+        _check(got, want, (algo, b1))
+        assert got["passes"] == 2 and got["radix_bits"] == 14
+    got = ctx.join(algo, dR, dS, radix_bits=13)  # default first pass: 11 bits
+    _check(got, want, (algo, "default"))
+    assert got["passes"] == 2
+    got = ctx.join(algo, dR, dS, radix_bits=9)  # 11 >= 9: one pass of 9 bits
+    _check(got, want, (algo, "one pass"))
+    assert got["passes"] == 1
+    ctx.free_relation(dR)
+    ctx.free_relation(dS)
+
+
+def test_first_pass_bits_on_plain_radix(ctx):
+    R, S = O.generate("pkfk", 1 << 16, 1 << 20, 16, seed=3)
+    want = O.pkfk_closed_form(S, 16)
+    dR, dS = _dev(ctx, R, 16), _dev(ctx, S, 16)
+    for b1 in (1, 3, 8, 12):
+        _check(ctx.join("phj_radix_bc", dR, dS, radix_bits=13, passes=2, radix_bits_pass1=b1), want, b1)
+    with pytest.raises(hjmi.HJError) as e:
+        ctx.join("phj_radix_bc", dR, dS, radix_bits=13, passes=2, radix_bits_pass1=13)
+    assert e.value.code == hjmi.HJ_EINVAL
+    ctx.free_relation(dR)
+    ctx.free_relation(dS)

@@ -41,7 +41,7 @@ def test_reference_logged_pkfk(ctx, name, nr, ns):
 def test_reference_logged_payload(ctx, width):
     """payload_pkfk_16 / _64: 268435456 / 38280597101084672."""
     R, S = ctx.generate("payload_ref", 1 << 24, 1 << 28, width)
-    for algo in ("phj_radix_bc", "nphj_sc", "phj_radix_lp"):
+    for algo in ("phj_radix_bc", "nphj_sc", "phj_radix_lp", "phj_asymm_radix_bc"):
         _check(ctx.join(algo, R, S), 268435456, 38280597101084672, (width, algo))
     ctx.free_relation(R)
     ctx.free_relation(S)
@@ -75,7 +75,7 @@ def test_reference_logged_generator_dependent(ctx, name):
     hR, hS = hjmi.ref_generate(hjmi.ref_spec(name))
     R, S = ctx.upload(hR, 16), ctx.upload(hS, 16)
     del hR, hS
-    algos = ["phj_radix_bc", "nphj_sc", "phj_radix_hr", "phj_radix_sc", "phj_radix_hro"]
+    algos = ["phj_radix_bc", "nphj_sc", "phj_radix_hr", "phj_radix_sc", "phj_radix_hro", "phj_asymm_radix_bc"]
     for algo in algos:
         _check(ctx.join(algo, R, S), *LOGGED_REF[name], (name, algo))
     ctx.free_relation(R)

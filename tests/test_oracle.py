@@ -348,3 +348,19 @@ def test_ptr_rejects_only_matched_out_of_range_row_ids():
             O.ptr_join("ptr_phj_radix_bc", R.view(np.uint8), S2.view(np.uint8), 16, radix_bits=1)
         with pytest.raises(O.RowIdError):
             O.ptr_bruteforce(R.view(np.uint8), S2.view(np.uint8), 16)
+
+
+@pytest.mark.parametrize("b1", [1, 4, 6, 9])
+def test_restated_asymmetric_split(b1):
+    """phj_asymm_radix: any first-pass width gives the same pairs (the
+    in-partition hash uses all radix bits); checked against aggregation on
+    duplicates and misses, and the reference-style closed form."""
+    R, S = O.generate("random", 3000, 20000, 16, key_range=1500, seed=4)
+    want = O.aggregate(R, S, 16)
+    for algo in ("phj_asymm_radix_bc", "phj_asymm_radix_sc", "phj_asymm_radix_hr", "phj_asymm_radix_hro"):
+        got = O.join(algo, R, S, 16, threads=3, radix_bits=10, radix_bits_pass1=b1)
+        assert (got["matched_cnt"], got["check_sum"]) == (want["matched_cnt"], want["check_sum"]), algo
+    R, S = O.generate("pkfk_ref", 1 << 12, 1 << 16, 16)
+    got = O.join("phj_asymm_radix_bc", R, S, 16, threads=3, radix_bits=12, radix_bits_pass1=b1)
+    cf = O.pkfk_ref_closed_form(1 << 12, 1 << 16)
+    assert (got["matched_cnt"], got["check_sum"]) == (cf["matched_cnt"], cf["check_sum"])
