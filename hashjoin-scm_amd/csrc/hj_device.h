@@ -54,6 +54,16 @@ __device__ __forceinline__ void hj_store_stream(uint4 *p, const uint4 &v) {
 	hj_u32x4 t = {v.x, v.y, v.z, v.w};
 	__builtin_nontemporal_store(t, (HJ_GLOBAL hj_u32x4 *)p);
 }
+// 8-byte forms for narrow tuples {u32 key; u32 row_id}.
+typedef unsigned int hj_u32x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ uint2 hj_load_stream8(const uint2 *p) {
+	const hj_u32x2 v = __builtin_nontemporal_load((const HJ_GLOBAL hj_u32x2 *)p);
+	return make_uint2(v.x, v.y);
+}
+__device__ __forceinline__ void hj_store_stream8(uint2 *p, uint32_t a, uint32_t b) {
+	hj_u32x2 t = {a, b};
+	__builtin_nontemporal_store(t, (HJ_GLOBAL hj_u32x2 *)p);
+}
 // Plain (cached) global load.
 __device__ __forceinline__ uint4 hj_load(const uint4 *p) {
 	const hj_u32x4 v = *(const HJ_GLOBAL hj_u32x4 *)p;

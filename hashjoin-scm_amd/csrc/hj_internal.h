@@ -62,6 +62,10 @@ struct hj_ctx {
 	uint4 *emit_pairs = nullptr;
 	uint64_t emit_cap = 0;
 	unsigned emit_chunk = 0;
+	// two-pass radix joins of 16-byte tuples: pass B may write 8-byte
+	// {key, row_id} tuples when every key and row_id fits 32 bits
+	bool narrow_ok = false;
+	uint8_t *tile_narrow = nullptr;  // per pass-A tile: stored as 8-byte tuples
 	hj_buf pairs;      // the pair buffer of late-materialising joins
 	hj_buf keyrid[2];  // {key, row_id} copies of wide R and S
 
@@ -122,6 +126,7 @@ struct hj_stats {
 #define HJ_DF_POOL_FULL 2u    // SC overflow pool exhausted
 #define HJ_DF_OVERSIZE 4u     // LP partition larger than its LDS table
 #define HJ_DF_TABLE_FULL 8u   // LP global table full on insert
+#define HJ_DF_WIDE 16u        // a 16-byte tuple whose key or row_id exceeds 32 bits
 
 // ---- launchers (each returns HJ_OK or an error code) -----------------------
 int hj_fill16(hj_ctx_t *ctx, void *p, uint64_t n16, uint4 v);

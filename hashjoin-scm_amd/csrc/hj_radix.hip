@@ -510,9 +510,10 @@ __global__ __launch_bounds__(NT) void k_join(const uint4 *__restrict__ Rb, const
 // as in k_scatter).
 template <int WV, int NT, int ITEMS>
 __global__ __launch_bounds__(NT) void k_sortchunk(const hj_seg *__restrict__ segs, const uint32_t *__restrict__ cmap,
-                                                  const hj_stats *__restrict__ st, uint32_t shift, uint32_t bits,
+                                                  hj_stats *__restrict__ st, uint32_t shift, uint32_t bits,
                                                   uint4 *__restrict__ out, uint16_t *__restrict__ run,
-                                                  uint64_t *__restrict__ cstart) {
+                                                  uint64_t *__restrict__ cstart, bool narrow_ok,
+                                                  uint8_t *__restrict__ tnarrow) {
 	constexpr int TILE = NT * ITEMS;
 	extern __shared__ uint4 lds_u4[];
 	const uint32_t F = 1u << bits, fmask = F - 1;
@@ -551,21 +552,28 @@ __global__ __launch_bounds__(NT) void k_sortchunk(const hj_seg *__restrict__ seg
 			for (int w = 0; w < WV; w++) v[k][w] = hj_load_stream(src + (uint64_t)idx * WV + w);
 		}
 	};
+	bool wide = false;
 	auto do_tile = [&](uint4 (&v)[ITEMS][WV], uint32_t c) {
 		const uint4 *src;
 		uint64_t pos;
 		uint32_t n;
 		tile_of(c, src, pos, n);
 		uint32_t dig[ITEMS], rank[ITEMS];
+		bool tw = false;
 #pragma unroll
 		for (int k = 0; k < ITEMS; k++) {
 			const uint32_t idx = k * NT + threadIdx.x;
 			if (idx < n) {
 				dig[k] = (uint32_t)(hj_key_of(v[k][0]) >> shift) & fmask;
 				rank[k] = atomicAdd(&tcnt[dig[k]], 1u);
+				if (WV == 1) tw |= (v[k][0].y | v[k][0].w) != 0;
 			}
 		}
-		__syncthreads();
+		// a 16-byte tile whose keys and row ids all fit 32 bits is stored as
+		// 8-byte {key, row_id} tuples in the first half of its range
+		const bool any_wide = __syncthreads_or(tw);  // also the barrier after the counts
+		const bool tile_narrow = WV == 1 && narrow_ok && !any_wide;
+		wide |= tw;
 		{
 			const uint32_t p0 = threadIdx.x * per;
 			uint32_t sum = 0;
@@ -583,6 +591,7 @@ __global__ __launch_bounds__(NT) void k_sortchunk(const hj_seg *__restrict__ seg
 			if (threadIdx.x == 0 && n) {
 				run[(uint64_t)c * (F + 1) + F] = (uint16_t)n;
 				cstart[c] = pos;
+				if (WV == 1 && narrow_ok) tnarrow[c] = tile_narrow;
 			}
 		}
 		__syncthreads();
@@ -596,8 +605,13 @@ __global__ __launch_bounds__(NT) void k_sortchunk(const hj_seg *__restrict__ seg
 			}
 		}
 		__syncthreads();
-		uint4 *dst = out + pos * WV;
-		for (uint32_t u = threadIdx.x; u < n * WV; u += NT) hj_store_stream(&dst[u], tile[u]);
+		if (tile_narrow) {
+			uint2 *dst8 = (uint2 *)(out + pos);
+			for (uint32_t u = threadIdx.x; u < n; u += NT) hj_store_stream8(&dst8[u], tile[u].x, tile[u].z);
+		} else {
+			uint4 *dst = out + pos * WV;
+			for (uint32_t u = threadIdx.x; u < n * WV; u += NT) hj_store_stream(&dst[u], tile[u]);
+		}
 		load(v, c + 2 * grid);
 		for (uint32_t p = threadIdx.x; p < F; p += NT) tcnt[p] = 0;
 		__syncthreads();
@@ -611,6 +625,9 @@ __global__ __launch_bounds__(NT) void k_sortchunk(const hj_seg *__restrict__ seg
 		do_tile(va, c);
 		do_tile(vb, c + grid);  // an empty tile past the end changes nothing
 	}
+	// 16-byte tuples: flag a key or row_id above 32 bits (pass B then keeps
+	// 16-byte tuples)
+	if (WV == 1 && __any(wide) && (threadIdx.x & (HJ_WAVE - 1)) == 0) atomicOr(&st->flags, HJ_DF_WIDE);
 }
 
 // Two-pass joins partition without a histogram pass.  Pass A sorts every
@@ -705,10 +722,15 @@ __global__ __launch_bounds__(NT) void k_sortchunk_g(
     const hj_seg *__restrict__ segs2, const uint32_t *__restrict__ chunk_seg2, const hj_seg *__restrict__ segs1,
     const uint64_t *__restrict__ hist, const uint16_t *__restrict__ beg_t, const uint32_t *__restrict__ first,
     const uint32_t *__restrict__ order, const hj_stats *__restrict__ st, uint32_t b1, uint32_t bits,
-    const uint4 *__restrict__ in, uint4 *__restrict__ out, uint16_t *__restrict__ run, uint64_t *__restrict__ cstart) {
+    const uint4 *__restrict__ in, uint4 *__restrict__ out, uint16_t *__restrict__ run, uint64_t *__restrict__ cstart,
+    bool narrow_ok, const uint8_t *__restrict__ tnarrow) {
 	constexpr int TILE = NT * ITEMS;
 	extern __shared__ uint4 lds_u4[];
 	const uint32_t F = 1u << bits, fmask = F - 1, F1 = 1u << b1;
+	// 16-byte tuples whose keys and row ids all fit 32 bits (pass A found no
+	// wider one) leave as 8-byte {key, row_id}: half the writes here and
+	// half the reads of the join
+	const bool narrow = WV == 1 && narrow_ok && !(st->flags & HJ_DF_WIDE);
 	uint4 *tile = lds_u4;
 	uint32_t *src0 = (uint32_t *)(tile + TILE * WV), *src1 = src0 + TILE;
 	uint32_t *tcnt = src1 + TILE;
@@ -733,7 +755,8 @@ __global__ __launch_bounds__(NT) void k_sortchunk_g(
 	auto base_of = [&](uint32_t c) -> const uint4 * {
 		return c < total ? in + segs1[chunk_seg2[c] >> b1].out_start * WV : in;
 	};
-	// source offset of every tuple of tile c into srcb[0..n)
+	// source offset of every tuple of tile c into srcb[0..n); bit 31 marks a
+	// pass-A tile stored as 8-byte tuples (segments hold < 2^31 tuples then)
 	auto stage = [&](uint32_t c, uint32_t *srcb) {
 		uint64_t pos;
 		uint32_t n;
@@ -748,7 +771,8 @@ __global__ __launch_bounds__(NT) void k_sortchunk_g(
 			const uint32_t ii = i + threadIdx.x;
 			if (ii < g1.nchunks) {
 				const uint64_t H = row[ii], lo = max(H, pos), hi = min(row[ii + 1], pos + n);
-				const uint32_t sp = ii * (uint32_t)TILE + brow[ii] + (uint32_t)(lo - H);
+				uint32_t sp = ii * (uint32_t)TILE + brow[ii] + (uint32_t)(lo - H);
+				if (WV == 1 && narrow_ok && lo < hi && tnarrow[g1.chunk_base + ii]) sp |= 1u << 31;
 				for (uint64_t q = lo; q < hi; q++) srcb[q - pos] = sp + (uint32_t)(q - lo);
 			}
 			if (i + NT >= g1.nchunks || row[i + NT] >= pos + n) break;
@@ -762,11 +786,23 @@ __global__ __launch_bounds__(NT) void k_sortchunk_g(
 #pragma unroll
 		for (int k = 0; k < ITEMS; k++) {
 			const uint32_t f = min((uint32_t)(k * NT) + threadIdx.x, n ? n - 1 : 0u);
-			const uint64_t sp = n ? srcb[f] : 0;
+			const uint32_t e = n ? srcb[f] : 0;
 			// cached loads: a run's neighbours belong to tiles of nearby digits
 			// that the same XCD works on at the same time
+			if (WV == 1) {
+				// an 8-byte tuple at offset o of tile t lies at 8-byte index
+				// t*2*TILE + o = sp + t*TILE; the 16-byte load of its pair keeps
+				// one load per tuple either way
+				const bool nw = narrow_ok && (e >> 31);
+				const uint64_t sp = narrow_ok ? (e & 0x7fffffffu) : e;
+				const uint64_t u8 = sp + (sp & ~(uint64_t)(TILE - 1));
+				const uint4 h = hj_load(base + (nw ? u8 >> 1 : sp));
+				v[k][0] = !nw ? h : ((u8 & 1) ? make_uint4(h.z, 0u, h.w, 0u) : make_uint4(h.x, 0u, h.y, 0u));
+			} else {
+				const uint64_t sp = e;
 #pragma unroll
-			for (int w = 0; w < WV; w++) v[k][w] = hj_load(base + sp * WV + w);
+				for (int w = 0; w < WV; w++) v[k][w] = hj_load(base + sp * WV + w);
+			}
 		}
 	};
 	auto sort_store = [&](uint4 (&v)[ITEMS][WV], uint32_t c) {
@@ -813,8 +849,13 @@ __global__ __launch_bounds__(NT) void k_sortchunk_g(
 			}
 		}
 		__syncthreads();
-		uint4 *dst = out + pos * WV;
-		for (uint32_t u = threadIdx.x; u < n * WV; u += NT) hj_store_stream(&dst[u], tile[u]);
+		if (narrow) {
+			uint2 *dst8 = (uint2 *)out + pos;
+			for (uint32_t u = threadIdx.x; u < n; u += NT) hj_store_stream8(&dst8[u], tile[u].x, tile[u].z);
+		} else {
+			uint4 *dst = out + pos * WV;
+			for (uint32_t u = threadIdx.x; u < n * WV; u += NT) hj_store_stream(&dst[u], tile[u]);
+		}
 		for (uint32_t p = threadIdx.x; p < F; p += NT) tcnt[p] = 0;
 	};
 
@@ -891,14 +932,20 @@ __global__ __launch_bounds__(256) void k_run_transpose(const uint16_t *__restric
 // fill, and the host recomputes that join exactly).  Probe side: each wave
 // loads the next RU tuples per lane while it probes the current ones.
 // LDS: keys u64[cap] | pays u64[cap] | heads u32[np2(cap)] | next u16[cap]
-template <int WV, int NT, int TK, bool EMIT>
+template <int WV, int NT, int TK, bool EMIT, bool NARROW>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_join2(const uint4 *__restrict__ P, const uint64_t *__restrict__ cstart,
                                               const uint16_t *__restrict__ run_t, uint32_t stride, uint32_t b1,
                                               const hj_pair *__restrict__ pairs,
                                               const uint64_t *__restrict__ tprefix, uint32_t npairs, uint32_t B,
                                               uint32_t cpt, uint32_t cap, hj_stats *st, uint4 *pout,
-                                              unsigned long long pcap, unsigned pchunk) {
+                                              unsigned long long pcap, unsigned pchunk, bool narrow_ok) {
 	constexpr bool LP = TK == HJ_TK_LP;
+	// NARROW reads the 8-byte {key, row_id} tuples that pass B writes when
+	// every key and row id fits 32 bits (see k_sortchunk_g).  Which layout
+	// pass B chose is known on the device only, so both instances are
+	// launched and the one that does not match returns at once.
+	static_assert(!NARROW || WV == 1, "narrow tuples are 16-byte tuples packed");
+	if (NARROW != (WV == 1 && narrow_ok && !(st->flags & HJ_DF_WIDE))) return;
 	constexpr bool TWO = TK == HJ_TK_HR || TK == HJ_TK_HRO;  // count + place builds
 	constexpr uint32_t NW = NT / HJ_WAVE;
 	constexpr uint32_t RU = 6;                  // tuples per lane per step
@@ -957,8 +1004,15 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
 			const uint32_t f = f0 + u * HJ_WAVE + lane;
 			if (f < fe) {
 				while (r_off[k + 1] <= f) k++;
-				const uint4 *src = &P[(r_src[k] + (f - r_off[k])) * WV];
-				t[u] = stream ? hj_load_stream(src) : *src;
+				const uint64_t at = r_src[k] + (f - r_off[k]);
+				if constexpr (NARROW) {
+					const uint2 *src8 = (const uint2 *)P + at;
+					const uint2 h = stream ? hj_load_stream8(src8) : *src8;
+					t[u] = make_uint4(h.x, 0u, h.y, 0u);
+				} else {
+					const uint4 *src = &P[at * WV];
+					t[u] = stream ? hj_load_stream(src) : *src;
+				}
 			}
 		}
 	};
@@ -1242,7 +1296,8 @@ static size_t join_static_lds(int tk, bool two_pass) {
 	// every width of one kernel template has the same __shared__ arrays
 	size_t r = 16384;
 #define S(K) \
-	r = two_pass ? static_lds((const void *)k_join2<1, 512, K, false>) : static_lds((const void *)k_join<1, 256, K, false>)
+	r = two_pass ? static_lds((const void *)k_join2<1, 512, K, false, false>) \
+	             : static_lds((const void *)k_join<1, 256, K, false>)
 	HJ_TK_SWITCH(tk, S)
 #undef S
 	return r;
@@ -1309,8 +1364,8 @@ static void launch_sortchunk_t(hj_ctx_t *ctx, unsigned grid, size_t lds, const h
 	auto kfn = k_sortchunk<WV, NT, ITEMS>;
 	allow_lds(ctx, (const void *)kfn);
 	grid = std::max(1u, std::min(grid, resident_grid(ctx, (const void *)kfn, NT, lds)));
-	hipLaunchKernelGGL(kfn, dim3(grid), dim3(NT), lds, ctx->stream, segs, cmap, (const hj_stats *)ctx->stats.p, shift,
-	                   bits, (uint4 *)out, run, cst);
+	hipLaunchKernelGGL(kfn, dim3(grid), dim3(NT), lds, ctx->stream, segs, cmap, (hj_stats *)ctx->stats.p, shift,
+	                   bits, (uint4 *)out, run, cst, ctx->narrow_ok, ctx->tile_narrow);
 }
 
 static void launch_sortchunk(hj_ctx_t *ctx, uint32_t wv, unsigned grid, size_t lds, const hj_seg *segs,
@@ -1336,7 +1391,8 @@ static void launch_sortchunk_g_t(hj_ctx_t *ctx, unsigned grid, size_t lds, const
 	// all resident workgroups, a multiple of the 8 XCDs (the kernel's mapping)
 	grid = std::max(8u, resident_grid(ctx, (const void *)kfn, NT, lds) & ~7u);
 	hipLaunchKernelGGL(kfn, dim3(grid), dim3(NT), lds, ctx->stream, segs2, cmap2, segs1, hist, beg_t, first, order,
-	                   (const hj_stats *)ctx->stats.p, b1, bits, (const uint4 *)in, (uint4 *)out, run, cst);
+	                   (const hj_stats *)ctx->stats.p, b1, bits, (const uint4 *)in, (uint4 *)out, run, cst,
+	                   ctx->narrow_ok, (const uint8_t *)ctx->tile_narrow);
 }
 
 // LDS of k_sortchunk_g: tile, two source-offset buffers, counters
@@ -1366,13 +1422,27 @@ template <int WV, int TK>
 static void launch_join2_t(hj_ctx_t *ctx, unsigned grid, size_t lds, const void *P, const uint64_t *cst,
                            const uint16_t *run_t, uint32_t stride, uint32_t b1, const hj_pair *pairs,
                            const uint64_t *tpre, uint32_t npairs, uint32_t B, uint32_t cpt, uint32_t cap) {
-	auto kfn = k_join2<WV, 512, TK, false>;
-	if constexpr (WV == 1 && TK != HJ_TK_HRO)
-		if (ctx->emit_pairs) kfn = k_join2<1, 512, TK, true>;
-	allow_lds(ctx, (const void *)kfn);
-	hipLaunchKernelGGL(kfn, dim3(grid), dim3(512), lds, ctx->stream, (const uint4 *)P, cst, run_t, stride, b1, pairs,
-	                   tpre, npairs, B, cpt, cap, (hj_stats *)ctx->stats.p, ctx->emit_pairs,
-	                   (unsigned long long)ctx->emit_cap, ctx->emit_chunk);
+	auto launch = [&](auto kfn) {
+		allow_lds(ctx, (const void *)kfn);
+		hipLaunchKernelGGL(kfn, dim3(grid), dim3(512), lds, ctx->stream, (const uint4 *)P, cst, run_t, stride, b1,
+		                   pairs, tpre, npairs, B, cpt, cap, (hj_stats *)ctx->stats.p, ctx->emit_pairs,
+		                   (unsigned long long)ctx->emit_cap, ctx->emit_chunk, ctx->narrow_ok);
+	};
+	if constexpr (WV == 1) {
+		// pass B's layout is decided on the device: one of the two returns
+		const bool emit = TK != HJ_TK_HRO && ctx->emit_pairs;
+		if (emit) {
+			if constexpr (TK != HJ_TK_HRO) {
+				if (ctx->narrow_ok) launch(k_join2<1, 512, TK, true, true>);
+				launch(k_join2<1, 512, TK, true, false>);
+			}
+		} else {
+			if (ctx->narrow_ok) launch(k_join2<1, 512, TK, false, true>);
+			launch(k_join2<1, 512, TK, false, false>);
+		}
+	} else {
+		launch(k_join2<WV, 512, TK, false, false>);
+	}
 }
 
 static void launch_join2(hj_ctx_t *ctx, uint32_t wv, int tk, unsigned grid, size_t lds, const void *P,
@@ -1451,6 +1521,8 @@ int hj_run_radix(hj_ctx_t *ctx, const hj_relation_t *R, const hj_relation_t *S, 
 	out->passes = P;
 	hj_stats *st = (hj_stats *)ctx->stats.p;
 	const uint32_t variant = prm->part_variant;
+	// narrow tuples mark pass-A sources in bit 31 of a segment offset
+	ctx->narrow_ok = wv == 1 && !(prm->flags & HJ_F_NO_NARROW) && nr < (1ull << 31) && ns < (1ull << 31);
 	const part_cfg pc = cfg_for(wv, variant);
 
 	// chunk size: a multiple of the scatter tile, ~4 chunks per CU
@@ -1497,6 +1569,7 @@ int hj_run_radix(hj_ctx_t *ctx, const hj_relation_t *R, const hj_relation_t *S, 
 	const size_t o_first = take(P == 2 ? chunks2_max * 4 : 8);
 	const size_t o_runt = take(P == 2 ? chunks2_max * (F2 + 1) * 2 : 8);
 	const size_t o_order = take(P == 2 ? chunks2_max * 4 : 8);
+	const size_t o_tnar = take(P == 2 ? chunks1 : 8);
 	if ((rc = hj_reserve(ctx, ctx->meta, off))) return rc;
 	char *meta = (char *)ctx->meta.p;
 	hj_seg *segs1 = (hj_seg *)(meta + o_seg1), *segs2 = (hj_seg *)(meta + o_seg2);
@@ -1509,6 +1582,7 @@ int hj_run_radix(hj_ctx_t *ctx, const hj_relation_t *R, const hj_relation_t *S, 
 	uint32_t *first = (uint32_t *)(meta + o_first);
 	uint16_t *beg_t = (uint16_t *)(meta + o_begt), *run_t = (uint16_t *)(meta + o_runt);
 	uint32_t *order = (uint32_t *)(meta + o_order);
+	ctx->tile_narrow = (uint8_t *)(meta + o_tnar);
 	uint64_t *hist = (uint64_t *)ctx->hist.p;
 
 	HJ_HIP(ctx, hipEventRecord(ctx->ev[0], ctx->stream));

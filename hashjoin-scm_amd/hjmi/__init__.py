@@ -24,6 +24,7 @@ LIB_PATH = os.path.join(_PKG_DIR, "libhjmi.so")
 HJ_OK, HJ_EINVAL, HJ_ENOMEM, HJ_EHIP, HJ_EREJECT, HJ_ENODEV, HJ_EBUSY = 0, -1, -2, -3, -4, -5, -6
 HJ_LOC_DEVICE, HJ_LOC_HOST = 0, 1
 HJ_F_KERNEL_TIMES = 1
+HJ_F_NO_NARROW = 2
 HJ_FB_LP_DUPLICATES, HJ_FB_LP_OVERSIZE, HJ_FB_SC_REBUILD = 1, 2, 4
 
 WL_PKFK_UNIFORM, WL_PKFK_REF, WL_PAYLOAD_REF, WL_ZIPF, WL_RANDOM = 0, 1, 2, 3, 4
@@ -304,7 +305,7 @@ class Context:
 
     # -- join
     def join(self, algo: str, R, S, radix_bits: int = 0, passes: int = 0, lp_slots_log2: int = 0,
-             kernel_times: bool = False, part_variant: int = 0, radix_bits_pass1: int = 0) -> dict:
+             kernel_times: bool = False, part_variant: int = 0, radix_bits_pass1: int = 0, flags: int = 0) -> dict:
         """Run one join; R/S are DeviceRelation or host numpy uint8 arrays."""
         rr, keep = [], []
         for rel, ts in ((R, None), (S, None)):
@@ -312,8 +313,8 @@ class Context:
                 rr.append(rel.as_c())
             else:
                 raise TypeError("pass DeviceRelation (use Context.upload or Context.host_relation)")
-        prm = Params(radix_bits, passes, lp_slots_log2, HJ_F_KERNEL_TIMES if kernel_times else 0, part_variant,
-                     radix_bits_pass1)
+        prm = Params(radix_bits, passes, lp_slots_log2, (HJ_F_KERNEL_TIMES if kernel_times else 0) | flags,
+                     part_variant, radix_bits_pass1)
         res = Result()
         self._check(self.lib.hj_join(self.h, algo.encode(), C.byref(rr[0]), C.byref(rr[1]), C.byref(prm),
                                      C.byref(res)), f"hj_join({algo})")

@@ -72,6 +72,8 @@ typedef struct {
 } hj_params_t;
 
 #define HJ_F_KERNEL_TIMES 1u  /* record a HIP event pair around every kernel */
+#define HJ_F_NO_NARROW 2u     /* keep 16-byte tuples through the second radix
+                                 pass even when keys and row ids fit 32 bits */
 
 /* ---- result: matched_cnt/check_sum as printed by the reference plus the
  *      timekeeper_t phase times (src/inc/store.h:74-78) ---------------------- */

@@ -244,3 +244,34 @@ def test_first_pass_bits_on_plain_radix(ctx):
     assert e.value.code == hjmi.HJ_EINVAL
     ctx.free_relation(dR)
     ctx.free_relation(dS)
+
+
+@pytest.mark.parametrize("algo", ["phj_radix_bc", "phj_radix_lp", "phj_radix_hr", "ptr_phj_radix_bc"])
+def test_narrow_second_pass_matches_wide(ctx, algo):
+    """Two passes over 16-byte tuples whose keys and row ids fit 32 bits
+    write 8-byte tuples from pass B on; one wide value anywhere (here the
+    last probe tuple's row id, or a build key) keeps 16-byte tuples.  Every
+    variant equals the oracle."""
+    nr, ns = 1 << 15, 1 << 19
+    R, S = O.generate("pkfk", nr, ns, 16, seed=21)
+    cases = [("narrow", R, S)]
+    S2 = S.copy()
+    S2.view(np.uint64)[-1] = np.uint64(1 << 40)  # last S tuple: row id above 32 bits
+    R2 = R.copy()
+    R2.view(np.uint64)[2 * 100] += np.uint64(1 << 33)  # one build key above 32 bits
+    cases += [("wide row id", R, S2), ("wide key", R2, S)]
+    for name, hR, hS in cases:
+        if algo.startswith("ptr_"):
+            want = O.ptr_direct_address(hR, hS, 16) if name == "narrow" else None
+            if want is None:
+                continue
+        else:
+            want = O.aggregate(hR, hS, 16, lp=algo.endswith("lp"))
+        dR, dS = _dev(ctx, hR, 16), _dev(ctx, hS, 16)
+        for bits in (12, 14):
+            a = ctx.join(algo, dR, dS, radix_bits=bits, passes=2)
+            b = ctx.join(algo, dR, dS, radix_bits=bits, passes=2, flags=hjmi.HJ_F_NO_NARROW)
+            _check(a, want, (name, bits))
+            _check(b, want, (name, bits, "no narrow"))
+        ctx.free_relation(dR)
+        ctx.free_relation(dS)
