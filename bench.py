@@ -41,19 +41,27 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def kernel_bytes(name, W, nr, ns):
+def kernel_bytes(name, W, nr, ns, res=None):
     """Algorithmic HBM bytes one launch of `name` must move (no histogram
     passes, table zeroing or partial lines): a radix scatter reads and writes
     every tuple once, the radix join reads every tuple once, an NPHJ build
     reads R and writes a 16-byte entry per tuple, a probe reads S and one
-    16-byte entry per probe."""
+    16-byte entry per probe.  Two passes over 16-byte tuples store a pass-A
+    tile as 8-byte {key, row_id} tuples when they fit 32 bits
+    (res["narrow_tuples"] tuples), and pass B and the join then move 8-byte
+    tuples when every tile did (res["narrow"])."""
     N = nr + ns
+    a_out = b_out = W * N
+    if res is not None and W == 16 and res.get("passes") == 2:
+        nt = res.get("narrow_tuples", 0)
+        a_out = 8 * nt + 16 * (N - nt)
+        b_out = (8 if res.get("narrow") else 16) * N
     return {
         "radix_scatter": 2 * W * N,
         "radix_sortchunk": 2 * W * N,
-        "radix_tilesort": 2 * W * N,
-        "radix_gathersort": 2 * W * N,
-        "radix_join": W * N,
+        "radix_tilesort": W * N + a_out,
+        "radix_gathersort": a_out + b_out,
+        "radix_join": b_out,
         "lp_build": (W + 16) * nr,
         "sc_build": (W + 16) * nr,
         "lp_probe": (W + 16) * ns,
@@ -331,13 +339,15 @@ def main():
             cnt[k] = cnt.get(k, 0) + len(lst)
     dom = max((k for k in tot if kernel_bytes(k, W, nr, ns)), key=lambda k: tot[k])
     avg_ms = tot[dom] / cnt[dom]
-    achieved = kernel_bytes(dom, W, nr, ns) / (avg_ms * 1e-3) / 1e9
+    achieved = kernel_bytes(dom, W, nr, ns, res[0]) / (avg_ms * 1e-3) / 1e9
     job_s = sum(r["total_time"] for r in res) / len(res)
     job_gbps = res[0]["algo_bytes"] / job_s / 1e9
     tag = f"{args.config}_{algo}"
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_GBPS, "unit": "GB/s",
                 "frac": round(achieved / PEAK_GBPS, 4), "traffic": load_traffic(dom, tag),
                 "kernel": dom, "kernel_avg_ms": round(avg_ms, 4),
+                "kernel_algo_bytes": kernel_bytes(dom, W, nr, ns, res[0]),
+                "narrow_tuples": res[0].get("narrow_tuples", 0), "narrow": res[0].get("narrow", 0),
                 "job_algo_bytes": res[0]["algo_bytes"], "job_device_ms": round(job_s * 1e3, 4),
                 "job_frac": round(job_gbps / PEAK_GBPS, 4),
                 "kernel_ms_per_step": {k: round(v / args.steps, 4) for k, v in sorted(tot.items())}}

@@ -119,7 +119,8 @@ struct hj_stats {
 	unsigned long long pair_next;  // late materialisation: row-id pairs emitted
 	unsigned long long retrieved;  // late materialisation: retrieve checksum
 	unsigned long long retrieved_ok; // pairs whose row ids lie inside R and S
-	unsigned long long pad1[3];
+	unsigned long long narrow_tuples; // tuples of pass-A tiles stored as 8 bytes
+	unsigned long long pad1[2];
 };
 
 #define HJ_DF_DUPKEY 1u       // duplicate build key met in an LP table

@@ -553,6 +553,7 @@ __global__ __launch_bounds__(NT) void k_sortchunk(const hj_seg *__restrict__ seg
 		}
 	};
 	bool wide = false;
+	unsigned long long narrow_cnt = 0;  // thread 0: tuples of this workgroup's narrow tiles
 	auto do_tile = [&](uint4 (&v)[ITEMS][WV], uint32_t c) {
 		const uint4 *src;
 		uint64_t pos;
@@ -591,7 +592,10 @@ __global__ __launch_bounds__(NT) void k_sortchunk(const hj_seg *__restrict__ seg
 			if (threadIdx.x == 0 && n) {
 				run[(uint64_t)c * (F + 1) + F] = (uint16_t)n;
 				cstart[c] = pos;
-				if (WV == 1 && narrow_ok) tnarrow[c] = tile_narrow;
+				if (WV == 1 && narrow_ok) {
+					tnarrow[c] = tile_narrow;
+					narrow_cnt += tile_narrow ? n : 0u;
+				}
 			}
 		}
 		__syncthreads();
@@ -628,6 +632,8 @@ __global__ __launch_bounds__(NT) void k_sortchunk(const hj_seg *__restrict__ seg
 	// 16-byte tuples: flag a key or row_id above 32 bits (pass B then keeps
 	// 16-byte tuples)
 	if (WV == 1 && __any(wide) && (threadIdx.x & (HJ_WAVE - 1)) == 0) atomicOr(&st->flags, HJ_DF_WIDE);
+	// one atomic per workgroup (one per tile on a single address serialises)
+	if (WV == 1 && threadIdx.x == 0 && narrow_cnt) atomicAdd(&st->narrow_tuples, narrow_cnt);
 }
 
 // Two-pass joins partition without a histogram pass.  Pass A sorts every
@@ -1683,6 +1689,15 @@ int hj_run_radix(hj_ctx_t *ctx, const hj_relation_t *R, const hj_relation_t *S, 
 	out->probejoin_time = hj_event_seconds(ctx, 1, 2);
 	out->total_time = hj_event_seconds(ctx, 0, 2);
 	out->algo_bytes = (uint64_t)(2 * P + 1) * W * N;
+	if (P == 2 && ctx->narrow_ok) {
+		// pass A reads 16 B and writes 8 or 16 B per tuple; pass B reads
+		// that and writes 8 B when no tile was wide; the join reads pass B's
+		const uint64_t nt = h.narrow_tuples, a_out = 8 * nt + 16 * (N - nt);
+		out->narrow_tuples = nt;
+		out->narrow = !(h.flags & HJ_DF_WIDE);
+		const uint64_t b_out = (out->narrow ? 8 : 16) * N;
+		out->algo_bytes = 16 * N + a_out + a_out + b_out + b_out;
+	}
 	if (lp && (h.flags & (HJ_DF_DUPKEY | HJ_DF_OVERSIZE))) {
 		// Exact path: linear probing keeps only the first build tuple per key
 		// (src/algo/phj.c:655-666 with a stable partition = smallest R row),

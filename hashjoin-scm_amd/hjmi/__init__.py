@@ -60,7 +60,8 @@ class Result(C.Structure):
                 ("probejoin_time", C.c_double), ("total_time", C.c_double), ("algo_bytes", C.c_uint64),
                 ("radix_bits", C.c_uint32), ("passes", C.c_uint32), ("fallback", C.c_uint32),
                 ("n_ktimes", C.c_uint32), ("ktime_ms", C.c_double * MAX_KTIMES),
-                ("ktime_id", C.c_uint32 * MAX_KTIMES)]
+                ("ktime_id", C.c_uint32 * MAX_KTIMES), ("narrow_tuples", C.c_uint64), ("narrow", C.c_uint32),
+                ("reserved", C.c_uint32)]
 
 
 class GenSpec(C.Structure):
@@ -381,6 +382,7 @@ def result_dict(res: Result) -> dict:
         "buildpart_time": res.buildpart_time, "probejoin_time": res.probejoin_time,
         "total_time": res.total_time, "algo_bytes": res.algo_bytes, "radix_bits": res.radix_bits,
         "passes": res.passes, "fallback": res.fallback, "kernel_ms": kt,
+        "narrow_tuples": res.narrow_tuples, "narrow": res.narrow,
     }
 
 

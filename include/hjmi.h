@@ -91,6 +91,11 @@ typedef struct {
 	uint32_t n_ktimes;
 	double ktime_ms[HJ_MAX_KTIMES];  /* per-kernel device ms (HJ_F_KERNEL_TIMES) */
 	uint32_t ktime_id[HJ_MAX_KTIMES];
+	uint64_t narrow_tuples;  /* two-pass radix, 16-byte tuples: tuples that the
+	                            first pass stored as 8-byte {key, row_id}     */
+	uint32_t narrow;         /* 1: the second pass and the join moved 8-byte
+	                            tuples (every key and row id fit 32 bits)     */
+	uint32_t reserved;
 } hj_result_t;
 
 #define HJ_FB_LP_DUPLICATES 1u  /* duplicate build keys under LP: re-run exact */
