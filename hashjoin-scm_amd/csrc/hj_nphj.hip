@@ -231,7 +231,8 @@ static int lp_build_table(hj_ctx_t *ctx, const hj_relation_t *R, unsigned long l
 	case 2: LP_B(2); break;
 	case 4: LP_B(4); break;
 	case 8: LP_B(8); break;
-	default: LP_B(16); break;
+	case 16: LP_B(16); break;
+	default: LP_B(32); break;
 	}
 #undef LP_B
 	hj_kt_end(ctx);
@@ -257,7 +258,8 @@ static int lp_build_table(hj_ctx_t *ctx, const hj_relation_t *R, unsigned long l
 		case 2: LP_F(2); break;
 		case 4: LP_F(4); break;
 		case 8: LP_F(8); break;
-		default: LP_F(16); break;
+		case 16: LP_F(16); break;
+		default: LP_F(32); break;
 		}
 #undef LP_F
 		hj_kt_end(ctx);
@@ -304,7 +306,8 @@ int hj_run_nphj_lp(hj_ctx_t *ctx, const hj_relation_t *R, const hj_relation_t *S
 	case 2: LP_P(2, false); break;
 	case 4: LP_P(4, false); break;
 	case 8: LP_P(8, false); break;
-	default: LP_P(16, false); break;
+	case 16: LP_P(16, false); break;
+	default: LP_P(32, false); break;
 	}
 #undef LP_P
 	hj_kt_end(ctx);
@@ -460,7 +463,8 @@ int hj_run_nphj_sc(hj_ctx_t *ctx, const hj_relation_t *R, const hj_relation_t *S
 		case 2: SC_B(2); break;
 		case 4: SC_B(4); break;
 		case 8: SC_B(8); break;
-		default: SC_B(16); break;
+		case 16: SC_B(16); break;
+		default: SC_B(32); break;
 		}
 #undef SC_B
 		hj_kt_end(ctx);
@@ -483,7 +487,8 @@ int hj_run_nphj_sc(hj_ctx_t *ctx, const hj_relation_t *R, const hj_relation_t *S
 		case 2: SC_P(2, false); break;
 		case 4: SC_P(4, false); break;
 		case 8: SC_P(8, false); break;
-		default: SC_P(16, false); break;
+		case 16: SC_P(16, false); break;
+		default: SC_P(32, false); break;
 		}
 #undef SC_P
 		hj_kt_end(ctx);

@@ -1357,7 +1357,8 @@ static void launch_join(hj_ctx_t *ctx, uint32_t wv, int tk, unsigned grid, size_
 	case 2: J(2, K); break;      \
 	case 4: J(4, K); break;      \
 	case 8: J(8, K); break;      \
-	default: J(16, K); break;    \
+	case 16: J(16, K); break;    \
+	default: J(32, K); break;    \
 	}
 	HJ_TK_SWITCH(tk, JW)
 #undef JW
@@ -1383,7 +1384,8 @@ static void launch_sortchunk(hj_ctx_t *ctx, uint32_t wv, unsigned grid, size_t l
 	case 2: launch_sortchunk_t<2, 512, 2>(ctx, grid, lds, segs, cmap, shift, bits, out, run, cst); break;
 	case 4: launch_sortchunk_t<4, 512, 1>(ctx, grid, lds, segs, cmap, shift, bits, out, run, cst); break;
 	case 8: launch_sortchunk_t<8, 256, 1>(ctx, grid, lds, segs, cmap, shift, bits, out, run, cst); break;
-	default: launch_sortchunk_t<16, 256, 1>(ctx, grid, lds, segs, cmap, shift, bits, out, run, cst); break;
+	case 16: launch_sortchunk_t<16, 256, 1>(ctx, grid, lds, segs, cmap, shift, bits, out, run, cst); break;
+	default: launch_sortchunk_t<32, 256, 1>(ctx, grid, lds, segs, cmap, shift, bits, out, run, cst); break;
 	}
 }
 
@@ -1419,7 +1421,8 @@ static void launch_sortchunk_g(hj_ctx_t *ctx, uint32_t wv, unsigned grid, size_t
 	case 2: G(2, 512, 2); break;
 	case 4: G(4, 512, 1); break;
 	case 8: G(8, 256, 1); break;
-	default: G(16, 256, 1); break;
+	case 16: G(16, 256, 1); break;
+	default: G(32, 256, 1); break;
 	}
 #undef G
 }
@@ -1464,7 +1467,8 @@ static void launch_join2(hj_ctx_t *ctx, uint32_t wv, int tk, unsigned grid, size
 	case 2: J(2, K); break;      \
 	case 4: J(4, K); break;      \
 	case 8: J(8, K); break;      \
-	default: J(16, K); break;    \
+	case 16: J(16, K); break;    \
+	default: J(32, K); break;    \
 	}
 	HJ_TK_SWITCH(tk, JW)
 #undef JW
@@ -1505,6 +1509,13 @@ int hj_run_radix(hj_ctx_t *ctx, const hj_relation_t *R, const hj_relation_t *S, 
 	if (P == 0) P = B == 0 ? 0 : (B > 11 ? 2 : 1);
 	if (B == 0) P = 0;
 	if (P == 2 && B < 2) P = 1;
+	// 512-byte tuples are partitioned by the tile sorts only (a one-pass
+	// scatter tile of them would not leave LDS for its write-combining
+	// lines); more radix bits never change the result
+	if (wv == 32 && P == 1) {
+		B = std::max(B, 2u);
+		P = 2;
+	}
 	// two passes: by default the first takes the larger half of the bits, so
 	// the chunk-local second pass sorts each tile into at most as many runs
 	// as pass 1; the tuner or the caller (NUM_RADIX_BITS_PASS1) may choose
@@ -2906,6 +2917,8 @@ extern "C" int hj_serve_submit(hj_server_t *s, int32_t build_id, const hj_relati
 	r.S = *S;
 	if (s->W == 0) s->W = r.R.tuple_size;
 	if (r.R.tuple_size != s->W || r.S.tuple_size != s->W) return hj_fail(ctx, HJ_EINVAL, "tuple sizes must match the server's");
+	if (s->W != 16 && s->W != 32 && s->W != 64 && s->W != 128 && s->W != 256)
+		return hj_fail(ctx, HJ_EINVAL, "tuple size %u", s->W);
 	if (r.R.location != HJ_LOC_DEVICE || r.S.location != HJ_LOC_DEVICE || (r.R.tuple_num && !r.R.tuples) ||
 	    (r.S.tuple_num && !r.S.tuples))
 		return hj_fail(ctx, HJ_EINVAL, "device relations required");

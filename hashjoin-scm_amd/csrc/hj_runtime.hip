@@ -264,10 +264,8 @@ bool hj_algo_is_lp(const char *algo, bool *radix) {
 	return n >= 3 && strcmp(algo + n - 3, "_lp") == 0;
 }
 
-static bool width_ok(uint32_t w) { return w == 16 || w == 32 || w == 64 || w == 128 || w == 256; }
-// late materialisation reads only the 16-byte head and one row id of a
-// tuple, so it also takes the reference's widest payload tuples
-static bool ptr_width_ok(uint32_t w) { return width_ok(w) || w == 512; }
+// the reference's payload sweep: 16..512-byte tuples (logs/payload-logs)
+static bool width_ok(uint32_t w) { return w == 16 || w == 32 || w == 64 || w == 128 || w == 256 || w == 512; }
 
 int hj_join(hj_ctx_t *ctx, const char *algo, const hj_relation_t *R, const hj_relation_t *S,
             const hj_params_t *params, hj_result_t *out) {
@@ -277,10 +275,9 @@ int hj_join(hj_ctx_t *ctx, const char *algo, const hj_relation_t *R, const hj_re
 	for (const hj_algo_entry *a = k_algos; a->name; a++)
 		if (strcmp(a->name, algo) == 0) e = a;
 	if (!e) return hj_fail(ctx, HJ_EINVAL, "unknown algo '%s'", algo);
-	const bool ptr = strncmp(algo, "ptr_", 4) == 0;
-	if (!(ptr ? ptr_width_ok(R->tuple_size) : width_ok(R->tuple_size)) || R->tuple_size != S->tuple_size)
-		return hj_fail(ctx, HJ_EINVAL, "tuple sizes %u/%u unsupported (16..%u, equal)", R->tuple_size,
-		               S->tuple_size, ptr ? 512u : 256u);
+	if (!width_ok(R->tuple_size) || R->tuple_size != S->tuple_size)
+		return hj_fail(ctx, HJ_EINVAL, "tuple sizes %u/%u unsupported (16..512, equal)", R->tuple_size,
+		               S->tuple_size);
 	if ((R->tuple_num && !R->tuples) || (S->tuple_num && !S->tuples))
 		return hj_fail(ctx, HJ_EINVAL, "null tuples pointer");
 	if (R->location > HJ_LOC_HOST || S->location > HJ_LOC_HOST) return hj_fail(ctx, HJ_EINVAL, "bad location");
@@ -362,7 +359,7 @@ int hj_sync(hj_ctx_t *ctx) {
 
 int hj_generate(hj_ctx_t *ctx, const hj_gen_spec_t *spec, void *r_dev, void *s_dev) {
 	if (!ctx || !spec) return HJ_EINVAL;
-	if (!ptr_width_ok(spec->width)) return hj_fail(ctx, HJ_EINVAL, "width %u unsupported", spec->width);
+	if (!width_ok(spec->width)) return hj_fail(ctx, HJ_EINVAL, "width %u unsupported", spec->width);
 	if ((spec->r_num && !r_dev) || (spec->s_num && !s_dev)) return hj_fail(ctx, HJ_EINVAL, "null buffer");
 	if (spec->workload > HJ_WL_RANDOM) return hj_fail(ctx, HJ_EINVAL, "unknown workload %u", spec->workload);
 	if (spec->workload == HJ_WL_RANDOM && spec->key_range == 0)

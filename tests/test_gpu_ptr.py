@@ -116,13 +116,29 @@ def test_ptr_empty_and_host_relations(ctx):
         assert _pair(ctx.join_host(algo, hR, empty, 64)) == (0, 0), algo
 
 
-def test_plain_joins_reject_512_byte_tuples(ctx):
-    R, S = ctx.generate("payload_ref", 256, 4096, 512)
-    with pytest.raises(hjmi.HJError) as e:
-        ctx.join("phj_radix_bc", R, S)
-    assert e.value.code == hjmi.HJ_EINVAL
-    want = O.payload_ref_closed_form(256, 4096)
-    assert _pair(ctx.join("ptr_phj_radix_bc", R, S)) == (want["matched_cnt"], want["check_sum"])
+def test_plain_joins_on_512_byte_tuples(ctx):
+    """The payload sweep's widest tuples through the plain joins too (hro
+    takes at most 64 bytes); a one-pass request runs as two tile sorts."""
+    nr, ns = 1 << 12, 1 << 16
+    R, S = ctx.generate("payload_ref", nr, ns, 512)
+    want = O.payload_ref_closed_form(nr, ns)
+    for algo in hjmi.ALGOS:
+        if algo.endswith("hro"):
+            with pytest.raises(hjmi.HJError) as e:
+                ctx.join(algo, R, S)
+            assert e.value.code == hjmi.HJ_EINVAL
+            continue
+        assert _pair(ctx.join(algo, R, S)) == (want["matched_cnt"], want["check_sum"]), algo
+    for bits, passes in ((0, 0), (1, 1), (6, 1), (6, 2), (10, 2)):
+        got = ctx.join("phj_radix_bc", R, S, radix_bits=bits, passes=passes)
+        assert _pair(got) == (want["matched_cnt"], want["check_sum"]), (bits, passes)
+    hR, hS = _pointer_relations(8, 2000, 9000, 512, 700)
+    for algo in ("phj_radix_bc", "phj_radix_lp", "nphj_sc", "phj_radix_hr"):
+        w = O.aggregate(hR, hS, 512, lp=algo.endswith("lp"))
+        dR, dS = ctx.upload(hR, 512), ctx.upload(hS, 512)
+        assert _pair(ctx.join(algo, dR, dS, radix_bits=6, passes=2)) == (w["matched_cnt"], w["check_sum"]), algo
+        ctx.free_relation(dR)
+        ctx.free_relation(dS)
     ctx.free_relation(R)
     ctx.free_relation(S)
 

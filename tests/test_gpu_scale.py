@@ -47,6 +47,25 @@ def test_reference_logged_payload(ctx, width):
     ctx.free_relation(S)
 
 
+def test_reference_logged_payload_512_plain(ctx):
+    """payload_pkfk_512 (logged 268435456 / 38280597101084672 for nphj_sc and
+    the asymmetric radix join; the reference's radix_bc log over-counts
+    there).  The radix joins at this width need three copies of S (input and
+    two partition buffers, 3 x 128 GiB), more than one GPU holds, so they run
+    a quarter of the size against the payload closed form."""
+    R, S = ctx.generate("payload_ref", 1 << 24, 1 << 28, 512)
+    _check(ctx.join("nphj_sc", R, S), 268435456, 38280597101084672, "nphj_sc")
+    ctx.free_relation(R)
+    ctx.free_relation(S)
+    nr, ns = 1 << 22, 1 << 26
+    R, S = ctx.generate("payload_ref", nr, ns, 512)
+    want = O.payload_ref_closed_form(nr, ns)
+    for algo in ("phj_radix_bc", "phj_asymm_radix_bc", "nphj_lp"):
+        _check(ctx.join(algo, R, S), want["matched_cnt"], want["check_sum"], algo)
+    ctx.free_relation(R)
+    ctx.free_relation(S)
+
+
 @pytest.mark.parametrize("width", [16, 64, 512])
 def test_reference_logged_payload_late_materialisation(ctx, width):
     """ptr_phj_radix_bc / ptr_nphj_sc on payload_pkfk_16 / _64 / _512 (logged
