@@ -46,8 +46,8 @@ extern "C" {
 typedef struct {
 	void *tuples;         /* packed AoS records {u64 key; u64 row_id; u8 rest[]} */
 	uint64_t tuple_num;
-	uint32_t tuple_size;  /* bytes per tuple: 16, 32, 64, 128 or 256 (and 512
-	                         for the ptr_* joins and the generator)      */
+	uint32_t tuple_size;  /* bytes per tuple: 16, 32, 64, 128, 256 or 512
+	                         (batches and serving: up to 256)            */
 	uint32_t location;    /* HJ_LOC_* */
 } hj_relation_t;
 
