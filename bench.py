@@ -34,6 +34,13 @@ CONFIGS = {
     "C1": {"algo": "nphj_sc", "workload": "pkfk", "nr": 1 << 20, "ns": 1 << 24, "width": 16, "theta": 1.0},
     "C2": {"algo": "phj_radix_bc", "workload": "pkfk", "nr": 1 << 24, "ns": 1 << 28, "width": 16, "theta": 1.0},
     "C3": {"algo": "phj_radix_bc", "workload": "zipf", "nr": 1 << 24, "ns": 1 << 27, "width": 64, "theta": 1.0},
+    # the reference's payload sweep (logs/payload-logs): row ids are positions
+    "PAY16": {"algo": "ptr_phj_radix_bc", "workload": "payload_ref", "nr": 1 << 24, "ns": 1 << 28, "width": 16,
+              "theta": 1.0},
+    "PAY64": {"algo": "ptr_phj_radix_bc", "workload": "payload_ref", "nr": 1 << 24, "ns": 1 << 28, "width": 64,
+              "theta": 1.0},
+    "PAY512": {"algo": "ptr_phj_radix_bc", "workload": "payload_ref", "nr": 1 << 24, "ns": 1 << 28, "width": 512,
+               "theta": 1.0},
 }
 
 
@@ -43,29 +50,36 @@ def log(*a):
 
 def kernel_bytes(name, W, nr, ns, res=None):
     """Algorithmic HBM bytes one launch of `name` must move (no histogram
-    passes, table zeroing or partial lines): a radix scatter reads and writes
-    every tuple once, the radix join reads every tuple once, an NPHJ build
-    reads R and writes a 16-byte entry per tuple, a probe reads S and one
-    16-byte entry per probe.  Two passes over 16-byte tuples store a pass-A
-    tile as 8-byte {key, row_id} tuples when they fit 32 bits
+    passes, table zeroing or partial lines): a radix pass reads and writes
+    every tuple once, the radix join reads every tuple's head once, an NPHJ
+    build reads R and writes a 16-byte entry per tuple, a probe reads S and
+    one 16-byte entry per probe.  A head-only read of a tuple wider than 64
+    bytes counts one 64-byte HBM burst.  Two passes over 16-byte tuples store
+    a pass-A tile as 8-byte {key, row_id} tuples when they fit 32 bits
     (res["narrow_tuples"] tuples), and pass B and the join then move 8-byte
     tuples when every tile did (res["narrow"])."""
     N = nr + ns
-    a_out = b_out = W * N
+    head = min(W, 64)
+    a_out = b_out = W * N  # pass A output, pass B output
+    join_in = head * N
     if res is not None and W == 16 and res.get("passes") == 2:
         nt = res.get("narrow_tuples", 0)
         a_out = 8 * nt + 16 * (N - nt)
-        b_out = (8 if res.get("narrow") else 16) * N
+        b_out = join_in = (8 if res.get("narrow") else 16) * N
     return {
         "radix_scatter": 2 * W * N,
         "radix_sortchunk": 2 * W * N,
         "radix_tilesort": W * N + a_out,
         "radix_gathersort": a_out + b_out,
-        "radix_join": b_out,
-        "lp_build": (W + 16) * nr,
-        "sc_build": (W + 16) * nr,
-        "lp_probe": (W + 16) * ns,
-        "sc_probe": (W + 16) * ns,
+        "radix_join": join_in,
+        "lp_build": (head + 16) * nr,
+        "sc_build": (head + 16) * nr,
+        "lp_probe": (head + 16) * ns,
+        "sc_probe": (head + 16) * ns,
+        # late materialisation: heads read and written once; per pair the pair
+        # read and two 8-byte row-id gathers
+        "ptr_keyrid": 32 * N if W > 16 else None,
+        "ptr_retrieve": 32 * res["matched_cnt"] if res else None,
     }.get(name)
 
 
@@ -90,7 +104,10 @@ def cpu_baseline(cfg, algo):
     base_algo = algo.replace("_bw_reg", "")
     times = []
     for _ in range(3):
-        r = O.join(base_algo, R, S, cfg["width"], threads=threads, radix_bits=11, passes=1)
+        if base_algo.startswith("ptr_"):
+            r = O.ptr_join(base_algo, R, S, cfg["width"], threads=threads, radix_bits=11, passes=1)
+        else:
+            r = O.join(base_algo, R, S, cfg["width"], threads=threads, radix_bits=11, passes=1)
         times.append(r["total_time"])
     t = sum(times[1:]) / len(times[1:])
     import platform
@@ -337,7 +354,7 @@ def main():
         for k, lst in r["kernel_ms"].items():
             tot[k] = tot.get(k, 0.0) + sum(lst)
             cnt[k] = cnt.get(k, 0) + len(lst)
-    dom = max((k for k in tot if kernel_bytes(k, W, nr, ns)), key=lambda k: tot[k])
+    dom = max((k for k in tot if kernel_bytes(k, W, nr, ns, res[0])), key=lambda k: tot[k])
     avg_ms = tot[dom] / cnt[dom]
     achieved = kernel_bytes(dom, W, nr, ns, res[0]) / (avg_ms * 1e-3) / 1e9
     job_s = sum(r["total_time"] for r in res) / len(res)

@@ -35,6 +35,10 @@
 #define PROBE_U 4
 
 // ---------------------------------------------------------------- utilities
+// Algorithmic bytes of reading the 16-byte head of a W-byte tuple: all of a
+// tuple up to one 64-byte HBM burst (W + 16 per tuple for NPHJ up to W = 64;
+// wider tuples are never read past the burst that holds the head).
+static inline uint32_t head_bytes(uint32_t W) { return W < 64 ? W : 64; }
 __global__ void k_fill_u4(uint4 *p, uint64_t n, uint4 v) {
 	const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
 	for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = v;
@@ -321,7 +325,7 @@ int hj_run_nphj_lp(hj_ctx_t *ctx, const hj_relation_t *R, const hj_relation_t *S
 	out->buildpart_time = hj_event_seconds(ctx, 1, 2);
 	out->probejoin_time = hj_event_seconds(ctx, 2, 3);
 	out->total_time = hj_event_seconds(ctx, 0, 3);
-	out->algo_bytes = (uint64_t)(R->tuple_size + 16) * (R->tuple_num + S->tuple_num);
+	out->algo_bytes = (uint64_t)(head_bytes(R->tuple_size) + 16) * (R->tuple_num + S->tuple_num);
 	return HJ_OK;
 }
 
@@ -509,7 +513,7 @@ int hj_run_nphj_sc(hj_ctx_t *ctx, const hj_relation_t *R, const hj_relation_t *S
 		out->buildpart_time = hj_event_seconds(ctx, 1, 2);
 		out->probejoin_time = hj_event_seconds(ctx, 2, 3);
 		out->total_time = hj_event_seconds(ctx, 0, 3);
-		out->algo_bytes = (uint64_t)(R->tuple_size + 16) * (R->tuple_num + S->tuple_num);
+		out->algo_bytes = (uint64_t)(head_bytes(R->tuple_size) + 16) * (R->tuple_num + S->tuple_num);
 		return HJ_OK;
 	}
 	return hj_fail(ctx, HJ_ENOMEM, "nphj_sc overflow pool exhausted after rebuild");

@@ -1699,7 +1699,9 @@ int hj_run_radix(hj_ctx_t *ctx, const hj_relation_t *R, const hj_relation_t *S, 
 	out->buildpart_time = hj_event_seconds(ctx, 0, 1);
 	out->probejoin_time = hj_event_seconds(ctx, 1, 2);
 	out->total_time = hj_event_seconds(ctx, 0, 2);
-	out->algo_bytes = (uint64_t)(2 * P + 1) * W * N;
+	// each pass reads and writes whole tuples; the join reads heads only, one
+	// 64-byte HBM burst of a wider tuple
+	out->algo_bytes = (uint64_t)(2 * P) * W * N + (uint64_t)std::min(W, 64u) * N;
 	if (P == 2 && ctx->narrow_ok) {
 		// pass A reads 16 B and writes 8 or 16 B per tuple; pass B reads
 		// that and writes 8 B when no tile was wide; the join reads pass B's
