@@ -124,6 +124,99 @@ def cpu_baseline(cfg, algo):
                       f"11 radix bits 1 pass, mean of runs 2-3 of 3, {threads} threads on {model or platform.machine()}"}
 
 
+def tpch_pushdown_bytes(q, np_, nl, res):
+    """Algorithmic HBM bytes of the two selection kernels of one query: every
+    column a predicate tests, read for every row; the key and money columns
+    read for the selected rows only; one 16-byte tuple written per output
+    row.  Q14: p_partkey + p_type (33 B/part), l_shipdate (10 B/lineitem),
+    l_partkey + l_extendedprice + l_discount (20 B/selected lineitem).  Q19:
+    p_brand + p_size + p_container (24 B/part), p_partkey (8 B/selected
+    part), l_quantity + l_shipmode + l_shipinstruct (39 B/lineitem), 20
+    B/selected lineitem."""
+    if q == 14:
+        rd = 33 * np_ + 10 * nl + 20 * res["lineitem_selected"]
+    else:
+        rd = 24 * np_ + 8 * res["part_selected"] + 39 * nl + 20 * res["lineitem_selected"]
+    return rd + 16 * (res["part_tuples"] + res["lineitem_tuples"])
+
+
+def tpch_cpu_baseline(q, ctx, T):
+    """The oracle's vectorised restatement of the query (numpy, one core) on
+    the SF1 cardinalities, generated on the device and copied to the host; 3
+    runs, first dropped."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import tpch_oracle as TO
+    np_, nl = T.cardinality(1)
+    t = T.DeviceTables.generate(ctx, np_, nl, seed=12345)
+    part, line = t.download()
+    t.free()
+    times = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        TO.query(q, part, line)
+        times.append(time.perf_counter() - t0)
+    el = sum(times[1:]) / 2
+    return {"value": (np_ + nl) / el, "unit": "rows/s", "cores": 1, "kind": "port",
+            "sample": f"Q{q} restated in numpy (oracle/tpch_oracle.py), SF1 cardinalities {np_} parts x {nl} "
+                      f"lineitems, mean of runs 2-3 of 3, 1 thread"}
+
+
+def run_tpch(args, hjmi):
+    """TPC-H Q14 / Q19 end to end (src/tpch/tpch_query.c): one step = the
+    selections, the joins on partkey and the sums, over columns resident in
+    HBM at scale factor --sf.  `value` is (part + lineitem rows) per second."""
+    import hjmi.tpch as T
+    q = int(args.config[1:])
+    algo = args.algo or "tpch_rdx_bc"
+    ctx = hjmi.Context(0)
+    info = ctx.device_info()
+    np_, nl = T.cardinality(args.sf)
+    t = T.DeviceTables.generate(ctx, np_, nl, seed=12345)
+    log(f"TPC-H SF{args.sf}: {np_} parts, {nl} lineitems, {t.nbytes / 1e9:.1f} GB of columns in HBM")
+    for _ in range(args.warmup):
+        T.query(ctx, q, algo, t)
+    ctx.sync()
+    t0 = time.perf_counter()
+    res = [T.query(ctx, q, algo, t) for _ in range(args.steps)]
+    ctx.sync()
+    elapsed = time.perf_counter() - t0
+    keys = ("matched_cnt", "numerator_e4", "denominator_e4", "revenue_e4")
+    if any(tuple(r[k] for k in keys) != tuple(res[0][k] for k in keys) for r in res):
+        raise SystemExit("results differ between steps")
+    r0 = res[0]
+    push_ms = sum(r["pushdown_time"] for r in res) / len(res) * 1e3
+    pbytes = tpch_pushdown_bytes(q, np_, nl, r0)
+    achieved = pbytes / (push_ms * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_GBPS, "unit": "GB/s",
+                "frac": round(achieved / PEAK_GBPS, 4), "traffic": load_traffic("tpch_select", f"Q{q}_SF{args.sf}"),
+                "kernel": "tpch_select (part + lineitem selections)", "kernel_avg_ms": round(push_ms, 4),
+                "kernel_algo_bytes": pbytes,
+                "phase_ms": {k: round(sum(r[k] for r in res) / len(res) * 1e3, 4)
+                             for k in ("pushdown_time", "buildpart_time", "probejoin_time", "total_time")}}
+    cpu = None
+    if not args.no_cpu_baseline:
+        try:
+            cpu = tpch_cpu_baseline(q, ctx, T)
+        except Exception as e:  # reported, never fatal for the GPU number
+            log(f"cpu baseline failed: {e}")
+    answer = ({"promo_revenue": r0["promo_revenue"], "numerator": r0["numerator"], "denominator": r0["denominator"]}
+              if q == 14 else {"revenue": r0["revenue"]})
+    out = {
+        "metric": f"TPC-H Q{q} rows/s (part + lineitem)", "value": (np_ + nl) * args.steps / elapsed,
+        "unit": "rows/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": elapsed / args.steps * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "u64", "data": "synthetic (TPC-H value domains, seeded on device)",
+        "config": {"workload": f"Q{q} SF{args.sf}: {algo}, {np_} parts x {nl} lineitems",
+                   "matched_cnt": r0["matched_cnt"], **answer, "part_selected": r0["part_selected"],
+                   "lineitem_selected": r0["lineitem_selected"], "joins": r0["joins"],
+                   "device": info["arch"], "cu": info["cu_count"]},
+        "roofline": roofline, "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    t.free()
+    ctx.close()
+
+
 def run_batched(args, hjmi):
     """C4: njoins independent PK-FK joins, |R_i| log-uniform in [2^lo, 2^hi],
     |S_i| = 16 |R_i|, 16-byte tuples, seeded per join.  Times one batched
@@ -281,7 +374,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--config", default="C2", choices=sorted(CONFIGS) + ["C4", "C5"])
+    ap.add_argument("--config", default="C2", choices=sorted(CONFIGS) + ["C4", "C5", "Q14", "Q19"])
+    ap.add_argument("--sf", type=int, default=100, choices=[1, 10, 100], help="Q14/Q19: TPC-H scale factor")
     ap.add_argument("--rates", default="1000,2000,5000,10000,20000", help="C5: offered joins/s")
     ap.add_argument("--requests", type=int, default=1000, help="C5: requests per rate")
     ap.add_argument("--pool", type=int, default=64, help="C5: distinct non-resident joins")
@@ -324,6 +418,9 @@ def main():
         return
     if args.config == "C5":
         run_served(args, hjmi)
+        return
+    if args.config in ("Q14", "Q19"):
+        run_tpch(args, hjmi)
         return
 
     cfg = dict(CONFIGS[args.config])

@@ -68,6 +68,7 @@ struct hj_ctx {
 	uint8_t *tile_narrow = nullptr;  // per pass-A tile: stored as 8-byte tuples
 	hj_buf pairs;      // the pair buffer of late-materialising joins
 	hj_buf keyrid[2];  // {key, row_id} copies of wide R and S
+	hj_buf tpch[5];    // TPC-H: staged part / lineitem tuples, selection counts, contiguous outputs
 
 	// size-class autotuner: radix bits / passes per log2 |R| (0xff = unset)
 	uint8_t tune_bits[40];

@@ -1669,6 +1669,14 @@ int hj_run_radix(hj_ctx_t *ctx, const hj_relation_t *R, const hj_relation_t *S, 
 	while (cap > 256u && hj_tab_lds(tk, cap, wv) > lds_max) cap -= 256;
 	const size_t jlds = hj_tab_lds(tk, cap, wv);
 	if (jlds > lds_max) return hj_fail(ctx, HJ_EINVAL, "join table of %u tuples does not fit LDS", cap);
+	const size_t jall = jlds + join_static_lds(tk, P == 2);
+	const int per_cu = (int)std::max<size_t>(1, std::min<size_t>(4, (size_t)(160 * 1024) / jall));
+	const unsigned jgrid = (unsigned)std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)ctx->num_cu * per_cu, (uint64_t)Q * 64));
+	// few partitions against a large probe side (a small build relation):
+	// tiles of twice the average partition would leave most of the grid
+	// idle, so cut the probe side into about two items per workgroup
+	if (P <= 1 && (uint64_t)Q * ((avg_s + s_tile - 1) / s_tile) < 2ull * jgrid)
+		s_tile = std::max<uint64_t>(8192, ns / (2ull * jgrid) + 1);
 	hj_kt_begin(ctx, HJ_K_PLAN);
 	uint32_t cpt = 0;
 	if (P == 2) {
@@ -1682,9 +1690,6 @@ int hj_run_radix(hj_ctx_t *ctx, const hj_relation_t *R, const hj_relation_t *S, 
 	rc = scan_u64(ctx, tiles, Q, (uint64_t *)ctx->scan_tmp.p);
 	hj_kt_end(ctx);
 	if (rc) return rc;
-	const size_t jall = jlds + join_static_lds(tk, P == 2);
-	const int per_cu = (int)std::max<size_t>(1, std::min<size_t>(4, (size_t)(160 * 1024) / jall));
-	const unsigned jgrid = (unsigned)std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)ctx->num_cu * per_cu, (uint64_t)Q * 64));
 	hj_kt_begin(ctx, HJ_K_JOIN);
 	if (P == 2) {
 		launch_join2(ctx, wv, tk, jgrid, jlds, Rb, cst, run_t, (uint32_t)chunks2_max, b1, pairs, tiles, Q, B, cpt, cap);

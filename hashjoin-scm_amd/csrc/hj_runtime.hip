@@ -136,7 +136,8 @@ int hj_ctx_destroy(hj_ctx_t *ctx) {
 	if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
 	hj_buf *bufs[] = {&ctx->table, &ctx->pool, &ctx->aux, &ctx->part[0], &ctx->part[1], &ctx->hist,
 	                  &ctx->meta, &ctx->scan_tmp, &ctx->stats, &ctx->hostcopy[0], &ctx->hostcopy[1],
-	                  &ctx->pairs, &ctx->keyrid[0], &ctx->keyrid[1]};
+	                  &ctx->pairs, &ctx->keyrid[0], &ctx->keyrid[1], &ctx->tpch[0], &ctx->tpch[1], &ctx->tpch[2],
+	                  &ctx->tpch[3], &ctx->tpch[4]};
 	for (hj_buf *b : bufs)
 		if (b->p) (void)hipFree(b->p);
 	for (int i = 0; i < hj_ctx::NEV; i++)

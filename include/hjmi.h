@@ -290,6 +290,78 @@ int hj_data_cfg_read(const char *cfg_path, hj_data_cfg_t *out);
 int hj_bin_read(hj_ctx_t *ctx, const char *path, uint64_t tuple_num, uint32_t tuple_size, void *dev);
 int hj_bin_write(hj_ctx_t *ctx, const char *path, const void *dev, uint64_t tuple_num, uint32_t tuple_size);
 
+/* ---- TPC-H Q14 and Q19 on the join engine (src/tpch/tpch_query.c) --------
+ * The columns the two queries read, one array per column.  Strings are
+ * fixed-width, NUL-padded char arrays of the reference's widths
+ * (src/tpch/inc/schema.h:37-67): row i's p_type is type[25 * i .. 25 * i + 24].
+ * Money is exact: l_extendedprice in cents, l_discount in hundredths,
+ * l_quantity in whole units.  Device pointers for hj_tpch_generate and
+ * hj_tpch_query, host pointers for the file readers. */
+typedef struct {
+	uint64_t rows;
+	uint64_t *partkey;
+	char *brand;       /* 10 bytes a row, "Brand#MN"                    */
+	char *type;        /* 25 bytes                                      */
+	int32_t *size;
+	char *container;   /* 10 bytes                                      */
+} hj_tpch_part_t;
+typedef struct {
+	uint64_t rows;
+	uint64_t *partkey;
+	int32_t *quantity;
+	int64_t *extendedprice;  /* cents                                    */
+	int32_t *discount;       /* hundredths (0..100)                      */
+	char *shipdate;          /* 10 bytes, "YYYY-MM-DD"                   */
+	char *shipinstruct;      /* 25 bytes                                 */
+	char *shipmode;          /* 10 bytes                                 */
+} hj_tpch_lineitem_t;
+/* What tpch_spj prints (src/tpch/tpch_query.c:739-745) plus its phase times.
+ * The sums of l_extendedprice * (1 - l_discount) are exact integers in units
+ * of 1e-4 (the reference adds doubles); the doubles are those sums / 1e4. */
+typedef struct {
+	uint64_t matched_cnt;      /* Q14: join pairs; Q19: pairs passing match_filter */
+	uint64_t numerator_e4;     /* Q14: PROMO pairs                              */
+	uint64_t denominator_e4;   /* Q14: all pairs                                */
+	uint64_t revenue_e4;       /* Q19                                           */
+	double promo_revenue;      /* Q14: 100 * numerator / denominator             */
+	double numerator, denominator, revenue;
+	uint64_t part_selected;    /* rows passing part_filter (tpch_utils.c:44-87)  */
+	uint64_t lineitem_selected;/* rows passing lineitem_filter (:89-112)         */
+	uint64_t part_tuples;      /* {partkey, 0} tuples the part selection wrote
+	                              (Q14: all parts + PROMO parts)                 */
+	uint64_t lineitem_tuples;  /* {partkey, revenue} tuples the lineitem
+	                              selection wrote (Q19: one per group it is in) */
+	uint32_t joins;            /* hj_join calls the query made                   */
+	uint32_t reserved;
+	double pushdown_time, buildpart_time, probejoin_time, total_time;  /* seconds */
+} hj_tpch_result_t;
+/* The reference's TPC-H algorithm names (tpch_query.c:93-99): tpch_nphj_sc,
+ * tpch_rdx_bc, tpch_rdx_hr, tpch_rdx_bw_reg_bc, tpch_rdx_bw_reg_hr. */
+int hj_tpch_algo_count(void);
+const char *hj_tpch_algo_name(int idx);
+/* part / lineitem rows of scale factor 1, 10, 100 (get_cardinality,
+ * src/tpch/inc/tpch_utils.c:20-40) */
+int hj_tpch_cardinality(int sf, uint64_t *part_rows, uint64_t *lineitem_rows);
+/* Seeded TPC-H-like columns in device memory: the value domains of the TPC-H
+ * specification (p_partkey = 1..rows, l_partkey uniform over them), drawn by
+ * this library's counter-based generator -- not dbgen's streams. */
+int hj_tpch_generate(hj_ctx_t *ctx, uint64_t seed, const hj_tpch_part_t *part, const hj_tpch_lineitem_t *lineitem);
+/* Q14 or Q19 over device columns: the selections of tpch_utils.c:44-160, the
+ * join on partkey by `algo`, the sums of tpch_nphj.c:302-319. */
+int hj_tpch_query(hj_ctx_t *ctx, int query, const char *algo, const hj_tpch_part_t *part,
+                  const hj_tpch_lineitem_t *lineitem, hj_tpch_result_t *out);
+/* dbgen .tbl files ('|'-separated rows) into host columns of exactly `rows`
+ * rows; hj_tpch_tbl_rows counts the rows of a file. */
+int hj_tpch_tbl_rows(const char *path, uint64_t *rows);
+int hj_tpch_tbl_read_part(const char *path, const hj_tpch_part_t *host);
+int hj_tpch_tbl_read_lineitem(const char *path, const hj_tpch_lineitem_t *host);
+/* tbl2bin (src/tpch/tpch_tbl2bin.c:23-150): table "part" or "lineitem" from a
+ * .tbl file into the reference's packed part.bin / lineitem.bin rows (168 and
+ * 175 bytes), and those files back into host columns. */
+int hj_tpch_tbl2bin(const char *table, const char *tbl_path, const char *bin_path, uint64_t *rows);
+int hj_tpch_bin_read_part(const char *path, const hj_tpch_part_t *host);
+int hj_tpch_bin_read_lineitem(const char *path, const hj_tpch_lineitem_t *host);
+
 /* ---- device properties used by the autotuner and the benchmark ---------- */
 typedef struct {
 	int cu_count;

@@ -43,7 +43,10 @@ def test_library_exports_every_declared_entry_point():
 
 def test_python_binding_covers_declared_entry_points():
     import hjmi
+    import hjmi.tpch
     assert set(hjmi.exported_symbols()) <= set(declared_functions())
+    assert set(hjmi.tpch.exported_symbols()) <= set(declared_functions())
+    assert "hj_tpch_query" in declared_functions()
 
 
 def test_registry_keeps_reference_names():
