@@ -26,6 +26,17 @@
  * Workload "baseline" with subtype C1, C2, C3 selects the benchmark
  * configurations: C1 2^20 x 2^24 uniform PK-FK, C2 2^24 x 2^28, C3 64-byte
  * Zipf 2^24 x 2^27 (theta from --theta, default 1.0).
+ *
+ * TPC-H, the drop-in for the reference's bin/tpch_query
+ * (src/tpch/tpch_query.c:1129-1190, which fixes the query at build time):
+ *   hjmain --tpch=14|19 --algo=tpch_rdx_bc --sf=1|10|100
+ *          [--tbl-dir=DIR | --tpch-bin-dir=DIR] [--seed=S] [--trials=T]
+ * reads DIR/part.tbl and DIR/lineitem.tbl (dbgen) or DIR/part.bin and
+ * DIR/lineitem.bin (the reference's tbl2bin output), or generates the
+ * scale factor's cardinalities on the GPU, and prints the reference's
+ * result line (tpch_query.c:739-745) and phase times per trial.
+ * --tbl2bin=DIR converts DIR/part.tbl and DIR/lineitem.tbl into the
+ * reference's .bin files next to them, as bin/tpch_tbl2bin does.
  */
 #include <getopt.h>
 #include <stdio.h>
@@ -75,6 +86,109 @@ static int die(hj_ctx_t *ctx, int rc, const char *what) {
 	return 1;
 }
 
+/* ---- TPC-H ---------------------------------------------------------------- */
+#define TPCH_PART_COLS 5
+#define TPCH_LINE_COLS 7
+static const uint64_t k_part_w[TPCH_PART_COLS] = {8, 10, 25, 4, 10};
+static const uint64_t k_line_w[TPCH_LINE_COLS] = {8, 4, 8, 4, 10, 25, 10};
+
+static void **part_cols(hj_tpch_part_t *p, int i) {
+	void **c[TPCH_PART_COLS] = {(void **)&p->partkey, (void **)&p->brand, (void **)&p->type, (void **)&p->size,
+	                            (void **)&p->container};
+	return c[i];
+}
+static void **line_cols(hj_tpch_lineitem_t *l, int i) {
+	void **c[TPCH_LINE_COLS] = {(void **)&l->partkey,  (void **)&l->quantity,     (void **)&l->extendedprice,
+	                            (void **)&l->discount, (void **)&l->shipdate,     (void **)&l->shipinstruct,
+	                            (void **)&l->shipmode};
+	return c[i];
+}
+
+static int run_tpch(int query, const char *algo, int sf, const char *tbl_dir, const char *bin_dir, uint64_t seed,
+                    int trials, int device) {
+	int found = 0;
+	for (int i = 0; i < hj_tpch_algo_count(); i++)
+		if (!strcmp(hj_tpch_algo_name(i), algo)) found = 1;
+	if (!found) {
+		printf("[ERROR] Join algorithm named `%s' does not exist!\n", algo);
+		return 1;
+	}
+	hj_tpch_part_t hp, dp;
+	hj_tpch_lineitem_t hl, dl;
+	memset(&hp, 0, sizeof hp), memset(&dp, 0, sizeof dp), memset(&hl, 0, sizeof hl), memset(&dl, 0, sizeof dl);
+	char pp[1024], lp[1024];
+	int rc;
+	if (tbl_dir || bin_dir) {
+		const char *dir = tbl_dir ? tbl_dir : bin_dir;
+		snprintf(pp, sizeof pp, "%s/part.%s", dir, tbl_dir ? "tbl" : "bin");
+		snprintf(lp, sizeof lp, "%s/lineitem.%s", dir, tbl_dir ? "tbl" : "bin");
+		if (tbl_dir) {
+			if (hj_tpch_tbl_rows(pp, &hp.rows) || hj_tpch_tbl_rows(lp, &hl.rows)) {
+				fprintf(stderr, "[ERROR] cannot read %s or %s\n", pp, lp);
+				return 1;
+			}
+		} else {
+			FILE *f = fopen(pp, "rb"), *g = fopen(lp, "rb");
+			if (!f || !g || fseek(f, 0, SEEK_END) || fseek(g, 0, SEEK_END)) {
+				fprintf(stderr, "[ERROR] cannot read %s or %s\n", pp, lp);
+				return 1;
+			}
+			hp.rows = (uint64_t)ftell(f) / 168, hl.rows = (uint64_t)ftell(g) / 175;
+			fclose(f), fclose(g);
+		}
+	} else if ((rc = hj_tpch_cardinality(sf, &hp.rows, &hl.rows))) {
+		fprintf(stderr, "[ERROR] unsupported scaling factor %d\n", sf);
+		return 1;
+	}
+	dp.rows = hp.rows, dl.rows = hl.rows;
+	hj_ctx_t *ctx = NULL;
+	if ((rc = hj_ctx_create(device, &ctx))) return die(NULL, rc, "hj_ctx_create");
+	for (int i = 0; i < TPCH_PART_COLS; i++)
+		if ((rc = hj_dev_alloc(ctx, dp.rows * k_part_w[i] + 16, part_cols(&dp, i)))) return die(ctx, rc, "alloc part");
+	for (int i = 0; i < TPCH_LINE_COLS; i++)
+		if ((rc = hj_dev_alloc(ctx, dl.rows * k_line_w[i] + 16, line_cols(&dl, i)))) return die(ctx, rc, "alloc lineitem");
+	if (tbl_dir || bin_dir) {
+		for (int i = 0; i < TPCH_PART_COLS; i++)
+			if (!(*part_cols(&hp, i) = malloc(hp.rows * k_part_w[i] + 16))) return die(ctx, HJ_ENOMEM, "host part");
+		for (int i = 0; i < TPCH_LINE_COLS; i++)
+			if (!(*line_cols(&hl, i) = malloc(hl.rows * k_line_w[i] + 16))) return die(ctx, HJ_ENOMEM, "host lineitem");
+		rc = tbl_dir ? hj_tpch_tbl_read_part(pp, &hp) : hj_tpch_bin_read_part(pp, &hp);
+		if (rc) return die(ctx, rc, pp);
+		rc = tbl_dir ? hj_tpch_tbl_read_lineitem(lp, &hl) : hj_tpch_bin_read_lineitem(lp, &hl);
+		if (rc) return die(ctx, rc, lp);
+		for (int i = 0; i < TPCH_PART_COLS; i++)
+			if ((rc = hj_copy_h2d(ctx, *part_cols(&dp, i), *part_cols(&hp, i), hp.rows * k_part_w[i])))
+				return die(ctx, rc, "copy part");
+		for (int i = 0; i < TPCH_LINE_COLS; i++)
+			if ((rc = hj_copy_h2d(ctx, *line_cols(&dl, i), *line_cols(&hl, i), hl.rows * k_line_w[i])))
+				return die(ctx, rc, "copy lineitem");
+		for (int i = 0; i < TPCH_PART_COLS; i++) free(*part_cols(&hp, i));
+		for (int i = 0; i < TPCH_LINE_COLS; i++) free(*line_cols(&hl, i));
+	} else if ((rc = hj_tpch_generate(ctx, seed, &dp, &dl))) {
+		return die(ctx, rc, "hj_tpch_generate");
+	}
+	printf("CONFIGURATION:\tQUERY: %d\tSF: %d\tpart_num: %llu\tlineitem_num: %llu\n", query, sf,
+	       (unsigned long long)dp.rows, (unsigned long long)dl.rows);
+	for (int t = 0; t < trials; t++) {
+		hj_tpch_result_t res;
+		if ((rc = hj_tpch_query(ctx, query, algo, &dp, &dl, &res))) return die(ctx, rc, "hj_tpch_query");
+		printf("matched_cnt: %llu\t", (unsigned long long)res.matched_cnt);
+		if (query == 14)
+			printf("promo_revenue: %.2f\tnumerator: %.2f\tdenominator: %.2f\n", res.promo_revenue, res.numerator,
+			       res.denominator);
+		else
+			printf("revenue: %.2f\n", res.revenue);
+		printf("algo: %s\tpart_selected: %llu\tlineitem_selected: %llu\tpushdown_time: %.9f\tbuildpart_time: "
+		       "%.9f\tprobejoin_time: %.9f\ttotal_time: %.9f\n",
+		       algo, (unsigned long long)res.part_selected, (unsigned long long)res.lineitem_selected,
+		       res.pushdown_time, res.buildpart_time, res.probejoin_time, res.total_time);
+	}
+	for (int i = 0; i < TPCH_PART_COLS; i++) hj_dev_free(ctx, *part_cols(&dp, i));
+	for (int i = 0; i < TPCH_LINE_COLS; i++) hj_dev_free(ctx, *line_cols(&dl, i));
+	hj_ctx_destroy(ctx);
+	return 0;
+}
+
 /* host tuples of a reference workload, copied into device memory */
 static int ref_upload(hj_ctx_t *ctx, const char *name, uint32_t width, int exact, void *r_dev, void *s_dev,
                       uint64_t *nr, uint64_t *ns) {
@@ -103,7 +217,8 @@ static int ref_upload(hj_ctx_t *ctx, const char *name, uint32_t width, int exact
 int main(int argc, char **argv) {
 	const char *algo = NULL, *workload = "baseline", *subtype = "C2", *param = "pkfk", *gen = NULL;
 	const char *cfg = NULL, *data_prefix = ".", *write_bin = NULL;
-	int exact = 0;
+	int exact = 0, tpch = 0, sf = 1;
+	const char *tbl_dir = NULL, *tpch_bin_dir = NULL, *tbl2bin_dir = NULL;
 	uint64_t r_num = 0, s_num = 0, key_range = 0, seed = 12345;
 	uint32_t width = 0, bits = 0, passes = 0, bits_pass1 = 0;
 	int trials = 6, device = 0, list = 0;
@@ -119,7 +234,9 @@ int main(int argc, char **argv) {
 	    {"device", required_argument, 0, 'd'},  {"list", no_argument, 0, 'l'},
 	    {"cfg", required_argument, 0, 'c'},     {"data-prefix", required_argument, 0, 'D'},
 	    {"write-bin", required_argument, 0, 'o'}, {"exact", no_argument, 0, 'x'},
-	    {"bits-pass1", required_argument, 0, '1'},
+	    {"bits-pass1", required_argument, 0, '1'}, {"tpch", required_argument, 0, 'Q'},
+	    {"sf", required_argument, 0, 'F'},      {"tbl-dir", required_argument, 0, 'L'},
+	    {"tpch-bin-dir", required_argument, 0, 'B'}, {"tbl2bin", required_argument, 0, 'C'},
 	    {0, 0, 0, 0}};
 	int c;
 	while ((c = getopt_long(argc, argv, "", opts, NULL)) != -1) {
@@ -145,8 +262,36 @@ int main(int argc, char **argv) {
 		case 'o': write_bin = optarg; break;
 		case 'x': exact = 1; break;
 		case '1': bits_pass1 = (uint32_t)strtoul(optarg, NULL, 0); break;
+		case 'Q': tpch = atoi(optarg); break;
+		case 'F': sf = atoi(optarg); break;
+		case 'L': tbl_dir = optarg; break;
+		case 'B': tpch_bin_dir = optarg; break;
+		case 'C': tbl2bin_dir = optarg; break;
 		default: fprintf(stderr, "unknown option\n"); return 2;
 		}
+	}
+	if (tbl2bin_dir) {
+		const char *tables[2] = {"part", "lineitem"};
+		for (int i = 0; i < 2; i++) {
+			char in[1024], out[1024];
+			uint64_t rows = 0;
+			snprintf(in, sizeof in, "%s/%s.tbl", tbl2bin_dir, tables[i]);
+			snprintf(out, sizeof out, "%s/%s.bin", tbl2bin_dir, tables[i]);
+			if (hj_tpch_tbl2bin(tables[i], in, out, &rows)) {
+				fprintf(stderr, "[ERROR] tbl2bin %s\n", in);
+				return 1;
+			}
+			printf("%s_num: %llu\n", tables[i], (unsigned long long)rows);
+		}
+		return 0;
+	}
+	if (tpch) {
+		if (tpch != 14 && tpch != 19) {
+			fprintf(stderr, "[ERROR] --tpch must be 14 or 19\n");
+			return 2;
+		}
+		return run_tpch(tpch, algo ? algo : "tpch_nphj_sc", sf, tbl_dir, tpch_bin_dir, seed, trials == 6 ? 1 : trials,
+		                device);
 	}
 	if (list) {
 		for (int i = 0; i < hj_algo_count(); i++) printf("%s\n", hj_algo_name(i));

@@ -132,3 +132,25 @@ def test_empty_tables_and_errors(ctx):
         T.query(ctx, 6, "tpch_rdx_bc", t)
     assert T.query(ctx, 14, "tpch_rdx_bc", t)["matched_cnt"] == 8  # still usable
     t.free()
+
+
+def test_cli_prints_the_reference_lines(tmp_path):
+    """hjmain --tpch=Q (the drop-in for bin/tpch_query) on the fixture, from
+    .tbl files and from the reference's .bin files."""
+    import re
+    import shutil
+    import subprocess
+    cli = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "hashjoin-scm_amd", "bin",
+                       "hjmain")
+    for t in ("part", "lineitem"):
+        shutil.copy(os.path.join(MINI, f"{t}.tbl"), tmp_path / f"{t}.tbl")
+    subprocess.run([cli, f"--tbl2bin={tmp_path}"], check=True, capture_output=True)
+    for src in (f"--tbl-dir={tmp_path}", f"--tpch-bin-dir={tmp_path}"):
+        out = subprocess.run([cli, "--tpch=14", "--algo=tpch_rdx_hr", src], capture_output=True, text=True,
+                             check=True).stdout
+        assert re.search(r"matched_cnt: 8\tpromo_revenue: 65\.66\tnumerator: 81435\.97\tdenominator: 124021\.57", out), out
+        out = subprocess.run([cli, "--tpch=19", "--algo=tpch_nphj_sc", src, "--trials=2"], capture_output=True,
+                             text=True, check=True).stdout
+        assert out.count("matched_cnt: 5\trevenue: 64534.85") == 2, out
+    out = subprocess.run([cli, "--tpch=14", "--sf=1"], capture_output=True, text=True, check=True).stdout
+    assert "lineitem_num: 6001215" in out and "promo_revenue:" in out

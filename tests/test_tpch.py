@@ -170,3 +170,26 @@ def test_reference_algo_names_and_cardinalities():
     assert T.cardinality(100) == (20000000, 600037902)
     with pytest.raises(hjmi.HJError):
         T.cardinality(3)
+
+
+CLI = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "hashjoin-scm_amd", "bin", "hjmain")
+
+
+def test_cli_tbl2bin_matches_library(tmp_path):
+    """hjmain --tbl2bin=DIR converts DIR/part.tbl and DIR/lineitem.tbl as
+    bin/tpch_tbl2bin does (host only, no GPU)."""
+    import shutil
+    import subprocess
+    for t in ("part", "lineitem"):
+        shutil.copy(os.path.join(MINI, f"{t}.tbl"), tmp_path / f"{t}.tbl")
+    out = subprocess.run([CLI, f"--tbl2bin={tmp_path}"], capture_output=True, text=True, check=True).stdout
+    assert "part_num: 6" in out and "lineitem_num: 12" in out
+    for t in ("part", "lineitem"):
+        T.tbl2bin(t, os.path.join(MINI, f"{t}.tbl"), str(tmp_path / f"{t}.lib.bin"))
+        assert (tmp_path / f"{t}.bin").read_bytes() == (tmp_path / f"{t}.lib.bin").read_bytes()
+
+
+def test_cli_tpch_rejects_unknown_algo_like_reference():
+    import subprocess
+    p = subprocess.run([CLI, "--tpch=14", "--algo=tpch_rdx_lp", "--sf=1"], capture_output=True, text=True)
+    assert p.returncode == 1 and "Join algorithm named `tpch_rdx_lp' does not exist!" in p.stdout
