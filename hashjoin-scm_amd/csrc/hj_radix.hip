@@ -66,11 +66,15 @@ __global__ __launch_bounds__(SCAN_NT) void k_scan_reduce(const uint64_t *__restr
                                                          uint64_t *__restrict__ part) {
 	__shared__ uint64_t red[SCAN_NT / HJ_WAVE + 1];
 	const uint64_t base = (uint64_t)blockIdx.x * SCAN_CH;
+	// unconditional loads (clamped, masked after): a load under a per-lane
+	// condition waits for its data before the next one issues.  a holds L + 1
+	// entries, so the clamp stays inside it when L is 0.
+	uint64_t v[SCAN_ITEMS];
+#pragma unroll
+	for (int k = 0; k < SCAN_ITEMS; k++) v[k] = a[min(base + (uint64_t)k * SCAN_NT + threadIdx.x, L ? L - 1 : 0)];
 	uint64_t s = 0;
-	for (int k = 0; k < SCAN_ITEMS; k++) {
-		uint64_t i = base + (uint64_t)k * SCAN_NT + threadIdx.x;
-		if (i < L) s += a[i];
-	}
+#pragma unroll
+	for (int k = 0; k < SCAN_ITEMS; k++) s += base + (uint64_t)k * SCAN_NT + threadIdx.x < L ? v[k] : 0;
 	uint64_t tot;
 	hj_block_exscan<uint64_t>(s, red, &tot);
 	if (threadIdx.x == 0) part[blockIdx.x] = tot;
@@ -96,23 +100,45 @@ __global__ __launch_bounds__(1024) void k_scan_top(uint64_t *__restrict__ part, 
 	if (threadIdx.x == 0) *total_out = tot;
 }
 
+// The block's SCAN_CH elements move between HBM and LDS coalesced (element
+// k * SCAN_NT + t by thread t) and each thread scans its SCAN_ITEMS
+// consecutive elements in LDS.
 __global__ __launch_bounds__(SCAN_NT) void k_scan_down(uint64_t *__restrict__ a, uint64_t L,
                                                        const uint64_t *__restrict__ part) {
 	__shared__ uint64_t red[SCAN_NT / HJ_WAVE + 1];
-	const uint64_t base = (uint64_t)blockIdx.x * SCAN_CH + (uint64_t)threadIdx.x * SCAN_ITEMS;
+	// row pitch SCAN_ITEMS + 1 words: a thread's consecutive elements sit in
+	// consecutive banks and the threads of a wave start in different banks
+	__shared__ uint64_t buf[SCAN_NT * (SCAN_ITEMS + 1)];
+	const uint64_t base = (uint64_t)blockIdx.x * SCAN_CH;
+	auto at = [](uint32_t e) { return (e / SCAN_ITEMS) * (SCAN_ITEMS + 1) + e % SCAN_ITEMS; };
+	uint64_t g[SCAN_ITEMS];
+#pragma unroll
+	for (int k = 0; k < SCAN_ITEMS; k++) g[k] = a[min(base + (uint64_t)k * SCAN_NT + threadIdx.x, L ? L - 1 : 0)];
+#pragma unroll
+	for (int k = 0; k < SCAN_ITEMS; k++) {
+		const uint32_t e = (uint32_t)k * SCAN_NT + threadIdx.x;
+		buf[at(e)] = base + e < L ? g[k] : 0;
+	}
+	__syncthreads();
 	uint64_t v[SCAN_ITEMS];
 	uint64_t s = 0;
+#pragma unroll
 	for (int k = 0; k < SCAN_ITEMS; k++) {
-		uint64_t i = base + k;
-		v[k] = i < L ? a[i] : 0;
+		v[k] = buf[threadIdx.x * (SCAN_ITEMS + 1) + k];
 		s += v[k];
 	}
 	uint64_t tot;
 	uint64_t run = hj_block_exscan<uint64_t>(s, red, &tot) + part[blockIdx.x];
+#pragma unroll
 	for (int k = 0; k < SCAN_ITEMS; k++) {
-		uint64_t i = base + k;
-		if (i < L) a[i] = run;
+		buf[threadIdx.x * (SCAN_ITEMS + 1) + k] = run;
 		run += v[k];
+	}
+	__syncthreads();
+#pragma unroll
+	for (int k = 0; k < SCAN_ITEMS; k++) {
+		const uint32_t e = (uint32_t)k * SCAN_NT + threadIdx.x;
+		if (base + e < L) a[base + e] = buf[at(e)];
 	}
 }
 
@@ -457,39 +483,66 @@ __global__ __launch_bounds__(NT) void k_join(const uint4 *__restrict__ Rb, const
 			const uint32_t m = (uint32_t)min((uint64_t)cap, pr.r1 - rc);
 			tab.reset(m, NT);
 			__syncthreads();
+			// build tuples four per thread and step, loads issued together
+			// (unconditional, clamped to the last tuple: a load under a per-lane
+			// condition waits for its data before the next one issues)
+			auto build = [&](auto &&op) {
+				for (uint32_t i0 = threadIdx.x; i0 < m; i0 += 4 * NT) {
+					uint4 t[4];
+#pragma unroll
+					for (int u = 0; u < 4; u++) t[u] = hj_load(&Rb[(rc + min(i0 + u * NT, m - 1)) * WV]);
+#pragma unroll
+					for (int u = 0; u < 4; u++)
+						if (i0 + u * NT < m) op(i0 + u * NT, t[u]);
+				}
+			};
 			if (TWO) {
-				for (uint32_t i = threadIdx.x; i < m; i += NT) tab.count(hj_key_of(hj_load(&Rb[(rc + i) * WV])), B);
+				build([&](uint32_t, const uint4 &t) { tab.count(hj_key_of(t), B); });
 				__syncthreads();
 				tab.starts(sred, NT);
 				__syncthreads();
-				for (uint32_t i = threadIdx.x; i < m; i += NT) {
-					const uint4 t = hj_load(&Rb[(rc + i) * WV]);
-					tab.place(hj_key_of(t), hj_pay_of(t), B);
-				}
+				build([&](uint32_t, const uint4 &t) { tab.place(hj_key_of(t), hj_pay_of(t), B); });
 			} else {
-				for (uint32_t i = threadIdx.x; i < m; i += NT) {
-					const uint4 t = hj_load(&Rb[(rc + i) * WV]);
-					tab.put(i, hj_key_of(t), hj_pay_of(t), B);
-				}
+				build([&](uint32_t i, const uint4 &t) { tab.put(i, hj_key_of(t), hj_pay_of(t), B); });
 			}
 			__syncthreads();
 			if (LP) {
 				tab.lp_insert(m, B, NT, dup);
 				__syncthreads();
 			}
+			// probe tuples U per thread and step; the next step's loads are
+			// issued before the current step is probed
 			constexpr int U = 4;
-			for (uint64_t j0 = sb + threadIdx.x; j0 < se; j0 += (uint64_t)U * NT) {
+			auto load = [&](uint4 (&t)[U], uint64_t j0) {
+#pragma unroll
+				for (int u = 0; u < U; u++) t[u] = hj_load_stream(&Sb[min(j0 + (uint64_t)u * NT, se - 1) * WV]);
+			};
+			auto probe = [&](const uint4 (&t)[U], uint64_t j0) {
 				uint64_t sk[U], sp[U];
 				bool live[U];
 #pragma unroll
 				for (int u = 0; u < U; u++) {
-					const uint64_t j = j0 + (uint64_t)u * NT;
-					live[u] = j < se;
-					const uint4 t = live[u] ? hj_load_stream(&Sb[j * WV]) : make_uint4(0, 0, 0, 0);
-					sk[u] = hj_key_of(t);
-					sp[u] = hj_pay_of(t);
+					live[u] = j0 + (uint64_t)u * NT < se;
+					sk[u] = hj_key_of(t[u]);
+					sp[u] = hj_pay_of(t[u]);
 				}
 				tab.template probe<U>(sk, sp, live, B, acc);
+			};
+			uint64_t j0 = sb + threadIdx.x;
+			if (j0 < se) {
+				uint4 ta[U], tb[U];
+				load(ta, j0);
+				for (;;) {
+					const uint64_t j1 = j0 + (uint64_t)U * NT;
+					if (j1 < se) load(tb, j1);
+					probe(ta, j0);
+					if (j1 >= se) break;
+					const uint64_t j2 = j1 + (uint64_t)U * NT;
+					if (j2 < se) load(ta, j2);
+					probe(tb, j1);
+					if (j2 >= se) break;
+					j0 = j2;
+				}
 			}
 			__syncthreads();
 		}
@@ -784,11 +837,16 @@ __global__ __launch_bounds__(NT) void k_sortchunk_g(
 			if (i + NT >= g1.nchunks || row[i + NT] >= pos + n) break;
 		}
 	};
-	auto load = [&](uint4 (&v)[ITEMS][WV], uint32_t c, const uint32_t *srcb) {
+	// Load tile c into v.  For 16-byte tuples, sel gets two bits per item
+	// (pass-A tile stored narrow, upper half of the loaded pair) and v the
+	// raw 16-byte load; unpack() picks the tuple out when the tile is sorted,
+	// since picking it here would wait for every load before the next issues.
+	auto load = [&](uint4 (&v)[ITEMS][WV], uint32_t &sel, uint32_t c, const uint32_t *srcb) {
 		uint64_t pos;
 		uint32_t n;
 		tile_of(c, pos, n);
 		const uint4 *base = base_of(c);
+		sel = 0;
 #pragma unroll
 		for (int k = 0; k < ITEMS; k++) {
 			const uint32_t f = min((uint32_t)(k * NT) + threadIdx.x, n ? n - 1 : 0u);
@@ -802,8 +860,8 @@ __global__ __launch_bounds__(NT) void k_sortchunk_g(
 				const bool nw = narrow_ok && (e >> 31);
 				const uint64_t sp = narrow_ok ? (e & 0x7fffffffu) : e;
 				const uint64_t u8 = sp + (sp & ~(uint64_t)(TILE - 1));
-				const uint4 h = hj_load(base + (nw ? u8 >> 1 : sp));
-				v[k][0] = !nw ? h : ((u8 & 1) ? make_uint4(h.z, 0u, h.w, 0u) : make_uint4(h.x, 0u, h.y, 0u));
+				v[k][0] = hj_load(base + (nw ? u8 >> 1 : sp));
+				sel |= ((nw ? 1u : 0u) | (nw && (u8 & 1) ? 2u : 0u)) << (2 * k);
 			} else {
 				const uint64_t sp = e;
 #pragma unroll
@@ -811,10 +869,20 @@ __global__ __launch_bounds__(NT) void k_sortchunk_g(
 			}
 		}
 	};
-	auto sort_store = [&](uint4 (&v)[ITEMS][WV], uint32_t c) {
+	auto unpack = [&](uint4 (&v)[ITEMS][WV], uint32_t sel) {
+		if (WV != 1) return;
+#pragma unroll
+		for (int k = 0; k < ITEMS; k++) {
+			const uint4 h = v[k][0];
+			const uint32_t b = sel >> (2 * k);
+			if (b & 1) v[k][0] = (b & 2) ? make_uint4(h.z, 0u, h.w, 0u) : make_uint4(h.x, 0u, h.y, 0u);
+		}
+	};
+	auto sort_store = [&](uint4 (&v)[ITEMS][WV], uint32_t sel, uint32_t c) {
 		uint64_t pos;
 		uint32_t n;
 		tile_of(c, pos, n);
+		unpack(v, sel);
 		uint32_t dig[ITEMS], rank[ITEMS];
 #pragma unroll
 		for (int k = 0; k < ITEMS; k++) {
@@ -877,20 +945,21 @@ __global__ __launch_bounds__(NT) void k_sortchunk_g(
 	};
 	uint4 va[ITEMS][WV], vb[ITEMS][WV];
 	uint32_t ca = tile_at(0), cb;
+	uint32_t sa, sb;
 	stage(ca, src0);
 	__syncthreads();
-	load(va, ca, src0);
+	load(va, sa, ca, src0);
 	for (uint32_t j = 0; j * grid < total; j += 2) {
 		cb = tile_at(j + 1);
 		stage(cb, src1);
 		__syncthreads();
-		load(vb, cb, src1);
-		sort_store(va, ca);
+		load(vb, sb, cb, src1);
+		sort_store(va, sa, ca);
 		ca = tile_at(j + 2);
 		stage(ca, src0);
 		__syncthreads();
-		load(va, ca, src0);
-		sort_store(vb, cb);  // an empty tile past the end changes nothing
+		load(va, sa, ca, src0);
+		sort_store(vb, sb, cb);  // an empty tile past the end changes nothing
 	}
 }
 
@@ -926,6 +995,13 @@ __global__ __launch_bounds__(256) void k_run_transpose(const uint16_t *__restric
 		if (c < total && p <= F) run_t[(uint64_t)p * stride + c] = t[ci][pj];
 	}
 }
+
+// key and payload of a loaded tuple: a 16-byte tuple's head, or a narrow
+// {u32 key, u32 row_id} pair
+__device__ __forceinline__ uint64_t key_of(const uint4 &t) { return hj_key_of(t); }
+__device__ __forceinline__ uint64_t pay_of(const uint4 &t) { return hj_pay_of(t); }
+__device__ __forceinline__ uint64_t key_of(const uint2 &t) { return t.x; }
+__device__ __forceinline__ uint64_t pay_of(const uint2 &t) { return t.y; }
 
 // Join over chunk runs.  A work item is (final partition q = (p2 << b1) | p1,
 // probe tile = a range of pass-1 chunks of S).  Descriptors of up to NT
@@ -1003,33 +1079,36 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
 		}
 		return lo;
 	};
-	// load the step of flat indices f0 + u*64 + lane (< fe); k only moves forward
-	auto load_step = [&](uint32_t f0, uint32_t fe, uint32_t &k, uint4 (&t)[RU], bool stream) {
+	// A loaded tuple: narrow ones as the {key, row_id} pair the load fills.
+	using tup_t = typename std::conditional<NARROW, uint2, uint4>::type;
+	// Load the step of flat indices f0 + u*64 + lane (f0 < fe); k only moves
+	// forward.  Every lane loads RU tuples, those past fe the tuple at fe - 1
+	// (the callers skip them): a load under a per-lane condition must be
+	// merged with the register's old value, and for narrow pairs that merge
+	// waits for the data, leaving one load in flight instead of RU.
+	auto load_step = [&](uint32_t f0, uint32_t fe, uint32_t &k, tup_t (&t)[RU], bool stream) {
 #pragma unroll
 		for (uint32_t u = 0; u < RU; u++) {
-			const uint32_t f = f0 + u * HJ_WAVE + lane;
-			if (f < fe) {
-				while (r_off[k + 1] <= f) k++;
-				const uint64_t at = r_src[k] + (f - r_off[k]);
-				if constexpr (NARROW) {
-					const uint2 *src8 = (const uint2 *)P + at;
-					const uint2 h = stream ? hj_load_stream8(src8) : *src8;
-					t[u] = make_uint4(h.x, 0u, h.y, 0u);
-				} else {
-					const uint4 *src = &P[at * WV];
-					t[u] = stream ? hj_load_stream(src) : *src;
-				}
+			const uint32_t f = min(f0 + u * HJ_WAVE + lane, fe - 1);
+			while (r_off[k + 1] <= f) k++;
+			const uint64_t at = r_src[k] + (f - r_off[k]);
+			if constexpr (NARROW) {
+				const uint2 *src8 = (const uint2 *)P + at;
+				t[u] = stream ? hj_load_stream8(src8) : *src8;
+			} else {
+				const uint4 *src = &P[at * WV];
+				t[u] = stream ? hj_load_stream(src) : *src;
 			}
 		}
 	};
-	auto probe_step = [&](uint32_t f0, uint32_t fe, const uint4 (&t)[RU]) {
+	auto probe_step = [&](uint32_t f0, uint32_t fe, const tup_t (&t)[RU]) {
 		uint64_t kk[RU], pp[RU];
 		bool live[RU];
 #pragma unroll
 		for (uint32_t u = 0; u < RU; u++) {
 			live[u] = f0 + u * HJ_WAVE + lane < fe;
-			kk[u] = hj_key_of(t[u]);
-			pp[u] = hj_pay_of(t[u]);
+			kk[u] = key_of(t[u]);
+			pp[u] = pay_of(t[u]);
 		}
 		tab.template probe<(int)RU>(kk, pp, live, B, acc);
 	};
@@ -1074,16 +1153,16 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
 				if (fb < fe) {
 					uint32_t k = find_run(fb);
 					for (uint32_t f0 = fb; f0 < fe; f0 += STEP) {
-						uint4 t[RU];
+						tup_t t[RU];
 						load_step(f0, fe, k, t, false);
 #pragma unroll
 						for (uint32_t u = 0; u < RU; u++) {
 							const uint32_t f = f0 + u * HJ_WAVE + lane;
 							if (f >= fe) continue;
-							const uint64_t key = hj_key_of(t[u]);
+							const uint64_t key = key_of(t[u]);
 							if (mode == 1) tab.count(key, B);
-							else if (mode == 2) tab.place(key, hj_pay_of(t[u]), B);
-							else tab.put(m + f, key, hj_pay_of(t[u]), B);
+							else if (mode == 2) tab.place(key, pay_of(t[u]), B);
+							else tab.put(m + f, key, pay_of(t[u]), B);
 						}
 					}
 				}
@@ -1122,7 +1201,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
 				const uint32_t fb = (uint32_t)((uint64_t)T * wid / NW), fe = (uint32_t)((uint64_t)T * (wid + 1) / NW);
 				if (fb < fe) {
 					uint32_t k = find_run(fb);
-					uint4 ta[RU] = {}, tb[RU] = {};
+					tup_t ta[RU] = {}, tb[RU] = {};
 					load_step(fb, fe, k, ta, true);
 					for (uint32_t f0 = fb;;) {
 						const uint32_t f1 = f0 + STEP;
@@ -1885,12 +1964,21 @@ __global__ __launch_bounds__(NT) void k_join_b(const hj_bpair *__restrict__ pair
 			const uint32_t hmask = nb - 1;
 			for (uint32_t i = threadIdx.x; i < nb; i += NT) heads[i] = 0;
 			__syncthreads();
-			for (uint32_t i = threadIdx.x; i < m; i += NT) {
-				const uint4 t = hj_load(&pr.rb[(rc + i) * WV]);
-				const uint64_t key = hj_key_of(t);
-				keys[i] = key;
-				pays[i] = hj_pay_of(t);
-				if (!LP) next[i] = (uint16_t)atomicExch(&heads[(uint32_t)(key >> B) & hmask], i + 1);
+			// four build tuples per thread and step, loads issued together
+			// (unconditional, clamped to the last tuple)
+			for (uint32_t i0 = threadIdx.x; i0 < m; i0 += 4 * NT) {
+				uint4 t[4];
+#pragma unroll
+				for (int u = 0; u < 4; u++) t[u] = hj_load(&pr.rb[(rc + min(i0 + u * NT, m - 1)) * WV]);
+#pragma unroll
+				for (int u = 0; u < 4; u++) {
+					const uint32_t i = i0 + u * NT;
+					if (i >= m) continue;
+					const uint64_t key = hj_key_of(t[u]);
+					keys[i] = key;
+					pays[i] = hj_pay_of(t[u]);
+					if (!LP) next[i] = (uint16_t)atomicExch(&heads[(uint32_t)(key >> B) & hmask], i + 1);
+				}
 			}
 			__syncthreads();
 			if (LP) {
@@ -1913,13 +2001,15 @@ __global__ __launch_bounds__(NT) void k_join_b(const hj_bpair *__restrict__ pair
 			for (uint64_t j0 = sb + threadIdx.x; j0 < se; j0 += (uint64_t)U * NT) {
 				uint64_t sk[U], sp[U];
 				bool live[U];
+				uint4 t[U];
+				// loads issued together (unconditional, clamped to the last tuple)
+#pragma unroll
+				for (int u = 0; u < U; u++) t[u] = hj_load_stream(&pr.sb[min(j0 + (uint64_t)u * NT, se - 1) * WV]);
 #pragma unroll
 				for (int u = 0; u < U; u++) {
-					const uint64_t j = j0 + (uint64_t)u * NT;
-					live[u] = j < se;
-					const uint4 t = live[u] ? hj_load_stream(&pr.sb[j * WV]) : make_uint4(0, 0, 0, 0);
-					sk[u] = hj_key_of(t);
-					sp[u] = hj_pay_of(t);
+					live[u] = j0 + (uint64_t)u * NT < se;
+					sk[u] = hj_key_of(t[u]);
+					sp[u] = hj_pay_of(t[u]);
 				}
 #pragma unroll
 				for (int u = 0; u < U; u++) {
