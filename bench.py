@@ -54,31 +54,36 @@ def kernel_bytes(name, W, nr, ns, res=None):
     every tuple once, the radix join reads every tuple's head once, an NPHJ
     build reads R and writes a 16-byte entry per tuple, a probe reads S and
     one 16-byte entry per probe.  A head-only read of a tuple wider than 64
-    bytes counts one 64-byte HBM burst.  Two passes over 16-byte tuples store
+    bytes counts one 64-byte HBM burst.  Radix joins of tuples wider than 16
+    bytes partition the 16-byte heads that ptr_keyrid copies out.  Two passes over 16-byte tuples store
     a pass-A tile as 8-byte {key, row_id} tuples when they fit 32 bits
     (res["narrow_tuples"] tuples), and pass B and the join then move 8-byte
     tuples when every tile did (res["narrow"])."""
     N = nr + ns
     head = min(W, 64)
-    a_out = b_out = W * N  # pass A output, pass B output
-    join_in = head * N
-    if res is not None and W == 16 and res.get("passes") == 2:
+    # radix joins of tuples wider than 16 bytes partition their 16-byte heads
+    # (copied out by ptr_keyrid), so the radix kernels move 16-byte tuples
+    Wr = 16
+    a_out = b_out = Wr * N  # pass A output, pass B output
+    join_in = Wr * N
+    if res is not None and res.get("passes") == 2:
         nt = res.get("narrow_tuples", 0)
         a_out = 8 * nt + 16 * (N - nt)
         b_out = join_in = (8 if res.get("narrow") else 16) * N
     return {
-        "radix_scatter": 2 * W * N,
-        "radix_sortchunk": 2 * W * N,
-        "radix_tilesort": W * N + a_out,
+        "radix_scatter": 2 * Wr * N,
+        "radix_sortchunk": 2 * Wr * N,
+        "radix_tilesort": Wr * N + a_out,
         "radix_gathersort": a_out + b_out,
         "radix_join": join_in,
         "lp_build": (head + 16) * nr,
         "sc_build": (head + 16) * nr,
         "lp_probe": (head + 16) * ns,
         "sc_probe": (head + 16) * ns,
-        # late materialisation: heads read and written once; per pair the pair
-        # read and two 8-byte row-id gathers
-        "ptr_keyrid": 32 * N if W > 16 else None,
+        # heads of wider tuples (radix and late materialisation): one burst
+        # read and a 16-byte write per tuple; per pair the pair read and two
+        # 8-byte row-id gathers
+        "ptr_keyrid": (head + 16) * N if W > 16 else None,
         "ptr_retrieve": 32 * res["matched_cnt"] if res else None,
     }.get(name)
 

@@ -147,6 +147,9 @@ int hj_run_radix(hj_ctx_t *ctx, const hj_relation_t *R, const hj_relation_t *S,
                  const hj_params_t *prm, int tk, hj_result_t *out);
 typedef int (*hj_algo_fn)(hj_ctx_t *, const hj_relation_t *, const hj_relation_t *, const hj_params_t *,
                           hj_result_t *);
+// heads (first 16 bytes) of n tuples of wv 16-byte words into a dense
+// 16-byte relation, on the context's stream
+void hj_launch_keyrid(hj_ctx_t *ctx, const void *in, uint64_t n, uint32_t wv, void *out);
 // late materialisation around `inner`: join the {key, row_id} words, emit
 // row-id pairs, then retrieve the checksum through them
 int hj_run_ptr(hj_ctx_t *ctx, const hj_relation_t *R, const hj_relation_t *S, const hj_params_t *prm,

@@ -43,11 +43,10 @@ __global__ __launch_bounds__(PT_THREADS) void k_keyrid(const uint4 *__restrict__
 	const uint64_t tile = (uint64_t)PT_THREADS * KR_U;
 	for (uint64_t base = (uint64_t)blockIdx.x * tile; base < n; base += (uint64_t)gridDim.x * tile) {
 		uint4 v[KR_U];
+		// unconditional loads (clamped to the last tuple): a load under a
+		// per-lane condition waits for its data before the next one issues
 #pragma unroll
-		for (int u = 0; u < KR_U; u++) {
-			const uint64_t i = base + (uint64_t)u * PT_THREADS + threadIdx.x;
-			if (i < n) v[u] = hj_load_stream(in + i * wv);
-		}
+		for (int u = 0; u < KR_U; u++) v[u] = hj_load_stream(in + min(base + (uint64_t)u * PT_THREADS + threadIdx.x, n - 1) * wv);
 #pragma unroll
 		for (int u = 0; u < KR_U; u++) {
 			const uint64_t i = base + (uint64_t)u * PT_THREADS + threadIdx.x;
@@ -77,7 +76,7 @@ __global__ __launch_bounds__(PT_THREADS) void k_retrieve(const uint4 *__restrict
 		for (int u = 0; u < RT_U; u++) {
 			const uint64_t i = base + (uint64_t)u * PT_THREADS + threadIdx.x;
 			const bool live = i < n;
-			const uint4 p = live ? hj_load_stream(pairs + i) : make_uint4(0, 0, 0, 0);
+			const uint4 p = hj_load_stream(pairs + min(i, n - 1));  // unconditional, clamped
 			const uint64_t r = hj_key_of(p) - 1, s = hj_pay_of(p) - 1;  // 0 wraps to out of range
 			ok[u] = live && r < nr && s < ns;  // a hole wraps to ~0 - 1: out of range
 			good += ok[u];
@@ -101,6 +100,12 @@ __global__ __launch_bounds__(PT_THREADS) void k_retrieve(const uint4 *__restrict
 static unsigned pt_grid(hj_ctx_t *ctx, uint64_t n, uint64_t per_block) {
 	const uint64_t want = (n + per_block - 1) / per_block;
 	return (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(want, (uint64_t)ctx->num_cu * 8));
+}
+
+void hj_launch_keyrid(hj_ctx_t *ctx, const void *in, uint64_t n, uint32_t wv, void *out) {
+	if (n)
+		hipLaunchKernelGGL(k_keyrid, dim3(pt_grid(ctx, n, PT_THREADS * KR_U)), dim3(PT_THREADS), 0, ctx->stream,
+		                   (const uint4 *)in, n, wv, (uint4 *)out);
 }
 
 // Disarms the pair buffer whatever way the inner join returns.
@@ -127,12 +132,8 @@ int hj_run_ptr(hj_ctx_t *ctx, const hj_relation_t *R, const hj_relation_t *S, co
 		if ((rc = hj_reserve(ctx, ctx->keyrid[0], nr * 16)) || (rc = hj_reserve(ctx, ctx->keyrid[1], ns * 16)))
 			return rc;
 		hj_kt_begin(ctx, HJ_K_KEYRID);
-		if (nr)
-			hipLaunchKernelGGL(k_keyrid, dim3(pt_grid(ctx, nr, PT_THREADS * KR_U)), dim3(PT_THREADS), 0, ctx->stream,
-			                   (const uint4 *)R->tuples, nr, wv, (uint4 *)ctx->keyrid[0].p);
-		if (ns)
-			hipLaunchKernelGGL(k_keyrid, dim3(pt_grid(ctx, ns, PT_THREADS * KR_U)), dim3(PT_THREADS), 0, ctx->stream,
-			                   (const uint4 *)S->tuples, ns, wv, (uint4 *)ctx->keyrid[1].p);
+		hj_launch_keyrid(ctx, R->tuples, nr, wv, ctx->keyrid[0].p);
+		hj_launch_keyrid(ctx, S->tuples, ns, wv, ctx->keyrid[1].p);
 		hj_kt_end(ctx);
 		HJ_HIP(ctx, hipGetLastError());
 		kr.tuples = ctx->keyrid[0].p;
@@ -196,8 +197,9 @@ int hj_run_ptr(hj_ctx_t *ctx, const hj_relation_t *R, const hj_relation_t *S, co
 	out->buildpart_time = t_extract + r.buildpart_time;
 	out->probejoin_time = r.probejoin_time;
 	out->total_time = t_extract + r.total_time + t_retrieve;
-	// heads read and written once, pairs written and read once, two 8-byte
-	// row ids gathered per pair
-	out->algo_bytes = r.algo_bytes + (wv > 1 ? 32 * (nr + ns) : 0) + 48 * r.matched_cnt;
+	// heads read (one burst of up to 64 bytes) and written once, pairs written
+	// and read once, two 8-byte row ids gathered per pair
+	out->algo_bytes = r.algo_bytes + (wv > 1 ? (uint64_t)(std::min<uint32_t>(W, 64) + 16) * (nr + ns) : 0) +
+	                  48 * r.matched_cnt;
 	return HJ_OK;
 }

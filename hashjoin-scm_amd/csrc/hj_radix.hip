@@ -1576,11 +1576,47 @@ void hj_radix_config(hj_ctx_t *ctx, uint64_t nr, uint32_t *bits, uint32_t *passe
 	hj_radix_default(nr, bits, passes);
 }
 
+static int radix_engine(hj_ctx_t *ctx, const hj_relation_t *R, const hj_relation_t *S, const hj_params_t *prm,
+                        int tk, hj_result_t *out);
+
+// A radix join's result (count, checksum) depends on every tuple's 16-byte
+// head {key, row_id} only, so tuples wider than 16 bytes have their heads
+// copied into dense 16-byte relations (one pass reading a burst per tuple)
+// and the partitions and the join move heads: at 64 bytes 64 + 16 + 48 B per
+// tuple instead of 5 x 64 (the reference carries whole tuples through both
+// passes, store_a_tuple, memaccess.h:309-355).
 int hj_run_radix(hj_ctx_t *ctx, const hj_relation_t *R, const hj_relation_t *S, const hj_params_t *prm, int tk,
                  hj_result_t *out) {
-	const bool lp = tk == HJ_TK_LP;
 	if (tk == HJ_TK_HRO && R->tuple_size > 64)
 		return hj_fail(ctx, HJ_EINVAL, "hro joins tuples of at most 64 bytes (one 512-bit block holds a tuple)");
+	const uint32_t W = R->tuple_size, wv = W / 16;
+	if (wv == 1) return radix_engine(ctx, R, S, prm, tk, out);
+	const uint64_t nr = R->tuple_num, ns = S->tuple_num;
+	int rc;
+	if ((rc = hj_reserve(ctx, ctx->keyrid[0], std::max<uint64_t>(nr, 1) * 16)) ||
+	    (rc = hj_reserve(ctx, ctx->keyrid[1], std::max<uint64_t>(ns, 1) * 16)))
+		return rc;
+	// outer events: the engine records ev[0..3] and the kernel pairs
+	const int e0 = hj_ctx::EV_PTR;
+	HJ_HIP(ctx, hipEventRecord(ctx->ev[e0], ctx->stream));
+	hj_kt_begin(ctx, HJ_K_KEYRID);
+	hj_launch_keyrid(ctx, R->tuples, nr, wv, ctx->keyrid[0].p);
+	hj_launch_keyrid(ctx, S->tuples, ns, wv, ctx->keyrid[1].p);
+	hj_kt_end(ctx);
+	HJ_HIP(ctx, hipGetLastError());
+	HJ_HIP(ctx, hipEventRecord(ctx->ev[e0 + 1], ctx->stream));
+	const hj_relation_t kr{ctx->keyrid[0].p, nr, 16, HJ_LOC_DEVICE}, ks{ctx->keyrid[1].p, ns, 16, HJ_LOC_DEVICE};
+	if ((rc = radix_engine(ctx, &kr, &ks, prm, tk, out))) return rc;
+	const double t_heads = hj_event_seconds(ctx, e0, e0 + 1);
+	out->buildpart_time += t_heads;
+	out->total_time += t_heads;
+	out->algo_bytes += (uint64_t)(std::min<uint32_t>(W, 64) + 16) * (nr + ns);
+	return HJ_OK;
+}
+
+static int radix_engine(hj_ctx_t *ctx, const hj_relation_t *R, const hj_relation_t *S, const hj_params_t *prm,
+                        int tk, hj_result_t *out) {
+	const bool lp = tk == HJ_TK_LP;
 	const uint32_t W = R->tuple_size, wv = W / 16;
 	const uint64_t nr = R->tuple_num, ns = S->tuple_num, N = nr + ns;
 	uint32_t B = prm->radix_bits, P = prm->passes, tuned_b1 = 0;
