@@ -2033,30 +2033,25 @@ __global__ __launch_bounds__(NT) void k_join_b(const hj_bpair *__restrict__ pair
 				}
 				__syncthreads();
 			}
+			// probe tuples U per thread and step; the next step's loads are
+			// issued (unconditional, clamped to the last tuple) before the
+			// current step is probed
 			constexpr int U = 4;
-			for (uint64_t j0 = sb + threadIdx.x; j0 < se; j0 += (uint64_t)U * NT) {
-				uint64_t sk[U], sp[U];
-				bool live[U];
-				uint4 t[U];
-				// loads issued together (unconditional, clamped to the last tuple)
+			auto load = [&](uint4 (&t)[U], uint64_t j0) {
 #pragma unroll
 				for (int u = 0; u < U; u++) t[u] = hj_load_stream(&pr.sb[min(j0 + (uint64_t)u * NT, se - 1) * WV]);
+			};
+			auto probe = [&](const uint4 (&t)[U], uint64_t j0) {
 #pragma unroll
 				for (int u = 0; u < U; u++) {
-					live[u] = j0 + (uint64_t)u * NT < se;
-					sk[u] = hj_key_of(t[u]);
-					sp[u] = hj_pay_of(t[u]);
-				}
-#pragma unroll
-				for (int u = 0; u < U; u++) {
-					if (!live[u]) continue;
-					const uint64_t key = sk[u];
+					if (j0 + (uint64_t)u * NT >= se) continue;
+					const uint64_t key = hj_key_of(t[u]), sp = hj_pay_of(t[u]);
 					uint32_t h = (uint32_t)(key >> B) & hmask;
 					if (!LP) {
 						for (uint32_t hit = heads[h]; hit; hit = next[hit - 1])
 							if (keys[hit - 1] == key) {
 								cnt++;
-								sum += pays[hit - 1] + sp[u];
+								sum += pays[hit - 1] + sp;
 							}
 					} else {
 						for (uint32_t step = 0; step < nb; step++) {
@@ -2064,12 +2059,28 @@ __global__ __launch_bounds__(NT) void k_join_b(const hj_bpair *__restrict__ pair
 							if (!o) break;
 							if (keys[o - 1] == key) {
 								cnt++;
-								sum += pays[o - 1] + sp[u];
+								sum += pays[o - 1] + sp;
 								break;
 							}
 							h = (h + 1) & hmask;
 						}
 					}
+				}
+			};
+			uint64_t j0 = sb + threadIdx.x;
+			if (j0 < se) {
+				uint4 ta[U], tb[U];
+				load(ta, j0);
+				for (;;) {
+					const uint64_t j1 = j0 + (uint64_t)U * NT;
+					if (j1 < se) load(tb, j1);
+					probe(ta, j0);
+					if (j1 >= se) break;
+					const uint64_t j2 = j1 + (uint64_t)U * NT;
+					if (j2 < se) load(ta, j2);
+					probe(tb, j1);
+					if (j2 >= se) break;
+					j0 = j2;
 				}
 			}
 			__syncthreads();
