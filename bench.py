@@ -55,7 +55,8 @@ def kernel_bytes(name, W, nr, ns, res=None):
     build reads R and writes a 16-byte entry per tuple, a probe reads S and
     one 16-byte entry per probe.  A head-only read of a tuple wider than 64
     bytes counts one 64-byte HBM burst.  Radix joins of tuples wider than 16
-    bytes partition the 16-byte heads that ptr_keyrid copies out.  Two passes over 16-byte tuples store
+    bytes partition the 16-byte heads: two passes read them in place in pass
+    A, one pass partitions the heads that ptr_keyrid copies out.  Two passes over 16-byte tuples store
     a pass-A tile as 8-byte {key, row_id} tuples when they fit 32 bits
     (res["narrow_tuples"] tuples), and pass B and the join then move 8-byte
     tuples when every tile did (res["narrow"])."""
@@ -70,10 +71,13 @@ def kernel_bytes(name, W, nr, ns, res=None):
         nt = res.get("narrow_tuples", 0)
         a_out = 8 * nt + 16 * (N - nt)
         b_out = join_in = (8 if res.get("narrow") else 16) * N
+    # two passes over wider tuples read the heads in place in pass A (one
+    # burst per tuple); fewer passes copy them out first (ptr_keyrid)
+    a_in = head * N if (W > 16 and res is not None and res.get("passes") == 2) else Wr * N
     return {
         "radix_scatter": 2 * Wr * N,
         "radix_sortchunk": 2 * Wr * N,
-        "radix_tilesort": Wr * N + a_out,
+        "radix_tilesort": a_in + a_out,
         "radix_gathersort": a_out + b_out,
         "radix_join": join_in,
         "lp_build": (head + 16) * nr,

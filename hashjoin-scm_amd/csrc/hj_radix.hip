@@ -50,6 +50,9 @@ struct hj_seg {
 	uint32_t nchunks;
 	uint32_t bits;       // radix digit width of this segment
 	uint32_t pb_base;    // first entry of its partition bounds (batches)
+	uint32_t in_wv;      // 16-byte words per input tuple when pass A reads only
+	                     // the heads of wider tuples (0: the kernel's own width)
+	uint32_t pad;
 };
 
 struct hj_pair {
@@ -167,12 +170,12 @@ static int scan_u64(hj_ctx_t *ctx, uint64_t *a, uint64_t L, uint64_t *tmp) {
 // ------------------------------------------------------------ segments
 // Pass 1: R and S as two segments; chunk map filled for both.
 __global__ void k_seg_pair(const uint4 *R, uint64_t nr, const uint4 *S, uint64_t ns, uint64_t ch, uint32_t F,
-                           hj_seg *segs, uint32_t *chunk_seg, hj_stats *st) {
+                           hj_seg *segs, uint32_t *chunk_seg, hj_stats *st, uint32_t in_wv) {
 	const uint32_t cr = (uint32_t)((nr + ch - 1) / ch), cs = (uint32_t)((ns + ch - 1) / ch);
 	if (blockIdx.x == 0 && threadIdx.x == 0) {
 		const uint32_t bits = 31 - __clz(F);
-		segs[0] = hj_seg{R, nr, 0, 0, 0, cr, bits, 0};
-		segs[1] = hj_seg{S, ns, nr, (uint64_t)cr * F, cr, cs, bits, F + 1};
+		segs[0] = hj_seg{R, nr, 0, 0, 0, cr, bits, 0, in_wv, 0};
+		segs[1] = hj_seg{S, ns, nr, (uint64_t)cr * F, cr, cs, bits, F + 1, in_wv, 0};
 		st->total_chunks = cr + cs;
 	}
 	for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < cr + cs; c += gridDim.x * blockDim.x)
@@ -202,7 +205,7 @@ __global__ void k_seg2_len(const uint64_t *__restrict__ pb1, uint32_t nseg2, uin
                            uint32_t wv, uint32_t bits2, hj_seg *__restrict__ segs2, uint64_t *__restrict__ nch) {
 	for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < nseg2; s += gridDim.x * blockDim.x) {
 		const uint64_t a = pb1[s], b = pb1[s + 1];
-		hj_seg g;
+		hj_seg g{};
 		g.in = in1 + a * wv;
 		g.len = b - a;
 		g.out_start = a;
@@ -579,17 +582,22 @@ __global__ __launch_bounds__(NT) void k_sortchunk(const hj_seg *__restrict__ seg
 	const uint32_t grid = gridDim.x;
 	// tile c: source, output position, length (an empty tile past the end
 	// reads `out`, whose first tuple always exists)
+	// input tuples are WV words apart, or in_wv when only the heads of wider
+	// tuples are read
+	uint32_t sw = WV;
 	auto tile_of = [&](uint32_t c, const uint4 *&src, uint64_t &pos, uint32_t &n) {
 		if (c >= total) {
 			src = out;
 			pos = 0;
 			n = 0;
+			sw = WV;
 			return;
 		}
 		const hj_seg &g = segs[cmap[c]];
 		const uint64_t cl = c - g.chunk_base;
+		sw = g.in_wv ? g.in_wv : WV;
 		pos = g.out_start + cl * TILE;
-		src = g.in + cl * TILE * WV;
+		src = g.in + cl * TILE * sw;
 		n = (uint32_t)min((uint64_t)TILE, g.len - cl * TILE);
 	};
 	// every thread issues exactly ITEMS*WV loads (see k_scatter)
@@ -602,7 +610,7 @@ __global__ __launch_bounds__(NT) void k_sortchunk(const hj_seg *__restrict__ seg
 		for (int k = 0; k < ITEMS; k++) {
 			const uint32_t idx = min((uint32_t)(k * NT) + threadIdx.x, n ? n - 1 : 0u);
 #pragma unroll
-			for (int w = 0; w < WV; w++) v[k][w] = hj_load_stream(src + (uint64_t)idx * WV + w);
+			for (int w = 0; w < WV; w++) v[k][w] = hj_load_stream(src + (uint64_t)idx * sw + w);
 		}
 	};
 	bool wide = false;
@@ -1577,22 +1585,39 @@ void hj_radix_config(hj_ctx_t *ctx, uint64_t nr, uint32_t *bits, uint32_t *passe
 }
 
 static int radix_engine(hj_ctx_t *ctx, const hj_relation_t *R, const hj_relation_t *S, const hj_params_t *prm,
-                        int tk, hj_result_t *out);
+                        int tk, hj_result_t *out, uint32_t in_wv, const hj_relation_t *Rfb, const hj_relation_t *Sfb);
+
+// passes the engine takes for |R| = nr (the first lines of radix_engine)
+static uint32_t radix_plan_passes(hj_ctx_t *ctx, uint64_t nr, const hj_params_t *prm) {
+	uint32_t B = prm->radix_bits, P = prm->passes, tb1 = 0;
+	if (B == 0 && P == 0) hj_radix_config(ctx, nr, &B, &P, &tb1);
+	if (P == 0) P = B == 0 ? 0 : (B > 11 ? 2 : 1);
+	if (B == 0) P = 0;
+	if (P == 2 && B < 2) P = 1;
+	return P;
+}
 
 // A radix join's result (count, checksum) depends on every tuple's 16-byte
-// head {key, row_id} only, so tuples wider than 16 bytes have their heads
-// copied into dense 16-byte relations (one pass reading a burst per tuple)
-// and the partitions and the join move heads: at 64 bytes 64 + 16 + 48 B per
-// tuple instead of 5 x 64 (the reference carries whole tuples through both
-// passes, store_a_tuple, memaccess.h:309-355).
+// head {key, row_id} only, so for tuples wider than 16 bytes the partitions
+// and the join move heads: at 64 bytes 64 + 8 + 16 + 8 B per tuple (narrow
+// heads) instead of 5 x 64 (the reference carries whole tuples through both
+// passes, store_a_tuple, memaccess.h:309-355).  Two passes read the heads in
+// place in pass A; one pass or none copies them out first.
 int hj_run_radix(hj_ctx_t *ctx, const hj_relation_t *R, const hj_relation_t *S, const hj_params_t *prm, int tk,
                  hj_result_t *out) {
 	if (tk == HJ_TK_HRO && R->tuple_size > 64)
 		return hj_fail(ctx, HJ_EINVAL, "hro joins tuples of at most 64 bytes (one 512-bit block holds a tuple)");
 	const uint32_t W = R->tuple_size, wv = W / 16;
-	if (wv == 1) return radix_engine(ctx, R, S, prm, tk, out);
+	if (wv == 1) return radix_engine(ctx, R, S, prm, tk, out, 0, R, S);
 	const uint64_t nr = R->tuple_num, ns = S->tuple_num;
 	int rc;
+	if (radix_plan_passes(ctx, nr, prm) == 2) {
+		const hj_relation_t hr{R->tuples, nr, 16, HJ_LOC_DEVICE}, hs{S->tuples, ns, 16, HJ_LOC_DEVICE};
+		if ((rc = radix_engine(ctx, &hr, &hs, prm, tk, out, wv, R, S))) return rc;
+		// pass A reads a burst of the wide tuple, not 16 bytes
+		out->algo_bytes += (uint64_t)(std::min<uint32_t>(W, 64) - 16) * (nr + ns);
+		return HJ_OK;
+	}
 	if ((rc = hj_reserve(ctx, ctx->keyrid[0], std::max<uint64_t>(nr, 1) * 16)) ||
 	    (rc = hj_reserve(ctx, ctx->keyrid[1], std::max<uint64_t>(ns, 1) * 16)))
 		return rc;
@@ -1606,7 +1631,7 @@ int hj_run_radix(hj_ctx_t *ctx, const hj_relation_t *R, const hj_relation_t *S, 
 	HJ_HIP(ctx, hipGetLastError());
 	HJ_HIP(ctx, hipEventRecord(ctx->ev[e0 + 1], ctx->stream));
 	const hj_relation_t kr{ctx->keyrid[0].p, nr, 16, HJ_LOC_DEVICE}, ks{ctx->keyrid[1].p, ns, 16, HJ_LOC_DEVICE};
-	if ((rc = radix_engine(ctx, &kr, &ks, prm, tk, out))) return rc;
+	if ((rc = radix_engine(ctx, &kr, &ks, prm, tk, out, 0, R, S))) return rc;
 	const double t_heads = hj_event_seconds(ctx, e0, e0 + 1);
 	out->buildpart_time += t_heads;
 	out->total_time += t_heads;
@@ -1614,8 +1639,11 @@ int hj_run_radix(hj_ctx_t *ctx, const hj_relation_t *R, const hj_relation_t *S, 
 	return HJ_OK;
 }
 
+// in_wv > 1: R and S hold tuples of in_wv 16-byte words and the engine
+// works on their heads (pass A reads them in place; two passes only).  Rfb /
+// Sfb: the caller's relations, for the exact recomputation of an lp join.
 static int radix_engine(hj_ctx_t *ctx, const hj_relation_t *R, const hj_relation_t *S, const hj_params_t *prm,
-                        int tk, hj_result_t *out) {
+                        int tk, hj_result_t *out, uint32_t in_wv, const hj_relation_t *Rfb, const hj_relation_t *Sfb) {
 	const bool lp = tk == HJ_TK_LP;
 	const uint32_t W = R->tuple_size, wv = W / 16;
 	const uint64_t nr = R->tuple_num, ns = S->tuple_num, N = nr + ns;
@@ -1726,7 +1754,7 @@ static int radix_engine(hj_ctx_t *ctx, const hj_relation_t *R, const hj_relation
 	if (P >= 1) {
 		hipLaunchKernelGGL(k_seg_pair, dim3(std::max<unsigned>(1, (unsigned)((chunks1 + 255) / 256))), dim3(256), 0,
 		                   ctx->stream, (const uint4 *)R->tuples, nr, (const uint4 *)S->tuples, ns, ch1, F1, segs1,
-		                   cmap, st);
+		                   cmap, st, in_wv);
 		if (P == 1) {
 			rc = launch_pass(ctx, wv, variant, (unsigned)chunks1, segs1, cmap, ch1, 0, b1, hist, hist1_len,
 			                 ctx->part[0].p, st, (uint64_t *)ctx->scan_tmp.p);
@@ -1837,7 +1865,7 @@ static int radix_engine(hj_ctx_t *ctx, const hj_relation_t *R, const hj_relation
 		// which is what nphj_lp computes with its duplicate fix-up.
 		hj_result_t r2{};
 		hj_params_t p2{};
-		rc = hj_run_nphj_lp(ctx, R, S, &p2, &r2);
+		rc = hj_run_nphj_lp(ctx, Rfb, Sfb, &p2, &r2);
 		if (rc) return rc;
 		out->matched_cnt = r2.matched_cnt;
 		out->check_sum = r2.check_sum;
