@@ -77,10 +77,12 @@ struct hj_ctx {
 	bool tuned = false;
 
 	// timing
-	// ev[0..3] phases of a join, ev[4..] kernel pairs, the last three the
-	// outer phases of a late-materialising join
-	static const int NEV = 2 * HJ_MAX_KTIMES + 7;
+	// ev[0..3] phases of a join, ev[4..] kernel pairs, then three for the
+	// outer phases of a late-materialising join and two for the head copy of
+	// a wide radix join
+	static const int NEV = 2 * HJ_MAX_KTIMES + 9;
 	static const int EV_PTR = 2 * HJ_MAX_KTIMES + 4;
+	static const int EV_HEAD = 2 * HJ_MAX_KTIMES + 7;  // head copy of a wide radix join
 	hipEvent_t ev[NEV] = {};
 	bool kernel_times = false;
 	int n_kt = 0;

@@ -125,10 +125,13 @@ int hj_run_ptr(hj_ctx_t *ctx, const hj_relation_t *R, const hj_relation_t *S, co
 	hj_stats *st = (hj_stats *)ctx->stats.p;
 	int rc;
 
-	// 1. heads of wide tuples
+	// 1. heads of wide tuples.  A radix join that takes two passes reads them
+	// in place in its first pass, and with one pass copies them itself;
+	// the NPHJ probes that emit pairs read dense 16-byte heads.
 	hj_relation_t kr = *R, ks = *S;
+	const bool nphj = inner == hj_run_nphj_sc || inner == hj_run_nphj_lp;
 	HJ_HIP(ctx, hipEventRecord(ctx->ev[e0], ctx->stream));
-	if (wv > 1) {
+	if (wv > 1 && nphj) {
 		if ((rc = hj_reserve(ctx, ctx->keyrid[0], nr * 16)) || (rc = hj_reserve(ctx, ctx->keyrid[1], ns * 16)))
 			return rc;
 		hj_kt_begin(ctx, HJ_K_KEYRID);
@@ -199,7 +202,8 @@ int hj_run_ptr(hj_ctx_t *ctx, const hj_relation_t *R, const hj_relation_t *S, co
 	out->total_time = t_extract + r.total_time + t_retrieve;
 	// heads read (one burst of up to 64 bytes) and written once, pairs written
 	// and read once, two 8-byte row ids gathered per pair
-	out->algo_bytes = r.algo_bytes + (wv > 1 ? (uint64_t)(std::min<uint32_t>(W, 64) + 16) * (nr + ns) : 0) +
+	// (a radix inner join counts its own head reads)
+	out->algo_bytes = r.algo_bytes + (wv > 1 && nphj ? (uint64_t)(std::min<uint32_t>(W, 64) + 16) * (nr + ns) : 0) +
 	                  48 * r.matched_cnt;
 	return HJ_OK;
 }

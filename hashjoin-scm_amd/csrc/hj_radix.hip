@@ -1611,7 +1611,9 @@ int hj_run_radix(hj_ctx_t *ctx, const hj_relation_t *R, const hj_relation_t *S, 
 	if (wv == 1) return radix_engine(ctx, R, S, prm, tk, out, 0, R, S);
 	const uint64_t nr = R->tuple_num, ns = S->tuple_num;
 	int rc;
-	if (radix_plan_passes(ctx, nr, prm) == 2) {
+	// (an lp join that emits row-id pairs may fall back to nphj_lp, whose
+	// pair-emitting probe reads dense 16-byte heads: it takes the copy)
+	if (radix_plan_passes(ctx, nr, prm) == 2 && !(ctx->emit_pairs && tk == HJ_TK_LP)) {
 		const hj_relation_t hr{R->tuples, nr, 16, HJ_LOC_DEVICE}, hs{S->tuples, ns, 16, HJ_LOC_DEVICE};
 		if ((rc = radix_engine(ctx, &hr, &hs, prm, tk, out, wv, R, S))) return rc;
 		// pass A reads a burst of the wide tuple, not 16 bytes
@@ -1621,8 +1623,9 @@ int hj_run_radix(hj_ctx_t *ctx, const hj_relation_t *R, const hj_relation_t *S, 
 	if ((rc = hj_reserve(ctx, ctx->keyrid[0], std::max<uint64_t>(nr, 1) * 16)) ||
 	    (rc = hj_reserve(ctx, ctx->keyrid[1], std::max<uint64_t>(ns, 1) * 16)))
 		return rc;
-	// outer events: the engine records ev[0..3] and the kernel pairs
-	const int e0 = hj_ctx::EV_PTR;
+	// own events: the engine records ev[0..3] and the kernel pairs, a
+	// late-materialising join around this one the EV_PTR ones
+	const int e0 = hj_ctx::EV_HEAD;
 	HJ_HIP(ctx, hipEventRecord(ctx->ev[e0], ctx->stream));
 	hj_kt_begin(ctx, HJ_K_KEYRID);
 	hj_launch_keyrid(ctx, R->tuples, nr, wv, ctx->keyrid[0].p);
@@ -1631,7 +1634,7 @@ int hj_run_radix(hj_ctx_t *ctx, const hj_relation_t *R, const hj_relation_t *S, 
 	HJ_HIP(ctx, hipGetLastError());
 	HJ_HIP(ctx, hipEventRecord(ctx->ev[e0 + 1], ctx->stream));
 	const hj_relation_t kr{ctx->keyrid[0].p, nr, 16, HJ_LOC_DEVICE}, ks{ctx->keyrid[1].p, ns, 16, HJ_LOC_DEVICE};
-	if ((rc = radix_engine(ctx, &kr, &ks, prm, tk, out, 0, R, S))) return rc;
+	if ((rc = radix_engine(ctx, &kr, &ks, prm, tk, out, 0, &kr, &ks))) return rc;
 	const double t_heads = hj_event_seconds(ctx, e0, e0 + 1);
 	out->buildpart_time += t_heads;
 	out->total_time += t_heads;
