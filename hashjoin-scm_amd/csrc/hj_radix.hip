@@ -741,37 +741,58 @@ __global__ __launch_bounds__(256) void k_runs_to_hist(const uint16_t *__restrict
 	}
 }
 
-// first[c2] = pass-A tile whose run holds the first tuple of pass-B tile c2
-// (largest i with row[i] <= tile start; row = scanned counts of its digit),
-// and order[] = the pass-B tiles sorted by (segment, tile index k, digit):
-// tiles of neighbouring digits at the same k read the same pass-A tiles, so
-// processing them together lets them share those lines in L2.
+// One pass-B tile, everything k_sortchunk_g needs to stage and place it,
+// so that a workgroup reads one uniform record per tile instead of a chain
+// of dependent lookups (tile -> segment -> pass-A segment -> first run).
+struct hj_gtile {
+	uint64_t pos;         // first output position
+	uint64_t at;          // offset of its digit's row in hist / beg_t
+	uint64_t base;        // first tuple of its pass-A segment
+	uint32_t c;           // tile (chunk) id: run table and chunk start
+	uint32_t n;           // tuples
+	uint32_t first;       // pass-A tile whose run holds the first tuple
+	uint32_t nchunks;     // pass-A tiles of the segment
+	uint32_t chunk_base;  // global id of the segment's first pass-A tile
+	uint32_t pad;
+};
+
+// gt[] = the pass-B tiles sorted by (segment, tile index k, digit): tiles of
+// neighbouring digits at the same k read the same pass-A tiles, so
+// processing them together lets them share those lines in L2.  first = the
+// largest i with row[i] <= tile start (row = scanned counts of its digit).
 __global__ void k_first_run(const hj_seg *__restrict__ segs1, const hj_seg *__restrict__ segs2,
                             const uint32_t *__restrict__ chunk_seg2, const hj_stats *__restrict__ st, uint32_t b1,
-                            uint32_t tile, const uint64_t *__restrict__ hist, uint32_t *__restrict__ first,
-                            uint32_t *__restrict__ order) {
+                            uint32_t tile, const uint64_t *__restrict__ hist, hj_gtile *__restrict__ gt) {
 	const uint32_t total = (uint32_t)st->total_chunks, F1 = 1u << b1;
 	for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < total; c += gridDim.x * blockDim.x) {
 		const uint32_t s2 = chunk_seg2[c];
 		const hj_seg &g2 = segs2[s2];
-		{
-			const uint32_t seg = s2 >> b1, d1 = s2 & (F1 - 1), k = c - g2.chunk_base;
-			uint32_t rank = seg ? segs2[F1].chunk_base : 0u;  // segment 1 follows all of segment 0
-			for (uint32_t d = 0; d < F1; d++) {
-				const uint32_t n = segs2[seg * F1 + d].nchunks;
-				rank += min(n, k) + (d < d1 && n > k ? 1u : 0u);
-			}
-			order[rank] = c;
+		const uint32_t seg = s2 >> b1, d1 = s2 & (F1 - 1), k = c - g2.chunk_base;
+		uint32_t rank = seg ? segs2[F1].chunk_base : 0u;  // segment 1 follows all of segment 0
+		for (uint32_t d = 0; d < F1; d++) {
+			const uint32_t n = segs2[seg * F1 + d].nchunks;
+			rank += min(n, k) + (d < d1 && n > k ? 1u : 0u);
 		}
-		const uint64_t ts = g2.out_start + (uint64_t)(c - g2.chunk_base) * tile;
-		const hj_seg &g1 = segs1[s2 >> b1];
-		const uint64_t *row = hist + g1.hist_base + (uint64_t)(s2 & ((1u << b1) - 1)) * g1.nchunks;
+		const uint64_t ts = g2.out_start + (uint64_t)k * tile;
+		const hj_seg &g1 = segs1[seg];
+		const uint64_t at = g1.hist_base + (uint64_t)d1 * g1.nchunks;
+		const uint64_t *row = hist + at;
 		uint32_t lo = 0, hi = g1.nchunks;
 		while (hi - lo > 1) {
 			const uint32_t mid = (lo + hi) >> 1;
 			if (row[mid] <= ts) lo = mid; else hi = mid;
 		}
-		first[c] = lo;
+		hj_gtile t;
+		t.pos = ts;
+		t.at = at;
+		t.base = g1.out_start;
+		t.c = c;
+		t.n = (uint32_t)min((uint64_t)tile, g2.out_start + g2.len - ts);
+		t.first = lo;
+		t.nchunks = g1.nchunks;
+		t.chunk_base = g1.chunk_base;
+		t.pad = 0;
+		gt[rank] = t;
 	}
 }
 
@@ -783,17 +804,17 @@ __global__ void k_first_run(const hj_seg *__restrict__ segs1, const hj_seg *__re
 // thread issues exactly ITEMS*WV of them (past the tile end they re-read its
 // last tuple) so the waits stay exact.  row = scanned counts of one digit
 // over the pass-A tiles of one segment (row[i+1] - row[i] = tile i's run
-// length), brow = where those runs start inside their tiles.
+// length), brow = where those runs start inside their tiles, and pass-A tile
+// i starts at tuple i*tileA of its segment.
 template <int WV, int NT, int ITEMS>
 __global__ __launch_bounds__(NT) void k_sortchunk_g(
-    const hj_seg *__restrict__ segs2, const uint32_t *__restrict__ chunk_seg2, const hj_seg *__restrict__ segs1,
-    const uint64_t *__restrict__ hist, const uint16_t *__restrict__ beg_t, const uint32_t *__restrict__ first,
-    const uint32_t *__restrict__ order, const hj_stats *__restrict__ st, uint32_t b1, uint32_t bits,
+    const hj_gtile *__restrict__ gt, const uint64_t *__restrict__ hist, const uint16_t *__restrict__ beg_t,
+    const hj_stats *__restrict__ st, uint32_t b1, uint32_t bits,
     const uint4 *__restrict__ in, uint4 *__restrict__ out, uint16_t *__restrict__ run, uint64_t *__restrict__ cstart,
-    bool narrow_ok, const uint8_t *__restrict__ tnarrow) {
+    bool narrow_ok, const uint8_t *__restrict__ tnarrow, uint32_t tileA) {
 	constexpr int TILE = NT * ITEMS;
 	extern __shared__ uint4 lds_u4[];
-	const uint32_t F = 1u << bits, fmask = F - 1, F1 = 1u << b1;
+	const uint32_t F = 1u << bits, fmask = F - 1;
 	// 16-byte tuples whose keys and row ids all fit 32 bits (pass A found no
 	// wider one) leave as 8-byte {key, row_id}: half the writes here and
 	// half the reads of the join
@@ -808,52 +829,63 @@ __global__ __launch_bounds__(NT) void k_sortchunk_g(
 	const uint32_t grid = gridDim.x;
 	for (uint32_t p = threadIdx.x; p < F; p += NT) tcnt[p] = 0;
 
-	auto tile_of = [&](uint32_t c, uint64_t &pos, uint32_t &n) {
-		if (c >= total) {
-			pos = 0;
-			n = 0;
-			return;
-		}
-		const hj_seg &g2 = segs2[chunk_seg2[c]];
-		pos = g2.out_start + (uint64_t)(c - g2.chunk_base) * TILE;
-		n = (uint32_t)min((uint64_t)TILE, g2.out_start + g2.len - pos);
+	// Staging is split in two so that its table reads are issued a tile
+	// ahead: prefetch() reads, for thread i, the run of pass-A tile first + i
+	// (every thread the same loads, at clamped indices), and stage() writes
+	// the source offset of every tuple of tile t into srcb[0..n) from those
+	// values; bit 31 marks a pass-A tile stored as 8-byte tuples (segments
+	// hold < 2^31 tuples then).  A tile covering more than NT pass-A runs
+	// reads the rest in stage() itself.
+	struct pre_t {
+		uint64_t h0, h1, stop;
+		uint32_t b, nar;
 	};
-	// the segment base that source offsets of tile c are relative to
-	auto base_of = [&](uint32_t c) -> const uint4 * {
-		return c < total ? in + segs1[chunk_seg2[c] >> b1].out_start * WV : in;
+	// (no branch around the loads: a load whose result merges with another
+	// value makes the compiler wait for every load in flight; an empty
+	// record has nchunks = 1 and reads entry 0 of the tables)
+	auto prefetch = [&](const hj_gtile &t) -> pre_t {
+		pre_t p;
+		const uint64_t *row = hist + t.at;
+		const uint32_t last = t.nchunks - 1;
+		const uint32_t ii = min(t.first + threadIdx.x, last);
+		p.h0 = row[ii];
+		p.h1 = row[ii + 1];
+		p.b = beg_t[t.at + ii];
+		p.nar = tnarrow[t.chunk_base + ii];
+		p.stop = row[min(t.first + (uint32_t)NT, last)];
+		return p;
 	};
-	// source offset of every tuple of tile c into srcb[0..n); bit 31 marks a
-	// pass-A tile stored as 8-byte tuples (segments hold < 2^31 tuples then)
-	auto stage = [&](uint32_t c, uint32_t *srcb) {
-		uint64_t pos;
-		uint32_t n;
-		tile_of(c, pos, n);
+	auto run_src = [&](uint32_t *srcb, uint64_t pos, uint32_t n, uint32_t ii, uint64_t H, uint64_t H1, uint32_t b,
+	                   uint32_t nar) {
+		const uint64_t lo = max(H, pos), hi = min(H1, pos + n);
+		uint32_t sp = ii * tileA + b + (uint32_t)(lo - H);
+		if (WV == 1 && narrow_ok && lo < hi && nar) sp |= 1u << 31;  // nar is unset unless narrow_ok
+		for (uint64_t q = lo; q < hi; q++) srcb[q - pos] = sp + (uint32_t)(q - lo);
+	};
+	auto stage = [&](const pre_t &p, const hj_gtile &t, uint32_t *srcb) {
+		const uint64_t pos = t.pos;
+		const uint32_t n = t.n;
 		if (n == 0) return;
-		const uint32_t s2 = chunk_seg2[c];
-		const hj_seg &g1 = segs1[s2 >> b1];
-		const uint64_t at = g1.hist_base + (uint64_t)(s2 & (F1 - 1)) * g1.nchunks;
-		const uint64_t *row = hist + at;
-		const uint16_t *brow = beg_t + at;
-		for (uint32_t i = first[c];; i += NT) {
+		if (t.first + threadIdx.x < t.nchunks)
+			run_src(srcb, pos, n, t.first + threadIdx.x, p.h0, p.h1, p.b, p.nar);
+		if (t.first + NT >= t.nchunks || p.stop >= pos + n) return;
+		const uint64_t *row = hist + t.at;
+		const uint16_t *brow = beg_t + t.at;
+		for (uint32_t i = t.first + NT;; i += NT) {
 			const uint32_t ii = i + threadIdx.x;
-			if (ii < g1.nchunks) {
-				const uint64_t H = row[ii], lo = max(H, pos), hi = min(row[ii + 1], pos + n);
-				uint32_t sp = ii * (uint32_t)TILE + brow[ii] + (uint32_t)(lo - H);
-				if (WV == 1 && narrow_ok && lo < hi && tnarrow[g1.chunk_base + ii]) sp |= 1u << 31;
-				for (uint64_t q = lo; q < hi; q++) srcb[q - pos] = sp + (uint32_t)(q - lo);
-			}
-			if (i + NT >= g1.nchunks || row[i + NT] >= pos + n) break;
+			if (ii < t.nchunks)
+				run_src(srcb, pos, n, ii, row[ii], row[ii + 1], brow[ii], tnarrow[t.chunk_base + ii]);
+			if (i + NT >= t.nchunks || row[i + NT] >= pos + n) break;
 		}
 	};
-	// Load tile c into v.  For 16-byte tuples, sel gets two bits per item
+	// Load tile t into v.  For 16-byte tuples, sel gets two bits per item
 	// (pass-A tile stored narrow, upper half of the loaded pair) and v the
 	// raw 16-byte load; unpack() picks the tuple out when the tile is sorted,
 	// since picking it here would wait for every load before the next issues.
-	auto load = [&](uint4 (&v)[ITEMS][WV], uint32_t &sel, uint32_t c, const uint32_t *srcb) {
-		uint64_t pos;
-		uint32_t n;
-		tile_of(c, pos, n);
-		const uint4 *base = base_of(c);
+	auto load = [&](uint4 (&v)[ITEMS][WV], uint32_t &sel, const hj_gtile &t, const uint32_t *srcb) {
+		const uint32_t n = t.n;
+		// source offsets are relative to the pass-A segment's first tuple
+		const uint4 *base = in + t.base * WV;
 		sel = 0;
 #pragma unroll
 		for (int k = 0; k < ITEMS; k++) {
@@ -862,12 +894,13 @@ __global__ __launch_bounds__(NT) void k_sortchunk_g(
 			// cached loads: a run's neighbours belong to tiles of nearby digits
 			// that the same XCD works on at the same time
 			if (WV == 1) {
-				// an 8-byte tuple at offset o of tile t lies at 8-byte index
-				// t*2*TILE + o = sp + t*TILE; the 16-byte load of its pair keeps
-				// one load per tuple either way
+				// an 8-byte tuple at offset o of pass-A tile t (tileA tuples, a
+				// power of two) lies at 8-byte index t*2*tileA + o = sp +
+				// t*tileA; the 16-byte load of its pair keeps one load per tuple
+				// either way
 				const bool nw = narrow_ok && (e >> 31);
 				const uint64_t sp = narrow_ok ? (e & 0x7fffffffu) : e;
-				const uint64_t u8 = sp + (sp & ~(uint64_t)(TILE - 1));
+				const uint64_t u8 = sp + (sp & ~(uint64_t)(tileA - 1));
 				v[k][0] = hj_load(base + (nw ? u8 >> 1 : sp));
 				sel |= ((nw ? 1u : 0u) | (nw && (u8 & 1) ? 2u : 0u)) << (2 * k);
 			} else {
@@ -886,10 +919,9 @@ __global__ __launch_bounds__(NT) void k_sortchunk_g(
 			if (b & 1) v[k][0] = (b & 2) ? make_uint4(h.z, 0u, h.w, 0u) : make_uint4(h.x, 0u, h.y, 0u);
 		}
 	};
-	auto sort_store = [&](uint4 (&v)[ITEMS][WV], uint32_t sel, uint32_t c) {
-		uint64_t pos;
-		uint32_t n;
-		tile_of(c, pos, n);
+	auto sort_store = [&](uint4 (&v)[ITEMS][WV], uint32_t sel, const hj_gtile &t) {
+		const uint64_t pos = t.pos;
+		const uint32_t n = t.n, c = t.c;
 		unpack(v, sel);
 		uint32_t dig[ITEMS], rank[ITEMS];
 #pragma unroll
@@ -901,20 +933,41 @@ __global__ __launch_bounds__(NT) void k_sortchunk_g(
 			}
 		}
 		__syncthreads();
-		{
+		// Every thread clears the counters it owns once it has read them (no
+		// other thread reads them), so the next tile's counting, which has no
+		// barrier of its own before it, finds them zero: the barriers below
+		// order the clearing before it.
+		if (per == 1) {
+			// one digit per thread (F <= NT), without a loop: a loop that
+			// stores makes the compiler wait for every load in flight first
+			const uint32_t cnt = threadIdx.x < F ? tcnt[threadIdx.x] : 0u;
+			if (threadIdx.x < F) tcnt[threadIdx.x] = 0;
+			uint32_t tot;
+			const uint32_t r = hj_block_exscan<uint32_t>(cnt, red, &tot);
+			if (n && threadIdx.x < F) {
+				tbase[threadIdx.x] = r;
+				run[(uint64_t)c * (F + 1) + threadIdx.x] = (uint16_t)r;
+			}
+			if (threadIdx.x == 0 && n) {
+				run[(uint64_t)c * (F + 1) + F] = (uint16_t)n;
+				cstart[c] = pos;
+			}
+		} else {
 			const uint32_t p0 = threadIdx.x * per;
 			uint32_t sum = 0;
 			for (uint32_t q = 0; q < per; q++)
 				if (p0 + q < F) sum += tcnt[p0 + q];
 			uint32_t tot;
 			uint32_t r = hj_block_exscan<uint32_t>(sum, red, &tot);
-			if (n)
-				for (uint32_t q = 0; q < per; q++)
-					if (p0 + q < F) {
+			for (uint32_t q = 0; q < per; q++)
+				if (p0 + q < F) {
+					if (n) {
 						tbase[p0 + q] = r;
 						run[(uint64_t)c * (F + 1) + p0 + q] = (uint16_t)r;
-						r += tcnt[p0 + q];
 					}
+					r += tcnt[p0 + q];
+					tcnt[p0 + q] = 0;
+				}
 			if (threadIdx.x == 0 && n) {
 				run[(uint64_t)c * (F + 1) + F] = (uint16_t)n;
 				cstart[c] = pos;
@@ -933,12 +986,19 @@ __global__ __launch_bounds__(NT) void k_sortchunk_g(
 		__syncthreads();
 		if (narrow) {
 			uint2 *dst8 = (uint2 *)out + pos;
-			for (uint32_t u = threadIdx.x; u < n; u += NT) hj_store_stream8(&dst8[u], tile[u].x, tile[u].z);
+#pragma unroll
+			for (int k = 0; k < ITEMS; k++) {  // fixed trip count: no loop to wait before
+				const uint32_t u = k * NT + threadIdx.x;
+				if (u < n) hj_store_stream8(&dst8[u], tile[u].x, tile[u].z);
+			}
 		} else {
 			uint4 *dst = out + pos * WV;
-			for (uint32_t u = threadIdx.x; u < n * WV; u += NT) hj_store_stream(&dst[u], tile[u]);
+#pragma unroll
+			for (int k = 0; k < ITEMS * WV; k++) {
+				const uint32_t u = k * NT + threadIdx.x;
+				if (u < n * WV) hj_store_stream(&dst[u], tile[u]);
+			}
 		}
-		for (uint32_t p = threadIdx.x; p < F; p += NT) tcnt[p] = 0;
 	};
 
 	// Step j of this workgroup takes ordered tile j*grid + xcd*(grid/8) + lane
@@ -947,27 +1007,49 @@ __global__ __launch_bounds__(NT) void k_sortchunk_g(
 	// order, i.e. neighbouring digits at the same k, in its own L2.  grid is
 	// a multiple of 8; past the end a step takes an empty tile.
 	const uint32_t lane = blockIdx.x >> 3, xslot = (blockIdx.x & 7) * (grid >> 3);
-	auto tile_at = [&](uint32_t j) -> uint32_t {
+	auto tile_at = [&](uint32_t j) -> hj_gtile {
 		const uint32_t w = j * grid + xslot + lane;
-		return w < total ? order[w] : total;
+		if (w < total) return gt[w];
+		hj_gtile e{};  // empty: n = 0, reads the first tuple of `in`
+		e.nchunks = 1;
+		return e;
 	};
+	if (total == 0) return;
+	// Half step h (tile h is in flight into v[h&1], tile h+1 is staged in
+	// src[(h+1)&1], tile h+2's tables are prefetched): prefetch tile h+3's
+	// tables, load tile h+1, stage tile h+2 into src[h&1] (free: its last
+	// reader, the load of tile h, is behind the barriers of the sort of tile
+	// h-1), then sort tile h.  Stores count in the same hardware counter as
+	// loads, so this order is the one in which every wait is for exactly the
+	// oldest loads it needs: the staging waits for a prefetch issued a half
+	// step earlier, the sort for a load issued a half step earlier, and
+	// neither for the loads issued after them.  Records are read two half
+	// steps before their prefetch.
 	uint4 va[ITEMS][WV], vb[ITEMS][WV];
-	uint32_t ca = tile_at(0), cb;
+	hj_gtile ta = tile_at(0), tb = tile_at(1), tc = tile_at(2), td = tile_at(3);
 	uint32_t sa, sb;
-	stage(ca, src0);
+	stage(prefetch(ta), ta, src0);
+	const pre_t p1 = prefetch(tb);
+	pre_t p2 = prefetch(tc), p3;
 	__syncthreads();
-	load(va, sa, ca, src0);
+	load(va, sa, ta, src0);
+	stage(p1, tb, src1);
+	__syncthreads();
 	for (uint32_t j = 0; j * grid < total; j += 2) {
-		cb = tile_at(j + 1);
-		stage(cb, src1);
-		__syncthreads();
-		load(vb, sb, cb, src1);
-		sort_store(va, sa, ca);
-		ca = tile_at(j + 2);
-		stage(ca, src0);
-		__syncthreads();
-		load(va, sa, ca, src0);
-		sort_store(vb, sb, cb);  // an empty tile past the end changes nothing
+		const hj_gtile te = tile_at(j + 4);
+		p3 = prefetch(td);
+		load(vb, sb, tb, src1);
+		stage(p2, tc, src0);
+		sort_store(va, sa, ta);
+		const hj_gtile tf = tile_at(j + 5);
+		p2 = prefetch(te);
+		load(va, sa, tc, src0);
+		stage(p3, td, src1);
+		sort_store(vb, sb, tb);  // an empty tile past the end changes nothing
+		ta = tc;
+		tb = td;
+		tc = te;
+		td = tf;
 	}
 }
 
@@ -1462,12 +1544,38 @@ static void launch_sortchunk_t(hj_ctx_t *ctx, unsigned grid, size_t lds, const h
 	                   bits, (uint4 *)out, run, cst, ctx->narrow_ok, ctx->tile_narrow);
 }
 
-static void launch_sortchunk(hj_ctx_t *ctx, uint32_t wv, unsigned grid, size_t lds, const hj_seg *segs,
-                             const uint32_t *cmap, uint32_t shift, uint32_t bits, void *out, uint16_t *run,
-                             uint64_t *cst) {
-	// shapes must match cfg_for(wv, 0)
+// Pass-A tile of 16-byte tuples (threads x tuples per thread): long enough
+// that a run of one first digit in a tile is about 16 tuples, since pass B
+// gathers those runs (2048 tuples up to 7 first-pass bits, 4096 above: by
+// 1024 threads for dense tuples, by 512 threads with 8 loads each for the
+// heads of wider tuples read in place, whose loads are spread further);
+// part_variant bits 12-13 force 1: 512x4, 2: 1024x4, 3: 512x8.  Wider
+// tuples use cfg_for(wv, 0).
+static part_cfg cfg_passA(uint32_t wv, uint32_t variant, uint32_t b1, uint32_t in_wv) {
+	if (wv != 1) return cfg_for(wv, 0);
+	switch ((variant >> 12) & 3) {
+	case 1: return part_cfg{512, 4, 2048};
+	case 2: return part_cfg{1024, 4, 4096};
+	case 3: return part_cfg{512, 8, 4096};
+	default:
+		if (b1 < 8) return part_cfg{512, 4, 2048};
+		return in_wv > 1 ? part_cfg{512, 8, 4096} : part_cfg{1024, 4, 4096};
+	}
+}
+
+static void launch_sortchunk(hj_ctx_t *ctx, uint32_t wv, const part_cfg &pc, unsigned grid, size_t lds,
+                             const hj_seg *segs, const uint32_t *cmap, uint32_t shift, uint32_t bits, void *out,
+                             uint16_t *run, uint64_t *cst) {
+	// shapes must match cfg_passA
 	switch (wv) {
-	case 1: launch_sortchunk_t<1, 512, 4>(ctx, grid, lds, segs, cmap, shift, bits, out, run, cst); break;
+	case 1:
+		if (pc.nt == 1024)
+			launch_sortchunk_t<1, 1024, 4>(ctx, grid, lds, segs, cmap, shift, bits, out, run, cst);
+		else if (pc.items == 8)
+			launch_sortchunk_t<1, 512, 8>(ctx, grid, lds, segs, cmap, shift, bits, out, run, cst);
+		else
+			launch_sortchunk_t<1, 512, 4>(ctx, grid, lds, segs, cmap, shift, bits, out, run, cst);
+		break;
 	case 2: launch_sortchunk_t<2, 512, 2>(ctx, grid, lds, segs, cmap, shift, bits, out, run, cst); break;
 	case 4: launch_sortchunk_t<4, 512, 1>(ctx, grid, lds, segs, cmap, shift, bits, out, run, cst); break;
 	case 8: launch_sortchunk_t<8, 256, 1>(ctx, grid, lds, segs, cmap, shift, bits, out, run, cst); break;
@@ -1477,17 +1585,16 @@ static void launch_sortchunk(hj_ctx_t *ctx, uint32_t wv, unsigned grid, size_t l
 }
 
 template <int WV, int NT, int ITEMS>
-static void launch_sortchunk_g_t(hj_ctx_t *ctx, unsigned grid, size_t lds, const hj_seg *segs2, const uint32_t *cmap2,
-                                 const hj_seg *segs1, const uint64_t *hist, const uint16_t *beg_t, const uint32_t *first,
-                                 const uint32_t *order, uint32_t b1, uint32_t bits, const void *in, void *out,
-                                 uint16_t *run, uint64_t *cst) {
+static void launch_sortchunk_g_t(hj_ctx_t *ctx, unsigned grid, size_t lds, const hj_gtile *gt, const uint64_t *hist,
+                                 const uint16_t *beg_t, uint32_t b1, uint32_t bits, const void *in, void *out,
+                                 uint16_t *run, uint64_t *cst, uint32_t tileA) {
 	auto kfn = k_sortchunk_g<WV, NT, ITEMS>;
 	allow_lds(ctx, (const void *)kfn);
 	// all resident workgroups, a multiple of the 8 XCDs (the kernel's mapping)
 	grid = std::max(8u, resident_grid(ctx, (const void *)kfn, NT, lds) & ~7u);
-	hipLaunchKernelGGL(kfn, dim3(grid), dim3(NT), lds, ctx->stream, segs2, cmap2, segs1, hist, beg_t, first, order,
-	                   (const hj_stats *)ctx->stats.p, b1, bits, (const uint4 *)in, (uint4 *)out, run, cst,
-	                   ctx->narrow_ok, (const uint8_t *)ctx->tile_narrow);
+	hipLaunchKernelGGL(kfn, dim3(grid), dim3(NT), lds, ctx->stream, gt, hist, beg_t, (const hj_stats *)ctx->stats.p,
+	                   b1, bits, (const uint4 *)in, (uint4 *)out, run, cst,
+	                   ctx->narrow_ok, (const uint8_t *)ctx->tile_narrow, tileA);
 }
 
 // LDS of k_sortchunk_g: tile, two source-offset buffers, counters
@@ -1495,14 +1602,12 @@ static size_t gathersort_lds(const part_cfg &c, uint32_t wv, uint32_t F) {
 	return (size_t)c.tile * wv * 16 + (size_t)c.tile * 8 + (2 * (size_t)F + 1) * 4 + 64 * 4;
 }
 
-static void launch_sortchunk_g(hj_ctx_t *ctx, uint32_t wv, unsigned grid, size_t lds, const hj_seg *segs2,
-                               const uint32_t *cmap2, const hj_seg *segs1, const uint64_t *hist, const uint16_t *beg_t,
-                               const uint32_t *first, const uint32_t *order, uint32_t b1, uint32_t bits, const void *in,
-                               void *out, uint16_t *run, uint64_t *cst) {
+static void launch_sortchunk_g(hj_ctx_t *ctx, uint32_t wv, unsigned grid, size_t lds, const hj_gtile *gt,
+                               const uint64_t *hist, const uint16_t *beg_t, uint32_t b1, uint32_t bits, const void *in,
+                               void *out, uint16_t *run, uint64_t *cst, uint32_t tileA) {
 	// shapes must match cfg_for(wv, 0)
 #define G(W, N, I) \
-	launch_sortchunk_g_t<W, N, I>(ctx, grid, lds, segs2, cmap2, segs1, hist, beg_t, first, order, b1, bits, in, out, run, \
-	                              cst)
+	launch_sortchunk_g_t<W, N, I>(ctx, grid, lds, gt, hist, beg_t, b1, bits, in, out, run, cst, tileA)
 	switch (wv) {
 	case 1: G(1, 512, 4); break;
 	case 2: G(2, 512, 2); break;
@@ -1662,11 +1767,12 @@ static int radix_engine(hj_ctx_t *ctx, const hj_relation_t *R, const hj_relation
 		B = std::max(B, 2u);
 		P = 2;
 	}
-	// two passes: by default the first takes the larger half of the bits, so
-	// the chunk-local second pass sorts each tile into at most as many runs
-	// as pass 1; the tuner or the caller (NUM_RADIX_BITS_PASS1) may choose
+	// two passes: by default the first takes one bit more than the larger
+	// half (its 4096-tuple tiles keep runs of 16 tuples for pass B to gather
+	// and the second pass's 2048-tuple tiles then leave longer runs for the
+	// join); the tuner or the caller (NUM_RADIX_BITS_PASS1) may choose
 	// another split
-	uint32_t b1 = P == 2 ? (B + 1) / 2 : (P ? B : 0);
+	uint32_t b1 = P == 2 ? std::min(std::min((B + 1) / 2 + 1, B - 1), 12u) : (P ? B : 0);
 	if (P == 2 && tuned_b1 && tuned_b1 < B) b1 = tuned_b1;
 	if (P == 2 && (prm->part_variant >> 8 & 15) && (prm->part_variant >> 8 & 15) < B)
 		b1 = prm->part_variant >> 8 & 15;  // first-pass width forced for tuning sweeps
@@ -1696,10 +1802,11 @@ static int radix_engine(hj_ctx_t *ctx, const hj_relation_t *R, const hj_relation
 		if (ch < (uint64_t)pc.tile) ch = pc.tile;
 		return ch;
 	};
-	const part_cfg pc2 = cfg_for(wv, 0);  // pass-A / pass-B tile = one 32 KB sort tile
+	const part_cfg pc2 = cfg_for(wv, 0);  // pass-B tile = one 32 KB sort tile
+	const part_cfg pcA = cfg_passA(wv, variant, b1, in_wv);
 	// one pass: histogram + scatter over large chunks; two passes: pass A
 	// sorts sort-tile-sized chunks in place
-	const uint64_t ch1 = P == 2 ? (uint64_t)pc2.tile : chunk_for(N), ch2 = pc2.tile;
+	const uint64_t ch1 = P == 2 ? (uint64_t)pcA.tile : chunk_for(N), ch2 = pc2.tile;
 	const uint64_t chunks1 = (nr + ch1 - 1) / ch1 + (ns + ch1 - 1) / ch1;
 	const uint32_t nseg2 = 2 * F1;
 	const uint64_t chunks2_max = N / ch2 + nseg2 + 1;
@@ -1729,9 +1836,8 @@ static int radix_engine(hj_ctx_t *ctx, const hj_relation_t *R, const hj_relation
 	const size_t o_cst = take(P == 2 ? chunks2_max * 8 : 8);
 	const size_t o_runA = take(P == 2 ? chunks1 * (F1 + 1) * 2 : 8), o_cstA = take(P == 2 ? chunks1 * 8 : 8);
 	const size_t o_begt = take(P == 2 ? (hist1_len + 1) * 2 : 8);
-	const size_t o_first = take(P == 2 ? chunks2_max * 4 : 8);
+	const size_t o_gt = take(P == 2 ? chunks2_max * sizeof(hj_gtile) : 8);
 	const size_t o_runt = take(P == 2 ? chunks2_max * (F2 + 1) * 2 : 8);
-	const size_t o_order = take(P == 2 ? chunks2_max * 4 : 8);
 	const size_t o_tnar = take(P == 2 ? chunks1 : 8);
 	if ((rc = hj_reserve(ctx, ctx->meta, off))) return rc;
 	char *meta = (char *)ctx->meta.p;
@@ -1742,9 +1848,8 @@ static int radix_engine(hj_ctx_t *ctx, const hj_relation_t *R, const hj_relation
 	hj_pair *pairs = (hj_pair *)(meta + o_pairs);
 	uint16_t *runs = (uint16_t *)(meta + o_run), *runsA = (uint16_t *)(meta + o_runA);
 	uint64_t *cst = (uint64_t *)(meta + o_cst), *cstA = (uint64_t *)(meta + o_cstA);
-	uint32_t *first = (uint32_t *)(meta + o_first);
+	hj_gtile *gt = (hj_gtile *)(meta + o_gt);
 	uint16_t *beg_t = (uint16_t *)(meta + o_begt), *run_t = (uint16_t *)(meta + o_runt);
-	uint32_t *order = (uint32_t *)(meta + o_order);
 	ctx->tile_narrow = (uint8_t *)(meta + o_tnar);
 	uint64_t *hist = (uint64_t *)ctx->hist.p;
 
@@ -1752,7 +1857,7 @@ static int radix_engine(hj_ctx_t *ctx, const hj_relation_t *R, const hj_relation
 	hj_stats_reset(ctx);
 	const void *Rb = R->tuples, *Sb = S->tuples;
 	const uint64_t *pbF = nullptr;
-	const size_t sldsA = (size_t)pc2.tile * wv * 16 + (2 * (size_t)F1 + 1) * 4 + 64 * 4;
+	const size_t sldsA = (size_t)pcA.tile * wv * 16 + (2 * (size_t)F1 + 1) * 4 + 64 * 4;
 	const size_t sldsB = gathersort_lds(pc2, wv, F2);
 	if (P >= 1) {
 		hipLaunchKernelGGL(k_seg_pair, dim3(std::max<unsigned>(1, (unsigned)((chunks1 + 255) / 256))), dim3(256), 0,
@@ -1769,7 +1874,7 @@ static int radix_engine(hj_ctx_t *ctx, const hj_relation_t *R, const hj_relation
 				return hj_fail(ctx, HJ_EINVAL, "radix bits per pass too large for LDS");
 			const unsigned gA = (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(chunks1, (uint64_t)ctx->num_cu * 4));
 			hj_kt_begin(ctx, HJ_K_TILESORT);
-			launch_sortchunk(ctx, wv, gA, sldsA, segs1, cmap, 0, b1, ctx->part[0].p, runsA, cstA);
+			launch_sortchunk(ctx, wv, pcA, gA, sldsA, segs1, cmap, 0, b1, ctx->part[0].p, runsA, cstA);
 			hj_kt_end(ctx);
 			hj_kt_begin(ctx, HJ_K_SCAN);
 			hipLaunchKernelGGL(k_runs_to_hist, dim3((unsigned)std::max<uint64_t>(1, (chunks1 + 63) / 64), (F1 + 63) / 64),
@@ -1790,13 +1895,13 @@ static int radix_engine(hj_ctx_t *ctx, const hj_relation_t *R, const hj_relation
 			hipLaunchKernelGGL(k_seg2_fin, dim3(std::min<uint32_t>(nseg2, 4096)), dim3(256), 0, ctx->stream, nch,
 			                   nseg2, F2, segs2, cmap, st);
 			hipLaunchKernelGGL(k_first_run, dim3((unsigned)std::min<uint64_t>(4096, (chunks2_max + 255) / 256)),
-			                   dim3(256), 0, ctx->stream, segs1, segs2, cmap, st, b1, (uint32_t)ch2, hist, first, order);
+			                   dim3(256), 0, ctx->stream, segs1, segs2, cmap, st, b1, (uint32_t)ch2, hist, gt);
 			// pass B: each first-level partition's tiles gathered from the
 			// pass-A runs and sorted by the second digit
 			const unsigned sgrid = (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(chunks2_max, (uint64_t)ctx->num_cu * 4));
 			hj_kt_begin(ctx, HJ_K_GATHERSORT);
-			launch_sortchunk_g(ctx, wv, sgrid, sldsB, segs2, cmap, segs1, hist, beg_t, first, order, b1, b2,
-			                   ctx->part[0].p, ctx->part[1].p, runs, cst);
+			launch_sortchunk_g(ctx, wv, sgrid, sldsB, gt, hist, beg_t, b1, b2, ctx->part[0].p, ctx->part[1].p, runs, cst,
+			                   (uint32_t)pcA.tile);
 			hj_kt_end(ctx);
 			hipLaunchKernelGGL(k_run_transpose, dim3((unsigned)((chunks2_max + 63) / 64), (F2 + 1 + 63) / 64),
 			                   dim3(256), 0, ctx->stream, runs, st, F2, (uint32_t)chunks2_max, run_t);

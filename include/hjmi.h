@@ -64,7 +64,10 @@ typedef struct {
 	                            for 16-byte tuples (0 default, 1..5 fixed);
 	                            bits 4-6 scatter write-combining line:
 	                            0 auto, k = 2^(k-1) tuples; bits 8-11 the
-	                            first-pass width of two passes (0 auto)       */
+	                            first-pass width of two passes (0 auto);
+	                            bits 12-13 the first-pass tile of two passes
+	                            (0 auto, 1: 2048, 2: 4096 by 1024 threads,
+	                            3: 4096 by 512 threads)                       */
 	uint32_t radix_bits_pass1; /* first-pass bits of two passes
 	                            (NUM_RADIX_BITS_PASS1, phj_asymm_radix.h:10-22);
 	                            0 = tuned / half of radix_bits                */
@@ -174,7 +177,7 @@ int hj_batch_destroy(hj_batch_t *batch);
  * radix_bits = 0 then use it.  hj_tuned_config reports the choice for |R|. */
 int hj_autotune(hj_ctx_t *ctx, uint32_t width, uint32_t s_ratio, uint32_t min_log2, uint32_t max_log2);
 int hj_tuned_config(hj_ctx_t *ctx, uint64_t r_num, uint32_t *radix_bits, uint32_t *passes);
-/* the tuned first-pass width of a two-pass class (0: the even split) */
+/* the tuned first-pass width of a two-pass class (0: the default split) */
 int hj_tuned_pass1_bits(hj_ctx_t *ctx, uint64_t r_num, uint32_t *radix_bits_pass1);
 
 /* ---- serving: continuous batching of arriving joins (no reference

@@ -121,7 +121,8 @@ def test_benchmark_config_c2(ctx):
     want = _s_closed_form(nr, ns)
     R, S = ctx.generate("pkfk", nr, ns, 16)
     for algo in ("phj_radix_bc", "phj_radix_lp", "nphj_lp", "nphj_sc", "phj_radix_sc", "phj_radix_hr", "phj_radix_hro"):
-        _check(ctx.join(algo, R, S), want["matched_cnt"], want["check_sum"], algo)
+        for rep in range(8 if algo == "phj_radix_bc" else 1):  # repeated: see the C3 test
+            _check(ctx.join(algo, R, S), want["matched_cnt"], want["check_sum"], (algo, rep))
     for bits, passes in ((10, 1), (11, 1), (12, 1), (12, 2), (14, 2), (16, 2)):
         _check(ctx.join("phj_radix_bc", R, S, radix_bits=bits, passes=passes), want["matched_cnt"],
                want["check_sum"], (bits, passes))
@@ -138,6 +139,9 @@ def test_benchmark_config_c3_zipf_wide(ctx, theta):
     del hR, hS
     R, S = ctx.generate("zipf", nr, ns, 64, theta=theta)
     for algo in ("phj_radix_bc", "phj_radix_lp", "nphj_sc", "nphj_lp"):
-        _check(ctx.join(algo, R, S), want["matched_cnt"], want["check_sum"], (theta, algo))
+        # the two-pass radix joins run several times: a race between the
+        # tiles of a persistent workgroup shows as a result that changes
+        for rep in range(8 if algo.startswith("phj_radix") else 1):
+            _check(ctx.join(algo, R, S), want["matched_cnt"], want["check_sum"], (theta, algo, rep))
     ctx.free_relation(R)
     ctx.free_relation(S)
