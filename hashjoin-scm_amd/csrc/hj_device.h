@@ -64,10 +64,14 @@ __device__ __forceinline__ void hj_store_stream8(uint2 *p, uint32_t a, uint32_t 
 	hj_u32x2 t = {a, b};
 	__builtin_nontemporal_store(t, (HJ_GLOBAL hj_u32x2 *)p);
 }
-// Plain (cached) global load.
+// Plain (cached) global loads.
 __device__ __forceinline__ uint4 hj_load(const uint4 *p) {
 	const hj_u32x4 v = *(const HJ_GLOBAL hj_u32x4 *)p;
 	return make_uint4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ uint2 hj_load8(const uint2 *p) {
+	const hj_u32x2 v = *(const HJ_GLOBAL hj_u32x2 *)p;
+	return make_uint2(v.x, v.y);
 }
 
 // Match sinks of the probe loops.  hj_sum_sink counts matches and adds the

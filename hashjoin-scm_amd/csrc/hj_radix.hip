@@ -806,7 +806,12 @@ __global__ void k_first_run(const hj_seg *__restrict__ segs1, const hj_seg *__re
 // over the pass-A tiles of one segment (row[i+1] - row[i] = tile i's run
 // length), brow = where those runs start inside their tiles, and pass-A tile
 // i starts at tuple i*tileA of its segment.
-template <int WV, int NT, int ITEMS>
+//
+// NIN: every pass-A tile was stored narrow (no tile was wide), so every
+// source is one 8-byte tuple: 8-byte loads and an 8-byte LDS tile.  Both
+// instances are launched for 16-byte tuples (which case holds is known on
+// the device only) and the one that does not match returns at once.
+template <int WV, int NT, int ITEMS, bool NIN>
 __global__ __launch_bounds__(NT) void k_sortchunk_g(
     const hj_gtile *__restrict__ gt, const uint64_t *__restrict__ hist, const uint16_t *__restrict__ beg_t,
     const hj_stats *__restrict__ st, uint32_t b1, uint32_t bits,
@@ -819,7 +824,9 @@ __global__ __launch_bounds__(NT) void k_sortchunk_g(
 	// wider one) leave as 8-byte {key, row_id}: half the writes here and
 	// half the reads of the join
 	const bool narrow = WV == 1 && narrow_ok && !(st->flags & HJ_DF_WIDE);
-	uint4 *tile = lds_u4;
+	if (NIN != narrow) return;
+	using tv = typename std::conditional<NIN, uint2, uint4>::type;
+	tv *tile = (tv *)lds_u4;
 	uint32_t *src0 = (uint32_t *)(tile + TILE * WV), *src1 = src0 + TILE;
 	uint32_t *tcnt = src1 + TILE;
 	uint32_t *tbase = tcnt + F;
@@ -882,7 +889,7 @@ __global__ __launch_bounds__(NT) void k_sortchunk_g(
 	// (pass-A tile stored narrow, upper half of the loaded pair) and v the
 	// raw 16-byte load; unpack() picks the tuple out when the tile is sorted,
 	// since picking it here would wait for every load before the next issues.
-	auto load = [&](uint4 (&v)[ITEMS][WV], uint32_t &sel, const hj_gtile &t, const uint32_t *srcb) {
+	auto load = [&](tv (&v)[ITEMS][WV], uint32_t &sel, const hj_gtile &t, const uint32_t *srcb) {
 		const uint32_t n = t.n;
 		// source offsets are relative to the pass-A segment's first tuple
 		const uint4 *base = in + t.base * WV;
@@ -893,7 +900,10 @@ __global__ __launch_bounds__(NT) void k_sortchunk_g(
 			const uint32_t e = n ? srcb[f] : 0;
 			// cached loads: a run's neighbours belong to tiles of nearby digits
 			// that the same XCD works on at the same time
-			if (WV == 1) {
+			if constexpr (NIN) {
+				const uint64_t sp = e & 0x7fffffffu;
+				v[k][0] = hj_load8((const uint2 *)base + sp + (sp & ~(uint64_t)(tileA - 1)));
+			} else if (WV == 1) {
 				// an 8-byte tuple at offset o of pass-A tile t (tileA tuples, a
 				// power of two) lies at 8-byte index t*2*tileA + o = sp +
 				// t*tileA; the 16-byte load of its pair keeps one load per tuple
@@ -910,16 +920,17 @@ __global__ __launch_bounds__(NT) void k_sortchunk_g(
 			}
 		}
 	};
-	auto unpack = [&](uint4 (&v)[ITEMS][WV], uint32_t sel) {
-		if (WV != 1) return;
+	auto unpack = [&](tv (&v)[ITEMS][WV], uint32_t sel) {
+		if constexpr (!NIN && WV == 1) {
 #pragma unroll
-		for (int k = 0; k < ITEMS; k++) {
-			const uint4 h = v[k][0];
-			const uint32_t b = sel >> (2 * k);
-			if (b & 1) v[k][0] = (b & 2) ? make_uint4(h.z, 0u, h.w, 0u) : make_uint4(h.x, 0u, h.y, 0u);
+			for (int k = 0; k < ITEMS; k++) {
+				const uint4 h = v[k][0];
+				const uint32_t b = sel >> (2 * k);
+				if (b & 1) v[k][0] = (b & 2) ? make_uint4(h.z, 0u, h.w, 0u) : make_uint4(h.x, 0u, h.y, 0u);
+			}
 		}
 	};
-	auto sort_store = [&](uint4 (&v)[ITEMS][WV], uint32_t sel, const hj_gtile &t) {
+	auto sort_store = [&](tv (&v)[ITEMS][WV], uint32_t sel, const hj_gtile &t) {
 		const uint64_t pos = t.pos;
 		const uint32_t n = t.n, c = t.c;
 		unpack(v, sel);
@@ -928,7 +939,10 @@ __global__ __launch_bounds__(NT) void k_sortchunk_g(
 		for (int k = 0; k < ITEMS; k++) {
 			const uint32_t idx = k * NT + threadIdx.x;
 			if (idx < n) {
-				dig[k] = (uint32_t)(hj_key_of(v[k][0]) >> b1) & fmask;
+				uint64_t key;
+				if constexpr (NIN) key = v[k][0].x;
+				else key = hj_key_of(v[k][0]);
+				dig[k] = (uint32_t)(key >> b1) & fmask;
 				rank[k] = atomicAdd(&tcnt[dig[k]], 1u);
 			}
 		}
@@ -984,10 +998,17 @@ __global__ __launch_bounds__(NT) void k_sortchunk_g(
 			}
 		}
 		__syncthreads();
-		if (narrow) {
+		if constexpr (NIN) {
 			uint2 *dst8 = (uint2 *)out + pos;
 #pragma unroll
 			for (int k = 0; k < ITEMS; k++) {  // fixed trip count: no loop to wait before
+				const uint32_t u = k * NT + threadIdx.x;
+				if (u < n) hj_store_stream8(&dst8[u], tile[u].x, tile[u].y);
+			}
+		} else if (narrow) {
+			uint2 *dst8 = (uint2 *)out + pos;
+#pragma unroll
+			for (int k = 0; k < ITEMS; k++) {
 				const uint32_t u = k * NT + threadIdx.x;
 				if (u < n) hj_store_stream8(&dst8[u], tile[u].x, tile[u].z);
 			}
@@ -1025,7 +1046,7 @@ __global__ __launch_bounds__(NT) void k_sortchunk_g(
 	// step earlier, the sort for a load issued a half step earlier, and
 	// neither for the loads issued after them.  Records are read two half
 	// steps before their prefetch.
-	uint4 va[ITEMS][WV], vb[ITEMS][WV];
+	tv va[ITEMS][WV], vb[ITEMS][WV];
 	hj_gtile ta = tile_at(0), tb = tile_at(1), tc = tile_at(2), td = tile_at(3);
 	uint32_t sa, sb;
 	stage(prefetch(ta), ta, src0);
@@ -1588,9 +1609,20 @@ template <int WV, int NT, int ITEMS>
 static void launch_sortchunk_g_t(hj_ctx_t *ctx, unsigned grid, size_t lds, const hj_gtile *gt, const uint64_t *hist,
                                  const uint16_t *beg_t, uint32_t b1, uint32_t bits, const void *in, void *out,
                                  uint16_t *run, uint64_t *cst, uint32_t tileA) {
-	auto kfn = k_sortchunk_g<WV, NT, ITEMS>;
+	// grid: all resident workgroups, a multiple of the 8 XCDs (the kernel's
+	// mapping)
+	if constexpr (WV == 1) if (ctx->narrow_ok) {
+		// all-narrow instance: 8-byte LDS tile (the other returns at once)
+		auto kn = k_sortchunk_g<WV, NT, ITEMS, true>;
+		const size_t ln = lds - (size_t)NT * ITEMS * 8;
+		allow_lds(ctx, (const void *)kn);
+		const unsigned gn = std::max(8u, resident_grid(ctx, (const void *)kn, NT, ln) & ~7u);
+		hipLaunchKernelGGL(kn, dim3(gn), dim3(NT), ln, ctx->stream, gt, hist, beg_t, (const hj_stats *)ctx->stats.p,
+		                   b1, bits, (const uint4 *)in, (uint4 *)out, run, cst, ctx->narrow_ok,
+		                   (const uint8_t *)ctx->tile_narrow, tileA);
+	}
+	auto kfn = k_sortchunk_g<WV, NT, ITEMS, false>;
 	allow_lds(ctx, (const void *)kfn);
-	// all resident workgroups, a multiple of the 8 XCDs (the kernel's mapping)
 	grid = std::max(8u, resident_grid(ctx, (const void *)kfn, NT, lds) & ~7u);
 	hipLaunchKernelGGL(kfn, dim3(grid), dim3(NT), lds, ctx->stream, gt, hist, beg_t, (const hj_stats *)ctx->stats.p,
 	                   b1, bits, (const uint4 *)in, (uint4 *)out, run, cst,
