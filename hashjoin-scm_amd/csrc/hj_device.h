@@ -20,6 +20,9 @@ __device__ __forceinline__ uint64_t hj_key_of(const uint4 &w0) {
 __device__ __forceinline__ uint64_t hj_pay_of(const uint4 &w0) {
 	return (uint64_t)w0.z | ((uint64_t)w0.w << 32);
 }
+// 8-byte narrow tuples {u32 key, u32 row_id}
+__device__ __forceinline__ uint64_t hj_key_of(const uint2 &t) { return t.x; }
+__device__ __forceinline__ uint64_t hj_pay_of(const uint2 &t) { return t.y; }
 
 // Plain (cached) 16-byte load of word 0 of tuple i.
 template <int WV>

@@ -235,9 +235,12 @@ __global__ void k_seg2_fin(const uint64_t *__restrict__ nch_scan, uint32_t nseg2
 }
 
 // ------------------------------------------------------------ histogram
-template <int WV>
+// DETECT (16-byte tuples whose pass may write 8-byte ones): also flag
+// HJ_DF_WIDE when a key or row id of the chunk needs more than 32 bits; the
+// loads are then whole tuples, which read the same lines.
+template <int WV, bool DETECT>
 __global__ __launch_bounds__(512) void k_hist(const hj_seg *__restrict__ segs, const uint32_t *__restrict__ chunk_seg,
-                                              const hj_stats *__restrict__ st, uint64_t ch, uint32_t shift,
+                                              hj_stats *__restrict__ st, uint64_t ch, uint32_t shift,
                                               uint32_t bits, uint64_t *__restrict__ hist) {
 	extern __shared__ uint32_t cnt[];
 	const uint32_t c = blockIdx.x;
@@ -250,12 +253,24 @@ __global__ __launch_bounds__(512) void k_hist(const hj_seg *__restrict__ segs, c
 	__syncthreads();
 	const uint64_t *keys = (const uint64_t *)g.in;
 	constexpr int U = 8;
+	bool wide = false;
 	for (uint64_t i0 = b + threadIdx.x; i0 < e; i0 += (uint64_t)U * blockDim.x) {
 		uint64_t k[U];
+		if constexpr (DETECT) {
+			uint4 t[U];
 #pragma unroll
-		for (int u = 0; u < U; u++) {
-			const uint64_t i = i0 + (uint64_t)u * blockDim.x;
-			k[u] = i < e ? __builtin_nontemporal_load(hj_g(&keys[i * 2 * WV])) : 0;
+			for (int u = 0; u < U; u++) t[u] = hj_load_stream(g.in + min(i0 + (uint64_t)u * blockDim.x, e - 1));
+#pragma unroll
+			for (int u = 0; u < U; u++) {
+				k[u] = hj_key_of(t[u]);
+				wide |= (t[u].y | t[u].w) != 0;  // past the end: the last tuple again
+			}
+		} else {
+#pragma unroll
+			for (int u = 0; u < U; u++) {
+				const uint64_t i = i0 + (uint64_t)u * blockDim.x;
+				k[u] = i < e ? __builtin_nontemporal_load(hj_g(&keys[i * 2 * WV])) : 0;
+			}
 		}
 #pragma unroll
 		for (int u = 0; u < U; u++) {
@@ -263,6 +278,7 @@ __global__ __launch_bounds__(512) void k_hist(const hj_seg *__restrict__ segs, c
 			if (i < e) atomicAdd(&cnt[(uint32_t)(k[u] >> shift) & fmask], 1u);
 		}
 	}
+	if (DETECT && __syncthreads_or(wide) && threadIdx.x == 0) atomicOr(&st->flags, HJ_DF_WIDE);
 	__syncthreads();
 	for (uint32_t p = threadIdx.x; p < F; p += blockDim.x) hist[g.hist_base + (uint64_t)p * g.nchunks + cl] = cnt[p];
 }
@@ -279,18 +295,24 @@ __global__ __launch_bounds__(512) void k_hist(const hj_seg *__restrict__ segs, c
 //      tcnt, tbase, cf [FL] u32 | pid[XT] u16 | scan scratch,
 //      XT = TILE + FL*(lt-1) (the tile plus every digit's carry),
 //      cf = carried tuples << 16 | tuples stored this tile
-template <int WV, int NT, int ITEMS>
+// NOUT: the pass writes 8-byte {key, row_id} tuples (16-byte input whose
+// keys and row ids all fit 32 bits: narrow_ok and the histogram pass found no
+// wider one), staged and carried as 8 bytes, at 8-byte positions of `out`.
+// Both instances are launched; the one that does not match returns at once.
+template <int WV, int NT, int ITEMS, bool NOUT>
 __global__ __launch_bounds__(NT) void k_scatter(
     const hj_seg *__restrict__ segs, const uint32_t *__restrict__ chunk_seg, const hj_stats *__restrict__ st,
     uint64_t ch, uint32_t shift, uint32_t bits, uint32_t lt, const uint64_t *__restrict__ scan,
-    uint4 *__restrict__ out) {
+    uint4 *__restrict__ out, bool narrow_ok) {
 	constexpr int TILE = NT * ITEMS;
 	extern __shared__ uint4 lds_u4[];
+	if (NOUT != (WV == 1 && narrow_ok && !(st->flags & HJ_DF_WIDE))) return;
+	using tv = typename std::conditional<NOUT, uint2, uint4>::type;
 	const uint32_t FL = 1u << bits;  // launch maximum: LDS layout
 	const uint32_t CW = lt - 1;      // carry slots per digit
 	const uint32_t XT = TILE + FL * CW;
-	uint4 *tile = lds_u4;
-	uint4 *carry = tile + (size_t)XT * WV;
+	tv *tile = (tv *)lds_u4;
+	tv *carry = tile + (size_t)XT * WV;
 	uint64_t *cursor = (uint64_t *)(carry + (size_t)FL * CW * WV);
 	uint32_t *tcnt = (uint32_t *)(cursor + FL);
 	uint32_t *tbase = tcnt + FL;
@@ -373,17 +395,25 @@ __global__ __launch_bounds__(NT) void k_scatter(
 			const uint32_t idx = k * NT + threadIdx.x;
 			if (idx < n) {
 				const uint32_t q = tbase[dig[k]] + (cf[dig[k]] >> 16) + rank[k];
+				if constexpr (NOUT) {
+					tile[q] = make_uint2(v[k][0].x, v[k][0].z);
+				} else {
 #pragma unroll
-				for (int w = 0; w < WV; w++) tile[q * WV + w] = v[k][w];
+					for (int w = 0; w < WV; w++) tile[q * WV + w] = v[k][w];
+				}
 				pid[q] = (uint16_t)dig[k];
 			}
 		}
 		__syncthreads();
-		// consecutive lanes store consecutive 16-byte words of a run
+		// consecutive lanes store consecutive words of a run
 		for (uint32_t u = threadIdx.x; u < tot * WV; u += NT) {
 			const uint32_t i = u / WV, w = u % WV, p = pid[i], j = i - tbase[p], fl = cf[p] & 0xffffu;
-			if (j < fl) hj_store_stream(&out[(cursor[p] + j) * WV + w], tile[u]);
-			else carry[(p * CW + j - fl) * WV + w] = tile[u];
+			if (j < fl) {
+				if constexpr (NOUT) hj_store_stream8((uint2 *)out + cursor[p] + j, tile[u].x, tile[u].y);
+				else hj_store_stream(&out[(cursor[p] + j) * WV + w], tile[u]);
+			} else {
+				carry[(p * CW + j - fl) * WV + w] = tile[u];
+			}
 		}
 		load_tile(v, t0 + 2 * TILE);
 		__syncthreads();
@@ -404,7 +434,10 @@ __global__ __launch_bounds__(NT) void k_scatter(
 	// the chunk's last partial lines
 	for (uint32_t x = threadIdx.x; x < F * CW * WV; x += NT) {
 		const uint32_t s = x / WV, w = x % WV, p = s / CW, j = s - p * CW;
-		if (j < (cf[p] >> 16)) hj_store_stream(&out[(cursor[p] + j) * WV + w], carry[x]);
+		if (j < (cf[p] >> 16)) {
+			if constexpr (NOUT) hj_store_stream8((uint2 *)out + cursor[p] + j, carry[x].x, carry[x].y);
+			else hj_store_stream(&out[(cursor[p] + j) * WV + w], carry[x]);
+		}
 	}
 }
 
@@ -531,20 +564,20 @@ __global__ __launch_bounds__(NT) void k_join(const uint4 *__restrict__ Rb, const
 				}
 				tab.template probe<U>(sk, sp, live, B, acc);
 			};
-			uint64_t j0 = sb + threadIdx.x;
-			if (j0 < se) {
+			// the same number of steps in every lane, loads clamped and never
+			// under a lane condition, so each probe waits only for its own
+			// buffer while the other one's loads are in flight
+			if (sb < se) {
+				const uint64_t step = (uint64_t)U * NT;
+				const uint64_t nsteps = (se - sb + step - 1) / step;
 				uint4 ta[U], tb[U];
-				load(ta, j0);
-				for (;;) {
-					const uint64_t j1 = j0 + (uint64_t)U * NT;
-					if (j1 < se) load(tb, j1);
+				load(ta, sb + threadIdx.x);
+				for (uint64_t k = 0; k < nsteps; k += 2) {
+					const uint64_t j0 = sb + k * step + threadIdx.x;
+					load(tb, j0 + step);
 					probe(ta, j0);
-					if (j1 >= se) break;
-					const uint64_t j2 = j1 + (uint64_t)U * NT;
-					if (j2 < se) load(ta, j2);
-					probe(tb, j1);
-					if (j2 >= se) break;
-					j0 = j2;
+					load(ta, j0 + 2 * step);
+					probe(tb, j0 + step);  // past the end: every lane skips
 				}
 			}
 			__syncthreads();
@@ -822,7 +855,8 @@ __global__ __launch_bounds__(NT) void k_sortchunk_g(
 	const uint32_t F = 1u << bits, fmask = F - 1;
 	// 16-byte tuples whose keys and row ids all fit 32 bits (pass A found no
 	// wider one) leave as 8-byte {key, row_id}: half the writes here and
-	// half the reads of the join
+	// half the reads of the join.  Then every pass-A tile is narrow too, and
+	// the NIN instance does the work.
 	const bool narrow = WV == 1 && narrow_ok && !(st->flags & HJ_DF_WIDE);
 	if (NIN != narrow) return;
 	using tv = typename std::conditional<NIN, uint2, uint4>::type;
@@ -998,19 +1032,14 @@ __global__ __launch_bounds__(NT) void k_sortchunk_g(
 			}
 		}
 		__syncthreads();
+		// (the other instance returns when the output is narrow, so only this
+		// one writes 8-byte tuples)
 		if constexpr (NIN) {
 			uint2 *dst8 = (uint2 *)out + pos;
 #pragma unroll
 			for (int k = 0; k < ITEMS; k++) {  // fixed trip count: no loop to wait before
 				const uint32_t u = k * NT + threadIdx.x;
 				if (u < n) hj_store_stream8(&dst8[u], tile[u].x, tile[u].y);
-			}
-		} else if (narrow) {
-			uint2 *dst8 = (uint2 *)out + pos;
-#pragma unroll
-			for (int k = 0; k < ITEMS; k++) {
-				const uint32_t u = k * NT + threadIdx.x;
-				if (u < n) hj_store_stream8(&dst8[u], tile[u].x, tile[u].z);
 			}
 		} else {
 			uint4 *dst = out + pos * WV;
@@ -1356,20 +1385,27 @@ static part_cfg cfg_for(uint32_t wv, uint32_t variant = 0) {
 	return part_cfg{nt, items, nt * items};
 }
 
-static size_t scatter_lds(const part_cfg &c, uint32_t wv, uint32_t F, uint32_t lt) {
+// LDS of k_scatter staging tb-byte tuples (16 * wv, or 8 for narrow output)
+static size_t scatter_lds_b(const part_cfg &c, uint32_t tb, uint32_t F, uint32_t lt) {
 	const size_t xt = (size_t)c.tile + (size_t)F * (lt - 1);
-	return xt * wv * 16 + (size_t)F * (lt - 1) * wv * 16 + (size_t)F * 20 + ((xt + 3) & ~(size_t)1) * 2 + 64 * 4;
+	return xt * tb + (size_t)F * (lt - 1) * tb + (size_t)F * 20 + ((xt + 3) & ~(size_t)1) * 2 + 64 * 4;
+}
+static size_t scatter_lds(const part_cfg &c, uint32_t wv, uint32_t F, uint32_t lt) {
+	return scatter_lds_b(c, 16 * wv, F, lt);
 }
 
-// Write-combining line of the scatter (tuples): the largest of 128 / W, 64 / W
-// (at least one tuple) whose carries still let two workgroups share a CU;
-// 1 when neither does.
-static uint32_t scatter_lt(const part_cfg &c, uint32_t wv, uint32_t F, size_t lds_cu) {
+// Write-combining line of the scatter (tuples of tb bytes): the largest of
+// 128 / tb, 64 / tb (at least one tuple) whose carries still let two
+// workgroups share a CU; 1 when neither does.
+static uint32_t scatter_lt_b(const part_cfg &c, uint32_t tb, uint32_t F, size_t lds_cu) {
 	for (uint32_t line = 128; line >= 64; line >>= 1) {
-		const uint32_t lt = line / (16 * wv);
-		if (lt >= 2 && scatter_lds(c, wv, F, lt) + 1024 <= lds_cu / 2) return lt;
+		const uint32_t lt = line / tb;
+		if (lt >= 2 && scatter_lds_b(c, tb, F, lt) + 1024 <= lds_cu / 2) return lt;
 	}
 	return 1;
+}
+static uint32_t scatter_lt(const part_cfg &c, uint32_t wv, uint32_t F, size_t lds_cu) {
+	return scatter_lt_b(c, 16 * wv, F, lds_cu);
 }
 
 // Static LDS of a kernel (its __shared__ arrays), so that static + dynamic
@@ -1416,40 +1452,58 @@ static bool capturing(hj_ctx_t *ctx) {
 // ctx->stats, a batch uses its own.  The kernels read the chunk count there.
 template <int WV>
 static void launch_hist(hj_ctx_t *ctx, unsigned grid, const hj_seg *segs, const uint32_t *cmap, const hj_stats *st,
-                        uint64_t ch, uint32_t shift, uint32_t bits, uint64_t *hist) {
-	hipLaunchKernelGGL(k_hist<WV>, dim3(grid), dim3(512), (size_t)4 << bits, ctx->stream, segs, cmap, st, ch, shift,
-	                   bits, hist);
+                        uint64_t ch, uint32_t shift, uint32_t bits, uint64_t *hist, bool detect) {
+	if (WV == 1 && detect)
+		hipLaunchKernelGGL((k_hist<WV, true>), dim3(grid), dim3(512), (size_t)4 << bits, ctx->stream, segs, cmap,
+		                   (hj_stats *)st, ch, shift, bits, hist);
+	else
+		hipLaunchKernelGGL((k_hist<WV, false>), dim3(grid), dim3(512), (size_t)4 << bits, ctx->stream, segs, cmap,
+		                   (hj_stats *)st, ch, shift, bits, hist);
 }
 
 template <int WV, int NT, int ITEMS>
 static void launch_scatter_t(hj_ctx_t *ctx, unsigned grid, size_t lds, const hj_seg *segs, const uint32_t *cmap,
                              const hj_stats *st, uint64_t ch, uint32_t shift, uint32_t bits, uint32_t lt,
-                             const uint64_t *scan, void *out) {
-	auto kfn = k_scatter<WV, NT, ITEMS>;
+                             const uint64_t *scan, void *out, bool narrow_ok = false) {
+	auto kfn = k_scatter<WV, NT, ITEMS, false>;
 	if (!capturing(ctx)) allow_lds(ctx, (const void *)kfn);
 	hipLaunchKernelGGL(kfn, dim3(grid), dim3(NT), lds, ctx->stream, segs, cmap, st, ch, shift, bits, lt, scan,
-	                   (uint4 *)out);
+	                   (uint4 *)out, narrow_ok);
 }
 
+// Write-combining line of a partition pass: scatter_lt's choice, or the one
+// part_variant bits 4-6 force for tuning sweeps.
+static uint32_t pass_line(hj_ctx_t *ctx, const part_cfg &c, uint32_t wv, uint32_t F, uint32_t variant) {
+	if (variant & 0x70) return 1u << (((variant >> 4) & 7) - 1);
+	return scatter_lt(c, wv, F, ctx->prop.sharedMemPerBlock);
+}
+
+// Narrow output of a batch pass (16-byte tuples): k_scatter<1, 512, 4, true>
+// with 8-byte staging; its line and LDS.
+static constexpr part_cfg k_narrow_scatter{512, 4, 2048};
+
+// narrow: the pass may write 8-byte tuples (16-byte input; the histogram
+// pass decides on the device, st->flags HJ_DF_WIDE) for a consumer that
+// reads both layouts (the batch join); both scatter instances are launched.
 static int launch_pass(hj_ctx_t *ctx, uint32_t wv, uint32_t variant, unsigned grid, const hj_seg *segs,
                        const uint32_t *cmap, uint64_t ch, uint32_t shift, uint32_t bits, uint64_t *hist,
-                       uint64_t hist_len, void *out, const hj_stats *st, uint64_t *scan_tmp) {
+                       uint64_t hist_len, void *out, const hj_stats *st, uint64_t *scan_tmp, bool narrow = false) {
 	const part_cfg c = cfg_for(wv, variant);
 	const uint32_t F = 1u << bits;
-	uint32_t lt = scatter_lt(c, wv, F, ctx->prop.sharedMemPerBlock);
-	if (variant & 0x70) lt = 1u << (((variant >> 4) & 7) - 1);  // forced line for tuning sweeps
+	const uint32_t lt = pass_line(ctx, c, wv, F, variant);
 	const size_t lds = scatter_lds(c, wv, F, lt);
 	if (lds + 1024 > ctx->prop.sharedMemPerBlock)
 		return hj_fail(ctx, HJ_EINVAL, "%u radix bits in one pass need %zu B of LDS (device: %zu)", bits, lds,
 		               (size_t)ctx->prop.sharedMemPerBlock);
+	narrow = narrow && wv == 1;
 	if (grid == 0) return HJ_OK;
 	hj_kt_begin(ctx, HJ_K_HIST);
 	switch (wv) {
-	case 1: launch_hist<1>(ctx, grid, segs, cmap, st, ch, shift, bits, hist); break;
-	case 2: launch_hist<2>(ctx, grid, segs, cmap, st, ch, shift, bits, hist); break;
-	case 4: launch_hist<4>(ctx, grid, segs, cmap, st, ch, shift, bits, hist); break;
-	case 8: launch_hist<8>(ctx, grid, segs, cmap, st, ch, shift, bits, hist); break;
-	default: launch_hist<16>(ctx, grid, segs, cmap, st, ch, shift, bits, hist); break;
+	case 1: launch_hist<1>(ctx, grid, segs, cmap, st, ch, shift, bits, hist, narrow); break;
+	case 2: launch_hist<2>(ctx, grid, segs, cmap, st, ch, shift, bits, hist, false); break;
+	case 4: launch_hist<4>(ctx, grid, segs, cmap, st, ch, shift, bits, hist, false); break;
+	case 8: launch_hist<8>(ctx, grid, segs, cmap, st, ch, shift, bits, hist, false); break;
+	default: launch_hist<16>(ctx, grid, segs, cmap, st, ch, shift, bits, hist, false); break;
 	}
 	hj_kt_end(ctx);
 	hj_kt_begin(ctx, HJ_K_SCAN);
@@ -1457,15 +1511,27 @@ static int launch_pass(hj_ctx_t *ctx, uint32_t wv, uint32_t variant, unsigned gr
 	hj_kt_end(ctx);
 	if (rc) return rc;
 	hj_kt_begin(ctx, HJ_K_SCATTER);
+	if (narrow) {
+		// 8-byte instance (returns unless every tuple is narrow); tiles of
+		// 2048 tuples, which every chunk size here is a multiple of
+		const uint32_t ltn = (variant & 0x70) ? lt : scatter_lt_b(k_narrow_scatter, 8, F, ctx->prop.sharedMemPerBlock);
+		const size_t ldsn = scatter_lds_b(k_narrow_scatter, 8, F, ltn);
+		auto kn = k_scatter<1, 512, 4, true>;
+		if (!capturing(ctx)) allow_lds(ctx, (const void *)kn);
+		hipLaunchKernelGGL(kn, dim3(grid), dim3(512), ldsn, ctx->stream, segs, cmap, st, ch, shift, bits, ltn, hist,
+		                   (uint4 *)out, true);
+	}
 	switch (wv) {
 	case 1:
 		switch ((variant & 15) < 6 ? (variant & 15) : 0) {
-		case 2: launch_scatter_t<1, 512, 8>(ctx, grid, lds, segs, cmap, st, ch, shift, bits, lt, hist, out); break;
-		case 3: launch_scatter_t<1, 256, 8>(ctx, grid, lds, segs, cmap, st, ch, shift, bits, lt, hist, out); break;
-		case 4: launch_scatter_t<1, 1024, 2>(ctx, grid, lds, segs, cmap, st, ch, shift, bits, lt, hist, out); break;
-		case 5: launch_scatter_t<1, 256, 16>(ctx, grid, lds, segs, cmap, st, ch, shift, bits, lt, hist, out); break;
-		case 1: launch_scatter_t<1, 1024, 4>(ctx, grid, lds, segs, cmap, st, ch, shift, bits, lt, hist, out); break;
-		default: launch_scatter_t<1, 512, 4>(ctx, grid, lds, segs, cmap, st, ch, shift, bits, lt, hist, out); break;
+#define S1(N, I) launch_scatter_t<1, N, I>(ctx, grid, lds, segs, cmap, st, ch, shift, bits, lt, hist, out, narrow)
+		case 2: S1(512, 8); break;
+		case 3: S1(256, 8); break;
+		case 4: S1(1024, 2); break;
+		case 5: S1(256, 16); break;
+		case 1: S1(1024, 4); break;
+		default: S1(512, 4); break;
+#undef S1
 		}
 		break;
 	case 2: launch_scatter_t<2, 512, 2>(ctx, grid, lds, segs, cmap, st, ch, shift, bits, lt, hist, out); break;
@@ -1503,11 +1569,20 @@ static size_t join_static_lds(int tk, bool two_pass) {
 static void set_scatter_attr(hj_ctx_t *ctx, uint32_t wv) {
 	const void *k = nullptr;
 	switch (wv) {
-	case 1: k = (const void *)k_scatter<1, 512, 4>; break;
-	case 2: k = (const void *)k_scatter<2, 512, 2>; break;
-	case 4: k = (const void *)k_scatter<4, 512, 1>; break;
-	case 8: k = (const void *)k_scatter<8, 256, 1>; break;
-	default: k = (const void *)k_scatter<16, 256, 1>; break;
+	case 1:
+		// every tile shape a variant may pick, and the narrow instance
+		allow_lds(ctx, (const void *)k_scatter<1, 1024, 4, false>);
+		allow_lds(ctx, (const void *)k_scatter<1, 512, 8, false>);
+		allow_lds(ctx, (const void *)k_scatter<1, 256, 8, false>);
+		allow_lds(ctx, (const void *)k_scatter<1, 1024, 2, false>);
+		allow_lds(ctx, (const void *)k_scatter<1, 256, 16, false>);
+		allow_lds(ctx, (const void *)k_scatter<1, 512, 4, true>);
+		k = (const void *)k_scatter<1, 512, 4, false>;
+		break;
+	case 2: k = (const void *)k_scatter<2, 512, 2, false>; break;
+	case 4: k = (const void *)k_scatter<4, 512, 1, false>; break;
+	case 8: k = (const void *)k_scatter<8, 256, 1, false>; break;
+	default: k = (const void *)k_scatter<16, 256, 1, false>; break;
 	}
 	allow_lds(ctx, k);
 }
@@ -2048,6 +2123,7 @@ struct hj_bpair {
 	const uint4 *rb, *sb;
 	uint64_t r0, r1, s0, s1;
 	uint32_t join, shift;
+	uint32_t rn, sn;  // build / probe side held as 8-byte tuples (narrow pass output)
 };
 
 __global__ void k_batch_reset(hj_stats *st, uint64_t nchunks, unsigned long long *res, uint32_t njoins,
@@ -2077,15 +2153,21 @@ __global__ void k_bounds_b(const hj_seg *__restrict__ segs, uint32_t nseg, const
 	}
 }
 
+// narrow_ok: the pass that wrote `part` could write 8-byte tuples; whether
+// it did is in st->flags (no HJ_DF_WIDE).  Inputs and resident partitions
+// are always 16-byte.
 __global__ void k_plan_b(const hj_bjoin *__restrict__ joins, const uint32_t *__restrict__ pair_join, uint32_t npairs,
                          const hj_seg *__restrict__ segs, const uint64_t *__restrict__ pb, const uint4 *part,
-                         hj_bpair *__restrict__ pairs, uint64_t *__restrict__ tiles) {
+                         hj_bpair *__restrict__ pairs, uint64_t *__restrict__ tiles, const hj_stats *__restrict__ st,
+                         bool narrow_ok) {
+	const uint32_t pn = narrow_ok && !(st->flags & HJ_DF_WIDE) ? 1u : 0u;
 	for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < npairs; i += gridDim.x * blockDim.x) {
 		const uint32_t j = pair_join[i];
 		const hj_bjoin J = joins[j];
 		hj_bpair pr;
 		pr.join = j;
 		pr.shift = J.bits;
+		pr.rn = pr.sn = 0;
 		if (J.bits == 0) {
 			pr.rb = J.R;
 			pr.sb = J.S;
@@ -2098,6 +2180,7 @@ __global__ void k_plan_b(const hj_bjoin *__restrict__ joins, const uint32_t *__r
 			const uint64_t sb = segs[J.seg].pb_base + q;
 			pr.rb = J.rpart;
 			pr.sb = part;
+			pr.sn = pn;
 			pr.r0 = *hj_g(&J.rpb[q]);
 			pr.r1 = *hj_g(&J.rpb[q + 1]);
 			pr.s0 = pb[sb];
@@ -2106,6 +2189,7 @@ __global__ void k_plan_b(const hj_bjoin *__restrict__ joins, const uint32_t *__r
 			const uint32_t q = i - J.pair_base;
 			const uint64_t rb = segs[J.seg].pb_base + q, sb = segs[J.seg + 1].pb_base + q;
 			pr.rb = pr.sb = part;
+			pr.rn = pr.sn = pn;
 			pr.r0 = pb[rb];
 			pr.r1 = pb[rb + 1];
 			pr.s0 = pb[sb];
@@ -2169,21 +2253,33 @@ __global__ __launch_bounds__(NT) void k_join_b(const hj_bpair *__restrict__ pair
 			for (uint32_t i = threadIdx.x; i < nb; i += NT) heads[i] = 0;
 			__syncthreads();
 			// four build tuples per thread and step, loads issued together
-			// (unconditional, clamped to the last tuple)
-			for (uint32_t i0 = threadIdx.x; i0 < m; i0 += 4 * NT) {
-				uint4 t[4];
+			// (unconditional, clamped to the last tuple); T = uint2 for a
+			// narrow side (8-byte index), uint4 otherwise (WV words a tuple)
+			auto build = [&](auto tag) {
+				using T = decltype(tag);
+				constexpr uint32_t SW = sizeof(T) == 8 ? 1 : WV;
+				const T *rb = (const T *)pr.rb;
+				for (uint32_t i0 = threadIdx.x; i0 < m; i0 += 4 * NT) {
+					T t[4];
 #pragma unroll
-				for (int u = 0; u < 4; u++) t[u] = hj_load(&pr.rb[(rc + min(i0 + u * NT, m - 1)) * WV]);
+					for (int u = 0; u < 4; u++) {
+						const T *a = &rb[(rc + min(i0 + u * NT, m - 1)) * SW];
+						if constexpr (sizeof(T) == 8) t[u] = hj_load8(a);
+						else t[u] = hj_load(a);
+					}
 #pragma unroll
-				for (int u = 0; u < 4; u++) {
-					const uint32_t i = i0 + u * NT;
-					if (i >= m) continue;
-					const uint64_t key = hj_key_of(t[u]);
-					keys[i] = key;
-					pays[i] = hj_pay_of(t[u]);
-					if (!LP) next[i] = (uint16_t)atomicExch(&heads[(uint32_t)(key >> B) & hmask], i + 1);
+					for (int u = 0; u < 4; u++) {
+						const uint32_t i = i0 + u * NT;
+						if (i >= m) continue;
+						const uint64_t key = hj_key_of(t[u]);
+						keys[i] = key;
+						pays[i] = hj_pay_of(t[u]);
+						if (!LP) next[i] = (uint16_t)atomicExch(&heads[(uint32_t)(key >> B) & hmask], i + 1);
+					}
 				}
-			}
+			};
+			if (WV == 1 && pr.rn) build(uint2{});
+			else build(uint4{});
 			__syncthreads();
 			if (LP) {
 				for (uint32_t i = threadIdx.x; i < m; i += NT) {
@@ -2205,11 +2301,19 @@ __global__ __launch_bounds__(NT) void k_join_b(const hj_bpair *__restrict__ pair
 			// issued (unconditional, clamped to the last tuple) before the
 			// current step is probed
 			constexpr int U = 4;
-			auto load = [&](uint4 (&t)[U], uint64_t j0) {
+			auto run_probe = [&](auto tag) {
+			using T = decltype(tag);
+			constexpr uint32_t SW = sizeof(T) == 8 ? 1 : WV;
+			const T *sbase = (const T *)pr.sb;
+			auto load = [&](T (&t)[U], uint64_t j0) {
 #pragma unroll
-				for (int u = 0; u < U; u++) t[u] = hj_load_stream(&pr.sb[min(j0 + (uint64_t)u * NT, se - 1) * WV]);
+				for (int u = 0; u < U; u++) {
+					const T *a = &sbase[min(j0 + (uint64_t)u * NT, se - 1) * SW];
+					if constexpr (sizeof(T) == 8) t[u] = hj_load_stream8(a);
+					else t[u] = hj_load_stream(a);
+				}
 			};
-			auto probe = [&](const uint4 (&t)[U], uint64_t j0) {
+			auto probe = [&](const T (&t)[U], uint64_t j0) {
 #pragma unroll
 				for (int u = 0; u < U; u++) {
 					if (j0 + (uint64_t)u * NT >= se) continue;
@@ -2235,22 +2339,23 @@ __global__ __launch_bounds__(NT) void k_join_b(const hj_bpair *__restrict__ pair
 					}
 				}
 			};
-			uint64_t j0 = sb + threadIdx.x;
-			if (j0 < se) {
-				uint4 ta[U], tb[U];
-				load(ta, j0);
-				for (;;) {
-					const uint64_t j1 = j0 + (uint64_t)U * NT;
-					if (j1 < se) load(tb, j1);
-					probe(ta, j0);
-					if (j1 >= se) break;
-					const uint64_t j2 = j1 + (uint64_t)U * NT;
-					if (j2 < se) load(ta, j2);
-					probe(tb, j1);
-					if (j2 >= se) break;
-					j0 = j2;
-				}
+			// the same number of steps in every lane (sb < se), loads clamped
+			// and never under a lane condition, so each probe waits only for
+			// its own buffer while the other one's loads are in flight
+			const uint64_t step = (uint64_t)U * NT;
+			const uint64_t nsteps = (se - sb + step - 1) / step;
+			T ta[U], tb[U];
+			load(ta, sb + threadIdx.x);
+			for (uint64_t k = 0; k < nsteps; k += 2) {
+				const uint64_t j0 = sb + k * step + threadIdx.x;
+				load(tb, j0 + step);
+				probe(ta, j0);
+				load(ta, j0 + 2 * step);
+				probe(tb, j0 + step);  // past the end: every lane skips
 			}
+			};
+			if (WV == 1 && pr.sn) run_probe(uint2{});
+			else run_probe(uint4{});
 			__syncthreads();
 		}
 		cnt = hj_wave_sum(cnt);
@@ -2263,12 +2368,16 @@ __global__ __launch_bounds__(NT) void k_join_b(const hj_bpair *__restrict__ pair
 	}
 }
 
+// Threads of a batch-join workgroup: its LDS table (about 64 KB for 2048
+// build tuples) lets two share a CU, so 512 threads keep 16 waves per CU.
+static constexpr int kJoinBNT = 512;
+
 template <int WV, bool LP>
 static void launch_join_b_t(hj_ctx_t *ctx, unsigned grid, size_t lds, const hj_bpair *pairs, const uint64_t *tiles,
                             uint32_t npairs, const hj_bjoin *joins, uint32_t cap, hj_stats *st, unsigned long long *res,
                             unsigned int *jflags) {
-	hipLaunchKernelGGL((k_join_b<WV, 256, LP>), dim3(grid), dim3(256), lds, ctx->stream, pairs, tiles, npairs, joins, cap,
-	                   st, res, jflags);
+	hipLaunchKernelGGL((k_join_b<WV, kJoinBNT, LP>), dim3(grid), dim3(kJoinBNT), lds, ctx->stream, pairs, tiles, npairs,
+	                   joins, cap, st, res, jflags);
 }
 
 static void launch_join_b(hj_ctx_t *ctx, uint32_t wv, bool lp, unsigned grid, size_t lds, const hj_bpair *pairs,
@@ -2315,7 +2424,7 @@ static void batch_free(hj_batch *b) {
 
 template <int WV, bool LP>
 static void *join_b_fn() {
-	return (void *)k_join_b<WV, 256, LP>;
+	return (void *)k_join_b<WV, kJoinBNT, LP>;
 }
 
 static const void *join_b_kernel(uint32_t wv, bool lp) {
@@ -2358,7 +2467,9 @@ extern "C" int hj_batch_create(hj_ctx_t *ctx, const char *algo, uint32_t njoins,
 	b->R.assign(R, R + njoins);
 	b->S.assign(S, S + njoins);
 	const uint32_t wv = b->wv;
-	const part_cfg pc = cfg_for(wv, 0);
+	// tile shape and write-combining line (part_variant bits 0-6, tuning)
+	const uint32_t variant = params ? (params->part_variant & 0x7f) : 0;
+	const part_cfg pc = cfg_for(wv, variant);
 
 	// ---- plan on the host
 	const uint32_t small_max = 2048;  // build sides up to this size are joined unpartitioned
@@ -2426,7 +2537,7 @@ extern "C" int hj_batch_create(hj_ctx_t *ctx, const char *algo, uint32_t njoins,
 	const uint64_t nchunks = cmap.size();
 	const uint32_t npairs = (uint32_t)pair_join.size();
 	const uint32_t nseg = (uint32_t)segs.size();
-	const size_t slds = scatter_lds(pc, wv, 1u << maxbits, scatter_lt(pc, wv, 1u << maxbits, ctx->prop.sharedMemPerBlock));
+	const size_t slds = scatter_lds(pc, wv, 1u << maxbits, pass_line(ctx, pc, wv, 1u << maxbits, variant));
 	// LDS table: expected partition size with headroom; bigger build sides
 	// are handled by the kernels (bc: in pieces, lp: exact host path)
 	const size_t lds_max = std::min<size_t>(ctx->prop.sharedMemPerBlock, 160 * 1024) - 1024;
@@ -2498,15 +2609,16 @@ extern "C" int hj_batch_create(hj_ctx_t *ctx, const char *algo, uint32_t njoins,
 	hipLaunchKernelGGL(k_batch_reset, dim3(std::max<uint32_t>(1, std::min<uint32_t>(1024, (njoins + 255) / 256))),
 	                   dim3(256), 0, ctx->stream, d_st, nchunks, b->d_res, njoins, b->d_jflags);
 	if (nseg) {
-		rc = launch_pass(ctx, wv, 0, (unsigned)nchunks, d_segs, d_cmap, ch, 0, maxbits, (uint64_t *)b->hist.p, hist_len,
-		                 b->part.p, d_st, (uint64_t *)b->scan_tmp.p);
+		rc = launch_pass(ctx, wv, variant, (unsigned)nchunks, d_segs, d_cmap, ch, 0, maxbits, (uint64_t *)b->hist.p, hist_len,
+		                 b->part.p, d_st, (uint64_t *)b->scan_tmp.p, true);
 		if (!rc)
 			hipLaunchKernelGGL(k_bounds_b, dim3(std::min<uint32_t>(nseg, 65535)), dim3(256), 0, ctx->stream, d_segs, nseg,
 			                   (const uint64_t *)b->hist.p, d_pb);
 	}
 	if (!rc && npairs) {
 		hipLaunchKernelGGL(k_plan_b, dim3(std::min<uint32_t>(65535, (npairs + 255) / 256)), dim3(256), 0, ctx->stream,
-		                   d_joins, d_pj, npairs, d_segs, d_pb, (const uint4 *)b->part.p, d_pairs, d_tiles);
+		                   d_joins, d_pj, npairs, d_segs, d_pb, (const uint4 *)b->part.p, d_pairs, d_tiles, d_st,
+		                   nseg && wv == 1);
 		rc = scan_u64(ctx, d_tiles, npairs, (uint64_t *)b->scan_tmp.p);
 		const int per_cu = jlds <= 40 * 1024 ? 4 : (jlds <= 76 * 1024 ? 2 : 1);
 		const unsigned jgrid = (unsigned)std::max<uint64_t>(1, (uint64_t)ctx->num_cu * per_cu);
@@ -2907,14 +3019,15 @@ static int serve_launch(hj_server *s, hj_slot &sl, const std::vector<uint64_t> &
 		                   dim3(256), 0, sl.stream, d_st, nchunks, d_res, n, d_fl);
 		if (nseg) {
 			rc = launch_pass(ctx, wv, 0, (unsigned)nchunks, d_segs, d_cmap, kServeChunk, 0, maxbits, d_hist, hist_len,
-			                 A + o_part, d_st, (uint64_t *)(A + o_scan));
+			                 A + o_part, d_st, (uint64_t *)(A + o_scan), true);
 			if (!rc)
 				hipLaunchKernelGGL(k_bounds_b, dim3(std::min<uint32_t>(nseg, 65535)), dim3(256), 0, sl.stream, d_segs,
 				                   nseg, (const uint64_t *)d_hist, d_pb);
 		}
 		if (!rc) {
 			hipLaunchKernelGGL(k_plan_b, dim3(std::min<uint32_t>(65535, (npairs + 255) / 256)), dim3(256), 0, sl.stream,
-			                   d_joins, d_pj, npairs, d_segs, d_pb, (const uint4 *)(A + o_part), d_pairs, d_tiles);
+			                   d_joins, d_pj, npairs, d_segs, d_pb, (const uint4 *)(A + o_part), d_pairs, d_tiles, d_st,
+			                   nseg && wv == 1);
 			rc = scan_u64(ctx, d_tiles, npairs, (uint64_t *)(A + o_scan));
 		}
 		if (!rc) {

@@ -343,8 +343,8 @@ class Context:
         self._check(self.lib.hj_bin_write(self.h, path.encode(), C.c_void_p(rel.ptr), rel.tuple_num,
                                           rel.tuple_size), "hj_bin_write")
 
-    def batch(self, algo: str, Rs, Ss) -> "Batch":
-        return Batch(self, algo, Rs, Ss)
+    def batch(self, algo: str, Rs, Ss, part_variant: int = 0) -> "Batch":
+        return Batch(self, algo, Rs, Ss, part_variant)
 
     def autotune(self, width: int = 16, s_ratio: int = 16, min_log2: int = 10, max_log2: int = 20):
         self._check(self.lib.hj_autotune(self.h, width, s_ratio, min_log2, max_log2), "hj_autotune")
@@ -436,14 +436,14 @@ def algo_names():
 class Batch:
     """Many joins as one captured HIP graph (hj_batch_*); per-join results."""
 
-    def __init__(self, ctx: Context, algo: str, Rs, Ss):
+    def __init__(self, ctx: Context, algo: str, Rs, Ss, part_variant: int = 0):
         if len(Rs) != len(Ss):
             raise ValueError("one build and one probe relation per join")
         self.ctx, self.n = ctx, len(Rs)
         self._R = (Relation * max(1, self.n))(*[r.as_c() for r in Rs])
         self._S = (Relation * max(1, self.n))(*[x.as_c() for x in Ss])
         h = C.c_void_p()
-        prm = Params()
+        prm = Params(part_variant=part_variant)
         ctx._check(ctx.lib.hj_batch_create(ctx.h, algo.encode(), self.n, self._R, self._S, C.byref(prm),
                                            C.byref(h)), "hj_batch_create")
         self.h = h
