@@ -101,6 +101,35 @@ def test_batch_empty_and_wide(ctx):
     _free(ctx, [r for r in Rs if r.ptr], [s for k, s in enumerate(Ss) if k != 2])
 
 
+def test_batch_switches_between_narrow_and_wide_partitions(ctx):
+    """The partition layout is decided on the device at every replay: 8-byte
+    tuples while every key and row id fits 32 bits, 16-byte ones once any
+    join of the batch holds a wider key.  Results stay those of the joins
+    run alone, and of the oracle for the join made wide."""
+    Rs, Ss = _make(ctx, 12, 8, 15, seed=6)
+    b = ctx.batch("phj_radix_bc", Rs, Ss)
+    first, info = b.run()
+    assert info["n_partitioned"] > 0
+    for i, (R, S) in enumerate(zip(Rs, Ss)):
+        alone = ctx.join("phj_radix_bc", R, S)
+        assert first[i] == (alone["matched_cnt"], alone["check_sum"]), i
+    # join j gets keys above 2^32 (same tuple counts), written in place
+    j = max(range(12), key=lambda i: Rs[i].tuple_num)
+    nr, ns = Rs[j].tuple_num, Ss[j].tuple_num
+    hR, hS = O.generate("random", nr, ns, 16, seed=77, key_range=max(1, nr // 2))
+    for h in (hR, hS):
+        h.view(np.uint64).reshape(-1, 2)[:, 0] += np.uint64(1 << 33)
+    want = O.aggregate(hR, hS, 16)
+    dR, dS = ctx.upload(hR, 16), ctx.upload(hS, 16)
+    ctx.copy_d2d(Rs[j].ptr, dR.ptr, dR.nbytes)
+    ctx.copy_d2d(Ss[j].ptr, dS.ptr, dS.nbytes)
+    second, _ = b.run()
+    assert second[j] == (want["matched_cnt"], want["check_sum"])
+    assert second[:j] == first[:j] and second[j + 1:] == first[j + 1:]
+    b.close()
+    _free(ctx, Rs, Ss, [dR, dS])
+
+
 def test_batch_rejects_bad_requests(ctx):
     R, S = ctx.generate("pkfk", 100, 1000, 16)
     with pytest.raises(hjmi.HJError):
