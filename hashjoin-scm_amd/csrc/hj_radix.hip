@@ -793,24 +793,54 @@ struct hj_gtile {
 // neighbouring digits at the same k read the same pass-A tiles, so
 // processing them together lets them share those lines in L2.  first = the
 // largest i with row[i] <= tile start (row = scanned counts of its digit).
+// The tile counts of all 2 * F1 second-level segments are read into LDS
+// once per workgroup (dynamic LDS: 8 * F1 bytes) for the rank loop.
 __global__ void k_first_run(const hj_seg *__restrict__ segs1, const hj_seg *__restrict__ segs2,
                             const uint32_t *__restrict__ chunk_seg2, const hj_stats *__restrict__ st, uint32_t b1,
                             uint32_t tile, const uint64_t *__restrict__ hist, hj_gtile *__restrict__ gt) {
+	extern __shared__ uint32_t nck[];
 	const uint32_t total = (uint32_t)st->total_chunks, F1 = 1u << b1;
+	for (uint32_t i = threadIdx.x; i < 2 * F1; i += blockDim.x) nck[i] = segs2[i].nchunks;
+	__syncthreads();
 	for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < total; c += gridDim.x * blockDim.x) {
 		const uint32_t s2 = chunk_seg2[c];
 		const hj_seg &g2 = segs2[s2];
 		const uint32_t seg = s2 >> b1, d1 = s2 & (F1 - 1), k = c - g2.chunk_base;
 		uint32_t rank = seg ? segs2[F1].chunk_base : 0u;  // segment 1 follows all of segment 0
+		const uint32_t *n_of = nck + seg * F1;
 		for (uint32_t d = 0; d < F1; d++) {
-			const uint32_t n = segs2[seg * F1 + d].nchunks;
+			const uint32_t n = n_of[d];
 			rank += min(n, k) + (d < d1 && n > k ? 1u : 0u);
 		}
 		const uint64_t ts = g2.out_start + (uint64_t)k * tile;
 		const hj_seg &g1 = segs1[seg];
 		const uint64_t at = g1.hist_base + (uint64_t)d1 * g1.nchunks;
 		const uint64_t *row = hist + at;
-		uint32_t lo = 0, hi = g1.nchunks;
+		// largest lo with row[lo] <= ts (row[0] <= ts < row[nchunks]): start
+		// at the proportional guess, gallop to a bracket, then bisect, so a
+		// near-uniform digit takes a few dependent loads instead of log2
+		const uint32_t nc = g1.nchunks;
+		const uint64_t r0 = row[0], span = row[nc] - r0;
+		uint32_t lo = 0, hi = nc;
+		if (span) {
+			const uint32_t g = (uint32_t)min((uint64_t)nc - 1, (ts - r0) * nc / span);
+			uint32_t step = 1;
+			if (row[g] <= ts) {
+				lo = g;
+				while (lo + step < hi && row[lo + step] <= ts) {
+					lo += step;
+					step <<= 1;
+				}
+				hi = min(hi, lo + step);
+			} else {
+				hi = g;
+				while (hi - lo > step && row[hi - step] > ts) {
+					hi -= step;
+					step <<= 1;
+				}
+				lo = hi > step ? max(lo, hi - step) : lo;
+			}
+		}
 		while (hi - lo > 1) {
 			const uint32_t mid = (lo + hi) >> 1;
 			if (row[mid] <= ts) lo = mid; else hi = mid;
@@ -1684,21 +1714,21 @@ template <int WV, int NT, int ITEMS>
 static void launch_sortchunk_g_t(hj_ctx_t *ctx, unsigned grid, size_t lds, const hj_gtile *gt, const uint64_t *hist,
                                  const uint16_t *beg_t, uint32_t b1, uint32_t bits, const void *in, void *out,
                                  uint16_t *run, uint64_t *cst, uint32_t tileA) {
-	// grid: all resident workgroups, a multiple of the 8 XCDs (the kernel's
-	// mapping)
+	// grid: all resident workgroups up to the cap `grid`, a multiple of the
+	// 8 XCDs (the kernel's mapping)
 	if constexpr (WV == 1) if (ctx->narrow_ok) {
 		// all-narrow instance: 8-byte LDS tile (the other returns at once)
 		auto kn = k_sortchunk_g<WV, NT, ITEMS, true>;
 		const size_t ln = lds - (size_t)NT * ITEMS * 8;
 		allow_lds(ctx, (const void *)kn);
-		const unsigned gn = std::max(8u, resident_grid(ctx, (const void *)kn, NT, ln) & ~7u);
+		const unsigned gn = std::max(8u, std::min(grid, resident_grid(ctx, (const void *)kn, NT, ln)) & ~7u);
 		hipLaunchKernelGGL(kn, dim3(gn), dim3(NT), ln, ctx->stream, gt, hist, beg_t, (const hj_stats *)ctx->stats.p,
 		                   b1, bits, (const uint4 *)in, (uint4 *)out, run, cst, ctx->narrow_ok,
 		                   (const uint8_t *)ctx->tile_narrow, tileA);
 	}
 	auto kfn = k_sortchunk_g<WV, NT, ITEMS, false>;
 	allow_lds(ctx, (const void *)kfn);
-	grid = std::max(8u, resident_grid(ctx, (const void *)kfn, NT, lds) & ~7u);
+	grid = std::max(8u, std::min(grid, resident_grid(ctx, (const void *)kfn, NT, lds)) & ~7u);
 	hipLaunchKernelGGL(kfn, dim3(grid), dim3(NT), lds, ctx->stream, gt, hist, beg_t, (const hj_stats *)ctx->stats.p,
 	                   b1, bits, (const uint4 *)in, (uint4 *)out, run, cst,
 	                   ctx->narrow_ok, (const uint8_t *)ctx->tile_narrow, tileA);
@@ -2002,10 +2032,17 @@ static int radix_engine(hj_ctx_t *ctx, const hj_relation_t *R, const hj_relation
 			hipLaunchKernelGGL(k_seg2_fin, dim3(std::min<uint32_t>(nseg2, 4096)), dim3(256), 0, ctx->stream, nch,
 			                   nseg2, F2, segs2, cmap, st);
 			hipLaunchKernelGGL(k_first_run, dim3((unsigned)std::min<uint64_t>(4096, (chunks2_max + 255) / 256)),
-			                   dim3(256), 0, ctx->stream, segs1, segs2, cmap, st, b1, (uint32_t)ch2, hist, gt);
+			                   dim3(256), (size_t)8 * F1, ctx->stream, segs1, segs2, cmap, st, b1, (uint32_t)ch2, hist,
+			                   gt);
 			// pass B: each first-level partition's tiles gathered from the
 			// pass-A runs and sorted by the second digit
-			const unsigned sgrid = (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(chunks2_max, (uint64_t)ctx->num_cu * 4));
+			// two workgroups per CU: with three the tiles in flight on an XCD
+			// touch about as many lines as its 4 MB L2 holds, and lines two
+			// runs share are evicted before the second read (pass B 1.06 vs
+			// 1.28 ms at C2); part_variant bits 14-15 (tuning sweeps) take
+			// 1: all resident, 2: one per CU
+			const uint32_t gsel = (variant >> 14) & 3;
+			const unsigned sgrid = gsel == 1 ? ~0u : (unsigned)ctx->num_cu * (gsel == 2 ? 1u : 2u);
 			hj_kt_begin(ctx, HJ_K_GATHERSORT);
 			launch_sortchunk_g(ctx, wv, sgrid, sldsB, gt, hist, beg_t, b1, b2, ctx->part[0].p, ctx->part[1].p, runs, cst,
 			                   (uint32_t)pcA.tile);
