@@ -94,25 +94,17 @@ struct stager {
 		const uint4 *g16 = (const uint4 *)(g - head);
 		const uint32_t end = head + rows * W;
 		nch = (end + 15) / 16;
+		// Whole aligned 16-byte chunks only, every thread the same number of
+		// loads (past nch: the last chunk again), none under a lane
+		// condition, so no load waits for the next one's issue.  The first
+		// and last chunk may hold bytes outside the tile; readers never look
+		// at them, and an aligned 16-byte chunk holding at least one byte of
+		// the column lies in that byte's page, so the load cannot fault.
+		if (nch == 0) return;  // an empty tile at an aligned address
 #pragma unroll
 		for (int k = 0; k < CH; k++) {
-			const uint32_t c = threadIdx.x + (uint32_t)k * TP_THREADS;
-			if (c >= nch) continue;
-			if (c * 16 >= head && c * 16 + 16 <= end) {
-				v[k] = hj_load_stream(g16 + c);
-			} else {
-				// a partial chunk: only where the range starts or ends off a
-				// 16-byte boundary (for a 16-byte aligned column, its last chunk)
-				uint64_t lo = 0, hi = 0;
-#pragma unroll 1
-				for (uint32_t b = 0; b < 16; b++) {
-					const uint32_t at = c * 16 + b;
-					if (at < head || at >= end) continue;
-					const uint64_t x = (unsigned char)g[at - head];
-					if (b < 8) lo |= x << (8 * b); else hi |= x << (8 * (b - 8));
-				}
-				v[k] = make_uint4((uint32_t)lo, (uint32_t)(lo >> 32), (uint32_t)hi, (uint32_t)(hi >> 32));
-			}
+			const uint32_t c = min(threadIdx.x + (uint32_t)k * TP_THREADS, nch - 1);
+			v[k] = hj_load_stream(g16 + c);
 		}
 	}
 	// write the chunks to `lds` (16-byte aligned, LDS bytes); returns head
@@ -246,7 +238,7 @@ struct sel_q19_parts {
 #pragma unroll
 		for (int u = 0; u < ITEMS; u++) {
 			const uint32_t r = (uint32_t)u * TP_THREADS + threadIdx.x;
-			p.size[u] = r < rows ? size[t0 + r] : 0;
+			p.size[u] = size[t0 + min(r, rows - 1)];  // clamped, unconditional (rows >= 1); masked at the test
 		}
 	}
 	__device__ void store(const pre &p, char *lds, cur &c) const {
@@ -287,7 +279,7 @@ struct sel_q19_lineitems {
 #pragma unroll
 		for (int u = 0; u < ITEMS; u++) {
 			const uint32_t r = (uint32_t)u * TP_THREADS + threadIdx.x;
-			p.q[u] = r < rows ? quantity[t0 + r] : 0;
+			p.q[u] = quantity[t0 + min(r, rows - 1)];  // clamped, unconditional (rows >= 1); masked at the test
 		}
 	}
 	__device__ void store(const pre &p, char *lds, cur &c) const {
@@ -404,18 +396,14 @@ __global__ __launch_bounds__(256) void k_select_gather(Sel sel, const uint32_t *
 	const unsigned long long beg = prefix[(uint64_t)b * 4 + o];
 	const unsigned long long cnt = (b + 1 < nblk ? prefix[(uint64_t)(b + 1) * 4 + o] : totals[o]) - beg;
 	for (unsigned long long i0 = 0; i0 < cnt; i0 += 4 * 256) {
+		// loads at clamped indices (cnt > 0 here), none under a lane
+		// condition, so the four rows' loads are in flight together
 		uint32_t off[4];
 #pragma unroll
-		for (int k = 0; k < 4; k++) {
-			const unsigned long long i = i0 + (unsigned long long)k * 256 + threadIdx.x;
-			off[k] = i < cnt ? src[i] : 0;
-		}
+		for (int k = 0; k < 4; k++) off[k] = src[min(i0 + (unsigned long long)k * 256 + threadIdx.x, cnt - 1)];
 		uint4 t[4];
 #pragma unroll
-		for (int k = 0; k < 4; k++) {
-			const unsigned long long i = i0 + (unsigned long long)k * 256 + threadIdx.x;
-			if (i < cnt) t[k] = sel.emit(row0 + off[k]);
-		}
+		for (int k = 0; k < 4; k++) t[k] = sel.emit(row0 + off[k]);
 #pragma unroll
 		for (int k = 0; k < 4; k++) {
 			const unsigned long long i = i0 + (unsigned long long)k * 256 + threadIdx.x;
