@@ -268,8 +268,9 @@ __global__ __launch_bounds__(512) void k_hist(const hj_seg *__restrict__ segs, c
 		} else {
 #pragma unroll
 			for (int u = 0; u < U; u++) {
-				const uint64_t i = i0 + (uint64_t)u * blockDim.x;
-				k[u] = i < e ? __builtin_nontemporal_load(hj_g(&keys[i * 2 * WV])) : 0;
+				// clamped, never under a lane condition (masked at the count)
+				const uint64_t i = min(i0 + (uint64_t)u * blockDim.x, e - 1);
+				k[u] = __builtin_nontemporal_load(hj_g(&keys[i * 2 * WV]));
 			}
 		}
 #pragma unroll
@@ -1483,12 +1484,15 @@ static bool capturing(hj_ctx_t *ctx) {
 template <int WV>
 static void launch_hist(hj_ctx_t *ctx, unsigned grid, const hj_seg *segs, const uint32_t *cmap, const hj_stats *st,
                         uint64_t ch, uint32_t shift, uint32_t bits, uint64_t *hist, bool detect) {
-	if (WV == 1 && detect)
-		hipLaunchKernelGGL((k_hist<WV, true>), dim3(grid), dim3(512), (size_t)4 << bits, ctx->stream, segs, cmap,
-		                   (hj_stats *)st, ch, shift, bits, hist);
-	else
-		hipLaunchKernelGGL((k_hist<WV, false>), dim3(grid), dim3(512), (size_t)4 << bits, ctx->stream, segs, cmap,
-		                   (hj_stats *)st, ch, shift, bits, hist);
+	if constexpr (WV == 1) {
+		if (detect) {
+			hipLaunchKernelGGL((k_hist<WV, true>), dim3(grid), dim3(512), (size_t)4 << bits, ctx->stream, segs, cmap,
+			                   (hj_stats *)st, ch, shift, bits, hist);
+			return;
+		}
+	}
+	hipLaunchKernelGGL((k_hist<WV, false>), dim3(grid), dim3(512), (size_t)4 << bits, ctx->stream, segs, cmap,
+	                   (hj_stats *)st, ch, shift, bits, hist);
 }
 
 template <int WV, int NT, int ITEMS>
