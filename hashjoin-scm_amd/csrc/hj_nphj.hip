@@ -403,32 +403,52 @@ __global__ __launch_bounds__(NPHJ_THREADS) void k_sc_probe(const uint4 *__restri
 	__shared__ unsigned long long red[2 * (NPHJ_THREADS / HJ_WAVE)];
 	__shared__ unsigned long long emit_ws[EMIT ? 2 * (NPHJ_THREADS / HJ_WAVE) : 2];
 	auto acc = hj_make_sink<EMIT>(pout, pcap, &st->pair_next, pchunk, emit_ws);
-	const uint64_t tile = (uint64_t)blockDim.x * PROBE_U;
-	for (uint64_t base = (uint64_t)blockIdx.x * tile; base < n; base += (uint64_t)gridDim.x * tile) {
-		uint64_t key[PROBE_U], pay[PROBE_U];
-		bool live[PROBE_U];
-		unsigned next[PROBE_U];
+	const uint64_t tile = (uint64_t)blockDim.x * PROBE_U, gstep = (uint64_t)gridDim.x * tile;
+	// Probe tuples of a tile and their bucket heads are loaded at clamped
+	// indices, never under a lane condition, and the next tile's probe
+	// tuples are issued after this tile's bucket loads and before its
+	// matches, so a match waits only for its own bucket and the next tile's
+	// tuples stay in flight (lanes past n are masked at the match).
+	auto load_s = [&](uint4 (&t)[PROBE_U], uint64_t base) {
+#pragma unroll
+		for (int u = 0; u < PROBE_U; u++)
+			t[u] = hj_load_stream(&S[min(base + (uint64_t)u * blockDim.x + threadIdx.x, n - 1) * WV]);
+	};
+	auto probe = [&](const uint4 (&t)[PROBE_U], uint64_t base, uint4 (&tn)[PROBE_U], uint64_t nbase) {
+		uint4 w[PROBE_U][4];
 #pragma unroll
 		for (int u = 0; u < PROBE_U; u++) {
-			const uint64_t i = base + (uint64_t)u * blockDim.x + threadIdx.x;
-			live[u] = i < n;
-			uint4 t = live[u] ? hj_load_stream(&S[i * WV]) : make_uint4(0, 0, 0, 0);
-			key[u] = hj_key_of(t);
-			pay[u] = hj_pay_of(t);
+			const uint4 *nd = (const uint4 *)&tab[hj_key_of(t[u]) & mask];
+#pragma unroll
+			for (int q = 0; q < 4; q++) w[u][q] = nd[q];
 		}
+		__builtin_amdgcn_sched_barrier(0);  // keep the bucket loads ahead of the next tile's
+		load_s(tn, nbase);
+		__builtin_amdgcn_sched_barrier(0);
 #pragma unroll
 		for (int u = 0; u < PROBE_U; u++) {
-			next[u] = 0;
-			if (live[u]) sc_match_node((const uint4 *)&tab[key[u] & mask], key[u], pay[u], acc, next[u]);
-		}
-#pragma unroll
-		for (int u = 0; u < PROBE_U; u++) {
+			if (base + (uint64_t)u * blockDim.x + threadIdx.x >= n) continue;
+			const uint64_t key = hj_key_of(t[u]), pay = hj_pay_of(t[u]);
+			const unsigned c = w[u][3].x < SC_SLOTS ? w[u][3].x : SC_SLOTS;
+			if (c > 0 && hj_key_of(w[u][0]) == key) acc.match(hj_pay_of(w[u][0]), pay);
+			if (c > 1 && hj_key_of(w[u][1]) == key) acc.match(hj_pay_of(w[u][1]), pay);
+			if (c > 2 && hj_key_of(w[u][2]) == key) acc.match(hj_pay_of(w[u][2]), pay);
+			unsigned next = w[u][3].y;
 			uint64_t hops = 0;
-			while (next[u] && hops++ < pool_cap) {
+			while (next && hops++ < pool_cap) {
 				unsigned nx;
-				sc_match_node((const uint4 *)&pool[next[u] - 1], key[u], pay[u], acc, nx);
-				next[u] = nx;
+				sc_match_node((const uint4 *)&pool[next - 1], key, pay, acc, nx);
+				next = nx;
 			}
+		}
+	};
+	const uint64_t b0 = (uint64_t)blockIdx.x * tile;
+	if (b0 < n) {
+		uint4 ta[PROBE_U], tb[PROBE_U];
+		load_s(ta, b0);
+		for (uint64_t base = b0; base < n; base += 2 * gstep) {
+			probe(ta, base, tb, base + gstep);
+			probe(tb, base + gstep, ta, base + 2 * gstep);  // past n: every lane masked
 		}
 	}
 	acc.finish();
