@@ -1743,14 +1743,17 @@ static size_t gathersort_lds(const part_cfg &c, uint32_t wv, uint32_t F) {
 	return (size_t)c.tile * wv * 16 + (size_t)c.tile * 8 + (2 * (size_t)F + 1) * 4 + 64 * 4;
 }
 
-static void launch_sortchunk_g(hj_ctx_t *ctx, uint32_t wv, unsigned grid, size_t lds, const hj_gtile *gt,
-                               const uint64_t *hist, const uint16_t *beg_t, uint32_t b1, uint32_t bits, const void *in,
-                               void *out, uint16_t *run, uint64_t *cst, uint32_t tileA) {
-	// shapes must match cfg_for(wv, 0)
+static void launch_sortchunk_g(hj_ctx_t *ctx, uint32_t wv, const part_cfg &pc, unsigned grid, size_t lds,
+                               const hj_gtile *gt, const uint64_t *hist, const uint16_t *beg_t, uint32_t b1,
+                               uint32_t bits, const void *in, void *out, uint16_t *run, uint64_t *cst, uint32_t tileA) {
+	// shapes must match the pass-B tile (cfg_for(wv, 0), or 1024 x 4)
 #define G(W, N, I) \
 	launch_sortchunk_g_t<W, N, I>(ctx, grid, lds, gt, hist, beg_t, b1, bits, in, out, run, cst, tileA)
 	switch (wv) {
-	case 1: G(1, 512, 4); break;
+	case 1:
+		if (pc.nt == 1024) G(1, 1024, 4);
+		else G(1, 512, 4);
+		break;
 	case 2: G(2, 512, 2); break;
 	case 4: G(4, 512, 1); break;
 	case 8: G(8, 256, 1); break;
@@ -1943,7 +1946,9 @@ static int radix_engine(hj_ctx_t *ctx, const hj_relation_t *R, const hj_relation
 		if (ch < (uint64_t)pc.tile) ch = pc.tile;
 		return ch;
 	};
-	const part_cfg pc2 = cfg_for(wv, 0);  // pass-B tile = one 32 KB sort tile
+	// pass-B tile: one 32 KB sort tile, or for 16-byte tuples 4096 by 1024
+	// threads (part_variant bits 18-19 = 1, tuning sweeps)
+	const part_cfg pc2 = (wv == 1 && ((variant >> 18) & 3) == 1) ? part_cfg{1024, 4, 4096} : cfg_for(wv, 0);
 	const part_cfg pcA = cfg_passA(wv, variant, b1, in_wv);
 	// one pass: histogram + scatter over large chunks; two passes: pass A
 	// sorts sort-tile-sized chunks in place
@@ -2048,8 +2053,8 @@ static int radix_engine(hj_ctx_t *ctx, const hj_relation_t *R, const hj_relation
 			const uint32_t gsel = (variant >> 14) & 3;
 			const unsigned sgrid = gsel == 1 ? ~0u : (unsigned)ctx->num_cu * (gsel == 2 ? 1u : 2u);
 			hj_kt_begin(ctx, HJ_K_GATHERSORT);
-			launch_sortchunk_g(ctx, wv, sgrid, sldsB, gt, hist, beg_t, b1, b2, ctx->part[0].p, ctx->part[1].p, runs, cst,
-			                   (uint32_t)pcA.tile);
+			launch_sortchunk_g(ctx, wv, pc2, sgrid, sldsB, gt, hist, beg_t, b1, b2, ctx->part[0].p, ctx->part[1].p, runs,
+			                   cst, (uint32_t)pcA.tile);
 			hj_kt_end(ctx);
 			hipLaunchKernelGGL(k_run_transpose, dim3((unsigned)((chunks2_max + 63) / 64), (F2 + 1 + 63) / 64),
 			                   dim3(256), 0, ctx->stream, runs, st, F2, (uint32_t)chunks2_max, run_t);
