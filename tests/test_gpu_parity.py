@@ -275,3 +275,28 @@ def test_narrow_second_pass_matches_wide(ctx, algo):
             _check(b, want, (name, bits, "no narrow"))
         ctx.free_relation(dR)
         ctx.free_relation(dS)
+
+
+@pytest.mark.parametrize("algo", ["phj_radix_bc", "phj_radix_lp"])
+def test_tuning_variants_match_oracle(ctx, algo):
+    """Every kernel shape a part_variant selects (one-pass tile shapes and
+    write-combining lines; two-pass first-pass tiles, second-pass
+    workgroups per CU and tile) gives the oracle's result, on narrow and on
+    wide (key above 32 bits) 16-byte tuples."""
+    nr, ns = 1 << 16, 1 << 20
+    R, S = O.generate("random", nr, ns, 16, seed=31, key_range=nr)
+    Rw = R.copy()
+    Rw.view(np.uint64)[2 * 7] += np.uint64(1 << 35)  # one wide build key
+    for tag, hR in (("narrow", R), ("wide", Rw)):
+        want = O.aggregate(hR, S, 16, lp=algo.endswith("lp"))
+        dR, dS = _dev(ctx, hR, 16), _dev(ctx, S, 16)
+        one = [v | (ln << 4) for v in range(6) for ln in (0, 2, 3)]
+        for pv in one:
+            _check(ctx.join(algo, dR, dS, radix_bits=9, passes=1, part_variant=pv), want, (tag, "one pass", hex(pv)))
+        two = [a << 12 for a in range(4)] + [g << 14 for g in range(3)] + [1 << 18, (1 << 18) | (1 << 14)]
+        for pv in two:
+            for bits in (11, 14):
+                _check(ctx.join(algo, dR, dS, radix_bits=bits, passes=2, part_variant=pv), want,
+                       (tag, "two passes", bits, hex(pv)))
+        ctx.free_relation(dR)
+        ctx.free_relation(dS)
