@@ -19,8 +19,10 @@
 // group, each against the lineitems whose quantity lies in that group's
 // range: the groups' brands differ, so a part is in at most one group and
 // every pair that passes match_filter is counted once.  The selections are one
-// pass over the columns with block-level compaction (one atomic per workgroup
-// and output).
+// pass over the columns: each workgroup writes the offsets of its selected rows
+// into its own region of a per-output list (counts kept in registers, no global
+// atomic), a scan of the per-workgroup counts places the regions, and a gather
+// builds the join inputs.
 //
 // Columns are structure-of-arrays in HBM; strings are the reference's
 // fixed-width, NUL-padded char arrays (schema.h:37-67).  A seeded generator
@@ -78,9 +80,10 @@ __device__ __forceinline__ int field_atoi(const char *s, int width) {
 // One thread's share of a tile's bytes [t0 * W, (t0 + rows) * W) of a column
 // of W-byte strings, in registers.  Byte j of the range goes to LDS byte
 // head + j, head = the range's address mod 16; thread x holds chunks
-// x, x + NT, ... of 16 bytes.  Whole chunks inside the range are one load; the
-// partial first and last chunks are read byte by byte, so no access leaves
-// the range.
+// x, x + NT, ... of 16 bytes.  Every load is a whole aligned 16-byte chunk;
+// the first and last chunk may hold bytes outside the tile, which no reader
+// looks at, and an aligned chunk holding a byte of the column lies in that
+// byte's page, so no load leaves mapped memory.
 template <int W, int ROWS>
 struct stager {
 	static constexpr uint32_t NCH = ((uint32_t)ROWS * W + 15) / 16 + 1;  // most chunks a tile spans

@@ -305,28 +305,35 @@ class Context:
         return (DeviceRelation(r, spec.r_num, spec.width, self), DeviceRelation(s, spec.s_num, spec.width, self))
 
     # -- join
-    def join(self, algo: str, R, S, radix_bits: int = 0, passes: int = 0, lp_slots_log2: int = 0,
-             kernel_times: bool = False, part_variant: int = 0, radix_bits_pass1: int = 0, flags: int = 0) -> dict:
-        """Run one join; R/S are DeviceRelation or host numpy uint8 arrays."""
-        rr, keep = [], []
-        for rel, ts in ((R, None), (S, None)):
+    @staticmethod
+    def _params(radix_bits: int = 0, passes: int = 0, lp_slots_log2: int = 0, kernel_times: bool = False,
+                part_variant: int = 0, radix_bits_pass1: int = 0, flags: int = 0) -> "Params":
+        return Params(radix_bits, passes, lp_slots_log2, (HJ_F_KERNEL_TIMES if kernel_times else 0) | flags,
+                      part_variant, radix_bits_pass1)
+
+    def join(self, algo: str, R, S, **kw) -> dict:
+        """Run one join on device relations (DeviceRelation).  Keyword
+        arguments are hj_params_t's fields: radix_bits, passes,
+        lp_slots_log2, kernel_times, part_variant, radix_bits_pass1, flags."""
+        rr = []
+        for rel in (R, S):
             if isinstance(rel, DeviceRelation):
                 rr.append(rel.as_c())
             else:
                 raise TypeError("pass DeviceRelation (use Context.upload or Context.host_relation)")
-        prm = Params(radix_bits, passes, lp_slots_log2, (HJ_F_KERNEL_TIMES if kernel_times else 0) | flags,
-                     part_variant, radix_bits_pass1)
+        prm = self._params(**kw)
         res = Result()
         self._check(self.lib.hj_join(self.h, algo.encode(), C.byref(rr[0]), C.byref(rr[1]), C.byref(prm),
                                      C.byref(res)), f"hj_join({algo})")
         return result_dict(res)
 
     def join_host(self, algo: str, r_host: np.ndarray, s_host: np.ndarray, tuple_size: int, **kw) -> dict:
-        """Join host-resident relations: the library copies them in (untimed)."""
+        """Join host-resident relations: the library copies them in (untimed).
+        Takes the same keyword arguments as join()."""
         ra, sa = np.ascontiguousarray(r_host), np.ascontiguousarray(s_host)
         R = Relation(ra.ctypes.data_as(C.c_void_p), ra.nbytes // tuple_size, tuple_size, HJ_LOC_HOST)
         S = Relation(sa.ctypes.data_as(C.c_void_p), sa.nbytes // tuple_size, tuple_size, HJ_LOC_HOST)
-        prm = Params(kw.get("radix_bits", 0), kw.get("passes", 0), kw.get("lp_slots_log2", 0), 0)
+        prm = self._params(**kw)
         res = Result()
         self._check(self.lib.hj_join(self.h, algo.encode(), C.byref(R), C.byref(S), C.byref(prm), C.byref(res)),
                     f"hj_join({algo})")

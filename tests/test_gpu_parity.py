@@ -281,22 +281,53 @@ def test_narrow_second_pass_matches_wide(ctx, algo):
 def test_tuning_variants_match_oracle(ctx, algo):
     """Every kernel shape a part_variant selects (one-pass tile shapes and
     write-combining lines; two-pass first-pass tiles, second-pass
-    workgroups per CU and tile) gives the oracle's result, on narrow and on
-    wide (key above 32 bits) 16-byte tuples."""
+    workgroups per CU and tile) gives the oracle's result, on narrow tuples
+    (PK-FK data: keys below 2^16, row ids below 2^20, so every pass-A tile
+    and pass B store 8 bytes) and on wide ones (one build key above 32 bits).
+    Build keys are unique, so lp joins run in LDS (no exact-path re-run) and
+    the result depends on the kernel shape under test."""
     nr, ns = 1 << 16, 1 << 20
-    R, S = O.generate("random", nr, ns, 16, seed=31, key_range=nr)
+    R, S = O.generate("pkfk", nr, ns, 16, seed=31)
     Rw = R.copy()
     Rw.view(np.uint64)[2 * 7] += np.uint64(1 << 35)  # one wide build key
     for tag, hR in (("narrow", R), ("wide", Rw)):
         want = O.aggregate(hR, S, 16, lp=algo.endswith("lp"))
         dR, dS = _dev(ctx, hR, 16), _dev(ctx, S, 16)
+
+        def run(**kw):
+            got = ctx.join(algo, dR, dS, **kw)
+            _check(got, want, (tag, kw))
+            assert got["fallback"] == 0, (tag, kw, got)
+            if kw.get("passes") == 2:
+                assert got["narrow"] == (1 if tag == "narrow" else 0), (tag, kw, got)
+            return got
+
         one = [v | (ln << 4) for v in range(6) for ln in (0, 2, 3)]
         for pv in one:
-            _check(ctx.join(algo, dR, dS, radix_bits=9, passes=1, part_variant=pv), want, (tag, "one pass", hex(pv)))
+            run(radix_bits=9, passes=1, part_variant=pv)
         two = [a << 12 for a in range(4)] + [g << 14 for g in range(3)] + [1 << 18, (1 << 18) | (1 << 14)]
         for pv in two:
             for bits in (11, 14):
-                _check(ctx.join(algo, dR, dS, radix_bits=bits, passes=2, part_variant=pv), want,
-                       (tag, "two passes", bits, hex(pv)))
+                run(radix_bits=bits, passes=2, part_variant=pv)
         ctx.free_relation(dR)
         ctx.free_relation(dS)
+
+
+@pytest.mark.parametrize("narrow", [True, False])
+def test_second_pass_4096_tiles_at_12_bits(ctx, narrow):
+    """The 4096-tuple second-pass tile (part_variant bits 18-19 = 1) at its
+    largest LDS footprint: 24 radix bits as 12 + 12, so pass B sorts by a
+    12-bit digit (4096 counters beside the 4096-tuple tile)."""
+    nr, ns = 1 << 17, 1 << 20
+    R, S = O.generate("pkfk", nr, ns, 16, seed=33)
+    if not narrow:
+        R = R.copy()
+        R.view(np.uint64)[2 * 11] += np.uint64(1 << 36)
+    want = O.aggregate(R, S, 16)
+    dR, dS = _dev(ctx, R, 16), _dev(ctx, S, 16)
+    for pv in (0, 1 << 18):
+        got = ctx.join("phj_radix_bc", dR, dS, radix_bits=24, passes=2, radix_bits_pass1=12, part_variant=pv)
+        _check(got, want, (narrow, hex(pv)))
+        assert got["narrow"] == int(narrow), got
+    ctx.free_relation(dR)
+    ctx.free_relation(dS)
