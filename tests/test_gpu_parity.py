@@ -316,8 +316,9 @@ def test_tuning_variants_match_oracle(ctx, algo):
 @pytest.mark.parametrize("narrow", [True, False])
 def test_second_pass_4096_tiles_at_12_bits(ctx, narrow):
     """The 4096-tuple second-pass tile (part_variant bits 18-19 = 1) at its
-    largest LDS footprint: 24 radix bits as 12 + 12, so pass B sorts by a
-    12-bit digit (4096 counters beside the 4096-tuple tile)."""
+    largest LDS footprint: 20 radix bits (the library's limit) as 8 + 12, so
+    pass B sorts by a 12-bit digit (4096 counters beside the 4096-tuple
+    tile)."""
     nr, ns = 1 << 17, 1 << 20
     R, S = O.generate("pkfk", nr, ns, 16, seed=33)
     if not narrow:
@@ -326,7 +327,7 @@ def test_second_pass_4096_tiles_at_12_bits(ctx, narrow):
     want = O.aggregate(R, S, 16)
     dR, dS = _dev(ctx, R, 16), _dev(ctx, S, 16)
     for pv in (0, 1 << 18):
-        got = ctx.join("phj_radix_bc", dR, dS, radix_bits=24, passes=2, radix_bits_pass1=12, part_variant=pv)
+        got = ctx.join("phj_radix_bc", dR, dS, radix_bits=20, passes=2, radix_bits_pass1=8, part_variant=pv)
         _check(got, want, (narrow, hex(pv)))
         assert got["narrow"] == int(narrow), got
     ctx.free_relation(dR)
