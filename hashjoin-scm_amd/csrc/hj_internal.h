@@ -70,6 +70,12 @@ struct hj_ctx {
 	hj_buf keyrid[2];  // {key, row_id} copies of wide R and S
 	hj_buf tpch[5];    // TPC-H: staged part / lineitem tuples, selection counts, contiguous outputs
 
+	// no-allocation joins (hj_join_noalloc): while `fixed` is set, the NPHJ
+	// runners take their counters (the first 256 bytes), table, overflow pool
+	// and fix-up indices from this device region instead of the arena
+	uint8_t *fixed = nullptr;
+	size_t fixed_cap = 0, fixed_off = 0;
+
 	// size-class autotuner: radix bits / passes per log2 |R| (0xff = unset)
 	uint8_t tune_bits[40];
 	uint8_t tune_passes[40];
@@ -93,6 +99,13 @@ struct hj_ctx {
 int hj_fail(hj_ctx_t *ctx, int code, const char *fmt, ...);
 // Ensure `b` holds at least `bytes`; returns HJ_OK or HJ_ENOMEM.
 int hj_reserve(hj_ctx_t *ctx, hj_buf &b, size_t bytes);
+// Device scratch of `bytes` for the role of `b`: `b` itself, grown on demand,
+// or, during a no-allocation join, the next 256-byte-aligned piece of the
+// fixed region (HJ_EINVAL if it does not fit; nothing is allocated).
+int hj_scratch(hj_ctx_t *ctx, hj_buf &b, size_t bytes, void **p);
+// Resolve the no-allocation region of the current device (hj_noalloc.hip);
+// called once per context at creation, with the context's device current.
+int hj_noalloc_setup(hj_ctx_t *ctx);
 
 #define HJ_HIP(ctx, call)                                                                   \
 	do {                                                                                    \
@@ -106,7 +119,8 @@ int hj_reserve(hj_ctx_t *ctx, hj_buf &b, size_t bytes);
 void hj_kt_begin(hj_ctx_t *ctx, uint32_t kid);
 void hj_kt_end(hj_ctx_t *ctx);
 
-// Device counters block (lives in ctx->stats).
+// Device counters block (lives in ctx->stats, or at the start of the fixed
+// region during a no-allocation join: see hj_stats_dev).
 struct hj_stats {
 	unsigned long long matched;    // match count
 	unsigned long long checksum;   // payload checksum mod 2^64
@@ -134,6 +148,9 @@ struct hj_stats {
 
 // ---- launchers (each returns HJ_OK or an error code) -----------------------
 int hj_fill16(hj_ctx_t *ctx, void *p, uint64_t n16, uint4 v);
+static inline hj_stats *hj_stats_dev(hj_ctx_t *ctx) {
+	return ctx->fixed ? (hj_stats *)ctx->fixed : (hj_stats *)ctx->stats.p;
+}
 int hj_stats_reset(hj_ctx_t *ctx);
 // copy the device counters to *h (synchronises the stream)
 int hj_stats_read(hj_ctx_t *ctx, hj_stats *h);

@@ -86,12 +86,12 @@ int hj_fill16(hj_ctx_t *ctx, void *p, uint64_t n16, uint4 v) {
 }
 
 int hj_stats_reset(hj_ctx_t *ctx) {
-	hipLaunchKernelGGL(k_stats_reset, dim3(1), dim3(64), 0, ctx->stream, (hj_stats *)ctx->stats.p);
+	hipLaunchKernelGGL(k_stats_reset, dim3(1), dim3(64), 0, ctx->stream, hj_stats_dev(ctx));
 	return HJ_OK;
 }
 
 int hj_stats_read(hj_ctx_t *ctx, hj_stats *h) {
-	HJ_HIP(ctx, hipMemcpyAsync(h, ctx->stats.p, sizeof *h, hipMemcpyDeviceToHost, ctx->stream));
+	HJ_HIP(ctx, hipMemcpyAsync(h, hj_stats_dev(ctx), sizeof *h, hipMemcpyDeviceToHost, ctx->stream));
 	HJ_HIP(ctx, hipStreamSynchronize(ctx->stream));
 	return HJ_OK;
 }
@@ -225,7 +225,7 @@ static int lp_build_table(hj_ctx_t *ctx, const hj_relation_t *R, unsigned long l
                           hj_result_t *out) {
 	const uint64_t mask = slots - 1;
 	const uint32_t wv = R->tuple_size / 16;
-	hj_stats *st = (hj_stats *)ctx->stats.p;
+	hj_stats *st = hj_stats_dev(ctx);
 	const uint64_t n = R->tuple_num;
 	const unsigned g = grid_for(ctx, n, NPHJ_THREADS);
 	hj_kt_begin(ctx, HJ_K_LP_BUILD);
@@ -248,15 +248,16 @@ static int lp_build_table(hj_ctx_t *ctx, const hj_relation_t *R, unsigned long l
 	HJ_HIP(ctx, hipStreamSynchronize(ctx->stream));
 	if (flags & HJ_DF_TABLE_FULL) return hj_fail(ctx, HJ_EINVAL, "nphj_lp table full: |R| exceeds slots");
 	if (flags & HJ_DF_DUPKEY) {
-		int rc = hj_reserve(ctx, ctx->aux, slots * 8);
+		void *aux = nullptr;
+		int rc = hj_scratch(ctx, ctx->aux, slots * 8, &aux);
 		if (rc) return rc;
-		hj_fill16(ctx, ctx->aux.p, (slots + 1) / 2, make_uint4(~0u, ~0u, ~0u, ~0u));
+		hj_fill16(ctx, aux, (slots + 1) / 2, make_uint4(~0u, ~0u, ~0u, ~0u));
 		hj_kt_begin(ctx, HJ_K_LP_FIXUP);
 #define LP_F(W)                                                                                                  \
 	hipLaunchKernelGGL(k_lp_minidx<W>, dim3(g), dim3(NPHJ_THREADS), 0, ctx->stream, (const uint4 *)R->tuples, n, tab, \
-	                   mask, (unsigned long long *)ctx->aux.p);                                                  \
+	                   mask, (unsigned long long *)aux);                                                         \
 	hipLaunchKernelGGL(k_lp_setpay<W>, dim3(g), dim3(NPHJ_THREADS), 0, ctx->stream, (const uint4 *)R->tuples, n, tab, \
-	                   mask, (const unsigned long long *)ctx->aux.p)
+	                   mask, (const unsigned long long *)aux)
 		switch (wv) {
 		case 1: LP_F(1); break;
 		case 2: LP_F(2); break;
@@ -277,10 +278,11 @@ int hj_run_nphj_lp(hj_ctx_t *ctx, const hj_relation_t *R, const hj_relation_t *S
                    hj_result_t *out) {
 	uint64_t slots = prm->lp_slots_log2 ? (1ULL << prm->lp_slots_log2) : hj_next_pow2(R->tuple_num ? R->tuple_num : 1);
 	if (slots < R->tuple_num) return hj_fail(ctx, HJ_EINVAL, "lp_slots_log2 too small for |R|");
-	int rc = hj_reserve(ctx, ctx->table, slots * 16);
+	void *tab_p = nullptr;
+	int rc = hj_scratch(ctx, ctx->table, slots * 16, &tab_p);
 	if (rc) return rc;
-	unsigned long long *tab = (unsigned long long *)ctx->table.p;
-	hj_stats *st = (hj_stats *)ctx->stats.p;
+	unsigned long long *tab = (unsigned long long *)tab_p;
+	hj_stats *st = hj_stats_dev(ctx);
 
 	HJ_HIP(ctx, hipEventRecord(ctx->ev[0], ctx->stream));
 	hj_stats_reset(ctx);
@@ -463,14 +465,18 @@ int hj_run_nphj_sc(hj_ctx_t *ctx, const hj_relation_t *R, const hj_relation_t *S
 	uint64_t nb = hj_next_pow2(R->tuple_num ? R->tuple_num : 1) / 2;
 	if (nb == 0) nb = 1;
 	uint64_t pool_cap = R->tuple_num / 4 + 1024;
-	hj_stats *st = (hj_stats *)ctx->stats.p;
+	// a no-allocation join takes the pool that never fills (one node per
+	// build tuple) from its fixed region at once, so it never rebuilds
+	if (ctx->fixed) pool_cap = R->tuple_num + 1024;
+	hj_stats *st = hj_stats_dev(ctx);
 	const uint32_t wv = R->tuple_size / 16;
-	int rc = hj_reserve(ctx, ctx->table, nb * sizeof(sc_node));
+	void *tab_p = nullptr, *pool_p = nullptr;
+	int rc = hj_scratch(ctx, ctx->table, nb * sizeof(sc_node), &tab_p);
 	if (rc) return rc;
 	for (int attempt = 0; attempt < 2; attempt++) {
-		rc = hj_reserve(ctx, ctx->pool, pool_cap * sizeof(sc_node));
+		if (attempt == 0 || !ctx->fixed) rc = hj_scratch(ctx, ctx->pool, pool_cap * sizeof(sc_node), &pool_p);
 		if (rc) return rc;
-		sc_node *tab = (sc_node *)ctx->table.p, *pool = (sc_node *)ctx->pool.p;
+		sc_node *tab = (sc_node *)tab_p, *pool = (sc_node *)pool_p;
 		HJ_HIP(ctx, hipEventRecord(ctx->ev[0], ctx->stream));
 		hj_stats_reset(ctx);
 		hj_kt_begin(ctx, HJ_K_FILL);

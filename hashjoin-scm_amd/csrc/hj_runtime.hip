@@ -48,6 +48,21 @@ int hj_reserve(hj_ctx_t *ctx, hj_buf &b, size_t bytes) {
 	return HJ_OK;
 }
 
+int hj_scratch(hj_ctx_t *ctx, hj_buf &b, size_t bytes, void **p) {
+	if (!ctx->fixed) {
+		int rc = hj_reserve(ctx, b, bytes);
+		*p = b.p;
+		return rc;
+	}
+	const size_t want = (bytes + 255) & ~(size_t)255;
+	if (ctx->fixed_off > ctx->fixed_cap || want > ctx->fixed_cap - ctx->fixed_off)
+		return hj_fail(ctx, HJ_EINVAL, "no-allocation join needs %zu more scratch bytes than its %zu-byte region holds",
+		               want - (ctx->fixed_cap - std::min(ctx->fixed_off, ctx->fixed_cap)), ctx->fixed_cap);
+	*p = ctx->fixed + ctx->fixed_off;
+	ctx->fixed_off += want;
+	return HJ_OK;
+}
+
 void hj_kt_begin(hj_ctx_t *ctx, uint32_t kid) {
 	if (!ctx->kernel_times || ctx->n_kt >= HJ_MAX_KTIMES) return;
 	ctx->kt_id[ctx->n_kt] = kid;
@@ -126,6 +141,8 @@ int hj_ctx_create(int device, hj_ctx_t **out) {
 		hj_ctx_destroy(ctx);
 		return HJ_ENOMEM;
 	}
+	// a failure here only disables hj_join_noalloc (it reports the error)
+	(void)hj_noalloc_setup(ctx);
 	*out = ctx;
 	return HJ_OK;
 }

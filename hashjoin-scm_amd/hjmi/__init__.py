@@ -142,6 +142,8 @@ def load_library(path: str = LIB_PATH):
         "hj_kernel_name": (C.c_char_p, [U32]),
         "hj_join": (I, [P, C.c_char_p, C.POINTER(Relation), C.POINTER(Relation), C.POINTER(Params),
                         C.POINTER(Result)]),
+        "hj_join_noalloc": (I, [P, C.c_char_p, C.POINTER(Relation), C.POINTER(Relation), C.POINTER(Params),
+                                C.POINTER(Result)]),
         "hj_dev_alloc": (I, [P, U64, C.POINTER(P)]),
         "hj_dev_free": (I, [P, P]),
         "hj_copy_h2d": (I, [P, P, P, U64]),
@@ -181,7 +183,7 @@ def load_library(path: str = LIB_PATH):
 def exported_symbols():
     """Names of the C entry points this binding requires."""
     return ["hj_ctx_create", "hj_ctx_destroy", "hj_strerror", "hj_last_error", "hj_algo_count",
-            "hj_algo_name", "hj_kernel_name", "hj_join", "hj_dev_alloc", "hj_dev_free", "hj_copy_h2d",
+            "hj_algo_name", "hj_kernel_name", "hj_join", "hj_join_noalloc", "hj_dev_alloc", "hj_dev_free", "hj_copy_h2d",
             "hj_copy_d2h", "hj_sync", "hj_generate", "hj_device_info", "hj_copy_d2d", "hj_batch_create",
             "hj_batch_run", "hj_batch_destroy", "hj_autotune", "hj_tuned_config", "hj_tuned_pass1_bits",
             "hj_serve_open",
@@ -325,6 +327,17 @@ class Context:
         res = Result()
         self._check(self.lib.hj_join(self.h, algo.encode(), C.byref(rr[0]), C.byref(rr[1]), C.byref(prm),
                                      C.byref(res)), f"hj_join({algo})")
+        return result_dict(res)
+
+    def join_noalloc(self, algo: str, R, S, **kw) -> dict:
+        """hj_join_noalloc: nphj_sc / nphj_lp on device relations with the
+        library's fixed per-GPU scratch (no device allocation in the call)."""
+        if not (isinstance(R, DeviceRelation) and isinstance(S, DeviceRelation)):
+            raise TypeError("hj_join_noalloc takes DeviceRelation inputs")
+        prm = self._params(**kw)
+        res = Result()
+        self._check(self.lib.hj_join_noalloc(self.h, algo.encode(), C.byref(R.as_c()), C.byref(S.as_c()),
+                                             C.byref(prm), C.byref(res)), f"hj_join_noalloc({algo})")
         return result_dict(res)
 
     def join_host(self, algo: str, r_host: np.ndarray, s_host: np.ndarray, tuple_size: int, **kw) -> dict:

@@ -141,6 +141,23 @@ const char *hj_kernel_name(uint32_t id);
 int hj_join(hj_ctx_t *ctx, const char *algo, const hj_relation_t *R, const hj_relation_t *S,
             const hj_params_t *params, hj_result_t *out);
 
+/* One join that allocates nothing.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory while they run (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * algo: nphj_sc or nphj_lp; R and S device-resident (HJ_LOC_DEVICE); params
+ * as hj_join (flags: HJ_F_KERNEL_TIMES only); results identical to hj_join.
+ * Its counters, table, overflow pool and fix-up indices live in one static
+ * device array of HJ_NOALLOC_SCRATCH_BYTES per GPU, held until the process
+ * ends: 256 B of counters, then nphj_lp 24 B per table slot
+ * (next_pow_2(|R|) slots: |R| <= 2^21) and nphj_sc 32 B per next_pow_2(|R|)
+ * plus 64 B per build tuple and 64 KiB (|R| <= 2^19).  A larger join is
+ * rejected with HJ_EINVAL before anything runs.  All calls on one GPU share
+ * that GPU's array: calls that overlap on one GPU (several contexts or host
+ * threads at once) race. */
+#define HJ_NOALLOC_SCRATCH_BYTES (64ull << 20)
+int hj_join_noalloc(hj_ctx_t *ctx, const char *algo, const hj_relation_t *R, const hj_relation_t *S,
+                    const hj_params_t *params, hj_result_t *out);
+
 /* ---- batched joins (no reference counterpart: the reference runs one join
  * per process).  Many independent joins run as one captured HIP graph: one
  * partition launch over all joins that need partitioning (radix bits chosen
