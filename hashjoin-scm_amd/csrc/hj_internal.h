@@ -138,6 +138,7 @@ struct hj_stats {
 	unsigned long long retrieved_ok; // pairs whose row ids lie inside R and S
 	unsigned long long narrow_tuples; // tuples of pass-A tiles stored as 8 bytes
 	unsigned long long pad1[2];
+	unsigned long long xq[8];      // two-pass join: work cursor of each XCD's queue
 };
 
 #define HJ_DF_DUPKEY 1u       // duplicate build key met in an LP table

@@ -22,6 +22,7 @@
 #include <mutex>
 
 static __device__ __align__(256) uint8_t scratch[HJ_NOALLOC_SCRATCH_BYTES];
+static_assert(sizeof(hj_stats) <= 256, "the counters take the first 256 bytes of the region");
 
 namespace {
 constexpr int kMaxDevices = 64;

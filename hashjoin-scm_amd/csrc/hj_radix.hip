@@ -1136,14 +1136,21 @@ __global__ __launch_bounds__(NT) void k_sortchunk_g(
 
 // Work list of the two-pass join: pair q = (p2 << b1) | p1 holds the chunk id
 // ranges of pass-1 partition p1 of R (segment p1) and of S (segment F1+p1).
-__global__ void k_plan2(const hj_seg *__restrict__ segs2, uint32_t Q, uint32_t b1, uint32_t cpt,
+// Work items of the two-pass join in (first digit, second digit) order:
+// item i is final partition q = (d2 << b1) | d1 with d1 = i >> b2 and
+// d2 = i & (F2 - 1), so the items of one first-level partition are
+// consecutive and the joins that read neighbouring runs of the same pass-B
+// tiles run together.  Both sides of every (d1, d2) gather from the pass-B
+// tiles of d1, so the pair depends on d1 only.
+__global__ void k_plan2(const hj_seg *__restrict__ segs2, uint32_t Q, uint32_t b1, uint32_t b2, uint32_t cpt,
                         hj_pair *__restrict__ pairs, uint64_t *__restrict__ tiles) {
 	const uint32_t F1 = 1u << b1;
-	for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < Q; q += gridDim.x * blockDim.x) {
-		const hj_seg &r = segs2[q & (F1 - 1)], &s = segs2[F1 + (q & (F1 - 1))];
+	for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < Q; i += gridDim.x * blockDim.x) {
+		const uint32_t d1 = i >> b2;
+		const hj_seg &r = segs2[d1], &s = segs2[F1 + d1];
 		hj_pair pr{r.chunk_base, (uint64_t)r.chunk_base + r.nchunks, s.chunk_base, (uint64_t)s.chunk_base + s.nchunks};
-		pairs[q] = pr;
-		tiles[q] = (r.len && s.len) ? (s.nchunks + cpt - 1) / cpt : 0;
+		pairs[i] = pr;
+		tiles[i] = (r.len && s.len) ? (s.nchunks + cpt - 1) / cpt : 0;
 	}
 }
 
@@ -1174,8 +1181,16 @@ __device__ __forceinline__ uint64_t pay_of(const uint4 &t) { return hj_pay_of(t)
 __device__ __forceinline__ uint64_t key_of(const uint2 &t) { return t.x; }
 __device__ __forceinline__ uint64_t pay_of(const uint2 &t) { return t.y; }
 
-// Join over chunk runs.  A work item is (final partition q = (p2 << b1) | p1,
-// probe tile = a range of pass-1 chunks of S).  Descriptors of up to NT
+// Join over chunk runs.  A work item is (final partition, probe tile = a range
+// of pass-B chunks of S), items in k_plan2's (first digit, second digit)
+// order.  The items are cut into eight queues of consecutive first-level
+// partitions, one per XCD (workgroups are dispatched round-robin over the
+// XCDs, so blockIdx % 8 names a workgroup's XCD): the 64 workgroups of an XCD
+// then join the 32 second digits of one or two first-level partitions at a
+// time, and a 128-byte line that two neighbouring runs of a pass-B tile share
+// is read from HBM once and found in that XCD's L2 the second time.  A
+// workgroup whose queue is empty takes items from the next queues, and leaves
+// once it has found all eight empty (the cursors only grow).  Descriptors of up to NT
 // chunk runs (absolute start, flat offset) are staged in LDS; the staged runs
 // form one flat sequence of tuples that the waves split into contiguous
 // ranges, so the lanes of a wave read consecutive tuples whatever the run
@@ -1214,6 +1229,8 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
 	tab.map(lds_u64, cap, &s_pool);
 	const uint32_t lane = threadIdx.x & (HJ_WAVE - 1), wid = threadIdx.x / HJ_WAVE;
 	const uint64_t total = tprefix[npairs];
+	const uint32_t b2 = B - b1, m2 = (1u << b2) - 1;
+	uint32_t cur = blockIdx.x & 7;  // this workgroup's queue (its XCD)
 	__shared__ unsigned long long emit_ws[EMIT ? 2 * (NT / HJ_WAVE) : 2];
 	auto acc = hj_make_sink<EMIT>(pout, pcap, &st->pair_next, pchunk, emit_ws);
 	bool dup = false, oversize = false;
@@ -1286,11 +1303,22 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
 
 	for (;;) {
 		if (threadIdx.x == 0) {
-			const uint64_t w = atomicAdd(&st->work_next, 1ULL);
-			uint32_t lo = 0, hi = npairs;
-			while (hi - lo > 1) {
-				const uint32_t mid = (lo + hi) >> 1;
-				if (tprefix[mid] <= w) lo = mid; else hi = mid;
+			uint64_t w = total;
+			uint32_t lo = 0;
+			for (uint32_t tried = 0; tried < 8; tried++, cur = (cur + 1) & 7) {
+				const uint32_t i0 = (uint32_t)((uint64_t)npairs * cur / 8), i1 = (uint32_t)((uint64_t)npairs * (cur + 1) / 8);
+				const uint64_t w0 = tprefix[i0], w1 = tprefix[i1];
+				if (w0 >= w1) continue;
+				const uint64_t k = atomicAdd(&st->xq[cur], 1ULL);
+				if (w0 + k >= w1) continue;
+				w = w0 + k;
+				uint32_t a = i0, b = i1;  // largest item a in [i0, i1) with tprefix[a] <= w
+				while (b - a > 1) {
+					const uint32_t mid = (a + b) >> 1;
+					if (tprefix[mid] <= w) a = mid; else b = mid;
+				}
+				lo = a;
+				break;
 			}
 			s_item[0] = w;
 			s_item[1] = lo;
@@ -1303,7 +1331,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
 		__syncthreads();
 		if (w >= total) break;
 		const hj_pair pr = pairs[q];
-		const uint32_t p2 = q >> b1;
+		const uint32_t p2 = q & m2;  // item q's second digit
 		const uint64_t sb = pr.s0 + tix * cpt, se = min(pr.s1, sb + cpt);
 		uint64_t rc = pr.r0;
 		uint32_t roff = 0;  // tuples of run rc already taken by an earlier fill
@@ -1396,6 +1424,219 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
 	if (fl && (threadIdx.x & (HJ_WAVE - 1)) == 0) atomicOr(&st->flags, fl);
 	hj_block_add2(acc.cnt, acc.sum, red, &st->matched, &st->checksum);
 }
+
+// Narrow bucket-chain join: k_join2's work loop for the common case of a bc
+// join over the 8-byte {u32 key, u32 row_id} tuples pass B writes when every
+// key and row id fits 32 bits.  Its table keeps one 8-byte word per build
+// tuple (key | payload << 32) with a u16 chain link and u32 bucket heads,
+// 10 B per tuple + 4 B per bucket instead of 18 + 4, so three workgroups
+// share a CU (more waves to cover the loads' latency), and it compares 32-bit
+// keys.  A staged run is one {source, flat offset} word, and each lane keeps
+// its current run's source and offset and the next run's offset in registers,
+// so it reads LDS for addressing only when its next tuple lies in another run.
+// Same table semantics as HJ_TK_BC (head insertion, every pair counted,
+// bucket (key >> B) & (nb - 1), fills of at most `cap` build tuples joined
+// one after the other), so the result is bc's.
+// LDS: kv u64[cap] | heads u32[np2(cap)] | next u16[cap]
+template <int NT>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(6))) void k_join2n(const uint2 *__restrict__ P, const uint64_t *__restrict__ cstart,
+                                               const uint16_t *__restrict__ run_t, uint32_t stride, uint32_t b1,
+                                               const hj_pair *__restrict__ pairs,
+                                               const uint64_t *__restrict__ tprefix, uint32_t npairs, uint32_t B,
+                                               uint32_t cpt, uint32_t cap, hj_stats *st, bool narrow_ok) {
+	if (!(narrow_ok && !(st->flags & HJ_DF_WIDE))) return;  // k_join2<1, ...> joins 16-byte tuples
+	constexpr uint32_t NW = NT / HJ_WAVE;
+	constexpr uint32_t RU = 4;               // tuples per lane per step
+	constexpr uint32_t STEP = HJ_WAVE * RU;  // tuples per wave per step
+	extern __shared__ uint64_t lds_u64[];
+	__shared__ unsigned long long red[2 * NW];
+	__shared__ uint32_t sred[NW + 1];
+	__shared__ uint64_t s_item[4];
+	__shared__ uint2 ent[NT + 1];  // staged run k: {first tuple, flat offset}; ent[NT].y = total
+	const uint32_t nb = hj_np2u(cap), hmask = nb - 1;
+	uint64_t *kv = lds_u64;
+	uint32_t *heads = (uint32_t *)(kv + cap);
+	uint16_t *next = (uint16_t *)(heads + nb);
+	const uint32_t lane = threadIdx.x & (HJ_WAVE - 1), wid = threadIdx.x / HJ_WAVE;
+	const uint64_t total = tprefix[npairs];
+	const uint32_t b2 = B - b1, m2 = (1u << b2) - 1;
+	uint32_t cur = blockIdx.x & 7;  // this workgroup's queue (its XCD), as k_join2
+	unsigned long long cnt = 0, sum = 0;
+
+	// runs of digit p2 of chunks [c0, c1), c1 - c0 <= NT; the first run starts
+	// `skip` tuples in.  Positions fit 32 bits: narrow tuples are written only
+	// when |R| and |S| are below 2^31.
+	auto stage = [&](uint64_t c0, uint64_t c1, uint32_t p2, uint32_t skip) -> uint32_t {
+		const uint64_t c = c0 + threadIdx.x;
+		uint32_t len = 0, src = 0;
+		if (c < c1) {
+			uint32_t a = run_t[(uint64_t)p2 * stride + c];
+			len = run_t[(uint64_t)(p2 + 1) * stride + c] - a;
+			if (threadIdx.x == 0) {
+				a += skip;
+				len -= skip;
+			}
+			src = (uint32_t)(cstart[c] + a);
+		}
+		uint32_t tot;
+		const uint32_t ex = hj_block_exscan<uint32_t>(len, sred, &tot);
+		ent[threadIdx.x] = make_uint2(src, ex);
+		if (threadIdx.x == NT - 1) ent[NT] = make_uint2(0u, tot);
+		return tot;
+	};
+	// a lane's position in the staged runs: run k holds flat indices
+	// [off, noff) starting at tuple src
+	struct cursor_t {
+		uint32_t k, src, off, noff;
+	};
+	auto seek = [&](uint32_t f) -> cursor_t {
+		uint32_t lo = 0, hi = NT;
+		while (hi - lo > 1) {
+			const uint32_t mid = (lo + hi) >> 1;
+			if (ent[mid].y <= f) lo = mid; else hi = mid;
+		}
+		const uint2 e = ent[lo];
+		return cursor_t{lo, e.x, e.y, ent[lo + 1].y};
+	};
+	// Load flat indices f0 + u*64 + lane (clamped to fe - 1: every lane issues
+	// RU loads, none under a lane condition); the cursor only moves forward.
+	auto load_step = [&](uint32_t f0, uint32_t fe, cursor_t &cs, uint2 (&t)[RU], bool stream) {
+#pragma unroll
+		for (uint32_t u = 0; u < RU; u++) {
+			const uint32_t f = min(f0 + u * HJ_WAVE + lane, fe - 1);
+			while (f >= cs.noff) {
+				cs.k++;
+				cs.off = cs.noff;
+				cs.src = ent[cs.k].x;
+				cs.noff = ent[cs.k + 1].y;
+			}
+			const uint2 *src8 = P + cs.src + (f - cs.off);
+			t[u] = stream ? hj_load_stream8(src8) : hj_load8(src8);
+		}
+	};
+	auto probe_step = [&](uint32_t f0, uint32_t fe, const uint2 (&t)[RU]) {
+		uint32_t c[RU];
+#pragma unroll
+		for (uint32_t u = 0; u < RU; u++)
+			c[u] = f0 + u * HJ_WAVE + lane < fe ? heads[(t[u].x >> B) & hmask] : 0u;
+		bool any = true;
+		while (any) {
+			any = false;
+#pragma unroll
+			for (uint32_t u = 0; u < RU; u++) {
+				if (!c[u]) continue;
+				const uint64_t e = kv[c[u] - 1];
+				if ((uint32_t)e == t[u].x) {
+					cnt++;
+					sum += (e >> 32) + t[u].y;
+				}
+				c[u] = next[c[u] - 1];
+				any |= c[u] != 0;
+			}
+		}
+	};
+
+	for (;;) {
+		if (threadIdx.x == 0) {
+			uint64_t w = total;
+			uint32_t lo = 0;
+			for (uint32_t tried = 0; tried < 8; tried++, cur = (cur + 1) & 7) {
+				const uint32_t i0 = (uint32_t)((uint64_t)npairs * cur / 8), i1 = (uint32_t)((uint64_t)npairs * (cur + 1) / 8);
+				const uint64_t w0 = tprefix[i0], w1 = tprefix[i1];
+				if (w0 >= w1) continue;
+				const uint64_t k = atomicAdd(&st->xq[cur], 1ULL);
+				if (w0 + k >= w1) continue;
+				w = w0 + k;
+				uint32_t a = i0, b = i1;
+				while (b - a > 1) {
+					const uint32_t mid = (a + b) >> 1;
+					if (tprefix[mid] <= w) a = mid; else b = mid;
+				}
+				lo = a;
+				break;
+			}
+			s_item[0] = w;
+			s_item[1] = lo;
+			s_item[2] = w < total ? w - tprefix[lo] : 0;
+		}
+		__syncthreads();
+		const uint64_t w = s_item[0];
+		const uint32_t q = (uint32_t)s_item[1];
+		const uint64_t tix = s_item[2];
+		__syncthreads();
+		if (w >= total) break;
+		const hj_pair pr = pairs[q];
+		const uint32_t p2 = q & m2;
+		const uint64_t sb = pr.s0 + tix * cpt, se = min(pr.s1, sb + cpt);
+		uint64_t rc = pr.r0;
+		uint32_t roff = 0;  // tuples of run rc already taken by an earlier fill
+		while (rc < pr.r1) {
+			// ---- fill the table with up to `cap` build tuples
+			for (uint32_t i = threadIdx.x; i < nb; i += NT) heads[i] = 0;
+			__syncthreads();
+			uint32_t m = 0;
+			for (;;) {
+				const uint64_t c1 = min(pr.r1, rc + NT);
+				const uint32_t n = (uint32_t)(c1 - rc);
+				const uint32_t tot = stage(rc, c1, p2, roff);
+				__syncthreads();
+				const uint32_t T = min(tot, cap - m);
+				// runs taken whole form a prefix; the next one may be taken in part
+				const uint32_t nfull = __syncthreads_count(threadIdx.x < n && ent[threadIdx.x + 1].y <= T);
+				const uint32_t roff_new = nfull < n ? (nfull == 0 ? roff : 0) + (T - ent[nfull].y) : 0;
+				const uint32_t fb = (uint32_t)((uint64_t)T * wid / NW), fe = (uint32_t)((uint64_t)T * (wid + 1) / NW);
+				if (fb < fe) {
+					cursor_t cs = seek(fb);
+					for (uint32_t f0 = fb; f0 < fe; f0 += STEP) {
+						uint2 t[RU];
+						load_step(f0, fe, cs, t, false);
+#pragma unroll
+						for (uint32_t u = 0; u < RU; u++) {
+							const uint32_t f = f0 + u * HJ_WAVE + lane;
+							if (f >= fe) continue;
+							const uint32_t i = m + f;
+							kv[i] = (uint64_t)t[u].x | (uint64_t)t[u].y << 32;
+							next[i] = (uint16_t)atomicExch(&heads[(t[u].x >> B) & hmask], i + 1);
+						}
+					}
+				}
+				m += T;
+				rc += nfull;
+				roff = roff_new;
+				__syncthreads();
+				if (m >= cap || rc >= pr.r1) break;
+			}
+			// ---- probe the tile's runs, NT chunks at a time
+			for (uint64_t kb = sb; m && kb < se; kb += NT) {
+				const uint32_t T = stage(kb, min(se, kb + NT), p2, 0);
+				__syncthreads();
+				const uint32_t fb = (uint32_t)((uint64_t)T * wid / NW), fe = (uint32_t)((uint64_t)T * (wid + 1) / NW);
+				if (fb < fe) {
+					cursor_t cs = seek(fb);
+					uint2 ta[RU], tb[RU];
+					load_step(fb, fe, cs, ta, true);
+					for (uint32_t f0 = fb;;) {
+						const uint32_t f1 = f0 + STEP;
+						if (f1 < fe) load_step(f1, fe, cs, tb, true);
+						probe_step(f0, fe, ta);
+						if (f1 >= fe) break;
+						const uint32_t f2 = f1 + STEP;
+						if (f2 < fe) load_step(f2, fe, cs, ta, true);
+						probe_step(f1, fe, tb);
+						if (f2 >= fe) break;
+						f0 = f2;
+					}
+				}
+				__syncthreads();
+			}
+		}
+		__syncthreads();
+	}
+	hj_block_add2(cnt, sum, red, &st->matched, &st->checksum);
+}
+
+// LDS bytes of k_join2n's table for `cap` build tuples
+static size_t join2n_lds(uint32_t cap) { return (size_t)cap * 10 + (size_t)hj_np2u(cap) * 4 + 8; }
 
 // ------------------------------------------------------------ host side
 struct part_cfg {
@@ -1782,7 +2023,21 @@ static void launch_join2_t(hj_ctx_t *ctx, unsigned grid, size_t lds, const void 
 				launch(k_join2<1, 512, TK, true, false>);
 			}
 		} else {
-			if (ctx->narrow_ok) launch(k_join2<1, 512, TK, false, true>);
+			if (ctx->narrow_ok) {
+				if constexpr (TK == HJ_TK_BC) {
+					// the compact table: as many workgroups per CU as its LDS allows
+					const size_t nl = join2n_lds(cap), all = nl + static_lds((const void *)k_join2n<512>);
+					const unsigned per_cu = (unsigned)std::max<size_t>(1, std::min<size_t>(4, (160 * 1024) / all));
+					const unsigned ng = (unsigned)std::max<uint64_t>(
+					    1, std::min<uint64_t>((uint64_t)ctx->num_cu * per_cu, (uint64_t)npairs * 64));
+					allow_lds(ctx, (const void *)k_join2n<512>);
+					hipLaunchKernelGGL(k_join2n<512>, dim3(ng), dim3(512), nl, ctx->stream,
+					                   (const uint2 *)P, cst, run_t, stride, b1, pairs, tpre, npairs, B, cpt, cap,
+					                   (hj_stats *)ctx->stats.p, ctx->narrow_ok);
+				} else {
+					launch(k_join2<1, 512, TK, false, true>);
+				}
+			}
 			launch(k_join2<1, 512, TK, false, false>);
 		}
 	} else {
@@ -2086,7 +2341,8 @@ static int radix_engine(hj_ctx_t *ctx, const hj_relation_t *R, const hj_relation
 	if (P == 2) {
 		const uint64_t avg_chunks = (ns / F1 + pc2.tile - 1) / pc2.tile;
 		cpt = (uint32_t)std::max<uint64_t>(16, avg_chunks + avg_chunks / 4 + 2);  // one item per uniform partition
-		hipLaunchKernelGGL(k_plan2, dim3((Q + 255) / 256), dim3(256), 0, ctx->stream, segs2, Q, b1, cpt, pairs, tiles);
+		hipLaunchKernelGGL(k_plan2, dim3((Q + 255) / 256), dim3(256), 0, ctx->stream, segs2, Q, b1, b2, cpt, pairs,
+		                   tiles);
 	} else {
 		hipLaunchKernelGGL(k_plan, dim3((Q + 255) / 256), dim3(256), 0, ctx->stream, pbF, P, Q, b1, F2, nr, ns, s_tile,
 		                   pairs, tiles);
