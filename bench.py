@@ -14,8 +14,9 @@ Also reported:
                 `traffic` is its measured HBM bytes per launch from the
                 committed rocprofv3 --pmc summary, when one exists;
   cpu_baseline  the CPU oracle's restatement of the reference's DRAM-mode
-                join (same algorithm, 11 radix bits, one pass) timed on this
-                host's cores on a bounded sample of the same workload.
+                join (same algorithm, 11 radix bits, one pass, and two passes)
+                timed on this host's cores on the full workload, 6 trials,
+                the first dropped.
 
 Prints exactly one JSON line on stdout; everything else goes to stderr.
 """
@@ -103,22 +104,38 @@ def load_traffic(kernel, workload_tag):
 
 
 def cpu_baseline(cfg, algo):
-    """Time the oracle's restatement of the reference join on host cores on a
-    1/4-scale sample of the workload (|R|/4 x |S|/4), 3 runs, first dropped."""
+    """The oracle's restatement of the reference's DRAM-mode join, timed on
+    this host's cores as the reference times itself (src/main.c:429-459):
+    the full workload, every core this process may run on (`nproc`; the GPU
+    box gives a GPU's process 16), 6 trials with the first dropped, mean and
+    min of total_time.  Radix joins run the shipped configuration (11 radix
+    bits, one pass) and are also reported with two passes."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
-    threads = min(16, os.cpu_count() or 1)
-    nr, ns = cfg["nr"] // 4, cfg["ns"] // 4
+    threads = max(1, min(16, len(os.sched_getaffinity(0))))
+    nr, ns = cfg["nr"], cfg["ns"]
     R, S = O.generate(cfg["workload"], nr, ns, cfg["width"], theta=cfg["theta"], threads=threads)
     base_algo = algo.replace("_bw_reg", "")
-    times = []
-    for _ in range(3):
-        if base_algo.startswith("ptr_"):
-            r = O.ptr_join(base_algo, R, S, cfg["width"], threads=threads, radix_bits=11, passes=1)
-        else:
-            r = O.join(base_algo, R, S, cfg["width"], threads=threads, radix_bits=11, passes=1)
-        times.append(r["total_time"])
-    t = sum(times[1:]) / len(times[1:])
+
+    def trials(passes):
+        runs = []
+        for _ in range(6):
+            if base_algo.startswith("ptr_"):
+                r = O.ptr_join(base_algo, R, S, cfg["width"], threads=threads, radix_bits=11, passes=passes)
+            else:
+                r = O.join(base_algo, R, S, cfg["width"], threads=threads, radix_bits=11, passes=passes)
+            runs.append(r)
+        kept = runs[1:]
+        mean = sum(r["total_time"] for r in kept) / len(kept)
+        return {"mean_s": mean, "min_s": min(r["total_time"] for r in kept),
+                "buildpart_s": sum(r["buildpart_time"] for r in kept) / len(kept),
+                "probejoin_s": sum(r["probejoin_time"] for r in kept) / len(kept),
+                "tuples_per_s": (nr + ns) / mean, "matched_cnt": kept[0]["matched_cnt"],
+                "check_sum": kept[0]["check_sum"]}
+
+    radix = "radix" in base_algo
+    one = trials(1)
+    two = trials(2) if radix else None
     import platform
     model = ""
     try:
@@ -128,9 +145,15 @@ def cpu_baseline(cfg, algo):
                 break
     except OSError:
         pass
-    return {"value": (nr + ns) / t, "unit": "tuples/s", "cores": threads, "kind": "port",
-            "sample": f"{base_algo} restated (oracle/hjoracle.c), {nr}x{ns} {cfg['width']}B {cfg['workload']}, "
-                      f"11 radix bits 1 pass, mean of runs 2-3 of 3, {threads} threads on {model or platform.machine()}"}
+    conf = "11 radix bits, 1 pass" if radix else "global table"
+    out = {"value": one["tuples_per_s"], "unit": "tuples/s", "cores": threads, "kind": "port",
+           "sample": f"{base_algo} restated (oracle/hjoracle.c), full workload {nr}x{ns} {cfg['width']}B "
+                     f"{cfg['workload']}, {conf}, mean of trials 2-6 of 6, {threads} threads on "
+                     f"{model or platform.machine()}",
+           "one_pass" if radix else "trials": one}
+    if two:
+        out["two_pass"] = two
+    return out
 
 
 def tpch_pushdown_bytes(q, np_, nl, res):
@@ -397,6 +420,7 @@ def main():
     ap.add_argument("--batch-lo", type=int, default=10, help="C4: log2 of the smallest |R|")
     ap.add_argument("--batch-hi", type=int, default=20, help="C4: log2 of the largest |R|")
     ap.add_argument("--algo", default=None)
+    ap.add_argument("--theta", type=float, default=None, help="C3: Zipf exponent of the probe keys (default 1.0)")
     ap.add_argument("--bits", type=int, default=0)
     ap.add_argument("--passes", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -433,6 +457,8 @@ def main():
         return
 
     cfg = dict(CONFIGS[args.config])
+    if args.theta is not None:
+        cfg["theta"] = args.theta
     algo = args.algo or cfg["algo"]
     nr, ns, W = cfg["nr"], cfg["ns"], cfg["width"]
     ctx = hjmi.Context(0)
@@ -484,7 +510,8 @@ def main():
         "metric": METRIC, "value": value, "unit": "tuples/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "u64", "data": "synthetic (seeded on device)",
-        "config": {"workload": f"{args.config}: {algo} |R|={nr} |S|={ns} {W}B {cfg['workload']}",
+        "config": {"workload": f"{args.config}: {algo} |R|={nr} |S|={ns} {W}B {cfg['workload']}"
+                               + (f" theta={cfg['theta']}" if cfg["workload"] == "zipf" else ""),
                    "radix_bits": res[0]["radix_bits"], "passes": res[0]["passes"],
                    "matched_cnt": res[0]["matched_cnt"], "check_sum": res[0]["check_sum"],
                    "device": info["arch"], "cu": info["cu_count"]},

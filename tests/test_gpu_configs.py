@@ -24,13 +24,24 @@ import oracle as O
 pytestmark = pytest.mark.gpu
 
 
+@pytest.fixture()
+def fresh():
+    """A context of its own: the configured-size runs take the library's
+    default radix rule, whatever size-class tuning earlier tests stored in
+    the session context."""
+    c = hjmi.Context(0)
+    yield c
+    c.close()
+
+
 def _want(nr, ns, seed):
     _, hS = O.generate("pkfk", nr, ns, 16, seed=seed, threads=16)
     w = O.pkfk_closed_form(hS, 16)
     return w["matched_cnt"], w["check_sum"]
 
 
-def test_c4_batch_at_configured_size_vs_oracle(ctx):
+def test_c4_batch_at_configured_size_vs_oracle(fresh):
+    ctx = fresh
     rng = np.random.default_rng(12345)
     sizes = [int(round(2 ** rng.uniform(10, 20))) for _ in range(2048)]
     Rs, Ss = [], []
@@ -61,7 +72,8 @@ def test_c4_batch_at_configured_size_vs_oracle(ctx):
     assert not bad, bad[:5]
 
 
-def test_c5_served_at_configured_size_vs_oracle(ctx):
+def test_c5_served_at_configured_size_vs_oracle(fresh):
+    ctx = fresh
     rng = np.random.default_rng(777)
 
     def rel_sizes():
