@@ -787,7 +787,7 @@ struct hj_gtile {
 	uint32_t first;       // pass-A tile whose run holds the first tuple
 	uint32_t nchunks;     // pass-A tiles of the segment
 	uint32_t chunk_base;  // global id of the segment's first pass-A tile
-	uint32_t pad;
+	uint32_t few;         // at most 32 pass-A runs cover the tile (some run is long)
 };
 
 // gt[] = the pass-B tiles sorted by (segment, tile index k, digit): tiles of
@@ -855,7 +855,9 @@ __global__ void k_first_run(const hj_seg *__restrict__ segs1, const hj_seg *__re
 		t.first = lo;
 		t.nchunks = g1.nchunks;
 		t.chunk_base = g1.chunk_base;
-		t.pad = 0;
+		// more than 32 covering runs iff run lo + 32 exists and starts inside
+		// the tile
+		t.few = !(lo + 32 < nc && row[lo + 32] < ts + t.n);
 		gt[rank] = t;
 	}
 }
@@ -896,6 +898,7 @@ __global__ __launch_bounds__(NT) void k_sortchunk_g(
 	uint32_t *tcnt = src1 + TILE;
 	uint32_t *tbase = tcnt + F;
 	uint32_t *red = tbase + F + 1;
+	__shared__ uint32_t rbeg[NT], rsp[NT];  // long-run staging: run i's first tile position and source
 	const uint32_t total = (uint32_t)st->total_chunks;
 	const uint32_t per = (F + NT - 1) / NT;
 	const uint32_t grid = gridDim.x;
@@ -934,12 +937,42 @@ __global__ __launch_bounds__(NT) void k_sortchunk_g(
 		if (WV == 1 && narrow_ok && lo < hi && nar) sp |= 1u << 31;  // nar is unset unless narrow_ok
 		for (uint64_t q = lo; q < hi; q++) srcb[q - pos] = sp + (uint32_t)(q - lo);
 	};
+	// A thread writes its own run's sources, so a tile made of a few long
+	// runs (a heavy digit under skew: one run can fill the whole tile) would
+	// be staged by a few threads in a loop of up to TILE steps while the rest
+	// wait.  For a tile covered by at most 32 runs (k_first_run's `few`), the
+	// runs' tile offsets and sources go to LDS instead and every thread
+	// stages its ITEMS positions, finding each one's run by bisection (run
+	// offsets are non-decreasing in the run).
 	auto stage = [&](const pre_t &p, const hj_gtile &t, uint32_t *srcb) {
 		const uint64_t pos = t.pos;
 		const uint32_t n = t.n;
 		if (n == 0) return;
-		if (t.first + threadIdx.x < t.nchunks)
-			run_src(srcb, pos, n, t.first + threadIdx.x, p.h0, p.h1, p.b, p.nar);
+		const bool mine = t.first + threadIdx.x < t.nchunks;
+		if (t.few) {  // uniform; the <= 32 runs are among the first NT
+			const uint64_t lo = max(p.h0, pos);
+			uint32_t sp = (t.first + threadIdx.x) * tileA + p.b + (uint32_t)(lo - p.h0);
+			if (WV == 1 && narrow_ok && p.nar) sp |= 1u << 31;
+			rbeg[threadIdx.x] = mine ? (uint32_t)(min(max(p.h0, pos), pos + n) - pos) : n;
+			rsp[threadIdx.x] = sp;
+			__syncthreads();
+			{
+				const uint32_t nr = min((uint32_t)NT, t.nchunks - t.first);
+#pragma unroll
+				for (int k = 0; k < ITEMS; k++) {
+					const uint32_t f = k * NT + threadIdx.x;
+					if (f >= n) continue;
+					uint32_t a = 0, b = nr;  // largest run a with rbeg[a] <= f
+					while (b - a > 1) {
+						const uint32_t mid = (a + b) >> 1;
+						if (rbeg[mid] <= f) a = mid; else b = mid;
+					}
+					srcb[f] = rsp[a] + (f - rbeg[a]);
+				}
+				return;
+			}
+		}
+		if (mine) run_src(srcb, pos, n, t.first + threadIdx.x, p.h0, p.h1, p.b, p.nar);
 		if (t.first + NT >= t.nchunks || p.stop >= pos + n) return;
 		const uint64_t *row = hist + t.at;
 		const uint16_t *brow = beg_t + t.at;
