@@ -512,12 +512,16 @@ __global__ __launch_bounds__(NT) void k_join(const uint4 *__restrict__ Rb, const
 		if (w >= total) break;
 		const hj_pair pr = pairs[q];
 		const uint64_t sb = pr.s0 + tix * s_tile, se = min(pr.s1, sb + s_tile);
-		if (LP && pr.r1 - pr.r0 > cap) {
-			oversize = true;  // whole join is recomputed by the host's exact path
-			continue;
-		}
-		for (uint64_t rc = pr.r0; rc < pr.r1; rc += cap) {
-			const uint32_t m = (uint32_t)min((uint64_t)cap, pr.r1 - rc);
+		// A partition larger than one table is joined fill by fill.  For bc,
+		// sc, hr and hro the pairs of the fills add up.  For lp (first match
+		// only) that holds while no key is in two fills, so each lp fill
+		// probes the later build tuples against its table and a hit flags a
+		// duplicate key (the host then recomputes the join exactly); lp fills
+		// of an oversized partition take half the table's slots, keeping
+		// linear probing at load factor 1/2.
+		const uint64_t fcap = LP && pr.r1 - pr.r0 > cap ? max(1u, hj_np2u(cap) / 2) : cap;
+		for (uint64_t rc = pr.r0; rc < pr.r1; rc += fcap) {
+			const uint32_t m = (uint32_t)min(fcap, pr.r1 - rc);
 			tab.reset(m, NT);
 			__syncthreads();
 			// build tuples four per thread and step, loads issued together
@@ -546,6 +550,11 @@ __global__ __launch_bounds__(NT) void k_join(const uint4 *__restrict__ Rb, const
 			if (LP) {
 				tab.lp_insert(m, B, NT, dup);
 				__syncthreads();
+				// keys of the later fills must not be in this one (the probe
+				// tile's first item checks; every item sees the same fills)
+				if (tix == 0)
+					for (uint64_t i = rc + m + threadIdx.x; i < pr.r1; i += NT)
+						if (tab.lp_has(hj_key_of(hj_load(&Rb[i * WV])), B)) dup = true;
 			}
 			// probe tuples U per thread and step; the next step's loads are
 			// issued before the current step is probed
@@ -2419,6 +2428,9 @@ static int radix_engine(hj_ctx_t *ctx, const hj_relation_t *R, const hj_relation
 		if (rc) return rc;
 		out->matched_cnt = r2.matched_cnt;
 		out->check_sum = r2.check_sum;
+		// the re-run is part of this join's work: its time joins the probe phase
+		out->probejoin_time += r2.total_time;
+		out->total_time += r2.total_time;
 		out->fallback |= (h.flags & HJ_DF_DUPKEY) ? HJ_FB_LP_DUPLICATES : 0;
 		out->fallback |= (h.flags & HJ_DF_OVERSIZE) ? HJ_FB_LP_OVERSIZE : 0;
 		return HJ_OK;

@@ -164,6 +164,18 @@ struct hj_tab {
 		}
 	}
 
+	// LP: is `key` in the table?  (cross-fill duplicate check)
+	__device__ bool lp_has(uint64_t key, uint32_t B) const {
+		uint32_t p = bucket(key, B);
+		for (uint32_t step = 0; step < nb; step++) {
+			const uint32_t o = heads[p];
+			if (!o) return false;
+			if (keys[o - 1] == key) return true;
+			p = (p + 1) & hmask;
+		}
+		return false;
+	}
+
 	// Probe RU keys at once (independent LDS reads overlap); adds matches.
 	template <int RU, class Sink>
 	__device__ void probe(const uint64_t (&kk)[RU], const uint64_t (&pp)[RU], const bool (&live)[RU], uint32_t B,

@@ -63,12 +63,41 @@ def test_radix_configurations(ctx, algo, bits, passes):
     dR, dS = _dev(ctx, R, 16), _dev(ctx, S, 16)
     got = ctx.join(algo, dR, dS, radix_bits=bits, passes=passes)
     _check(got, want, (bits, passes))
-    if (nr >> got["radix_bits"]) <= 4096:  # partitions fit the LDS table: no exact-path re-run
+    # unique build keys: one pass (or none) joins oversized lp partitions
+    # fill by fill; two passes re-run them exactly when a partition exceeds
+    # the LDS table
+    if (nr >> got["radix_bits"]) <= 4096 or got["passes"] <= 1:
         assert got["fallback"] == 0
     elif algo.endswith("lp"):
         assert got["fallback"] == hjmi.HJ_FB_LP_OVERSIZE
     ctx.free_relation(dR)
     ctx.free_relation(dS)
+
+
+@pytest.mark.parametrize("bits,passes", [(0, 0), (1, 1), (2, 1)])
+def test_lp_fills_and_cross_fill_duplicates(ctx, bits, passes):
+    """lp partitions larger than one LDS table are joined fill by fill: with
+    unique build keys the result is exact with no re-run; a key repeated in a
+    later fill of the same partition (here the last R tuple repeats the key
+    of an early one, same radix digit) is caught and the join recomputed, so
+    the first build row of each key still wins."""
+    nr, ns = 1 << 15, 1 << 18
+    R, S = O.generate("pkfk", nr, ns, 16, seed=41)
+    want = O.aggregate(R, S, 16, lp=True)
+    dR, dS = _dev(ctx, R, 16), _dev(ctx, S, 16)
+    got = ctx.join("phj_radix_lp", dR, dS, radix_bits=bits, passes=passes)
+    _check(got, want, ("unique", bits))
+    assert got["fallback"] == 0, got
+    R2 = R.copy()
+    w = R2.view(np.uint64)
+    w[2 * (nr - 1)] = w[2 * 5]  # last R key := key of R[5]: same digit, a later fill
+    want2 = O.aggregate(R2, S, 16, lp=True)
+    dR2 = _dev(ctx, R2, 16)
+    got2 = ctx.join("phj_radix_lp", dR2, dS, radix_bits=bits, passes=passes)
+    _check(got2, want2, ("duplicate", bits))
+    assert got2["fallback"] & hjmi.HJ_FB_LP_DUPLICATES, got2
+    for d in (dR, dS, dR2):
+        ctx.free_relation(d)
 
 
 @pytest.mark.parametrize("algo", ALGOS)

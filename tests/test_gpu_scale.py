@@ -126,6 +126,12 @@ def test_benchmark_config_c2(ctx):
     for bits, passes in ((10, 1), (11, 1), (12, 1), (12, 2), (14, 2), (16, 2)):
         _check(ctx.join("phj_radix_bc", R, S, radix_bits=bits, passes=passes), want["matched_cnt"],
                want["check_sum"], (bits, passes))
+    # the reference's shipped configuration (11 bits, one pass): 8192-tuple
+    # partitions exceed one lp table and are joined fill by fill on the GPU
+    for bits in (10, 11):
+        got = ctx.join("phj_radix_lp", R, S, radix_bits=bits, passes=1)
+        _check(got, want["matched_cnt"], want["check_sum"], ("lp", bits))
+        assert got["fallback"] == 0, got
     ctx.free_relation(R)
     ctx.free_relation(S)
 
