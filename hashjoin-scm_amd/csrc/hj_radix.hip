@@ -2110,12 +2110,16 @@ static void launch_join2(hj_ctx_t *ctx, uint32_t wv, int tk, unsigned grid, size
 
 // Default radix configuration for |R| (the size-class autotuner's base
 // rule): aim for ~2K build tuples per final partition so that the LDS table
-// stays near 40 KB; two passes once one pass would exceed 2^11 partitions.
+// stays near 40 KB; two passes from 9 bits on.  Measured at |S| = 4..64 |R|:
+// one pass wins up to 8 bits (|R| <= 2^19), two passes from 9 (2^20 x 2^24:
+// 0.34 vs 0.52 ms; 2^22 x 2^28: 2.9 vs 11.0 ms), where a one-pass scatter's
+// runs get short.
+static uint32_t default_passes(uint32_t b) { return b == 0 ? 0 : (b > 8 ? 2 : 1); }
 void hj_radix_default(uint64_t nr, uint32_t *bits, uint32_t *passes) {
 	uint32_t b = 0;
 	while (b < 18 && (nr >> b) > 2048) b++;
 	*bits = b;
-	*passes = b == 0 ? 0 : (b > 11 ? 2 : 1);
+	*passes = default_passes(b);
 }
 
 void hj_radix_config(hj_ctx_t *ctx, uint64_t nr, uint32_t *bits, uint32_t *passes, uint32_t *pass1) {
@@ -2137,7 +2141,7 @@ static int radix_engine(hj_ctx_t *ctx, const hj_relation_t *R, const hj_relation
 static uint32_t radix_plan_passes(hj_ctx_t *ctx, uint64_t nr, const hj_params_t *prm) {
 	uint32_t B = prm->radix_bits, P = prm->passes, tb1 = 0;
 	if (B == 0 && P == 0) hj_radix_config(ctx, nr, &B, &P, &tb1);
-	if (P == 0) P = B == 0 ? 0 : (B > 11 ? 2 : 1);
+	if (P == 0) P = default_passes(B);
 	if (B == 0) P = 0;
 	if (P == 2 && B < 2) P = 1;
 	return P;
@@ -2198,7 +2202,7 @@ static int radix_engine(hj_ctx_t *ctx, const hj_relation_t *R, const hj_relation
 	const uint64_t nr = R->tuple_num, ns = S->tuple_num, N = nr + ns;
 	uint32_t B = prm->radix_bits, P = prm->passes, tuned_b1 = 0;
 	if (B == 0 && P == 0) hj_radix_config(ctx, nr, &B, &P, &tuned_b1);
-	if (P == 0) P = B == 0 ? 0 : (B > 11 ? 2 : 1);
+	if (P == 0) P = default_passes(B);
 	if (B == 0) P = 0;
 	if (P == 2 && B < 2) P = 1;
 	// 512-byte tuples are partitioned by the tile sorts only (a one-pass
@@ -3072,7 +3076,7 @@ extern "C" int hj_autotune(hj_ctx_t *ctx, uint32_t width, uint32_t s_ratio, uint
 			if (b < 0) continue;
 			for (uint32_t q = 0; q < 4; q++) {
 				const uint32_t p = q == 0 ? (b == 0 ? 0u : 1u) : 2u;
-				if (p == 2 && b < 12) break;
+				if (p == 2 && b < 6) break;  // two passes below 6 bits never win
 				if (b > 12 && p == 1) continue;
 				// q = 1, 2, 3: first pass takes the even split -1, +0, +1
 				const uint32_t s1 = p == 2 ? (uint32_t)(b + 1) / 2 + q - 2 : 0;
@@ -3522,10 +3526,27 @@ static void solo_loop(hj_server *s, hj_solo_lane *lane) {
 	}
 }
 
-// scratch of the single-join engine for one request (two passes worst case)
-static uint64_t solo_footprint(const hj_sreq &r, uint32_t W) {
-	const uint64_t n = r.R.tuple_num + r.S.tuple_num;
-	return 2 * n * W + n / 64 + (64u << 20);
+// Scratch of the single-join engine for n = |R| + |S| tuples of W bytes (two
+// passes, the worst case): both partition buffers, the pass-A count matrix
+// (at most 2 B per tuple: a 4096-tuple tile has at most 1024 first-pass
+// digits of 8 B, or 128 digits per 2048-tuple tile below 8 first-pass bits),
+// run tables, tile records and work lists (below 1 B per tuple), and a fixed
+// allowance for the small buffers.
+static uint64_t solo_scratch(uint64_t n, uint32_t W) { return 2 * n * W + 3 * n + (64u << 20); }
+static uint64_t solo_footprint(const hj_sreq &r, uint32_t W) { return solo_scratch(r.R.tuple_num + r.S.tuple_num, W); }
+
+// A solo lane's scratch for the largest request it admits, reserved when the
+// server opens: a request never waits for the lane's buffers to grow (a grow
+// synchronises the lane and reallocates gigabytes).
+static int solo_reserve(hj_ctx_t *lc, uint64_t budget, uint32_t W) {
+	if (budget <= (64u << 20)) return HJ_OK;
+	const uint64_t n = (budget - (64u << 20)) / (2 * (uint64_t)W + 3);
+	int rc;
+	if ((rc = hj_reserve(lc, lc->part[0], n * W + 4096)) || (rc = hj_reserve(lc, lc->part[1], n * W + 4096)) ||
+	    (rc = hj_reserve(lc, lc->hist, 2 * n + 4096)) || (rc = hj_reserve(lc, lc->meta, n + (32u << 20))) ||
+	    (rc = hj_reserve(lc, lc->scan_tmp, (2 * n / 8) / SCAN_CH * 8 + (1u << 20))))
+		return rc;
+	return HJ_OK;
 }
 
 extern "C" int hj_serve_open(hj_ctx_t *ctx, const char *algo, const hj_serve_config_t *cfg, hj_server_t **out) {
@@ -3597,6 +3618,25 @@ extern "C" int hj_serve_open(hj_ctx_t *ctx, const char *algo, const hj_serve_con
 			memcpy(l.ctx->tune_pass1, ctx->tune_pass1, sizeof ctx->tune_pass1);
 			l.ctx->tuned = true;
 		}
+		// sized for 16-byte tuples (the width is known at the first request;
+		// wider ones may grow the partition buffers once)
+		rc = solo_reserve(l.ctx, s->solo_budget, 16);
+		if (rc) break;
+	}
+	// the slots' pinned staging buffers, sized for a full batch's plan
+	for (hj_slot &sl : s->slots)
+		if (!rc) rc = serve_grow_pinned(s, sl, 8u << 20);
+	if (rc) {
+		for (hj_solo_lane &m : s->solo)
+			if (m.ctx) hj_ctx_destroy(m.ctx);
+		for (hj_slot &sl : s->slots) {
+			if (sl.hbuf) (void)hipHostFree(sl.hbuf);
+			(void)hipFree(sl.arena);
+			(void)hipEventDestroy(sl.done);
+			(void)hipStreamDestroy(sl.stream);
+		}
+		delete s;
+		return rc;
 	}
 	s->th = std::thread(serve_loop, s);
 	for (hj_solo_lane &l : s->solo) l.th = std::thread(solo_loop, s, &l);
