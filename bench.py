@@ -287,6 +287,17 @@ def run_batched(args, hjmi):
     if not same:
         raise SystemExit("batched results differ from plain joins")
     tb, tp = min(batch_times), min(plain_times)
+    # Roofline of the batch graph (one launch: histogram, scatter, plan,
+    # join): algorithmic bytes of every join / the graph's device time.  A
+    # partitioned join reads its 16-byte tuples, writes 8-byte narrow ones and
+    # the join reads those (32 B per tuple); a join whose build side fits one
+    # LDS table is read once (16 B).  The unpartitioned joins are the
+    # smallest build sides (the bits rule is monotone in |R|).
+    n_unp = info["n_unpartitioned"]
+    order = sorted(sizes)
+    algo_bytes = sum((16 if k < n_unp else 32) * 17 * nr for k, nr in enumerate(order))
+    dev_s = min(dev_ms) * 1e-3
+    achieved = algo_bytes / dev_s / 1e9
     res = {
         "metric": "batched joins/s (one hipGraph launch) vs back-to-back plain joins", "value": args.joins / tb,
         "unit": "joins/s", "n_gpus": 1, "steps": args.steps, "warmup": 1, "ms_per_step": tb * 1e3,
@@ -298,6 +309,9 @@ def run_batched(args, hjmi):
         "batch_tuples_per_s": tuples / tb, "plain_tuples_per_s": tuples / tp,
         "batch_device_ms": min(dev_ms), "identical_results": same, "autotune_s": tune_s,
         "batch_info": {k: v for k, v in info.items()},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_GBPS, "unit": "GB/s",
+                     "frac": round(achieved / PEAK_GBPS, 4), "traffic": None, "kernel": "batch graph",
+                     "algo_bytes": algo_bytes, "device_ms": round(dev_s * 1e3, 3)},
     }
     b.close()
     ctx.close()

@@ -80,6 +80,11 @@ struct hj_ctx {
 	uint8_t tune_bits[40];
 	uint8_t tune_passes[40];
 	uint8_t tune_pass1[40];  // first-pass bits of a two-pass class (0 = even split)
+	// the fastest one-pass configuration per class (batches and serving
+	// partition in one pass): radix bits (0xff = unset) and the scatter's
+	// tile shape / line (part_variant bits 0-6)
+	uint8_t tune_bits1[40];
+	uint8_t tune_var1[40];
 	bool tuned = false;
 
 	// timing
@@ -177,6 +182,9 @@ int hj_run_ptr(hj_ctx_t *ctx, const hj_relation_t *R, const hj_relation_t *S, co
 // radix configuration for |R|: the tuned class if the context has one, else
 // the default rule
 void hj_radix_config(hj_ctx_t *ctx, uint64_t nr, uint32_t *bits, uint32_t *passes, uint32_t *pass1 = nullptr);
+// one-pass radix bits for |R| (batches, serving): the tuned class, else the
+// default rule's bits capped at 8; *variant gets the class's scatter shape
+uint32_t hj_radix_bits_onepass(hj_ctx_t *ctx, uint64_t nr, uint32_t *variant = nullptr);
 extern "C" bool hj_algo_is_lp(const char *algo, bool *radix);
 
 // Time between two recorded events of ctx->ev, in seconds.

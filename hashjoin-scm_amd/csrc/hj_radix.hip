@@ -2133,6 +2133,20 @@ void hj_radix_config(hj_ctx_t *ctx, uint64_t nr, uint32_t *bits, uint32_t *passe
 	}
 	hj_radix_default(nr, bits, passes);
 }
+uint32_t hj_radix_bits_onepass(hj_ctx_t *ctx, uint64_t nr, uint32_t *variant) {
+	const uint32_t c = hj_log2_ceil(nr ? nr : 1);
+	if (variant) *variant = 0;
+	if (ctx && ctx->tuned && c < 40 && ctx->tune_bits1[c] != 0xff) {
+		if (variant) *variant = ctx->tune_var1[c];
+		return ctx->tune_bits1[c];
+	}
+	// untuned: the default bits capped at 8, since a one-pass scatter's runs
+	// get short above that (C4's 2048-join batch: 78 ms at <= 8 bits against
+	// 108 ms with the default's up to 9-10 bits)
+	uint32_t b = 0, p = 0;
+	hj_radix_default(nr, &b, &p);
+	return std::min(b, 8u);
+}
 
 static int radix_engine(hj_ctx_t *ctx, const hj_relation_t *R, const hj_relation_t *S, const hj_params_t *prm,
                         int tk, hj_result_t *out, uint32_t in_wv, const hj_relation_t *Rfb, const hj_relation_t *Sfb);
@@ -2818,8 +2832,19 @@ extern "C" int hj_batch_create(hj_ctx_t *ctx, const char *algo, uint32_t njoins,
 	b->R.assign(R, R + njoins);
 	b->S.assign(S, S + njoins);
 	const uint32_t wv = b->wv;
-	// tile shape and write-combining line (part_variant bits 0-6, tuning)
-	const uint32_t variant = params ? (params->part_variant & 0x7f) : 0;
+	// tile shape and write-combining line (part_variant bits 0-6): the
+	// caller's, else the tuned shape of the size class holding most of the
+	// batch's partitioned tuples (one scatter launch serves every join)
+	uint32_t variant = params ? (params->part_variant & 0x7f) : 0;
+	if (!variant && ctx->tuned) {
+		uint64_t per_class[40] = {};
+		for (uint32_t j = 0; j < njoins; j++)
+			if (R[j].tuple_num > 2048) per_class[hj_log2_ceil(R[j].tuple_num) % 40] += R[j].tuple_num + S[j].tuple_num;
+		uint32_t top = 0;
+		for (uint32_t c = 1; c < 40; c++)
+			if (per_class[c] > per_class[top]) top = c;
+		if (per_class[top] && ctx->tune_bits1[top] != 0xff) variant = ctx->tune_var1[top];
+	}
 	const part_cfg pc = cfg_for(wv, variant);
 
 	// ---- plan on the host
@@ -2840,10 +2865,7 @@ extern "C" int hj_batch_create(hj_ctx_t *ctx, const char *algo, uint32_t njoins,
 		J.ns = S[j].tuple_num;
 		tuples += J.nr + J.ns;
 		uint32_t bits = 0, passes = 0;
-		if (J.nr > small_max) {
-			hj_radix_config(ctx, J.nr, &bits, &passes);
-			if (bits > 12) bits = 12;  // one pass in a batch
-		}
+		if (J.nr > small_max) bits = hj_radix_bits_onepass(ctx, J.nr);  // one pass in a batch
 		J.bits = bits;
 		J.seg = ~0u;
 		J.pair_base = (uint32_t)pair_join.size();
@@ -3046,7 +3068,7 @@ extern "C" int hj_autotune(hj_ctx_t *ctx, uint32_t width, uint32_t s_ratio, uint
 	if (!ctx || s_ratio == 0 || min_log2 > max_log2 || max_log2 > 28) return hj_fail(ctx, HJ_EINVAL, "bad tuning range");
 	if (width != 16 && width != 32 && width != 64 && width != 128 && width != 256)
 		return hj_fail(ctx, HJ_EINVAL, "tuple size %u", width);
-	std::lock_guard<std::mutex> lk(ctx->mu);
+	std::unique_lock<std::mutex> lk(ctx->mu);
 	HJ_HIP(ctx, hipSetDevice(ctx->device));
 	const uint64_t nr_max = 1ull << max_log2, ns_max = nr_max * s_ratio;
 	void *r = nullptr, *s = nullptr;
@@ -3067,8 +3089,8 @@ extern "C" int hj_autotune(hj_ctx_t *ctx, uint32_t width, uint32_t s_ratio, uint
 		spec.s_num = (1ull << c) * s_ratio;
 		if ((rc = hj_launch_generate(ctx, &spec, r, s))) break;
 		hj_relation_t R{r, spec.r_num, width, HJ_LOC_DEVICE}, S{s, spec.s_num, width, HJ_LOC_DEVICE};
-		double best = 1e30;
-		uint32_t bb = 0, bp = 0, bs = 0;
+		double best = 1e30, best1 = 1e30;
+		uint32_t bb = 0, bp = 0, bs = 0, bb1 = 0;
 		// candidates: partitions of 512..8192 build tuples, one or two passes,
 		// and for two passes the first-pass width around the even split
 		// (asymmetric bits, phj_asymm_radix.h:10-22)
@@ -3104,12 +3126,47 @@ extern "C" int hj_autotune(hj_ctx_t *ctx, uint32_t width, uint32_t s_ratio, uint
 					bp = b ? p : 0;
 					bs = s1;
 				}
+				if (p <= 1 && t < best1) {  // the one-pass choice of batches and serving
+					best1 = t;
+					bb1 = (uint32_t)b;
+				}
 			}
 			if (rc) break;
 		}
 		ctx->tune_bits[c] = (uint8_t)bb;
 		ctx->tune_passes[c] = (uint8_t)bp;
 		ctx->tune_pass1[c] = (uint8_t)bs;
+		ctx->tune_bits1[c] = (uint8_t)bb1;
+		ctx->tune_var1[c] = 0;
+		ctx->tuned = true;
+		// The one-pass scatter's tile shape and write-combining line for this
+		// class, timed through a batch of this one join (the path that uses
+		// them: batches and serving partition in one pass).  hj_batch_* take
+		// the context's lock themselves.
+		if (rc == HJ_OK && bb1) {
+			static const uint32_t k_vars[] = {0, 1, 2, 3, 4, 5, 3 << 4, 4 << 4, 5 << 4};
+			double bestv = 1e30;
+			lk.unlock();
+			for (uint32_t v : k_vars) {
+				hj_params_t prm{};
+				prm.part_variant = v;
+				hj_batch_t *bt = nullptr;
+				if (hj_batch_create(ctx, "phj_radix_bc", 1, &R, &S, &prm, &bt) != HJ_OK) continue;
+				double t = 1e30;
+				for (int rep = 0; rep < 3; rep++) {
+					hj_join_result_t jr{};
+					hj_batch_info_t bi{};
+					if (hj_batch_run(bt, &jr, &bi) != HJ_OK) break;
+					t = std::min(t, bi.device_ms);
+				}
+				hj_batch_destroy(bt);
+				if (t < bestv) {
+					bestv = t;
+					ctx->tune_var1[c] = (uint8_t)v;
+				}
+			}
+			lk.lock();
+		}
 	}
 	ctx->kernel_times = kt;
 	(void)hipStreamSynchronize(ctx->stream);
@@ -3222,9 +3279,7 @@ static serve_req_plan serve_plan_req(hj_server *s, const hj_sreq &r) {
 		p.bits = s->builds[r.build].bits;
 		p.r_here = false;
 	} else {
-		uint32_t b = 0, passes = 0;
-		if (nr > 2048) hj_radix_config(s->ctx, nr, &b, &passes);
-		p.bits = b > 12 ? 12 : b;
+		p.bits = nr > 2048 ? hj_radix_bits_onepass(s->ctx, nr) : 0;
 		p.r_here = p.bits > 0;
 	}
 	const uint64_t F = 1ull << p.bits;
@@ -3616,6 +3671,8 @@ extern "C" int hj_serve_open(hj_ctx_t *ctx, const char *algo, const hj_serve_con
 			memcpy(l.ctx->tune_bits, ctx->tune_bits, sizeof ctx->tune_bits);
 			memcpy(l.ctx->tune_passes, ctx->tune_passes, sizeof ctx->tune_passes);
 			memcpy(l.ctx->tune_pass1, ctx->tune_pass1, sizeof ctx->tune_pass1);
+			memcpy(l.ctx->tune_bits1, ctx->tune_bits1, sizeof ctx->tune_bits1);
+			memcpy(l.ctx->tune_var1, ctx->tune_var1, sizeof ctx->tune_var1);
 			l.ctx->tuned = true;
 		}
 		// sized for 16-byte tuples (the width is known at the first request;
@@ -3654,9 +3711,7 @@ extern "C" int hj_serve_register_build(hj_server_t *s, const hj_relation_t *R, u
 		return hj_fail(ctx, HJ_EINVAL, "tuple size %u", R->tuple_size);
 	hj_build_res b;
 	b.R = *R;
-	uint32_t bits = 0, passes = 0;
-	if (R->tuple_num > 2048) hj_radix_config(ctx, R->tuple_num, &bits, &passes);
-	b.bits = bits > 12 ? 12 : bits;
+	b.bits = R->tuple_num > 2048 ? hj_radix_bits_onepass(ctx, R->tuple_num) : 0;
 	if (b.bits) {
 		std::lock_guard<std::mutex> lk2(ctx->mu);
 		const uint32_t wv = s->W / 16, F = 1u << b.bits;

@@ -130,6 +130,8 @@ int hj_ctx_create(int device, hj_ctx_t **out) {
 	memset(ctx->tune_bits, 0xff, sizeof ctx->tune_bits);
 	memset(ctx->tune_passes, 0xff, sizeof ctx->tune_passes);
 	memset(ctx->tune_pass1, 0, sizeof ctx->tune_pass1);
+	memset(ctx->tune_bits1, 0xff, sizeof ctx->tune_bits1);
+	memset(ctx->tune_var1, 0, sizeof ctx->tune_var1);
 	for (int i = 0; i < hj_ctx::NEV; i++) {
 		if (hipEventCreate(&ctx->ev[i]) != hipSuccess) {
 			(void)hipGetLastError();
