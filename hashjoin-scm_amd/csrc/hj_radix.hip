@@ -2167,11 +2167,64 @@ static uint32_t radix_plan_passes(hj_ctx_t *ctx, uint64_t nr, const hj_params_t 
 // heads) instead of 5 x 64 (the reference carries whole tuples through both
 // passes, store_a_tuple, memaccess.h:309-355).  Two passes read the heads in
 // place in pass A; one pass or none copies them out first.
+// Two radix passes address a tuple inside its relation with 32 bits (31 with
+// the narrow-tuple mark).  Larger relations are joined in pieces of 2^30
+// tuples: every pair (r, s) lies in exactly one (R piece, S piece) join and
+// the count and checksum are sums, so the pieces' results add up to the
+// join's.  An lp join keeps the first build row per key, which pieces of R
+// would not see across pieces: such a join runs as nphj_lp, whose result is
+// the same.
+static const uint64_t kPiece = 1ull << 30;
+
 int hj_run_radix(hj_ctx_t *ctx, const hj_relation_t *R, const hj_relation_t *S, const hj_params_t *prm, int tk,
                  hj_result_t *out) {
 	if (tk == HJ_TK_HRO && R->tuple_size > 64)
 		return hj_fail(ctx, HJ_EINVAL, "hro joins tuples of at most 64 bytes (one 512-bit block holds a tuple)");
 	const uint32_t W = R->tuple_size, wv = W / 16;
+	if ((R->tuple_num >= (1ull << 31) || S->tuple_num >= (1ull << 31)) &&
+	    radix_plan_passes(ctx, R->tuple_num, prm) == 2) {
+		if (ctx->emit_pairs)
+			return hj_fail(ctx, HJ_EINVAL, "late-materialising radix joins take relations below 2^31 tuples");
+		if (tk == HJ_TK_LP && R->tuple_num >= (1ull << 31)) {
+			hj_params_t p2{};
+			return hj_run_nphj_lp(ctx, R, S, &p2, out);
+		}
+		// the radix configuration of the whole build side, for every piece
+		hj_params_t pp = *prm;
+		if (pp.radix_bits == 0 && pp.passes == 0) {
+			uint32_t tb1 = 0;
+			hj_radix_config(ctx, R->tuple_num, &pp.radix_bits, &pp.passes, &tb1);
+			if (!pp.radix_bits_pass1) pp.radix_bits_pass1 = tb1;
+		}
+		hj_result_t acc{};
+		for (uint64_t r0 = 0; r0 < std::max<uint64_t>(R->tuple_num, 1); r0 += kPiece) {
+			hj_relation_t rp = *R;
+			rp.tuples = (void *)((const char *)R->tuples + r0 * W);
+			rp.tuple_num = std::min(kPiece, R->tuple_num - r0);
+			for (uint64_t s0 = 0; s0 < std::max<uint64_t>(S->tuple_num, 1); s0 += kPiece) {
+				hj_relation_t sp = *S;
+				sp.tuples = (void *)((const char *)S->tuples + s0 * W);
+				sp.tuple_num = std::min(kPiece, S->tuple_num - s0);
+				hj_result_t part{};
+				const int rc = hj_run_radix(ctx, &rp, &sp, &pp, tk, &part);
+				if (rc) return rc;
+				acc.matched_cnt += part.matched_cnt;
+				acc.check_sum += part.check_sum;
+				acc.buildpart_time += part.buildpart_time;
+				acc.probejoin_time += part.probejoin_time;
+				acc.total_time += part.total_time;
+				acc.algo_bytes += part.algo_bytes;
+				acc.fallback |= part.fallback;
+				acc.radix_bits = part.radix_bits;
+				acc.passes = part.passes;
+				acc.narrow_tuples += part.narrow_tuples;
+				acc.narrow = part.narrow;
+			}
+		}
+		acc.n_ktimes = out->n_ktimes;
+		*out = acc;
+		return HJ_OK;
+	}
 	if (wv == 1) return radix_engine(ctx, R, S, prm, tk, out, 0, R, S);
 	const uint64_t nr = R->tuple_num, ns = S->tuple_num;
 	int rc;
@@ -2864,7 +2917,7 @@ extern "C" int hj_batch_create(hj_ctx_t *ctx, const char *algo, uint32_t njoins,
 		J.nr = R[j].tuple_num;
 		J.ns = S[j].tuple_num;
 		tuples += J.nr + J.ns;
-		uint32_t bits = 0, passes = 0;
+		uint32_t bits = 0;
 		if (J.nr > small_max) bits = hj_radix_bits_onepass(ctx, J.nr);  // one pass in a batch
 		J.bits = bits;
 		J.seg = ~0u;

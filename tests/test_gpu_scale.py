@@ -37,6 +37,25 @@ def test_reference_logged_pkfk(ctx, name, nr, ns):
     ctx.free_relation(S)
 
 
+@pytest.mark.parametrize("name,nr,logged", [("fndly_H", 1 << 28, 1152921508901814272),
+                                             ("fndly_T", 1 << 29, 2305843013508661248)])
+def test_reference_logged_friendly_2p32(ctx, name, nr, logged):
+    """The friendly datasets H (2^28 x 2^32) and T (2^29 x 2^32), logged
+    4294967296 / 1152921508901814272 and 4294967296 / 2305843013508661248
+    (logs/epsilon-asymmetry-logs/rdx_bc_fndly_h_nvm.log:2, ..._t_nvm.log:2;
+    sizes in config/fndly-data/uniform/*.cfg).  2^32 probe tuples: the two
+    radix passes join them in pieces of 2^30 tuples."""
+    ns = 1 << 32
+    R, S = ctx.generate("pkfk_ref", nr, ns, 16)
+    for algo in ("phj_radix_bc", "nphj_sc", "phj_radix_lp"):
+        got = ctx.join(algo, R, S)
+        _check(got, ns, logged, (name, algo))
+        if algo.startswith("phj_radix"):
+            assert got["passes"] == 2 and got["fallback"] == 0, got
+    ctx.free_relation(R)
+    ctx.free_relation(S)
+
+
 @pytest.mark.parametrize("width", [16, 64])
 def test_reference_logged_payload(ctx, width):
     """payload_pkfk_16 / _64: 268435456 / 38280597101084672."""
