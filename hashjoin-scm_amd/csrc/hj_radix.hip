@@ -703,19 +703,30 @@ __global__ __launch_bounds__(NT) void k_sortchunk(const hj_seg *__restrict__ seg
 			}
 		}
 		__syncthreads();
+		// a narrow tile is placed as 8-byte {key, row_id} words, so the store
+		// below reads consecutive 8-byte LDS words (16-byte slots read half by
+		// half hit the same banks from several lanes)
+		uint2 *tile8 = (uint2 *)tile;
 #pragma unroll
 		for (int k = 0; k < ITEMS; k++) {
 			const uint32_t idx = k * NT + threadIdx.x;
 			if (idx < n) {
 				const uint32_t q = tbase[dig[k]] + rank[k];
+				if (WV == 1 && tile_narrow) {
+					tile8[q] = make_uint2(v[k][0].x, v[k][0].z);
+				} else {
 #pragma unroll
-				for (int w = 0; w < WV; w++) tile[q * WV + w] = v[k][w];
+					for (int w = 0; w < WV; w++) tile[q * WV + w] = v[k][w];
+				}
 			}
 		}
 		__syncthreads();
 		if (tile_narrow) {
 			uint2 *dst8 = (uint2 *)(out + pos);
-			for (uint32_t u = threadIdx.x; u < n; u += NT) hj_store_stream8(&dst8[u], tile[u].x, tile[u].z);
+			for (uint32_t u = threadIdx.x; u < n; u += NT) {
+				const uint2 t = tile8[u];
+				hj_store_stream8(&dst8[u], t.x, t.y);
+			}
 		} else {
 			uint4 *dst = out + pos * WV;
 			for (uint32_t u = threadIdx.x; u < n * WV; u += NT) hj_store_stream(&dst[u], tile[u]);
