@@ -807,7 +807,7 @@ struct hj_gtile {
 	uint32_t first;       // pass-A tile whose run holds the first tuple
 	uint32_t nchunks;     // pass-A tiles of the segment
 	uint32_t chunk_base;  // global id of the segment's first pass-A tile
-	uint32_t few;         // at most 32 pass-A runs cover the tile (some run is long)
+	uint32_t ncov;        // pass-A runs (from `first` on) that start before the tile ends
 };
 
 // gt[] = the pass-B tiles sorted by (segment, tile index k, digit): tiles of
@@ -875,9 +875,23 @@ __global__ void k_first_run(const hj_seg *__restrict__ segs1, const hj_seg *__re
 		t.first = lo;
 		t.nchunks = g1.nchunks;
 		t.chunk_base = g1.chunk_base;
-		// more than 32 covering runs iff run lo + 32 exists and starts inside
-		// the tile
-		t.few = !(lo + 32 < nc && row[lo + 32] < ts + t.n);
+		// covering runs: lo up to the last run that starts before the tile's
+		// end (largest e in [lo, nc) with row[e] < ts + n), found by galloping
+		// from lo and bisecting
+		{
+			const uint64_t te = ts + t.n;
+			uint32_t a = lo, b = nc, step = 1;
+			while (a + step < nc && row[a + step] < te) {
+				a += step;
+				step <<= 1;
+			}
+			b = min(nc, a + step);
+			while (b - a > 1) {
+				const uint32_t mid = (a + b) >> 1;
+				if (row[mid] < te) a = mid; else b = mid;
+			}
+			t.ncov = t.n ? a - lo + 1 : 0;
+		}
 		gt[rank] = t;
 	}
 }
@@ -932,22 +946,23 @@ __global__ __launch_bounds__(NT) void k_sortchunk_g(
 	// hold < 2^31 tuples then).  A tile covering more than NT pass-A runs
 	// reads the rest in stage() itself.
 	struct pre_t {
-		uint64_t h0, h1, stop;
+		uint64_t h0, h1;
 		uint32_t b, nar;
 	};
 	// (no branch around the loads: a load whose result merges with another
 	// value makes the compiler wait for every load in flight; an empty
 	// record has nchunks = 1 and reads entry 0 of the tables)
+	// Thread i reads the entries of covering run first + i; threads past the
+	// covering runs read the last one's again (same lines: only the tables of
+	// the runs the tile needs are fetched).
 	auto prefetch = [&](const hj_gtile &t) -> pre_t {
 		pre_t p;
 		const uint64_t *row = hist + t.at;
-		const uint32_t last = t.nchunks - 1;
-		const uint32_t ii = min(t.first + threadIdx.x, last);
+		const uint32_t ii = t.first + min(threadIdx.x, max(t.ncov, 1u) - 1);
 		p.h0 = row[ii];
 		p.h1 = row[ii + 1];
 		p.b = beg_t[t.at + ii];
 		p.nar = tnarrow[t.chunk_base + ii];
-		p.stop = row[min(t.first + (uint32_t)NT, last)];
 		return p;
 	};
 	auto run_src = [&](uint32_t *srcb, uint64_t pos, uint32_t n, uint32_t ii, uint64_t H, uint64_t H1, uint32_t b,
@@ -960,16 +975,16 @@ __global__ __launch_bounds__(NT) void k_sortchunk_g(
 	// A thread writes its own run's sources, so a tile made of a few long
 	// runs (a heavy digit under skew: one run can fill the whole tile) would
 	// be staged by a few threads in a loop of up to TILE steps while the rest
-	// wait.  For a tile covered by at most 32 runs (k_first_run's `few`), the
-	// runs' tile offsets and sources go to LDS instead and every thread
-	// stages its ITEMS positions, finding each one's run by bisection (run
-	// offsets are non-decreasing in the run).
+	// wait.  For a tile covered by at most 32 runs, the runs' tile offsets and
+	// sources go to LDS instead and every thread stages its ITEMS positions,
+	// finding each one's run by bisection (run offsets are non-decreasing in
+	// the run).
 	auto stage = [&](const pre_t &p, const hj_gtile &t, uint32_t *srcb) {
 		const uint64_t pos = t.pos;
 		const uint32_t n = t.n;
 		if (n == 0) return;
-		const bool mine = t.first + threadIdx.x < t.nchunks;
-		if (t.few) {  // uniform; the <= 32 runs are among the first NT
+		const bool mine = threadIdx.x < t.ncov;
+		if (t.ncov <= 32) {  // uniform
 			const uint64_t lo = max(p.h0, pos);
 			uint32_t sp = (t.first + threadIdx.x) * tileA + p.b + (uint32_t)(lo - p.h0);
 			if (WV == 1 && narrow_ok && p.nar) sp |= 1u << 31;
@@ -977,7 +992,7 @@ __global__ __launch_bounds__(NT) void k_sortchunk_g(
 			rsp[threadIdx.x] = sp;
 			__syncthreads();
 			{
-				const uint32_t nr = min((uint32_t)NT, t.nchunks - t.first);
+				const uint32_t nr = t.ncov;
 #pragma unroll
 				for (int k = 0; k < ITEMS; k++) {
 					const uint32_t f = k * NT + threadIdx.x;
@@ -993,14 +1008,13 @@ __global__ __launch_bounds__(NT) void k_sortchunk_g(
 			}
 		}
 		if (mine) run_src(srcb, pos, n, t.first + threadIdx.x, p.h0, p.h1, p.b, p.nar);
-		if (t.first + NT >= t.nchunks || p.stop >= pos + n) return;
+		if (t.ncov <= NT) return;  // uniform
 		const uint64_t *row = hist + t.at;
 		const uint16_t *brow = beg_t + t.at;
-		for (uint32_t i = t.first + NT;; i += NT) {
-			const uint32_t ii = i + threadIdx.x;
-			if (ii < t.nchunks)
+		for (uint32_t i = NT; i < t.ncov; i += NT) {
+			const uint32_t ii = t.first + i + threadIdx.x;
+			if (i + threadIdx.x < t.ncov)
 				run_src(srcb, pos, n, ii, row[ii], row[ii + 1], brow[ii], tnarrow[t.chunk_base + ii]);
-			if (i + NT >= t.nchunks || row[i + NT] >= pos + n) break;
 		}
 	};
 	// Load tile t into v.  For 16-byte tuples, sel gets two bits per item
