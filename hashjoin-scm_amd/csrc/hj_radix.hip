@@ -65,14 +65,15 @@ struct hj_pair {
 
 // ------------------------------------------------------------------ scans
 // In-place exclusive scan of a[0..L) (u64); a[L] receives the total.
-__global__ __launch_bounds__(SCAN_NT) void k_scan_reduce(const uint64_t *__restrict__ a, uint64_t L,
+template <typename T>
+__global__ __launch_bounds__(SCAN_NT) void k_scan_reduce(const T *__restrict__ a, uint64_t L,
                                                          uint64_t *__restrict__ part) {
 	__shared__ uint64_t red[SCAN_NT / HJ_WAVE + 1];
 	const uint64_t base = (uint64_t)blockIdx.x * SCAN_CH;
 	// unconditional loads (clamped, masked after): a load under a per-lane
 	// condition waits for its data before the next one issues.  a holds L + 1
 	// entries, so the clamp stays inside it when L is 0.
-	uint64_t v[SCAN_ITEMS];
+	T v[SCAN_ITEMS];
 #pragma unroll
 	for (int k = 0; k < SCAN_ITEMS; k++) v[k] = a[min(base + (uint64_t)k * SCAN_NT + threadIdx.x, L ? L - 1 : 0)];
 	uint64_t s = 0;
@@ -83,8 +84,9 @@ __global__ __launch_bounds__(SCAN_NT) void k_scan_reduce(const uint64_t *__restr
 	if (threadIdx.x == 0) part[blockIdx.x] = tot;
 }
 
+template <typename T>
 __global__ __launch_bounds__(1024) void k_scan_top(uint64_t *__restrict__ part, uint32_t nparts,
-                                                   uint64_t *__restrict__ total_out) {
+                                                   T *__restrict__ total_out) {
 	__shared__ uint64_t red[1024 / HJ_WAVE + 1];
 	// each thread owns a contiguous run of partials
 	const uint32_t per = (nparts + blockDim.x - 1) / blockDim.x;
@@ -100,13 +102,14 @@ __global__ __launch_bounds__(1024) void k_scan_top(uint64_t *__restrict__ part, 
 			part[b0 + k] = run;
 			run += t;
 		}
-	if (threadIdx.x == 0) *total_out = tot;
+	if (threadIdx.x == 0) *total_out = (T)tot;
 }
 
 // The block's SCAN_CH elements move between HBM and LDS coalesced (element
 // k * SCAN_NT + t by thread t) and each thread scans its SCAN_ITEMS
 // consecutive elements in LDS.
-__global__ __launch_bounds__(SCAN_NT) void k_scan_down(uint64_t *__restrict__ a, uint64_t L,
+template <typename T>
+__global__ __launch_bounds__(SCAN_NT) void k_scan_down(T *__restrict__ a, uint64_t L,
                                                        const uint64_t *__restrict__ part) {
 	__shared__ uint64_t red[SCAN_NT / HJ_WAVE + 1];
 	// row pitch SCAN_ITEMS + 1 words: a thread's consecutive elements sit in
@@ -114,7 +117,7 @@ __global__ __launch_bounds__(SCAN_NT) void k_scan_down(uint64_t *__restrict__ a,
 	__shared__ uint64_t buf[SCAN_NT * (SCAN_ITEMS + 1)];
 	const uint64_t base = (uint64_t)blockIdx.x * SCAN_CH;
 	auto at = [](uint32_t e) { return (e / SCAN_ITEMS) * (SCAN_ITEMS + 1) + e % SCAN_ITEMS; };
-	uint64_t g[SCAN_ITEMS];
+	T g[SCAN_ITEMS];
 #pragma unroll
 	for (int k = 0; k < SCAN_ITEMS; k++) g[k] = a[min(base + (uint64_t)k * SCAN_NT + threadIdx.x, L ? L - 1 : 0)];
 #pragma unroll
@@ -141,31 +144,37 @@ __global__ __launch_bounds__(SCAN_NT) void k_scan_down(uint64_t *__restrict__ a,
 #pragma unroll
 	for (int k = 0; k < SCAN_ITEMS; k++) {
 		const uint32_t e = (uint32_t)k * SCAN_NT + threadIdx.x;
-		if (base + e < L) a[base + e] = buf[at(e)];
+		if (base + e < L) a[base + e] = (T)buf[at(e)];
 	}
 }
 
-__global__ void k_copy_u64_1(const uint64_t *__restrict__ src, uint64_t *__restrict__ dst) { *dst = *src; }
+template <typename T>
+__global__ void k_copy_total(const uint64_t *__restrict__ src, T *__restrict__ dst) { *dst = (T)*src; }
 
 // Exclusive scan of a[0..L) in place, a[L] = total.  `tmp` holds the
 // ceil(L / SCAN_CH) partials; beyond 65536 partials they are scanned the
 // same way one level up, which needs ceil(nb / SCAN_CH) + 2 more entries
 // (callers reserve partials + 2 + 512 entries: L up to 2^33).
-static int scan_u64(hj_ctx_t *ctx, uint64_t *a, uint64_t L, uint64_t *tmp) {
+template <typename T>
+static int scan_t(hj_ctx_t *ctx, T *a, uint64_t L, uint64_t *tmp) {
 	uint64_t nb = (L + SCAN_CH - 1) / SCAN_CH;
 	if (nb == 0) nb = 1;
 	if (nb > ((uint64_t)1 << 31) - 1) return hj_fail(ctx, HJ_EINVAL, "scan too long (%llu)", (unsigned long long)L);
-	hipLaunchKernelGGL(k_scan_reduce, dim3((unsigned)nb), dim3(SCAN_NT), 0, ctx->stream, a, L, tmp);
+	hipLaunchKernelGGL(k_scan_reduce<T>, dim3((unsigned)nb), dim3(SCAN_NT), 0, ctx->stream, a, L, tmp);
 	if (nb <= 1024 * 64) {
-		hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, ctx->stream, tmp, (uint32_t)nb, a + L);
+		hipLaunchKernelGGL(k_scan_top<T>, dim3(1), dim3(1024), 0, ctx->stream, tmp, (uint32_t)nb, a + L);
 	} else {
-		const int rc = scan_u64(ctx, tmp, nb, tmp + nb + 1);
+		const int rc = scan_t<uint64_t>(ctx, tmp, nb, tmp + nb + 1);
 		if (rc) return rc;
-		hipLaunchKernelGGL(k_copy_u64_1, dim3(1), dim3(1), 0, ctx->stream, tmp + nb, a + L);
+		hipLaunchKernelGGL(k_copy_total<T>, dim3(1), dim3(1), 0, ctx->stream, tmp + nb, a + L);
 	}
-	hipLaunchKernelGGL(k_scan_down, dim3((unsigned)nb), dim3(SCAN_NT), 0, ctx->stream, a, L, tmp);
+	hipLaunchKernelGGL(k_scan_down<T>, dim3((unsigned)nb), dim3(SCAN_NT), 0, ctx->stream, a, L, tmp);
 	return HJ_OK;
 }
+static int scan_u64(hj_ctx_t *ctx, uint64_t *a, uint64_t L, uint64_t *tmp) { return scan_t<uint64_t>(ctx, a, L, tmp); }
+// the two-pass count matrix: positions below 2^32 (relations of at most 2^31
+// tuples each)
+static int scan_u32(hj_ctx_t *ctx, uint32_t *a, uint64_t L, uint64_t *tmp) { return scan_t<uint32_t>(ctx, a, L, tmp); }
 
 // ------------------------------------------------------------ segments
 // Pass 1: R and S as two segments; chunk map filled for both.
@@ -184,7 +193,8 @@ __global__ void k_seg_pair(const uint4 *R, uint64_t nr, const uint4 *S, uint64_t
 
 // Partition bounds: pb[s*F + p] = first output position of digit p of
 // segment s; pb[nseg*F] = end of the last segment.
-__global__ void k_bounds(const hj_seg *__restrict__ segs, uint32_t nseg, uint32_t F, const uint64_t *__restrict__ scan,
+template <typename T>
+__global__ void k_bounds(const hj_seg *__restrict__ segs, uint32_t nseg, uint32_t F, const T *__restrict__ scan,
                          uint64_t *__restrict__ pb) {
 	const uint64_t total = (uint64_t)nseg * F;
 	for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i <= total;
@@ -767,7 +777,7 @@ __global__ __launch_bounds__(NT) void k_sortchunk(const hj_seg *__restrict__ seg
 // 64 digits through LDS so that reads and writes are both coalesced.
 __global__ __launch_bounds__(256) void k_runs_to_hist(const uint16_t *__restrict__ run, const hj_seg *__restrict__ segs,
                                                       const uint32_t *__restrict__ chunk_seg, uint32_t total,
-                                                      uint32_t F, uint64_t *__restrict__ hist,
+                                                      uint32_t F, uint32_t *__restrict__ hist,
                                                       uint16_t *__restrict__ beg_t) {
 	__shared__ uint32_t cnt[64][65];
 	__shared__ uint32_t beg[64][65];
@@ -818,7 +828,7 @@ struct hj_gtile {
 // once per workgroup (dynamic LDS: 8 * F1 bytes) for the rank loop.
 __global__ void k_first_run(const hj_seg *__restrict__ segs1, const hj_seg *__restrict__ segs2,
                             const uint32_t *__restrict__ chunk_seg2, const hj_stats *__restrict__ st, uint32_t b1,
-                            uint32_t tile, const uint64_t *__restrict__ hist, hj_gtile *__restrict__ gt) {
+                            uint32_t tile, const uint32_t *__restrict__ hist, hj_gtile *__restrict__ gt) {
 	extern __shared__ uint32_t nck[];
 	const uint32_t total = (uint32_t)st->total_chunks, F1 = 1u << b1;
 	for (uint32_t i = threadIdx.x; i < 2 * F1; i += blockDim.x) nck[i] = segs2[i].nchunks;
@@ -836,7 +846,7 @@ __global__ void k_first_run(const hj_seg *__restrict__ segs1, const hj_seg *__re
 		const uint64_t ts = g2.out_start + (uint64_t)k * tile;
 		const hj_seg &g1 = segs1[seg];
 		const uint64_t at = g1.hist_base + (uint64_t)d1 * g1.nchunks;
-		const uint64_t *row = hist + at;
+		const uint32_t *row = hist + at;
 		// largest lo with row[lo] <= ts (row[0] <= ts < row[nchunks]): start
 		// at the proportional guess, gallop to a bracket, then bisect, so a
 		// near-uniform digit takes a few dependent loads instead of log2
@@ -913,7 +923,7 @@ __global__ void k_first_run(const hj_seg *__restrict__ segs1, const hj_seg *__re
 // the device only) and the one that does not match returns at once.
 template <int WV, int NT, int ITEMS, bool NIN>
 __global__ __launch_bounds__(NT) void k_sortchunk_g(
-    const hj_gtile *__restrict__ gt, const uint64_t *__restrict__ hist, const uint16_t *__restrict__ beg_t,
+    const hj_gtile *__restrict__ gt, const uint32_t *__restrict__ hist, const uint16_t *__restrict__ beg_t,
     const hj_stats *__restrict__ st, uint32_t b1, uint32_t bits,
     const uint4 *__restrict__ in, uint4 *__restrict__ out, uint16_t *__restrict__ run, uint64_t *__restrict__ cstart,
     bool narrow_ok, const uint8_t *__restrict__ tnarrow, uint32_t tileA) {
@@ -946,8 +956,7 @@ __global__ __launch_bounds__(NT) void k_sortchunk_g(
 	// hold < 2^31 tuples then).  A tile covering more than NT pass-A runs
 	// reads the rest in stage() itself.
 	struct pre_t {
-		uint64_t h0, h1;
-		uint32_t b, nar;
+		uint32_t h0, h1, b, nar;
 	};
 	// (no branch around the loads: a load whose result merges with another
 	// value makes the compiler wait for every load in flight; an empty
@@ -957,7 +966,7 @@ __global__ __launch_bounds__(NT) void k_sortchunk_g(
 	// the runs the tile needs are fetched).
 	auto prefetch = [&](const hj_gtile &t) -> pre_t {
 		pre_t p;
-		const uint64_t *row = hist + t.at;
+		const uint32_t *row = hist + t.at;
 		const uint32_t ii = t.first + min(threadIdx.x, max(t.ncov, 1u) - 1);
 		p.h0 = row[ii];
 		p.h1 = row[ii + 1];
@@ -1009,7 +1018,7 @@ __global__ __launch_bounds__(NT) void k_sortchunk_g(
 		}
 		if (mine) run_src(srcb, pos, n, t.first + threadIdx.x, p.h0, p.h1, p.b, p.nar);
 		if (t.ncov <= NT) return;  // uniform
-		const uint64_t *row = hist + t.at;
+		const uint32_t *row = hist + t.at;
 		const uint16_t *brow = beg_t + t.at;
 		for (uint32_t i = NT; i < t.ncov; i += NT) {
 			const uint32_t ii = t.first + i + threadIdx.x;
@@ -2023,7 +2032,7 @@ static void launch_sortchunk(hj_ctx_t *ctx, uint32_t wv, const part_cfg &pc, uns
 }
 
 template <int WV, int NT, int ITEMS>
-static void launch_sortchunk_g_t(hj_ctx_t *ctx, unsigned grid, size_t lds, const hj_gtile *gt, const uint64_t *hist,
+static void launch_sortchunk_g_t(hj_ctx_t *ctx, unsigned grid, size_t lds, const hj_gtile *gt, const uint32_t *hist,
                                  const uint16_t *beg_t, uint32_t b1, uint32_t bits, const void *in, void *out,
                                  uint16_t *run, uint64_t *cst, uint32_t tileA) {
 	// grid: all resident workgroups up to the cap `grid`, a multiple of the
@@ -2052,7 +2061,7 @@ static size_t gathersort_lds(const part_cfg &c, uint32_t wv, uint32_t F) {
 }
 
 static void launch_sortchunk_g(hj_ctx_t *ctx, uint32_t wv, const part_cfg &pc, unsigned grid, size_t lds,
-                               const hj_gtile *gt, const uint64_t *hist, const uint16_t *beg_t, uint32_t b1,
+                               const hj_gtile *gt, const uint32_t *hist, const uint16_t *beg_t, uint32_t b1,
                                uint32_t bits, const void *in, void *out, uint16_t *run, uint64_t *cst, uint32_t tileA) {
 	// shapes must match the pass-B tile (cfg_for(wv, 0), or 1024 x 4)
 #define G(W, N, I) \
@@ -2321,8 +2330,8 @@ static int radix_engine(hj_ctx_t *ctx, const hj_relation_t *R, const hj_relation
 	const uint32_t b2 = B - b1;
 	const uint32_t F1 = 1u << b1, F2 = 1u << b2, Q = 1u << B;
 	if (b1 > 12 || (P == 2 && b2 > 12)) return hj_fail(ctx, HJ_EINVAL, "at most 12 radix bits per pass");
-	if (P == 2 && (nr >> 32 || ns >> 32))
-		return hj_fail(ctx, HJ_EINVAL, "two radix passes take relations of fewer than 2^32 tuples");
+	if (P == 2 && (nr >> 31 || ns >> 31))  // hj_run_radix splits larger relations
+		return hj_fail(ctx, HJ_EINVAL, "two radix passes take relations of fewer than 2^31 tuples");
 	out->radix_bits = B;
 	out->passes = P;
 	hj_stats *st = (hj_stats *)ctx->stats.p;
@@ -2391,6 +2400,9 @@ static int radix_engine(hj_ctx_t *ctx, const hj_relation_t *R, const hj_relation
 	uint16_t *beg_t = (uint16_t *)(meta + o_begt), *run_t = (uint16_t *)(meta + o_runt);
 	ctx->tile_narrow = (uint8_t *)(meta + o_tnar);
 	uint64_t *hist = (uint64_t *)ctx->hist.p;
+	// two passes: the count matrix holds 32-bit positions (a piece's R and S
+	// have at most 2^31 tuples each, so every position fits)
+	uint32_t *hist32 = (uint32_t *)ctx->hist.p;
 
 	HJ_HIP(ctx, hipEventRecord(ctx->ev[0], ctx->stream));
 	hj_stats_reset(ctx);
@@ -2418,13 +2430,17 @@ static int radix_engine(hj_ctx_t *ctx, const hj_relation_t *R, const hj_relation
 			hj_kt_begin(ctx, HJ_K_SCAN);
 			hipLaunchKernelGGL(k_runs_to_hist, dim3((unsigned)std::max<uint64_t>(1, (chunks1 + 63) / 64), (F1 + 63) / 64),
 			                   dim3(256), 0,
-			                   ctx->stream, runsA, segs1, cmap, (uint32_t)chunks1, F1, hist, beg_t);
-			rc = scan_u64(ctx, hist, hist1_len, (uint64_t *)ctx->scan_tmp.p);
+			                   ctx->stream, runsA, segs1, cmap, (uint32_t)chunks1, F1, hist32, beg_t);
+			rc = scan_u32(ctx, hist32, hist1_len, (uint64_t *)ctx->scan_tmp.p);
 			hj_kt_end(ctx);
 			if (rc) return rc;
 		}
-		hipLaunchKernelGGL(k_bounds, dim3((unsigned)std::min<uint64_t>(1024, (2ull * F1 + 256) / 256)), dim3(256), 0,
-		                   ctx->stream, segs1, 2u, F1, hist, pb1);
+		if (P == 1)
+			hipLaunchKernelGGL(k_bounds<uint64_t>, dim3((unsigned)std::min<uint64_t>(1024, (2ull * F1 + 256) / 256)),
+			                   dim3(256), 0, ctx->stream, segs1, 2u, F1, (const uint64_t *)hist, pb1);
+		else
+			hipLaunchKernelGGL(k_bounds<uint32_t>, dim3((unsigned)std::min<uint64_t>(1024, (2ull * F1 + 256) / 256)),
+			                   dim3(256), 0, ctx->stream, segs1, 2u, F1, (const uint32_t *)hist32, pb1);
 		pbF = pb1;
 		Rb = Sb = ctx->part[0].p;
 		if (P == 2) {
@@ -2434,7 +2450,7 @@ static int radix_engine(hj_ctx_t *ctx, const hj_relation_t *R, const hj_relation
 			hipLaunchKernelGGL(k_seg2_fin, dim3(std::min<uint32_t>(nseg2, 4096)), dim3(256), 0, ctx->stream, nch,
 			                   nseg2, F2, segs2, cmap, st);
 			hipLaunchKernelGGL(k_first_run, dim3((unsigned)std::min<uint64_t>(4096, (chunks2_max + 255) / 256)),
-			                   dim3(256), (size_t)8 * F1, ctx->stream, segs1, segs2, cmap, st, b1, (uint32_t)ch2, hist,
+			                   dim3(256), (size_t)8 * F1, ctx->stream, segs1, segs2, cmap, st, b1, (uint32_t)ch2, hist32,
 			                   gt);
 			// pass B: each first-level partition's tiles gathered from the
 			// pass-A runs and sorted by the second digit
@@ -2446,7 +2462,7 @@ static int radix_engine(hj_ctx_t *ctx, const hj_relation_t *R, const hj_relation
 			const uint32_t gsel = (variant >> 14) & 3;
 			const unsigned sgrid = gsel == 1 ? ~0u : (unsigned)ctx->num_cu * (gsel == 2 ? 1u : 2u);
 			hj_kt_begin(ctx, HJ_K_GATHERSORT);
-			launch_sortchunk_g(ctx, wv, pc2, sgrid, sldsB, gt, hist, beg_t, b1, b2, ctx->part[0].p, ctx->part[1].p, runs,
+			launch_sortchunk_g(ctx, wv, pc2, sgrid, sldsB, gt, hist32, beg_t, b1, b2, ctx->part[0].p, ctx->part[1].p, runs,
 			                   cst, (uint32_t)pcA.tile);
 			hj_kt_end(ctx);
 			hipLaunchKernelGGL(k_run_transpose, dim3((unsigned)((chunks2_max + 63) / 64), (F2 + 1 + 63) / 64),
