@@ -3009,6 +3009,11 @@ extern "C" int hj_batch_create(hj_ctx_t *ctx, const char *algo, uint32_t njoins,
 	// are handled by the kernels (bc: in pieces, lp: exact host path)
 	const size_t lds_max = std::min<size_t>(ctx->prop.sharedMemPerBlock, 160 * 1024) - 1024;
 	uint32_t cap = (uint32_t)std::min<uint64_t>(65535, std::max<uint64_t>(small_max, max_rq + max_rq / 4 + 64));
+	// bc joins a larger partition in several fills, so its table stays small
+	// enough for two workgroups per CU (the batch's largest partition would
+	// otherwise size every workgroup's table: C4 77 -> 68 ms); lp needs the
+	// whole partition in one table
+	if (!lp) cap = std::min<uint32_t>(cap, 3400);
 	while (cap > 256 && join_lds(cap, lp) > lds_max) cap -= 256;
 	const size_t jlds = join_lds(cap, lp);
 
@@ -3502,6 +3507,7 @@ static int serve_launch(hj_server *s, hj_slot &sl, const std::vector<uint64_t> &
 	const size_t lds_max =
 	    std::min<size_t>(ctx->prop.sharedMemPerBlock, 160 * 1024) - static_lds(join_b_kernel(wv, s->lp)) - 256;
 	uint32_t cap = (uint32_t)std::min<uint64_t>(65535, std::max<uint64_t>(2048, max_rq + max_rq / 4 + 64));
+	if (!s->lp) cap = std::min<uint32_t>(cap, 3400);  // as in batches: two workgroups per CU
 	while (cap > 256 && join_lds(cap, s->lp) > lds_max) cap -= 256;
 	const size_t jlds = join_lds(cap, s->lp);
 	if (jlds > lds_max) return hj_fail(ctx, HJ_EINVAL, "serve join table does not fit LDS");
