@@ -182,8 +182,9 @@ int hj_run_ptr(hj_ctx_t *ctx, const hj_relation_t *R, const hj_relation_t *S, co
 // radix configuration for |R|: the tuned class if the context has one, else
 // the default rule
 void hj_radix_config(hj_ctx_t *ctx, uint64_t nr, uint32_t *bits, uint32_t *passes, uint32_t *pass1 = nullptr);
-// one-pass radix bits for |R| (batches, serving): the tuned class, else the
-// default rule's bits capped at 8; *variant gets the class's scatter shape
+// one-pass radix bits for |R| (batches, serving): the tuned class's (the
+// default rule's when untuned, or when tuning left a build side above 2048
+// tuples unpartitioned), at most 8; *variant gets the class's scatter shape
 uint32_t hj_radix_bits_onepass(hj_ctx_t *ctx, uint64_t nr, uint32_t *variant = nullptr);
 extern "C" bool hj_algo_is_lp(const char *algo, bool *radix);
 

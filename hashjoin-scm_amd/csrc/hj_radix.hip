@@ -2170,15 +2170,19 @@ void hj_radix_config(hj_ctx_t *ctx, uint64_t nr, uint32_t *bits, uint32_t *passe
 uint32_t hj_radix_bits_onepass(hj_ctx_t *ctx, uint64_t nr, uint32_t *variant) {
 	const uint32_t c = hj_log2_ceil(nr ? nr : 1);
 	if (variant) *variant = 0;
-	if (ctx && ctx->tuned && c < 40 && ctx->tune_bits1[c] != 0xff) {
-		if (variant) *variant = ctx->tune_var1[c];
-		return ctx->tune_bits1[c];
-	}
-	// untuned: the default bits capped at 8, since a one-pass scatter's runs
-	// get short above that (C4's 2048-join batch: 78 ms at <= 8 bits against
-	// 108 ms with the default's up to 9-10 bits)
+	// 8 bits are the ceiling: a one-pass scatter's runs get short above that
+	// and its write-combining lines take more LDS than two workgroups per CU
+	// leave (C4's 2048-join batch: 78 ms untuned at <= 8 bits against 108 ms
+	// with the default's up to 9-10 bits; tuned 65-68 ms at <= 7-8 bits).
+	// A tuned class may leave a single join unpartitioned (one table); in a
+	// batch, whose join table is sized for ~2K-tuple partitions, a build side
+	// above 2048 tuples then takes the default's bits instead.
 	uint32_t b = 0, p = 0;
 	hj_radix_default(nr, &b, &p);
+	if (ctx && ctx->tuned && c < 40 && ctx->tune_bits1[c] != 0xff) {
+		if (variant) *variant = ctx->tune_var1[c];
+		if (ctx->tune_bits1[c] || nr <= 2048) b = ctx->tune_bits1[c];
+	}
 	return std::min(b, 8u);
 }
 
