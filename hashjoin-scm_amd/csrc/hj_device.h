@@ -67,6 +67,10 @@ __device__ __forceinline__ void hj_store_stream8(uint2 *p, uint32_t a, uint32_t 
 	hj_u32x2 t = {a, b};
 	__builtin_nontemporal_store(t, (HJ_GLOBAL hj_u32x2 *)p);
 }
+// 4-byte form for the positional words of the first radix pass.
+__device__ __forceinline__ void hj_store_stream4(uint32_t *p, uint32_t a) {
+	__builtin_nontemporal_store(a, (HJ_GLOBAL uint32_t *)p);
+}
 // Plain (cached) global loads.
 __device__ __forceinline__ uint4 hj_load(const uint4 *p) {
 	const hj_u32x4 v = *(const HJ_GLOBAL hj_u32x4 *)p;

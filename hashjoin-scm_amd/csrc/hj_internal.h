@@ -65,7 +65,11 @@ struct hj_ctx {
 	// two-pass radix joins of 16-byte tuples: pass B may write 8-byte
 	// {key, row_id} tuples when every key and row_id fits 32 bits
 	bool narrow_ok = false;
-	uint8_t *tile_narrow = nullptr;  // per pass-A tile: stored as 8-byte tuples
+	// ... and pass A may store a tile whose row ids are its positions + 1 as
+	// 4-byte {key >> first digit, index in the tile} words (the row id is
+	// restored from the position in pass B)
+	bool dense_ok = false;
+	uint8_t *tile_narrow = nullptr;  // per pass-A tile: 0 16-byte, 1 8-byte, 2 4-byte tuples
 	hj_buf pairs;      // the pair buffer of late-materialising joins
 	hj_buf keyrid[2];  // {key, row_id} copies of wide R and S
 	hj_buf tpch[5];    // TPC-H: staged part / lineitem tuples, selection counts, contiguous outputs
@@ -142,7 +146,8 @@ struct hj_stats {
 	unsigned long long retrieved;  // late materialisation: retrieve checksum
 	unsigned long long retrieved_ok; // pairs whose row ids lie inside R and S
 	unsigned long long narrow_tuples; // tuples of pass-A tiles stored as 8 bytes
-	unsigned long long pad1[2];
+	unsigned long long dense_tuples; // tuples of pass-A tiles stored as 4-byte positional words
+	unsigned long long pad1;
 	unsigned long long xq[8];      // two-pass join: work cursor of each XCD's queue
 };
 

@@ -621,15 +621,20 @@ template <int WV, int NT, int ITEMS>
 __global__ __launch_bounds__(NT) void k_sortchunk(const hj_seg *__restrict__ segs, const uint32_t *__restrict__ cmap,
                                                   hj_stats *__restrict__ st, uint32_t shift, uint32_t bits,
                                                   uint4 *__restrict__ out, uint16_t *__restrict__ run,
-                                                  uint64_t *__restrict__ cstart, bool narrow_ok,
+                                                  uint64_t *__restrict__ cstart, bool narrow_ok, bool dense_ok,
                                                   uint8_t *__restrict__ tnarrow) {
 	constexpr int TILE = NT * ITEMS;
+	constexpr uint32_t LT = __builtin_ctz(TILE);  // bits of a tuple's index in its tile
 	extern __shared__ uint4 lds_u4[];
 	const uint32_t F = 1u << bits, fmask = F - 1;
 	uint4 *tile = lds_u4;
 	uint32_t *tcnt = (uint32_t *)(tile + TILE * WV);
 	uint32_t *tbase = tcnt + F;
 	uint32_t *red = tbase + F + 1;
+	__shared__ uint32_t tflag;  // per tile: 1 a wide tuple, 2 a row id that is not position + 1
+	// a positional word holds key >> bits above the LT index bits: keys
+	// below 2^(32 - LT + bits) fit
+	const uint32_t kshift = bits + 32 - LT;
 	const uint32_t total = (uint32_t)st->total_chunks;
 	const uint32_t per = (F + NT - 1) / NT;
 	const uint32_t grid = gridDim.x;
@@ -638,18 +643,21 @@ __global__ __launch_bounds__(NT) void k_sortchunk(const hj_seg *__restrict__ seg
 	// input tuples are WV words apart, or in_wv when only the heads of wider
 	// tuples are read
 	uint32_t sw = WV;
+	uint64_t spos = 0;  // position of the tile's first tuple in its relation
 	auto tile_of = [&](uint32_t c, const uint4 *&src, uint64_t &pos, uint32_t &n) {
 		if (c >= total) {
 			src = out;
 			pos = 0;
 			n = 0;
 			sw = WV;
+			spos = 0;
 			return;
 		}
 		const hj_seg &g = segs[cmap[c]];
 		const uint64_t cl = c - g.chunk_base;
 		sw = g.in_wv ? g.in_wv : WV;
 		pos = g.out_start + cl * TILE;
+		spos = cl * TILE;
 		src = g.in + cl * TILE * sw;
 		n = (uint32_t)min((uint64_t)TILE, g.len - cl * TILE);
 	};
@@ -667,27 +675,39 @@ __global__ __launch_bounds__(NT) void k_sortchunk(const hj_seg *__restrict__ seg
 		}
 	};
 	bool wide = false;
-	unsigned long long narrow_cnt = 0;  // thread 0: tuples of this workgroup's narrow tiles
+	unsigned long long narrow_cnt = 0, dense_cnt = 0;  // thread 0: tuples of this workgroup's 8- and 4-byte tiles
 	auto do_tile = [&](uint4 (&v)[ITEMS][WV], uint32_t c) {
 		const uint4 *src;
 		uint64_t pos;
 		uint32_t n;
 		tile_of(c, src, pos, n);
 		uint32_t dig[ITEMS], rank[ITEMS];
-		bool tw = false;
+		bool tw = false, tp = false;
 #pragma unroll
 		for (int k = 0; k < ITEMS; k++) {
 			const uint32_t idx = k * NT + threadIdx.x;
 			if (idx < n) {
 				dig[k] = (uint32_t)(hj_key_of(v[k][0]) >> shift) & fmask;
 				rank[k] = atomicAdd(&tcnt[dig[k]], 1u);
-				if (WV == 1) tw |= (v[k][0].y | v[k][0].w) != 0;
+				if (WV == 1) {
+					tw |= (v[k][0].y | v[k][0].w) != 0;
+					tp |= v[k][0].z != (uint32_t)(spos + idx + 1) || (kshift < 32 && (v[k][0].x >> kshift) != 0);
+				}
 			}
 		}
 		// a 16-byte tile whose keys and row ids all fit 32 bits is stored as
-		// 8-byte {key, row_id} tuples in the first half of its range
-		const bool any_wide = __syncthreads_or(tw);  // also the barrier after the counts
-		const bool tile_narrow = WV == 1 && narrow_ok && !any_wide;
+		// 8-byte {key, row_id} tuples in the first half of its range; one
+		// whose row ids are also its relation positions + 1 (and whose keys
+		// fit beside the index) as 4-byte {key >> bits, index} words in the
+		// first quarter: pass B restores the row id from the position
+		if (WV == 1) {
+			const unsigned fw = (__any(tw) ? 1u : 0u) | (__any(tp) ? 2u : 0u);
+			if (fw && (threadIdx.x & (HJ_WAVE - 1)) == 0) atomicOr(&tflag, fw);
+		}
+		__syncthreads();  // the counts and tflag are complete
+		const uint32_t tf = WV == 1 ? tflag : 1u;
+		const bool tile_narrow = WV == 1 && narrow_ok && !(tf & 1u);
+		const bool tile_dense = tile_narrow && dense_ok && !(tf & 2u);
 		wide |= tw;
 		{
 			const uint32_t p0 = threadIdx.x * per;
@@ -707,8 +727,9 @@ __global__ __launch_bounds__(NT) void k_sortchunk(const hj_seg *__restrict__ seg
 				run[(uint64_t)c * (F + 1) + F] = (uint16_t)n;
 				cstart[c] = pos;
 				if (WV == 1 && narrow_ok) {
-					tnarrow[c] = tile_narrow;
+					tnarrow[c] = tile_dense ? 2 : tile_narrow ? 1 : 0;
 					narrow_cnt += tile_narrow ? n : 0u;
+					dense_cnt += tile_dense ? n : 0u;
 				}
 			}
 		}
@@ -717,12 +738,15 @@ __global__ __launch_bounds__(NT) void k_sortchunk(const hj_seg *__restrict__ seg
 		// below reads consecutive 8-byte LDS words (16-byte slots read half by
 		// half hit the same banks from several lanes)
 		uint2 *tile8 = (uint2 *)tile;
+		uint32_t *tile4 = (uint32_t *)tile;
 #pragma unroll
 		for (int k = 0; k < ITEMS; k++) {
 			const uint32_t idx = k * NT + threadIdx.x;
 			if (idx < n) {
 				const uint32_t q = tbase[dig[k]] + rank[k];
-				if (WV == 1 && tile_narrow) {
+				if (WV == 1 && tile_dense) {
+					tile4[q] = (v[k][0].x >> bits) << LT | idx;
+				} else if (WV == 1 && tile_narrow) {
 					tile8[q] = make_uint2(v[k][0].x, v[k][0].z);
 				} else {
 #pragma unroll
@@ -731,7 +755,10 @@ __global__ __launch_bounds__(NT) void k_sortchunk(const hj_seg *__restrict__ seg
 			}
 		}
 		__syncthreads();
-		if (tile_narrow) {
+		if (tile_dense) {
+			uint32_t *dst4 = (uint32_t *)(out + pos);
+			for (uint32_t u = threadIdx.x; u < n; u += NT) hj_store_stream4(&dst4[u], tile4[u]);
+		} else if (tile_narrow) {
 			uint2 *dst8 = (uint2 *)(out + pos);
 			for (uint32_t u = threadIdx.x; u < n; u += NT) {
 				const uint2 t = tile8[u];
@@ -743,9 +770,11 @@ __global__ __launch_bounds__(NT) void k_sortchunk(const hj_seg *__restrict__ seg
 		}
 		load(v, c + 2 * grid);
 		for (uint32_t p = threadIdx.x; p < F; p += NT) tcnt[p] = 0;
+		if (threadIdx.x == 0) tflag = 0;
 		__syncthreads();
 	};
 	for (uint32_t p = threadIdx.x; p < F; p += NT) tcnt[p] = 0;
+	if (threadIdx.x == 0) tflag = 0;
 	__syncthreads();
 	uint4 va[ITEMS][WV], vb[ITEMS][WV];
 	load(va, blockIdx.x);
@@ -759,6 +788,7 @@ __global__ __launch_bounds__(NT) void k_sortchunk(const hj_seg *__restrict__ seg
 	if (WV == 1 && __any(wide) && (threadIdx.x & (HJ_WAVE - 1)) == 0) atomicOr(&st->flags, HJ_DF_WIDE);
 	// one atomic per workgroup (one per tile on a single address serialises)
 	if (WV == 1 && threadIdx.x == 0 && narrow_cnt) atomicAdd(&st->narrow_tuples, narrow_cnt);
+	if (WV == 1 && threadIdx.x == 0 && dense_cnt) atomicAdd(&st->dense_tuples, dense_cnt);
 }
 
 // Two-pass joins partition without a histogram pass.  Pass A sorts every
@@ -819,6 +849,8 @@ struct hj_gtile {
 	uint32_t chunk_base;  // global id of the segment's first pass-A tile
 	uint32_t ncov;        // pass-A runs (from `first` on) that start before the tile ends
 };
+// (48 bytes: pass B holds six records in scalar registers)
+static_assert(sizeof(hj_gtile) == 48, "pass-B tile record size");
 
 // gt[] = the pass-B tiles sorted by (segment, tile index k, digit): tiles of
 // neighbouring digits at the same k read the same pass-A tiles, so
@@ -921,12 +953,12 @@ __global__ void k_first_run(const hj_seg *__restrict__ segs1, const hj_seg *__re
 // source is one 8-byte tuple: 8-byte loads and an 8-byte LDS tile.  Both
 // instances are launched for 16-byte tuples (which case holds is known on
 // the device only) and the one that does not match returns at once.
-template <int WV, int NT, int ITEMS, bool NIN>
-__global__ __launch_bounds__(NT) void k_sortchunk_g(
+template <int WV, int NT, int ITEMS, bool NIN, bool DENSE>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_sortchunk_g(
     const hj_gtile *__restrict__ gt, const uint32_t *__restrict__ hist, const uint16_t *__restrict__ beg_t,
     const hj_stats *__restrict__ st, uint32_t b1, uint32_t bits,
     const uint4 *__restrict__ in, uint4 *__restrict__ out, uint16_t *__restrict__ run, uint64_t *__restrict__ cstart,
-    bool narrow_ok, const uint8_t *__restrict__ tnarrow, uint32_t tileA) {
+    bool narrow_ok, const uint8_t *__restrict__ tnarrow, uint32_t tileA, uint32_t grid2, uint32_t pmode) {
 	constexpr int TILE = NT * ITEMS;
 	extern __shared__ uint4 lds_u4[];
 	const uint32_t F = 1u << bits, fmask = F - 1;
@@ -936,16 +968,27 @@ __global__ __launch_bounds__(NT) void k_sortchunk_g(
 	// the NIN instance does the work.
 	const bool narrow = WV == 1 && narrow_ok && !(st->flags & HJ_DF_WIDE);
 	if (NIN != narrow) return;
+	// pmode 1 / 2: this instance runs only when pass A did / did not write
+	// positional words (a DENSE and a plain instance are both launched)
+	if ((pmode == 1 && st->dense_tuples == 0) || (pmode == 2 && st->dense_tuples != 0)) return;
 	using tv = typename std::conditional<NIN, uint2, uint4>::type;
 	tv *tile = (tv *)lds_u4;
-	uint32_t *src0 = (uint32_t *)(tile + TILE * WV), *src1 = src0 + TILE;
-	uint32_t *tcnt = src1 + TILE;
+	// three source buffers: tile h is staged, loaded and sorted from
+	// src[h % 3], so its sources are still there when it is sorted
+	uint32_t *src0 = (uint32_t *)(tile + TILE * WV), *src1 = src0 + TILE, *src2 = src1 + TILE;
+	uint32_t *tcnt = src2 + TILE;
 	uint32_t *tbase = tcnt + F;
 	uint32_t *red = tbase + F + 1;
 	__shared__ uint32_t rbeg[NT], rsp[NT];  // long-run staging: run i's first tile position and source
 	const uint32_t total = (uint32_t)st->total_chunks;
 	const uint32_t per = (F + NT - 1) / NT;
-	const uint32_t grid = gridDim.x;
+	// When most tuples came as positional words, a tile gathers half the
+	// bytes and three workgroups per CU keep their tiles in flight within an
+	// XCD's L2; otherwise the first grid2 (two per CU) work and the rest
+	// leave (workgroups are dispatched breadth-first over the CUs).
+	uint32_t grid = gridDim.x;
+	if (DENSE && 2 * st->dense_tuples < st->narrow_tuples) grid = min(grid, grid2);
+	if (blockIdx.x >= grid) return;
 	for (uint32_t p = threadIdx.x; p < F; p += NT) tcnt[p] = 0;
 
 	// Staging is split in two so that its table reads are issued a tile
@@ -958,6 +1001,13 @@ __global__ __launch_bounds__(NT) void k_sortchunk_g(
 	struct pre_t {
 		uint32_t h0, h1, b, nar;
 	};
+	// A staged source is its offset in the pass-A segment; with narrow_ok,
+	// bit 31 marks a pass-A tile stored in 8 or 4 bytes per tuple and, in the
+	// DENSE instances (launched when pass A may have written positional
+	// words: segments below 2^30), bit 30 one of 4-byte positional words.
+	auto src_code = [](uint32_t nar) -> uint32_t { return nar == 2 ? 3u << 30 : 1u << 31; };
+	const uint32_t omask = DENSE ? 0x3fffffffu : 0x7fffffffu;
+	const uint32_t LTA = __ffs(tileA) - 1;  // index bits of a positional word
 	// (no branch around the loads: a load whose result merges with another
 	// value makes the compiler wait for every load in flight; an empty
 	// record has nchunks = 1 and reads entry 0 of the tables)
@@ -978,7 +1028,7 @@ __global__ __launch_bounds__(NT) void k_sortchunk_g(
 	                   uint32_t nar) {
 		const uint64_t lo = max(H, pos), hi = min(H1, pos + n);
 		uint32_t sp = ii * tileA + b + (uint32_t)(lo - H);
-		if (WV == 1 && narrow_ok && lo < hi && nar) sp |= 1u << 31;  // nar is unset unless narrow_ok
+		if (WV == 1 && narrow_ok && lo < hi && nar) sp |= src_code(nar);  // nar is unset unless narrow_ok
 		for (uint64_t q = lo; q < hi; q++) srcb[q - pos] = sp + (uint32_t)(q - lo);
 	};
 	// A thread writes its own run's sources, so a tile made of a few long
@@ -996,7 +1046,7 @@ __global__ __launch_bounds__(NT) void k_sortchunk_g(
 		if (t.ncov <= 32) {  // uniform
 			const uint64_t lo = max(p.h0, pos);
 			uint32_t sp = (t.first + threadIdx.x) * tileA + p.b + (uint32_t)(lo - p.h0);
-			if (WV == 1 && narrow_ok && p.nar) sp |= 1u << 31;
+			if (WV == 1 && narrow_ok && p.nar) sp |= src_code(p.nar);
 			rbeg[threadIdx.x] = mine ? (uint32_t)(min(max(p.h0, pos), pos + n) - pos) : n;
 			rsp[threadIdx.x] = sp;
 			__syncthreads();
@@ -1030,11 +1080,19 @@ __global__ __launch_bounds__(NT) void k_sortchunk_g(
 	// (pass-A tile stored narrow, upper half of the loaded pair) and v the
 	// raw 16-byte load; unpack() picks the tuple out when the tile is sorted,
 	// since picking it here would wait for every load before the next issues.
-	auto load = [&](tv (&v)[ITEMS][WV], uint32_t &sel, const hj_gtile &t, const uint32_t *srcb) {
+	// A source e: offset sp = e & omask in the pass-A segment, pass-A tile
+	// start t0 = sp & ~(tileA - 1).  A tuple at offset o of that tile lies at
+	// 8-byte index 2*t0 + o = sp + t0 when stored narrow and at 4-byte index
+	// 4*t0 + o = sp + 3*t0 when stored as a positional word; the load of the
+	// aligned 8 (16) bytes holding it keeps one load per tuple in every case,
+	// and unpack() reads e again to pick the tuple out (picking it at load
+	// time would wait for every load before the next issues).  32-bit index
+	// arithmetic: offsets are below 2^31, and below 2^30 where positional
+	// words occur.
+	auto load = [&](tv (&v)[ITEMS][WV], const hj_gtile &t, const uint32_t *srcb) {
 		const uint32_t n = t.n;
 		// source offsets are relative to the pass-A segment's first tuple
 		const uint4 *base = in + t.base * WV;
-		sel = 0;
 #pragma unroll
 		for (int k = 0; k < ITEMS; k++) {
 			const uint32_t f = min((uint32_t)(k * NT) + threadIdx.x, n ? n - 1 : 0u);
@@ -1042,18 +1100,22 @@ __global__ __launch_bounds__(NT) void k_sortchunk_g(
 			// cached loads: a run's neighbours belong to tiles of nearby digits
 			// that the same XCD works on at the same time
 			if constexpr (NIN) {
-				const uint64_t sp = e & 0x7fffffffu;
-				v[k][0] = hj_load8((const uint2 *)base + sp + (sp & ~(uint64_t)(tileA - 1)));
+				const uint32_t sp = e & omask, t0 = sp & ~(tileA - 1);
+				uint32_t i8 = sp + t0;
+				if constexpr (DENSE) {
+					const uint32_t db = (e >> 30) & 1u;
+					i8 += (((sp + 3u * t0) >> 1) - i8) & (0u - db);
+				}
+				v[k][0] = hj_load8((const uint2 *)base + i8);
 			} else if (WV == 1) {
-				// an 8-byte tuple at offset o of pass-A tile t (tileA tuples, a
-				// power of two) lies at 8-byte index t*2*tileA + o = sp +
-				// t*tileA; the 16-byte load of its pair keeps one load per tuple
-				// either way
-				const bool nw = narrow_ok && (e >> 31);
-				const uint64_t sp = narrow_ok ? (e & 0x7fffffffu) : e;
-				const uint64_t u8 = sp + (sp & ~(uint64_t)(tileA - 1));
-				v[k][0] = hj_load(base + (nw ? u8 >> 1 : sp));
-				sel |= ((nw ? 1u : 0u) | (nw && (u8 & 1) ? 2u : 0u)) << (2 * k);
+				const uint32_t nw = narrow_ok ? e >> 31 : 0u;
+				const uint32_t sp = narrow_ok ? (e & omask) : e, t0 = sp & ~(tileA - 1);
+				uint32_t i16 = sp + ((((sp + t0) >> 1) - sp) & (0u - nw));
+				if constexpr (DENSE) {
+					const uint32_t db = (e >> 30) & nw;
+					i16 += (((sp + 3u * t0) >> 2) - i16) & (0u - db);
+				}
+				v[k][0] = hj_load(base + i16);
 			} else {
 				const uint64_t sp = e;
 #pragma unroll
@@ -1061,29 +1123,62 @@ __global__ __launch_bounds__(NT) void k_sortchunk_g(
 			}
 		}
 	};
-	auto unpack = [&](tv (&v)[ITEMS][WV], uint32_t sel) {
-		if constexpr (!NIN && WV == 1) {
+	// Pass B leaves every key shifted right by the first digit's b1 bits
+	// (key >> b1): all keys of a first-level partition share those bits, so
+	// equality and the join's bucket (key >> B = (key >> b1) >> (B - b1)) are
+	// unchanged, and a positional word w holds exactly that, w >> LTA; its
+	// row id is its pass-A tile start + 1 + (w & (tileA - 1)).
+	auto unpack = [&](tv (&v)[ITEMS][WV], const hj_gtile &t, const uint32_t *srcb) {
+		const uint32_t n = t.n;
 #pragma unroll
-			for (int k = 0; k < ITEMS; k++) {
+		for (int k = 0; k < ITEMS; k++) {
+			if constexpr (NIN) {
+				if constexpr (DENSE) {
+					const uint32_t e = n ? srcb[min((uint32_t)(k * NT) + threadIdx.x, n ? n - 1 : 0u)] : 0;
+					const uint32_t sp = e & omask, t0 = sp & ~(tileA - 1);
+					// the word's 8-byte pair: index sp + 3*t0, whose parity is sp's
+					const uint32_t w = (sp & 1) ? v[k][0].y : v[k][0].x;
+					const bool dn = (e >> 30) & 1u;
+					v[k][0].y = dn ? t0 + 1 + (w & (tileA - 1)) : v[k][0].y;
+					v[k][0].x = dn ? w >> LTA : v[k][0].x >> b1;
+				} else {
+					v[k][0].x >>= b1;
+				}
+			} else if constexpr (WV == 1) {
+				const uint32_t e = n ? srcb[min((uint32_t)(k * NT) + threadIdx.x, n ? n - 1 : 0u)] : 0;
+				const uint32_t nw = narrow_ok ? e >> 31 : 0u;
+				const uint32_t sp = narrow_ok ? (e & omask) : e, t0 = sp & ~(tileA - 1);
 				const uint4 h = v[k][0];
-				const uint32_t b = sel >> (2 * k);
-				if (b & 1) v[k][0] = (b & 2) ? make_uint4(h.z, 0u, h.w, 0u) : make_uint4(h.x, 0u, h.y, 0u);
+				uint4 o;
+				if (DENSE && nw && ((e >> 30) & 1u)) {
+					const uint32_t q = (sp + 3u * t0) & 3;
+					const uint32_t w = q == 0 ? h.x : q == 1 ? h.y : q == 2 ? h.z : h.w;
+					o = make_uint4(w >> LTA, 0u, t0 + 1 + (w & (tileA - 1)), 0u);
+				} else if (nw) {
+					o = ((sp + t0) & 1) ? make_uint4(h.z >> b1, 0u, h.w, 0u) : make_uint4(h.x >> b1, 0u, h.y, 0u);
+				} else {
+					const uint64_t key = hj_key_of(h) >> b1;
+					o = make_uint4((uint32_t)key, (uint32_t)(key >> 32), h.z, h.w);
+				}
+				v[k][0] = o;
+			} else {
+				const uint64_t key = hj_key_of(v[k][0]) >> b1;
+				v[k][0].x = (uint32_t)key;
+				v[k][0].y = (uint32_t)(key >> 32);
 			}
 		}
 	};
-	auto sort_store = [&](tv (&v)[ITEMS][WV], uint32_t sel, const hj_gtile &t) {
+	auto sort_store = [&](tv (&v)[ITEMS][WV], const hj_gtile &t, const uint32_t *srcb) {
 		const uint64_t pos = t.pos;
 		const uint32_t n = t.n, c = t.c;
-		unpack(v, sel);
+		unpack(v, t, srcb);
 		uint32_t dig[ITEMS], rank[ITEMS];
 #pragma unroll
 		for (int k = 0; k < ITEMS; k++) {
 			const uint32_t idx = k * NT + threadIdx.x;
 			if (idx < n) {
-				uint64_t key;
-				if constexpr (NIN) key = v[k][0].x;
-				else key = hj_key_of(v[k][0]);
-				dig[k] = (uint32_t)(key >> b1) & fmask;
+				// keys arrive shifted by b1 (unpack)
+				dig[k] = v[k][0].x & fmask;
 				rank[k] = atomicAdd(&tcnt[dig[k]], 1u);
 			}
 		}
@@ -1173,36 +1268,41 @@ __global__ __launch_bounds__(NT) void k_sortchunk_g(
 	};
 	if (total == 0) return;
 	// Half step h (tile h is in flight into v[h&1], tile h+1 is staged in
-	// src[(h+1)&1], tile h+2's tables are prefetched): prefetch tile h+3's
-	// tables, load tile h+1, stage tile h+2 into src[h&1] (free: its last
-	// reader, the load of tile h, is behind the barriers of the sort of tile
-	// h-1), then sort tile h.  Stores count in the same hardware counter as
-	// loads, so this order is the one in which every wait is for exactly the
-	// oldest loads it needs: the staging waits for a prefetch issued a half
-	// step earlier, the sort for a load issued a half step earlier, and
-	// neither for the loads issued after them.  Records are read two half
-	// steps before their prefetch.
+	// src[(h+1)%3], tile h+2's tables are prefetched): prefetch tile h+3's
+	// tables, load tile h+1, stage tile h+2 into src[(h+2)%3] (free: its last
+	// reader, the sort of tile h-1, is behind that sort's barriers), then
+	// sort tile h, which reads its sources in src[h%3] again.  Stores count
+	// in the same hardware counter as loads, so this order is the one in
+	// which every wait is for exactly the oldest loads it needs: the staging
+	// waits for a prefetch issued a half step earlier, the sort for a load
+	// issued a half step earlier, and neither for the loads issued after
+	// them.  Records are read two half steps before their prefetch.
 	tv va[ITEMS][WV], vb[ITEMS][WV];
 	hj_gtile ta = tile_at(0), tb = tile_at(1), tc = tile_at(2), td = tile_at(3);
-	uint32_t sa, sb;
 	stage(prefetch(ta), ta, src0);
 	const pre_t p1 = prefetch(tb);
 	pre_t p2 = prefetch(tc), p3;
 	__syncthreads();
-	load(va, sa, ta, src0);
+	load(va, ta, src0);
 	stage(p1, tb, src1);
 	__syncthreads();
 	for (uint32_t j = 0; j * grid < total; j += 2) {
+		// ta, tb in src0, src1; src2 free
 		const hj_gtile te = tile_at(j + 4);
 		p3 = prefetch(td);
-		load(vb, sb, tb, src1);
-		stage(p2, tc, src0);
-		sort_store(va, sa, ta);
+		load(vb, tb, src1);
+		stage(p2, tc, src2);
+		sort_store(va, ta, src0);
 		const hj_gtile tf = tile_at(j + 5);
 		p2 = prefetch(te);
-		load(va, sa, tc, src0);
-		stage(p3, td, src1);
-		sort_store(vb, sb, tb);  // an empty tile past the end changes nothing
+		load(va, tc, src2);
+		stage(p3, td, src0);
+		sort_store(vb, tb, src1);  // an empty tile past the end changes nothing
+		// tc, td in src2, src0; src1 free
+		uint32_t *const s0 = src0;
+		src0 = src2;
+		src2 = src1;
+		src1 = s0;
 		ta = tc;
 		tb = td;
 		tc = te;
@@ -1289,6 +1389,9 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
 	// pass B chose is known on the device only, so both instances are
 	// launched and the one that does not match returns at once.
 	static_assert(!NARROW || WV == 1, "narrow tuples are 16-byte tuples packed");
+	// pass B leaves keys shifted right by b1 bits: bucket (key >> B) is then
+	// (key >> b1) >> hs
+	const uint32_t hs = B - b1;
 	if (NARROW != (WV == 1 && narrow_ok && !(st->flags & HJ_DF_WIDE))) return;
 	constexpr bool TWO = TK == HJ_TK_HR || TK == HJ_TK_HRO;  // count + place builds
 	constexpr uint32_t NW = NT / HJ_WAVE;
@@ -1374,7 +1477,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
 			kk[u] = key_of(t[u]);
 			pp[u] = pay_of(t[u]);
 		}
-		tab.template probe<(int)RU>(kk, pp, live, B, acc);
+		tab.template probe<(int)RU>(kk, pp, live, hs, acc);
 	};
 
 	for (;;) {
@@ -1435,9 +1538,9 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
 							const uint32_t f = f0 + u * HJ_WAVE + lane;
 							if (f >= fe) continue;
 							const uint64_t key = key_of(t[u]);
-							if (mode == 1) tab.count(key, B);
-							else if (mode == 2) tab.place(key, pay_of(t[u]), B);
-							else tab.put(m + f, key, pay_of(t[u]), B);
+							if (mode == 1) tab.count(key, hs);
+							else if (mode == 2) tab.place(key, pay_of(t[u]), hs);
+							else tab.put(m + f, key, pay_of(t[u]), hs);
 						}
 					}
 				}
@@ -1466,7 +1569,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
 					oversize = true;  // the host recomputes this join exactly
 					break;
 				}
-				tab.lp_insert(m, B, NT, dup);
+				tab.lp_insert(m, hs, NT, dup);
 				__syncthreads();
 			}
 			// ---- probe the tile's runs, NT chunks at a time
@@ -1521,6 +1624,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(6))) void k_
                                                const uint64_t *__restrict__ tprefix, uint32_t npairs, uint32_t B,
                                                uint32_t cpt, uint32_t cap, hj_stats *st, bool narrow_ok) {
 	if (!(narrow_ok && !(st->flags & HJ_DF_WIDE))) return;  // k_join2<1, ...> joins 16-byte tuples
+	const uint32_t hs = B - b1;  // keys arrive shifted right by b1 bits (k_sortchunk_g)
 	constexpr uint32_t NW = NT / HJ_WAVE;
 	constexpr uint32_t RU = 4;               // tuples per lane per step
 	constexpr uint32_t STEP = HJ_WAVE * RU;  // tuples per wave per step
@@ -1594,7 +1698,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(6))) void k_
 		uint32_t c[RU];
 #pragma unroll
 		for (uint32_t u = 0; u < RU; u++)
-			c[u] = f0 + u * HJ_WAVE + lane < fe ? heads[(t[u].x >> B) & hmask] : 0u;
+			c[u] = f0 + u * HJ_WAVE + lane < fe ? heads[(t[u].x >> hs) & hmask] : 0u;
 		bool any = true;
 		while (any) {
 			any = false;
@@ -1672,7 +1776,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(6))) void k_
 							if (f >= fe) continue;
 							const uint32_t i = m + f;
 							kv[i] = (uint64_t)t[u].x | (uint64_t)t[u].y << 32;
-							next[i] = (uint16_t)atomicExch(&heads[(t[u].x >> B) & hmask], i + 1);
+							next[i] = (uint16_t)atomicExch(&heads[(t[u].x >> hs) & hmask], i + 1);
 						}
 					}
 				}
@@ -1988,7 +2092,7 @@ static void launch_sortchunk_t(hj_ctx_t *ctx, unsigned grid, size_t lds, const h
 	allow_lds(ctx, (const void *)kfn);
 	grid = std::max(1u, std::min(grid, resident_grid(ctx, (const void *)kfn, NT, lds)));
 	hipLaunchKernelGGL(kfn, dim3(grid), dim3(NT), lds, ctx->stream, segs, cmap, (hj_stats *)ctx->stats.p, shift,
-	                   bits, (uint4 *)out, run, cst, ctx->narrow_ok, ctx->tile_narrow);
+	                   bits, (uint4 *)out, run, cst, ctx->narrow_ok, ctx->dense_ok, ctx->tile_narrow);
 }
 
 // Pass-A tile of 16-byte tuples (threads x tuples per thread): long enough
@@ -2031,33 +2135,51 @@ static void launch_sortchunk(hj_ctx_t *ctx, uint32_t wv, const part_cfg &pc, uns
 	}
 }
 
+template <int WV, int NT, int ITEMS, bool DENSE, bool NIN>
+static void launch_sortchunk_g_i(hj_ctx_t *ctx, unsigned grid, size_t lds, const hj_gtile *gt, const uint32_t *hist,
+                                 const uint16_t *beg_t, uint32_t b1, uint32_t bits, const void *in, void *out,
+                                 uint16_t *run, uint64_t *cst, uint32_t tileA, uint32_t pmode) {
+	// grid: all resident workgroups up to the cap `grid`, a multiple of the
+	// 8 XCDs (the kernel's mapping); DENSE instances launch half as many
+	// again, and use them only for mostly positional inputs
+	const unsigned grid2 = std::max(8u, grid & ~7u);
+	if (DENSE && grid != ~0u) grid += grid / 2;
+	auto kfn = k_sortchunk_g<WV, NT, ITEMS, NIN, DENSE>;
+	// the all-narrow instance's LDS tile holds 8-byte tuples
+	const size_t l = NIN ? lds - (size_t)NT * ITEMS * 8 : lds;
+	allow_lds(ctx, (const void *)kfn);
+	grid = std::max(8u, std::min(grid, resident_grid(ctx, (const void *)kfn, NT, l)) & ~7u);
+	hipLaunchKernelGGL(kfn, dim3(grid), dim3(NT), l, ctx->stream, gt, hist, beg_t, (const hj_stats *)ctx->stats.p, b1,
+	                   bits, (const uint4 *)in, (uint4 *)out, run, cst, ctx->narrow_ok,
+	                   (const uint8_t *)ctx->tile_narrow, tileA, std::min(grid2, grid), pmode);
+}
+
+// Which instance does the work is known on the device only: the all-narrow
+// (NIN) one when no pass-A tile was wide, the 16-byte one otherwise; when
+// pass A may have written positional words, the all-narrow work goes to the
+// instance that decodes them if it did and to the plain one if not (which
+// is faster on 8-byte sources).  The others return at once.
 template <int WV, int NT, int ITEMS>
 static void launch_sortchunk_g_t(hj_ctx_t *ctx, unsigned grid, size_t lds, const hj_gtile *gt, const uint32_t *hist,
                                  const uint16_t *beg_t, uint32_t b1, uint32_t bits, const void *in, void *out,
                                  uint16_t *run, uint64_t *cst, uint32_t tileA) {
-	// grid: all resident workgroups up to the cap `grid`, a multiple of the
-	// 8 XCDs (the kernel's mapping)
-	if constexpr (WV == 1) if (ctx->narrow_ok) {
-		// all-narrow instance: 8-byte LDS tile (the other returns at once)
-		auto kn = k_sortchunk_g<WV, NT, ITEMS, true>;
-		const size_t ln = lds - (size_t)NT * ITEMS * 8;
-		allow_lds(ctx, (const void *)kn);
-		const unsigned gn = std::max(8u, std::min(grid, resident_grid(ctx, (const void *)kn, NT, ln)) & ~7u);
-		hipLaunchKernelGGL(kn, dim3(gn), dim3(NT), ln, ctx->stream, gt, hist, beg_t, (const hj_stats *)ctx->stats.p,
-		                   b1, bits, (const uint4 *)in, (uint4 *)out, run, cst, ctx->narrow_ok,
-		                   (const uint8_t *)ctx->tile_narrow, tileA);
+#define GI(D, N, M) 	launch_sortchunk_g_i<WV, NT, ITEMS, D, N>(ctx, grid, lds, gt, hist, beg_t, b1, bits, in, out, run, cst, tileA, M)
+	if constexpr (WV == 1) {
+		if (ctx->dense_ok) {
+			GI(true, true, 1);
+			GI(false, true, 2);
+			GI(true, false, 0);
+			return;
+		}
+		if (ctx->narrow_ok) GI(false, true, 0);
 	}
-	auto kfn = k_sortchunk_g<WV, NT, ITEMS, false>;
-	allow_lds(ctx, (const void *)kfn);
-	grid = std::max(8u, std::min(grid, resident_grid(ctx, (const void *)kfn, NT, lds)) & ~7u);
-	hipLaunchKernelGGL(kfn, dim3(grid), dim3(NT), lds, ctx->stream, gt, hist, beg_t, (const hj_stats *)ctx->stats.p,
-	                   b1, bits, (const uint4 *)in, (uint4 *)out, run, cst,
-	                   ctx->narrow_ok, (const uint8_t *)ctx->tile_narrow, tileA);
+	GI(false, false, 0);
+#undef GI
 }
 
-// LDS of k_sortchunk_g: tile, two source-offset buffers, counters
+// LDS of k_sortchunk_g: tile, three source-offset buffers, counters
 static size_t gathersort_lds(const part_cfg &c, uint32_t wv, uint32_t F) {
-	return (size_t)c.tile * wv * 16 + (size_t)c.tile * 8 + (2 * (size_t)F + 1) * 4 + 64 * 4;
+	return (size_t)c.tile * wv * 16 + (size_t)c.tile * 12 + (2 * (size_t)F + 1) * 4 + 64 * 4;
 }
 
 static void launch_sortchunk_g(hj_ctx_t *ctx, uint32_t wv, const part_cfg &pc, unsigned grid, size_t lds,
@@ -2256,6 +2378,7 @@ int hj_run_radix(hj_ctx_t *ctx, const hj_relation_t *R, const hj_relation_t *S, 
 				acc.radix_bits = part.radix_bits;
 				acc.passes = part.passes;
 				acc.narrow_tuples += part.narrow_tuples;
+				acc.positional_tuples += part.positional_tuples;
 				acc.narrow = part.narrow;
 			}
 		}
@@ -2342,6 +2465,12 @@ static int radix_engine(hj_ctx_t *ctx, const hj_relation_t *R, const hj_relation
 	const uint32_t variant = prm->part_variant;
 	// narrow tuples mark pass-A sources in bit 31 of a segment offset
 	ctx->narrow_ok = wv == 1 && !(prm->flags & HJ_F_NO_NARROW) && nr < (1ull << 31) && ns < (1ull << 31);
+	// positional words: row ids equal to relation positions + 1 (as every
+	// relation of the reference's generator, src/gen.c:205, :244, :262) are
+	// not stored by pass A; offsets below 2^30 leave two code bits in pass
+	// B's staged sources
+	ctx->dense_ok = ctx->narrow_ok && P == 2 && !(prm->flags & HJ_F_NO_POSITIONAL) && nr < (1ull << 30) &&
+	                ns < (1ull << 30);
 	const part_cfg pc = cfg_for(wv, variant);
 
 	// chunk size: a multiple of the scatter tile, ~4 chunks per CU
@@ -2528,8 +2657,9 @@ static int radix_engine(hj_ctx_t *ctx, const hj_relation_t *R, const hj_relation
 	if (P == 2 && ctx->narrow_ok) {
 		// pass A reads 16 B and writes 8 or 16 B per tuple; pass B reads
 		// that and writes 8 B when no tile was wide; the join reads pass B's
-		const uint64_t nt = h.narrow_tuples, a_out = 8 * nt + 16 * (N - nt);
+		const uint64_t nt = h.narrow_tuples, nd = h.dense_tuples, a_out = 4 * nd + 8 * (nt - nd) + 16 * (N - nt);
 		out->narrow_tuples = nt;
+		out->positional_tuples = nd;
 		out->narrow = !(h.flags & HJ_DF_WIDE);
 		const uint64_t b_out = (out->narrow ? 8 : 16) * N;
 		out->algo_bytes = 16 * N + a_out + a_out + b_out + b_out;

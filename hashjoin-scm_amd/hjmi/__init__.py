@@ -25,6 +25,7 @@ HJ_OK, HJ_EINVAL, HJ_ENOMEM, HJ_EHIP, HJ_EREJECT, HJ_ENODEV, HJ_EBUSY = 0, -1, -
 HJ_LOC_DEVICE, HJ_LOC_HOST = 0, 1
 HJ_F_KERNEL_TIMES = 1
 HJ_F_NO_NARROW = 2
+HJ_F_NO_POSITIONAL = 4
 HJ_FB_LP_DUPLICATES, HJ_FB_LP_OVERSIZE, HJ_FB_SC_REBUILD = 1, 2, 4
 
 WL_PKFK_UNIFORM, WL_PKFK_REF, WL_PAYLOAD_REF, WL_ZIPF, WL_RANDOM = 0, 1, 2, 3, 4
@@ -61,7 +62,7 @@ class Result(C.Structure):
                 ("radix_bits", C.c_uint32), ("passes", C.c_uint32), ("fallback", C.c_uint32),
                 ("n_ktimes", C.c_uint32), ("ktime_ms", C.c_double * MAX_KTIMES),
                 ("ktime_id", C.c_uint32 * MAX_KTIMES), ("narrow_tuples", C.c_uint64), ("narrow", C.c_uint32),
-                ("reserved", C.c_uint32)]
+                ("reserved", C.c_uint32), ("positional_tuples", C.c_uint64)]
 
 
 class GenSpec(C.Structure):
@@ -402,7 +403,7 @@ def result_dict(res: Result) -> dict:
         "buildpart_time": res.buildpart_time, "probejoin_time": res.probejoin_time,
         "total_time": res.total_time, "algo_bytes": res.algo_bytes, "radix_bits": res.radix_bits,
         "passes": res.passes, "fallback": res.fallback, "kernel_ms": kt,
-        "narrow_tuples": res.narrow_tuples, "narrow": res.narrow,
+        "narrow_tuples": res.narrow_tuples, "narrow": res.narrow, "positional_tuples": res.positional_tuples,
     }
 
 

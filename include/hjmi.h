@@ -81,6 +81,9 @@ typedef struct {
 #define HJ_F_KERNEL_TIMES 1u  /* record a HIP event pair around every kernel */
 #define HJ_F_NO_NARROW 2u     /* keep 16-byte tuples through the second radix
                                  pass even when keys and row ids fit 32 bits */
+#define HJ_F_NO_POSITIONAL 4u /* two radix passes: store every narrow first-
+                                 pass tile as 8-byte {key, row_id}, also where
+                                 the row ids are the positions + 1            */
 
 /* ---- result: matched_cnt/check_sum as printed by the reference plus the
  *      timekeeper_t phase times (src/inc/store.h:74-78) ---------------------- */
@@ -103,6 +106,11 @@ typedef struct {
 	uint32_t narrow;         /* 1: the second pass and the join moved 8-byte
 	                            tuples (every key and row id fit 32 bits)     */
 	uint32_t reserved;
+	uint64_t positional_tuples; /* of those, tuples of first-pass tiles whose
+	                            row ids are their relation positions + 1,
+	                            stored as 4-byte {key >> first digit, index in
+	                            the tile} words (row id restored from the
+	                            position by the second pass)                  */
 } hj_result_t;
 
 #define HJ_FB_LP_DUPLICATES 1u  /* duplicate build keys under LP: re-run exact */

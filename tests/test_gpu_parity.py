@@ -306,6 +306,66 @@ def test_narrow_second_pass_matches_wide(ctx, algo):
         ctx.free_relation(dS)
 
 
+@pytest.mark.parametrize("algo", ["phj_radix_bc", "phj_radix_lp", "phj_radix_hr", "ptr_phj_radix_bc"])
+def test_positional_first_pass_tiles(ctx, algo):
+    """Pass A stores a narrow tile whose row ids are its relation positions
+    + 1 as 4-byte {key >> first digit, index} words and pass B restores key
+    and row id; any other tile keeps 8 or 16 bytes.  The PK-FK generator
+    (as the reference's, src/gen.c:198-283) gives S row ids = positions + 1
+    and R row ids = keys, so S's tiles qualify and R's do not.  Cases: as
+    generated; S rows shuffled (no tile qualifies); one S row id off by one;
+    one probe key too large for the word (>= 2^28 at these widths, < 2^32);
+    one build key above 32 bits (a wide tile, so the 16-byte second pass
+    decodes S's words).  Each equals the oracle and the same join with
+    HJ_F_NO_POSITIONAL, and reports how many tuples went as words."""
+    nr, ns = 1 << 15, 1 << 19
+    R, S = O.generate("pkfk", nr, ns, 16, seed=23)
+    rng = np.random.default_rng(5)
+    Ssh = S.reshape(ns, 16)[rng.permutation(ns)].reshape(-1).copy()
+    Soff = S.copy()
+    Soff.view(np.uint64)[2 * 77777 + 1] += np.uint64(1)  # one row id off its position
+    Sbig = S.copy()
+    Sbig.view(np.uint64)[2 * 300] = np.uint64((1 << 29) + 5)  # narrow, but no room beside the index
+    Rwide = R.copy()
+    Rwide.view(np.uint64)[2 * 9000] += np.uint64(1 << 34)  # one wide build key
+    one_tile = (ns - 4096, ns - 1)  # every S tile but the one holding the changed tuple
+    cases = [("as generated", R, S, (ns, ns)), ("shuffled S", R, Ssh, (0, 0)), ("S row id off", R, Soff, one_tile),
+             ("big probe key", R, Sbig, one_tile), ("wide build key", Rwide, S, (ns, ns))]
+    for name, hR, hS, (lo, hi) in cases:
+        if algo.startswith("ptr_"):
+            if name != "as generated":
+                continue
+            want = O.ptr_direct_address(hR, hS, 16)
+        else:
+            want = O.aggregate(hR, hS, 16, lp=algo.endswith("lp"))
+        dR, dS = _dev(ctx, hR, 16), _dev(ctx, hS, 16)
+        for bits in (12, 14):
+            a = ctx.join(algo, dR, dS, radix_bits=bits, passes=2)
+            b = ctx.join(algo, dR, dS, radix_bits=bits, passes=2, flags=hjmi.HJ_F_NO_POSITIONAL)
+            _check(a, want, (name, bits))
+            _check(b, want, (name, bits, "no positional"))
+            assert b["positional_tuples"] == 0, (name, b)
+            assert lo <= a["positional_tuples"] <= hi, (name, bits, a["positional_tuples"])
+            assert a["narrow"] == (0 if name == "wide build key" else 1), (name, a)
+        ctx.free_relation(dR)
+        ctx.free_relation(dS)
+
+
+def test_positional_tiles_of_wide_tuple_heads(ctx):
+    """64-byte tuples: pass A reads the 16-byte heads in place and stores
+    positional words for them too (S's tiles)."""
+    nr, ns = 1 << 14, 1 << 18
+    R, S = O.generate("pkfk", nr, ns, 64, seed=29)
+    want = O.aggregate(R, S, 64)
+    dR, dS = _dev(ctx, R, 64), _dev(ctx, S, 64)
+    for bits in (11, 13):
+        got = ctx.join("phj_radix_bc", dR, dS, radix_bits=bits, passes=2)
+        _check(got, want, bits)
+        assert got["positional_tuples"] == ns, got
+    ctx.free_relation(dR)
+    ctx.free_relation(dS)
+
+
 @pytest.mark.parametrize("algo", ["phj_radix_bc", "phj_radix_lp"])
 def test_tuning_variants_match_oracle(ctx, algo):
     """Every kernel shape a part_variant selects (one-pass tile shapes and
