@@ -59,8 +59,10 @@ def kernel_bytes(name, W, nr, ns, res=None):
     bytes partition the 16-byte heads: two passes read them in place in pass
     A, one pass partitions the heads that ptr_keyrid copies out.  Two passes over 16-byte tuples store
     a pass-A tile as 8-byte {key, row_id} tuples when they fit 32 bits
-    (res["narrow_tuples"] tuples), and pass B and the join then move 8-byte
-    tuples when every tile did (res["narrow"])."""
+    (res["narrow_tuples"] tuples), of those a tile whose row ids are its
+    positions + 1 as 4-byte words (res["positional_tuples"]), and pass B and
+    the join then move 8-byte tuples when every tile was narrow
+    (res["narrow"])."""
     N = nr + ns
     head = min(W, 64)
     # radix joins of tuples wider than 16 bytes partition their 16-byte heads
@@ -69,8 +71,8 @@ def kernel_bytes(name, W, nr, ns, res=None):
     a_out = b_out = Wr * N  # pass A output, pass B output
     join_in = Wr * N
     if res is not None and res.get("passes") == 2:
-        nt = res.get("narrow_tuples", 0)
-        a_out = 8 * nt + 16 * (N - nt)
+        nt, nd = res.get("narrow_tuples", 0), res.get("positional_tuples", 0)
+        a_out = 4 * nd + 8 * (nt - nd) + 16 * (N - nt)
         b_out = join_in = (8 if res.get("narrow") else 16) * N
     # two passes over wider tuples read the heads in place in pass A (one
     # burst per tuple); fewer passes copy them out first (ptr_keyrid)
@@ -511,6 +513,7 @@ def main():
                 "kernel": dom, "kernel_avg_ms": round(avg_ms, 4),
                 "kernel_algo_bytes": kernel_bytes(dom, W, nr, ns, res[0]),
                 "narrow_tuples": res[0].get("narrow_tuples", 0), "narrow": res[0].get("narrow", 0),
+                "positional_tuples": res[0].get("positional_tuples", 0),
                 "job_algo_bytes": res[0]["algo_bytes"], "job_device_ms": round(job_s * 1e3, 4),
                 "job_frac": round(job_gbps / PEAK_GBPS, 4),
                 "kernel_ms_per_step": {k: round(v / args.steps, 4) for k, v in sorted(tot.items())}}

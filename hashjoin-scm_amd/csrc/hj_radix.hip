@@ -210,40 +210,6 @@ __global__ void k_bounds(const hj_seg *__restrict__ segs, uint32_t nseg, uint32_
 	}
 }
 
-// Pass 2 segments: one per pass-1 partition (s2 = s1*F1 + p1).
-__global__ void k_seg2_len(const uint64_t *__restrict__ pb1, uint32_t nseg2, uint64_t ch, const uint4 *in1,
-                           uint32_t wv, uint32_t bits2, hj_seg *__restrict__ segs2, uint64_t *__restrict__ nch) {
-	for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < nseg2; s += gridDim.x * blockDim.x) {
-		const uint64_t a = pb1[s], b = pb1[s + 1];
-		hj_seg g{};
-		g.in = in1 + a * wv;
-		g.len = b - a;
-		g.out_start = a;
-		g.nchunks = (uint32_t)((g.len + ch - 1) / ch);
-		g.chunk_base = 0;
-		g.hist_base = 0;
-		g.bits = bits2;
-		g.pb_base = 0;
-		segs2[s] = g;
-		nch[s] = g.nchunks;
-	}
-}
-
-__global__ void k_seg2_fin(const uint64_t *__restrict__ nch_scan, uint32_t nseg2, uint32_t F2,
-                           hj_seg *__restrict__ segs2, uint32_t *__restrict__ chunk_seg, hj_stats *st) {
-	// one workgroup per segment
-	for (uint32_t s = blockIdx.x; s < nseg2; s += gridDim.x) {
-		const uint32_t cb = (uint32_t)nch_scan[s];
-		const uint32_t nc = segs2[s].nchunks;
-		if (threadIdx.x == 0) {
-			segs2[s].chunk_base = cb;
-			segs2[s].hist_base = (uint64_t)cb * F2;
-		}
-		for (uint32_t c = threadIdx.x; c < nc; c += blockDim.x) chunk_seg[cb + c] = s;
-	}
-	if (blockIdx.x == 0 && threadIdx.x == 0) st->total_chunks = nch_scan[nseg2];
-}
-
 // ------------------------------------------------------------ histogram
 // DETECT (16-byte tuples whose pass may write 8-byte ones): also flag
 // HJ_DF_WIDE when a key or row id of the chunk needs more than 32 bits; the
@@ -801,22 +767,67 @@ __global__ __launch_bounds__(NT) void k_sortchunk(const hj_seg *__restrict__ seg
 // sorts it by the second digit, which gives the same layout as a
 // histogram + scatter pass followed by the chunk-local second pass.
 
-// Count matrix from pass-A run tables: hist[g.hist_base + d*g.nchunks + cl]
-// = tuples of digit d in tile cl of segment g, and beg_t at the same index =
-// where that run starts inside the tile.  A workgroup transposes 64 tiles x
-// 64 digits through LDS so that reads and writes are both coalesced.
-__global__ __launch_bounds__(256) void k_runs_to_hist(const uint16_t *__restrict__ run, const hj_seg *__restrict__ segs,
-                                                      const uint32_t *__restrict__ chunk_seg, uint32_t total,
-                                                      uint32_t F, uint32_t *__restrict__ hist,
-                                                      uint16_t *__restrict__ beg_t) {
+// Count matrix from pass-A run tables, as two kernels around a scan of
+// per-group sums: the tiles of each segment form groups of 64 (segment 0's
+// GR groups, then segment 1's GS), and part[seg][digit][group] (segment 1's
+// rows after F*GR entries) = tuples of that digit in the group's tiles.
+// After an exclusive scan of `part` in that order, k_runs_pos writes
+// hist[g.hist_base + d*g.nchunks + cl] = the position of tile cl's run of
+// digit d in the fully partitioned order (its group's scanned sum + the runs
+// of the group's earlier tiles) and beg_t at the same index = where that run
+// starts inside its tile.  Together that is the exclusive scan of the whole
+// [segment][digit][tile] count matrix, with the scan's reduce phase done on
+// the run tables (a third of the matrix's bytes).  A workgroup takes 64
+// tiles x 64 digits; run[c*(F+1) + d] is u16, so a wave reads 64
+// consecutive digits of one tile, and the matrix rows are written 64
+// consecutive tiles at a time.
+struct hj_grp {
+	uint32_t seg, g, G, c0, nt, cl0;
+};
+__device__ __forceinline__ hj_grp hj_group_of(const hj_seg *segs, uint32_t grp, uint32_t GR, uint32_t GS) {
+	hj_grp r;
+	r.seg = grp < GR ? 0u : 1u;
+	r.g = r.seg ? grp - GR : grp;
+	r.G = r.seg ? GS : GR;
+	const hj_seg &s = segs[r.seg];
+	r.cl0 = r.g * 64;
+	r.c0 = s.chunk_base + r.cl0;
+	r.nt = min(64u, s.nchunks - r.cl0);
+	return r;
+}
+
+__global__ __launch_bounds__(256) void k_runs_part(const uint16_t *__restrict__ run, const hj_seg *__restrict__ segs,
+                                                   uint32_t F, uint32_t GR, uint32_t GS, uint32_t *__restrict__ part) {
+	__shared__ uint32_t acc[4][64];
+	if (blockIdx.x >= GR + GS) return;
+	const hj_grp q = hj_group_of(segs, blockIdx.x, GR, GS);
+	const uint32_t dj = threadIdx.x & 63, qt = threadIdx.x >> 6, d = blockIdx.y * 64 + dj;
+	uint32_t sum = 0;
+	if (d < F)
+		for (uint32_t i = qt; i < q.nt; i += 4) {
+			const uint16_t *r = run + (uint64_t)(q.c0 + i) * (F + 1) + d;
+			sum += (uint32_t)r[1] - r[0];
+		}
+	acc[qt][dj] = sum;
+	__syncthreads();
+	if (threadIdx.x < 64 && d < F)
+		part[(q.seg ? (uint64_t)F * GR : 0) + (uint64_t)d * q.G + q.g] = acc[0][dj] + acc[1][dj] + acc[2][dj] + acc[3][dj];
+}
+
+__global__ __launch_bounds__(256) void k_runs_pos(const uint16_t *__restrict__ run, const hj_seg *__restrict__ segs,
+                                                  uint32_t F, uint32_t GR, uint32_t GS, const uint32_t *__restrict__ part,
+                                                  uint32_t *__restrict__ hist, uint16_t *__restrict__ beg_t) {
 	__shared__ uint32_t cnt[64][65];
 	__shared__ uint32_t beg[64][65];
-	const uint32_t c0 = blockIdx.x * 64, d0 = blockIdx.y * 64;
+	__shared__ uint32_t acc[4][64];
+	if (blockIdx.x >= GR + GS) return;
+	const hj_grp q = hj_group_of(segs, blockIdx.x, GR, GS);
+	const uint32_t d0 = blockIdx.y * 64;
 	for (uint32_t e = threadIdx.x; e < 64 * 64; e += 256) {
-		const uint32_t ci = e / 64, dj = e % 64, c = c0 + ci, d = d0 + dj;
+		const uint32_t ci = e / 64, dj = e % 64, d = d0 + dj;
 		uint32_t v = 0, a = 0;
-		if (c < total && d < F) {
-			const uint16_t *r = run + (uint64_t)c * (F + 1) + d;
+		if (ci < q.nt && d < F) {
+			const uint16_t *r = run + (uint64_t)(q.c0 + ci) * (F + 1) + d;
 			a = r[0];
 			v = (uint32_t)r[1] - a;
 		}
@@ -824,15 +835,119 @@ __global__ __launch_bounds__(256) void k_runs_to_hist(const uint16_t *__restrict
 		beg[ci][dj] = a;
 	}
 	__syncthreads();
-	for (uint32_t e = threadIdx.x; e < 64 * 64; e += 256) {
-		const uint32_t dj = e / 64, ci = e % 64, c = c0 + ci, d = d0 + dj;
-		if (c < total && d < F) {
-			const hj_seg &g = segs[chunk_seg[c]];
-			const uint64_t at = g.hist_base + (uint64_t)d * g.nchunks + (c - g.chunk_base);
-			hist[at] = cnt[ci][dj];
-			beg_t[at] = (uint16_t)beg[ci][dj];
+	// digit dj's runs over the group's tiles, 16 tiles per thread, turned
+	// into positions in place
+	const uint32_t dj = threadIdx.x & 63, qt = threadIdx.x >> 6, d = d0 + dj;
+	uint32_t sum = 0;
+	for (uint32_t i = 0; i < 16; i++) sum += cnt[qt * 16 + i][dj];
+	acc[qt][dj] = sum;
+	__syncthreads();
+	if (d < F) {
+		uint32_t run_pos = part[(q.seg ? (uint64_t)F * GR : 0) + (uint64_t)d * q.G + q.g];
+		for (uint32_t k = 0; k < qt; k++) run_pos += acc[k][dj];
+		for (uint32_t i = 0; i < 16; i++) {
+			const uint32_t x = cnt[qt * 16 + i][dj];
+			cnt[qt * 16 + i][dj] = run_pos;
+			run_pos += x;
 		}
 	}
+	__syncthreads();
+	const hj_seg &g = segs[q.seg];
+	for (uint32_t e = threadIdx.x; e < 64 * 64; e += 256) {
+		const uint32_t dj2 = e / 64, ci = e % 64, d2 = d0 + dj2;
+		if (ci < q.nt && d2 < F) {
+			const uint64_t at = g.hist_base + (uint64_t)d2 * g.nchunks + q.cl0 + ci;
+			hist[at] = cnt[ci][dj2];
+			beg_t[at] = (uint16_t)beg[ci][dj2];
+		}
+	}
+}
+
+// The second pass's segments (one per first-level partition, s2 = seg*F1 +
+// d1) and the join's per-first-digit work, by one workgroup of 1024
+// threads: pb[s2] = first position of partition s2 (its group-0 sum in the
+// scanned `part`, or the segment start when the segment has no tiles),
+// segs2[s2] its bounds and pass-B tiles (ch tuples each, numbered
+// consecutively over all s2), st->total_chunks their count, and for the
+// join's items (d1, d2): tiles[d1] = probe tiles of cpt pass-B tiles each
+// (0 when either side of d1 is empty) and dpre[d1] = items of all earlier
+// first digits (F2 each), dpre[F1] = all.  It also ends the count matrix:
+// hist[hist_len] = the total, the end of the last digit's row (k_runs_pos
+// writes positions of runs, which start).  Dynamic LDS: 8 bytes per
+// second-pass segment.
+__global__ __launch_bounds__(1024) void k_segs2(const hj_seg *__restrict__ segs1, const uint32_t *__restrict__ part,
+                                                uint32_t F1, uint32_t GR, uint32_t GS, uint64_t ch, const uint4 *in1,
+                                                uint32_t wv, uint32_t b2, uint32_t cpt, hj_seg *__restrict__ segs2,
+                                                uint64_t *__restrict__ pb, uint64_t *__restrict__ dpre,
+                                                uint32_t *__restrict__ dtiles, hj_stats *__restrict__ st,
+                                                uint32_t *__restrict__ hist, uint64_t hist_len) {
+	extern __shared__ uint32_t s2lds[];
+	uint32_t *slen = s2lds, *snch = s2lds + 2 * F1;
+	__shared__ uint32_t red32[1024 / HJ_WAVE + 1];
+	__shared__ uint64_t red64[1024 / HJ_WAVE + 1];
+	const uint32_t nseg2 = 2 * F1, per = (nseg2 + 1023) / 1024;
+	const uint64_t end = segs1[1].out_start + segs1[1].len;
+	auto start = [&](uint32_t x) -> uint64_t {
+		if (x >= nseg2) return end;
+		const uint32_t seg = x / F1, d = x % F1;
+		const hj_seg &g = segs1[seg];
+		return g.nchunks ? part[(seg ? (uint64_t)F1 * GR : 0) + (uint64_t)d * (seg ? GS : GR)] : g.out_start;
+	};
+	const uint32_t x0 = threadIdx.x * per;
+	uint32_t mine = 0;
+	for (uint32_t k = 0; k < per; k++) {
+		const uint32_t x = x0 + k;
+		if (x < nseg2) {
+			const uint64_t a = start(x), len = start(x + 1) - a;
+			slen[x] = (uint32_t)len;
+			snch[x] = (uint32_t)((len + ch - 1) / ch);
+			mine += snch[x];
+			pb[x] = a;
+		}
+	}
+	uint32_t tot;
+	uint32_t cb = hj_block_exscan<uint32_t>(mine, red32, &tot);
+	for (uint32_t k = 0; k < per; k++) {
+		const uint32_t x = x0 + k;
+		if (x < nseg2) {
+			hj_seg g{};
+			g.out_start = pb[x];
+			g.in = in1 + g.out_start * wv;
+			g.len = slen[x];
+			g.nchunks = snch[x];
+			g.chunk_base = cb;
+			g.hist_base = (uint64_t)cb << b2;
+			g.bits = b2;
+			segs2[x] = g;
+			cb += g.nchunks;
+		}
+	}
+	if (threadIdx.x == 0) {
+		pb[nseg2] = end;
+		st->total_chunks = tot;
+		hist[hist_len] = (uint32_t)end;
+	}
+	__syncthreads();  // slen / snch complete
+	const uint32_t perd = (F1 + 1023) / 1024, y0 = threadIdx.x * perd;
+	uint64_t items = 0;
+	for (uint32_t k = 0; k < perd; k++) {
+		const uint32_t d1 = y0 + k;
+		if (d1 < F1) {
+			const uint32_t t = (slen[d1] && slen[F1 + d1]) ? (snch[F1 + d1] + cpt - 1) / cpt : 0u;
+			dtiles[d1] = t;
+			items += (uint64_t)t << b2;
+		}
+	}
+	uint64_t tot64;
+	uint64_t run = hj_block_exscan<uint64_t>(items, red64, &tot64);
+	for (uint32_t k = 0; k < perd; k++) {
+		const uint32_t d1 = y0 + k;
+		if (d1 < F1) {
+			dpre[d1] = run;
+			run += (uint64_t)dtiles[d1] << b2;
+		}
+	}
+	if (threadIdx.x == 0) dpre[F1] = tot64;
 }
 
 // One pass-B tile, everything k_sortchunk_g needs to stage and place it,
@@ -856,17 +971,46 @@ static_assert(sizeof(hj_gtile) == 48, "pass-B tile record size");
 // neighbouring digits at the same k read the same pass-A tiles, so
 // processing them together lets them share those lines in L2.  first = the
 // largest i with row[i] <= tile start (row = scanned counts of its digit).
-// The tile counts of all 2 * F1 second-level segments are read into LDS
-// once per workgroup (dynamic LDS: 8 * F1 bytes) for the rank loop.
+// The tile counts and first tile ids of all 2 * F1 second-level segments
+// are read into LDS once per workgroup (dynamic LDS: 16 * F1 bytes) for the
+// rank loop and for finding a tile's segment (the last one whose first tile
+// id is at most the tile's: empty segments share the next one's).
+//
+// The same kernel writes the join's work list, items in (first digit,
+// second digit) order: item i is final partition q = (d2 << b1) | d1 with
+// d1 = i >> b2 and d2 = i & (F2 - 1), so the items of one first-level
+// partition are consecutive and the joins that read neighbouring runs of
+// the same pass-B tiles run together.  Both sides of every (d1, d2) gather
+// from the pass-B tiles of d1, so pairs[i] (the chunk id ranges of R's and
+// S's segment d1) depends on d1 only, and so do its probe tiles (dtiles,
+// from k_segs2): tpre[i] = dpre[d1] + d2 * dtiles[d1] is the exclusive scan
+// of the items' tiles, tpre[Q] their total.
 __global__ void k_first_run(const hj_seg *__restrict__ segs1, const hj_seg *__restrict__ segs2,
-                            const uint32_t *__restrict__ chunk_seg2, const hj_stats *__restrict__ st, uint32_t b1,
-                            uint32_t tile, const uint32_t *__restrict__ hist, hj_gtile *__restrict__ gt) {
+                            const hj_stats *__restrict__ st, uint32_t b1, uint32_t tile,
+                            const uint32_t *__restrict__ hist, hj_gtile *__restrict__ gt, uint32_t b2,
+                            const uint64_t *__restrict__ dpre, const uint32_t *__restrict__ dtiles,
+                            hj_pair *__restrict__ pairs, uint64_t *__restrict__ tpre) {
 	extern __shared__ uint32_t nck[];
 	const uint32_t total = (uint32_t)st->total_chunks, F1 = 1u << b1;
-	for (uint32_t i = threadIdx.x; i < 2 * F1; i += blockDim.x) nck[i] = segs2[i].nchunks;
+	uint32_t *cbs = nck + 2 * F1;
+	for (uint32_t i = threadIdx.x; i < 2 * F1; i += blockDim.x) {
+		nck[i] = segs2[i].nchunks;
+		cbs[i] = segs2[i].chunk_base;
+	}
 	__syncthreads();
+	const uint32_t Q = F1 << b2;
+	for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < Q; i += gridDim.x * blockDim.x) {
+		const uint32_t d1 = i >> b2;
+		pairs[i] = hj_pair{cbs[d1], (uint64_t)cbs[d1] + nck[d1], cbs[F1 + d1], (uint64_t)cbs[F1 + d1] + nck[F1 + d1]};
+		tpre[i] = dpre[d1] + (uint64_t)(i & ((1u << b2) - 1)) * dtiles[d1];
+	}
+	if (blockIdx.x == 0 && threadIdx.x == 0) tpre[Q] = dpre[F1];
 	for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < total; c += gridDim.x * blockDim.x) {
-		const uint32_t s2 = chunk_seg2[c];
+		uint32_t s2 = 0;
+		for (uint32_t top = 2 * F1; top - s2 > 1;) {
+			const uint32_t mid = (s2 + top) >> 1;
+			if (cbs[mid] <= c) s2 = mid; else top = mid;
+		}
 		const hj_seg &g2 = segs2[s2];
 		const uint32_t seg = s2 >> b1, d1 = s2 & (F1 - 1), k = c - g2.chunk_base;
 		uint32_t rank = seg ? segs2[F1].chunk_base : 0u;  // segment 1 follows all of segment 0
@@ -1307,26 +1451,6 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
 		tb = td;
 		tc = te;
 		td = tf;
-	}
-}
-
-// Work list of the two-pass join: pair q = (p2 << b1) | p1 holds the chunk id
-// ranges of pass-1 partition p1 of R (segment p1) and of S (segment F1+p1).
-// Work items of the two-pass join in (first digit, second digit) order:
-// item i is final partition q = (d2 << b1) | d1 with d1 = i >> b2 and
-// d2 = i & (F2 - 1), so the items of one first-level partition are
-// consecutive and the joins that read neighbouring runs of the same pass-B
-// tiles run together.  Both sides of every (d1, d2) gather from the pass-B
-// tiles of d1, so the pair depends on d1 only.
-__global__ void k_plan2(const hj_seg *__restrict__ segs2, uint32_t Q, uint32_t b1, uint32_t b2, uint32_t cpt,
-                        hj_pair *__restrict__ pairs, uint64_t *__restrict__ tiles) {
-	const uint32_t F1 = 1u << b1;
-	for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < Q; i += gridDim.x * blockDim.x) {
-		const uint32_t d1 = i >> b2;
-		const hj_seg &r = segs2[d1], &s = segs2[F1 + d1];
-		hj_pair pr{r.chunk_base, (uint64_t)r.chunk_base + r.nchunks, s.chunk_base, (uint64_t)s.chunk_base + s.nchunks};
-		pairs[i] = pr;
-		tiles[i] = (r.len && s.len) ? (s.nchunks + cpt - 1) / cpt : 0;
 	}
 }
 
@@ -2501,7 +2625,7 @@ static int radix_engine(hj_ctx_t *ctx, const hj_relation_t *R, const hj_relation
 	const uint64_t hist_len = std::max(hist1_len, hist2_len) + 1;
 	if ((rc = hj_reserve(ctx, ctx->hist, hist_len * 8))) return rc;
 	if ((rc = hj_reserve(ctx, ctx->scan_tmp, ((hist_len + Q + nseg2) / SCAN_CH + 2) * 8 + 4096))) return rc;
-	// meta: segs1[2] | segs2[nseg2] | cmap[max chunks] | pb1 | nch | pairs | tiles | runs | chunk starts
+	// meta: segs1[2] | segs2[nseg2] | cmap[max chunks] | pb1 | pairs | tiles | runs | chunk starts
 	const uint64_t cmap_n = std::max<uint64_t>(chunks1, chunks2_max);
 	size_t off = 0;
 	auto take = [&](size_t bytes) {
@@ -2511,7 +2635,6 @@ static int radix_engine(hj_ctx_t *ctx, const hj_relation_t *R, const hj_relation
 	};
 	const size_t o_seg1 = take(2 * sizeof(hj_seg)), o_seg2 = take(nseg2 * sizeof(hj_seg));
 	const size_t o_cmap = take(cmap_n * 4), o_pb1 = take(((uint64_t)2 * F1 + 1) * 8);
-	const size_t o_nch = take(((uint64_t)nseg2 + 1) * 8);
 	const size_t o_pairs = take((size_t)Q * sizeof(hj_pair)), o_tiles = take(((size_t)Q + 1) * 8);
 	const size_t o_run = take(P == 2 ? chunks2_max * (F2 + 1) * 2 : 8);
 	const size_t o_cst = take(P == 2 ? chunks2_max * 8 : 8);
@@ -2520,18 +2643,32 @@ static int radix_engine(hj_ctx_t *ctx, const hj_relation_t *R, const hj_relation
 	const size_t o_gt = take(P == 2 ? chunks2_max * sizeof(hj_gtile) : 8);
 	const size_t o_runt = take(P == 2 ? chunks2_max * (F2 + 1) * 2 : 8);
 	const size_t o_tnar = take(P == 2 ? chunks1 : 8);
+	// two passes: groups of 64 pass-A tiles per segment, their per-digit sums
+	// (scanned), and the join's per-first-digit items
+	const uint32_t GR = (uint32_t)(((nr + ch1 - 1) / ch1 + 63) / 64), GS = (uint32_t)(((ns + ch1 - 1) / ch1 + 63) / 64);
+	const size_t o_gpart = take(P == 2 ? ((uint64_t)F1 * (GR + GS) + 1) * 4 : 8);
+	const size_t o_dpre = take(P == 2 ? ((uint64_t)F1 + 1) * 8 : 8), o_dtiles = take(P == 2 ? (uint64_t)F1 * 4 : 8);
 	if ((rc = hj_reserve(ctx, ctx->meta, off))) return rc;
 	char *meta = (char *)ctx->meta.p;
 	hj_seg *segs1 = (hj_seg *)(meta + o_seg1), *segs2 = (hj_seg *)(meta + o_seg2);
 	uint32_t *cmap = (uint32_t *)(meta + o_cmap);
 	uint64_t *pb1 = (uint64_t *)(meta + o_pb1);
-	uint64_t *nch = (uint64_t *)(meta + o_nch), *tiles = (uint64_t *)(meta + o_tiles);
+	uint64_t *tiles = (uint64_t *)(meta + o_tiles);
 	hj_pair *pairs = (hj_pair *)(meta + o_pairs);
 	uint16_t *runs = (uint16_t *)(meta + o_run), *runsA = (uint16_t *)(meta + o_runA);
 	uint64_t *cst = (uint64_t *)(meta + o_cst), *cstA = (uint64_t *)(meta + o_cstA);
 	hj_gtile *gt = (hj_gtile *)(meta + o_gt);
 	uint16_t *beg_t = (uint16_t *)(meta + o_begt), *run_t = (uint16_t *)(meta + o_runt);
 	ctx->tile_narrow = (uint8_t *)(meta + o_tnar);
+	uint32_t *gpart = (uint32_t *)(meta + o_gpart), *dtiles = (uint32_t *)(meta + o_dtiles);
+	uint64_t *dpre = (uint64_t *)(meta + o_dpre);
+	// probe tiles of the two-pass join: cpt pass-B tiles of S per item, one
+	// item per second digit of a uniform partition
+	uint32_t cpt = 0;
+	if (P == 2) {
+		const uint64_t avg_chunks = (ns / F1 + pc2.tile - 1) / pc2.tile;
+		cpt = (uint32_t)std::max<uint64_t>(16, avg_chunks + avg_chunks / 4 + 2);
+	}
 	uint64_t *hist = (uint64_t *)ctx->hist.p;
 	// two passes: the count matrix holds 32-bit positions (a piece's R and S
 	// have at most 2^31 tuples each, so every position fits)
@@ -2560,31 +2697,33 @@ static int radix_engine(hj_ctx_t *ctx, const hj_relation_t *R, const hj_relation
 			hj_kt_begin(ctx, HJ_K_TILESORT);
 			launch_sortchunk(ctx, wv, pcA, gA, sldsA, segs1, cmap, 0, b1, ctx->part[0].p, runsA, cstA);
 			hj_kt_end(ctx);
+			// the count matrix: group sums, their scan, positions
 			hj_kt_begin(ctx, HJ_K_SCAN);
-			hipLaunchKernelGGL(k_runs_to_hist, dim3((unsigned)std::max<uint64_t>(1, (chunks1 + 63) / 64), (F1 + 63) / 64),
-			                   dim3(256), 0,
-			                   ctx->stream, runsA, segs1, cmap, (uint32_t)chunks1, F1, hist32, beg_t);
-			rc = scan_u32(ctx, hist32, hist1_len, (uint64_t *)ctx->scan_tmp.p);
+			const dim3 gg(std::max(1u, GR + GS), (F1 + 63) / 64);
+			hipLaunchKernelGGL(k_runs_part, gg, dim3(256), 0, ctx->stream, runsA, segs1, F1, GR, GS, gpart);
+			rc = scan_u32(ctx, gpart, (uint64_t)F1 * (GR + GS), (uint64_t *)ctx->scan_tmp.p);
+			hipLaunchKernelGGL(k_runs_pos, gg, dim3(256), 0, ctx->stream, runsA, segs1, F1, GR, GS, gpart, hist32,
+			                   beg_t);
 			hj_kt_end(ctx);
 			if (rc) return rc;
 		}
 		if (P == 1)
 			hipLaunchKernelGGL(k_bounds<uint64_t>, dim3((unsigned)std::min<uint64_t>(1024, (2ull * F1 + 256) / 256)),
 			                   dim3(256), 0, ctx->stream, segs1, 2u, F1, (const uint64_t *)hist, pb1);
-		else
-			hipLaunchKernelGGL(k_bounds<uint32_t>, dim3((unsigned)std::min<uint64_t>(1024, (2ull * F1 + 256) / 256)),
-			                   dim3(256), 0, ctx->stream, segs1, 2u, F1, (const uint32_t *)hist32, pb1);
 		pbF = pb1;
 		Rb = Sb = ctx->part[0].p;
 		if (P == 2) {
-			hipLaunchKernelGGL(k_seg2_len, dim3((nseg2 + 255) / 256), dim3(256), 0, ctx->stream, pb1, nseg2, ch2,
-			                   (const uint4 *)ctx->part[0].p, wv, b2, segs2, nch);
-			if ((rc = scan_u64(ctx, nch, nseg2, (uint64_t *)ctx->scan_tmp.p))) return rc;
-			hipLaunchKernelGGL(k_seg2_fin, dim3(std::min<uint32_t>(nseg2, 4096)), dim3(256), 0, ctx->stream, nch,
-			                   nseg2, F2, segs2, cmap, st);
+			// second-pass segments, pass-B tile records and the join's work list
+			hj_kt_begin(ctx, HJ_K_PLAN);
+			allow_lds(ctx, (const void *)k_segs2);
+			hipLaunchKernelGGL(k_segs2, dim3(1), dim3(1024), (size_t)16 * F1, ctx->stream, segs1, gpart, F1, GR, GS,
+			                   ch2, (const uint4 *)ctx->part[0].p, wv, b2, cpt, segs2, pb1, dpre, dtiles, st, hist32,
+			                   hist1_len);
+			allow_lds(ctx, (const void *)k_first_run);
 			hipLaunchKernelGGL(k_first_run, dim3((unsigned)std::min<uint64_t>(4096, (chunks2_max + 255) / 256)),
-			                   dim3(256), (size_t)8 * F1, ctx->stream, segs1, segs2, cmap, st, b1, (uint32_t)ch2, hist32,
-			                   gt);
+			                   dim3(256), (size_t)16 * F1, ctx->stream, segs1, segs2, st, b1, (uint32_t)ch2, hist32,
+			                   gt, b2, dpre, dtiles, pairs, tiles);
+			hj_kt_end(ctx);
 			// pass B: each first-level partition's tiles gathered from the
 			// pass-A runs and sorted by the second digit
 			// two workgroups per CU: with three the tiles in flight on an XCD
@@ -2623,20 +2762,14 @@ static int radix_engine(hj_ctx_t *ctx, const hj_relation_t *R, const hj_relation
 	// idle, so cut the probe side into about two items per workgroup
 	if (P <= 1 && (uint64_t)Q * ((avg_s + s_tile - 1) / s_tile) < 2ull * jgrid)
 		s_tile = std::max<uint64_t>(8192, ns / (2ull * jgrid) + 1);
-	hj_kt_begin(ctx, HJ_K_PLAN);
-	uint32_t cpt = 0;
-	if (P == 2) {
-		const uint64_t avg_chunks = (ns / F1 + pc2.tile - 1) / pc2.tile;
-		cpt = (uint32_t)std::max<uint64_t>(16, avg_chunks + avg_chunks / 4 + 2);  // one item per uniform partition
-		hipLaunchKernelGGL(k_plan2, dim3((Q + 255) / 256), dim3(256), 0, ctx->stream, segs2, Q, b1, b2, cpt, pairs,
-		                   tiles);
-	} else {
+	if (P <= 1) {
+		hj_kt_begin(ctx, HJ_K_PLAN);
 		hipLaunchKernelGGL(k_plan, dim3((Q + 255) / 256), dim3(256), 0, ctx->stream, pbF, P, Q, b1, F2, nr, ns, s_tile,
 		                   pairs, tiles);
+		rc = scan_u64(ctx, tiles, Q, (uint64_t *)ctx->scan_tmp.p);
+		hj_kt_end(ctx);
+		if (rc) return rc;
 	}
-	rc = scan_u64(ctx, tiles, Q, (uint64_t *)ctx->scan_tmp.p);
-	hj_kt_end(ctx);
-	if (rc) return rc;
 	hj_kt_begin(ctx, HJ_K_JOIN);
 	if (P == 2) {
 		launch_join2(ctx, wv, tk, jgrid, jlds, Rb, cst, run_t, (uint32_t)chunks2_max, b1, pairs, tiles, Q, B, cpt, cap);
