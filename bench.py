@@ -277,7 +277,7 @@ def run_batched(args, hjmi):
         t0 = time.perf_counter()
         plain_res = [ctx.join(algo, R, S) for R, S in zip(Rs, Ss)]
         plain_times.append(time.perf_counter() - t0)
-    b = ctx.batch(algo, Rs, Ss)
+    b = ctx.batch(algo, Rs, Ss, flags=args.batch_flags)
     out, info = b.run()  # warm-up
     batch_times, dev_ms = [], []
     for _ in range(args.steps):
@@ -289,15 +289,13 @@ def run_batched(args, hjmi):
     if not same:
         raise SystemExit("batched results differ from plain joins")
     tb, tp = min(batch_times), min(plain_times)
-    # Roofline of the batch graph (one launch: histogram, scatter, plan,
-    # join): algorithmic bytes of every join / the graph's device time.  A
-    # partitioned join reads its 16-byte tuples, writes 8-byte narrow ones and
-    # the join reads those (32 B per tuple); a join whose build side fits one
-    # LDS table is read once (16 B).  The unpartitioned joins are the
-    # smallest build sides (the bits rule is monotone in |R|).
-    n_unp = info["n_unpartitioned"]
-    order = sorted(sizes)
-    algo_bytes = sum((16 if k < n_unp else 32) * 17 * nr for k, nr in enumerate(order))
+    # Roofline of the batch graph (one launch): the library's algorithmic
+    # bytes of the run -- gathered batches: pass A reads 16 B per tuple and
+    # writes each tile at its layout (4-byte positional words, 8-byte or
+    # 16-byte tuples), the join reads those; scattered batches: the scatter
+    # reads 16 and writes 8 (narrow) and the join reads 8, unpartitioned
+    # joins read 16 -- over the graph's device time.
+    algo_bytes = info["algo_bytes"]
     dev_s = min(dev_ms) * 1e-3
     achieved = algo_bytes / dev_s / 1e9
     res = {
@@ -435,6 +433,8 @@ def main():
     ap.add_argument("--joins", type=int, default=2048, help="C4: joins per batch")
     ap.add_argument("--batch-lo", type=int, default=10, help="C4: log2 of the smallest |R|")
     ap.add_argument("--batch-hi", type=int, default=20, help="C4: log2 of the largest |R|")
+    ap.add_argument("--batch-flags", type=int, default=0,
+                    help="C4: hj_params_t flags of the batch (8 = HJ_F_BATCH_SCATTER: histogram + scatter path)")
     ap.add_argument("--algo", default=None)
     ap.add_argument("--theta", type=float, default=None, help="C3: Zipf exponent of the probe keys (default 1.0)")
     ap.add_argument("--bits", type=int, default=0)

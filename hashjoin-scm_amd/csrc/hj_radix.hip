@@ -583,26 +583,28 @@ __global__ __launch_bounds__(NT) void k_join(const uint4 *__restrict__ Rb, const
 // position of chunk c.  Pass A of a two-pass join runs it over the input.
 // The next tiles are loaded while a tile is sorted (two register buffers,
 // as in k_scatter).
-template <int WV, int NT, int ITEMS>
+template <int WV, int NT, int ITEMS, bool BATCH = false>
 __global__ __launch_bounds__(NT) void k_sortchunk(const hj_seg *__restrict__ segs, const uint32_t *__restrict__ cmap,
                                                   hj_stats *__restrict__ st, uint32_t shift, uint32_t bits,
                                                   uint4 *__restrict__ out, uint16_t *__restrict__ run,
                                                   uint64_t *__restrict__ cstart, bool narrow_ok, bool dense_ok,
                                                   uint8_t *__restrict__ tnarrow) {
+	// BATCH: the segments are the relations of many joins, each with its own
+	// digit width (hj_seg.bits; `bits` is the largest), `shift` is 0, tile
+	// c's run table starts at segs[.].hist_base + (c - chunk_base) * (F + 1),
+	// and cstart[c] receives the tile's address with its layout in the low
+	// two bits (2: 4-byte positional words, 1: 8-byte, 0: 16-byte tuples)
+	// instead of a position plus tnarrow[c].
 	constexpr int TILE = NT * ITEMS;
 	constexpr uint32_t LT = __builtin_ctz(TILE);  // bits of a tuple's index in its tile
 	extern __shared__ uint4 lds_u4[];
-	const uint32_t F = 1u << bits, fmask = F - 1;
+	const uint32_t Fmax = 1u << bits;
 	uint4 *tile = lds_u4;
 	uint32_t *tcnt = (uint32_t *)(tile + TILE * WV);
-	uint32_t *tbase = tcnt + F;
-	uint32_t *red = tbase + F + 1;
+	uint32_t *tbase = tcnt + Fmax;
+	uint32_t *red = tbase + Fmax + 1;
 	__shared__ uint32_t tflag;  // per tile: 1 a wide tuple, 2 a row id that is not position + 1
-	// a positional word holds key >> bits above the LT index bits: keys
-	// below 2^(32 - LT + bits) fit
-	const uint32_t kshift = bits + 32 - LT;
 	const uint32_t total = (uint32_t)st->total_chunks;
-	const uint32_t per = (F + NT - 1) / NT;
 	const uint32_t grid = gridDim.x;
 	// tile c: source, output position, length (an empty tile past the end
 	// reads `out`, whose first tuple always exists)
@@ -610,6 +612,8 @@ __global__ __launch_bounds__(NT) void k_sortchunk(const hj_seg *__restrict__ seg
 	// tuples are read
 	uint32_t sw = WV;
 	uint64_t spos = 0;  // position of the tile's first tuple in its relation
+	uint32_t tbits = bits;  // the tile's digit width
+	uint64_t rbase = 0;     // first entry of the tile's run table
 	auto tile_of = [&](uint32_t c, const uint4 *&src, uint64_t &pos, uint32_t &n) {
 		if (c >= total) {
 			src = out;
@@ -617,6 +621,8 @@ __global__ __launch_bounds__(NT) void k_sortchunk(const hj_seg *__restrict__ seg
 			n = 0;
 			sw = WV;
 			spos = 0;
+			tbits = 0;
+			rbase = 0;
 			return;
 		}
 		const hj_seg &g = segs[cmap[c]];
@@ -626,6 +632,8 @@ __global__ __launch_bounds__(NT) void k_sortchunk(const hj_seg *__restrict__ seg
 		spos = cl * TILE;
 		src = g.in + cl * TILE * sw;
 		n = (uint32_t)min((uint64_t)TILE, g.len - cl * TILE);
+		tbits = BATCH ? g.bits : bits;
+		rbase = BATCH ? g.hist_base + cl * ((1ull << tbits) + 1) : (uint64_t)c * (Fmax + 1);
 	};
 	// every thread issues exactly ITEMS*WV loads (see k_scatter)
 	auto load = [&](uint4 (&v)[ITEMS][WV], uint32_t c) {
@@ -647,6 +655,13 @@ __global__ __launch_bounds__(NT) void k_sortchunk(const hj_seg *__restrict__ seg
 		uint64_t pos;
 		uint32_t n;
 		tile_of(c, src, pos, n);
+		// this tile's digit width (tile_of is called again by the prefetch
+		// below, so everything after it uses these copies)
+		const uint32_t tb = tbits, F = 1u << tb, fmask = F - 1, per = (F + NT - 1) / NT;
+		const uint64_t rb = rbase;
+		// a positional word holds key >> tb above the LT index bits: keys
+		// below 2^(32 - LT + tb) fit
+		const uint32_t kshift = tb + 32 - LT;
 		uint32_t dig[ITEMS], rank[ITEMS];
 		bool tw = false, tp = false;
 #pragma unroll
@@ -686,14 +701,16 @@ __global__ __launch_bounds__(NT) void k_sortchunk(const hj_seg *__restrict__ seg
 				for (uint32_t q = 0; q < per; q++)
 					if (p0 + q < F) {
 						tbase[p0 + q] = r;
-						run[(uint64_t)c * (F + 1) + p0 + q] = (uint16_t)r;
+						run[rb + p0 + q] = (uint16_t)r;
 						r += tcnt[p0 + q];
 					}
 			if (threadIdx.x == 0 && n) {
-				run[(uint64_t)c * (F + 1) + F] = (uint16_t)n;
-				cstart[c] = pos;
+				run[rb + F] = (uint16_t)n;
+				const uint32_t code = tile_dense ? 2u : tile_narrow ? 1u : 0u;
+				if (BATCH) cstart[c] = (uint64_t)(uintptr_t)(out + pos) | code;
+				else cstart[c] = pos;
 				if (WV == 1 && narrow_ok) {
-					tnarrow[c] = tile_dense ? 2 : tile_narrow ? 1 : 0;
+					if (!BATCH) tnarrow[c] = (uint8_t)code;
 					narrow_cnt += tile_narrow ? n : 0u;
 					dense_cnt += tile_dense ? n : 0u;
 				}
@@ -711,7 +728,7 @@ __global__ __launch_bounds__(NT) void k_sortchunk(const hj_seg *__restrict__ seg
 			if (idx < n) {
 				const uint32_t q = tbase[dig[k]] + rank[k];
 				if (WV == 1 && tile_dense) {
-					tile4[q] = (v[k][0].x >> bits) << LT | idx;
+					tile4[q] = (v[k][0].x >> tb) << LT | idx;
 				} else if (WV == 1 && tile_narrow) {
 					tile8[q] = make_uint2(v[k][0].x, v[k][0].z);
 				} else {
@@ -739,7 +756,7 @@ __global__ __launch_bounds__(NT) void k_sortchunk(const hj_seg *__restrict__ seg
 		if (threadIdx.x == 0) tflag = 0;
 		__syncthreads();
 	};
-	for (uint32_t p = threadIdx.x; p < F; p += NT) tcnt[p] = 0;
+	for (uint32_t p = threadIdx.x; p < Fmax; p += NT) tcnt[p] = 0;
 	if (threadIdx.x == 0) tflag = 0;
 	__syncthreads();
 	uint4 va[ITEMS][WV], vb[ITEMS][WV];
@@ -2860,10 +2877,13 @@ __global__ void k_batch_reset(hj_stats *st, uint64_t nchunks, unsigned long long
 		res[2 * i + 1] = 0;
 		jflags[i] = 0;
 	}
+	if (blockIdx.x == 0 && threadIdx.x < 8) st->xq[threadIdx.x] = 0;  // gathered batches' queues
 	if (blockIdx.x == 0 && threadIdx.x == 0) {
 		st->work_next = 0;
 		st->flags = 0;
 		st->total_chunks = nchunks;
+		st->narrow_tuples = 0;
+		st->dense_tuples = 0;
 	}
 }
 
@@ -3094,6 +3114,313 @@ __global__ __launch_bounds__(NT) void k_join_b(const hj_bpair *__restrict__ pair
 	}
 }
 
+// ---------------------------------------------------------------------------
+// Gathered batches (16-byte bc batches): no histogram pass and no scatter.
+// Every relation of every join is cut into 4096-tuple tiles that pass A
+// (k_sortchunk<..., true>) counting-sorts by the join's own digit in place,
+// storing a tile as 4-byte positional words, 8-byte {key, row_id} or 16-byte
+// tuples, with a run table per tile.  k_run_transpose_b makes the run tables
+// digit-major per segment, and one persistent k_join_g joins partition
+// (join, digit d) by gathering run d of every tile of the join's R (the LDS
+// table) and of a range of its S tiles (the probes).  A partition's lines
+// that neighbouring digits share are read by workgroups of one XCD at about
+// the same time (items in (join, digit) order from one queue per XCD).
+
+static constexpr uint32_t kGTile = 4096;  // pass-A tile of gathered batches (1024 threads x 4)
+static constexpr uint32_t kGTileBits = 12;
+
+// One 64 x 64 block of a segment's run table ([tile][digit], F + 1 entries
+// a tile, from segs[seg].hist_base) written digit-major into run_t (row d of
+// the segment at hist_base + d * nchunks).
+struct hj_tblk {
+	uint32_t seg, c0, p0, pad;
+};
+__global__ __launch_bounds__(256) void k_run_transpose_b(const uint16_t *__restrict__ run, const hj_seg *__restrict__ segs,
+                                                         const hj_tblk *__restrict__ blk, uint16_t *__restrict__ run_t) {
+	__shared__ uint16_t t[64][66];
+	const hj_tblk b = blk[blockIdx.x];
+	const hj_seg g = segs[b.seg];
+	const uint32_t F = 1u << g.bits, nc = g.nchunks;
+	for (uint32_t e = threadIdx.x; e < 64 * 64; e += 256) {
+		const uint32_t ci = e / 64, pj = e % 64, c = b.c0 + ci, p = b.p0 + pj;
+		t[ci][pj] = (c < nc && p <= F) ? run[g.hist_base + (uint64_t)c * (F + 1) + p] : (uint16_t)0;
+	}
+	__syncthreads();
+	for (uint32_t e = threadIdx.x; e < 64 * 64; e += 256) {
+		const uint32_t pj = e / 64, ci = e % 64, c = b.c0 + ci, p = b.p0 + pj;
+		if (c < nc && p <= F) run_t[g.hist_base + (uint64_t)p * nc + c] = t[ci][pj];
+	}
+}
+
+struct hj_gjoin {
+	uint64_t rtR, rtS;   // digit-major run rows of R's and S's segments
+	uint32_t cR, ncR;    // R: first global tile, tiles
+	uint32_t cS, ncS;    // S: first global tile, tiles
+	uint32_t bits, cpt;  // radix bits; S tiles per work item
+};
+struct hj_gpair {
+	uint32_t join, d;
+};
+
+// 16 bytes at a 4-byte aligned address (a run element of any layout: global
+// loads of several dwords need dword alignment only)
+typedef unsigned int hj_u32x4a __attribute__((ext_vector_type(4), aligned(4)));
+__device__ __forceinline__ uint4 hj_load16a4(const char *p) {
+	const hj_u32x4a v = __builtin_nontemporal_load((const HJ_GLOBAL hj_u32x4a *)p);
+	return make_uint4(v.x, v.y, v.z, v.w);
+}
+
+// A gathered tuple: v its 16 loaded bytes, pz its tile's segment position
+// (a multiple of kGTile) | layout code.  kh = key >> bits (all keys of a
+// partition share the low `bits` bits, so kh decides equality and hashes as
+// bc_join's (key >> B)); rid = row id.
+__device__ __forceinline__ void hj_gdecode(const uint4 &v, uint32_t pz, uint32_t bits, uint64_t &kh, uint64_t &rid) {
+	const uint32_t code = pz & 3u;
+	const uint64_t k16 = (uint64_t)v.x | (uint64_t)v.y << 32, r16 = (uint64_t)v.z | (uint64_t)v.w << 32;
+	const uint64_t kp = v.x >> kGTileBits, rp = (uint64_t)((pz & ~(kGTile - 1)) + (v.x & (kGTile - 1)) + 1);
+	const uint64_t k8 = (uint64_t)(v.x >> bits), r8 = v.y;
+	kh = code == 2 ? kp : code == 1 ? k8 : (k16 >> bits);
+	rid = code == 2 ? rp : code == 1 ? r8 : r16;
+}
+
+// Persistent join of a gathered batch.  Work item = (pair (join, digit),
+// probe-tile index): S tiles [tix * cpt, (tix + 1) * cpt) of that join.
+// NARROW (no 16-byte tile anywhere: every kh and rid fits 32 bits) keeps
+// the LDS table as u64 {kh, rid}; otherwise as two u64 arrays.  Both are
+// launched; the one that does not match the pass-A flag returns at once.
+// LDS: table (kv | [rid] | heads u32[np2(cap)] | next u16[cap]).
+template <int NT, bool NARROW>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_join_g(
+    const hj_gjoin *__restrict__ joins, const hj_gpair *__restrict__ pairs, const uint64_t *__restrict__ tprefix,
+    uint32_t npairs, const uint16_t *__restrict__ run_t, const uint64_t *__restrict__ tdesc, uint32_t cap,
+    hj_stats *st, unsigned long long *__restrict__ res) {
+	if (NARROW == ((st->flags & HJ_DF_WIDE) != 0)) return;
+	constexpr uint32_t NW = NT / HJ_WAVE;
+	constexpr uint32_t RU = 4;               // tuples per lane per step
+	constexpr uint32_t STEP = HJ_WAVE * RU;  // tuples per wave per step
+	extern __shared__ uint64_t lds_u64[];
+	__shared__ uint32_t sred[NW + 1];
+	__shared__ uint64_t s_item[4];
+	__shared__ uint64_t ent_a[NT + 1];  // staged run k: address of its first tuple | layout code
+	__shared__ uint2 ent_o[NT + 1];     // {flat offset, segment position of its tile}; ent_o[NT].x = total
+	const uint32_t nb = hj_np2u(cap), hmask = nb - 1;
+	uint64_t *kv = lds_u64;
+	uint64_t *rv = kv + cap;  // wide table: row ids
+	uint32_t *heads = (uint32_t *)(kv + (NARROW ? cap : 2 * cap));
+	uint16_t *next = (uint16_t *)(heads + nb);
+	const uint32_t lane = threadIdx.x & (HJ_WAVE - 1), wid = threadIdx.x / HJ_WAVE;
+	const uint64_t total = tprefix[npairs];
+	uint32_t cur = blockIdx.x & 7;  // this workgroup's queue (its XCD)
+
+	// runs of digit d of tiles [t0, t1) (t1 - t0 <= NT) of a segment (first
+	// global tile cb, nc tiles, run rows at rt); the first run starts `skip`
+	// tuples in
+	auto stage = [&](uint32_t cb, uint32_t nc, uint64_t rt, uint32_t t0, uint32_t t1, uint32_t d,
+	                 uint32_t skip) -> uint32_t {
+		const uint32_t t = t0 + threadIdx.x;
+		uint32_t len = 0;
+		uint64_t addr = 0;
+		if (t < t1) {
+			uint32_t a = run_t[rt + (uint64_t)d * nc + t];
+			len = run_t[rt + (uint64_t)(d + 1) * nc + t] - a;
+			if (threadIdx.x == 0) {
+				a += skip;
+				len -= skip;
+			}
+			const uint64_t desc = tdesc[cb + t];
+			const uint32_t code = (uint32_t)desc & 3u;
+			addr = ((desc & ~3ull) + (uint64_t)a * (16u >> code)) | code;
+		}
+		uint32_t tot;
+		const uint32_t ex = hj_block_exscan<uint32_t>(len, sred, &tot);
+		ent_a[threadIdx.x] = addr;
+		ent_o[threadIdx.x] = make_uint2(ex, t * kGTile);
+		if (threadIdx.x == NT - 1) ent_o[NT] = make_uint2(tot, 0u);
+		return tot;
+	};
+	struct cursor_t {
+		uint32_t k, off, noff, pz;
+		uint64_t a;
+	};
+	auto seek = [&](uint32_t f) -> cursor_t {
+		uint32_t lo = 0, hi = NT;
+		while (hi - lo > 1) {
+			const uint32_t mid = (lo + hi) >> 1;
+			if (ent_o[mid].x <= f) lo = mid; else hi = mid;
+		}
+		const uint64_t a = ent_a[lo];
+		return cursor_t{lo, ent_o[lo].x, ent_o[lo + 1].x, ent_o[lo].y | ((uint32_t)a & 3u), a};
+	};
+	// flat indices f0 + u*64 + lane, clamped to fe - 1 (every lane issues RU
+	// loads, none under a lane condition); the cursor only moves forward
+	auto load_step = [&](uint32_t f0, uint32_t fe, cursor_t &cs, uint4 (&v)[RU], uint32_t (&pz)[RU]) {
+#pragma unroll
+		for (uint32_t u = 0; u < RU; u++) {
+			const uint32_t f = min(f0 + u * HJ_WAVE + lane, fe - 1);
+			while (f >= cs.noff) {
+				cs.k++;
+				cs.off = cs.noff;
+				cs.noff = ent_o[cs.k + 1].x;
+				cs.a = ent_a[cs.k];
+				cs.pz = ent_o[cs.k].y | ((uint32_t)cs.a & 3u);
+			}
+			const uint32_t code = (uint32_t)cs.a & 3u;
+			v[u] = hj_load16a4((const char *)(uintptr_t)(cs.a & ~3ull) + (uint64_t)(f - cs.off) * (16u >> code));
+			pz[u] = cs.pz;
+		}
+	};
+	unsigned long long cnt = 0, sum = 0;
+	auto probe_step = [&](uint32_t f0, uint32_t fe, const uint4 (&v)[RU], const uint32_t (&pz)[RU], uint32_t bits) {
+		uint32_t c[RU];
+		uint64_t kh[RU], rid[RU];
+#pragma unroll
+		for (uint32_t u = 0; u < RU; u++) {
+			hj_gdecode(v[u], pz[u], bits, kh[u], rid[u]);
+			c[u] = f0 + u * HJ_WAVE + lane < fe ? heads[(uint32_t)kh[u] & hmask] : 0u;
+		}
+		bool any = true;
+		while (any) {
+			any = false;
+#pragma unroll
+			for (uint32_t u = 0; u < RU; u++) {
+				if (!c[u]) continue;
+				if (NARROW) {
+					const uint64_t e = kv[c[u] - 1];
+					if ((uint32_t)e == (uint32_t)kh[u]) {
+						cnt++;
+						sum += (e >> 32) + rid[u];
+					}
+				} else if (kv[c[u] - 1] == kh[u]) {
+					cnt++;
+					sum += rv[c[u] - 1] + rid[u];
+				}
+				c[u] = next[c[u] - 1];
+				any |= c[u] != 0;
+			}
+		}
+	};
+
+	for (;;) {
+		if (threadIdx.x == 0) {
+			uint64_t w = total;
+			uint32_t lo = 0;
+			for (uint32_t tried = 0; tried < 8; tried++, cur = (cur + 1) & 7) {
+				const uint32_t i0 = (uint32_t)((uint64_t)npairs * cur / 8), i1 = (uint32_t)((uint64_t)npairs * (cur + 1) / 8);
+				const uint64_t w0 = tprefix[i0], w1 = tprefix[i1];
+				if (w0 >= w1) continue;
+				const uint64_t k = atomicAdd(&st->xq[cur], 1ULL);
+				if (w0 + k >= w1) continue;
+				w = w0 + k;
+				uint32_t a = i0, b = i1;
+				while (b - a > 1) {
+					const uint32_t mid = (a + b) >> 1;
+					if (tprefix[mid] <= w) a = mid; else b = mid;
+				}
+				lo = a;
+				break;
+			}
+			s_item[0] = w;
+			s_item[1] = lo;
+			s_item[2] = w < total ? w - tprefix[lo] : 0;
+		}
+		__syncthreads();
+		const uint64_t w = s_item[0];
+		const uint32_t q = (uint32_t)s_item[1];
+		const uint32_t tix = (uint32_t)s_item[2];
+		__syncthreads();
+		if (w >= total) break;
+		const hj_gpair pq = pairs[q];
+		const hj_gjoin J = joins[pq.join];
+		const uint32_t d = pq.d;
+		const uint32_t sb = tix * J.cpt, se = min(J.ncS, sb + J.cpt);
+		uint32_t rc = 0;    // next R tile
+		uint32_t roff = 0;  // tuples of tile rc's run already taken by an earlier fill
+		while (rc < J.ncR) {
+			// ---- fill the table with up to `cap` build tuples
+			for (uint32_t i = threadIdx.x; i < nb; i += NT) heads[i] = 0;
+			__syncthreads();
+			uint32_t m = 0;
+			for (;;) {
+				const uint32_t c1 = min(J.ncR, rc + NT);
+				const uint32_t n = c1 - rc;
+				const uint32_t tot = stage(J.cR, J.ncR, J.rtR, rc, c1, d, roff);
+				__syncthreads();
+				const uint32_t T = min(tot, cap - m);
+				// runs taken whole form a prefix; the next one may be taken in part
+				const uint32_t nfull = __syncthreads_count(threadIdx.x < n && ent_o[threadIdx.x + 1].x <= T);
+				const uint32_t roff_new = nfull < n ? (nfull == 0 ? roff : 0) + (T - ent_o[nfull].x) : 0;
+				const uint32_t fb = (uint32_t)((uint64_t)T * wid / NW), fe = (uint32_t)((uint64_t)T * (wid + 1) / NW);
+				if (fb < fe) {
+					cursor_t cs = seek(fb);
+					for (uint32_t f0 = fb; f0 < fe; f0 += STEP) {
+						uint4 v[RU];
+						uint32_t pz[RU];
+						load_step(f0, fe, cs, v, pz);
+#pragma unroll
+						for (uint32_t u = 0; u < RU; u++) {
+							const uint32_t f = f0 + u * HJ_WAVE + lane;
+							if (f >= fe) continue;
+							uint64_t kh, rid;
+							hj_gdecode(v[u], pz[u], J.bits, kh, rid);
+							const uint32_t i = m + f;
+							if (NARROW) {
+								kv[i] = (uint64_t)(uint32_t)kh | rid << 32;
+							} else {
+								kv[i] = kh;
+								rv[i] = rid;
+							}
+							next[i] = (uint16_t)atomicExch(&heads[(uint32_t)kh & hmask], i + 1);
+						}
+					}
+				}
+				m += T;
+				rc += nfull;
+				roff = roff_new;
+				__syncthreads();
+				if (m >= cap || rc >= J.ncR) break;
+			}
+			// ---- probe S tiles [sb, se), NT at a time
+			for (uint32_t kb = sb; m && kb < se; kb += NT) {
+				const uint32_t T = stage(J.cS, J.ncS, J.rtS, kb, min(se, kb + NT), d, 0);
+				__syncthreads();
+				const uint32_t fb = (uint32_t)((uint64_t)T * wid / NW), fe = (uint32_t)((uint64_t)T * (wid + 1) / NW);
+				if (fb < fe) {
+					cursor_t cs = seek(fb);
+					uint4 va[RU], vb[RU];
+					uint32_t pa[RU], pb[RU];
+					load_step(fb, fe, cs, va, pa);
+					for (uint32_t f0 = fb;;) {
+						const uint32_t f1 = f0 + STEP;
+						if (f1 < fe) load_step(f1, fe, cs, vb, pb);
+						probe_step(f0, fe, va, pa, J.bits);
+						if (f1 >= fe) break;
+						const uint32_t f2 = f1 + STEP;
+						if (f2 < fe) load_step(f2, fe, cs, va, pa);
+						probe_step(f1, fe, vb, pb, J.bits);
+						if (f2 >= fe) break;
+						f0 = f2;
+					}
+				}
+				__syncthreads();
+			}
+		}
+		// this item's matches go to its join's slot (one atomic pair per wave)
+		cnt = hj_wave_sum(cnt);
+		sum = hj_wave_sum(sum);
+		if (lane == 0 && cnt) {
+			atomicAdd(&res[2 * pq.join], cnt);
+			atomicAdd(&res[2 * pq.join + 1], sum);
+		}
+		cnt = sum = 0;
+		__syncthreads();
+	}
+}
+
+// LDS bytes of k_join_g's table for `cap` build tuples
+static size_t join_g_lds(uint32_t cap, bool narrow) {
+	return (size_t)cap * (narrow ? 10 : 18) + (size_t)hj_np2u(cap) * 4 + 8;
+}
+
 // Threads of a batch-join workgroup: its LDS table (about 64 KB for 2048
 // build tuples) lets two share a CU, so 512 threads keep 16 waves per CU.
 static constexpr int kJoinBNT = 512;
@@ -3135,6 +3462,10 @@ struct hj_batch {
 	// device pointers into meta
 	unsigned long long *d_res = nullptr;
 	unsigned int *d_jflags = nullptr;
+	// algorithmic bytes: gathered batches count from the tile layouts pass A
+	// chose; scattered ones from these tuple counts and the narrow flag
+	bool gathered = false;
+	uint64_t part_tuples = 0, unpart_tuples = 0;
 };
 
 static void batch_free(hj_batch *b) {
@@ -3161,6 +3492,212 @@ static const void *join_b_kernel(uint32_t wv, bool lp) {
 	case 8: return lp ? join_b_fn<8, true>() : join_b_fn<8, false>();
 	default: return lp ? join_b_fn<16, true>() : join_b_fn<16, false>();
 	}
+}
+
+// Radix bits of a join in a gathered batch: about 2048 build tuples per
+// partition (at most 12 bits); build sides up to 2048 tuples are one
+// partition.
+static uint32_t gbatch_bits(uint64_t nr) {
+	if (nr <= 2048) return 0;
+	return std::min<uint32_t>(12, hj_log2_ceil((nr + 2047) / 2048));
+}
+
+// Whether a batch runs gathered: 16-byte bc-family joins whose relations
+// each hold fewer than 2^30 tuples (positions and row ids of 4-byte words
+// fit 32 bits), unless the caller asks for the scatter path.
+static bool gbatch_eligible(bool lp, uint32_t W, uint32_t njoins, const hj_relation_t *R, const hj_relation_t *S,
+                            const hj_params_t *params) {
+	if (lp || W != 16 || (params && (params->flags & HJ_F_BATCH_SCATTER))) return false;
+	for (uint32_t j = 0; j < njoins; j++)
+		if (R[j].tuple_num >= (1ull << 30) || S[j].tuple_num >= (1ull << 30)) return false;
+	return true;
+}
+
+// Plan and capture a gathered batch into b (whose R, S, njoins are set).
+static int gbatch_create(hj_ctx_t *ctx, hj_batch *b) {
+	const uint32_t njoins = b->njoins;
+	std::vector<hj_seg> segs;
+	std::vector<uint32_t> cmap;
+	std::vector<hj_gjoin> joins(njoins);
+	std::vector<hj_gpair> pairs;
+	std::vector<uint64_t> tprefix(1, 0);
+	std::vector<hj_tblk> blocks;
+	uint64_t part_n = 0, runs_n = 0, tuples = 0, maxr = 1;
+	uint32_t maxbits = 0;
+	auto add_seg = [&](const hj_relation_t &rel, uint32_t bits) -> uint32_t {
+		hj_seg g{};
+		g.in = (const uint4 *)rel.tuples;
+		g.len = rel.tuple_num;
+		g.out_start = part_n;
+		g.bits = bits;
+		g.chunk_base = (uint32_t)cmap.size();
+		g.nchunks = (uint32_t)((g.len + kGTile - 1) / kGTile);
+		g.hist_base = runs_n;
+		const uint32_t F = 1u << bits;
+		runs_n += (uint64_t)g.nchunks * (F + 1);
+		part_n += g.len;
+		const uint32_t si = (uint32_t)segs.size();
+		for (uint32_t c = 0; c < g.nchunks; c++) cmap.push_back(si);
+		for (uint32_t c0 = 0; c0 < g.nchunks; c0 += 64)
+			for (uint32_t p0 = 0; p0 <= F; p0 += 64) blocks.push_back(hj_tblk{si, c0, p0, 0});
+		segs.push_back(g);
+		return si;
+	};
+	for (uint32_t j = 0; j < njoins; j++) {
+		const uint64_t nr = b->R[j].tuple_num, ns = b->S[j].tuple_num;
+		tuples += nr + ns;
+		const uint32_t bits = gbatch_bits(nr);
+		maxbits = std::max(maxbits, bits);
+		const hj_seg &gr = segs[add_seg(b->R[j], bits)];
+		const uint64_t rt_r = gr.hist_base, c_r = gr.chunk_base, nc_r = gr.nchunks;
+		const hj_seg &gs = segs[add_seg(b->S[j], bits)];
+		hj_gjoin &J = joins[j];
+		J.rtR = rt_r;
+		J.cR = (uint32_t)c_r;
+		J.ncR = (uint32_t)nc_r;
+		J.rtS = gs.hist_base;
+		J.cS = gs.chunk_base;
+		J.ncS = gs.nchunks;
+		J.bits = bits;
+		J.cpt = std::max<uint32_t>(1, J.ncS);
+		maxr = std::max<uint64_t>(maxr, (nr >> bits) + 1);
+		if (bits) b->info.n_partitioned++;
+		else b->info.n_unpartitioned++;
+	}
+	// work items: one per (join, digit) with the whole probe side; when that
+	// leaves fewer than four items per workgroup of the join grid, probe sides
+	// are cut into tile ranges of at least four times the build side's tiles
+	const unsigned approx_grid = (unsigned)ctx->num_cu * 3;
+	uint64_t items = 0;
+	for (uint32_t j = 0; j < njoins; j++)
+		if (joins[j].ncR && joins[j].ncS) items += 1ull << joins[j].bits;
+	if (items && items < 4ull * approx_grid) {
+		const uint64_t k = (4ull * approx_grid + items - 1) / items;
+		for (uint32_t j = 0; j < njoins; j++) {
+			hj_gjoin &J = joins[j];
+			const uint32_t cpt = (uint32_t)((J.ncS + k - 1) / k);
+			J.cpt = std::max<uint32_t>(std::max<uint32_t>(1, 4 * J.ncR), cpt);
+		}
+	}
+	for (uint32_t j = 0; j < njoins; j++) {
+		const hj_gjoin &J = joins[j];
+		if (!J.ncR || !J.ncS) continue;
+		const uint64_t per = (J.ncS + J.cpt - 1) / J.cpt;
+		for (uint32_t d = 0; d < (1u << J.bits); d++) {
+			pairs.push_back(hj_gpair{j, d});
+			tprefix.push_back(tprefix.back() + per);
+		}
+	}
+	const uint32_t nseg = (uint32_t)segs.size(), npairs = (uint32_t)pairs.size();
+	const uint64_t nchunks = cmap.size(), nblk = blocks.size();
+	b->info.njoins = njoins;
+	b->info.tuples = tuples;
+	b->info.work_items = tprefix.back();
+	b->info.gathered = 1;
+	b->gathered = true;
+	b->part_tuples = tuples;
+	if (nblk >= (1ull << 31)) return hj_fail(ctx, HJ_EINVAL, "batch too large (%llu run-table blocks)", (unsigned long long)nblk);
+	// LDS table: the largest expected partition with headroom (larger ones are
+	// joined in fills), at most what lets three workgroups share a CU
+	const void *kn = (const void *)k_join_g<512, true>, *kw = (const void *)k_join_g<512, false>;
+	const size_t lds_cu = 160 * 1024;
+	uint32_t cap = (uint32_t)std::min<uint64_t>(65535, std::max<uint64_t>(1024, maxr + maxr / 4 + 64));
+	while (cap > 1024 && 3 * (join_g_lds(cap, true) + static_lds(kn) + 1024) > lds_cu) cap -= 256;
+	const size_t ldsn = join_g_lds(cap, true), ldsw = join_g_lds(cap, false);
+	if (ldsw + static_lds(kw) > ctx->prop.sharedMemPerBlock) return hj_fail(ctx, HJ_EINVAL, "batch LDS table too big");
+	const size_t sldsA = (size_t)kGTile * 16 + (2 * ((size_t)1 << maxbits) + 1) * 4 + 64 * 4;
+	if (sldsA + 1024 > ctx->prop.sharedMemPerBlock) return hj_fail(ctx, HJ_EINVAL, "batch radix bits too big");
+
+	// ---- device buffers: tuples | run tables | digit-major run rows | tile
+	// descriptors, and the plan
+	int rc;
+	if ((rc = hj_reserve(ctx, b->part, part_n * 16 + 64))) return rc;
+	size_t off = 0;
+	auto take = [&](size_t bytes) {
+		size_t o = off;
+		off += (bytes + 255) & ~(size_t)255;
+		return o;
+	};
+	const size_t o_run = take(runs_n * 2 + 2), o_runt = take(runs_n * 2 + 2), o_desc = take(nchunks * 8 + 8);
+	if ((rc = hj_reserve(ctx, b->hist, off))) return rc;
+	off = 0;
+	const size_t o_segs = take(nseg * sizeof(hj_seg)), o_cmap = take(nchunks * 4 + 4);
+	const size_t o_joins = take(njoins * sizeof(hj_gjoin) + 8), o_pairs = take((size_t)npairs * sizeof(hj_gpair) + 8);
+	const size_t o_tp = take(tprefix.size() * 8), o_blk = take(nblk * sizeof(hj_tblk) + 16);
+	if ((rc = hj_reserve(ctx, b->meta, off))) return rc;
+	if ((rc = hj_reserve(ctx, b->stats, sizeof(hj_stats)))) return rc;
+	if ((rc = hj_reserve(ctx, b->res, (size_t)njoins * 20 + 64))) return rc;
+	char *H = (char *)b->hist.p, *M = (char *)b->meta.p;
+	uint16_t *d_run = (uint16_t *)(H + o_run), *d_runt = (uint16_t *)(H + o_runt);
+	uint64_t *d_desc = (uint64_t *)(H + o_desc);
+	hj_seg *d_segs = (hj_seg *)(M + o_segs);
+	uint32_t *d_cmap = (uint32_t *)(M + o_cmap);
+	hj_gjoin *d_joins = (hj_gjoin *)(M + o_joins);
+	hj_gpair *d_pairs = (hj_gpair *)(M + o_pairs);
+	uint64_t *d_tp = (uint64_t *)(M + o_tp);
+	hj_tblk *d_blk = (hj_tblk *)(M + o_blk);
+	b->d_res = (unsigned long long *)b->res.p;
+	b->d_jflags = (unsigned int *)((char *)b->res.p + (size_t)njoins * 16);
+	hj_stats *d_st = (hj_stats *)b->stats.p;
+	auto up = [&](void *dst, const void *src, size_t bytes) -> int {
+		if (!bytes) return HJ_OK;
+		hipError_t e = hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice);
+		return e == hipSuccess ? HJ_OK : hj_fail(ctx, HJ_EHIP, "batch upload: %s", hipGetErrorString(e));
+	};
+	if ((rc = up(d_segs, segs.data(), nseg * sizeof(hj_seg))) || (rc = up(d_cmap, cmap.data(), nchunks * 4)) ||
+	    (rc = up(d_joins, joins.data(), njoins * sizeof(hj_gjoin))) ||
+	    (rc = up(d_pairs, pairs.data(), (size_t)npairs * sizeof(hj_gpair))) ||
+	    (rc = up(d_tp, tprefix.data(), tprefix.size() * 8)) || (rc = up(d_blk, blocks.data(), nblk * sizeof(hj_tblk))))
+		return rc;
+	b->info.scratch_bytes = b->part.cap + b->hist.cap + b->meta.cap + b->res.cap + b->stats.cap;
+
+	// kernel attributes before capture
+	auto kA = k_sortchunk<1, 1024, 4, true>;
+	allow_lds(ctx, (const void *)kA);
+	allow_lds(ctx, kn);
+	allow_lds(ctx, kw);
+	const unsigned gA = std::max(1u, std::min<unsigned>((unsigned)std::min<uint64_t>(nchunks, (uint64_t)ctx->num_cu * 4),
+	                                                    resident_grid(ctx, (const void *)kA, 1024, sldsA)));
+	const unsigned gn = resident_grid(ctx, kn, 512, ldsn), gw = resident_grid(ctx, kw, 512, ldsw);
+	const bool narrow_ok = true, dense_ok = true;  // every relation holds fewer than 2^30 tuples
+
+	const bool kt = ctx->kernel_times;
+	ctx->kernel_times = false;
+	if (hipEventCreate(&b->e0) != hipSuccess || hipEventCreate(&b->e1) != hipSuccess) {
+		(void)hipGetLastError();
+		ctx->kernel_times = kt;
+		return hj_fail(ctx, HJ_EHIP, "event create");
+	}
+	hipError_t e = hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal);
+	if (e != hipSuccess) {
+		ctx->kernel_times = kt;
+		return hj_fail(ctx, HJ_EHIP, "begin capture: %s", hipGetErrorString(e));
+	}
+	hipLaunchKernelGGL(k_batch_reset, dim3(std::max<uint32_t>(1, std::min<uint32_t>(1024, (njoins + 255) / 256))),
+	                   dim3(256), 0, ctx->stream, d_st, nchunks, b->d_res, njoins, b->d_jflags);
+	if (nchunks) {
+		hipLaunchKernelGGL(kA, dim3(gA), dim3(1024), sldsA, ctx->stream, d_segs, d_cmap, d_st, 0u, maxbits,
+		                   (uint4 *)b->part.p, d_run, d_desc, narrow_ok, dense_ok, (uint8_t *)nullptr);
+		hipLaunchKernelGGL(k_run_transpose_b, dim3((unsigned)nblk), dim3(256), 0, ctx->stream, d_run, d_segs, d_blk,
+		                   d_runt);
+	}
+	if (npairs) {
+		hipLaunchKernelGGL((k_join_g<512, true>), dim3(gn), dim3(512), ldsn, ctx->stream, d_joins, d_pairs, d_tp, npairs,
+		                   d_runt, d_desc, cap, d_st, b->d_res);
+		hipLaunchKernelGGL((k_join_g<512, false>), dim3(gw), dim3(512), ldsw, ctx->stream, d_joins, d_pairs, d_tp, npairs,
+		                   d_runt, d_desc, cap, d_st, b->d_res);
+	}
+	hipGraph_t g = nullptr;
+	e = hipStreamEndCapture(ctx->stream, &g);
+	ctx->kernel_times = kt;
+	if (e != hipSuccess) {
+		if (g) (void)hipGraphDestroy(g);
+		return hj_fail(ctx, HJ_EHIP, "end capture: %s", hipGetErrorString(e));
+	}
+	b->graph = g;
+	e = hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0);
+	if (e != hipSuccess) return hj_fail(ctx, HJ_EHIP, "graph instantiate: %s", hipGetErrorString(e));
+	return HJ_OK;
 }
 
 extern "C" int hj_batch_create(hj_ctx_t *ctx, const char *algo, uint32_t njoins, const hj_relation_t *R,
@@ -3192,6 +3729,15 @@ extern "C" int hj_batch_create(hj_ctx_t *ctx, const char *algo, uint32_t njoins,
 	b->wv = W / 16;
 	b->R.assign(R, R + njoins);
 	b->S.assign(S, S + njoins);
+	if (gbatch_eligible(lp, W, njoins, R, S, params)) {
+		const int grc = gbatch_create(ctx, b);
+		if (grc != HJ_OK) {
+			batch_free(b);
+			return grc;
+		}
+		*out = b;
+		return HJ_OK;
+	}
 	const uint32_t wv = b->wv;
 	// tile shape and write-combining line (part_variant bits 0-6): the
 	// caller's, else the tuned shape of the size class holding most of the
@@ -3248,8 +3794,10 @@ extern "C" int hj_batch_create(hj_ctx_t *ctx, const char *algo, uint32_t njoins,
 				part_n += g.len;
 				segs.push_back(g);
 			}
+			b->part_tuples += J.nr + J.ns;
 		} else {
 			b->info.n_unpartitioned++;
+			b->unpart_tuples += J.nr + J.ns;
 		}
 	}
 	b->info.n_partitioned = njoins - b->info.n_unpartitioned;
@@ -3389,7 +3937,20 @@ extern "C" int hj_batch_run(hj_batch_t *b, hj_join_result_t *results, hj_batch_i
 	std::vector<unsigned char> host((size_t)b->njoins * 20 + 8);
 	if (b->njoins)
 		HJ_HIP(ctx, hipMemcpyAsync(host.data(), b->res.p, (size_t)b->njoins * 20, hipMemcpyDeviceToHost, ctx->stream));
+	hj_stats hs{};
+	HJ_HIP(ctx, hipMemcpyAsync(&hs, b->stats.p, sizeof(hj_stats), hipMemcpyDeviceToHost, ctx->stream));
 	HJ_HIP(ctx, hipStreamSynchronize(ctx->stream));
+	if (b->gathered) {
+		// pass A reads 16 B and writes each tile at its layout; the join reads that
+		const uint64_t N = b->part_tuples, nt = hs.narrow_tuples, nd = hs.dense_tuples;
+		b->info.algo_bytes = 16 * N + 2 * (4 * nd + 8 * (nt - nd) + 16 * (N - nt));
+	} else {
+		// the scatter reads W and writes W (8 B for narrow 16-byte tuples), the
+		// join reads that; unpartitioned joins are read once (the histogram
+		// pass is not counted)
+		const uint64_t out = (b->W == 16 && !(hs.flags & HJ_DF_WIDE)) ? 8 : b->W;
+		b->info.algo_bytes = b->part_tuples * (b->W + 2 * out) + b->unpart_tuples * b->W;
+	}
 	float ms = 0.f;
 	HJ_HIP(ctx, hipEventElapsedTime(&ms, b->e0, b->e1));
 	b->info.device_ms = ms;

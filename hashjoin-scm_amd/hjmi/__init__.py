@@ -26,6 +26,7 @@ HJ_LOC_DEVICE, HJ_LOC_HOST = 0, 1
 HJ_F_KERNEL_TIMES = 1
 HJ_F_NO_NARROW = 2
 HJ_F_NO_POSITIONAL = 4
+HJ_F_BATCH_SCATTER = 8
 HJ_FB_LP_DUPLICATES, HJ_FB_LP_OVERSIZE, HJ_FB_SC_REBUILD = 1, 2, 4
 
 WL_PKFK_UNIFORM, WL_PKFK_REF, WL_PAYLOAD_REF, WL_ZIPF, WL_RANDOM = 0, 1, 2, 3, 4
@@ -83,7 +84,8 @@ class JoinResult(C.Structure):
 class BatchInfo(C.Structure):
     _fields_ = [("device_ms", C.c_double), ("fallback_ms", C.c_double), ("njoins", C.c_uint32),
                 ("n_unpartitioned", C.c_uint32), ("n_partitioned", C.c_uint32), ("n_fallback", C.c_uint32),
-                ("work_items", C.c_uint64), ("tuples", C.c_uint64), ("scratch_bytes", C.c_uint64)]
+                ("work_items", C.c_uint64), ("tuples", C.c_uint64), ("scratch_bytes", C.c_uint64),
+                ("algo_bytes", C.c_uint64), ("gathered", C.c_uint32), ("reserved", C.c_uint32)]
 
 
 class ServeConfig(C.Structure):
@@ -364,8 +366,8 @@ class Context:
         self._check(self.lib.hj_bin_write(self.h, path.encode(), C.c_void_p(rel.ptr), rel.tuple_num,
                                           rel.tuple_size), "hj_bin_write")
 
-    def batch(self, algo: str, Rs, Ss, part_variant: int = 0) -> "Batch":
-        return Batch(self, algo, Rs, Ss, part_variant)
+    def batch(self, algo: str, Rs, Ss, part_variant: int = 0, flags: int = 0) -> "Batch":
+        return Batch(self, algo, Rs, Ss, part_variant, flags)
 
     def autotune(self, width: int = 16, s_ratio: int = 16, min_log2: int = 10, max_log2: int = 20):
         self._check(self.lib.hj_autotune(self.h, width, s_ratio, min_log2, max_log2), "hj_autotune")
@@ -457,14 +459,14 @@ def algo_names():
 class Batch:
     """Many joins as one captured HIP graph (hj_batch_*); per-join results."""
 
-    def __init__(self, ctx: Context, algo: str, Rs, Ss, part_variant: int = 0):
+    def __init__(self, ctx: Context, algo: str, Rs, Ss, part_variant: int = 0, flags: int = 0):
         if len(Rs) != len(Ss):
             raise ValueError("one build and one probe relation per join")
         self.ctx, self.n = ctx, len(Rs)
         self._R = (Relation * max(1, self.n))(*[r.as_c() for r in Rs])
         self._S = (Relation * max(1, self.n))(*[x.as_c() for x in Ss])
         h = C.c_void_p()
-        prm = Params(part_variant=part_variant)
+        prm = Params(part_variant=part_variant, flags=flags)
         ctx._check(ctx.lib.hj_batch_create(ctx.h, algo.encode(), self.n, self._R, self._S, C.byref(prm),
                                            C.byref(h)), "hj_batch_create")
         self.h = h

@@ -84,6 +84,9 @@ typedef struct {
 #define HJ_F_NO_POSITIONAL 4u /* two radix passes: store every narrow first-
                                  pass tile as 8-byte {key, row_id}, also where
                                  the row ids are the positions + 1            */
+#define HJ_F_BATCH_SCATTER 8u /* batches of 16-byte bc joins: partition with a
+                                 histogram and a one-pass scatter instead of
+                                 first-pass tiles and gathered joins          */
 
 /* ---- result: matched_cnt/check_sum as printed by the reference plus the
  *      timekeeper_t phase times (src/inc/store.h:74-78) ---------------------- */
@@ -168,10 +171,13 @@ int hj_join_noalloc(hj_ctx_t *ctx, const char *algo, const hj_relation_t *R, con
 
 /* ---- batched joins (no reference counterpart: the reference runs one join
  * per process).  Many independent joins run as one captured HIP graph: one
- * partition launch over all joins that need partitioning (radix bits chosen
- * per join by size class), then one persistent join kernel pulling (join,
- * partition, probe tile) items from a device work queue.  Results are
- * per join and bit-identical to hj_join on the same relations.
+ * partition launch over all joins (radix bits chosen per join by size),
+ * then one persistent join kernel pulling (join, partition, probe tile)
+ * items from a device work queue.  16-byte bc-family batches sort every
+ * relation's tiles by the join's digit in place and join each partition by
+ * gathering its runs from those tiles; lp batches, wider tuples and
+ * HJ_F_BATCH_SCATTER partition with a histogram and a one-pass scatter.
+ * Results are per join and bit-identical to hj_join on the same relations.
  * Relations are borrowed for the life of the batch and must stay in device
  * memory; their contents may change between runs. */
 typedef struct {
@@ -189,6 +195,12 @@ typedef struct {
 	uint64_t work_items;     /* (join, partition, tile) items in the queue */
 	uint64_t tuples;         /* sum of |R|+|S|                             */
 	uint64_t scratch_bytes;  /* device memory held by the batch            */
+	uint64_t algo_bytes;     /* algorithmic HBM bytes of the last run: each
+	                            pass reads and writes its tuples at the
+	                            layout it chose (4-, 8- or 16-byte), the join
+	                            reads them once                             */
+	uint32_t gathered;       /* 1: tile-sorted and gathered, 0: scattered  */
+	uint32_t reserved;
 } hj_batch_info_t;
 
 typedef struct hj_batch hj_batch_t;

@@ -1,0 +1,156 @@
+"""GPU tests of gathered batches: 16-byte bc batches partitioned by
+first-pass tiles (4-, 8- or 16-byte tile layouts) and joined by gathering each
+partition's runs from those tiles.  Every join's (matched_cnt, check_sum) is
+checked against the oracle and against the same batch on the histogram +
+scatter path (HJ_F_BATCH_SCATTER): empty and one-tuple relations, build sides
+on both sides of the 2048-tuple partition target, duplicate build keys,
+misses, keys and row ids above 32 bits (16-byte tiles), row ids that are not
+positions (8-byte tiles), build sides that one digit holds whole (joined in
+several LDS fills), and a single large join whose probe side is cut into
+tile ranges."""
+import numpy as np
+import pytest
+
+import hjmi
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _tuples(keys, rids):
+    a = np.empty((len(keys), 2), dtype=np.uint64)
+    a[:, 0] = keys
+    a[:, 1] = rids
+    return a.view(np.uint8).reshape(-1)
+
+
+def _run_both(ctx, Rs, Ss, algo="phj_radix_bc"):
+    g = ctx.batch(algo, Rs, Ss)
+    got, info = g.run()
+    again, _ = g.run()  # graph replay
+    g.close()
+    sc = ctx.batch(algo, Rs, Ss, flags=hjmi.HJ_F_BATCH_SCATTER)
+    ref, _ = sc.run()
+    sc.close()
+    assert again == got
+    return got, ref, info
+
+
+def _upload_all(ctx, pairs):
+    Rs = [ctx.upload(r, 16) if r.size else hjmi.DeviceRelation(0, 0, 16) for r, _ in pairs]
+    Ss = [ctx.upload(s, 16) if s.size else hjmi.DeviceRelation(0, 0, 16) for _, s in pairs]
+    return Rs, Ss
+
+
+def _free(ctx, rels):
+    for r in rels:
+        if r.ptr:
+            ctx.free_relation(r)
+
+
+def test_gathered_batch_pkfk_sizes_vs_oracle(ctx):
+    sizes = [(1, 16), (3, 48), (2048, 32768), (2049, 32784), (5000, 80000), (70000, 1 << 20), (300001, 4800016),
+             (1 << 20, 1 << 24)]
+    hs = [O.generate("pkfk", nr, ns, 16, seed=400 + i) for i, (nr, ns) in enumerate(sizes)]
+    want = [O.pkfk_closed_form(S, 16) for _, S in hs]
+    Rs, Ss = _upload_all(ctx, hs)
+    got, ref, info = _run_both(ctx, Rs, Ss)
+    assert info["n_partitioned"] >= 4 and info["n_unpartitioned"] >= 2
+    for i in range(len(sizes)):
+        assert got[i] == (want[i]["matched_cnt"], want[i]["check_sum"]), (i, sizes[i])
+        assert ref[i] == got[i]
+    _free(ctx, Rs + Ss)
+
+
+def test_gathered_batch_empty_sides(ctx):
+    R1, S1 = O.generate("pkfk", 5000, 20000, 16, seed=7)
+    e = np.zeros(0, dtype=np.uint8)
+    pairs = [(R1, e), (e, S1), (e, e), (R1, S1)]
+    Rs, Ss = _upload_all(ctx, pairs)
+    got, ref, _ = _run_both(ctx, Rs, Ss)
+    w = O.pkfk_closed_form(S1, 16)
+    assert got[0] == (0, 0) and got[1] == (0, 0) and got[2] == (0, 0)
+    assert got[3] == (w["matched_cnt"], w["check_sum"])
+    assert ref == got
+    _free(ctx, Rs + Ss)
+
+
+def test_gathered_batch_duplicates_misses_and_row_ids(ctx):
+    """Random keys over a narrow range (duplicate build keys, probe misses)
+    with row ids that are not positions: 8-byte tiles on both sides."""
+    rng = np.random.default_rng(11)
+    pairs = []
+    for nr, ns, kr in ((100, 5000, 60), (3000, 30000, 1500), (9000, 100000, 12000), (40000, 200000, 30000)):
+        rk = rng.integers(1, kr + 1, nr, dtype=np.uint64)
+        sk = rng.integers(1, kr + kr // 4 + 1, ns, dtype=np.uint64)
+        rr = rng.integers(1, 1 << 31, nr, dtype=np.uint64)
+        sr = rng.integers(1, 1 << 31, ns, dtype=np.uint64)
+        pairs.append((_tuples(rk, rr), _tuples(sk, sr)))
+    want = [O.aggregate(R, S, 16) for R, S in pairs]
+    Rs, Ss = _upload_all(ctx, pairs)
+    got, ref, _ = _run_both(ctx, Rs, Ss)
+    for i, w in enumerate(want):
+        assert got[i] == (w["matched_cnt"], w["check_sum"]), i
+    assert ref == got
+    _free(ctx, Rs + Ss)
+
+
+def test_gathered_batch_wide_keys_and_row_ids(ctx):
+    """A key above 2^32 in one join and a row id above 2^32 in another make
+    those tiles 16-byte, and the whole batch joins with the wide table."""
+    rng = np.random.default_rng(12)
+    pairs = []
+    for i, (nr, ns) in enumerate(((4000, 64000), (20000, 320000), (700, 11200))):
+        rk = rng.permutation(nr).astype(np.uint64) + 1
+        sk = rk[rng.integers(0, nr, ns)]
+        rr = np.arange(1, nr + 1, dtype=np.uint64)
+        sr = np.arange(1, ns + 1, dtype=np.uint64)
+        if i == 0:
+            rk = rk + np.uint64(1 << 35)
+            sk = sk + np.uint64(1 << 35)
+        if i == 1:
+            sr = sr + np.uint64(1 << 40)
+        pairs.append((_tuples(rk, rr), _tuples(sk, sr)))
+    want = [O.aggregate(R, S, 16) for R, S in pairs]
+    Rs, Ss = _upload_all(ctx, pairs)
+    got, ref, _ = _run_both(ctx, Rs, Ss)
+    for i, w in enumerate(want):
+        assert got[i] == (w["matched_cnt"], w["check_sum"]), i
+    assert ref == got
+    _free(ctx, Rs + Ss)
+
+
+def test_gathered_batch_one_digit_holds_the_build_side(ctx):
+    """Build keys that are multiples of 2^12 all fall into digit 0 (any
+    radix width of a batch), so that partition is joined in many LDS fills,
+    and the fills split runs."""
+    rng = np.random.default_rng(13)
+    nr, ns = 30000, 120000
+    rk = (rng.permutation(nr).astype(np.uint64) + 1) << np.uint64(12)
+    sk = rk[rng.integers(0, nr, ns)]
+    sk[::7] += np.uint64(1)  # misses, in digit 1
+    R = _tuples(rk, np.arange(1, nr + 1, dtype=np.uint64))
+    S = _tuples(sk, np.arange(1, ns + 1, dtype=np.uint64))
+    R2, S2 = O.generate("pkfk", 6000, 96000, 16, seed=5)
+    pairs = [(R, S), (R2, S2)]
+    want = [O.aggregate(a, b, 16) for a, b in pairs]
+    Rs, Ss = _upload_all(ctx, pairs)
+    got, ref, _ = _run_both(ctx, Rs, Ss)
+    for i, w in enumerate(want):
+        assert got[i] == (w["matched_cnt"], w["check_sum"]), i
+    assert ref == got
+    _free(ctx, Rs + Ss)
+
+
+def test_gathered_batch_single_large_join(ctx):
+    """One join alone in a batch: its probe side is cut into tile ranges so
+    the join grid has work; the result equals the plain join's."""
+    R, S = ctx.generate("pkfk", 1 << 22, 1 << 26, 16, seed=21)
+    b = ctx.batch("phj_radix_bc", [R], [S])
+    got, info = b.run()
+    b.close()
+    assert info["work_items"] > (1 << 11)
+    alone = ctx.join("phj_radix_bc", R, S)
+    assert got[0] == (alone["matched_cnt"], alone["check_sum"])
+    assert got[0][0] == 1 << 26
+    _free(ctx, [R, S])
