@@ -288,6 +288,18 @@ def run_batched(args, hjmi):
     same = all(o == (p["matched_cnt"], p["check_sum"]) for o, p in zip(out, plain_res))
     if not same:
         raise SystemExit("batched results differ from plain joins")
+    # per-kernel device times: the same batch launched without its graph,
+    # each kernel between two events (HJ_F_KERNEL_TIMES), after the timed runs
+    kernel_ms = {}
+    if info.get("gathered"):
+        bt = ctx.batch(algo, Rs, Ss, flags=args.batch_flags | hjmi.HJ_F_KERNEL_TIMES)
+        bt.run()
+        kout, kinfo = bt.run()
+        bt.close()
+        if kout != out:
+            raise SystemExit("batch with kernel times differs from the graph's results")
+        kernel_ms = {k: round(v, 3) for k, v in kinfo["kernel_ms"].items()}
+        log("batch kernels (ms):", kernel_ms)
     tb, tp = min(batch_times), min(plain_times)
     # Roofline of the batch graph (one launch): the library's algorithmic
     # bytes of the run -- gathered batches: pass A reads 16 B per tuple and
@@ -308,7 +320,7 @@ def run_batched(args, hjmi):
         "plain_joins_per_s": args.joins / tp, "speedup_vs_plain": tp / tb,
         "batch_tuples_per_s": tuples / tb, "plain_tuples_per_s": tuples / tp,
         "batch_device_ms": min(dev_ms), "identical_results": same, "autotune_s": tune_s,
-        "batch_info": {k: v for k, v in info.items()},
+        "batch_info": {k: v for k, v in info.items() if k != "kernel_ms"}, "batch_kernel_ms": kernel_ms,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_GBPS, "unit": "GB/s",
                      "frac": round(achieved / PEAK_GBPS, 4), "traffic": None, "kernel": "batch graph",
                      "algo_bytes": algo_bytes, "device_ms": round(dev_s * 1e3, 3)},

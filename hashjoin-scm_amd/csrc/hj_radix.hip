@@ -3126,8 +3126,8 @@ __global__ __launch_bounds__(NT) void k_join_b(const hj_bpair *__restrict__ pair
 // that neighbouring digits share are read by workgroups of one XCD at about
 // the same time (items in (join, digit) order from one queue per XCD).
 
-static constexpr uint32_t kGTile = 4096;  // pass-A tile of gathered batches (1024 threads x 4)
-static constexpr uint32_t kGTileBits = 12;
+static constexpr uint32_t kGTile = 8192;  // pass-A tile of gathered batches (1024 threads x 8)
+static constexpr uint32_t kGTileBits = 13;
 
 // One 64 x 64 block of a segment's run table ([tile][digit], F + 1 entries
 // a tile, from segs[seg].hist_base) written digit-major into run_t (row d of
@@ -3158,8 +3158,10 @@ struct hj_gjoin {
 	uint32_t cS, ncS;    // S: first global tile, tiles
 	uint32_t bits, cpt;  // radix bits; S tiles per work item
 };
-struct hj_gpair {
-	uint32_t join, d;
+// A work item: join, digit (12 bits) | probe-tile range index << 12.
+static constexpr uint32_t kGItemDigitBits = 12, kGItemDigits = 1u << kGItemDigitBits;
+struct hj_gitem {
+	uint32_t join, dt;
 };
 
 // 16 bytes at a 4-byte aligned address (a run element of any layout: global
@@ -3169,6 +3171,14 @@ __device__ __forceinline__ uint4 hj_load16a4(const char *p) {
 	const hj_u32x4a v = __builtin_nontemporal_load((const HJ_GLOBAL hj_u32x4a *)p);
 	return make_uint4(v.x, v.y, v.z, v.w);
 }
+
+typedef unsigned int hj_u32x2a __attribute__((ext_vector_type(2), aligned(4)));
+__device__ __forceinline__ uint2 hj_load8a4(const char *p) {
+	const hj_u32x2a v = __builtin_nontemporal_load((const HJ_GLOBAL hj_u32x2a *)p);
+	return make_uint2(v.x, v.y);
+}
+__device__ __forceinline__ void hj_gload(uint4 &v, const char *p) { v = hj_load16a4(p); }
+__device__ __forceinline__ void hj_gload(uint2 &v, const char *p) { v = hj_load8a4(p); }
 
 // A gathered tuple: v its 16 loaded bytes, pz its tile's segment position
 // (a multiple of kGTile) | layout code.  kh = key >> bits (all keys of a
@@ -3182,6 +3192,13 @@ __device__ __forceinline__ void hj_gdecode(const uint4 &v, uint32_t pz, uint32_t
 	kh = code == 2 ? kp : code == 1 ? k8 : (k16 >> bits);
 	rid = code == 2 ? rp : code == 1 ? r8 : r16;
 }
+// the same from the first 8 bytes, when no tile is 16-byte (layout 1 or 2):
+// kh and rid fit 32 bits
+__device__ __forceinline__ void hj_gdecode(const uint2 &v, uint32_t pz, uint32_t bits, uint32_t &kh, uint32_t &rid) {
+	const bool pos = (pz & 3u) == 2;
+	kh = pos ? v.x >> kGTileBits : v.x >> bits;
+	rid = pos ? (pz & ~(kGTile - 1)) + (v.x & (kGTile - 1)) + 1 : v.y;
+}
 
 // Persistent join of a gathered batch.  Work item = (pair (join, digit),
 // probe-tile index): S tiles [tix * cpt, (tix + 1) * cpt) of that join.
@@ -3191,10 +3208,14 @@ __device__ __forceinline__ void hj_gdecode(const uint4 &v, uint32_t pz, uint32_t
 // LDS: table (kv | [rid] | heads u32[np2(cap)] | next u16[cap]).
 template <int NT, bool NARROW>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_join_g(
-    const hj_gjoin *__restrict__ joins, const hj_gpair *__restrict__ pairs, const uint64_t *__restrict__ tprefix,
-    uint32_t npairs, const uint16_t *__restrict__ run_t, const uint64_t *__restrict__ tdesc, uint32_t cap,
-    hj_stats *st, unsigned long long *__restrict__ res) {
+    const hj_gjoin *__restrict__ joins, const hj_gitem *__restrict__ items, const uint64_t *__restrict__ qb,
+    const uint16_t *__restrict__ run_t, const uint64_t *__restrict__ tdesc, uint32_t cap, hj_stats *st,
+    unsigned long long *__restrict__ res) {
 	if (NARROW == ((st->flags & HJ_DF_WIDE) != 0)) return;
+	// loaded bytes per tuple: 8 when no tile is 16-byte (an 8-byte load of a
+	// 4-byte word reads 4 bytes of its tile beyond it), else 16
+	using VT = typename std::conditional<NARROW, uint2, uint4>::type;
+	using KT = typename std::conditional<NARROW, uint32_t, uint64_t>::type;
 	constexpr uint32_t NW = NT / HJ_WAVE;
 	constexpr uint32_t RU = 4;               // tuples per lane per step
 	constexpr uint32_t STEP = HJ_WAVE * RU;  // tuples per wave per step
@@ -3203,14 +3224,22 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
 	__shared__ uint64_t s_item[4];
 	__shared__ uint64_t ent_a[NT + 1];  // staged run k: address of its first tuple | layout code
 	__shared__ uint2 ent_o[NT + 1];     // {flat offset, segment position of its tile}; ent_o[NT].x = total
+	// the probe side is staged by each wave alone: 64 tiles' runs at a time
+	__shared__ uint64_t went_a[NW][HJ_WAVE];
+	__shared__ uint2 went_o[NW][HJ_WAVE + 1];
 	const uint32_t nb = hj_np2u(cap), hmask = nb - 1;
 	uint64_t *kv = lds_u64;
 	uint64_t *rv = kv + cap;  // wide table: row ids
 	uint32_t *heads = (uint32_t *)(kv + (NARROW ? cap : 2 * cap));
 	uint16_t *next = (uint16_t *)(heads + nb);
 	const uint32_t lane = threadIdx.x & (HJ_WAVE - 1), wid = threadIdx.x / HJ_WAVE;
-	const uint64_t total = tprefix[npairs];
+	const uint64_t total = qb[8];
 	uint32_t cur = blockIdx.x & 7;  // this workgroup's queue (its XCD)
+	// thread 0 claims the next item while the current one runs: kpend is the
+	// position taken in queue `cur` (its value is waited for only when the
+	// next item starts)
+	uint64_t kpend = 0;
+	if (threadIdx.x == 0) kpend = atomicAdd(&st->xq[cur], 1ULL);
 
 	// runs of digit d of tiles [t0, t1) (t1 - t0 <= NT) of a segment (first
 	// global tile cb, nc tiles, run rows at rt); the first run starts `skip`
@@ -3251,28 +3280,34 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
 		const uint64_t a = ent_a[lo];
 		return cursor_t{lo, ent_o[lo].x, ent_o[lo + 1].x, ent_o[lo].y | ((uint32_t)a & 3u), a};
 	};
+	// cursor at flat index 0 of entry arrays EA / EO
+	auto first = [&](const uint64_t *EA, const uint2 *EO) -> cursor_t {
+		const uint64_t a = EA[0];
+		return cursor_t{0, EO[0].x, EO[1].x, EO[0].y | ((uint32_t)a & 3u), a};
+	};
 	// flat indices f0 + u*64 + lane, clamped to fe - 1 (every lane issues RU
 	// loads, none under a lane condition); the cursor only moves forward
-	auto load_step = [&](uint32_t f0, uint32_t fe, cursor_t &cs, uint4 (&v)[RU], uint32_t (&pz)[RU]) {
+	auto load_step = [&](uint32_t f0, uint32_t fe, cursor_t &cs, VT (&v)[RU], uint32_t (&pz)[RU], const uint64_t *EA,
+	                     const uint2 *EO) {
 #pragma unroll
 		for (uint32_t u = 0; u < RU; u++) {
 			const uint32_t f = min(f0 + u * HJ_WAVE + lane, fe - 1);
 			while (f >= cs.noff) {
 				cs.k++;
 				cs.off = cs.noff;
-				cs.noff = ent_o[cs.k + 1].x;
-				cs.a = ent_a[cs.k];
-				cs.pz = ent_o[cs.k].y | ((uint32_t)cs.a & 3u);
+				cs.noff = EO[cs.k + 1].x;
+				cs.a = EA[cs.k];
+				cs.pz = EO[cs.k].y | ((uint32_t)cs.a & 3u);
 			}
 			const uint32_t code = (uint32_t)cs.a & 3u;
-			v[u] = hj_load16a4((const char *)(uintptr_t)(cs.a & ~3ull) + (uint64_t)(f - cs.off) * (16u >> code));
+			hj_gload(v[u], (const char *)(uintptr_t)(cs.a & ~3ull) + (uint64_t)(f - cs.off) * (16u >> code));
 			pz[u] = cs.pz;
 		}
 	};
 	unsigned long long cnt = 0, sum = 0;
-	auto probe_step = [&](uint32_t f0, uint32_t fe, const uint4 (&v)[RU], const uint32_t (&pz)[RU], uint32_t bits) {
+	auto probe_step = [&](uint32_t f0, uint32_t fe, const VT (&v)[RU], const uint32_t (&pz)[RU], uint32_t bits) {
 		uint32_t c[RU];
-		uint64_t kh[RU], rid[RU];
+		KT kh[RU], rid[RU];
 #pragma unroll
 		for (uint32_t u = 0; u < RU; u++) {
 			hj_gdecode(v[u], pz[u], bits, kh[u], rid[u]);
@@ -3288,7 +3323,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
 					const uint64_t e = kv[c[u] - 1];
 					if ((uint32_t)e == (uint32_t)kh[u]) {
 						cnt++;
-						sum += (e >> 32) + rid[u];
+						sum += (e >> 32) + (uint64_t)rid[u];
 					}
 				} else if (kv[c[u] - 1] == kh[u]) {
 					cnt++;
@@ -3303,35 +3338,26 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
 	for (;;) {
 		if (threadIdx.x == 0) {
 			uint64_t w = total;
-			uint32_t lo = 0;
-			for (uint32_t tried = 0; tried < 8; tried++, cur = (cur + 1) & 7) {
-				const uint32_t i0 = (uint32_t)((uint64_t)npairs * cur / 8), i1 = (uint32_t)((uint64_t)npairs * (cur + 1) / 8);
-				const uint64_t w0 = tprefix[i0], w1 = tprefix[i1];
-				if (w0 >= w1) continue;
-				const uint64_t k = atomicAdd(&st->xq[cur], 1ULL);
-				if (w0 + k >= w1) continue;
-				w = w0 + k;
-				uint32_t a = i0, b = i1;
-				while (b - a > 1) {
-					const uint32_t mid = (a + b) >> 1;
-					if (tprefix[mid] <= w) a = mid; else b = mid;
+			for (uint32_t tried = 0; tried < 8;) {
+				const uint64_t w0 = qb[cur], w1 = qb[cur + 1];
+				if (w0 + kpend < w1) {
+					w = w0 + kpend;
+					break;
 				}
-				lo = a;
-				break;
+				if (++tried == 8) break;
+				cur = (cur + 1) & 7;
+				kpend = atomicAdd(&st->xq[cur], 1ULL);
 			}
 			s_item[0] = w;
-			s_item[1] = lo;
-			s_item[2] = w < total ? w - tprefix[lo] : 0;
+			if (w < total) kpend = atomicAdd(&st->xq[cur], 1ULL);  // the next item's claim
 		}
 		__syncthreads();
 		const uint64_t w = s_item[0];
-		const uint32_t q = (uint32_t)s_item[1];
-		const uint32_t tix = (uint32_t)s_item[2];
 		__syncthreads();
 		if (w >= total) break;
-		const hj_gpair pq = pairs[q];
-		const hj_gjoin J = joins[pq.join];
-		const uint32_t d = pq.d;
+		const hj_gitem it = items[w];
+		const hj_gjoin J = joins[it.join];
+		const uint32_t d = it.dt & (kGItemDigits - 1), tix = it.dt >> kGItemDigitBits;
 		const uint32_t sb = tix * J.cpt, se = min(J.ncS, sb + J.cpt);
 		uint32_t rc = 0;    // next R tile
 		uint32_t roff = 0;  // tuples of tile rc's run already taken by an earlier fill
@@ -3353,18 +3379,18 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
 				if (fb < fe) {
 					cursor_t cs = seek(fb);
 					for (uint32_t f0 = fb; f0 < fe; f0 += STEP) {
-						uint4 v[RU];
+						VT v[RU];
 						uint32_t pz[RU];
-						load_step(f0, fe, cs, v, pz);
+						load_step(f0, fe, cs, v, pz, ent_a, ent_o);
 #pragma unroll
 						for (uint32_t u = 0; u < RU; u++) {
 							const uint32_t f = f0 + u * HJ_WAVE + lane;
 							if (f >= fe) continue;
-							uint64_t kh, rid;
+							KT kh, rid;
 							hj_gdecode(v[u], pz[u], J.bits, kh, rid);
 							const uint32_t i = m + f;
 							if (NARROW) {
-								kv[i] = (uint64_t)(uint32_t)kh | rid << 32;
+								kv[i] = (uint64_t)(uint32_t)kh | (uint64_t)rid << 32;
 							} else {
 								kv[i] = kh;
 								rv[i] = rid;
@@ -3379,42 +3405,83 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
 				__syncthreads();
 				if (m >= cap || rc >= J.ncR) break;
 			}
-			// ---- probe S tiles [sb, se), NT at a time
-			for (uint32_t kb = sb; m && kb < se; kb += NT) {
-				const uint32_t T = stage(J.cS, J.ncS, J.rtS, kb, min(se, kb + NT), d, 0);
-				__syncthreads();
-				const uint32_t fb = (uint32_t)((uint64_t)T * wid / NW), fe = (uint32_t)((uint64_t)T * (wid + 1) / NW);
-				if (fb < fe) {
-					cursor_t cs = seek(fb);
-					uint4 va[RU], vb[RU];
-					uint32_t pa[RU], pb[RU];
-					load_step(fb, fe, cs, va, pa);
-					for (uint32_t f0 = fb;;) {
-						const uint32_t f1 = f0 + STEP;
-						if (f1 < fe) load_step(f1, fe, cs, vb, pb);
-						probe_step(f0, fe, va, pa, J.bits);
-						if (f1 >= fe) break;
-						const uint32_t f2 = f1 + STEP;
-						if (f2 < fe) load_step(f2, fe, cs, va, pa);
-						probe_step(f1, fe, vb, pb, J.bits);
-						if (f2 >= fe) break;
-						f0 = f2;
+			// ---- probe S tiles [sb, se): each wave stages and probes chunks of
+			// cw tiles (wave w: chunks w, w + NW, ...) with no block barrier,
+			// the next chunk's run bounds and tile descriptors loaded while the
+			// current one is probed
+			if (m) {
+				const uint32_t nt = se - sb;
+				const uint32_t cw = min((uint32_t)HJ_WAVE, max(1u, (nt + NW - 1) / NW));
+				uint64_t *EA = went_a[wid];
+				uint2 *EO = went_o[wid];
+				uint32_t ma = 0, me = 0;
+				uint64_t mdesc = 0;
+				// chunk [t0, t0 + cw) ∩ [.., se): lane l's tile, clamped (every lane loads)
+				auto meta_load = [&](uint32_t t0) {
+					const uint32_t t = min(t0 + lane, se - 1);
+					ma = run_t[J.rtS + (uint64_t)d * J.ncS + t];
+					me = run_t[J.rtS + (uint64_t)(d + 1) * J.ncS + t];
+					mdesc = tdesc[J.cS + t];
+				};
+				auto meta_commit = [&](uint32_t t0) -> uint32_t {
+					const uint32_t t = t0 + lane;
+					const uint32_t len = (lane < cw && t < se) ? me - ma : 0u;
+					uint32_t incl = len;
+#pragma unroll
+					for (int o = 1; o < HJ_WAVE; o <<= 1) {
+						const uint32_t x = __shfl_up(incl, o, HJ_WAVE);
+						if ((int)lane >= o) incl += x;
 					}
+					const uint32_t tot = __shfl(incl, HJ_WAVE - 1, HJ_WAVE);
+					const uint32_t code = (uint32_t)mdesc & 3u;
+					EA[lane] = ((mdesc & ~3ull) + (uint64_t)ma * (16u >> code)) | code;
+					EO[lane] = make_uint2(incl - len, t * kGTile);
+					if (lane == HJ_WAVE - 1) EO[HJ_WAVE] = make_uint2(tot, 0u);
+					__builtin_amdgcn_wave_barrier();
+					return tot;
+				};
+				uint32_t t0 = sb + wid * cw;
+				if (t0 < se) meta_load(t0);
+				while (t0 < se) {
+					const uint32_t T = meta_commit(t0);
+					const uint32_t tn = t0 + NW * cw;
+					if (tn < se) meta_load(tn);
+					if (T) {
+						cursor_t cs = first(EA, EO);
+						VT va[RU], vb[RU];
+						uint32_t pa[RU], pb[RU];
+						load_step(0, T, cs, va, pa, EA, EO);
+						for (uint32_t f0 = 0;;) {
+							const uint32_t f1 = f0 + STEP;
+							if (f1 < T) load_step(f1, T, cs, vb, pb, EA, EO);
+							probe_step(f0, T, va, pa, J.bits);
+							if (f1 >= T) break;
+							const uint32_t f2 = f1 + STEP;
+							if (f2 < T) load_step(f2, T, cs, va, pa, EA, EO);
+							probe_step(f1, T, vb, pb, J.bits);
+							if (f2 >= T) break;
+							f0 = f2;
+						}
+					}
+					__builtin_amdgcn_wave_barrier();
+					t0 = tn;
 				}
-				__syncthreads();
 			}
+			__syncthreads();  // every wave is done with this fill's table
 		}
 		// this item's matches go to its join's slot (one atomic pair per wave)
 		cnt = hj_wave_sum(cnt);
 		sum = hj_wave_sum(sum);
 		if (lane == 0 && cnt) {
-			atomicAdd(&res[2 * pq.join], cnt);
-			atomicAdd(&res[2 * pq.join + 1], sum);
+			atomicAdd(&res[2 * it.join], cnt);
+			atomicAdd(&res[2 * it.join + 1], sum);
 		}
 		cnt = sum = 0;
 		__syncthreads();
 	}
 }
+
+static constexpr int kGJoinNT = 1024;  // one join workgroup per CU (its table fills the LDS)
 
 // LDS bytes of k_join_g's table for `cap` build tuples
 static size_t join_g_lds(uint32_t cap, bool narrow) {
@@ -3466,6 +3533,11 @@ struct hj_batch {
 	// chose; scattered ones from these tuple counts and the narrow flag
 	bool gathered = false;
 	uint64_t part_tuples = 0, unpart_tuples = 0;
+	// the batch's launch sequence on ctx->stream (captured into the graph;
+	// a batch created with HJ_F_KERNEL_TIMES runs it directly, each kernel
+	// between two events)
+	std::function<void()> launch;
+	bool timed = false;
 };
 
 static void batch_free(hj_batch *b) {
@@ -3494,22 +3566,26 @@ static const void *join_b_kernel(uint32_t wv, bool lp) {
 	}
 }
 
-// Radix bits of a join in a gathered batch: about 2048 build tuples per
-// partition (at most 12 bits); build sides up to 2048 tuples are one
-// partition.
+// Radix bits of a join in a gathered batch: at most 6144 build tuples per
+// partition on average (the join's LDS table holds 8192), so that a run of
+// one digit in an 8192-tuple tile is 8192 * 3072..6144 / |R| tuples long;
+// build sides up to 6144 tuples are one partition.
+static constexpr uint64_t kGPart = 6144;
 static uint32_t gbatch_bits(uint64_t nr) {
-	if (nr <= 2048) return 0;
-	return std::min<uint32_t>(12, hj_log2_ceil((nr + 2047) / 2048));
+	if (nr <= kGPart) return 0;
+	return std::min<uint32_t>(10, hj_log2_ceil((nr + kGPart - 1) / kGPart));
 }
 
-// Whether a batch runs gathered: 16-byte bc-family joins whose relations
-// each hold fewer than 2^30 tuples (positions and row ids of 4-byte words
-// fit 32 bits), unless the caller asks for the scatter path.
+// Whether a batch runs gathered: 16-byte bc-family joins whose build sides
+// hold at most 2^22 tuples (10 radix bits keep their partitions within one
+// LDS table) and whose probe sides hold fewer than 2^30 (positions and row
+// ids of 4-byte words fit 32 bits), unless the caller asks for the scatter
+// path.
 static bool gbatch_eligible(bool lp, uint32_t W, uint32_t njoins, const hj_relation_t *R, const hj_relation_t *S,
                             const hj_params_t *params) {
 	if (lp || W != 16 || (params && (params->flags & HJ_F_BATCH_SCATTER))) return false;
 	for (uint32_t j = 0; j < njoins; j++)
-		if (R[j].tuple_num >= (1ull << 30) || S[j].tuple_num >= (1ull << 30)) return false;
+		if (R[j].tuple_num > (1ull << 22) || S[j].tuple_num >= (1ull << 30)) return false;
 	return true;
 }
 
@@ -3519,8 +3595,7 @@ static int gbatch_create(hj_ctx_t *ctx, hj_batch *b) {
 	std::vector<hj_seg> segs;
 	std::vector<uint32_t> cmap;
 	std::vector<hj_gjoin> joins(njoins);
-	std::vector<hj_gpair> pairs;
-	std::vector<uint64_t> tprefix(1, 0);
+	std::vector<hj_gitem> items;
 	std::vector<hj_tblk> blocks;
 	uint64_t part_n = 0, runs_n = 0, tuples = 0, maxr = 1;
 	uint32_t maxbits = 0;
@@ -3567,44 +3642,53 @@ static int gbatch_create(hj_ctx_t *ctx, hj_batch *b) {
 	// work items: one per (join, digit) with the whole probe side; when that
 	// leaves fewer than four items per workgroup of the join grid, probe sides
 	// are cut into tile ranges of at least four times the build side's tiles
-	const unsigned approx_grid = (unsigned)ctx->num_cu * 3;
-	uint64_t items = 0;
+	const unsigned approx_grid = (unsigned)ctx->num_cu;
+	uint64_t n_whole = 0;
 	for (uint32_t j = 0; j < njoins; j++)
-		if (joins[j].ncR && joins[j].ncS) items += 1ull << joins[j].bits;
-	if (items && items < 4ull * approx_grid) {
-		const uint64_t k = (4ull * approx_grid + items - 1) / items;
+		if (joins[j].ncR && joins[j].ncS) n_whole += 1ull << joins[j].bits;
+	if (n_whole && n_whole < 4ull * approx_grid) {
+		const uint64_t k = (4ull * approx_grid + n_whole - 1) / n_whole;
 		for (uint32_t j = 0; j < njoins; j++) {
 			hj_gjoin &J = joins[j];
 			const uint32_t cpt = (uint32_t)((J.ncS + k - 1) / k);
 			J.cpt = std::max<uint32_t>(std::max<uint32_t>(1, 4 * J.ncR), cpt);
 		}
 	}
+	// items in (join, digit, probe range) order; eight queues of consecutive
+	// items, one per XCD
 	for (uint32_t j = 0; j < njoins; j++) {
 		const hj_gjoin &J = joins[j];
 		if (!J.ncR || !J.ncS) continue;
-		const uint64_t per = (J.ncS + J.cpt - 1) / J.cpt;
-		for (uint32_t d = 0; d < (1u << J.bits); d++) {
-			pairs.push_back(hj_gpair{j, d});
-			tprefix.push_back(tprefix.back() + per);
-		}
+		const uint32_t per = (J.ncS + J.cpt - 1) / J.cpt;
+		for (uint32_t d = 0; d < (1u << J.bits); d++)
+			for (uint32_t t = 0; t < per; t++) items.push_back(hj_gitem{j, d | t << kGItemDigitBits});
 	}
-	const uint32_t nseg = (uint32_t)segs.size(), npairs = (uint32_t)pairs.size();
+	uint64_t qb[9];
+	for (int x = 0; x <= 8; x++) qb[x] = (uint64_t)items.size() * x / 8;
+	const uint32_t nseg = (uint32_t)segs.size();
+	const uint64_t nitems = items.size();
 	const uint64_t nchunks = cmap.size(), nblk = blocks.size();
 	b->info.njoins = njoins;
 	b->info.tuples = tuples;
-	b->info.work_items = tprefix.back();
+	b->info.work_items = nitems;
 	b->info.gathered = 1;
 	b->gathered = true;
 	b->part_tuples = tuples;
 	if (nblk >= (1ull << 31)) return hj_fail(ctx, HJ_EINVAL, "batch too large (%llu run-table blocks)", (unsigned long long)nblk);
-	// LDS table: the largest expected partition with headroom (larger ones are
-	// joined in fills), at most what lets three workgroups share a CU
-	const void *kn = (const void *)k_join_g<512, true>, *kw = (const void *)k_join_g<512, false>;
-	const size_t lds_cu = 160 * 1024;
-	uint32_t cap = (uint32_t)std::min<uint64_t>(65535, std::max<uint64_t>(1024, maxr + maxr / 4 + 64));
-	while (cap > 1024 && 3 * (join_g_lds(cap, true) + static_lds(kn) + 1024) > lds_cu) cap -= 256;
-	const size_t ldsn = join_g_lds(cap, true), ldsw = join_g_lds(cap, false);
-	if (ldsw + static_lds(kw) > ctx->prop.sharedMemPerBlock) return hj_fail(ctx, HJ_EINVAL, "batch LDS table too big");
+	// LDS tables: one workgroup of 1024 threads per CU holds the largest table
+	// that fits (8192 tuples narrow); larger partitions are joined in fills
+	const void *kn = (const void *)k_join_g<kGJoinNT, true>, *kw = (const void *)k_join_g<kGJoinNT, false>;
+	const size_t lds_wg = std::min<size_t>(ctx->prop.sharedMemPerBlock, 160 * 1024) - 1024;
+	auto fit_cap = [&](bool narrow, const void *kfn) {
+		uint32_t c = 65535;
+		while (c > 256 && join_g_lds(c, narrow) + static_lds(kfn) > lds_wg) c = c > 8192 ? hj_np2u(c) / 2 : c - 256;
+		return c;
+	};
+	const uint32_t capn = fit_cap(true, kn), capw = fit_cap(false, kw);
+	const size_t ldsn = join_g_lds(capn, true), ldsw = join_g_lds(capw, false);
+	if (ldsw + static_lds(kw) > ctx->prop.sharedMemPerBlock || ldsn + static_lds(kn) > ctx->prop.sharedMemPerBlock)
+		return hj_fail(ctx, HJ_EINVAL, "batch LDS table too big");
+	(void)maxr;
 	const size_t sldsA = (size_t)kGTile * 16 + (2 * ((size_t)1 << maxbits) + 1) * 4 + 64 * 4;
 	if (sldsA + 1024 > ctx->prop.sharedMemPerBlock) return hj_fail(ctx, HJ_EINVAL, "batch radix bits too big");
 
@@ -3622,8 +3706,8 @@ static int gbatch_create(hj_ctx_t *ctx, hj_batch *b) {
 	if ((rc = hj_reserve(ctx, b->hist, off))) return rc;
 	off = 0;
 	const size_t o_segs = take(nseg * sizeof(hj_seg)), o_cmap = take(nchunks * 4 + 4);
-	const size_t o_joins = take(njoins * sizeof(hj_gjoin) + 8), o_pairs = take((size_t)npairs * sizeof(hj_gpair) + 8);
-	const size_t o_tp = take(tprefix.size() * 8), o_blk = take(nblk * sizeof(hj_tblk) + 16);
+	const size_t o_joins = take(njoins * sizeof(hj_gjoin) + 8), o_items = take(nitems * sizeof(hj_gitem) + 8);
+	const size_t o_qb = take(9 * 8), o_blk = take(nblk * sizeof(hj_tblk) + 16);
 	if ((rc = hj_reserve(ctx, b->meta, off))) return rc;
 	if ((rc = hj_reserve(ctx, b->stats, sizeof(hj_stats)))) return rc;
 	if ((rc = hj_reserve(ctx, b->res, (size_t)njoins * 20 + 64))) return rc;
@@ -3633,8 +3717,8 @@ static int gbatch_create(hj_ctx_t *ctx, hj_batch *b) {
 	hj_seg *d_segs = (hj_seg *)(M + o_segs);
 	uint32_t *d_cmap = (uint32_t *)(M + o_cmap);
 	hj_gjoin *d_joins = (hj_gjoin *)(M + o_joins);
-	hj_gpair *d_pairs = (hj_gpair *)(M + o_pairs);
-	uint64_t *d_tp = (uint64_t *)(M + o_tp);
+	hj_gitem *d_items = (hj_gitem *)(M + o_items);
+	uint64_t *d_qb = (uint64_t *)(M + o_qb);
 	hj_tblk *d_blk = (hj_tblk *)(M + o_blk);
 	b->d_res = (unsigned long long *)b->res.p;
 	b->d_jflags = (unsigned int *)((char *)b->res.p + (size_t)njoins * 16);
@@ -3646,47 +3730,62 @@ static int gbatch_create(hj_ctx_t *ctx, hj_batch *b) {
 	};
 	if ((rc = up(d_segs, segs.data(), nseg * sizeof(hj_seg))) || (rc = up(d_cmap, cmap.data(), nchunks * 4)) ||
 	    (rc = up(d_joins, joins.data(), njoins * sizeof(hj_gjoin))) ||
-	    (rc = up(d_pairs, pairs.data(), (size_t)npairs * sizeof(hj_gpair))) ||
-	    (rc = up(d_tp, tprefix.data(), tprefix.size() * 8)) || (rc = up(d_blk, blocks.data(), nblk * sizeof(hj_tblk))))
+	    (rc = up(d_items, items.data(), nitems * sizeof(hj_gitem))) || (rc = up(d_qb, qb, sizeof(qb))) ||
+	    (rc = up(d_blk, blocks.data(), nblk * sizeof(hj_tblk))))
 		return rc;
 	b->info.scratch_bytes = b->part.cap + b->hist.cap + b->meta.cap + b->res.cap + b->stats.cap;
 
 	// kernel attributes before capture
-	auto kA = k_sortchunk<1, 1024, 4, true>;
+	auto kA = k_sortchunk<1, 1024, kGTile / 1024, true>;
 	allow_lds(ctx, (const void *)kA);
 	allow_lds(ctx, kn);
 	allow_lds(ctx, kw);
 	const unsigned gA = std::max(1u, std::min<unsigned>((unsigned)std::min<uint64_t>(nchunks, (uint64_t)ctx->num_cu * 4),
 	                                                    resident_grid(ctx, (const void *)kA, 1024, sldsA)));
-	const unsigned gn = resident_grid(ctx, kn, 512, ldsn), gw = resident_grid(ctx, kw, 512, ldsw);
+	const unsigned gn = resident_grid(ctx, kn, kGJoinNT, ldsn), gw = resident_grid(ctx, kw, kGJoinNT, ldsw);
 	const bool narrow_ok = true, dense_ok = true;  // every relation holds fewer than 2^30 tuples
 
-	const bool kt = ctx->kernel_times;
-	ctx->kernel_times = false;
+	uint4 *d_part = (uint4 *)b->part.p;
+	unsigned long long *d_res = b->d_res;
+	unsigned int *d_fl = b->d_jflags;
+	hipStream_t strm = ctx->stream;
+	b->launch = [=]() {
+		hj_kt_begin(ctx, HJ_K_FILL);
+		hipLaunchKernelGGL(k_batch_reset, dim3(std::max<uint32_t>(1, std::min<uint32_t>(1024, (njoins + 255) / 256))),
+		                   dim3(256), 0, strm, d_st, nchunks, d_res, njoins, d_fl);
+		hj_kt_end(ctx);
+		if (nchunks) {
+			hj_kt_begin(ctx, HJ_K_TILESORT);
+			hipLaunchKernelGGL(kA, dim3(gA), dim3(1024), sldsA, strm, d_segs, d_cmap, d_st, 0u, maxbits, d_part, d_run,
+			                   d_desc, narrow_ok, dense_ok, (uint8_t *)nullptr);
+			hj_kt_end(ctx);
+			hj_kt_begin(ctx, HJ_K_PLAN);
+			hipLaunchKernelGGL(k_run_transpose_b, dim3((unsigned)nblk), dim3(256), 0, strm, d_run, d_segs, d_blk, d_runt);
+			hj_kt_end(ctx);
+		}
+		if (nitems) {
+			// both table layouts; the one that does not match pass A's flag returns
+			hj_kt_begin(ctx, HJ_K_JOIN);
+			hipLaunchKernelGGL((k_join_g<kGJoinNT, true>), dim3(gn), dim3(kGJoinNT), ldsn, strm, d_joins, d_items, d_qb,
+			                   d_runt, d_desc, capn, d_st, d_res);
+			hipLaunchKernelGGL((k_join_g<kGJoinNT, false>), dim3(gw), dim3(kGJoinNT), ldsw, strm, d_joins, d_items, d_qb,
+			                   d_runt, d_desc, capw, d_st, d_res);
+			hj_kt_end(ctx);
+		}
+	};
 	if (hipEventCreate(&b->e0) != hipSuccess || hipEventCreate(&b->e1) != hipSuccess) {
 		(void)hipGetLastError();
-		ctx->kernel_times = kt;
 		return hj_fail(ctx, HJ_EHIP, "event create");
 	}
+	if (b->timed) return HJ_OK;  // runs without a graph
+	const bool kt = ctx->kernel_times;
+	ctx->kernel_times = false;
 	hipError_t e = hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal);
 	if (e != hipSuccess) {
 		ctx->kernel_times = kt;
 		return hj_fail(ctx, HJ_EHIP, "begin capture: %s", hipGetErrorString(e));
 	}
-	hipLaunchKernelGGL(k_batch_reset, dim3(std::max<uint32_t>(1, std::min<uint32_t>(1024, (njoins + 255) / 256))),
-	                   dim3(256), 0, ctx->stream, d_st, nchunks, b->d_res, njoins, b->d_jflags);
-	if (nchunks) {
-		hipLaunchKernelGGL(kA, dim3(gA), dim3(1024), sldsA, ctx->stream, d_segs, d_cmap, d_st, 0u, maxbits,
-		                   (uint4 *)b->part.p, d_run, d_desc, narrow_ok, dense_ok, (uint8_t *)nullptr);
-		hipLaunchKernelGGL(k_run_transpose_b, dim3((unsigned)nblk), dim3(256), 0, ctx->stream, d_run, d_segs, d_blk,
-		                   d_runt);
-	}
-	if (npairs) {
-		hipLaunchKernelGGL((k_join_g<512, true>), dim3(gn), dim3(512), ldsn, ctx->stream, d_joins, d_pairs, d_tp, npairs,
-		                   d_runt, d_desc, cap, d_st, b->d_res);
-		hipLaunchKernelGGL((k_join_g<512, false>), dim3(gw), dim3(512), ldsw, ctx->stream, d_joins, d_pairs, d_tp, npairs,
-		                   d_runt, d_desc, cap, d_st, b->d_res);
-	}
+	b->launch();
 	hipGraph_t g = nullptr;
 	e = hipStreamEndCapture(ctx->stream, &g);
 	ctx->kernel_times = kt;
@@ -3730,6 +3829,7 @@ extern "C" int hj_batch_create(hj_ctx_t *ctx, const char *algo, uint32_t njoins,
 	b->R.assign(R, R + njoins);
 	b->S.assign(S, S + njoins);
 	if (gbatch_eligible(lp, W, njoins, R, S, params)) {
+		b->timed = params && (params->flags & HJ_F_KERNEL_TIMES);
 		const int grc = gbatch_create(ctx, b);
 		if (grc != HJ_OK) {
 			batch_free(b);
@@ -3932,7 +4032,14 @@ extern "C" int hj_batch_run(hj_batch_t *b, hj_join_result_t *results, hj_batch_i
 	std::lock_guard<std::mutex> lk(ctx->mu);
 	HJ_HIP(ctx, hipSetDevice(ctx->device));
 	HJ_HIP(ctx, hipEventRecord(b->e0, ctx->stream));
-	HJ_HIP(ctx, hipGraphLaunch(b->exec, ctx->stream));
+	if (b->timed) {
+		ctx->kernel_times = true;
+		ctx->n_kt = 0;
+		b->launch();
+		HJ_HIP(ctx, hipGetLastError());
+	} else {
+		HJ_HIP(ctx, hipGraphLaunch(b->exec, ctx->stream));
+	}
 	HJ_HIP(ctx, hipEventRecord(b->e1, ctx->stream));
 	std::vector<unsigned char> host((size_t)b->njoins * 20 + 8);
 	if (b->njoins)
@@ -3954,6 +4061,15 @@ extern "C" int hj_batch_run(hj_batch_t *b, hj_join_result_t *results, hj_batch_i
 	float ms = 0.f;
 	HJ_HIP(ctx, hipEventElapsedTime(&ms, b->e0, b->e1));
 	b->info.device_ms = ms;
+	b->info.n_ktimes = 0;
+	if (b->timed) {
+		b->info.n_ktimes = (uint32_t)ctx->n_kt;
+		for (int i = 0; i < ctx->n_kt; i++) {
+			b->info.ktime_id[i] = ctx->kt_id[i];
+			b->info.ktime_ms[i] = hj_event_seconds(ctx, 4 + 2 * i, 4 + 2 * i + 1) * 1e3;
+		}
+		ctx->kernel_times = false;
+	}
 	b->info.fallback_ms = 0;
 	b->info.n_fallback = 0;
 	const unsigned long long *res = (const unsigned long long *)host.data();
@@ -4077,6 +4193,7 @@ extern "C" int hj_autotune(hj_ctx_t *ctx, uint32_t width, uint32_t s_ratio, uint
 			for (uint32_t v : k_vars) {
 				hj_params_t prm{};
 				prm.part_variant = v;
+				prm.flags = HJ_F_BATCH_SCATTER;  // the shapes of the scatter path
 				hj_batch_t *bt = nullptr;
 				if (hj_batch_create(ctx, "phj_radix_bc", 1, &R, &S, &prm, &bt) != HJ_OK) continue;
 				double t = 1e30;

@@ -85,7 +85,8 @@ class BatchInfo(C.Structure):
     _fields_ = [("device_ms", C.c_double), ("fallback_ms", C.c_double), ("njoins", C.c_uint32),
                 ("n_unpartitioned", C.c_uint32), ("n_partitioned", C.c_uint32), ("n_fallback", C.c_uint32),
                 ("work_items", C.c_uint64), ("tuples", C.c_uint64), ("scratch_bytes", C.c_uint64),
-                ("algo_bytes", C.c_uint64), ("gathered", C.c_uint32), ("reserved", C.c_uint32)]
+                ("algo_bytes", C.c_uint64), ("gathered", C.c_uint32), ("n_ktimes", C.c_uint32),
+                ("ktime_ms", C.c_double * 16), ("ktime_id", C.c_uint32 * 16)]
 
 
 class ServeConfig(C.Structure):
@@ -476,7 +477,12 @@ class Batch:
         info = BatchInfo()
         self.ctx._check(self.ctx.lib.hj_batch_run(self.h, res, C.byref(info)), "hj_batch_run")
         out = [(res[i].matched_cnt, res[i].check_sum) for i in range(self.n)]
-        return out, {f: getattr(info, f) for f, _ in BatchInfo._fields_}
+        d = {f: getattr(info, f) for f, _ in BatchInfo._fields_ if f not in ("ktime_ms", "ktime_id")}
+        d["kernel_ms"] = {}
+        for i in range(info.n_ktimes):
+            name = kernel_name(info.ktime_id[i])
+            d["kernel_ms"][name] = d["kernel_ms"].get(name, 0.0) + info.ktime_ms[i]
+        return out, d
 
     def close(self):
         if getattr(self, "h", None):

@@ -200,7 +200,11 @@ typedef struct {
 	                            layout it chose (4-, 8- or 16-byte), the join
 	                            reads them once                             */
 	uint32_t gathered;       /* 1: tile-sorted and gathered, 0: scattered  */
-	uint32_t reserved;
+	uint32_t n_ktimes;       /* a gathered batch created with
+	                            HJ_F_KERNEL_TIMES runs its kernels without
+	                            the graph, each between two events:        */
+	double ktime_ms[HJ_MAX_KTIMES];  /* per-kernel device ms               */
+	uint32_t ktime_id[HJ_MAX_KTIMES];/* hj_kernel_name ids                 */
 } hj_batch_info_t;
 
 typedef struct hj_batch hj_batch_t;

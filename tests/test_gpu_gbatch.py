@@ -3,7 +3,7 @@ first-pass tiles (4-, 8- or 16-byte tile layouts) and joined by gathering each
 partition's runs from those tiles.  Every join's (matched_cnt, check_sum) is
 checked against the oracle and against the same batch on the histogram +
 scatter path (HJ_F_BATCH_SCATTER): empty and one-tuple relations, build sides
-on both sides of the 2048-tuple partition target, duplicate build keys,
+on both sides of the 6144-tuple partition target, duplicate build keys,
 misses, keys and row ids above 32 bits (16-byte tiles), row ids that are not
 positions (8-byte tiles), build sides that one digit holds whole (joined in
 several LDS fills), and a single large join whose probe side is cut into
@@ -49,8 +49,8 @@ def _free(ctx, rels):
 
 
 def test_gathered_batch_pkfk_sizes_vs_oracle(ctx):
-    sizes = [(1, 16), (3, 48), (2048, 32768), (2049, 32784), (5000, 80000), (70000, 1 << 20), (300001, 4800016),
-             (1 << 20, 1 << 24)]
+    sizes = [(1, 16), (3, 48), (2048, 32768), (6144, 98304), (6145, 98320), (20000, 320000), (70000, 1 << 20),
+             (300001, 4800016), (1 << 20, 1 << 24)]
     hs = [O.generate("pkfk", nr, ns, 16, seed=400 + i) for i, (nr, ns) in enumerate(sizes)]
     want = [O.pkfk_closed_form(S, 16) for _, S in hs]
     Rs, Ss = _upload_all(ctx, hs)
@@ -145,12 +145,34 @@ def test_gathered_batch_one_digit_holds_the_build_side(ctx):
 def test_gathered_batch_single_large_join(ctx):
     """One join alone in a batch: its probe side is cut into tile ranges so
     the join grid has work; the result equals the plain join's."""
-    R, S = ctx.generate("pkfk", 1 << 22, 1 << 26, 16, seed=21)
+    R, S = ctx.generate("pkfk", 1 << 20, 1 << 26, 16, seed=21)
     b = ctx.batch("phj_radix_bc", [R], [S])
     got, info = b.run()
     b.close()
-    assert info["work_items"] > (1 << 11)
+    assert info["work_items"] > 256  # more items than the join's 2^8 partitions
     alone = ctx.join("phj_radix_bc", R, S)
     assert got[0] == (alone["matched_cnt"], alone["check_sum"])
     assert got[0][0] == 1 << 26
     _free(ctx, [R, S])
+
+
+def test_gathered_batch_kernel_times(ctx):
+    """HJ_F_KERNEL_TIMES: the batch runs without its graph, each kernel
+    between two events; results equal the graph's."""
+    Rs, Ss = [], []
+    for i, nr in enumerate((3000, 50000, 200000)):
+        R, S = ctx.generate("pkfk", nr, 16 * nr, 16, seed=60 + i)
+        Rs.append(R)
+        Ss.append(S)
+    g = ctx.batch("phj_radix_bc", Rs, Ss)
+    want, _ = g.run()
+    g.close()
+    t = ctx.batch("phj_radix_bc", Rs, Ss, flags=hjmi.HJ_F_KERNEL_TIMES)
+    got, info = t.run()
+    got2, info2 = t.run()
+    t.close()
+    assert got == want and got2 == want
+    assert info2["n_ktimes"] >= 3
+    assert {"radix_tilesort", "radix_plan", "radix_join"} <= set(info2["kernel_ms"])
+    assert all(v >= 0 for v in info2["kernel_ms"].values())
+    _free(ctx, Rs + Ss)
