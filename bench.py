@@ -449,6 +449,8 @@ def main():
                     help="C4: hj_params_t flags of the batch (8 = HJ_F_BATCH_SCATTER: histogram + scatter path)")
     ap.add_argument("--algo", default=None)
     ap.add_argument("--theta", type=float, default=None, help="C3: Zipf exponent of the probe keys (default 1.0)")
+    ap.add_argument("--part-variant", type=lambda x: int(x, 0), default=0,
+                    help="hj_params_t.part_variant of the timed joins (tuning sweeps)")
     ap.add_argument("--bits", type=int, default=0)
     ap.add_argument("--passes", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -494,12 +496,13 @@ def main():
     log(f"device {info}")
     R, S = ctx.generate(cfg["workload"], nr, ns, W, theta=cfg["theta"])
     for _ in range(args.warmup):
-        ref = ctx.join(algo, R, S, radix_bits=args.bits, passes=args.passes)
+        ref = ctx.join(algo, R, S, radix_bits=args.bits, passes=args.passes, part_variant=args.part_variant)
     ctx.sync()
     t0 = time.perf_counter()
     res = []
     for _ in range(args.steps):
-        res.append(ctx.join(algo, R, S, radix_bits=args.bits, passes=args.passes, kernel_times=True))
+        res.append(ctx.join(algo, R, S, radix_bits=args.bits, passes=args.passes, part_variant=args.part_variant,
+                            kernel_times=True))
     ctx.sync()
     elapsed = time.perf_counter() - t0
     for r in res:

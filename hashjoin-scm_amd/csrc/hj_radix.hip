@@ -2237,21 +2237,26 @@ static void launch_sortchunk_t(hj_ctx_t *ctx, unsigned grid, size_t lds, const h
 }
 
 // Pass-A tile of 16-byte tuples (threads x tuples per thread): long enough
-// that a run of one first digit in a tile is about 16 tuples, since pass B
-// gathers those runs (2048 tuples up to 7 first-pass bits, 4096 above: by
-// 1024 threads for dense tuples, by 512 threads with 8 loads each for the
-// heads of wider tuples read in place, whose loads are spread further);
-// part_variant bits 12-13 force 1: 512x4, 2: 1024x4, 3: 512x8.  Wider
-// tuples use cfg_for(wv, 0).
+// that a run of one first digit in a tile is 16-32 tuples, since pass B
+// gathers those runs (2048 tuples up to 7 first-pass bits; above, 8192 by
+// 1024 threads x 8 for dense tuples up to 11 bits and 4096 by 1024 x 4 at
+// 12, 4096 by 512 threads with 8 loads each for the heads of wider tuples
+// read in place, whose loads are spread further); part_variant bits 12-13
+// force 1: 512x4, 2: 1024x4, 3: 512x8, bit 16 1024x8.  Wider tuples use
+// cfg_for(wv, 0).
 static part_cfg cfg_passA(uint32_t wv, uint32_t variant, uint32_t b1, uint32_t in_wv) {
 	if (wv != 1) return cfg_for(wv, 0);
+	if ((variant >> 16) & 1) return part_cfg{1024, 8, 8192};  // part_variant bit 16
 	switch ((variant >> 12) & 3) {
 	case 1: return part_cfg{512, 4, 2048};
 	case 2: return part_cfg{1024, 4, 4096};
 	case 3: return part_cfg{512, 8, 4096};
 	default:
 		if (b1 < 8) return part_cfg{512, 4, 2048};
-		return in_wv > 1 ? part_cfg{512, 8, 4096} : part_cfg{1024, 4, 4096};
+		if (in_wv > 1) return part_cfg{512, 8, 4096};
+		// 8192 tuples while the tile and 2^b1 counters fit the LDS: pass B's
+		// runs get twice as long (C2 2.81 -> 2.71 ms)
+		return b1 <= 11 ? part_cfg{1024, 8, 8192} : part_cfg{1024, 4, 4096};
 	}
 }
 
@@ -2261,7 +2266,9 @@ static void launch_sortchunk(hj_ctx_t *ctx, uint32_t wv, const part_cfg &pc, uns
 	// shapes must match cfg_passA
 	switch (wv) {
 	case 1:
-		if (pc.nt == 1024)
+		if (pc.nt == 1024 && pc.items == 8)
+			launch_sortchunk_t<1, 1024, 8>(ctx, grid, lds, segs, cmap, shift, bits, out, run, cst);
+		else if (pc.nt == 1024)
 			launch_sortchunk_t<1, 1024, 4>(ctx, grid, lds, segs, cmap, shift, bits, out, run, cst);
 		else if (pc.items == 8)
 			launch_sortchunk_t<1, 512, 8>(ctx, grid, lds, segs, cmap, shift, bits, out, run, cst);
@@ -3418,7 +3425,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
 				uint64_t mdesc = 0;
 				// chunk [t0, t0 + cw) ∩ [.., se): lane l's tile, clamped (every lane loads)
 				auto meta_load = [&](uint32_t t0) {
-					const uint32_t t = min(t0 + lane, se - 1);
+					const uint32_t t = min(t0 + lane, se - 1);  // t0 may be past the end
 					ma = run_t[J.rtS + (uint64_t)d * J.ncS + t];
 					me = run_t[J.rtS + (uint64_t)(d + 1) * J.ncS + t];
 					mdesc = tdesc[J.cS + t];
@@ -3440,27 +3447,25 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
 					__builtin_amdgcn_wave_barrier();
 					return tot;
 				};
+				// loads are never under a condition (indices are clamped): a load
+				// in a conditional block makes the compiler wait at the merge for
+				// every load issued in it, which would serialise the prefetch
 				uint32_t t0 = sb + wid * cw;
-				if (t0 < se) meta_load(t0);
+				meta_load(t0);
 				while (t0 < se) {
 					const uint32_t T = meta_commit(t0);
 					const uint32_t tn = t0 + NW * cw;
-					if (tn < se) meta_load(tn);
+					meta_load(tn);
 					if (T) {
 						cursor_t cs = first(EA, EO);
 						VT va[RU], vb[RU];
 						uint32_t pa[RU], pb[RU];
 						load_step(0, T, cs, va, pa, EA, EO);
-						for (uint32_t f0 = 0;;) {
-							const uint32_t f1 = f0 + STEP;
-							if (f1 < T) load_step(f1, T, cs, vb, pb, EA, EO);
+						for (uint32_t f0 = 0; f0 < T; f0 += 2 * STEP) {
+							load_step(f0 + STEP, T, cs, vb, pb, EA, EO);
 							probe_step(f0, T, va, pa, J.bits);
-							if (f1 >= T) break;
-							const uint32_t f2 = f1 + STEP;
-							if (f2 < T) load_step(f2, T, cs, va, pa, EA, EO);
-							probe_step(f1, T, vb, pb, J.bits);
-							if (f2 >= T) break;
-							f0 = f2;
+							load_step(f0 + 2 * STEP, T, cs, va, pa, EA, EO);
+							probe_step(f0 + STEP, T, vb, pb, J.bits);  // lanes past T skip
 						}
 					}
 					__builtin_amdgcn_wave_barrier();

@@ -69,9 +69,10 @@ typedef struct {
 	                            (0 auto, 1: 2048, 2: 4096 by 1024 threads,
 	                            3: 4096 by 512 threads); bits 14-15 the
 	                            second pass's workgroups per CU (0: two,
-	                            1: all resident, 2: one); bits 18-19 = 1:
-	                            second-pass tiles of 4096 16-byte tuples
-	                            (0: 2048)                                     */
+	                            1: all resident, 2: one); bit 16: first-pass
+	                            tiles of 8192 16-byte tuples (1024 x 8);
+	                            bits 18-19 = 1: second-pass tiles of 4096
+	                            16-byte tuples (0: 2048)                      */
 	uint32_t radix_bits_pass1; /* first-pass bits of two passes
 	                            (NUM_RADIX_BITS_PASS1, phj_asymm_radix.h:10-22);
 	                            0 = tuned / half of radix_bits                */

@@ -328,7 +328,7 @@ def test_positional_first_pass_tiles(ctx, algo):
     Sbig.view(np.uint64)[2 * 300] = np.uint64((1 << 29) + 5)  # narrow, but no room beside the index
     Rwide = R.copy()
     Rwide.view(np.uint64)[2 * 9000] += np.uint64(1 << 34)  # one wide build key
-    one_tile = (ns - 4096, ns - 1)  # every S tile but the one holding the changed tuple
+    one_tile = (ns - 8192, ns - 1)  # every S tile but the one holding the changed tuple (tiles of <= 8192)
     cases = [("as generated", R, S, (ns, ns)), ("shuffled S", R, Ssh, (0, 0)), ("S row id off", R, Soff, one_tile),
              ("big probe key", R, Sbig, one_tile), ("wide build key", Rwide, S, (ns, ns))]
     for name, hR, hS, (lo, hi) in cases:
@@ -370,7 +370,7 @@ def test_positional_tiles_of_wide_tuple_heads(ctx):
 def test_tuning_variants_match_oracle(ctx, algo):
     """Every kernel shape a part_variant selects (one-pass tile shapes and
     write-combining lines; two-pass first-pass tiles, second-pass
-    workgroups per CU and tile) gives the oracle's result, on narrow tuples
+    workgroups per CU and tile, 8192-tuple first-pass tiles) gives the oracle's result, on narrow tuples
     (PK-FK data: keys below 2^16, row ids below 2^20, so every pass-A tile
     and pass B store 8 bytes) and on wide ones (one build key above 32 bits).
     Build keys are unique, so lp joins run in LDS (no exact-path re-run) and
@@ -394,7 +394,8 @@ def test_tuning_variants_match_oracle(ctx, algo):
         one = [v | (ln << 4) for v in range(6) for ln in (0, 2, 3)]
         for pv in one:
             run(radix_bits=9, passes=1, part_variant=pv)
-        two = [a << 12 for a in range(4)] + [g << 14 for g in range(3)] + [1 << 18, (1 << 18) | (1 << 14)]
+        two = [a << 12 for a in range(4)] + [g << 14 for g in range(3)] + [1 << 16, (1 << 16) | (1 << 18),
+                                                                           1 << 18, (1 << 18) | (1 << 14)]
         for pv in two:
             for bits in (11, 14):
                 run(radix_bits=bits, passes=2, part_variant=pv)
