@@ -3159,11 +3159,14 @@ __global__ __launch_bounds__(256) void k_run_transpose_b(const uint16_t *__restr
 	}
 }
 
+// A join of a gathered launch: per side its digit-major run rows (row d of
+// a side with nc tiles at rt + d * nc) and tile descriptors.  A resident
+// build relation keeps its own (serving), so these are pointers.
 struct hj_gjoin {
-	uint64_t rtR, rtS;   // digit-major run rows of R's and S's segments
-	uint32_t cR, ncR;    // R: first global tile, tiles
-	uint32_t cS, ncS;    // S: first global tile, tiles
-	uint32_t bits, cpt;  // radix bits; S tiles per work item
+	const uint16_t *rtR, *rtS;
+	const uint64_t *dR, *dS;
+	uint32_t ncR, ncS;    // tiles of R, of S
+	uint32_t bits, cpt;   // radix bits; S tiles per work item
 };
 // A work item: join, digit (12 bits) | probe-tile range index << 12.
 static constexpr uint32_t kGItemDigitBits = 12, kGItemDigits = 1u << kGItemDigitBits;
@@ -3216,8 +3219,7 @@ __device__ __forceinline__ void hj_gdecode(const uint2 &v, uint32_t pz, uint32_t
 template <int NT, bool NARROW>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_join_g(
     const hj_gjoin *__restrict__ joins, const hj_gitem *__restrict__ items, const uint64_t *__restrict__ qb,
-    const uint16_t *__restrict__ run_t, const uint64_t *__restrict__ tdesc, uint32_t cap, hj_stats *st,
-    unsigned long long *__restrict__ res) {
+    uint32_t cap, hj_stats *st, unsigned long long *__restrict__ res) {
 	if (NARROW == ((st->flags & HJ_DF_WIDE) != 0)) return;
 	// loaded bytes per tuple: 8 when no tile is 16-byte (an 8-byte load of a
 	// 4-byte word reads 4 bytes of its tile beyond it), else 16
@@ -3248,22 +3250,21 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
 	uint64_t kpend = 0;
 	if (threadIdx.x == 0) kpend = atomicAdd(&st->xq[cur], 1ULL);
 
-	// runs of digit d of tiles [t0, t1) (t1 - t0 <= NT) of a segment (first
-	// global tile cb, nc tiles, run rows at rt); the first run starts `skip`
-	// tuples in
-	auto stage = [&](uint32_t cb, uint32_t nc, uint64_t rt, uint32_t t0, uint32_t t1, uint32_t d,
+	// runs of digit d of tiles [t0, t1) (t1 - t0 <= NT) of a side (run rows
+	// at rt, nc tiles, descriptors dsc); the first run starts `skip` tuples in
+	auto stage = [&](const uint16_t *rt, const uint64_t *dsc, uint32_t nc, uint32_t t0, uint32_t t1, uint32_t d,
 	                 uint32_t skip) -> uint32_t {
 		const uint32_t t = t0 + threadIdx.x;
 		uint32_t len = 0;
 		uint64_t addr = 0;
 		if (t < t1) {
-			uint32_t a = run_t[rt + (uint64_t)d * nc + t];
-			len = run_t[rt + (uint64_t)(d + 1) * nc + t] - a;
+			uint32_t a = hj_g(rt)[(uint64_t)d * nc + t];
+			len = hj_g(rt)[(uint64_t)(d + 1) * nc + t] - a;
 			if (threadIdx.x == 0) {
 				a += skip;
 				len -= skip;
 			}
-			const uint64_t desc = tdesc[cb + t];
+			const uint64_t desc = hj_g(dsc)[t];
 			const uint32_t code = (uint32_t)desc & 3u;
 			addr = ((desc & ~3ull) + (uint64_t)a * (16u >> code)) | code;
 		}
@@ -3376,7 +3377,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
 			for (;;) {
 				const uint32_t c1 = min(J.ncR, rc + NT);
 				const uint32_t n = c1 - rc;
-				const uint32_t tot = stage(J.cR, J.ncR, J.rtR, rc, c1, d, roff);
+				const uint32_t tot = stage(J.rtR, J.dR, J.ncR, rc, c1, d, roff);
 				__syncthreads();
 				const uint32_t T = min(tot, cap - m);
 				// runs taken whole form a prefix; the next one may be taken in part
@@ -3426,9 +3427,9 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
 				// chunk [t0, t0 + cw) ∩ [.., se): lane l's tile, clamped (every lane loads)
 				auto meta_load = [&](uint32_t t0) {
 					const uint32_t t = min(t0 + lane, se - 1);  // t0 may be past the end
-					ma = run_t[J.rtS + (uint64_t)d * J.ncS + t];
-					me = run_t[J.rtS + (uint64_t)(d + 1) * J.ncS + t];
-					mdesc = tdesc[J.cS + t];
+					ma = hj_g(J.rtS)[(uint64_t)d * J.ncS + t];
+					me = hj_g(J.rtS)[(uint64_t)(d + 1) * J.ncS + t];
+					mdesc = hj_g(J.dS)[t];
 				};
 				auto meta_commit = [&](uint32_t t0) -> uint32_t {
 					const uint32_t t = t0 + lane;
@@ -3594,190 +3595,275 @@ static bool gbatch_eligible(bool lp, uint32_t W, uint32_t njoins, const hj_relat
 	return true;
 }
 
-// Plan and capture a gathered batch into b (whose R, S, njoins are set).
-static int gbatch_create(hj_ctx_t *ctx, hj_batch *b) {
-	const uint32_t njoins = b->njoins;
+// A build relation kept in gathered layout (serving's resident builds):
+// tiles sorted by the digit of `bits`, digit-major run rows, descriptors.
+struct hj_gres {
+	const uint16_t *rt = nullptr;
+	const uint64_t *desc = nullptr;
+	uint32_t nc = 0, bits = 0;
+};
+
+// The plan of a gathered launch: segments (pass-A input), run-table
+// transposition blocks, joins and work items, and where each lives in one
+// contiguous device region:
+//   part | run | run_t | desc | (uploaded:) segs | cmap | joins | items | qb | blocks
+struct hj_gplan {
 	std::vector<hj_seg> segs;
 	std::vector<uint32_t> cmap;
-	std::vector<hj_gjoin> joins(njoins);
-	std::vector<hj_gitem> items;
 	std::vector<hj_tblk> blocks;
-	uint64_t part_n = 0, runs_n = 0, tuples = 0, maxr = 1;
-	uint32_t maxbits = 0;
-	auto add_seg = [&](const hj_relation_t &rel, uint32_t bits) -> uint32_t {
-		hj_seg g{};
-		g.in = (const uint4 *)rel.tuples;
-		g.len = rel.tuple_num;
-		g.out_start = part_n;
-		g.bits = bits;
-		g.chunk_base = (uint32_t)cmap.size();
-		g.nchunks = (uint32_t)((g.len + kGTile - 1) / kGTile);
-		g.hist_base = runs_n;
-		const uint32_t F = 1u << bits;
-		runs_n += (uint64_t)g.nchunks * (F + 1);
-		part_n += g.len;
-		const uint32_t si = (uint32_t)segs.size();
-		for (uint32_t c = 0; c < g.nchunks; c++) cmap.push_back(si);
-		for (uint32_t c0 = 0; c0 < g.nchunks; c0 += 64)
-			for (uint32_t p0 = 0; p0 <= F; p0 += 64) blocks.push_back(hj_tblk{si, c0, p0, 0});
-		segs.push_back(g);
-		return si;
-	};
-	for (uint32_t j = 0; j < njoins; j++) {
-		const uint64_t nr = b->R[j].tuple_num, ns = b->S[j].tuple_num;
-		tuples += nr + ns;
-		const uint32_t bits = gbatch_bits(nr);
-		maxbits = std::max(maxbits, bits);
-		const hj_seg &gr = segs[add_seg(b->R[j], bits)];
-		const uint64_t rt_r = gr.hist_base, c_r = gr.chunk_base, nc_r = gr.nchunks;
-		const hj_seg &gs = segs[add_seg(b->S[j], bits)];
-		hj_gjoin &J = joins[j];
-		J.rtR = rt_r;
-		J.cR = (uint32_t)c_r;
-		J.ncR = (uint32_t)nc_r;
-		J.rtS = gs.hist_base;
-		J.cS = gs.chunk_base;
-		J.ncS = gs.nchunks;
-		J.bits = bits;
-		J.cpt = std::max<uint32_t>(1, J.ncS);
-		maxr = std::max<uint64_t>(maxr, (nr >> bits) + 1);
-		if (bits) b->info.n_partitioned++;
-		else b->info.n_unpartitioned++;
+	std::vector<hj_gjoin> joins;
+	std::vector<int32_t> jseg;  // R segment of a join (-1: resident), S segment is the next one unless resident
+	std::vector<int32_t> jsseg; // S segment
+	std::vector<hj_gitem> items;
+	uint64_t qb[9] = {};
+	uint64_t part_n = 0, runs_n = 0, tuples = 0;
+	uint32_t maxbits = 0, n_part = 0, n_unpart = 0;
+	// byte offsets
+	size_t o_part = 0, o_run = 0, o_runt = 0, o_desc = 0, o_up = 0, o_segs = 0, o_cmap = 0, o_joins = 0, o_items = 0,
+	       o_qb = 0, o_blk = 0, bytes = 0;
+};
+
+static uint32_t gplan_seg(hj_gplan &P, const hj_relation_t &rel, uint32_t bits) {
+	hj_seg g{};
+	g.in = (const uint4 *)rel.tuples;
+	g.len = rel.tuple_num;
+	g.out_start = P.part_n;
+	g.bits = bits;
+	g.chunk_base = (uint32_t)P.cmap.size();
+	g.nchunks = (uint32_t)((g.len + kGTile - 1) / kGTile);
+	g.hist_base = P.runs_n;
+	const uint32_t F = 1u << bits;
+	P.runs_n += (uint64_t)g.nchunks * (F + 1);
+	P.part_n += g.len;
+	P.maxbits = std::max(P.maxbits, bits);
+	const uint32_t si = (uint32_t)P.segs.size();
+	for (uint32_t c = 0; c < g.nchunks; c++) P.cmap.push_back(si);
+	for (uint32_t c0 = 0; c0 < g.nchunks; c0 += 64)
+		for (uint32_t p0 = 0; p0 <= F; p0 += 64) P.blocks.push_back(hj_tblk{si, c0, p0, 0});
+	P.segs.push_back(g);
+	return si;
+}
+
+// Add join (R ⋈ S); `res` non-null: R is resident in gathered layout.
+static void gplan_add(hj_gplan &P, const hj_relation_t &R, const hj_relation_t &S, const hj_gres *res) {
+	hj_gjoin J{};
+	const uint32_t bits = res ? res->bits : gbatch_bits(R.tuple_num);
+	int32_t rs = -1;
+	if (res) {
+		J.rtR = res->rt;
+		J.dR = res->desc;
+		J.ncR = res->nc;
+	} else {
+		rs = (int32_t)gplan_seg(P, R, bits);
+		J.ncR = P.segs[rs].nchunks;
 	}
-	// work items: one per (join, digit) with the whole probe side; when that
-	// leaves fewer than four items per workgroup of the join grid, probe sides
-	// are cut into tile ranges of at least four times the build side's tiles
-	const unsigned approx_grid = (unsigned)ctx->num_cu;
+	const int32_t ss = (int32_t)gplan_seg(P, S, bits);
+	J.ncS = P.segs[ss].nchunks;
+	J.bits = bits;
+	J.cpt = std::max<uint32_t>(1, J.ncS);
+	P.tuples += R.tuple_num + S.tuple_num;
+	if (bits) P.n_part++;
+	else P.n_unpart++;
+	P.joins.push_back(J);
+	P.jseg.push_back(rs);
+	P.jsseg.push_back(ss);
+}
+
+// Work items and the region layout.  Items: one per (join, digit) with the
+// whole probe side; when that leaves fewer than four items per workgroup of
+// the join grid (one per CU), probe sides are cut into tile ranges of at
+// least four times the build side's tiles.  Items in (join, digit, range)
+// order; eight queues of consecutive items, one per XCD.
+static void gplan_finish(hj_gplan &P, uint32_t grid) {
 	uint64_t n_whole = 0;
-	for (uint32_t j = 0; j < njoins; j++)
-		if (joins[j].ncR && joins[j].ncS) n_whole += 1ull << joins[j].bits;
-	if (n_whole && n_whole < 4ull * approx_grid) {
-		const uint64_t k = (4ull * approx_grid + n_whole - 1) / n_whole;
-		for (uint32_t j = 0; j < njoins; j++) {
-			hj_gjoin &J = joins[j];
+	for (const hj_gjoin &J : P.joins)
+		if (J.ncR && J.ncS) n_whole += 1ull << J.bits;
+	if (n_whole && n_whole < 4ull * grid) {
+		const uint64_t k = (4ull * grid + n_whole - 1) / n_whole;
+		for (hj_gjoin &J : P.joins) {
 			const uint32_t cpt = (uint32_t)((J.ncS + k - 1) / k);
 			J.cpt = std::max<uint32_t>(std::max<uint32_t>(1, 4 * J.ncR), cpt);
 		}
 	}
-	// items in (join, digit, probe range) order; eight queues of consecutive
-	// items, one per XCD
-	for (uint32_t j = 0; j < njoins; j++) {
-		const hj_gjoin &J = joins[j];
+	for (uint32_t j = 0; j < P.joins.size(); j++) {
+		const hj_gjoin &J = P.joins[j];
 		if (!J.ncR || !J.ncS) continue;
 		const uint32_t per = (J.ncS + J.cpt - 1) / J.cpt;
 		for (uint32_t d = 0; d < (1u << J.bits); d++)
-			for (uint32_t t = 0; t < per; t++) items.push_back(hj_gitem{j, d | t << kGItemDigitBits});
+			for (uint32_t t = 0; t < per; t++) P.items.push_back(hj_gitem{j, d | t << kGItemDigitBits});
 	}
-	uint64_t qb[9];
-	for (int x = 0; x <= 8; x++) qb[x] = (uint64_t)items.size() * x / 8;
-	const uint32_t nseg = (uint32_t)segs.size();
-	const uint64_t nitems = items.size();
-	const uint64_t nchunks = cmap.size(), nblk = blocks.size();
-	b->info.njoins = njoins;
-	b->info.tuples = tuples;
-	b->info.work_items = nitems;
-	b->info.gathered = 1;
-	b->gathered = true;
-	b->part_tuples = tuples;
-	if (nblk >= (1ull << 31)) return hj_fail(ctx, HJ_EINVAL, "batch too large (%llu run-table blocks)", (unsigned long long)nblk);
-	// LDS tables: one workgroup of 1024 threads per CU holds the largest table
-	// that fits (8192 tuples narrow); larger partitions are joined in fills
-	const void *kn = (const void *)k_join_g<kGJoinNT, true>, *kw = (const void *)k_join_g<kGJoinNT, false>;
-	const size_t lds_wg = std::min<size_t>(ctx->prop.sharedMemPerBlock, 160 * 1024) - 1024;
-	auto fit_cap = [&](bool narrow, const void *kfn) {
-		uint32_t c = 65535;
-		while (c > 256 && join_g_lds(c, narrow) + static_lds(kfn) > lds_wg) c = c > 8192 ? hj_np2u(c) / 2 : c - 256;
-		return c;
-	};
-	const uint32_t capn = fit_cap(true, kn), capw = fit_cap(false, kw);
-	const size_t ldsn = join_g_lds(capn, true), ldsw = join_g_lds(capw, false);
-	if (ldsw + static_lds(kw) > ctx->prop.sharedMemPerBlock || ldsn + static_lds(kn) > ctx->prop.sharedMemPerBlock)
-		return hj_fail(ctx, HJ_EINVAL, "batch LDS table too big");
-	(void)maxr;
-	const size_t sldsA = (size_t)kGTile * 16 + (2 * ((size_t)1 << maxbits) + 1) * 4 + 64 * 4;
-	if (sldsA + 1024 > ctx->prop.sharedMemPerBlock) return hj_fail(ctx, HJ_EINVAL, "batch radix bits too big");
-
-	// ---- device buffers: tuples | run tables | digit-major run rows | tile
-	// descriptors, and the plan
-	int rc;
-	if ((rc = hj_reserve(ctx, b->part, part_n * 16 + 64))) return rc;
+	for (int x = 0; x <= 8; x++) P.qb[x] = (uint64_t)P.items.size() * x / 8;
 	size_t off = 0;
 	auto take = [&](size_t bytes) {
 		size_t o = off;
 		off += (bytes + 255) & ~(size_t)255;
 		return o;
 	};
-	const size_t o_run = take(runs_n * 2 + 2), o_runt = take(runs_n * 2 + 2), o_desc = take(nchunks * 8 + 8);
-	if ((rc = hj_reserve(ctx, b->hist, off))) return rc;
-	off = 0;
-	const size_t o_segs = take(nseg * sizeof(hj_seg)), o_cmap = take(nchunks * 4 + 4);
-	const size_t o_joins = take(njoins * sizeof(hj_gjoin) + 8), o_items = take(nitems * sizeof(hj_gitem) + 8);
-	const size_t o_qb = take(9 * 8), o_blk = take(nblk * sizeof(hj_tblk) + 16);
-	if ((rc = hj_reserve(ctx, b->meta, off))) return rc;
-	if ((rc = hj_reserve(ctx, b->stats, sizeof(hj_stats)))) return rc;
-	if ((rc = hj_reserve(ctx, b->res, (size_t)njoins * 20 + 64))) return rc;
-	char *H = (char *)b->hist.p, *M = (char *)b->meta.p;
-	uint16_t *d_run = (uint16_t *)(H + o_run), *d_runt = (uint16_t *)(H + o_runt);
-	uint64_t *d_desc = (uint64_t *)(H + o_desc);
-	hj_seg *d_segs = (hj_seg *)(M + o_segs);
-	uint32_t *d_cmap = (uint32_t *)(M + o_cmap);
-	hj_gjoin *d_joins = (hj_gjoin *)(M + o_joins);
-	hj_gitem *d_items = (hj_gitem *)(M + o_items);
-	uint64_t *d_qb = (uint64_t *)(M + o_qb);
-	hj_tblk *d_blk = (hj_tblk *)(M + o_blk);
-	b->d_res = (unsigned long long *)b->res.p;
-	b->d_jflags = (unsigned int *)((char *)b->res.p + (size_t)njoins * 16);
-	hj_stats *d_st = (hj_stats *)b->stats.p;
-	auto up = [&](void *dst, const void *src, size_t bytes) -> int {
-		if (!bytes) return HJ_OK;
-		hipError_t e = hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice);
-		return e == hipSuccess ? HJ_OK : hj_fail(ctx, HJ_EHIP, "batch upload: %s", hipGetErrorString(e));
-	};
-	if ((rc = up(d_segs, segs.data(), nseg * sizeof(hj_seg))) || (rc = up(d_cmap, cmap.data(), nchunks * 4)) ||
-	    (rc = up(d_joins, joins.data(), njoins * sizeof(hj_gjoin))) ||
-	    (rc = up(d_items, items.data(), nitems * sizeof(hj_gitem))) || (rc = up(d_qb, qb, sizeof(qb))) ||
-	    (rc = up(d_blk, blocks.data(), nblk * sizeof(hj_tblk))))
-		return rc;
-	b->info.scratch_bytes = b->part.cap + b->hist.cap + b->meta.cap + b->res.cap + b->stats.cap;
+	P.o_part = take(P.part_n * 16 + 64);
+	P.o_run = take(P.runs_n * 2 + 2);
+	P.o_runt = take(P.runs_n * 2 + 2);
+	P.o_desc = take(P.cmap.size() * 8 + 8);
+	P.o_up = off;
+	P.o_segs = take(P.segs.size() * sizeof(hj_seg) + 8);
+	P.o_cmap = take(P.cmap.size() * 4 + 4);
+	P.o_joins = take(P.joins.size() * sizeof(hj_gjoin) + 8);
+	P.o_items = take(P.items.size() * sizeof(hj_gitem) + 8);
+	P.o_qb = take(sizeof(P.qb));
+	P.o_blk = take(P.blocks.size() * sizeof(hj_tblk) + 16);
+	P.bytes = off;
+}
 
-	// kernel attributes before capture
+// Device pointers of every planned side, given the region's base; then the
+// host image of the uploaded part [o_up, bytes) into `img`.
+static void gplan_place(hj_gplan &P, char *base, unsigned char *img) {
+	const uint16_t *runt = (const uint16_t *)(base + P.o_runt);
+	const uint64_t *desc = (const uint64_t *)(base + P.o_desc);
+	for (uint32_t j = 0; j < P.joins.size(); j++) {
+		hj_gjoin &J = P.joins[j];
+		if (P.jseg[j] >= 0) {
+			const hj_seg &g = P.segs[P.jseg[j]];
+			J.rtR = runt + g.hist_base;
+			J.dR = desc + g.chunk_base;
+		}
+		const hj_seg &g = P.segs[P.jsseg[j]];
+		J.rtS = runt + g.hist_base;
+		J.dS = desc + g.chunk_base;
+	}
+	auto put = [&](size_t o, const void *src, size_t n) {
+		if (n) memcpy(img + (o - P.o_up), src, n);
+	};
+	put(P.o_segs, P.segs.data(), P.segs.size() * sizeof(hj_seg));
+	put(P.o_cmap, P.cmap.data(), P.cmap.size() * 4);
+	put(P.o_joins, P.joins.data(), P.joins.size() * sizeof(hj_gjoin));
+	put(P.o_items, P.items.data(), P.items.size() * sizeof(hj_gitem));
+	put(P.o_qb, P.qb, sizeof(P.qb));
+	put(P.o_blk, P.blocks.data(), P.blocks.size() * sizeof(hj_tblk));
+}
+
+// Everything a gathered launch needs on the device (plain values, so a batch
+// can keep it in its launcher).
+struct hj_glaunch {
+	char *base = nullptr;
+	uint64_t nchunks = 0, nblk = 0, nitems = 0;
+	uint32_t njoins = 0, maxbits = 0, capn = 0, capw = 0;
+	unsigned gA = 1, gn = 1, gw = 1;
+	size_t sldsA = 0, ldsn = 0, ldsw = 0;
+	size_t o_part, o_run, o_runt, o_desc, o_segs, o_cmap, o_joins, o_items, o_qb, o_blk;
+};
+
+static int gplan_prepare(hj_ctx_t *ctx, const hj_gplan &P, char *base, hj_glaunch &L) {
+	const void *kn = (const void *)k_join_g<kGJoinNT, true>, *kw = (const void *)k_join_g<kGJoinNT, false>;
 	auto kA = k_sortchunk<1, 1024, kGTile / 1024, true>;
+	// LDS tables: one workgroup of 1024 threads per CU holds the largest
+	// table that fits (8192 tuples narrow); larger partitions join in fills
+	const size_t lds_wg = std::min<size_t>(ctx->prop.sharedMemPerBlock, 160 * 1024) - 1024;
+	auto fit_cap = [&](bool narrow, const void *kfn) {
+		uint32_t c = 65535;
+		while (c > 256 && join_g_lds(c, narrow) + static_lds(kfn) > lds_wg) c = c > 8192 ? hj_np2u(c) / 2 : c - 256;
+		return c;
+	};
+	L.base = base;
+	L.capn = fit_cap(true, kn);
+	L.capw = fit_cap(false, kw);
+	L.ldsn = join_g_lds(L.capn, true);
+	L.ldsw = join_g_lds(L.capw, false);
+	if (L.ldsw + static_lds(kw) > ctx->prop.sharedMemPerBlock || L.ldsn + static_lds(kn) > ctx->prop.sharedMemPerBlock)
+		return hj_fail(ctx, HJ_EINVAL, "batch LDS table too big");
+	L.sldsA = (size_t)kGTile * 16 + (2 * ((size_t)1 << P.maxbits) + 1) * 4 + 64 * 4;
+	if (L.sldsA + 1024 > ctx->prop.sharedMemPerBlock) return hj_fail(ctx, HJ_EINVAL, "batch radix bits too big");
+	if (P.blocks.size() >= (1ull << 31)) return hj_fail(ctx, HJ_EINVAL, "batch too large");
 	allow_lds(ctx, (const void *)kA);
 	allow_lds(ctx, kn);
 	allow_lds(ctx, kw);
-	const unsigned gA = std::max(1u, std::min<unsigned>((unsigned)std::min<uint64_t>(nchunks, (uint64_t)ctx->num_cu * 4),
-	                                                    resident_grid(ctx, (const void *)kA, 1024, sldsA)));
-	const unsigned gn = resident_grid(ctx, kn, kGJoinNT, ldsn), gw = resident_grid(ctx, kw, kGJoinNT, ldsw);
-	const bool narrow_ok = true, dense_ok = true;  // every relation holds fewer than 2^30 tuples
+	L.nchunks = P.cmap.size();
+	L.nblk = P.blocks.size();
+	L.nitems = P.items.size();
+	L.njoins = (uint32_t)P.joins.size();
+	L.maxbits = P.maxbits;
+	L.gA = std::max(1u, std::min<unsigned>((unsigned)std::min<uint64_t>(std::max<uint64_t>(1, L.nchunks), (uint64_t)ctx->num_cu * 4),
+	                                       resident_grid(ctx, (const void *)kA, 1024, L.sldsA)));
+	L.gn = resident_grid(ctx, kn, kGJoinNT, L.ldsn);
+	L.gw = resident_grid(ctx, kw, kGJoinNT, L.ldsw);
+	L.o_part = P.o_part;
+	L.o_run = P.o_run;
+	L.o_runt = P.o_runt;
+	L.o_desc = P.o_desc;
+	L.o_segs = P.o_segs;
+	L.o_cmap = P.o_cmap;
+	L.o_joins = P.o_joins;
+	L.o_items = P.o_items;
+	L.o_qb = P.o_qb;
+	L.o_blk = P.o_blk;
+	return HJ_OK;
+}
 
-	uint4 *d_part = (uint4 *)b->part.p;
+// The launch sequence on `strm` (per-kernel events when ctx->kernel_times).
+static void gplan_launch(hj_ctx_t *ctx, hipStream_t strm, const hj_glaunch &L, hj_stats *st, unsigned long long *res,
+                         unsigned int *fl) {
+	char *B = L.base;
+	hj_kt_begin(ctx, HJ_K_FILL);
+	hipLaunchKernelGGL(k_batch_reset, dim3(std::max<uint32_t>(1, std::min<uint32_t>(1024, (L.njoins + 255) / 256))),
+	                   dim3(256), 0, strm, st, L.nchunks, res, L.njoins, fl);
+	hj_kt_end(ctx);
+	if (L.nchunks) {
+		hj_kt_begin(ctx, HJ_K_TILESORT);
+		hipLaunchKernelGGL((k_sortchunk<1, 1024, kGTile / 1024, true>), dim3(L.gA), dim3(1024), L.sldsA, strm,
+		                   (const hj_seg *)(B + L.o_segs), (const uint32_t *)(B + L.o_cmap), st, 0u, L.maxbits,
+		                   (uint4 *)(B + L.o_part), (uint16_t *)(B + L.o_run), (uint64_t *)(B + L.o_desc), true, true,
+		                   (uint8_t *)nullptr);
+		hj_kt_end(ctx);
+		hj_kt_begin(ctx, HJ_K_PLAN);
+		hipLaunchKernelGGL(k_run_transpose_b, dim3((unsigned)L.nblk), dim3(256), 0, strm, (const uint16_t *)(B + L.o_run),
+		                   (const hj_seg *)(B + L.o_segs), (const hj_tblk *)(B + L.o_blk), (uint16_t *)(B + L.o_runt));
+		hj_kt_end(ctx);
+	}
+	if (L.nitems) {
+		// both table layouts; the one that does not match pass A's flag returns
+		const hj_gjoin *J = (const hj_gjoin *)(B + L.o_joins);
+		const hj_gitem *I = (const hj_gitem *)(B + L.o_items);
+		const uint64_t *Q = (const uint64_t *)(B + L.o_qb);
+		hj_kt_begin(ctx, HJ_K_JOIN);
+		hipLaunchKernelGGL((k_join_g<kGJoinNT, true>), dim3(L.gn), dim3(kGJoinNT), L.ldsn, strm, J, I, Q, L.capn, st, res);
+		hipLaunchKernelGGL((k_join_g<kGJoinNT, false>), dim3(L.gw), dim3(kGJoinNT), L.ldsw, strm, J, I, Q, L.capw, st,
+		                   res);
+		hj_kt_end(ctx);
+	}
+}
+
+// Plan and capture a gathered batch into b (whose R, S, njoins are set).
+// Every relation holds fewer than 2^30 tuples, so pass A may store 4- and
+// 8-byte tiles everywhere.
+static int gbatch_create(hj_ctx_t *ctx, hj_batch *b) {
+	const uint32_t njoins = b->njoins;
+	hj_gplan P;
+	for (uint32_t j = 0; j < njoins; j++) gplan_add(P, b->R[j], b->S[j], nullptr);
+	gplan_finish(P, (uint32_t)ctx->num_cu);
+	b->info.njoins = njoins;
+	b->info.tuples = P.tuples;
+	b->info.work_items = P.items.size();
+	b->info.n_partitioned = P.n_part;
+	b->info.n_unpartitioned = P.n_unpart;
+	b->info.gathered = 1;
+	b->gathered = true;
+	b->part_tuples = P.tuples;
+	int rc;
+	if ((rc = hj_reserve(ctx, b->part, P.bytes))) return rc;
+	if ((rc = hj_reserve(ctx, b->stats, sizeof(hj_stats)))) return rc;
+	if ((rc = hj_reserve(ctx, b->res, (size_t)njoins * 20 + 64))) return rc;
+	std::vector<unsigned char> img(P.bytes - P.o_up);
+	gplan_place(P, (char *)b->part.p, img.data());
+	if (!img.empty() && hipMemcpy((char *)b->part.p + P.o_up, img.data(), img.size(), hipMemcpyHostToDevice) != hipSuccess)
+		return hj_fail(ctx, HJ_EHIP, "batch upload");
+	b->d_res = (unsigned long long *)b->res.p;
+	b->d_jflags = (unsigned int *)((char *)b->res.p + (size_t)njoins * 16);
+	b->info.scratch_bytes = b->part.cap + b->res.cap + b->stats.cap;
+	hj_glaunch L;
+	if ((rc = gplan_prepare(ctx, P, (char *)b->part.p, L))) return rc;
+	hj_stats *d_st = (hj_stats *)b->stats.p;
 	unsigned long long *d_res = b->d_res;
 	unsigned int *d_fl = b->d_jflags;
 	hipStream_t strm = ctx->stream;
-	b->launch = [=]() {
-		hj_kt_begin(ctx, HJ_K_FILL);
-		hipLaunchKernelGGL(k_batch_reset, dim3(std::max<uint32_t>(1, std::min<uint32_t>(1024, (njoins + 255) / 256))),
-		                   dim3(256), 0, strm, d_st, nchunks, d_res, njoins, d_fl);
-		hj_kt_end(ctx);
-		if (nchunks) {
-			hj_kt_begin(ctx, HJ_K_TILESORT);
-			hipLaunchKernelGGL(kA, dim3(gA), dim3(1024), sldsA, strm, d_segs, d_cmap, d_st, 0u, maxbits, d_part, d_run,
-			                   d_desc, narrow_ok, dense_ok, (uint8_t *)nullptr);
-			hj_kt_end(ctx);
-			hj_kt_begin(ctx, HJ_K_PLAN);
-			hipLaunchKernelGGL(k_run_transpose_b, dim3((unsigned)nblk), dim3(256), 0, strm, d_run, d_segs, d_blk, d_runt);
-			hj_kt_end(ctx);
-		}
-		if (nitems) {
-			// both table layouts; the one that does not match pass A's flag returns
-			hj_kt_begin(ctx, HJ_K_JOIN);
-			hipLaunchKernelGGL((k_join_g<kGJoinNT, true>), dim3(gn), dim3(kGJoinNT), ldsn, strm, d_joins, d_items, d_qb,
-			                   d_runt, d_desc, capn, d_st, d_res);
-			hipLaunchKernelGGL((k_join_g<kGJoinNT, false>), dim3(gw), dim3(kGJoinNT), ldsw, strm, d_joins, d_items, d_qb,
-			                   d_runt, d_desc, capw, d_st, d_res);
-			hj_kt_end(ctx);
-		}
-	};
+	b->launch = [=]() { gplan_launch(ctx, strm, L, d_st, d_res, d_fl); };
 	if (hipEventCreate(&b->e0) != hipSuccess || hipEventCreate(&b->e1) != hipSuccess) {
 		(void)hipGetLastError();
 		return hj_fail(ctx, HJ_EHIP, "event create");
