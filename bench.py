@@ -359,7 +359,16 @@ def run_served(args, hjmi):
     want = [ctx.join(algo, R, S) for R, S in pool] + [ctx.join(algo, R, S) for R, S in builds]
     want = [(w["matched_cnt"], w["check_sum"]) for w in want]
     srv = hjmi.Server(ctx, algo, hbm_cap_bytes=int(args.cap_gb * (1 << 30)), slots=args.slots,
-                      max_batch=args.max_batch, solo_lanes=args.solo_lanes, solo_min_log2=args.solo_min_log2)
+                      max_batch=args.max_batch, solo_lanes=args.solo_lanes, solo_min_log2=args.solo_min_log2,
+                      flags=args.serve_flags)
+    # 16-byte bc servers batch gathered: every |R| <= 2^20 (2^22 without
+    # lanes) in continuous batches, larger build sides in the solo lanes
+    gathered = algo.endswith("bc") and not (args.serve_flags & hjmi.HJ_SERVE_SCATTER)
+    if gathered:
+        lanes = (f"{args.solo_lanes} solo lanes taking |R| > 2^20 while idle" if args.solo_lanes else "no solo lanes") + \
+            ", gathered batches"
+    else:
+        lanes = f"{args.solo_lanes} solo lanes for |R| >= 2^{args.solo_min_log2}, scattered batches"
     bids = [srv.register_build(R) for R, _ in builds]
     rates = [int(x) for x in args.rates.split(",")]
     out_rates = []
@@ -414,8 +423,7 @@ def run_served(args, hjmi):
         "warmup": 0, "ms_per_step": 1e3 / best["served"]["joins_per_s"], "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "u64", "data": "synthetic (seeded on device)",
         "config": {"workload": f"C5: Poisson arrivals, {algo}, |R| log-uniform [2^12, 2^22], |S| = U{{4..64}}|R|, "
-                               f"30% reuse of 64 resident builds, cap {args.cap_gb} GB, {args.slots} slots, "
-                               f"{args.solo_lanes} solo lanes for |R| >= 2^{args.solo_min_log2}",
+                               f"30% reuse of 64 resident builds, cap {args.cap_gb} GB, {args.slots} slots, {lanes}",
                    "requests_per_rate": args.requests},
         "rates": out_rates, "server": st,
         "all_identical": all(r["served"]["identical_results"] for r in out_rates),
@@ -440,7 +448,9 @@ def main():
     ap.add_argument("--cap-gb", type=float, default=200.0, help="C5: admission cap (HBM GB)")
     ap.add_argument("--slots", type=int, default=3, help="C5: concurrent batches")
     ap.add_argument("--max-batch", type=int, default=64, help="C5: joins per launch")
-    ap.add_argument("--solo-lanes", type=int, default=2, help="C5: lanes running large joins alone")
+    ap.add_argument("--solo-lanes", type=int, default=0,
+                    help="C5: lanes running large joins alone (gathered servers: |R| > 2^20)")
+    ap.add_argument("--serve-flags", type=int, default=0, help="C5: hj_serve_config_t flags (1 = HJ_SERVE_SCATTER)")
     ap.add_argument("--solo-min-log2", type=int, default=16, help="C5: |R| >= 2^x runs in a solo lane")
     ap.add_argument("--joins", type=int, default=2048, help="C4: joins per batch")
     ap.add_argument("--batch-lo", type=int, default=10, help="C4: log2 of the smallest |R|")

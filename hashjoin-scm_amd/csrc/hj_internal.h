@@ -91,6 +91,13 @@ struct hj_ctx {
 	uint8_t tune_var1[40];
 	bool tuned = false;
 
+	// gathered launches: kernel attributes set, LDS table capacities and
+	// resident grids (pass A per first-pass width), queried once
+	bool g_ready = false;
+	uint32_t g_capn = 0, g_capw = 0;
+	size_t g_ldsn = 0, g_ldsw = 0;
+	unsigned g_gn = 1, g_gw = 1, g_gA[11] = {};
+
 	// timing
 	// ev[0..3] phases of a join, ev[4..] kernel pairs, then three for the
 	// outer phases of a late-materialising join and two for the head copy of

@@ -3751,7 +3751,11 @@ struct hj_glaunch {
 	size_t o_part, o_run, o_runt, o_desc, o_segs, o_cmap, o_joins, o_items, o_qb, o_blk;
 };
 
-static int gplan_prepare(hj_ctx_t *ctx, const hj_gplan &P, char *base, hj_glaunch &L) {
+// Kernel attributes and grids of gathered launches, queried once per context
+// (a server launches many small batches; each query is a runtime call).
+// Called with ctx->mu held.
+static int gplan_setup(hj_ctx_t *ctx) {
+	if (ctx->g_ready) return HJ_OK;
 	const void *kn = (const void *)k_join_g<kGJoinNT, true>, *kw = (const void *)k_join_g<kGJoinNT, false>;
 	auto kA = k_sortchunk<1, 1024, kGTile / 1024, true>;
 	// LDS tables: one workgroup of 1024 threads per CU holds the largest
@@ -3762,28 +3766,44 @@ static int gplan_prepare(hj_ctx_t *ctx, const hj_gplan &P, char *base, hj_glaunc
 		while (c > 256 && join_g_lds(c, narrow) + static_lds(kfn) > lds_wg) c = c > 8192 ? hj_np2u(c) / 2 : c - 256;
 		return c;
 	};
-	L.base = base;
-	L.capn = fit_cap(true, kn);
-	L.capw = fit_cap(false, kw);
-	L.ldsn = join_g_lds(L.capn, true);
-	L.ldsw = join_g_lds(L.capw, false);
-	if (L.ldsw + static_lds(kw) > ctx->prop.sharedMemPerBlock || L.ldsn + static_lds(kn) > ctx->prop.sharedMemPerBlock)
+	ctx->g_capn = fit_cap(true, kn);
+	ctx->g_capw = fit_cap(false, kw);
+	ctx->g_ldsn = join_g_lds(ctx->g_capn, true);
+	ctx->g_ldsw = join_g_lds(ctx->g_capw, false);
+	if (ctx->g_ldsw + static_lds(kw) > ctx->prop.sharedMemPerBlock || ctx->g_ldsn + static_lds(kn) > ctx->prop.sharedMemPerBlock)
 		return hj_fail(ctx, HJ_EINVAL, "batch LDS table too big");
-	L.sldsA = (size_t)kGTile * 16 + (2 * ((size_t)1 << P.maxbits) + 1) * 4 + 64 * 4;
-	if (L.sldsA + 1024 > ctx->prop.sharedMemPerBlock) return hj_fail(ctx, HJ_EINVAL, "batch radix bits too big");
-	if (P.blocks.size() >= (1ull << 31)) return hj_fail(ctx, HJ_EINVAL, "batch too large");
 	allow_lds(ctx, (const void *)kA);
 	allow_lds(ctx, kn);
 	allow_lds(ctx, kw);
+	for (uint32_t b = 0; b <= 10; b++)
+		ctx->g_gA[b] = resident_grid(ctx, (const void *)kA, 1024, (size_t)kGTile * 16 + (2 * ((size_t)1 << b) + 1) * 4 + 64 * 4);
+	ctx->g_gn = resident_grid(ctx, kn, kGJoinNT, ctx->g_ldsn);
+	ctx->g_gw = resident_grid(ctx, kw, kGJoinNT, ctx->g_ldsw);
+	ctx->g_ready = true;
+	return HJ_OK;
+}
+
+static int gplan_prepare(hj_ctx_t *ctx, const hj_gplan &P, char *base, hj_glaunch &L) {
+	int rc;
+	if ((rc = gplan_setup(ctx))) return rc;
+	L.base = base;
+	L.capn = ctx->g_capn;
+	L.capw = ctx->g_capw;
+	L.ldsn = ctx->g_ldsn;
+	L.ldsw = ctx->g_ldsw;
+	if (P.maxbits > 10) return hj_fail(ctx, HJ_EINVAL, "batch radix bits too big");
+	L.sldsA = (size_t)kGTile * 16 + (2 * ((size_t)1 << P.maxbits) + 1) * 4 + 64 * 4;
+	if (L.sldsA + 1024 > ctx->prop.sharedMemPerBlock) return hj_fail(ctx, HJ_EINVAL, "batch radix bits too big");
+	if (P.blocks.size() >= (1ull << 31)) return hj_fail(ctx, HJ_EINVAL, "batch too large");
 	L.nchunks = P.cmap.size();
 	L.nblk = P.blocks.size();
 	L.nitems = P.items.size();
 	L.njoins = (uint32_t)P.joins.size();
 	L.maxbits = P.maxbits;
 	L.gA = std::max(1u, std::min<unsigned>((unsigned)std::min<uint64_t>(std::max<uint64_t>(1, L.nchunks), (uint64_t)ctx->num_cu * 4),
-	                                       resident_grid(ctx, (const void *)kA, 1024, L.sldsA)));
-	L.gn = resident_grid(ctx, kn, kGJoinNT, L.ldsn);
-	L.gw = resident_grid(ctx, kw, kGJoinNT, L.ldsw);
+	                                       ctx->g_gA[P.maxbits]));
+	L.gn = ctx->g_gn;
+	L.gw = ctx->g_gw;
 	L.o_part = P.o_part;
 	L.o_run = P.o_run;
 	L.o_runt = P.o_runt;
@@ -4341,12 +4361,17 @@ struct hj_build_res {
 	void *part = nullptr;    // partitioned copy of R (bits > 0)
 	uint64_t *pb = nullptr;  // 2^bits + 1 partition bounds
 	size_t bytes = 0;
+	// gathered servers: R's tiles sorted by the digit, run rows, descriptors
+	bool gathered = false;
+	hj_gres g;
+	void *gbuf = nullptr;
 };
 
 struct hj_sreq {
 	int32_t build = -1;
 	hj_relation_t R{}, S{};
 	uint64_t footprint = 0;
+	bool gathered = false;  // runs in a gathered batch
 	hj_clock::time_point t_arr, t_done;
 	int state = 0;  // 0 queued, 1 running, 2 done
 	int err = HJ_OK; // status reported by poll for a request that could not run
@@ -4388,6 +4413,7 @@ struct hj_server {
 	std::deque<uint64_t> queue;
 	std::unordered_map<uint64_t, hj_sreq> reqs;
 	uint64_t next_id = 1;
+	size_t solo_busy = 0;  // lanes running a request
 	bool stop = false;
 	std::thread th;
 	hj_serve_stats_t st{};
@@ -4398,6 +4424,37 @@ struct hj_server {
 static const uint64_t kServeChunk = 16384;
 
 static size_t al256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+// Gathered serving: 16-byte bc servers run their batches gathered (first-
+// pass tiles, gathered joins), like gathered hj_batch_* batches.
+static bool serve_gathered(const hj_server *s) { return s->W == 16 && !s->lp && !(s->cfg.flags & HJ_SERVE_SCATTER); }
+
+// Largest build side a gathered server batches (the gathered limit).
+static uint64_t serve_gmax(const hj_server *) { return 1ull << 22; }
+// Above 2^20 build tuples a partition's run in an 8192-tuple tile gets
+// shorter than 32 tuples and a lane's two-pass engine is faster alone, so
+// such a request goes to a solo lane while one is idle (low load: the
+// latency of the two-pass engine) and into a gathered batch otherwise
+// (saturation: the batches' throughput).
+static const uint64_t kServeSoloGathered = 1ull << 20;
+
+// Whether request r can join a gathered batch: its build side is a resident
+// build kept in gathered layout, or holds at most serve_gmax tuples, and its
+// probe side fewer than 2^30.
+static bool serve_req_gatherable(const hj_server *s, const hj_sreq &r) {
+	if (!serve_gathered(s) || r.S.tuple_num >= (1ull << 30)) return false;
+	return r.build >= 0 ? s->builds[r.build].gathered : r.R.tuple_num <= serve_gmax(s);
+}
+
+// Scratch a gathered request adds to a batch: its plan alone (the batch's
+// regions are aligned once, and extra probe-range items when the batch is
+// small are within the batch overhead).
+static uint64_t serve_gfootprint(const hj_server *s, const hj_sreq &r) {
+	hj_gplan P;
+	gplan_add(P, r.R, r.S, r.build >= 0 ? &s->builds[r.build].g : nullptr);
+	gplan_finish(P, 0);
+	return P.bytes + 20;
+}
 
 // Exact scratch a request adds to a batch (its share of every arena region).
 struct serve_req_plan {
@@ -4590,6 +4647,49 @@ static int serve_launch(hj_server *s, hj_slot &sl, const std::vector<uint64_t> &
 	return HJ_OK;
 }
 
+// Plan and enqueue one gathered batch (every request gatherable) on a free
+// slot: the plan's region, the stats block and the results in the slot's
+// arena, the plan uploaded from the slot's pinned buffer.  Called with s->mu
+// held; takes ctx->mu for the launches.
+static int serve_launch_g(hj_server *s, hj_slot &sl, const std::vector<uint64_t> &ids) {
+	hj_ctx_t *ctx = s->ctx;
+	const uint32_t n = (uint32_t)ids.size();
+	hj_gplan P;
+	for (uint64_t id : ids) {
+		const hj_sreq &r = s->reqs[id];
+		gplan_add(P, r.R, r.S, r.build >= 0 ? &s->builds[r.build].g : nullptr);
+	}
+	gplan_finish(P, (uint32_t)ctx->num_cu);
+	const size_t o_st = al256(P.bytes), o_res = o_st + al256(sizeof(hj_stats)), total = o_res + al256((size_t)n * 20 + 64);
+	if (total > sl.arena_bytes) return hj_fail(ctx, HJ_ENOMEM, "batch does not fit its arena (%zu > %zu)", total, sl.arena_bytes);
+	const size_t up_bytes = P.bytes - P.o_up;
+	int rc = serve_grow_pinned(s, sl, al256(up_bytes) + (size_t)n * 20 + 4096);
+	if (rc) return rc;
+	char *A = sl.arena;
+	gplan_place(P, A, sl.hbuf);
+	sl.res_off = al256(up_bytes);
+	hj_stats *d_st = (hj_stats *)(A + o_st);
+	unsigned long long *d_res = (unsigned long long *)(A + o_res);
+	unsigned int *d_fl = (unsigned int *)(A + o_res + (size_t)n * 16);
+	std::lock_guard<std::mutex> lk(ctx->mu);
+	hj_glaunch L;
+	if ((rc = gplan_prepare(ctx, P, A, L))) return rc;
+	const bool kt = ctx->kernel_times;
+	ctx->kernel_times = false;
+	hipError_t e = hipMemcpyAsync(A + P.o_up, sl.hbuf, up_bytes, hipMemcpyHostToDevice, sl.stream);
+	if (e == hipSuccess) {
+		gplan_launch(ctx, sl.stream, L, d_st, d_res, d_fl);
+		e = hipGetLastError();
+	}
+	if (e == hipSuccess) e = hipMemcpyAsync(sl.hbuf + sl.res_off, d_res, (size_t)n * 20, hipMemcpyDeviceToHost, sl.stream);
+	if (e == hipSuccess) e = hipEventRecord(sl.done, sl.stream);
+	ctx->kernel_times = kt;
+	if (e != hipSuccess) return hj_fail(ctx, HJ_EHIP, "serve launch: %s", hipGetErrorString(e));
+	sl.reqs = ids;
+	sl.busy = true;
+	return HJ_OK;
+}
+
 static void serve_finish(hj_server *s, hj_slot &sl) {
 	const uint32_t n = (uint32_t)sl.reqs.size();
 	const unsigned long long *res = (const unsigned long long *)(sl.hbuf + sl.res_off);
@@ -4620,6 +4720,7 @@ static void serve_finish(hj_server *s, hj_slot &sl) {
 static void serve_loop(hj_server *s) {
 	(void)hipSetDevice(s->ctx->device);
 	std::unique_lock<std::mutex> lk(s->mu);
+	hj_clock::time_point last_active = hj_clock::now();
 	for (;;) {
 		bool progress = false;
 		uint32_t busy = 0;
@@ -4639,19 +4740,24 @@ static void serve_loop(hj_server *s) {
 			std::vector<uint64_t> ids;
 			uint64_t fp = kServeBatchOverhead;
 			uint64_t hist_entries = 0;
+			// a batch is all gathered or all scattered, like the queue's front
+			const bool g = s->reqs[s->queue.front()].gathered;
 			while (!s->queue.empty() && ids.size() < s->cfg.max_batch) {
 				hj_sreq &r = s->reqs[s->queue.front()];
+				if (r.gathered != g) break;
 				if (fp + r.footprint + 64 * 256 > sl.arena_bytes) break;  // FIFO: wait for memory
-				const serve_req_plan p = serve_plan_req(s, r);
-				if (!ids.empty() && hist_entries + p.hist_entries + p.npairs > (uint64_t)SCAN_CH * 60000) break;
-				hist_entries += p.hist_entries + p.npairs;
+				if (!g) {
+					const serve_req_plan p = serve_plan_req(s, r);
+					if (!ids.empty() && hist_entries + p.hist_entries + p.npairs > (uint64_t)SCAN_CH * 60000) break;
+					hist_entries += p.hist_entries + p.npairs;
+				}
 				fp += r.footprint + 64 * 256;
 				ids.push_back(s->queue.front());
 				s->queue.pop_front();
 			}
 			if (ids.empty()) continue;
 			for (uint64_t id : ids) s->reqs[id].state = 1;
-			const int rc = serve_launch(s, sl, ids);
+			const int rc = g ? serve_launch_g(s, sl, ids) : serve_launch(s, sl, ids);
 			if (rc != HJ_OK) {  // these requests fail with the launch's status
 				for (uint64_t id : ids) {
 					hj_sreq &r = s->reqs[id];
@@ -4672,12 +4778,18 @@ static void serve_loop(hj_server *s) {
 		s->st.max_busy_slots = std::max(s->st.max_busy_slots, busy);
 		if (progress) s->cv.notify_all();
 		if (s->stop && s->queue.empty() && busy == 0) break;
+		// While work is in flight, and for 2 ms after the last, the loop polls
+		// with a yield (a timed sleep of tens of microseconds lasts about 70 on
+		// Linux, and waking from the condition variable costs as much, both
+		// straight onto a request's latency); an idle server blocks.
+		const hj_clock::time_point now = hj_clock::now();
+		if (progress || busy) last_active = now;
 		if (!progress) {
-			if (busy == 0 && s->queue.empty())
+			if (busy == 0 && s->queue.empty() && now - last_active > std::chrono::milliseconds(2))
 				s->cv.wait_for(lk, std::chrono::milliseconds(5));
 			else {
 				lk.unlock();
-				std::this_thread::sleep_for(std::chrono::microseconds(20));
+				std::this_thread::yield();
 				lk.lock();
 			}
 		}
@@ -4700,6 +4812,7 @@ static void solo_loop(hj_server *s, hj_solo_lane *lane) {
 		s->solo_queue.pop_front();
 		hj_sreq &r = s->reqs[id];  // stable: unordered_map nodes do not move
 		r.state = 1;
+		s->solo_busy++;
 		const hj_relation_t R = r.R, S = r.S;
 		lk.unlock();
 		hj_params_t p{};
@@ -4711,6 +4824,7 @@ static void solo_loop(hj_server *s, hj_solo_lane *lane) {
 		r.res.check_sum = out.check_sum;
 		r.t_done = hj_clock::now();
 		r.state = 2;
+		s->solo_busy--;
 		s->st.completed++;
 		s->st.solo_completed++;
 		s->cv.notify_all();
@@ -4746,7 +4860,7 @@ extern "C" int hj_serve_open(hj_ctx_t *ctx, const char *algo, const hj_serve_con
 	bool radix = false;
 	const bool lp = hj_algo_is_lp(algo, &radix);
 	if (!radix) return hj_fail(ctx, HJ_EINVAL, "serving runs the radix joins (phj_radix_bc/lp), not '%s'", algo);
-	if (cfg->slots < 1 || cfg->slots > 8 || cfg->max_batch < 1 || cfg->solo_lanes > 4 ||
+	if (cfg->slots < 1 || cfg->slots > 8 || cfg->max_batch < 1 || cfg->solo_lanes > 4 || (cfg->flags & ~HJ_SERVE_SCATTER) ||
 	    cfg->hbm_cap_bytes < ((uint64_t)(cfg->slots + cfg->solo_lanes) << 24))
 		return hj_fail(ctx, HJ_EINVAL, "serve config: 1..8 slots, 0..4 solo lanes, max_batch >= 1, cap >= 16 MiB per lane");
 	hj_server *s = new hj_server();
@@ -4847,8 +4961,52 @@ extern "C" int hj_serve_register_build(hj_server_t *s, const hj_relation_t *R, u
 		return hj_fail(ctx, HJ_EINVAL, "tuple size %u", R->tuple_size);
 	hj_build_res b;
 	b.R = *R;
-	b.bits = R->tuple_num > 2048 ? hj_radix_bits_onepass(ctx, R->tuple_num) : 0;
-	if (b.bits) {
+	if (serve_gathered(s) && R->tuple_num <= serve_gmax(s)) {
+		// gathered layout: pass A over R alone; the tiles, digit-major run rows
+		// and descriptors stay in one device region for the server's life
+		hj_gplan P;
+		const uint32_t bits = gbatch_bits(R->tuple_num);
+		const uint32_t si = gplan_seg(P, *R, bits);
+		gplan_finish(P, (uint32_t)ctx->num_cu);
+		const size_t o_st = al256(P.bytes), total = o_st + al256(sizeof(hj_stats));
+		std::vector<unsigned char> img(P.bytes - P.o_up);
+		std::lock_guard<std::mutex> lk2(ctx->mu);
+		if (hipMalloc(&b.gbuf, total) != hipSuccess) {
+			(void)hipGetLastError();
+			b.gbuf = nullptr;
+			return hj_fail(ctx, HJ_ENOMEM, "resident build of %llu tuples", (unsigned long long)R->tuple_num);
+		}
+		char *base = (char *)b.gbuf;
+		gplan_place(P, base, img.data());
+		hj_glaunch L;
+		int rc = gplan_prepare(ctx, P, base, L);
+		if (!rc && !img.empty() && hipMemcpy(base + P.o_up, img.data(), img.size(), hipMemcpyHostToDevice) != hipSuccess)
+			rc = hj_fail(ctx, HJ_EHIP, "build plan upload");
+		if (!rc) {
+			const bool kt = ctx->kernel_times;
+			ctx->kernel_times = false;
+			gplan_launch(ctx, ctx->stream, L, (hj_stats *)(base + o_st), nullptr, nullptr);
+			ctx->kernel_times = kt;
+			const hipError_t e = hipStreamSynchronize(ctx->stream);
+			if (e != hipSuccess) rc = hj_fail(ctx, HJ_EHIP, "build partition: %s", hipGetErrorString(e));
+		}
+		if (rc) {
+			(void)hipFree(b.gbuf);
+			return rc;
+		}
+		b.gathered = true;
+		b.bits = bits;
+		b.g.rt = (const uint16_t *)(base + P.o_runt) + P.segs[si].hist_base;
+		b.g.desc = (const uint64_t *)(base + P.o_desc) + P.segs[si].chunk_base;
+		b.g.nc = P.segs[si].nchunks;
+		b.g.bits = bits;
+		b.bytes = total;
+	}
+	// a gathered server's builds above the gathered limit run in its solo
+	// lanes, on R itself
+	if (!b.gathered && !(serve_gathered(s) && !s->solo.empty()))
+		b.bits = R->tuple_num > 2048 ? hj_radix_bits_onepass(ctx, R->tuple_num) : 0;
+	if (!b.gathered && b.bits) {
 		std::lock_guard<std::mutex> lk2(ctx->mu);
 		const uint32_t wv = s->W / 16, F = 1u << b.bits;
 		const uint64_t n = R->tuple_num, nch = (n + kServeChunk - 1) / kServeChunk;
@@ -4925,8 +5083,17 @@ extern "C" int hj_serve_submit(hj_server_t *s, int32_t build_id, const hj_relati
 	if (r.R.location != HJ_LOC_DEVICE || r.S.location != HJ_LOC_DEVICE || (r.R.tuple_num && !r.R.tuples) ||
 	    (r.S.tuple_num && !r.S.tuples))
 		return hj_fail(ctx, HJ_EINVAL, "device relations required");
-	const bool solo = !s->solo.empty() && r.R.tuple_num >= s->solo_min;
-	r.footprint = solo ? solo_footprint(r, s->W) : serve_footprint(serve_plan_req(s, r), s->W);
+	// gathered servers batch every gatherable request, but send build sides
+	// above 2^20 to an idle solo lane; otherwise large build sides
+	// (>= solo_min) go to the solo lanes
+	r.gathered = serve_req_gatherable(s, r);
+	const bool lane_idle = s->solo_busy + s->solo_queue.size() < s->solo.size();
+	bool solo = !s->solo.empty() && !r.gathered && (serve_gathered(s) || r.R.tuple_num >= s->solo_min);
+	if (r.gathered && lane_idle && r.R.tuple_num > kServeSoloGathered) {
+		solo = true;
+		r.gathered = false;
+	}
+	r.footprint = solo ? solo_footprint(r, s->W) : r.gathered ? serve_gfootprint(s, r) : serve_footprint(serve_plan_req(s, r), s->W);
 	s->st.submitted++;
 	const uint64_t room = solo ? s->solo_budget : s->slots[0].arena_bytes - kServeBatchOverhead - 64 * 256;
 	if (r.footprint > room) {
@@ -4994,6 +5161,7 @@ extern "C" int hj_serve_close(hj_server_t *s) {
 	for (hj_build_res &b : s->builds) {
 		if (b.part) (void)hipFree(b.part);
 		if (b.pb) (void)hipFree(b.pb);
+		if (b.gbuf) (void)hipFree(b.gbuf);
 	}
 	delete s;
 	return HJ_OK;

@@ -27,6 +27,7 @@ HJ_F_KERNEL_TIMES = 1
 HJ_F_NO_NARROW = 2
 HJ_F_NO_POSITIONAL = 4
 HJ_F_BATCH_SCATTER = 8
+HJ_SERVE_SCATTER = 1
 HJ_FB_LP_DUPLICATES, HJ_FB_LP_OVERSIZE, HJ_FB_SC_REBUILD = 1, 2, 4
 
 WL_PKFK_UNIFORM, WL_PKFK_REF, WL_PAYLOAD_REF, WL_ZIPF, WL_RANDOM = 0, 1, 2, 3, 4
@@ -91,7 +92,8 @@ class BatchInfo(C.Structure):
 
 class ServeConfig(C.Structure):
     _fields_ = [("hbm_cap_bytes", C.c_uint64), ("slots", C.c_uint32), ("max_batch", C.c_uint32),
-                ("solo_lanes", C.c_uint32), ("solo_min_log2", C.c_uint32), ("reserved", C.c_uint32 * 2)]
+                ("solo_lanes", C.c_uint32), ("solo_min_log2", C.c_uint32), ("flags", C.c_uint32),
+                ("reserved", C.c_uint32)]
 
 
 class ServeStats(C.Structure):
@@ -500,9 +502,9 @@ class Server:
     """Continuous-batching join server (hj_serve_*)."""
 
     def __init__(self, ctx: Context, algo: str = "phj_radix_bc", hbm_cap_bytes: int = 8 << 30, slots: int = 3,
-                 max_batch: int = 64, solo_lanes: int = 0, solo_min_log2: int = 0):
+                 max_batch: int = 64, solo_lanes: int = 0, solo_min_log2: int = 0, flags: int = 0):
         self.ctx = ctx
-        cfg = ServeConfig(hbm_cap_bytes, slots, max_batch, solo_lanes, solo_min_log2)
+        cfg = ServeConfig(hbm_cap_bytes, slots, max_batch, solo_lanes, solo_min_log2, flags)
         h = C.c_void_p()
         ctx._check(ctx.lib.hj_serve_open(ctx.h, algo.encode(), C.byref(cfg), C.byref(h)), "hj_serve_open")
         self.h = h

@@ -233,22 +233,30 @@ int hj_tuned_pass1_bits(hj_ctx_t *ctx, uint64_t r_num, uint32_t *radix_bits_pass
  * whenever a slot is free, requests from the head of the queue are admitted
  * while their exact scratch footprint fits that slot's arena (at most
  * max_batch per launch), planned as one batch and launched asynchronously,
- * so batches on different slots overlap on the device.  Requests with a
- * large build side instead go to solo lanes: worker threads with their own
- * context and stream that run the full single-join engine (two radix passes)
- * one request at a time, overlapping with the batches.  Registered build
- * relations are partitioned once and stay resident in HBM; requests naming
- * them only partition their probe side.  A request whose footprint exceeds
- * one arena is rejected (HJ_EREJECT).  Results are bit-identical to hj_join
- * on the same relations.  All relations are borrowed until their request
- * (or, for builds, the server) is done. */
+ * so batches on different slots overlap on the device.  16-byte bc servers
+ * run their batches gathered (as hj_batch_* does) and batch every request
+ * whose build side holds at most 2^20 tuples (2^22 without solo lanes);
+ * other servers (lp, wider tuples, HJ_SERVE_SCATTER) partition batches with
+ * a histogram and a one-pass scatter.  Larger requests (gathered servers) or
+ * those with |R| >= 2^solo_min_log2 (scattering ones) go to solo lanes:
+ * worker threads with their own context and stream that run the full
+ * single-join engine (two radix passes) one request at a time, overlapping
+ * with the batches.  Registered build relations are partitioned once and
+ * stay resident in HBM; requests naming them only partition their probe
+ * side.  A request whose footprint exceeds one arena is rejected
+ * (HJ_EREJECT).  Results are bit-identical to hj_join on the same
+ * relations.  All relations are borrowed until their request (or, for
+ * builds, the server) is done. */
+#define HJ_SERVE_SCATTER 1u  /* flags: never gathered batches              */
 typedef struct {
 	uint64_t hbm_cap_bytes;  /* scratch cap over batch slots and solo lanes */
 	uint32_t slots;          /* concurrent batches (streams), 1..8          */
 	uint32_t max_batch;      /* joins per launch                            */
 	uint32_t solo_lanes;     /* lanes for large joins, 0..4 (0: none)       */
-	uint32_t solo_min_log2;  /* |R| >= 2^x runs in a solo lane (0: 16)      */
-	uint32_t reserved[2];
+	uint32_t solo_min_log2;  /* scattering servers: |R| >= 2^x runs in a
+	                            solo lane (0: 16)                           */
+	uint32_t flags;          /* HJ_SERVE_*                                  */
+	uint32_t reserved;
 } hj_serve_config_t;
 
 typedef struct {

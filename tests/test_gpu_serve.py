@@ -64,31 +64,40 @@ def test_serve_results_equal_alone_and_in_mix(ctx, algo):
 
 @pytest.mark.parametrize("algo", ["phj_radix_bc", "phj_radix_lp"])
 def test_serve_solo_lanes_in_mix(ctx, algo):
-    """Requests with |R| >= 2^14 run in solo lanes (full two-pass engine on
-    their own streams) while small ones are batched; every result equals the
-    join run alone."""
+    """lp servers send requests with |R| >= 2^14 to solo lanes (full
+    two-pass engine on their own streams) and batch the small ones; bc
+    servers batch every request up to |R| = 2^22 (gathered batches) and send
+    only larger build sides to the lanes.  Every result equals the join run
+    alone."""
     pool = _pool(ctx, 10, 8, 13, 21) + _pool(ctx, 6, 14, 18, 22)
     builds = _pool(ctx, 2, 12, 16, 23)
+    big = ctx.generate("pkfk", (1 << 22) + 1, 1 << 24, 16, seed=24)  # above the gathered limit
     alone = [ctx.join(algo, R, S) for R, S in pool] + [ctx.join(algo, R, S) for R, S in builds]
     want = [(a["matched_cnt"], a["check_sum"]) for a in alone]
+    a_big = ctx.join(algo, *big)
     srv = hjmi.Server(ctx, algo, hbm_cap_bytes=8 << 30, slots=2, max_batch=8, solo_lanes=2, solo_min_log2=14)
     bids = [srv.register_build(R) for R, _ in builds]
     rng = np.random.default_rng(3)
     reqs = []
-    for _ in range(60):
+    for k in range(60):
         i = int(rng.integers(0, len(pool) + len(builds)))
         if i < len(pool):
             rid = srv.submit(pool[i][1], R=pool[i][0])
         else:
             rid = srv.submit(builds[i - len(pool)][1], build=bids[i - len(pool)])
         reqs.append((rid, i))
+        if k == 30:
+            rid_big = srv.submit(big[1], R=big[0])
     srv.drain()
     for rid, i in reqs:
         assert srv.poll(rid)[0] == want[i], (rid, i)
+    assert srv.poll(rid_big)[0] == (a_big["matched_cnt"], a_big["check_sum"])
     st = srv.stats()
-    assert st["completed"] == 60 and st["solo_completed"] > 0 and st["batches"] > 0
+    assert st["completed"] == 61 and st["solo_completed"] > 0 and st["batches"] > 0
+    if algo == "phj_radix_bc":
+        assert st["solo_completed"] == 1  # only the build side above 2^22
     srv.close()
-    for R, S in pool + builds:
+    for R, S in pool + builds + [big]:
         ctx.free_relation(R)
         ctx.free_relation(S)
 
