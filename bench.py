@@ -267,7 +267,8 @@ def run_batched(args, hjmi):
         Ss.append(S)
     tuples = sum(17 * n for n in sizes)
     t0 = time.perf_counter()
-    ctx.autotune(16, 16, args.batch_lo, args.batch_hi)
+    if not args.no_autotune:  # radix bits per size class for the plain joins
+        ctx.autotune(16, 16, args.batch_lo, args.batch_hi)
     tune_s = time.perf_counter() - t0
     algo = args.algo or "phj_radix_bc"
     # plain: one synchronous hj_join per join
@@ -355,7 +356,8 @@ def run_served(args, hjmi):
     for i in range(64):
         nr, ns = rel_sizes()
         builds.append(ctx.generate("pkfk", nr, ns, 16, seed=90000 + i))
-    ctx.autotune(16, 16, 12, 22)
+    if not args.no_autotune:
+        ctx.autotune(16, 16, 12, 22)
     want = [ctx.join(algo, R, S) for R, S in pool] + [ctx.join(algo, R, S) for R, S in builds]
     want = [(w["matched_cnt"], w["check_sum"]) for w in want]
     srv = hjmi.Server(ctx, algo, hbm_cap_bytes=int(args.cap_gb * (1 << 30)), slots=args.slots,
@@ -464,6 +466,8 @@ def main():
     ap.add_argument("--bits", type=int, default=0)
     ap.add_argument("--passes", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="C4/C5: skip the size-class autotuner (plain joins then use the default radix bits)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="instead of a join, time a plain device copy of MIB mebibytes (HBM counter calibration)")
     args = ap.parse_args()
