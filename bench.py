@@ -463,6 +463,9 @@ def main():
     ap.add_argument("--theta", type=float, default=None, help="C3: Zipf exponent of the probe keys (default 1.0)")
     ap.add_argument("--part-variant", type=lambda x: int(x, 0), default=0,
                     help="hj_params_t.part_variant of the timed joins (tuning sweeps)")
+    ap.add_argument("--skew-tuned", action="store_true",
+                    help="tune the workload's size class on Zipf probe keys (hj_autotune_zipf) and join with "
+                         "HJ_F_SKEW_HANDLING")
     ap.add_argument("--bits", type=int, default=0)
     ap.add_argument("--passes", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -509,14 +512,21 @@ def main():
     info = ctx.device_info()
     log(f"device {info}")
     R, S = ctx.generate(cfg["workload"], nr, ns, W, theta=cfg["theta"])
+    flags = 0
+    if args.skew_tuned:  # tune this class on Zipf probe keys, join with HJ_F_SKEW_HANDLING
+        c = (nr - 1).bit_length()
+        ctx.autotune_zipf(W, max(1, ns // nr), c, c, theta=cfg["theta"])
+        flags = hjmi.HJ_F_SKEW_HANDLING
+        log(f"skew-tuned configuration for |R| = {nr}: {ctx.tuned_skew_config(nr)}")
     for _ in range(args.warmup):
-        ref = ctx.join(algo, R, S, radix_bits=args.bits, passes=args.passes, part_variant=args.part_variant)
+        ref = ctx.join(algo, R, S, radix_bits=args.bits, passes=args.passes, part_variant=args.part_variant,
+                       flags=flags)
     ctx.sync()
     t0 = time.perf_counter()
     res = []
     for _ in range(args.steps):
         res.append(ctx.join(algo, R, S, radix_bits=args.bits, passes=args.passes, part_variant=args.part_variant,
-                            kernel_times=True))
+                            kernel_times=True, flags=flags))
     ctx.sync()
     elapsed = time.perf_counter() - t0
     for r in res:

@@ -90,6 +90,14 @@ struct hj_ctx {
 	uint8_t tune_bits1[40];
 	uint8_t tune_var1[40];
 	bool tuned = false;
+	// the same tuned on Zipf-skewed probe sides (hj_autotune_zipf), used by
+	// joins with HJ_F_SKEW_HANDLING: bits / passes (0xff = unset), first-pass
+	// bits, and the probe-range scale (part_variant bits 20-21)
+	uint8_t tune_bits_z[40];
+	uint8_t tune_passes_z[40];
+	uint8_t tune_pass1_z[40];
+	uint8_t tune_pscale_z[40];
+	bool tuned_z = false;
 
 	// gathered launches: kernel attributes set, LDS table capacities and
 	// resident grids (pass A per first-pass width), queried once

@@ -2459,10 +2459,28 @@ uint32_t hj_radix_bits_onepass(hj_ctx_t *ctx, uint64_t nr, uint32_t *variant) {
 static int radix_engine(hj_ctx_t *ctx, const hj_relation_t *R, const hj_relation_t *S, const hj_params_t *prm,
                         int tk, hj_result_t *out, uint32_t in_wv, const hj_relation_t *Rfb, const hj_relation_t *Sfb);
 
+// The tuned configuration for |R| = nr when the caller leaves radix_bits and
+// passes at 0: the skew-tuned one (hj_autotune_zipf) under
+// HJ_F_SKEW_HANDLING when its class was tuned, else hj_radix_config's.
+// *pscale: the probe-range scale the skew tuner chose (0 otherwise).
+static void radix_choose(hj_ctx_t *ctx, uint64_t nr, const hj_params_t *prm, uint32_t *B, uint32_t *P, uint32_t *b1,
+                         uint32_t *pscale) {
+	const uint32_t c = hj_log2_ceil(nr ? nr : 1);
+	*pscale = 0;
+	if ((prm->flags & HJ_F_SKEW_HANDLING) && ctx && ctx->tuned_z && c < 40 && ctx->tune_bits_z[c] != 0xff) {
+		*B = ctx->tune_bits_z[c];
+		*P = ctx->tune_passes_z[c];
+		*b1 = ctx->tune_pass1_z[c];
+		*pscale = ctx->tune_pscale_z[c];
+		return;
+	}
+	hj_radix_config(ctx, nr, B, P, b1);
+}
+
 // passes the engine takes for |R| = nr (the first lines of radix_engine)
 static uint32_t radix_plan_passes(hj_ctx_t *ctx, uint64_t nr, const hj_params_t *prm) {
-	uint32_t B = prm->radix_bits, P = prm->passes, tb1 = 0;
-	if (B == 0 && P == 0) hj_radix_config(ctx, nr, &B, &P, &tb1);
+	uint32_t B = prm->radix_bits, P = prm->passes, tb1 = 0, ps = 0;
+	if (B == 0 && P == 0) radix_choose(ctx, nr, prm, &B, &P, &tb1, &ps);
 	if (P == 0) P = default_passes(B);
 	if (B == 0) P = 0;
 	if (P == 2 && B < 2) P = 1;
@@ -2500,9 +2518,10 @@ int hj_run_radix(hj_ctx_t *ctx, const hj_relation_t *R, const hj_relation_t *S, 
 		// the radix configuration of the whole build side, for every piece
 		hj_params_t pp = *prm;
 		if (pp.radix_bits == 0 && pp.passes == 0) {
-			uint32_t tb1 = 0;
-			hj_radix_config(ctx, R->tuple_num, &pp.radix_bits, &pp.passes, &tb1);
+			uint32_t tb1 = 0, ps = 0;
+			radix_choose(ctx, R->tuple_num, &pp, &pp.radix_bits, &pp.passes, &tb1, &ps);
 			if (!pp.radix_bits_pass1) pp.radix_bits_pass1 = tb1;
+			if (!((pp.part_variant >> 20) & 3)) pp.part_variant |= ps << 20;
 		}
 		hj_result_t acc{};
 		for (uint64_t r0 = 0; r0 < std::max<uint64_t>(R->tuple_num, 1); r0 += kPiece) {
@@ -2576,8 +2595,18 @@ static int radix_engine(hj_ctx_t *ctx, const hj_relation_t *R, const hj_relation
 	const bool lp = tk == HJ_TK_LP;
 	const uint32_t W = R->tuple_size, wv = W / 16;
 	const uint64_t nr = R->tuple_num, ns = S->tuple_num, N = nr + ns;
-	uint32_t B = prm->radix_bits, P = prm->passes, tuned_b1 = 0;
-	if (B == 0 && P == 0) hj_radix_config(ctx, nr, &B, &P, &tuned_b1);
+	uint32_t B = prm->radix_bits, P = prm->passes, tuned_b1 = 0, tuned_ps = 0;
+	if (B == 0 && P == 0) radix_choose(ctx, nr, prm, &B, &P, &tuned_b1, &tuned_ps);
+	// probe tuples per work item: part_variant bits 20-21, else the skew tuner's
+	const uint32_t pscale = ((prm->part_variant >> 20) & 3) ? ((prm->part_variant >> 20) & 3) : tuned_ps;
+	auto scale_probe = [&](uint64_t v, uint64_t lo) -> uint64_t {
+		switch (pscale) {
+		case 1: return std::max<uint64_t>(lo, v / 2);
+		case 2: return std::max<uint64_t>(lo, v / 4);
+		case 3: return v * 2;
+		default: return v;
+		}
+	};
 	if (P == 0) P = default_passes(B);
 	if (B == 0) P = 0;
 	if (P == 2 && B < 2) P = 1;
@@ -2691,7 +2720,7 @@ static int radix_engine(hj_ctx_t *ctx, const hj_relation_t *R, const hj_relation
 	uint32_t cpt = 0;
 	if (P == 2) {
 		const uint64_t avg_chunks = (ns / F1 + pc2.tile - 1) / pc2.tile;
-		cpt = (uint32_t)std::max<uint64_t>(16, avg_chunks + avg_chunks / 4 + 2);
+		cpt = (uint32_t)scale_probe(std::max<uint64_t>(16, avg_chunks + avg_chunks / 4 + 2), 4);
 	}
 	uint64_t *hist = (uint64_t *)ctx->hist.p;
 	// two passes: the count matrix holds 32-bit positions (a piece's R and S
@@ -2771,7 +2800,7 @@ static int radix_engine(hj_ctx_t *ctx, const hj_relation_t *R, const hj_relation
 
 	// join: tile the probe side so a skewed partition spreads over workgroups
 	const uint64_t avg_s = (ns >> B) + 1;
-	uint64_t s_tile = std::max<uint64_t>(8192, 2 * avg_s);
+	uint64_t s_tile = scale_probe(std::max<uint64_t>(8192, 2 * avg_s), 2048);
 	const uint64_t avg_r = (nr + Q - 1) / Q;
 	const size_t lds_max = std::min<size_t>(ctx->prop.sharedMemPerBlock, 160 * 1024) - join_static_lds(tk, P == 2) - 256;
 	uint32_t cap = (uint32_t)std::min<uint64_t>(65535, std::max<uint64_t>(256, avg_r + avg_r / 4 + 64));
@@ -4329,6 +4358,118 @@ extern "C" int hj_autotune(hj_ctx_t *ctx, uint32_t width, uint32_t s_ratio, uint
 	(void)hipFree(s);
 	if (rc == HJ_OK) ctx->tuned = true;
 	return rc;
+}
+
+// Skew-aware tuning: the same (bits, passes, first-pass bits) candidates as
+// hj_autotune, timed on probe sides whose keys are Zipf-distributed over the
+// build keys (a few heavy partitions), then the probe-range scale for the
+// best of them: a smaller scale spreads a heavy partition over more work
+// items, a larger one rebuilds each partition's table less often.  The
+// reference's counterpart is its SKEW_HANDLING build (phj.c:1374-1669:
+// re-partitioning large partitions and splitting large probe sides), a
+// compile-time choice; here joins opt in with HJ_F_SKEW_HANDLING.
+extern "C" int hj_autotune_zipf(hj_ctx_t *ctx, uint32_t width, uint32_t s_ratio, uint32_t min_log2, uint32_t max_log2,
+                                double theta) {
+	if (!ctx || s_ratio == 0 || min_log2 > max_log2 || max_log2 > 28 || !(theta >= 0.0 && theta <= 4.0))
+		return hj_fail(ctx, HJ_EINVAL, "bad tuning range");
+	if (width != 16 && width != 32 && width != 64 && width != 128 && width != 256)
+		return hj_fail(ctx, HJ_EINVAL, "tuple size %u", width);
+	std::lock_guard<std::mutex> lk(ctx->mu);
+	HJ_HIP(ctx, hipSetDevice(ctx->device));
+	const uint64_t nr_max = 1ull << max_log2, ns_max = nr_max * s_ratio;
+	void *r = nullptr, *s = nullptr;
+	if (hipMalloc(&r, nr_max * width) != hipSuccess || hipMalloc(&s, ns_max * width) != hipSuccess) {
+		(void)hipGetLastError();
+		if (r) (void)hipFree(r);
+		return hj_fail(ctx, HJ_ENOMEM, "autotune buffers");
+	}
+	int rc = HJ_OK;
+	const bool kt = ctx->kernel_times;
+	ctx->kernel_times = false;
+	auto time_of = [&](const hj_relation_t &R, const hj_relation_t &S, const hj_params_t &prm, double &t) {
+		t = 1e30;
+		for (int rep = 0; rep < 3; rep++) {
+			hj_result_t out{};
+			const int e = hj_run_radix(ctx, &R, &S, &prm, false, &out);
+			if (e) return e;
+			t = std::min(t, out.total_time);
+		}
+		return HJ_OK;
+	};
+	for (uint32_t c = min_log2; c <= max_log2 && rc == HJ_OK; c++) {
+		hj_gen_spec_t spec{};
+		spec.workload = HJ_WL_ZIPF;
+		spec.width = width;
+		spec.seed = 991 + c;
+		spec.r_num = 1ull << c;
+		spec.s_num = (1ull << c) * s_ratio;
+		spec.theta = theta;
+		if ((rc = hj_launch_generate(ctx, &spec, r, s))) break;
+		hj_relation_t R{r, spec.r_num, width, HJ_LOC_DEVICE}, S{s, spec.s_num, width, HJ_LOC_DEVICE};
+		double best = 1e30;
+		hj_params_t bestp{};
+		bool found = false;
+		for (int b = (int)c - 13; b <= (int)c - 9; b++) {
+			if (b < 0) continue;
+			for (uint32_t q = 0; q < 4; q++) {
+				const uint32_t p = q == 0 ? (b == 0 ? 0u : 1u) : 2u;
+				if (p == 2 && b < 6) break;
+				if (b > 12 && p == 1) continue;
+				const uint32_t s1 = p == 2 ? (uint32_t)(b + 1) / 2 + q - 2 : 0;
+				if (p == 2 && (s1 < 1 || s1 >= (uint32_t)b || s1 > 12 || b - s1 > 12)) continue;
+				hj_params_t prm{};
+				prm.radix_bits = (uint32_t)b;
+				prm.passes = b ? p : 0;
+				prm.radix_bits_pass1 = s1;
+				double t;
+				rc = time_of(R, S, prm, t);
+				if (rc == HJ_EINVAL) {  // not supported here: skip it
+					rc = HJ_OK;
+					continue;
+				}
+				if (rc) break;
+				if (t < best) {
+					best = t;
+					bestp = prm;
+					found = true;
+				}
+			}
+			if (rc) break;
+		}
+		if (rc || !found) continue;
+		uint32_t bests = 0;
+		for (uint32_t sc = 1; sc <= 3 && bestp.radix_bits; sc++) {
+			hj_params_t prm = bestp;
+			prm.part_variant = sc << 20;
+			double t;
+			rc = time_of(R, S, prm, t);
+			if (rc) break;
+			if (t < best) {
+				best = t;
+				bests = sc;
+			}
+		}
+		if (rc) break;
+		ctx->tune_bits_z[c] = (uint8_t)bestp.radix_bits;
+		ctx->tune_passes_z[c] = (uint8_t)bestp.passes;
+		ctx->tune_pass1_z[c] = (uint8_t)bestp.radix_bits_pass1;
+		ctx->tune_pscale_z[c] = (uint8_t)bests;
+		ctx->tuned_z = true;
+	}
+	ctx->kernel_times = kt;
+	(void)hipStreamSynchronize(ctx->stream);
+	(void)hipFree(r);
+	(void)hipFree(s);
+	return rc;
+}
+
+extern "C" int hj_tuned_skew_config(hj_ctx_t *ctx, uint64_t r_num, uint32_t *radix_bits, uint32_t *passes,
+                                    uint32_t *radix_bits_pass1, uint32_t *probe_scale) {
+	if (!ctx || !radix_bits || !passes || !radix_bits_pass1 || !probe_scale) return HJ_EINVAL;
+	hj_params_t prm{};
+	prm.flags = HJ_F_SKEW_HANDLING;
+	radix_choose(ctx, r_num, &prm, radix_bits, passes, radix_bits_pass1, probe_scale);
+	return HJ_OK;
 }
 
 extern "C" int hj_tuned_config(hj_ctx_t *ctx, uint64_t r_num, uint32_t *radix_bits, uint32_t *passes) {

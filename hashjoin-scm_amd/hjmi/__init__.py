@@ -27,6 +27,7 @@ HJ_F_KERNEL_TIMES = 1
 HJ_F_NO_NARROW = 2
 HJ_F_NO_POSITIONAL = 4
 HJ_F_BATCH_SCATTER = 8
+HJ_F_SKEW_HANDLING = 16
 HJ_SERVE_SCATTER = 1
 HJ_FB_LP_DUPLICATES, HJ_FB_LP_OVERSIZE, HJ_FB_SC_REBUILD = 1, 2, 4
 
@@ -165,6 +166,8 @@ def load_library(path: str = LIB_PATH):
         "hj_autotune": (I, [P, U32, U32, U32, U32]),
         "hj_tuned_config": (I, [P, U64, C.POINTER(U32), C.POINTER(U32)]),
         "hj_tuned_pass1_bits": (I, [P, U64, C.POINTER(U32)]),
+        "hj_autotune_zipf": (I, [P, U32, U32, U32, U32, C.c_double]),
+        "hj_tuned_skew_config": (I, [P, U64, C.POINTER(U32), C.POINTER(U32), C.POINTER(U32), C.POINTER(U32)]),
         "hj_serve_open": (I, [P, C.c_char_p, C.POINTER(ServeConfig), C.POINTER(P)]),
         "hj_serve_register_build": (I, [P, C.POINTER(Relation), C.POINTER(U32)]),
         "hj_serve_submit": (I, [P, C.c_int32, C.POINTER(Relation), C.POINTER(Relation), C.POINTER(U64)]),
@@ -192,6 +195,7 @@ def exported_symbols():
             "hj_algo_name", "hj_kernel_name", "hj_join", "hj_join_noalloc", "hj_dev_alloc", "hj_dev_free", "hj_copy_h2d",
             "hj_copy_d2h", "hj_sync", "hj_generate", "hj_device_info", "hj_copy_d2d", "hj_batch_create",
             "hj_batch_run", "hj_batch_destroy", "hj_autotune", "hj_tuned_config", "hj_tuned_pass1_bits",
+            "hj_autotune_zipf", "hj_tuned_skew_config",
             "hj_serve_open",
             "hj_serve_register_build", "hj_serve_submit", "hj_serve_poll", "hj_serve_drain", "hj_serve_stats",
             "hj_serve_close", "hj_ref_spec_for", "hj_ref_generate", "hj_ref_rand", "hj_data_cfg_read",
@@ -379,6 +383,17 @@ class Context:
         b, p = C.c_uint32(), C.c_uint32()
         self._check(self.lib.hj_tuned_config(self.h, r_num, C.byref(b), C.byref(p)), "hj_tuned_config")
         return b.value, p.value
+
+    def autotune_zipf(self, width: int = 16, s_ratio: int = 8, min_log2: int = 10, max_log2: int = 20,
+                      theta: float = 1.0):
+        self._check(self.lib.hj_autotune_zipf(self.h, width, s_ratio, min_log2, max_log2, float(theta)),
+                    "hj_autotune_zipf")
+
+    def tuned_skew_config(self, r_num: int):
+        b, p, b1, ps = C.c_uint32(), C.c_uint32(), C.c_uint32(), C.c_uint32()
+        self._check(self.lib.hj_tuned_skew_config(self.h, r_num, C.byref(b), C.byref(p), C.byref(b1), C.byref(ps)),
+                    "hj_tuned_skew_config")
+        return {"radix_bits": b.value, "passes": p.value, "radix_bits_pass1": b1.value, "probe_scale": ps.value}
 
     def tuned_pass1_bits(self, r_num: int) -> int:
         b1 = C.c_uint32()

@@ -72,7 +72,9 @@ typedef struct {
 	                            1: all resident, 2: one); bit 16: first-pass
 	                            tiles of 8192 16-byte tuples (1024 x 8);
 	                            bits 18-19 = 1: second-pass tiles of 4096
-	                            16-byte tuples (0: 2048)                      */
+	                            16-byte tuples (0: 2048); bits 20-21: probe
+	                            tuples per join work item (0: default, 1:
+	                            half, 2: a quarter, 3: double)               */
 	uint32_t radix_bits_pass1; /* first-pass bits of two passes
 	                            (NUM_RADIX_BITS_PASS1, phj_asymm_radix.h:10-22);
 	                            0 = tuned / half of radix_bits                */
@@ -88,6 +90,11 @@ typedef struct {
 #define HJ_F_BATCH_SCATTER 8u /* batches of 16-byte bc joins: partition with a
                                  histogram and a one-pass scatter instead of
                                  first-pass tiles and gathered joins          */
+#define HJ_F_SKEW_HANDLING 16u /* radix joins with radix_bits = 0: take the
+                                 configuration hj_autotune_zipf chose for the
+                                 size class (radix bits, passes, first-pass
+                                 bits, probe-range scale); the reference's
+                                 SKEW_HANDLING build flag (params.h:37-42)    */
 
 /* ---- result: matched_cnt/check_sum as printed by the reference plus the
  *      timekeeper_t phase times (src/inc/store.h:74-78) ---------------------- */
@@ -225,6 +232,15 @@ int hj_autotune(hj_ctx_t *ctx, uint32_t width, uint32_t s_ratio, uint32_t min_lo
 int hj_tuned_config(hj_ctx_t *ctx, uint64_t r_num, uint32_t *radix_bits, uint32_t *passes);
 /* the tuned first-pass width of a two-pass class (0: the default split) */
 int hj_tuned_pass1_bits(hj_ctx_t *ctx, uint64_t r_num, uint32_t *radix_bits_pass1);
+/* The autotuner on skewed data: each class's candidates (as hj_autotune)
+ * timed on probe keys Zipf-distributed over the build keys with exponent
+ * theta, then the probe-range scale (part_variant bits 20-21) for the best;
+ * joins with HJ_F_SKEW_HANDLING use these choices.  hj_tuned_skew_config
+ * reports them for |R| (probe_scale: the bits 20-21 value). */
+int hj_autotune_zipf(hj_ctx_t *ctx, uint32_t width, uint32_t s_ratio, uint32_t min_log2, uint32_t max_log2,
+                     double theta);
+int hj_tuned_skew_config(hj_ctx_t *ctx, uint64_t r_num, uint32_t *radix_bits, uint32_t *passes,
+                         uint32_t *radix_bits_pass1, uint32_t *probe_scale);
 
 /* ---- serving: continuous batching of arriving joins (no reference
  * counterpart).  A scheduler thread owns `slots` HIP streams, each with an

@@ -370,7 +370,7 @@ def test_positional_tiles_of_wide_tuple_heads(ctx):
 def test_tuning_variants_match_oracle(ctx, algo):
     """Every kernel shape a part_variant selects (one-pass tile shapes and
     write-combining lines; two-pass first-pass tiles, second-pass
-    workgroups per CU and tile, 8192-tuple first-pass tiles) gives the oracle's result, on narrow tuples
+    workgroups per CU and tile, 8192-tuple first-pass tiles, probe tuples per work item) gives the oracle's result, on narrow tuples
     (PK-FK data: keys below 2^16, row ids below 2^20, so every pass-A tile
     and pass B store 8 bytes) and on wide ones (one build key above 32 bits).
     Build keys are unique, so lp joins run in LDS (no exact-path re-run) and
@@ -391,11 +391,12 @@ def test_tuning_variants_match_oracle(ctx, algo):
                 assert got["narrow"] == (1 if tag == "narrow" else 0), (tag, kw, got)
             return got
 
-        one = [v | (ln << 4) for v in range(6) for ln in (0, 2, 3)]
+        one = [v | (ln << 4) for v in range(6) for ln in (0, 2, 3)] + [1 << 20, 2 << 20, 3 << 20]
         for pv in one:
             run(radix_bits=9, passes=1, part_variant=pv)
         two = [a << 12 for a in range(4)] + [g << 14 for g in range(3)] + [1 << 16, (1 << 16) | (1 << 18),
-                                                                           1 << 18, (1 << 18) | (1 << 14)]
+                                                                           1 << 18, (1 << 18) | (1 << 14),
+                                                                           1 << 20, 2 << 20, 3 << 20]
         for pv in two:
             for bits in (11, 14):
                 run(radix_bits=bits, passes=2, part_variant=pv)
@@ -422,3 +423,26 @@ def test_second_pass_4096_tiles_at_12_bits(ctx, narrow):
         assert got["narrow"] == int(narrow), got
     ctx.free_relation(dR)
     ctx.free_relation(dS)
+
+
+def test_autotune_zipf_and_skew_handling():
+    """hj_autotune_zipf times the radix candidates on Zipf-skewed probe sides
+    and then the probe-range scale; joins with HJ_F_SKEW_HANDLING take those
+    choices and give the oracle's result (as do joins without the flag)."""
+    c = hjmi.Context(0)
+    try:
+        c.autotune_zipf(16, 8, 12, 15, theta=1.5)
+        for k in range(12, 16):
+            t = c.tuned_skew_config(1 << k)
+            assert t["radix_bits"] <= k and t["passes"] in (0, 1, 2) and t["probe_scale"] in (0, 1, 2, 3), (k, t)
+            assert t["radix_bits_pass1"] == 0 or (t["passes"] == 2 and 1 <= t["radix_bits_pass1"] < t["radix_bits"])
+        for theta, nr in ((1.5, 1 << 15), (1.0, 1 << 14), (0.5, 40000)):
+            R, S = O.generate("zipf", nr, 8 * nr, 16, seed=57, theta=theta)
+            want = O.aggregate(R, S, 16)
+            dR, dS = _dev(c, R, 16), _dev(c, S, 16)
+            for flags in (hjmi.HJ_F_SKEW_HANDLING, 0):
+                _check(c.join("phj_radix_bc", dR, dS, flags=flags), want, (theta, nr, flags))
+            c.free_relation(dR)
+            c.free_relation(dS)
+    finally:
+        c.close()
