@@ -89,6 +89,8 @@ struct hj_ctx {
 	// tile shape / line (part_variant bits 0-6)
 	uint8_t tune_bits1[40];
 	uint8_t tune_var1[40];
+	// gathered batches: radix bits per class (0xff = the 6144-tuple rule)
+	uint8_t tune_gbits[40];
 	bool tuned = false;
 	// the same tuned on Zipf-skewed probe sides (hj_autotune_zipf), used by
 	// joins with HJ_F_SKEW_HANDLING: bits / passes (0xff = unset), first-pass

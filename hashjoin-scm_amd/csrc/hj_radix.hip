@@ -3610,6 +3610,13 @@ static uint32_t gbatch_bits(uint64_t nr) {
 	if (nr <= kGPart) return 0;
 	return std::min<uint32_t>(10, hj_log2_ceil((nr + kGPart - 1) / kGPart));
 }
+// ... or the autotuner's choice for the size class, timed through one-join
+// gathered batches
+static uint32_t gbatch_bits(const hj_ctx_t *ctx, uint64_t nr) {
+	const uint32_t c = hj_log2_ceil(nr ? nr : 1);
+	if (ctx && ctx->tuned && c < 40 && ctx->tune_gbits[c] != 0xff) return std::min<uint32_t>(10, ctx->tune_gbits[c]);
+	return gbatch_bits(nr);
+}
 
 // Whether a batch runs gathered: 16-byte bc-family joins whose build sides
 // hold at most 2^22 tuples (10 radix bits keep their partitions within one
@@ -3674,9 +3681,10 @@ static uint32_t gplan_seg(hj_gplan &P, const hj_relation_t &rel, uint32_t bits) 
 }
 
 // Add join (R ⋈ S); `res` non-null: R is resident in gathered layout.
-static void gplan_add(hj_gplan &P, const hj_relation_t &R, const hj_relation_t &S, const hj_gres *res) {
+static void gplan_add(const hj_ctx_t *ctx, hj_gplan &P, const hj_relation_t &R, const hj_relation_t &S,
+                      const hj_gres *res) {
 	hj_gjoin J{};
-	const uint32_t bits = res ? res->bits : gbatch_bits(R.tuple_num);
+	const uint32_t bits = res ? res->bits : gbatch_bits(ctx, R.tuple_num);
 	int32_t rs = -1;
 	if (res) {
 		J.rtR = res->rt;
@@ -3885,7 +3893,7 @@ static void gplan_launch(hj_ctx_t *ctx, hipStream_t strm, const hj_glaunch &L, h
 static int gbatch_create(hj_ctx_t *ctx, hj_batch *b) {
 	const uint32_t njoins = b->njoins;
 	hj_gplan P;
-	for (uint32_t j = 0; j < njoins; j++) gplan_add(P, b->R[j], b->S[j], nullptr);
+	for (uint32_t j = 0; j < njoins; j++) gplan_add(ctx, P, b->R[j], b->S[j], nullptr);
 	gplan_finish(P, (uint32_t)ctx->num_cu);
 	b->info.njoins = njoins;
 	b->info.tuples = P.tuples;
@@ -4351,6 +4359,36 @@ extern "C" int hj_autotune(hj_ctx_t *ctx, uint32_t width, uint32_t s_ratio, uint
 			}
 			lk.lock();
 		}
+		// Gathered batches (16-byte bc): the radix bits of this class, the
+		// 6144-tuple rule and one bit either side, timed through a gathered
+		// batch of this one join
+		if (rc == HJ_OK && width == 16 && spec.r_num <= (1ull << 22)) {
+			const uint32_t b0 = gbatch_bits(spec.r_num);
+			double bestg = 1e30;
+			uint32_t bestb = b0;
+			lk.unlock();
+			for (int db = -1; db <= 1; db++) {
+				const int b = (int)b0 + db;
+				if (b < 0 || b > 10) continue;
+				ctx->tune_gbits[c] = (uint8_t)b;
+				hj_batch_t *bt = nullptr;
+				if (hj_batch_create(ctx, "phj_radix_bc", 1, &R, &S, nullptr, &bt) != HJ_OK) continue;
+				double t = 1e30;
+				for (int rep = 0; rep < 3; rep++) {
+					hj_join_result_t jr{};
+					hj_batch_info_t bi{};
+					if (hj_batch_run(bt, &jr, &bi) != HJ_OK) break;
+					t = std::min(t, bi.device_ms);
+				}
+				hj_batch_destroy(bt);
+				if (t < bestg) {
+					bestg = t;
+					bestb = (uint32_t)b;
+				}
+			}
+			ctx->tune_gbits[c] = (uint8_t)bestb;
+			lk.lock();
+		}
 	}
 	ctx->kernel_times = kt;
 	(void)hipStreamSynchronize(ctx->stream);
@@ -4469,6 +4507,12 @@ extern "C" int hj_tuned_skew_config(hj_ctx_t *ctx, uint64_t r_num, uint32_t *rad
 	hj_params_t prm{};
 	prm.flags = HJ_F_SKEW_HANDLING;
 	radix_choose(ctx, r_num, &prm, radix_bits, passes, radix_bits_pass1, probe_scale);
+	return HJ_OK;
+}
+
+extern "C" int hj_tuned_batch_bits(hj_ctx_t *ctx, uint64_t r_num, uint32_t *radix_bits) {
+	if (!ctx || !radix_bits) return HJ_EINVAL;
+	*radix_bits = gbatch_bits(ctx, r_num);
 	return HJ_OK;
 }
 
@@ -4592,7 +4636,7 @@ static bool serve_req_gatherable(const hj_server *s, const hj_sreq &r) {
 // small are within the batch overhead).
 static uint64_t serve_gfootprint(const hj_server *s, const hj_sreq &r) {
 	hj_gplan P;
-	gplan_add(P, r.R, r.S, r.build >= 0 ? &s->builds[r.build].g : nullptr);
+	gplan_add(s->ctx, P, r.R, r.S, r.build >= 0 ? &s->builds[r.build].g : nullptr);
 	gplan_finish(P, 0);
 	return P.bytes + 20;
 }
@@ -4798,7 +4842,7 @@ static int serve_launch_g(hj_server *s, hj_slot &sl, const std::vector<uint64_t>
 	hj_gplan P;
 	for (uint64_t id : ids) {
 		const hj_sreq &r = s->reqs[id];
-		gplan_add(P, r.R, r.S, r.build >= 0 ? &s->builds[r.build].g : nullptr);
+		gplan_add(ctx, P, r.R, r.S, r.build >= 0 ? &s->builds[r.build].g : nullptr);
 	}
 	gplan_finish(P, (uint32_t)ctx->num_cu);
 	const size_t o_st = al256(P.bytes), o_res = o_st + al256(sizeof(hj_stats)), total = o_res + al256((size_t)n * 20 + 64);
@@ -5106,7 +5150,7 @@ extern "C" int hj_serve_register_build(hj_server_t *s, const hj_relation_t *R, u
 		// gathered layout: pass A over R alone; the tiles, digit-major run rows
 		// and descriptors stay in one device region for the server's life
 		hj_gplan P;
-		const uint32_t bits = gbatch_bits(R->tuple_num);
+		const uint32_t bits = gbatch_bits(ctx, R->tuple_num);
 		const uint32_t si = gplan_seg(P, *R, bits);
 		gplan_finish(P, (uint32_t)ctx->num_cu);
 		const size_t o_st = al256(P.bytes), total = o_st + al256(sizeof(hj_stats));

@@ -132,6 +132,7 @@ int hj_ctx_create(int device, hj_ctx_t **out) {
 	memset(ctx->tune_pass1, 0, sizeof ctx->tune_pass1);
 	memset(ctx->tune_bits1, 0xff, sizeof ctx->tune_bits1);
 	memset(ctx->tune_var1, 0, sizeof ctx->tune_var1);
+	memset(ctx->tune_gbits, 0xff, sizeof ctx->tune_gbits);
 	memset(ctx->tune_bits_z, 0xff, sizeof ctx->tune_bits_z);
 	memset(ctx->tune_passes_z, 0xff, sizeof ctx->tune_passes_z);
 	memset(ctx->tune_pass1_z, 0, sizeof ctx->tune_pass1_z);

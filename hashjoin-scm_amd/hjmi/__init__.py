@@ -167,6 +167,7 @@ def load_library(path: str = LIB_PATH):
         "hj_tuned_config": (I, [P, U64, C.POINTER(U32), C.POINTER(U32)]),
         "hj_tuned_pass1_bits": (I, [P, U64, C.POINTER(U32)]),
         "hj_autotune_zipf": (I, [P, U32, U32, U32, U32, C.c_double]),
+        "hj_tuned_batch_bits": (I, [P, U64, C.POINTER(U32)]),
         "hj_tuned_skew_config": (I, [P, U64, C.POINTER(U32), C.POINTER(U32), C.POINTER(U32), C.POINTER(U32)]),
         "hj_serve_open": (I, [P, C.c_char_p, C.POINTER(ServeConfig), C.POINTER(P)]),
         "hj_serve_register_build": (I, [P, C.POINTER(Relation), C.POINTER(U32)]),
@@ -195,7 +196,7 @@ def exported_symbols():
             "hj_algo_name", "hj_kernel_name", "hj_join", "hj_join_noalloc", "hj_dev_alloc", "hj_dev_free", "hj_copy_h2d",
             "hj_copy_d2h", "hj_sync", "hj_generate", "hj_device_info", "hj_copy_d2d", "hj_batch_create",
             "hj_batch_run", "hj_batch_destroy", "hj_autotune", "hj_tuned_config", "hj_tuned_pass1_bits",
-            "hj_autotune_zipf", "hj_tuned_skew_config",
+            "hj_autotune_zipf", "hj_tuned_skew_config", "hj_tuned_batch_bits",
             "hj_serve_open",
             "hj_serve_register_build", "hj_serve_submit", "hj_serve_poll", "hj_serve_drain", "hj_serve_stats",
             "hj_serve_close", "hj_ref_spec_for", "hj_ref_generate", "hj_ref_rand", "hj_data_cfg_read",
@@ -394,6 +395,11 @@ class Context:
         self._check(self.lib.hj_tuned_skew_config(self.h, r_num, C.byref(b), C.byref(p), C.byref(b1), C.byref(ps)),
                     "hj_tuned_skew_config")
         return {"radix_bits": b.value, "passes": p.value, "radix_bits_pass1": b1.value, "probe_scale": ps.value}
+
+    def tuned_batch_bits(self, r_num: int) -> int:
+        b = C.c_uint32()
+        self._check(self.lib.hj_tuned_batch_bits(self.h, r_num, C.byref(b)), "hj_tuned_batch_bits")
+        return b.value
 
     def tuned_pass1_bits(self, r_num: int) -> int:
         b1 = C.c_uint32()

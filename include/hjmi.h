@@ -232,6 +232,9 @@ int hj_autotune(hj_ctx_t *ctx, uint32_t width, uint32_t s_ratio, uint32_t min_lo
 int hj_tuned_config(hj_ctx_t *ctx, uint64_t r_num, uint32_t *radix_bits, uint32_t *passes);
 /* the tuned first-pass width of a two-pass class (0: the default split) */
 int hj_tuned_pass1_bits(hj_ctx_t *ctx, uint64_t r_num, uint32_t *radix_bits_pass1);
+/* the radix bits a gathered batch gives a join of |R| = r_num (tuned for
+ * 16-byte tuples by hj_autotune, else about 6144 build tuples a partition) */
+int hj_tuned_batch_bits(hj_ctx_t *ctx, uint64_t r_num, uint32_t *radix_bits);
 /* The autotuner on skewed data: each class's candidates (as hj_autotune)
  * timed on probe keys Zipf-distributed over the build keys with exponent
  * theta, then the probe-range scale (part_variant bits 20-21) for the best;

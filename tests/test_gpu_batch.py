@@ -145,6 +145,10 @@ def test_autotune_classes_and_parity(ctx):
     for c in range(10, 17):
         bits, passes = ctx.tuned_config(1 << c)
         assert bits <= c and passes in (0, 1, 2)
+        # gathered batches: the 6144-tuple rule or one bit either side
+        rule = 0 if (1 << c) <= 6144 else ((1 << c) + 6143) // 6144 - 1
+        rule = rule.bit_length()
+        assert abs(ctx.tuned_batch_bits(1 << c) - rule) <= 1, c
         b1 = ctx.tuned_pass1_bits(1 << c)
         assert b1 == 0 or (passes == 2 and 1 <= b1 < bits)
     R, S = ctx.generate("pkfk", 1 << 16, 1 << 20, 16)
