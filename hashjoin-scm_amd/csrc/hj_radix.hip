@@ -4574,6 +4574,7 @@ struct hj_slot {
 	std::vector<uint64_t> reqs;
 	uint64_t admitted = 0;
 	bool busy = false;
+	bool big = false;  // the batch in flight holds a large request
 };
 
 struct hj_solo_lane {
@@ -4622,6 +4623,11 @@ static uint64_t serve_gmax(const hj_server *) { return 1ull << 22; }
 // latency of the two-pass engine) and into a gathered batch otherwise
 // (saturation: the batches' throughput).
 static const uint64_t kServeSoloGathered = 1ull << 20;
+// A request of at least this many tuples (|R| + |S|, about half a
+// millisecond of device time) runs alone while fewer than kServeHoldQueue
+// requests wait (see serve_loop).
+static const uint64_t kServeBigTuples = 1ull << 26;
+static const size_t kServeHoldQueue = 16;
 
 // Whether request r can join a gathered batch: its build side is a resident
 // build kept in gathered layout, or holds at most serve_gmax tuples, and its
@@ -4920,8 +4926,16 @@ static void serve_loop(hj_server *s) {
 				busy++;
 			}
 		}
+		// Light load: while a batch holding a large request runs and few
+		// requests wait, no other batch starts, so the large one has the
+		// device to itself (as one join at a time would) instead of sharing it
+		// with the small ones arriving meanwhile; a longer queue brings the
+		// overlap back for throughput.
+		bool big_running = false;
+		for (const hj_slot &sl : s->slots) big_running |= sl.busy && sl.big;
+		const bool hold = big_running && s->queue.size() < kServeHoldQueue;
 		for (hj_slot &sl : s->slots) {
-			if (sl.busy || s->queue.empty()) continue;
+			if (sl.busy || s->queue.empty() || hold) continue;
 			std::vector<uint64_t> ids;
 			uint64_t fp = kServeBatchOverhead;
 			uint64_t hist_entries = 0;
@@ -4943,6 +4957,11 @@ static void serve_loop(hj_server *s) {
 			if (ids.empty()) continue;
 			for (uint64_t id : ids) s->reqs[id].state = 1;
 			const int rc = g ? serve_launch_g(s, sl, ids) : serve_launch(s, sl, ids);
+			sl.big = false;
+			for (uint64_t id : ids) {
+				const hj_sreq &r = s->reqs[id];
+				sl.big |= r.R.tuple_num + r.S.tuple_num >= kServeBigTuples;
+			}
 			if (rc != HJ_OK) {  // these requests fail with the launch's status
 				for (uint64_t id : ids) {
 					hj_sreq &r = s->reqs[id];
