@@ -1721,17 +1721,15 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
 				if (fb < fe) {
 					uint32_t k = find_run(fb);
 					tup_t ta[RU] = {}, tb[RU] = {};
+					// loads never under a condition (indices clamp to fe - 1): a
+					// conditional prefetch makes the compiler wait at the merge
+					// for the loads it has just issued
 					load_step(fb, fe, k, ta, true);
-					for (uint32_t f0 = fb;;) {
-						const uint32_t f1 = f0 + STEP;
-						if (f1 < fe) load_step(f1, fe, k, tb, true);
+					for (uint32_t f0 = fb; f0 < fe; f0 += 2 * STEP) {
+						load_step(f0 + STEP, fe, k, tb, true);
 						probe_step(f0, fe, ta);
-						if (f1 >= fe) break;
-						const uint32_t f2 = f1 + STEP;
-						if (f2 < fe) load_step(f2, fe, k, ta, true);
-						probe_step(f1, fe, tb);
-						if (f2 >= fe) break;
-						f0 = f2;
+						load_step(f0 + 2 * STEP, fe, k, ta, true);
+						probe_step(f0 + STEP, fe, tb);  // lanes past fe skip
 					}
 				}
 				__syncthreads();
@@ -1935,17 +1933,13 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(6))) void k_
 				if (fb < fe) {
 					cursor_t cs = seek(fb);
 					uint2 ta[RU], tb[RU];
+					// unconditional prefetch (indices clamp to fe - 1), see k_join2
 					load_step(fb, fe, cs, ta, true);
-					for (uint32_t f0 = fb;;) {
-						const uint32_t f1 = f0 + STEP;
-						if (f1 < fe) load_step(f1, fe, cs, tb, true);
+					for (uint32_t f0 = fb; f0 < fe; f0 += 2 * STEP) {
+						load_step(f0 + STEP, fe, cs, tb, true);
 						probe_step(f0, fe, ta);
-						if (f1 >= fe) break;
-						const uint32_t f2 = f1 + STEP;
-						if (f2 < fe) load_step(f2, fe, cs, ta, true);
-						probe_step(f1, fe, tb);
-						if (f2 >= fe) break;
-						f0 = f2;
+						load_step(f0 + 2 * STEP, fe, cs, ta, true);
+						probe_step(f0 + STEP, fe, tb);  // lanes past fe skip
 					}
 				}
 				__syncthreads();
