@@ -186,6 +186,9 @@ int hj_launch_generate(hj_ctx_t *ctx, const hj_gen_spec_t *spec, void *r, void *
 
 int hj_run_nphj_lp(hj_ctx_t *ctx, const hj_relation_t *R, const hj_relation_t *S,
                    const hj_params_t *prm, hj_result_t *out);
+// first-match join with nphj_lp's semantics and a multiplicative hash (the
+// exact path of the radix, batch and serving lp joins)
+int hj_run_lp_exact(hj_ctx_t *ctx, const hj_relation_t *R, const hj_relation_t *S, hj_result_t *out);
 int hj_run_nphj_sc(hj_ctx_t *ctx, const hj_relation_t *R, const hj_relation_t *S,
                    const hj_params_t *prm, hj_result_t *out);
 // tk = per-partition table kind (HJ_TK_* of hj_tables.h: 0 bc, 1 lp, 2 sc,

@@ -97,10 +97,23 @@ int hj_stats_read(hj_ctx_t *ctx, hj_stats *h) {
 }
 
 // ------------------------------------------------------------------ nphj_lp
-template <int WV>
+// Home slot of a key.  nphj_lp uses the reference's identity hash key & mask;
+// the exact first-match path of the radix, batch and serving lp joins (MIX)
+// uses a multiplicative (Fibonacci) hash, (key * 2^64/phi) >> (64 - log2
+// slots), so that keys sharing their low bits (multiples of 2^12, say) do not
+// all start at a few slots: with the identity hash 30000 such keys in 2^15
+// slots form 8 clusters, and the build's claims (device-scope CAS on a few hot
+// lines) grow with the square of the cluster size.  The result does not
+// depend on the hash: every key keeps the payload of its smallest R row.
+template <bool MIX>
+__device__ __forceinline__ uint64_t lp_home(uint64_t key, uint64_t mask, uint32_t shift) {
+	return MIX ? (key * 0x9E3779B97F4A7C15ULL) >> shift : key & mask;
+}
+
+template <int WV, bool MIX>
 __global__ __launch_bounds__(NPHJ_THREADS) void k_lp_build(const uint4 *__restrict__ R, uint64_t n,
                                                            unsigned long long *__restrict__ tab, uint64_t mask,
-                                                           hj_stats *st) {
+                                                           uint32_t shift, hj_stats *st) {
 	const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
 	bool dup = false, full = false;
 	for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
@@ -111,7 +124,7 @@ __global__ __launch_bounds__(NPHJ_THREADS) void k_lp_build(const uint4 *__restri
 			atomicMin(&st->sent_min, (unsigned long long)i);
 			continue;
 		}
-		uint64_t pos = key & mask;
+		uint64_t pos = lp_home<MIX>(key, mask, shift);
 		uint64_t step = 0;
 		for (; step <= mask; step++) {
 			const unsigned long long old = atomicCAS(&tab[2 * pos], LP_EMPTY, key);
@@ -133,8 +146,9 @@ __global__ __launch_bounds__(NPHJ_THREADS) void k_lp_build(const uint4 *__restri
 }
 
 // slot holding `key` (it must be present); used by the duplicate fix-up
-__device__ __forceinline__ uint64_t lp_find(const unsigned long long *tab, uint64_t mask, uint64_t key) {
-	uint64_t pos = key & mask;
+template <bool MIX>
+__device__ __forceinline__ uint64_t lp_find(const unsigned long long *tab, uint64_t mask, uint32_t shift, uint64_t key) {
+	uint64_t pos = lp_home<MIX>(key, mask, shift);
 	for (uint64_t step = 0; step <= mask; step++) {
 		if (tab[2 * pos] == key) return pos;
 		pos = (pos + 1) & mask;
@@ -142,37 +156,38 @@ __device__ __forceinline__ uint64_t lp_find(const unsigned long long *tab, uint6
 	return ~0ULL;
 }
 
-template <int WV>
+template <int WV, bool MIX>
 __global__ __launch_bounds__(NPHJ_THREADS) void k_lp_minidx(const uint4 *__restrict__ R, uint64_t n,
                                                             const unsigned long long *__restrict__ tab, uint64_t mask,
-                                                            unsigned long long *__restrict__ minidx) {
+                                                            uint32_t shift, unsigned long long *__restrict__ minidx) {
 	const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
 	for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
 		const uint64_t key = hj_key_of(R[i * WV]);
 		if (key == LP_EMPTY) continue;
-		const uint64_t pos = lp_find(tab, mask, key);
+		const uint64_t pos = lp_find<MIX>(tab, mask, shift, key);
 		if (pos != ~0ULL) atomicMin(&minidx[pos], (unsigned long long)i);
 	}
 }
 
-template <int WV>
+template <int WV, bool MIX>
 __global__ __launch_bounds__(NPHJ_THREADS) void k_lp_setpay(const uint4 *__restrict__ R, uint64_t n,
                                                             unsigned long long *__restrict__ tab, uint64_t mask,
-                                                            const unsigned long long *__restrict__ minidx) {
+                                                            uint32_t shift, const unsigned long long *__restrict__ minidx) {
 	const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
 	for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
 		const uint4 t = R[i * WV];
 		const uint64_t key = hj_key_of(t);
 		if (key == LP_EMPTY) continue;
-		const uint64_t pos = lp_find(tab, mask, key);
+		const uint64_t pos = lp_find<MIX>(tab, mask, shift, key);
 		if (pos != ~0ULL && minidx[pos] == i) tab[2 * pos + 1] = hj_pay_of(t);
 	}
 }
 
-template <int WV, bool EMIT>
+template <int WV, bool EMIT, bool MIX>
 __global__ __launch_bounds__(NPHJ_THREADS) void k_lp_probe(const uint4 *__restrict__ S, uint64_t n,
                                                            const ulonglong2 *__restrict__ tab, uint64_t mask,
-                                                           const uint4 *__restrict__ R, hj_stats *st, uint4 *pout,
+                                                           uint32_t shift, const uint4 *__restrict__ R, hj_stats *st,
+                                                           uint4 *pout,
                                                            unsigned long long pcap, unsigned pchunk) {
 	__shared__ unsigned long long red[2 * (NPHJ_THREADS / HJ_WAVE)];
 	__shared__ unsigned long long emit_ws[EMIT ? 2 * (NPHJ_THREADS / HJ_WAVE) : 2];
@@ -194,7 +209,7 @@ __global__ __launch_bounds__(NPHJ_THREADS) void k_lp_probe(const uint4 *__restri
 		}
 #pragma unroll
 		for (int u = 0; u < PROBE_U; u++) {
-			pos[u] = key[u] & mask;
+			pos[u] = lp_home<MIX>(key[u], mask, shift);
 			if (live[u] && key[u] != LP_EMPTY) e[u] = tab[pos[u]];
 		}
 #pragma unroll
@@ -220,16 +235,25 @@ __global__ __launch_bounds__(NPHJ_THREADS) void k_lp_probe(const uint4 *__restri
 }
 
 // Build (and, if duplicates were seen, fix up) the LP table for R.
-// `tab` has `slots` 16-byte slots.  Returns device flags through *flags.
-static int lp_build_table(hj_ctx_t *ctx, const hj_relation_t *R, unsigned long long *tab, uint64_t slots,
+// `tab` has `slots` 16-byte slots (slots >= 2 when mix).
+static int lp_build_table(hj_ctx_t *ctx, const hj_relation_t *R, unsigned long long *tab, uint64_t slots, bool mix,
                           hj_result_t *out) {
 	const uint64_t mask = slots - 1;
+	const uint32_t shift = 64 - hj_log2_ceil(slots);
 	const uint32_t wv = R->tuple_size / 16;
 	hj_stats *st = hj_stats_dev(ctx);
 	const uint64_t n = R->tuple_num;
 	const unsigned g = grid_for(ctx, n, NPHJ_THREADS);
 	hj_kt_begin(ctx, HJ_K_LP_BUILD);
-#define LP_B(W) hipLaunchKernelGGL(k_lp_build<W>, dim3(g), dim3(NPHJ_THREADS), 0, ctx->stream, (const uint4 *)R->tuples, n, tab, mask, st)
+#define LP_B1(W, M)                                                                                              \
+	hipLaunchKernelGGL((k_lp_build<W, M>), dim3(g), dim3(NPHJ_THREADS), 0, ctx->stream, (const uint4 *)R->tuples, n, \
+	                   tab, mask, shift, st)
+#define LP_B(W)          \
+	if (mix) {           \
+		LP_B1(W, true);  \
+	} else {             \
+		LP_B1(W, false); \
+	}
 	switch (wv) {
 	case 1: LP_B(1); break;
 	case 2: LP_B(2); break;
@@ -239,6 +263,7 @@ static int lp_build_table(hj_ctx_t *ctx, const hj_relation_t *R, unsigned long l
 	default: LP_B(32); break;
 	}
 #undef LP_B
+#undef LP_B1
 	hj_kt_end(ctx);
 	HJ_HIP(ctx, hipGetLastError());
 	// The duplicate flag is read back only to choose the fix-up path; PK
@@ -253,11 +278,17 @@ static int lp_build_table(hj_ctx_t *ctx, const hj_relation_t *R, unsigned long l
 		if (rc) return rc;
 		hj_fill16(ctx, aux, (slots + 1) / 2, make_uint4(~0u, ~0u, ~0u, ~0u));
 		hj_kt_begin(ctx, HJ_K_LP_FIXUP);
-#define LP_F(W)                                                                                                  \
-	hipLaunchKernelGGL(k_lp_minidx<W>, dim3(g), dim3(NPHJ_THREADS), 0, ctx->stream, (const uint4 *)R->tuples, n, tab, \
-	                   mask, (unsigned long long *)aux);                                                         \
-	hipLaunchKernelGGL(k_lp_setpay<W>, dim3(g), dim3(NPHJ_THREADS), 0, ctx->stream, (const uint4 *)R->tuples, n, tab, \
-	                   mask, (const unsigned long long *)aux)
+#define LP_F1(W, M)                                                                                              \
+	hipLaunchKernelGGL((k_lp_minidx<W, M>), dim3(g), dim3(NPHJ_THREADS), 0, ctx->stream, (const uint4 *)R->tuples, n, \
+	                   tab, mask, shift, (unsigned long long *)aux);                                               \
+	hipLaunchKernelGGL((k_lp_setpay<W, M>), dim3(g), dim3(NPHJ_THREADS), 0, ctx->stream, (const uint4 *)R->tuples, n, \
+	                   tab, mask, shift, (const unsigned long long *)aux)
+#define LP_F(W)          \
+	if (mix) {           \
+		LP_F1(W, true);  \
+	} else {             \
+		LP_F1(W, false); \
+	}
 		switch (wv) {
 		case 1: LP_F(1); break;
 		case 2: LP_F(2); break;
@@ -267,6 +298,7 @@ static int lp_build_table(hj_ctx_t *ctx, const hj_relation_t *R, unsigned long l
 		default: LP_F(32); break;
 		}
 #undef LP_F
+#undef LP_F1
 		hj_kt_end(ctx);
 		HJ_HIP(ctx, hipGetLastError());
 		out->fallback |= HJ_FB_LP_DUPLICATES;
@@ -274,10 +306,8 @@ static int lp_build_table(hj_ctx_t *ctx, const hj_relation_t *R, unsigned long l
 	return HJ_OK;
 }
 
-int hj_run_nphj_lp(hj_ctx_t *ctx, const hj_relation_t *R, const hj_relation_t *S, const hj_params_t *prm,
-                   hj_result_t *out) {
-	uint64_t slots = prm->lp_slots_log2 ? (1ULL << prm->lp_slots_log2) : hj_next_pow2(R->tuple_num ? R->tuple_num : 1);
-	if (slots < R->tuple_num) return hj_fail(ctx, HJ_EINVAL, "lp_slots_log2 too small for |R|");
+static int lp_run(hj_ctx_t *ctx, const hj_relation_t *R, const hj_relation_t *S, uint64_t slots, bool mix,
+                  hj_result_t *out) {
 	void *tab_p = nullptr;
 	int rc = hj_scratch(ctx, ctx->table, slots * 16, &tab_p);
 	if (rc) return rc;
@@ -290,7 +320,7 @@ int hj_run_nphj_lp(hj_ctx_t *ctx, const hj_relation_t *R, const hj_relation_t *S
 	hj_fill16(ctx, tab, slots, make_uint4(~0u, ~0u, 0u, 0u));
 	hj_kt_end(ctx);
 	HJ_HIP(ctx, hipEventRecord(ctx->ev[1], ctx->stream));
-	rc = lp_build_table(ctx, R, tab, slots, out);
+	rc = lp_build_table(ctx, R, tab, slots, mix, out);
 	if (rc) return rc;
 	HJ_HIP(ctx, hipEventRecord(ctx->ev[2], ctx->stream));
 
@@ -298,16 +328,23 @@ int hj_run_nphj_lp(hj_ctx_t *ctx, const hj_relation_t *R, const hj_relation_t *S
 	const uint64_t n = S->tuple_num;
 	const unsigned g = grid_for(ctx, (n + PROBE_U - 1) / PROBE_U, NPHJ_THREADS, 8);
 	hj_kt_begin(ctx, HJ_K_LP_PROBE);
-#define LP_P(W, E)                                                                                                  \
-	hipLaunchKernelGGL((k_lp_probe<W, E>), dim3(g), dim3(NPHJ_THREADS), 0, ctx->stream, (const uint4 *)S->tuples, n, \
-	                   (const ulonglong2 *)tab, slots - 1, (const uint4 *)R->tuples, st, ctx->emit_pairs,           \
-	                   (unsigned long long)ctx->emit_cap, ctx->emit_chunk)
+#define LP_P1(W, E, M)                                                                                           \
+	hipLaunchKernelGGL((k_lp_probe<W, E, M>), dim3(g), dim3(NPHJ_THREADS), 0, ctx->stream, (const uint4 *)S->tuples, \
+	                   n, (const ulonglong2 *)tab, slots - 1, 64 - hj_log2_ceil(slots), (const uint4 *)R->tuples, st, \
+	                   ctx->emit_pairs, (unsigned long long)ctx->emit_cap, ctx->emit_chunk)
+#define LP_P(W, E)          \
+	if (mix) {              \
+		LP_P1(W, E, true);  \
+	} else {                \
+		LP_P1(W, E, false); \
+	}
 	switch (wv) {
 	case 1:
-		if (ctx->emit_pairs)
+		if (ctx->emit_pairs) {
 			LP_P(1, true);
-		else
+		} else {
 			LP_P(1, false);
+		}
 		break;
 	case 2: LP_P(2, false); break;
 	case 4: LP_P(4, false); break;
@@ -316,6 +353,7 @@ int hj_run_nphj_lp(hj_ctx_t *ctx, const hj_relation_t *R, const hj_relation_t *S
 	default: LP_P(32, false); break;
 	}
 #undef LP_P
+#undef LP_P1
 	hj_kt_end(ctx);
 	HJ_HIP(ctx, hipGetLastError());
 	HJ_HIP(ctx, hipEventRecord(ctx->ev[3], ctx->stream));
@@ -329,6 +367,21 @@ int hj_run_nphj_lp(hj_ctx_t *ctx, const hj_relation_t *R, const hj_relation_t *S
 	out->total_time = hj_event_seconds(ctx, 0, 3);
 	out->algo_bytes = (uint64_t)(head_bytes(R->tuple_size) + 16) * (R->tuple_num + S->tuple_num);
 	return HJ_OK;
+}
+
+int hj_run_nphj_lp(hj_ctx_t *ctx, const hj_relation_t *R, const hj_relation_t *S, const hj_params_t *prm,
+                   hj_result_t *out) {
+	uint64_t slots = prm->lp_slots_log2 ? (1ULL << prm->lp_slots_log2) : hj_next_pow2(R->tuple_num ? R->tuple_num : 1);
+	if (slots < R->tuple_num) return hj_fail(ctx, HJ_EINVAL, "lp_slots_log2 too small for |R|");
+	return lp_run(ctx, R, S, slots, false, out);
+}
+
+// Exact first-match join for the lp fallbacks: nphj_lp's table and duplicate
+// fix-up with the multiplicative home slot over next_pow_2(1.5 |R|) slots
+// (at most 2/3 full), so its cost does not depend on the keys' low bits.
+int hj_run_lp_exact(hj_ctx_t *ctx, const hj_relation_t *R, const hj_relation_t *S, hj_result_t *out) {
+	const uint64_t n = R->tuple_num;
+	return lp_run(ctx, R, S, std::max<uint64_t>(2, hj_next_pow2(n + n / 2)), true, out);
 }
 
 // ------------------------------------------------------------------ nphj_sc

@@ -2847,10 +2847,10 @@ static int radix_engine(hj_ctx_t *ctx, const hj_relation_t *R, const hj_relation
 	if (lp && (h.flags & (HJ_DF_DUPKEY | HJ_DF_OVERSIZE))) {
 		// Exact path: linear probing keeps only the first build tuple per key
 		// (src/algo/phj.c:655-666 with a stable partition = smallest R row),
-		// which is what nphj_lp computes with its duplicate fix-up.
+		// which is what nphj_lp computes with its duplicate fix-up (here with
+		// a multiplicative hash, whatever the keys' low bits).
 		hj_result_t r2{};
-		hj_params_t p2{};
-		rc = hj_run_nphj_lp(ctx, Rfb, Sfb, &p2, &r2);
+		rc = hj_run_lp_exact(ctx, Rfb, Sfb, &r2);
 		if (rc) return rc;
 		out->matched_cnt = r2.matched_cnt;
 		out->check_sum = r2.check_sum;
@@ -2977,13 +2977,14 @@ __global__ void k_plan_b(const hj_bjoin *__restrict__ joins, const uint32_t *__r
 	}
 }
 
-// Persistent join over (join, partition, tile) items; per-item results are
-// wave-reduced and added to res[2*join], res[2*join+1].
-template <int WV, int NT, bool LP>
+// Persistent bc join over (join, partition, tile) items; per-item results
+// are wave-reduced and added to res[2*join], res[2*join+1].  A partition
+// larger than the table is joined in several fills.
+template <int WV, int NT>
 __global__ __launch_bounds__(NT) void k_join_b(const hj_bpair *__restrict__ pairs,
                                                const uint64_t *__restrict__ tprefix, uint32_t npairs,
                                                const hj_bjoin *__restrict__ joins, uint32_t cap, hj_stats *st,
-                                               unsigned long long *__restrict__ res, unsigned int *__restrict__ jflags) {
+                                               unsigned long long *__restrict__ res) {
 	extern __shared__ uint64_t lds_u64[];
 	__shared__ uint64_t s_item[4];
 	uint64_t *keys = lds_u64;
@@ -3016,12 +3017,7 @@ __global__ __launch_bounds__(NT) void k_join_b(const hj_bpair *__restrict__ pair
 		const uint32_t B = pr.shift;
 		const uint64_t s_tile = joins[pr.join].s_tile;
 		const uint64_t sb = pr.s0 + tix * s_tile, se = min(pr.s1, sb + s_tile);
-		if (LP && pr.r1 - pr.r0 > cap) {
-			if (threadIdx.x == 0) atomicOr(&jflags[pr.join], HJ_DF_OVERSIZE);
-			continue;
-		}
 		unsigned long long cnt = 0, sum = 0;
-		bool dup = false;
 		for (uint64_t rc = pr.r0; rc < pr.r1; rc += cap) {
 			const uint32_t m = (uint32_t)min((uint64_t)cap, pr.r1 - rc);
 			const uint32_t nb = m <= 1 ? 1u : 1u << (32 - __clz(m - 1));
@@ -3050,29 +3046,13 @@ __global__ __launch_bounds__(NT) void k_join_b(const hj_bpair *__restrict__ pair
 						const uint64_t key = hj_key_of(t[u]);
 						keys[i] = key;
 						pays[i] = hj_pay_of(t[u]);
-						if (!LP) next[i] = (uint16_t)atomicExch(&heads[(uint32_t)(key >> B) & hmask], i + 1);
+						next[i] = (uint16_t)atomicExch(&heads[(uint32_t)(key >> B) & hmask], i + 1);
 					}
 				}
 			};
 			if (WV == 1 && pr.rn) build(uint2{});
 			else build(uint4{});
 			__syncthreads();
-			if (LP) {
-				for (uint32_t i = threadIdx.x; i < m; i += NT) {
-					const uint64_t key = keys[i];
-					uint32_t pos = (uint32_t)(key >> B) & hmask;
-					for (uint32_t step = 0; step < nb; step++) {
-						const uint32_t old = atomicCAS(&heads[pos], 0u, i + 1);
-						if (old == 0) break;
-						if (keys[old - 1] == key) {
-							dup = true;
-							break;
-						}
-						pos = (pos + 1) & hmask;
-					}
-				}
-				__syncthreads();
-			}
 			// probe tuples U per thread and step; the next step's loads are
 			// issued (unconditional, clamped to the last tuple) before the
 			// current step is probed
@@ -3094,25 +3074,11 @@ __global__ __launch_bounds__(NT) void k_join_b(const hj_bpair *__restrict__ pair
 				for (int u = 0; u < U; u++) {
 					if (j0 + (uint64_t)u * NT >= se) continue;
 					const uint64_t key = hj_key_of(t[u]), sp = hj_pay_of(t[u]);
-					uint32_t h = (uint32_t)(key >> B) & hmask;
-					if (!LP) {
-						for (uint32_t hit = heads[h]; hit; hit = next[hit - 1])
-							if (keys[hit - 1] == key) {
-								cnt++;
-								sum += pays[hit - 1] + sp;
-							}
-					} else {
-						for (uint32_t step = 0; step < nb; step++) {
-							const uint32_t o = heads[h];
-							if (!o) break;
-							if (keys[o - 1] == key) {
-								cnt++;
-								sum += pays[o - 1] + sp;
-								break;
-							}
-							h = (h + 1) & hmask;
+					for (uint32_t hit = heads[(uint32_t)(key >> B) & hmask]; hit; hit = next[hit - 1])
+						if (keys[hit - 1] == key) {
+							cnt++;
+							sum += pays[hit - 1] + sp;
 						}
-					}
 				}
 			};
 			// the same number of steps in every lane (sb < se), loads clamped
@@ -3140,13 +3106,12 @@ __global__ __launch_bounds__(NT) void k_join_b(const hj_bpair *__restrict__ pair
 			atomicAdd(&res[2 * pr.join], cnt);
 			atomicAdd(&res[2 * pr.join + 1], sum);
 		}
-		if (__any(dup) && lane0) atomicOr(&jflags[pr.join], HJ_DF_DUPKEY);
 	}
 }
 
 // ---------------------------------------------------------------------------
-// Gathered batches (16-byte bc batches): no histogram pass and no scatter.
-// Every relation of every join is cut into 4096-tuple tiles that pass A
+// Gathered batches (16-byte tuples, bc and lp): no histogram pass and no scatter.
+// Every relation of every join is cut into 8192-tuple tiles that pass A
 // (k_sortchunk<..., true>) counting-sorts by the join's own digit in place,
 // storing a tile as 4-byte positional words, 8-byte {key, row_id} or 16-byte
 // tuples, with a run table per tile.  k_run_transpose_b makes the run tables
@@ -3242,7 +3207,7 @@ __device__ __forceinline__ void hj_gdecode(const uint2 &v, uint32_t pz, uint32_t
 template <int NT, bool NARROW>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_join_g(
     const hj_gjoin *__restrict__ joins, const hj_gitem *__restrict__ items, const uint64_t *__restrict__ qb,
-    uint32_t cap, hj_stats *st, unsigned long long *__restrict__ res) {
+    uint32_t cap, hj_stats *st, unsigned long long *__restrict__ res, unsigned int *__restrict__ jflags, bool lp) {
 	if (NARROW == ((st->flags & HJ_DF_WIDE) != 0)) return;
 	// loaded bytes per tuple: 8 when no tile is 16-byte (an 8-byte load of a
 	// 4-byte word reads 4 bytes of its tile beyond it), else 16
@@ -3436,6 +3401,21 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
 				__syncthreads();
 				if (m >= cap || rc >= J.ncR) break;
 			}
+			// lp (first build tuple per key): without duplicate build keys its
+			// result is bc's; a duplicate in the table, or a partition needing a
+			// second fill (a duplicate could span fills), flags the join for the
+			// host's exact path, which replaces its result
+			if (lp) {
+				bool dup = false;
+				for (uint32_t i = threadIdx.x; i < m; i += NT) {
+					const uint64_t ki = NARROW ? (uint64_t)(uint32_t)kv[i] : kv[i];
+					for (uint32_t c = heads[(uint32_t)ki & hmask]; c && !dup; c = next[c - 1])
+						dup = c - 1 != i && (NARROW ? (uint64_t)(uint32_t)kv[c - 1] : kv[c - 1]) == ki;
+				}
+				const bool more = rc < J.ncR;
+				if (__syncthreads_or(dup) && threadIdx.x == 0) atomicOr(&jflags[it.join], HJ_DF_DUPKEY);
+				if (more && threadIdx.x == 0) atomicOr(&jflags[it.join], HJ_DF_OVERSIZE);
+			}
 			// ---- probe S tiles [sb, se): each wave stages and probes chunks of
 			// cw tiles (wave w: chunks w, w + NW, ...) with no block barrier,
 			// the next chunk's run bounds and tile descriptors loaded while the
@@ -3521,20 +3501,17 @@ static size_t join_g_lds(uint32_t cap, bool narrow) {
 // build tuples) lets two share a CU, so 512 threads keep 16 waves per CU.
 static constexpr int kJoinBNT = 512;
 
-template <int WV, bool LP>
+template <int WV>
 static void launch_join_b_t(hj_ctx_t *ctx, unsigned grid, size_t lds, const hj_bpair *pairs, const uint64_t *tiles,
-                            uint32_t npairs, const hj_bjoin *joins, uint32_t cap, hj_stats *st, unsigned long long *res,
-                            unsigned int *jflags) {
-	hipLaunchKernelGGL((k_join_b<WV, kJoinBNT, LP>), dim3(grid), dim3(kJoinBNT), lds, ctx->stream, pairs, tiles, npairs,
-	                   joins, cap, st, res, jflags);
+                            uint32_t npairs, const hj_bjoin *joins, uint32_t cap, hj_stats *st, unsigned long long *res) {
+	hipLaunchKernelGGL((k_join_b<WV, kJoinBNT>), dim3(grid), dim3(kJoinBNT), lds, ctx->stream, pairs, tiles, npairs, joins,
+	                   cap, st, res);
 }
 
-static void launch_join_b(hj_ctx_t *ctx, uint32_t wv, bool lp, unsigned grid, size_t lds, const hj_bpair *pairs,
+static void launch_join_b(hj_ctx_t *ctx, uint32_t wv, unsigned grid, size_t lds, const hj_bpair *pairs,
                           const uint64_t *tiles, uint32_t npairs, const hj_bjoin *joins, uint32_t cap, hj_stats *st,
-                          unsigned long long *res, unsigned int *jflags) {
-#define JB(W)                                                                                          \
-	if (lp) launch_join_b_t<W, true>(ctx, grid, lds, pairs, tiles, npairs, joins, cap, st, res, jflags); \
-	else launch_join_b_t<W, false>(ctx, grid, lds, pairs, tiles, npairs, joins, cap, st, res, jflags)
+                          unsigned long long *res) {
+#define JB(W) launch_join_b_t<W>(ctx, grid, lds, pairs, tiles, npairs, joins, cap, st, res)
 	switch (wv) {
 	case 1: JB(1); break;
 	case 2: JB(2); break;
@@ -3567,6 +3544,8 @@ struct hj_batch {
 	// between two events)
 	std::function<void()> launch;
 	bool timed = false;
+	// lp batch run join by join through the exact first-match path (no graph)
+	bool exact = false;
 };
 
 static void batch_free(hj_batch *b) {
@@ -3580,18 +3559,13 @@ static void batch_free(hj_batch *b) {
 	delete b;
 }
 
-template <int WV, bool LP>
-static void *join_b_fn() {
-	return (void *)k_join_b<WV, kJoinBNT, LP>;
-}
-
-static const void *join_b_kernel(uint32_t wv, bool lp) {
+static const void *join_b_kernel(uint32_t wv) {
 	switch (wv) {
-	case 1: return lp ? join_b_fn<1, true>() : join_b_fn<1, false>();
-	case 2: return lp ? join_b_fn<2, true>() : join_b_fn<2, false>();
-	case 4: return lp ? join_b_fn<4, true>() : join_b_fn<4, false>();
-	case 8: return lp ? join_b_fn<8, true>() : join_b_fn<8, false>();
-	default: return lp ? join_b_fn<16, true>() : join_b_fn<16, false>();
+	case 1: return (const void *)k_join_b<1, kJoinBNT>;
+	case 2: return (const void *)k_join_b<2, kJoinBNT>;
+	case 4: return (const void *)k_join_b<4, kJoinBNT>;
+	case 8: return (const void *)k_join_b<8, kJoinBNT>;
+	default: return (const void *)k_join_b<16, kJoinBNT>;
 	}
 }
 
@@ -3612,14 +3586,16 @@ static uint32_t gbatch_bits(const hj_ctx_t *ctx, uint64_t nr) {
 	return gbatch_bits(nr);
 }
 
-// Whether a batch runs gathered: 16-byte bc-family joins whose build sides
+// Whether a batch runs gathered: 16-byte joins (lp ones flag duplicate build
+// keys and multi-fill partitions for the exact path) whose build sides
 // hold at most 2^22 tuples (10 radix bits keep their partitions within one
 // LDS table) and whose probe sides hold fewer than 2^30 (positions and row
 // ids of 4-byte words fit 32 bits), unless the caller asks for the scatter
 // path.
 static bool gbatch_eligible(bool lp, uint32_t W, uint32_t njoins, const hj_relation_t *R, const hj_relation_t *S,
                             const hj_params_t *params) {
-	if (lp || W != 16 || (params && (params->flags & HJ_F_BATCH_SCATTER))) return false;
+	(void)lp;  // lp joins flag duplicate build keys for the exact path, as scattered batches do
+	if (W != 16 || (params && (params->flags & HJ_F_BATCH_SCATTER))) return false;
 	for (uint32_t j = 0; j < njoins; j++)
 		if (R[j].tuple_num > (1ull << 22) || S[j].tuple_num >= (1ull << 30)) return false;
 	return true;
@@ -3778,6 +3754,7 @@ struct hj_glaunch {
 	uint64_t nchunks = 0, nblk = 0, nitems = 0;
 	uint32_t njoins = 0, maxbits = 0, capn = 0, capw = 0;
 	unsigned gA = 1, gn = 1, gw = 1;
+	bool lp = false;  // first-match joins: flag duplicate build keys and multi-fill partitions
 	size_t sldsA = 0, ldsn = 0, ldsw = 0;
 	size_t o_part, o_run, o_runt, o_desc, o_segs, o_cmap, o_joins, o_items, o_qb, o_blk;
 };
@@ -3814,10 +3791,11 @@ static int gplan_setup(hj_ctx_t *ctx) {
 	return HJ_OK;
 }
 
-static int gplan_prepare(hj_ctx_t *ctx, const hj_gplan &P, char *base, hj_glaunch &L) {
+static int gplan_prepare(hj_ctx_t *ctx, const hj_gplan &P, char *base, hj_glaunch &L, bool lp = false) {
 	int rc;
 	if ((rc = gplan_setup(ctx))) return rc;
 	L.base = base;
+	L.lp = lp;
 	L.capn = ctx->g_capn;
 	L.capw = ctx->g_capw;
 	L.ldsn = ctx->g_ldsn;
@@ -3874,9 +3852,10 @@ static void gplan_launch(hj_ctx_t *ctx, hipStream_t strm, const hj_glaunch &L, h
 		const hj_gitem *I = (const hj_gitem *)(B + L.o_items);
 		const uint64_t *Q = (const uint64_t *)(B + L.o_qb);
 		hj_kt_begin(ctx, HJ_K_JOIN);
-		hipLaunchKernelGGL((k_join_g<kGJoinNT, true>), dim3(L.gn), dim3(kGJoinNT), L.ldsn, strm, J, I, Q, L.capn, st, res);
+		hipLaunchKernelGGL((k_join_g<kGJoinNT, true>), dim3(L.gn), dim3(kGJoinNT), L.ldsn, strm, J, I, Q, L.capn, st, res,
+		                   fl, L.lp);
 		hipLaunchKernelGGL((k_join_g<kGJoinNT, false>), dim3(L.gw), dim3(kGJoinNT), L.ldsw, strm, J, I, Q, L.capw, st,
-		                   res);
+		                   res, fl, L.lp);
 		hj_kt_end(ctx);
 	}
 }
@@ -3909,7 +3888,7 @@ static int gbatch_create(hj_ctx_t *ctx, hj_batch *b) {
 	b->d_jflags = (unsigned int *)((char *)b->res.p + (size_t)njoins * 16);
 	b->info.scratch_bytes = b->part.cap + b->res.cap + b->stats.cap;
 	hj_glaunch L;
-	if ((rc = gplan_prepare(ctx, P, (char *)b->part.p, L))) return rc;
+	if ((rc = gplan_prepare(ctx, P, (char *)b->part.p, L, b->lp))) return rc;
 	hj_stats *d_st = (hj_stats *)b->stats.p;
 	unsigned long long *d_res = b->d_res;
 	unsigned int *d_fl = b->d_jflags;
@@ -3977,6 +3956,17 @@ extern "C" int hj_batch_create(hj_ctx_t *ctx, const char *algo, uint32_t njoins,
 			batch_free(b);
 			return grc;
 		}
+		*out = b;
+		return HJ_OK;
+	}
+	if (lp) {
+		// lp batches that do not run gathered (wider tuples, build sides above
+		// 2^22, or HJ_F_BATCH_SCATTER) run every join through the exact
+		// first-match path when the batch runs
+		b->exact = true;
+		b->info.njoins = njoins;
+		for (uint32_t j = 0; j < njoins; j++) b->info.tuples += R[j].tuple_num + S[j].tuple_num;
+		b->info.n_unpartitioned = njoins;
 		*out = b;
 		return HJ_OK;
 	}
@@ -4062,17 +4052,14 @@ extern "C" int hj_batch_create(hj_ctx_t *ctx, const char *algo, uint32_t njoins,
 	const uint32_t npairs = (uint32_t)pair_join.size();
 	const uint32_t nseg = (uint32_t)segs.size();
 	const size_t slds = scatter_lds(pc, wv, 1u << maxbits, pass_line(ctx, pc, wv, 1u << maxbits, variant));
-	// LDS table: expected partition size with headroom; bigger build sides
-	// are handled by the kernels (bc: in pieces, lp: exact host path)
+	// LDS table: expected partition size with headroom, at most 3400 tuples
+	// so that two workgroups share a CU (the batch's largest partition would
+	// otherwise size every workgroup's table: C4 77 -> 68 ms); a larger
+	// partition is joined in several fills
 	const size_t lds_max = std::min<size_t>(ctx->prop.sharedMemPerBlock, 160 * 1024) - 1024;
-	uint32_t cap = (uint32_t)std::min<uint64_t>(65535, std::max<uint64_t>(small_max, max_rq + max_rq / 4 + 64));
-	// bc joins a larger partition in several fills, so its table stays small
-	// enough for two workgroups per CU (the batch's largest partition would
-	// otherwise size every workgroup's table: C4 77 -> 68 ms); lp needs the
-	// whole partition in one table
-	if (!lp) cap = std::min<uint32_t>(cap, 3400);
-	while (cap > 256 && join_lds(cap, lp) > lds_max) cap -= 256;
-	const size_t jlds = join_lds(cap, lp);
+	uint32_t cap = (uint32_t)std::min<uint64_t>(3400, std::max<uint64_t>(small_max, max_rq + max_rq / 4 + 64));
+	while (cap > 256 && join_lds(cap, false) > lds_max) cap -= 256;
+	const size_t jlds = join_lds(cap, false);
 
 	// ---- device buffers owned by the batch
 	int rc = HJ_OK;
@@ -4116,7 +4103,7 @@ extern "C" int hj_batch_create(hj_ctx_t *ctx, const char *algo, uint32_t njoins,
 	b->info.scratch_bytes = b->part.cap + b->hist.cap + b->scan_tmp.cap + b->meta.cap + b->res.cap + b->stats.cap;
 
 	// kernel attributes are set before capture (not stream work)
-	const void *jk = join_b_kernel(wv, lp);
+	const void *jk = join_b_kernel(wv);
 	allow_lds(ctx, jk);
 	if (jlds + static_lds(jk) > ctx->prop.sharedMemPerBlock) return fail(hj_fail(ctx, HJ_EINVAL, "batch LDS table too big"));
 	if (nseg && slds + 1024 > ctx->prop.sharedMemPerBlock) return fail(hj_fail(ctx, HJ_EINVAL, "batch radix bits too big"));
@@ -4151,7 +4138,7 @@ extern "C" int hj_batch_create(hj_ctx_t *ctx, const char *algo, uint32_t njoins,
 		rc = scan_u64(ctx, d_tiles, npairs, (uint64_t *)b->scan_tmp.p);
 		const int per_cu = jlds <= 40 * 1024 ? 4 : (jlds <= 76 * 1024 ? 2 : 1);
 		const unsigned jgrid = (unsigned)std::max<uint64_t>(1, (uint64_t)ctx->num_cu * per_cu);
-		if (!rc) launch_join_b(ctx, wv, lp, jgrid, jlds, d_pairs, d_tiles, npairs, d_joins, cap, d_st, b->d_res, b->d_jflags);
+		if (!rc) launch_join_b(ctx, wv, jgrid, jlds, d_pairs, d_tiles, npairs, d_joins, cap, d_st, b->d_res);
 	}
 	hipGraph_t g = nullptr;
 	e = hipStreamEndCapture(ctx->stream, &g);
@@ -4173,6 +4160,27 @@ extern "C" int hj_batch_run(hj_batch_t *b, hj_join_result_t *results, hj_batch_i
 	if (b->njoins && !results) return hj_fail(ctx, HJ_EINVAL, "null results");
 	std::lock_guard<std::mutex> lk(ctx->mu);
 	HJ_HIP(ctx, hipSetDevice(ctx->device));
+	if (b->exact) {
+		b->info.device_ms = b->info.fallback_ms = 0;
+		b->info.algo_bytes = 0;
+		b->info.n_fallback = b->njoins;
+		b->info.n_ktimes = 0;
+		for (uint32_t j = 0; j < b->njoins; j++) {
+			hj_result_t r{};
+			const bool kt = ctx->kernel_times;
+			ctx->kernel_times = false;
+			const int rc = hj_run_lp_exact(ctx, &b->R[j], &b->S[j], &r);
+			ctx->kernel_times = kt;
+			if (rc) return rc;
+			results[j].matched_cnt = r.matched_cnt;
+			results[j].check_sum = r.check_sum;
+			b->info.device_ms += r.total_time * 1e3;
+			b->info.algo_bytes += r.algo_bytes;
+		}
+		b->info.fallback_ms = b->info.device_ms;
+		if (info) *info = b->info;
+		return HJ_OK;
+	}
 	HJ_HIP(ctx, hipEventRecord(b->e0, ctx->stream));
 	if (b->timed) {
 		ctx->kernel_times = true;
@@ -4222,10 +4230,9 @@ extern "C" int hj_batch_run(hj_batch_t *b, hj_join_result_t *results, hj_batch_i
 		if (b->lp && (fl[j] & (HJ_DF_DUPKEY | HJ_DF_OVERSIZE))) {
 			// exact first-match path for this join, outside the graph
 			hj_result_t r{};
-			hj_params_t p{};
 			const bool kt = ctx->kernel_times;
 			ctx->kernel_times = false;
-			int rc = hj_run_nphj_lp(ctx, &b->R[j], &b->S[j], &p, &r);
+			int rc = hj_run_lp_exact(ctx, &b->R[j], &b->S[j], &r);
 			ctx->kernel_times = kt;
 			if (rc) return rc;
 			results[j].matched_cnt = r.matched_cnt;
@@ -4605,9 +4612,9 @@ static const uint64_t kServeChunk = 16384;
 
 static size_t al256(size_t x) { return (x + 255) & ~(size_t)255; }
 
-// Gathered serving: 16-byte bc servers run their batches gathered (first-
+// Gathered serving: 16-byte servers run their batches gathered (first-
 // pass tiles, gathered joins), like gathered hj_batch_* batches.
-static bool serve_gathered(const hj_server *s) { return s->W == 16 && !s->lp && !(s->cfg.flags & HJ_SERVE_SCATTER); }
+static bool serve_gathered(const hj_server *s) { return s->W == 16 && !(s->cfg.flags & HJ_SERVE_SCATTER); }
 
 // Largest build side a gathered server batches (the gathered limit).
 static uint64_t serve_gmax(const hj_server *) { return 1ull << 22; }
@@ -4783,11 +4790,11 @@ static int serve_launch(hj_server *s, hj_slot &sl, const std::vector<uint64_t> &
 	unsigned long long *d_res = (unsigned long long *)(A + o_res);
 	unsigned int *d_fl = (unsigned int *)(A + o_res + (size_t)n * 16);
 	const size_t lds_max =
-	    std::min<size_t>(ctx->prop.sharedMemPerBlock, 160 * 1024) - static_lds(join_b_kernel(wv, s->lp)) - 256;
-	uint32_t cap = (uint32_t)std::min<uint64_t>(65535, std::max<uint64_t>(2048, max_rq + max_rq / 4 + 64));
-	if (!s->lp) cap = std::min<uint32_t>(cap, 3400);  // as in batches: two workgroups per CU
-	while (cap > 256 && join_lds(cap, s->lp) > lds_max) cap -= 256;
-	const size_t jlds = join_lds(cap, s->lp);
+	    std::min<size_t>(ctx->prop.sharedMemPerBlock, 160 * 1024) - static_lds(join_b_kernel(wv)) - 256;
+	// as in batches: at most 3400 tuples, two workgroups per CU
+	uint32_t cap = (uint32_t)std::min<uint64_t>(3400, std::max<uint64_t>(2048, max_rq + max_rq / 4 + 64));
+	while (cap > 256 && join_lds(cap, false) > lds_max) cap -= 256;
+	const size_t jlds = join_lds(cap, false);
 	if (jlds > lds_max) return hj_fail(ctx, HJ_EINVAL, "serve join table does not fit LDS");
 
 	std::lock_guard<std::mutex> lk(ctx->mu);
@@ -4798,26 +4805,28 @@ static int serve_launch(hj_server *s, hj_slot &sl, const std::vector<uint64_t> &
 	hipError_t e = hipMemcpyAsync(A + up0, sl.hbuf, up_bytes, hipMemcpyHostToDevice, sl.stream);
 	if (e == hipSuccess) {
 		if (nseg) set_scatter_attr(ctx, wv);
-		allow_lds(ctx, join_b_kernel(wv, s->lp));
+		allow_lds(ctx, join_b_kernel(wv));
 		hipLaunchKernelGGL(k_batch_reset, dim3(std::max<uint32_t>(1, std::min<uint32_t>(1024, (n + 255) / 256))),
 		                   dim3(256), 0, sl.stream, d_st, nchunks, d_res, n, d_fl);
-		if (nseg) {
+		// an lp server's requests that do not run gathered are joined by the
+		// exact first-match path when the batch finishes: no device join here
+		if (nseg && !s->lp) {
 			rc = launch_pass(ctx, wv, 0, (unsigned)nchunks, d_segs, d_cmap, kServeChunk, 0, maxbits, d_hist, hist_len,
 			                 A + o_part, d_st, (uint64_t *)(A + o_scan), true);
 			if (!rc)
 				hipLaunchKernelGGL(k_bounds_b, dim3(std::min<uint32_t>(nseg, 65535)), dim3(256), 0, sl.stream, d_segs,
 				                   nseg, (const uint64_t *)d_hist, d_pb);
 		}
-		if (!rc) {
+		if (!rc && !s->lp) {
 			hipLaunchKernelGGL(k_plan_b, dim3(std::min<uint32_t>(65535, (npairs + 255) / 256)), dim3(256), 0, sl.stream,
 			                   d_joins, d_pj, npairs, d_segs, d_pb, (const uint4 *)(A + o_part), d_pairs, d_tiles, d_st,
 			                   nseg && wv == 1);
 			rc = scan_u64(ctx, d_tiles, npairs, (uint64_t *)(A + o_scan));
-		}
-		if (!rc) {
-			const int per_cu = jlds <= 40 * 1024 ? 4 : (jlds <= 76 * 1024 ? 2 : 1);
-			launch_join_b(ctx, wv, s->lp, (unsigned)ctx->num_cu * per_cu, jlds, d_pairs, d_tiles, npairs, d_joins, cap,
-			              d_st, d_res, d_fl);
+			if (!rc) {
+				const int per_cu = jlds <= 40 * 1024 ? 4 : (jlds <= 76 * 1024 ? 2 : 1);
+				launch_join_b(ctx, wv, (unsigned)ctx->num_cu * per_cu, jlds, d_pairs, d_tiles, npairs, d_joins, cap, d_st,
+				              d_res);
+			}
 		}
 		e = hipGetLastError();
 		if (e == hipSuccess) e = hipMemcpyAsync(sl.hbuf + sl.res_off, d_res, (size_t)n * 20, hipMemcpyDeviceToHost, sl.stream);
@@ -4858,7 +4867,7 @@ static int serve_launch_g(hj_server *s, hj_slot &sl, const std::vector<uint64_t>
 	unsigned int *d_fl = (unsigned int *)(A + o_res + (size_t)n * 16);
 	std::lock_guard<std::mutex> lk(ctx->mu);
 	hj_glaunch L;
-	if ((rc = gplan_prepare(ctx, P, A, L))) return rc;
+	if ((rc = gplan_prepare(ctx, P, A, L, s->lp))) return rc;
 	const bool kt = ctx->kernel_times;
 	ctx->kernel_times = false;
 	hipError_t e = hipMemcpyAsync(A + P.o_up, sl.hbuf, up_bytes, hipMemcpyHostToDevice, sl.stream);
@@ -4883,13 +4892,14 @@ static void serve_finish(hj_server *s, hj_slot &sl) {
 		hj_sreq &r = s->reqs[sl.reqs[j]];
 		r.res.matched_cnt = res[2 * j];
 		r.res.check_sum = res[2 * j + 1];
-		if (s->lp && (fl[j] & (HJ_DF_DUPKEY | HJ_DF_OVERSIZE))) {
-			// exact first-match path for this request
+		if (s->lp && (!r.gathered || (fl[j] & (HJ_DF_DUPKEY | HJ_DF_OVERSIZE)))) {
+			// exact first-match path for this request (every request of an lp
+			// server that did not run gathered; a gathered one that flagged a
+			// duplicate build key or a multi-fill partition)
 			std::lock_guard<std::mutex> lk(s->ctx->mu);
 			hj_result_t out{};
-			hj_params_t p{};
 			const hj_relation_t &R = r.build >= 0 ? s->builds[r.build].R : r.R;
-			r.err = hj_run_nphj_lp(s->ctx, &R, &r.S, &p, &out);
+			r.err = hj_run_lp_exact(s->ctx, &R, &r.S, &out);
 			r.res.matched_cnt = out.matched_cnt;
 			r.res.check_sum = out.check_sum;
 		}

@@ -87,7 +87,7 @@ typedef struct {
 #define HJ_F_NO_POSITIONAL 4u /* two radix passes: store every narrow first-
                                  pass tile as 8-byte {key, row_id}, also where
                                  the row ids are the positions + 1            */
-#define HJ_F_BATCH_SCATTER 8u /* batches of 16-byte bc joins: partition with a
+#define HJ_F_BATCH_SCATTER 8u /* batches of 16-byte joins: partition with a
                                  histogram and a one-pass scatter instead of
                                  first-pass tiles and gathered joins          */
 #define HJ_F_SKEW_HANDLING 16u /* radix joins with radix_bits = 0: take the
@@ -181,9 +181,10 @@ int hj_join_noalloc(hj_ctx_t *ctx, const char *algo, const hj_relation_t *R, con
  * per process).  Many independent joins run as one captured HIP graph: one
  * partition launch over all joins (radix bits chosen per join by size),
  * then one persistent join kernel pulling (join, partition, probe tile)
- * items from a device work queue.  16-byte bc-family batches sort every
+ * items from a device work queue.  Batches of 16-byte tuples sort every
  * relation's tiles by the join's digit in place and join each partition by
- * gathering its runs from those tiles; lp batches, wider tuples and
+ * gathering its runs from those tiles (an lp join that meets a duplicate
+ * build key is recomputed by the exact path); wider tuples and
  * HJ_F_BATCH_SCATTER partition with a histogram and a one-pass scatter.
  * Results are per join and bit-identical to hj_join on the same relations.
  * Relations are borrowed for the life of the batch and must stay in device
@@ -252,10 +253,10 @@ int hj_tuned_skew_config(hj_ctx_t *ctx, uint64_t r_num, uint32_t *radix_bits, ui
  * whenever a slot is free, requests from the head of the queue are admitted
  * while their exact scratch footprint fits that slot's arena (at most
  * max_batch per launch), planned as one batch and launched asynchronously,
- * so batches on different slots overlap on the device.  16-byte bc servers
+ * so batches on different slots overlap on the device.  16-byte servers
  * run their batches gathered (as hj_batch_* does) and batch every request
  * whose build side holds at most 2^20 tuples (2^22 without solo lanes);
- * other servers (lp, wider tuples, HJ_SERVE_SCATTER) partition batches with
+ * other servers (wider tuples, HJ_SERVE_SCATTER) partition batches with
  * a histogram and a one-pass scatter.  Larger requests (gathered servers) or
  * those with |R| >= 2^solo_min_log2 (scattering ones) go to solo lanes:
  * worker threads with their own context and stream that run the full

@@ -69,6 +69,31 @@ def test_batch_duplicates_misses_vs_oracle(ctx, algo):
     _free(ctx, Rs, Ss)
 
 
+@pytest.mark.parametrize("width,flags", [(64, 0), (16, hjmi.HJ_F_BATCH_SCATTER)])
+def test_lp_batch_outside_the_gathered_path_is_exact(ctx, width, flags):
+    """lp batches that do not run gathered (tuples wider than 16 bytes, or
+    HJ_F_BATCH_SCATTER) join every pair through the exact first-match path:
+    results equal the joins run alone and the oracle's."""
+    sizes = [5, 2000, 30000]
+    Rs, Ss, want = [], [], []
+    for i, nr in enumerate(sizes):
+        hR, hS = O.generate("random", nr, nr * 4, width, seed=70 + i, key_range=max(1, nr // 2))
+        want.append(O.aggregate(hR, hS, width, lp=True))
+        Rs.append(ctx.upload(hR, width))
+        Ss.append(ctx.upload(hS, width))
+    b = ctx.batch("phj_radix_lp", Rs, Ss, flags=flags)
+    got, info = b.run()
+    assert info["gathered"] == 0 and info["n_fallback"] == len(sizes)
+    for i in range(len(sizes)):
+        assert got[i] == (want[i]["matched_cnt"], want[i]["check_sum"]), i
+        alone = ctx.join("phj_radix_lp", Rs[i], Ss[i])
+        assert got[i] == (alone["matched_cnt"], alone["check_sum"]), i
+    again, _ = b.run()
+    assert again == got
+    b.close()
+    _free(ctx, Rs, Ss)
+
+
 def test_batch_sees_new_inputs_on_replay(ctx):
     Rs, Ss = _make(ctx, 8, 8, 14, seed=3)
     b = ctx.batch("phj_radix_bc", Rs, Ss)

@@ -116,6 +116,22 @@ def test_serve_lp_duplicates_exact(ctx):
     ctx.free_relation(S)
 
 
+def test_serve_lp_scatter_flag_is_exact(ctx):
+    """An lp server told not to gather (HJ_SERVE_SCATTER) joins every
+    request through the exact first-match path."""
+    pool = _pool(ctx, 6, 6, 14, 31, wl="random")
+    want = [(a["matched_cnt"], a["check_sum"]) for a in (ctx.join("phj_radix_lp", R, S) for R, S in pool)]
+    srv = hjmi.Server(ctx, "phj_radix_lp", hbm_cap_bytes=2 << 30, slots=2, max_batch=4, flags=hjmi.HJ_SERVE_SCATTER)
+    ids = [(srv.submit(S, R=R), i) for i, (R, S) in enumerate(pool) for _ in range(2)]
+    srv.drain()
+    for rid, i in ids:
+        assert srv.poll(rid)[0] == want[i], (rid, i)
+    srv.close()
+    for R, S in pool:
+        ctx.free_relation(R)
+        ctx.free_relation(S)
+
+
 def test_serve_admission_rejects_oversized(ctx):
     srv = hjmi.Server(ctx, "phj_radix_bc", hbm_cap_bytes=64 << 20, slots=2, max_batch=4)
     R, S = ctx.generate("pkfk", 1 << 16, 1 << 21, 16)  # 32 MiB probe side > one 32 MiB arena
