@@ -184,8 +184,10 @@ int hj_join_noalloc(hj_ctx_t *ctx, const char *algo, const hj_relation_t *R, con
  * items from a device work queue.  Batches of 16-byte tuples sort every
  * relation's tiles by the join's digit in place and join each partition by
  * gathering its runs from those tiles (an lp join that meets a duplicate
- * build key is recomputed by the exact path); wider tuples and
- * HJ_F_BATCH_SCATTER partition with a histogram and a one-pass scatter.
+ * build key or a partition larger than one LDS table is recomputed by the
+ * exact first-match path); bc batches of wider tuples and with
+ * HJ_F_BATCH_SCATTER partition with a histogram and a one-pass scatter, and
+ * lp batches off the gathered path run every join through the exact path.
  * Results are per join and bit-identical to hj_join on the same relations.
  * Relations are borrowed for the life of the batch and must stay in device
  * memory; their contents may change between runs. */
@@ -256,8 +258,9 @@ int hj_tuned_skew_config(hj_ctx_t *ctx, uint64_t r_num, uint32_t *radix_bits, ui
  * so batches on different slots overlap on the device.  16-byte servers
  * run their batches gathered (as hj_batch_* does) and batch every request
  * whose build side holds at most 2^20 tuples (2^22 without solo lanes);
- * other servers (wider tuples, HJ_SERVE_SCATTER) partition batches with
- * a histogram and a one-pass scatter.  Larger requests (gathered servers) or
+ * other bc servers (wider tuples, HJ_SERVE_SCATTER) partition batches with
+ * a histogram and a one-pass scatter, and lp requests that do not run
+ * gathered are joined by the exact first-match path.  Larger requests (gathered servers) or
  * those with |R| >= 2^solo_min_log2 (scattering ones) go to solo lanes:
  * worker threads with their own context and stream that run the full
  * single-join engine (two radix passes) one request at a time, overlapping
