@@ -71,6 +71,18 @@ __device__ __forceinline__ void hj_store_stream8(uint2 *p, uint32_t a, uint32_t 
 __device__ __forceinline__ void hj_store_stream4(uint32_t *p, uint32_t a) {
 	__builtin_nontemporal_store(a, (HJ_GLOBAL uint32_t *)p);
 }
+// 16 and 8 bytes at a 4-byte aligned address (global loads of several
+// dwords need dword alignment only), streaming.
+typedef unsigned int hj_u32x4a __attribute__((ext_vector_type(4), aligned(4)));
+__device__ __forceinline__ uint4 hj_load16a4(const char *p) {
+	const hj_u32x4a v = __builtin_nontemporal_load((const HJ_GLOBAL hj_u32x4a *)p);
+	return make_uint4(v.x, v.y, v.z, v.w);
+}
+typedef unsigned int hj_u32x2a __attribute__((ext_vector_type(2), aligned(4)));
+__device__ __forceinline__ uint2 hj_load8a4(const char *p) {
+	const hj_u32x2a v = __builtin_nontemporal_load((const HJ_GLOBAL hj_u32x2a *)p);
+	return make_uint2(v.x, v.y);
+}
 // Plain (cached) global loads.
 __device__ __forceinline__ uint4 hj_load(const uint4 *p) {
 	const hj_u32x4 v = *(const HJ_GLOBAL hj_u32x4 *)p;

@@ -1871,21 +1871,67 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(6))) void k_
 			t[u] = stream ? hj_load_stream8(src8) : hj_load8(src8);
 		}
 	};
-	auto probe_step = [&](uint32_t f0, uint32_t fe, const uint2 (&t)[RU]) {
-		uint32_t c[RU];
+	// The probe side is read in pairs: a staged run of L tuples is
+	// ceil(L/2) pairs, each one 16-byte load of the run's tuples 2j and 2j+1
+	// (half the load instructions of 8-byte loads); the last pair of an odd
+	// run holds the next run's first tuple in its upper half, masked.  Bit 31
+	// of a staged source marks an odd run (positions are below 2^31).
+	constexpr uint32_t RP = RU / 2;             // pairs per lane per step
+	constexpr uint32_t PSTEP = HJ_WAVE * RP;    // pairs per wave per step
+	auto stage_pairs = [&](uint64_t c0, uint64_t c1, uint32_t p2) -> uint32_t {
+		const uint64_t c = c0 + threadIdx.x;
+		uint32_t np = 0, src = 0;
+		if (c < c1) {
+			const uint32_t a = run_t[(uint64_t)p2 * stride + c];
+			const uint32_t len = run_t[(uint64_t)(p2 + 1) * stride + c] - a;
+			np = (len + 1) >> 1;
+			src = (uint32_t)(cstart[c] + a) | ((len & 1u) << 31);
+		}
+		uint32_t tot;
+		const uint32_t ex = hj_block_exscan<uint32_t>(np, sred, &tot);
+		ent[threadIdx.x] = make_uint2(src, ex);
+		if (threadIdx.x == NT - 1) ent[NT] = make_uint2(0u, tot);
+		return tot;
+	};
+	// pairs f0 + u*64 + lane, clamped to fe - 1 (every lane loads); bit u of
+	// ok2: the pair's upper tuple belongs to its run
+	auto load_pairs = [&](uint32_t f0, uint32_t fe, cursor_t &cs, uint4 (&t)[RP], uint32_t &ok2) {
+		ok2 = 0;
 #pragma unroll
-		for (uint32_t u = 0; u < RU; u++)
-			c[u] = f0 + u * HJ_WAVE + lane < fe ? heads[(t[u].x >> hs) & hmask] : 0u;
+		for (uint32_t u = 0; u < RP; u++) {
+			const uint32_t f = min(f0 + u * HJ_WAVE + lane, fe - 1);
+			while (f >= cs.noff) {
+				cs.k++;
+				cs.off = cs.noff;
+				cs.src = ent[cs.k].x;
+				cs.noff = ent[cs.k + 1].y;
+			}
+			t[u] = hj_load16a4((const char *)(P + (cs.src & 0x7fffffffu) + 2 * (f - cs.off)));
+			ok2 |= (uint32_t)(!((cs.src >> 31) && f + 1 == cs.noff)) << u;
+		}
+	};
+	auto probe_pairs = [&](uint32_t f0, uint32_t fe, const uint4 (&t)[RP], uint32_t ok2) {
+		uint32_t c[2 * RP], key[2 * RP], rid[2 * RP];
+#pragma unroll
+		for (uint32_t u = 0; u < RP; u++) {
+			const bool live = f0 + u * HJ_WAVE + lane < fe;
+			key[2 * u] = t[u].x;
+			rid[2 * u] = t[u].y;
+			key[2 * u + 1] = t[u].z;
+			rid[2 * u + 1] = t[u].w;
+			c[2 * u] = live ? heads[(t[u].x >> hs) & hmask] : 0u;
+			c[2 * u + 1] = live && ((ok2 >> u) & 1u) ? heads[(t[u].z >> hs) & hmask] : 0u;
+		}
 		bool any = true;
 		while (any) {
 			any = false;
 #pragma unroll
-			for (uint32_t u = 0; u < RU; u++) {
+			for (uint32_t u = 0; u < 2 * RP; u++) {
 				if (!c[u]) continue;
 				const uint64_t e = kv[c[u] - 1];
-				if ((uint32_t)e == t[u].x) {
+				if ((uint32_t)e == key[u]) {
 					cnt++;
-					sum += (e >> 32) + t[u].y;
+					sum += (e >> 32) + rid[u];
 				}
 				c[u] = next[c[u] - 1];
 				any |= c[u] != 0;
@@ -1949,21 +1995,22 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(6))) void k_
 				__syncthreads();
 				if (m >= cap || rc >= pr.r1) break;
 			}
-			// ---- probe the tile's runs, NT chunks at a time
+			// ---- probe the tile's runs, NT chunks at a time, in pairs
 			for (uint64_t kb = sb; m && kb < se; kb += NT) {
-				const uint32_t T = stage(kb, min(se, kb + NT), p2, 0);
+				const uint32_t T = stage_pairs(kb, min(se, kb + NT), p2);
 				__syncthreads();
 				const uint32_t fb = (uint32_t)((uint64_t)T * wid / NW), fe = (uint32_t)((uint64_t)T * (wid + 1) / NW);
 				if (fb < fe) {
 					cursor_t cs = seek(fb);
-					uint2 ta[RU], tb[RU];
+					uint4 ta[RP], tb[RP];
+					uint32_t oa, ob;
 					// unconditional prefetch (indices clamp to fe - 1), see k_join2
-					load_step(fb, fe, cs, ta, true);
-					for (uint32_t f0 = fb; f0 < fe; f0 += 2 * STEP) {
-						load_step(f0 + STEP, fe, cs, tb, true);
-						probe_step(f0, fe, ta);
-						load_step(f0 + 2 * STEP, fe, cs, ta, true);
-						probe_step(f0 + STEP, fe, tb);  // lanes past fe skip
+					load_pairs(fb, fe, cs, ta, oa);
+					for (uint32_t f0 = fb; f0 < fe; f0 += 2 * PSTEP) {
+						load_pairs(f0 + PSTEP, fe, cs, tb, ob);
+						probe_pairs(f0, fe, ta, oa);
+						load_pairs(f0 + 2 * PSTEP, fe, cs, ta, oa);
+						probe_pairs(f0 + PSTEP, fe, tb, ob);  // lanes past fe skip
 					}
 				}
 				__syncthreads();
@@ -3186,19 +3233,7 @@ struct hj_gitem {
 	uint32_t join, dt;
 };
 
-// 16 bytes at a 4-byte aligned address (a run element of any layout: global
-// loads of several dwords need dword alignment only)
-typedef unsigned int hj_u32x4a __attribute__((ext_vector_type(4), aligned(4)));
-__device__ __forceinline__ uint4 hj_load16a4(const char *p) {
-	const hj_u32x4a v = __builtin_nontemporal_load((const HJ_GLOBAL hj_u32x4a *)p);
-	return make_uint4(v.x, v.y, v.z, v.w);
-}
-
-typedef unsigned int hj_u32x2a __attribute__((ext_vector_type(2), aligned(4)));
-__device__ __forceinline__ uint2 hj_load8a4(const char *p) {
-	const hj_u32x2a v = __builtin_nontemporal_load((const HJ_GLOBAL hj_u32x2a *)p);
-	return make_uint2(v.x, v.y);
-}
+// a run element of any layout (hj_load16a4 / hj_load8a4: dword alignment)
 __device__ __forceinline__ void hj_gload(uint4 &v, const char *p) { v = hj_load16a4(p); }
 __device__ __forceinline__ void hj_gload(uint2 &v, const char *p) { v = hj_load8a4(p); }
 
