@@ -83,6 +83,16 @@ __device__ __forceinline__ uint2 hj_load8a4(const char *p) {
 	const hj_u32x2a v = __builtin_nontemporal_load((const HJ_GLOBAL hj_u32x2a *)p);
 	return make_uint2(v.x, v.y);
 }
+// cached forms: lines that neighbouring runs share stay in L2 for the
+// second reader (a streaming load marks them for early eviction)
+__device__ __forceinline__ uint4 hj_load16a4c(const char *p) {
+	const hj_u32x4a v = *(const HJ_GLOBAL hj_u32x4a *)p;
+	return make_uint4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ uint2 hj_load8a4c(const char *p) {
+	const hj_u32x2a v = *(const HJ_GLOBAL hj_u32x2a *)p;
+	return make_uint2(v.x, v.y);
+}
 // Plain (cached) global loads.
 __device__ __forceinline__ uint4 hj_load(const uint4 *p) {
 	const hj_u32x4 v = *(const HJ_GLOBAL hj_u32x4 *)p;

@@ -1761,12 +1761,14 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
 					tup_t ta[RU] = {}, tb[RU] = {};
 					// loads never under a condition (indices clamp to fe - 1): a
 					// conditional prefetch makes the compiler wait at the merge
-					// for the loads it has just issued
-					load_step(fb, fe, k, ta, true);
+					// for the loads it has just issued.  Cached loads: the line
+					// two neighbouring runs share stays in L2 for the second
+					// reader (streaming loads fetched it twice)
+					load_step(fb, fe, k, ta, false);
 					for (uint32_t f0 = fb; f0 < fe; f0 += 2 * STEP) {
-						load_step(f0 + STEP, fe, k, tb, true);
+						load_step(f0 + STEP, fe, k, tb, false);
 						probe_step(f0, fe, ta);
-						load_step(f0 + 2 * STEP, fe, k, ta, true);
+						load_step(f0 + 2 * STEP, fe, k, ta, false);
 						probe_step(f0 + STEP, fe, tb);  // lanes past fe skip
 					}
 				}
@@ -1810,6 +1812,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(6))) void k_
 	__shared__ uint32_t sred[NW + 1];
 	__shared__ uint64_t s_item[4];
 	__shared__ uint2 ent[NT + 1];  // staged run k: {first tuple, flat offset}; ent[NT].y = total
+	__shared__ uint32_t pend[NT];  // staged probe run k: its end (tuple index)
 	const uint32_t nb = hj_np2u(cap), hmask = nb - 1;
 	uint64_t *kv = lds_u64;
 	uint32_t *heads = (uint32_t *)(kv + cap);
@@ -1871,46 +1874,66 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(6))) void k_
 			t[u] = stream ? hj_load_stream8(src8) : hj_load8(src8);
 		}
 	};
-	// The probe side is read in pairs: a staged run of L tuples is
-	// ceil(L/2) pairs, each one 16-byte load of the run's tuples 2j and 2j+1
-	// (half the load instructions of 8-byte loads); the last pair of an odd
-	// run holds the next run's first tuple in its upper half, masked.  Bit 31
-	// of a staged source marks an odd run (positions are below 2^31).
+	// The probe side is read in aligned pairs: the 16 bytes at tuple index
+	// 2j hold tuples 2j and 2j+1, and a staged run [s, e) covers pairs s/2 to
+	// (e-1)/2 (half the load instructions of 8-byte loads, none crossing a
+	// 16-byte boundary); a pair's half outside the run (the lower one of the
+	// first pair when s is odd, the upper one of the last when e is odd)
+	// belongs to another run and is masked.  A staged run keeps its start s
+	// in ent[k].x and its end e in pend[k].
 	constexpr uint32_t RP = RU / 2;             // pairs per lane per step
 	constexpr uint32_t PSTEP = HJ_WAVE * RP;    // pairs per wave per step
 	auto stage_pairs = [&](uint64_t c0, uint64_t c1, uint32_t p2) -> uint32_t {
 		const uint64_t c = c0 + threadIdx.x;
-		uint32_t np = 0, src = 0;
+		uint32_t np = 0, s0 = 0, e = 0;
 		if (c < c1) {
 			const uint32_t a = run_t[(uint64_t)p2 * stride + c];
 			const uint32_t len = run_t[(uint64_t)(p2 + 1) * stride + c] - a;
-			np = (len + 1) >> 1;
-			src = (uint32_t)(cstart[c] + a) | ((len & 1u) << 31);
+			s0 = (uint32_t)(cstart[c] + a);
+			e = s0 + len;
+			np = len ? ((e - 1) >> 1) - (s0 >> 1) + 1 : 0u;
 		}
 		uint32_t tot;
 		const uint32_t ex = hj_block_exscan<uint32_t>(np, sred, &tot);
-		ent[threadIdx.x] = make_uint2(src, ex);
+		ent[threadIdx.x] = make_uint2(s0, ex);
+		pend[threadIdx.x] = e;
 		if (threadIdx.x == NT - 1) ent[NT] = make_uint2(0u, tot);
 		return tot;
 	};
-	// pairs f0 + u*64 + lane, clamped to fe - 1 (every lane loads); bit u of
-	// ok2: the pair's upper tuple belongs to its run
-	auto load_pairs = [&](uint32_t f0, uint32_t fe, cursor_t &cs, uint4 (&t)[RP], uint32_t &ok2) {
-		ok2 = 0;
+	// a lane's pair cursor: run k (tuples [s, e)) holds flat pairs
+	// [off, noff), the first of them the pair at tuple index 2*(s/2)
+	struct pcursor_t {
+		uint32_t k, s, off, noff, e;
+	};
+	auto pseek = [&](uint32_t f) -> pcursor_t {
+		uint32_t lo = 0, hi = NT;
+		while (hi - lo > 1) {
+			const uint32_t mid = (lo + hi) >> 1;
+			if (ent[mid].y <= f) lo = mid; else hi = mid;
+		}
+		const uint2 x = ent[lo];
+		return pcursor_t{lo, x.x, x.y, ent[lo + 1].y, pend[lo]};
+	};
+	// pairs f0 + u*64 + lane, clamped to fe - 1 (every lane loads); bits 2u
+	// and 2u+1 of ok: the pair's lower and upper tuple lie in its run
+	auto load_pairs = [&](uint32_t f0, uint32_t fe, pcursor_t &cs, uint4 (&t)[RP], uint32_t &ok) {
+		ok = 0;
 #pragma unroll
 		for (uint32_t u = 0; u < RP; u++) {
 			const uint32_t f = min(f0 + u * HJ_WAVE + lane, fe - 1);
 			while (f >= cs.noff) {
 				cs.k++;
 				cs.off = cs.noff;
-				cs.src = ent[cs.k].x;
+				cs.s = ent[cs.k].x;
 				cs.noff = ent[cs.k + 1].y;
+				cs.e = pend[cs.k];
 			}
-			t[u] = hj_load16a4((const char *)(P + (cs.src & 0x7fffffffu) + 2 * (f - cs.off)));
-			ok2 |= (uint32_t)(!((cs.src >> 31) && f + 1 == cs.noff)) << u;
+			const uint32_t t2 = 2 * ((cs.s >> 1) + (f - cs.off));  // tuple index of the pair's lower half
+			t[u] = hj_load((const uint4 *)(P + t2));
+			ok |= ((uint32_t)(t2 >= cs.s) | ((uint32_t)(t2 + 1 < cs.e) << 1)) << (2 * u);
 		}
 	};
-	auto probe_pairs = [&](uint32_t f0, uint32_t fe, const uint4 (&t)[RP], uint32_t ok2) {
+	auto probe_pairs = [&](uint32_t f0, uint32_t fe, const uint4 (&t)[RP], uint32_t ok) {
 		uint32_t c[2 * RP], key[2 * RP], rid[2 * RP];
 #pragma unroll
 		for (uint32_t u = 0; u < RP; u++) {
@@ -1919,8 +1942,8 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(6))) void k_
 			rid[2 * u] = t[u].y;
 			key[2 * u + 1] = t[u].z;
 			rid[2 * u + 1] = t[u].w;
-			c[2 * u] = live ? heads[(t[u].x >> hs) & hmask] : 0u;
-			c[2 * u + 1] = live && ((ok2 >> u) & 1u) ? heads[(t[u].z >> hs) & hmask] : 0u;
+			c[2 * u] = live && ((ok >> (2 * u)) & 1u) ? heads[(t[u].x >> hs) & hmask] : 0u;
+			c[2 * u + 1] = live && ((ok >> (2 * u + 1)) & 1u) ? heads[(t[u].z >> hs) & hmask] : 0u;
 		}
 		bool any = true;
 		while (any) {
@@ -2001,7 +2024,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(6))) void k_
 				__syncthreads();
 				const uint32_t fb = (uint32_t)((uint64_t)T * wid / NW), fe = (uint32_t)((uint64_t)T * (wid + 1) / NW);
 				if (fb < fe) {
-					cursor_t cs = seek(fb);
+					pcursor_t cs = pseek(fb);
 					uint4 ta[RP], tb[RP];
 					uint32_t oa, ob;
 					// unconditional prefetch (indices clamp to fe - 1), see k_join2
@@ -3234,8 +3257,8 @@ struct hj_gitem {
 };
 
 // a run element of any layout (hj_load16a4 / hj_load8a4: dword alignment)
-__device__ __forceinline__ void hj_gload(uint4 &v, const char *p) { v = hj_load16a4(p); }
-__device__ __forceinline__ void hj_gload(uint2 &v, const char *p) { v = hj_load8a4(p); }
+__device__ __forceinline__ void hj_gload(uint4 &v, const char *p) { v = hj_load16a4c(p); }
+__device__ __forceinline__ void hj_gload(uint2 &v, const char *p) { v = hj_load8a4c(p); }
 
 // A gathered tuple: v its 16 loaded bytes, pz its tile's segment position
 // (a multiple of kGTile) | layout code.  kh = key >> bits (all keys of a
