@@ -984,9 +984,10 @@ struct hj_gtile {
 // (48 bytes: pass B holds six records in scalar registers)
 static_assert(sizeof(hj_gtile) == 48, "pass-B tile record size");
 
-// gt[] = the pass-B tiles sorted by (segment, tile index k, digit): tiles of
-// neighbouring digits at the same k read the same pass-A tiles, so
-// processing them together lets them share those lines in L2.  first = the
+// gt[] = the pass-B tiles sorted by (segment, block of dblk first digits,
+// tile index k, digit): tiles of neighbouring digits at about the same k read
+// the same pass-A tiles, so processing them together lets them share those
+// lines in L2.  first = the
 // largest i with row[i] <= tile start (row = scanned counts of its digit).
 // The tile counts and first tile ids of all 2 * F1 second-level segments
 // are read into LDS once per workgroup (dynamic LDS: 16 * F1 bytes) for the
@@ -1006,7 +1007,7 @@ __global__ void k_first_run(const hj_seg *__restrict__ segs1, const hj_seg *__re
                             const hj_stats *__restrict__ st, uint32_t b1, uint32_t tile,
                             const uint32_t *__restrict__ hist, hj_gtile *__restrict__ gt, uint32_t b2,
                             const uint64_t *__restrict__ dpre, const uint32_t *__restrict__ dtiles,
-                            hj_pair *__restrict__ pairs, uint64_t *__restrict__ tpre) {
+                            hj_pair *__restrict__ pairs, uint64_t *__restrict__ tpre, uint32_t dblk) {
 	extern __shared__ uint32_t nck[];
 	const uint32_t total = (uint32_t)st->total_chunks, F1 = 1u << b1;
 	uint32_t *cbs = nck + 2 * F1;
@@ -1030,11 +1031,19 @@ __global__ void k_first_run(const hj_seg *__restrict__ segs1, const hj_seg *__re
 		}
 		const hj_seg &g2 = segs2[s2];
 		const uint32_t seg = s2 >> b1, d1 = s2 & (F1 - 1), k = c - g2.chunk_base;
+		// pass-B order: blocks of dblk neighbouring first digits, and in a
+		// block (tile index k, digit).  The tiles of digits d and d+1 that
+		// read the same pass-A tiles (their indices differ by up to one, as
+		// the partitions' runs accumulate slightly different lengths) then
+		// lie within a few places of each other and run together on one XCD,
+		// so a line that the two runs share is found in its L2 (C3: pass B
+		// 1.23 -> 1.08 ms with blocks of 8 against all digits at each k)
 		uint32_t rank = seg ? segs2[F1].chunk_base : 0u;  // segment 1 follows all of segment 0
 		const uint32_t *n_of = nck + seg * F1;
+		const uint32_t blk = d1 & ~(dblk - 1);
 		for (uint32_t d = 0; d < F1; d++) {
 			const uint32_t n = n_of[d];
-			rank += min(n, k) + (d < d1 && n > k ? 1u : 0u);
+			rank += d < blk ? n : (d < blk + dblk ? min(n, k) + (d < d1 && n > k ? 1u : 0u) : 0u);
 		}
 		const uint64_t ts = g2.out_start + (uint64_t)k * tile;
 		const hj_seg &g1 = segs1[seg];
@@ -1417,7 +1426,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
 	// Step j of this workgroup takes ordered tile j*grid + xcd*(grid/8) + lane
 	// where workgroups are dispatched round-robin over the 8 XCDs (xcd =
 	// blockIdx % 8): at each step an XCD works on a contiguous run of the
-	// order, i.e. neighbouring digits at the same k, in its own L2.  grid is
+	// order, i.e. neighbouring digits at about the same k, in its own L2.  grid is
 	// a multiple of 8; past the end a step takes an empty tile.
 	const uint32_t lane = blockIdx.x >> 3, xslot = (blockIdx.x & 7) * (grid >> 3);
 	auto tile_at = [&](uint32_t j) -> hj_gtile {
@@ -2860,10 +2869,14 @@ static int radix_engine(hj_ctx_t *ctx, const hj_relation_t *R, const hj_relation
 			hipLaunchKernelGGL(k_segs2, dim3(1), dim3(1024), (size_t)16 * F1, ctx->stream, segs1, gpart, F1, GR, GS,
 			                   ch2, (const uint4 *)ctx->part[0].p, wv, b2, cpt, segs2, pb1, dpre, dtiles, st, hist32,
 			                   hist1_len);
+			// pass-B order's digit blocks: part_variant bits 22-25 = v > 0
+			// take 2^(v-1) digits (tuning sweeps), else 8
+			const uint32_t pb_dv = (variant >> 22) & 15;
+			const uint32_t pb_dblk = pb_dv ? 1u << (pb_dv - 1) : 8u;
 			allow_lds(ctx, (const void *)k_first_run);
 			hipLaunchKernelGGL(k_first_run, dim3((unsigned)std::min<uint64_t>(4096, (chunks2_max + 255) / 256)),
 			                   dim3(256), (size_t)16 * F1, ctx->stream, segs1, segs2, st, b1, (uint32_t)ch2, hist32,
-			                   gt, b2, dpre, dtiles, pairs, tiles);
+			                   gt, b2, dpre, dtiles, pairs, tiles, std::min(pb_dblk, F1));
 			hj_kt_end(ctx);
 			// pass B: each first-level partition's tiles gathered from the
 			// pass-A runs and sorted by the second digit

@@ -74,7 +74,9 @@ typedef struct {
 	                            bits 18-19 = 1: second-pass tiles of 4096
 	                            16-byte tuples (0: 2048); bits 20-21: probe
 	                            tuples per join work item (0: default, 1:
-	                            half, 2: a quarter, 3: double)               */
+	                            half, 2: a quarter, 3: double); bits 22-25
+	                            = v > 0: second-pass tiles ordered in
+	                            blocks of 2^(v-1) first digits (0: 8)       */
 	uint32_t radix_bits_pass1; /* first-pass bits of two passes
 	                            (NUM_RADIX_BITS_PASS1, phj_asymm_radix.h:10-22);
 	                            0 = tuned / half of radix_bits                */
