@@ -5,9 +5,12 @@
 //           next_pow_2(|R|) slots, identity hash key & mask.  The reference
 //           claims a slot with a CAS on a separate hashkey word and then
 //           stores the 16-byte tuple (32-byte entries); here a slot is the
-//           16-byte {key, payload} pair and the claim is one 64-bit
-//           atomicCAS of the key word against an empty marker (all ones).
-//           R tuples whose key equals the marker are counted on the side.
+//           8-byte {u32 key, u32 payload} pair, claimed whole by one 64-bit
+//           atomicCAS against an empty marker (all ones), when every R key
+//           is below 2^32 - 1 and every payload fits 32 bits; otherwise the
+//           build flags the relation and a 16-byte {key, payload} table is
+//           built instead (claim: CAS of the key word), where R tuples whose
+//           key equals the marker are counted on the side.
 //           Probes stop at the first match, as lp_lookup_with_pos does
 //           (:1162-1165); a miss adds nothing (the reference dereferences
 //           NULL there, :1253).  Duplicate build keys keep one slot per key
@@ -234,20 +237,149 @@ __global__ __launch_bounds__(NPHJ_THREADS) void k_lp_probe(const uint4 *__restri
 	hj_block_add2(acc.cnt, acc.sum, red, &st->matched, &st->checksum);
 }
 
-// Build (and, if duplicates were seen, fix up) the LP table for R.
-// `tab` has `slots` 16-byte slots (slots >= 2 when mix).
+// ---- narrow LP table: one 8-byte slot {u32 key, u32 payload} per entry,
+// claimed whole by one 64-bit CAS against the empty marker (all ones).  It
+// takes every R whose keys are below 2^32 - 1 and whose payloads fit 32 bits
+// (the build flags HJ_DF_WIDE on any other tuple and the host rebuilds the
+// 16-byte table): half the bytes, so C2's 2^24 slots (128 MB) stay in the
+// 256 MB Infinity Cache while S streams past, and a probe reads 8 bytes.
+#define LPN_EMPTY 0xFFFFFFFFFFFFFFFFULL
+__device__ __forceinline__ bool lpn_fits(uint64_t key, uint64_t pay) { return key < 0xFFFFFFFFULL && pay <= 0xFFFFFFFFULL; }
+
+template <int WV, bool MIX>
+__global__ __launch_bounds__(NPHJ_THREADS) void k_lpn_build(const uint4 *__restrict__ R, uint64_t n,
+                                                            unsigned long long *__restrict__ tab, uint64_t mask,
+                                                            uint32_t shift, hj_stats *st) {
+	const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+	bool dup = false, full = false, wide = false;
+	for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+		const uint4 t = R[i * WV];
+		const uint64_t key = hj_key_of(t), pay = hj_pay_of(t);
+		if (!lpn_fits(key, pay)) {
+			wide = true;
+			continue;
+		}
+		const unsigned long long v = key | pay << 32;
+		uint64_t pos = lp_home<MIX>(key, mask, shift);
+		uint64_t step = 0;
+		for (; step <= mask; step++) {
+			const unsigned long long old = atomicCAS(&tab[pos], LPN_EMPTY, v);
+			if (old == LPN_EMPTY) break;
+			if ((uint32_t)old == (uint32_t)key) {
+				dup = true;
+				break;
+			}
+			pos = (pos + 1) & mask;
+		}
+		if (step > mask) full = true;
+	}
+	const bool lane0 = (threadIdx.x & (HJ_WAVE - 1)) == 0;
+	if (__any(dup) && lane0) atomicOr(&st->flags, HJ_DF_DUPKEY);
+	if (__any(full) && lane0) atomicOr(&st->flags, HJ_DF_TABLE_FULL);
+	if (__any(wide) && lane0) atomicOr(&st->flags, HJ_DF_WIDE);
+}
+
+// slot holding `key` (it must be present), narrow table
+template <bool MIX>
+__device__ __forceinline__ uint64_t lpn_find(const unsigned long long *tab, uint64_t mask, uint32_t shift, uint64_t key) {
+	uint64_t pos = lp_home<MIX>(key, mask, shift);
+	for (uint64_t step = 0; step <= mask; step++) {
+		if ((uint32_t)tab[pos] == (uint32_t)key) return pos;
+		pos = (pos + 1) & mask;
+	}
+	return ~0ULL;
+}
+
+template <int WV, bool MIX>
+__global__ __launch_bounds__(NPHJ_THREADS) void k_lpn_minidx(const uint4 *__restrict__ R, uint64_t n,
+                                                             const unsigned long long *__restrict__ tab, uint64_t mask,
+                                                             uint32_t shift, unsigned long long *__restrict__ minidx) {
+	const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+	for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+		const uint64_t pos = lpn_find<MIX>(tab, mask, shift, hj_key_of(R[i * WV]));
+		if (pos != ~0ULL) atomicMin(&minidx[pos], (unsigned long long)i);
+	}
+}
+
+template <int WV, bool MIX>
+__global__ __launch_bounds__(NPHJ_THREADS) void k_lpn_setpay(const uint4 *__restrict__ R, uint64_t n,
+                                                             unsigned long long *__restrict__ tab, uint64_t mask,
+                                                             uint32_t shift, const unsigned long long *__restrict__ minidx) {
+	const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+	for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+		const uint4 t = R[i * WV];
+		const uint64_t key = hj_key_of(t);
+		const uint64_t pos = lpn_find<MIX>(tab, mask, shift, key);
+		if (pos != ~0ULL && minidx[pos] == i) tab[pos] = key | hj_pay_of(t) << 32;
+	}
+}
+
+// Probe of the narrow table: every lane issues its PROBE_U first-slot loads
+// together (clamped home slots, no lane condition), then walks on.  A probe
+// key of 2^32 - 1 or more matches nothing (no such R key is in the table).
+template <int WV, bool EMIT, bool MIX>
+__global__ __launch_bounds__(NPHJ_THREADS) void k_lpn_probe(const uint4 *__restrict__ S, uint64_t n,
+                                                            const unsigned long long *__restrict__ tab, uint64_t mask,
+                                                            uint32_t shift, hj_stats *st, uint4 *pout,
+                                                            unsigned long long pcap, unsigned pchunk) {
+	__shared__ unsigned long long red[2 * (NPHJ_THREADS / HJ_WAVE)];
+	__shared__ unsigned long long emit_ws[EMIT ? 2 * (NPHJ_THREADS / HJ_WAVE) : 2];
+	auto acc = hj_make_sink<EMIT>(pout, pcap, &st->pair_next, pchunk, emit_ws);
+	const uint64_t tile = (uint64_t)blockDim.x * PROBE_U;
+	for (uint64_t base = (uint64_t)blockIdx.x * tile; base < n; base += (uint64_t)gridDim.x * tile) {
+		uint64_t key[PROBE_U], pay[PROBE_U], pos[PROBE_U];
+		unsigned long long e[PROBE_U];
+#pragma unroll
+		for (int u = 0; u < PROBE_U; u++) {
+			const uint64_t i = min(base + (uint64_t)u * blockDim.x + threadIdx.x, n - 1);
+			const uint4 t = hj_load_stream(&S[i * WV]);
+			key[u] = hj_key_of(t);
+			pay[u] = hj_pay_of(t);
+		}
+#pragma unroll
+		for (int u = 0; u < PROBE_U; u++) {
+			pos[u] = lp_home<MIX>(key[u], mask, shift);
+			e[u] = tab[pos[u]];
+		}
+#pragma unroll
+		for (int u = 0; u < PROBE_U; u++) {
+			if (base + (uint64_t)u * blockDim.x + threadIdx.x >= n || key[u] >= 0xFFFFFFFFULL) continue;
+			for (uint64_t step = 0; step <= mask; step++) {
+				if ((uint32_t)e[u] == (uint32_t)key[u]) {
+					acc.match(e[u] >> 32, pay[u]);
+					break;
+				}
+				if (e[u] == LPN_EMPTY) break;
+				pos[u] = (pos[u] + 1) & mask;
+				e[u] = tab[pos[u]];
+			}
+		}
+	}
+	acc.finish();
+	hj_block_add2(acc.cnt, acc.sum, red, &st->matched, &st->checksum);
+}
+
+// Build (and, if duplicates were seen, fix up) the LP table for R: the
+// narrow table (8-byte slots) or the 16-byte one; `tab` has room for `slots`
+// 16-byte slots (slots >= 2 when mix).  *wide_out: the narrow build met a
+// tuple it cannot hold (the caller rebuilds the 16-byte table).
 static int lp_build_table(hj_ctx_t *ctx, const hj_relation_t *R, unsigned long long *tab, uint64_t slots, bool mix,
-                          hj_result_t *out) {
+                          bool narrow, bool *wide_out, hj_result_t *out) {
 	const uint64_t mask = slots - 1;
 	const uint32_t shift = 64 - hj_log2_ceil(slots);
 	const uint32_t wv = R->tuple_size / 16;
 	hj_stats *st = hj_stats_dev(ctx);
 	const uint64_t n = R->tuple_num;
 	const unsigned g = grid_for(ctx, n, NPHJ_THREADS);
+	*wide_out = false;
 	hj_kt_begin(ctx, HJ_K_LP_BUILD);
-#define LP_B1(W, M)                                                                                              \
-	hipLaunchKernelGGL((k_lp_build<W, M>), dim3(g), dim3(NPHJ_THREADS), 0, ctx->stream, (const uint4 *)R->tuples, n, \
-	                   tab, mask, shift, st)
+#define LP_B1(W, M)                                                                                               \
+	if (narrow)                                                                                                   \
+		hipLaunchKernelGGL((k_lpn_build<W, M>), dim3(g), dim3(NPHJ_THREADS), 0, ctx->stream, (const uint4 *)R->tuples, \
+		                   n, tab, mask, shift, st);                                                              \
+	else                                                                                                          \
+		hipLaunchKernelGGL((k_lp_build<W, M>), dim3(g), dim3(NPHJ_THREADS), 0, ctx->stream, (const uint4 *)R->tuples,  \
+		                   n, tab, mask, shift, st)
 #define LP_B(W)          \
 	if (mix) {           \
 		LP_B1(W, true);  \
@@ -271,6 +403,10 @@ static int lp_build_table(hj_ctx_t *ctx, const hj_relation_t *R, unsigned long l
 	unsigned int flags = 0;
 	HJ_HIP(ctx, hipMemcpyAsync(&flags, &st->flags, sizeof flags, hipMemcpyDeviceToHost, ctx->stream));
 	HJ_HIP(ctx, hipStreamSynchronize(ctx->stream));
+	if (narrow && (flags & HJ_DF_WIDE)) {
+		*wide_out = true;
+		return HJ_OK;
+	}
 	if (flags & HJ_DF_TABLE_FULL) return hj_fail(ctx, HJ_EINVAL, "nphj_lp table full: |R| exceeds slots");
 	if (flags & HJ_DF_DUPKEY) {
 		void *aux = nullptr;
@@ -278,11 +414,18 @@ static int lp_build_table(hj_ctx_t *ctx, const hj_relation_t *R, unsigned long l
 		if (rc) return rc;
 		hj_fill16(ctx, aux, (slots + 1) / 2, make_uint4(~0u, ~0u, ~0u, ~0u));
 		hj_kt_begin(ctx, HJ_K_LP_FIXUP);
-#define LP_F1(W, M)                                                                                              \
-	hipLaunchKernelGGL((k_lp_minidx<W, M>), dim3(g), dim3(NPHJ_THREADS), 0, ctx->stream, (const uint4 *)R->tuples, n, \
-	                   tab, mask, shift, (unsigned long long *)aux);                                               \
-	hipLaunchKernelGGL((k_lp_setpay<W, M>), dim3(g), dim3(NPHJ_THREADS), 0, ctx->stream, (const uint4 *)R->tuples, n, \
-	                   tab, mask, shift, (const unsigned long long *)aux)
+#define LP_F1(W, M)                                                                                               \
+	if (narrow) {                                                                                                 \
+		hipLaunchKernelGGL((k_lpn_minidx<W, M>), dim3(g), dim3(NPHJ_THREADS), 0, ctx->stream, (const uint4 *)R->tuples, \
+		                   n, tab, mask, shift, (unsigned long long *)aux);                                       \
+		hipLaunchKernelGGL((k_lpn_setpay<W, M>), dim3(g), dim3(NPHJ_THREADS), 0, ctx->stream, (const uint4 *)R->tuples, \
+		                   n, tab, mask, shift, (const unsigned long long *)aux);                                 \
+	} else {                                                                                                      \
+		hipLaunchKernelGGL((k_lp_minidx<W, M>), dim3(g), dim3(NPHJ_THREADS), 0, ctx->stream, (const uint4 *)R->tuples,  \
+		                   n, tab, mask, shift, (unsigned long long *)aux);                                       \
+		hipLaunchKernelGGL((k_lp_setpay<W, M>), dim3(g), dim3(NPHJ_THREADS), 0, ctx->stream, (const uint4 *)R->tuples,  \
+		                   n, tab, mask, shift, (const unsigned long long *)aux);                                 \
+	}
 #define LP_F(W)          \
 	if (mix) {           \
 		LP_F1(W, true);  \
@@ -317,21 +460,38 @@ static int lp_run(hj_ctx_t *ctx, const hj_relation_t *R, const hj_relation_t *S,
 	HJ_HIP(ctx, hipEventRecord(ctx->ev[0], ctx->stream));
 	hj_stats_reset(ctx);
 	hj_kt_begin(ctx, HJ_K_FILL);
-	hj_fill16(ctx, tab, slots, make_uint4(~0u, ~0u, 0u, 0u));
+	hj_fill16(ctx, tab, (slots + 1) / 2, make_uint4(~0u, ~0u, ~0u, ~0u));  // narrow: 8-byte empty slots
 	hj_kt_end(ctx);
 	HJ_HIP(ctx, hipEventRecord(ctx->ev[1], ctx->stream));
-	rc = lp_build_table(ctx, R, tab, slots, mix, out);
+	bool narrow = true, wide = false;
+	rc = lp_build_table(ctx, R, tab, slots, mix, true, &wide, out);
 	if (rc) return rc;
+	if (wide) {
+		// a key of 2^32 - 1 or more, or a payload above 32 bits: the 16-byte
+		// table (the counters start again; its fill is build time here)
+		narrow = false;
+		hj_stats_reset(ctx);
+		hj_fill16(ctx, tab, slots, make_uint4(~0u, ~0u, 0u, 0u));
+		rc = lp_build_table(ctx, R, tab, slots, mix, false, &wide, out);
+		if (rc) return rc;
+	}
 	HJ_HIP(ctx, hipEventRecord(ctx->ev[2], ctx->stream));
 
 	const uint32_t wv = S->tuple_size / 16;
 	const uint64_t n = S->tuple_num;
 	const unsigned g = grid_for(ctx, (n + PROBE_U - 1) / PROBE_U, NPHJ_THREADS, 8);
 	hj_kt_begin(ctx, HJ_K_LP_PROBE);
-#define LP_P1(W, E, M)                                                                                           \
-	hipLaunchKernelGGL((k_lp_probe<W, E, M>), dim3(g), dim3(NPHJ_THREADS), 0, ctx->stream, (const uint4 *)S->tuples, \
-	                   n, (const ulonglong2 *)tab, slots - 1, 64 - hj_log2_ceil(slots), (const uint4 *)R->tuples, st, \
-	                   ctx->emit_pairs, (unsigned long long)ctx->emit_cap, ctx->emit_chunk)
+#define LP_P1(W, E, M)                                                                                            \
+	if (narrow)                                                                                                   \
+		hipLaunchKernelGGL((k_lpn_probe<W, E, M>), dim3(g), dim3(NPHJ_THREADS), 0, ctx->stream,                     \
+		                   (const uint4 *)S->tuples, n, (const unsigned long long *)tab, slots - 1,               \
+		                   64 - hj_log2_ceil(slots), st, ctx->emit_pairs, (unsigned long long)ctx->emit_cap,      \
+		                   ctx->emit_chunk);                                                                      \
+	else                                                                                                          \
+		hipLaunchKernelGGL((k_lp_probe<W, E, M>), dim3(g), dim3(NPHJ_THREADS), 0, ctx->stream,                      \
+		                   (const uint4 *)S->tuples, n, (const ulonglong2 *)tab, slots - 1, 64 - hj_log2_ceil(slots), \
+		                   (const uint4 *)R->tuples, st, ctx->emit_pairs, (unsigned long long)ctx->emit_cap,      \
+		                   ctx->emit_chunk)
 #define LP_P(W, E)          \
 	if (mix) {              \
 		LP_P1(W, E, true);  \
