@@ -142,6 +142,34 @@ def test_extreme_keys(ctx, algo):
     ctx.free_relation(dS)
 
 
+@pytest.mark.parametrize("case", ["narrow", "key_ffffffff", "pay_2_32", "probe_wide_keys"])
+def test_lp_narrow_table_boundaries(ctx, case):
+    """nphj_lp keeps an 8-byte {key, payload} table while every build key is
+    below 2^32 - 1 and every payload fits 32 bits, and rebuilds the 16-byte
+    table when one tuple does not: a single such tuple among many, keys at
+    the boundary, duplicates (the first build row wins), probe keys above
+    32 bits against a narrow table."""
+    rng = np.random.default_rng(17)
+    rk = list(rng.integers(1, 40000, 30000, dtype=np.uint64)) + [0xFFFFFFFE, 0xFFFFFFFE]
+    rp = list(rng.integers(0, 1 << 32, 30000, dtype=np.uint64)) + [7, 8]
+    if case == "key_ffffffff":
+        rk[123] = 0xFFFFFFFF
+    elif case == "pay_2_32":
+        rp[456] = 1 << 32
+    sk = list(rng.choice(np.array(rk, dtype=np.uint64), 60000)) + [0xFFFFFFFE, 0xFFFFFFFF]
+    if case == "probe_wide_keys":
+        sk += [(1 << 32) + int(k) for k in rk[:500]] + [(1 << 64) - 1]
+    sp = list(rng.integers(0, 1 << 63, len(sk), dtype=np.uint64))
+    R, S = _tuples(rk, rp), _tuples(sk, sp)
+    want = O.aggregate(R, S, 16, lp=True)
+    dR, dS = _dev(ctx, R, 16), _dev(ctx, S, 16)
+    got = ctx.join("nphj_lp", dR, dS)
+    _check(got, want, case)
+    assert got["fallback"] & hjmi.HJ_FB_LP_DUPLICATES
+    ctx.free_relation(dR)
+    ctx.free_relation(dS)
+
+
 @pytest.mark.parametrize("algo", ALGOS)
 def test_empty_and_tiny_relations(ctx, algo):
     one = _tuples([9], [1])
