@@ -83,9 +83,10 @@ def kernel_bytes(name, W, nr, ns, res=None):
         "radix_tilesort": a_in + a_out,
         "radix_gathersort": a_out + b_out,
         "radix_join": join_in,
-        "lp_build": (head + 16) * nr,
+        # nphj_lp's narrow table (res["narrow"]): 8-byte entries
+        "lp_build": (head + (8 if res is not None and res.get("narrow") else 16)) * nr,
         "sc_build": (head + 16) * nr,
-        "lp_probe": (head + 16) * ns,
+        "lp_probe": (head + (8 if res is not None and res.get("narrow") else 16)) * ns,
         "sc_probe": (head + 16) * ns,
         # heads of wider tuples (radix and late materialisation): one burst
         # read and a 16-byte write per tuple; per pair the pair read and two

@@ -525,7 +525,9 @@ static int lp_run(hj_ctx_t *ctx, const hj_relation_t *R, const hj_relation_t *S,
 	out->buildpart_time = hj_event_seconds(ctx, 1, 2);
 	out->probejoin_time = hj_event_seconds(ctx, 2, 3);
 	out->total_time = hj_event_seconds(ctx, 0, 3);
-	out->algo_bytes = (uint64_t)(head_bytes(R->tuple_size) + 16) * (R->tuple_num + S->tuple_num);
+	// a table entry per tuple: 8 bytes in the narrow table, else 16
+	out->narrow = narrow;
+	out->algo_bytes = (uint64_t)(head_bytes(R->tuple_size) + (narrow ? 8 : 16)) * (R->tuple_num + S->tuple_num);
 	return HJ_OK;
 }
 

@@ -117,7 +117,8 @@ typedef struct {
 	uint64_t narrow_tuples;  /* two-pass radix, 16-byte tuples: tuples that the
 	                            first pass stored as 8-byte {key, row_id}     */
 	uint32_t narrow;         /* 1: the second pass and the join moved 8-byte
-	                            tuples (every key and row id fit 32 bits)     */
+	                            tuples (every key and row id fit 32 bits);
+	                            nphj_lp: its table held 8-byte entries       */
 	uint32_t reserved;
 	uint64_t positional_tuples; /* of those, tuples of first-pass tiles whose
 	                            row ids are their relation positions + 1,
