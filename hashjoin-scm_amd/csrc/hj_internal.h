@@ -54,6 +54,7 @@ struct hj_ctx {
 	hj_buf meta;       // segments, chunk maps, bounds, work lists
 	hj_buf scan_tmp;   // scan partials
 	hj_buf stats;      // device counters and results
+	void *hstats = nullptr;  // pinned host copy of the counters (hj_stats_read)
 	hj_buf hostcopy[2];// device copies of host-resident relations
 
 	// late materialisation: while set, joins on 16-byte tuples also append

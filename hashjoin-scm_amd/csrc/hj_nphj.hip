@@ -94,8 +94,9 @@ int hj_stats_reset(hj_ctx_t *ctx) {
 }
 
 int hj_stats_read(hj_ctx_t *ctx, hj_stats *h) {
-	HJ_HIP(ctx, hipMemcpyAsync(h, hj_stats_dev(ctx), sizeof *h, hipMemcpyDeviceToHost, ctx->stream));
+	HJ_HIP(ctx, hipMemcpyAsync(ctx->hstats, hj_stats_dev(ctx), sizeof *h, hipMemcpyDeviceToHost, ctx->stream));
 	HJ_HIP(ctx, hipStreamSynchronize(ctx->stream));
+	*h = *(const hj_stats *)ctx->hstats;
 	return HJ_OK;
 }
 

@@ -144,6 +144,14 @@ int hj_ctx_create(int device, hj_ctx_t **out) {
 			return HJ_EHIP;
 		}
 	}
+	// pinned, so the end-of-join counter read is one DMA into host memory
+	// (a pageable destination is staged through the runtime's bounce buffer)
+	if (hipHostMalloc(&ctx->hstats, sizeof(hj_stats), hipHostMallocDefault) != hipSuccess) {
+		(void)hipGetLastError();
+		ctx->hstats = nullptr;
+		hj_ctx_destroy(ctx);
+		return HJ_ENOMEM;
+	}
 	if (hj_reserve(ctx, ctx->stats, sizeof(hj_stats)) != HJ_OK) {
 		hj_ctx_destroy(ctx);
 		return HJ_ENOMEM;
@@ -167,6 +175,7 @@ int hj_ctx_destroy(hj_ctx_t *ctx) {
 	for (int i = 0; i < hj_ctx::NEV; i++)
 		if (ctx->ev[i]) (void)hipEventDestroy(ctx->ev[i]);
 	if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+	if (ctx->hstats) (void)hipHostFree(ctx->hstats);
 	delete ctx;
 	return HJ_OK;
 }
