@@ -1654,20 +1654,17 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
 	// forward.  Every lane loads RU tuples, those past fe the tuple at fe - 1
 	// (the callers skip them): a load under a per-lane condition must be
 	// merged with the register's old value, and for narrow pairs that merge
-	// waits for the data, leaving one load in flight instead of RU.
-	auto load_step = [&](uint32_t f0, uint32_t fe, uint32_t &k, tup_t (&t)[RU], bool stream) {
+	// waits for the data, leaving one load in flight instead of RU.  Cached
+	// loads: the line two neighbouring runs share stays in L2 for the second
+	// reader (streaming loads fetched it twice).
+	auto load_step = [&](uint32_t f0, uint32_t fe, uint32_t &k, tup_t (&t)[RU]) {
 #pragma unroll
 		for (uint32_t u = 0; u < RU; u++) {
 			const uint32_t f = min(f0 + u * HJ_WAVE + lane, fe - 1);
 			while (r_off[k + 1] <= f) k++;
 			const uint64_t at = r_src[k] + (f - r_off[k]);
-			if constexpr (NARROW) {
-				const uint2 *src8 = (const uint2 *)P + at;
-				t[u] = stream ? hj_load_stream8(src8) : *src8;
-			} else {
-				const uint4 *src = &P[at * WV];
-				t[u] = stream ? hj_load_stream(src) : *src;
-			}
+			if constexpr (NARROW) t[u] = *((const uint2 *)P + at);
+			else t[u] = P[at * WV];
 		}
 	};
 	auto probe_step = [&](uint32_t f0, uint32_t fe, const tup_t (&t)[RU]) {
@@ -1720,7 +1717,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
 					uint32_t k = find_run(fb);
 					for (uint32_t f0 = fb; f0 < fe; f0 += STEP) {
 						tup_t t[RU];
-						load_step(f0, fe, k, t, false);
+						load_step(f0, fe, k, t);
 #pragma unroll
 						for (uint32_t u = 0; u < RU; u++) {
 							const uint32_t f = f0 + u * HJ_WAVE + lane;
@@ -1770,14 +1767,12 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
 					tup_t ta[RU] = {}, tb[RU] = {};
 					// loads never under a condition (indices clamp to fe - 1): a
 					// conditional prefetch makes the compiler wait at the merge
-					// for the loads it has just issued.  Cached loads: the line
-					// two neighbouring runs share stays in L2 for the second
-					// reader (streaming loads fetched it twice)
-					load_step(fb, fe, k, ta, false);
+					// for the loads it has just issued
+					load_step(fb, fe, k, ta);
 					for (uint32_t f0 = fb; f0 < fe; f0 += 2 * STEP) {
-						load_step(f0 + STEP, fe, k, tb, false);
+						load_step(f0 + STEP, fe, k, tb);
 						probe_step(f0, fe, ta);
-						load_step(f0 + 2 * STEP, fe, k, ta, false);
+						load_step(f0 + 2 * STEP, fe, k, ta);
 						probe_step(f0 + STEP, fe, tb);  // lanes past fe skip
 					}
 				}
@@ -1869,7 +1864,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(6))) void k_
 	};
 	// Load flat indices f0 + u*64 + lane (clamped to fe - 1: every lane issues
 	// RU loads, none under a lane condition); the cursor only moves forward.
-	auto load_step = [&](uint32_t f0, uint32_t fe, cursor_t &cs, uint2 (&t)[RU], bool stream) {
+	auto load_step = [&](uint32_t f0, uint32_t fe, cursor_t &cs, uint2 (&t)[RU]) {
 #pragma unroll
 		for (uint32_t u = 0; u < RU; u++) {
 			const uint32_t f = min(f0 + u * HJ_WAVE + lane, fe - 1);
@@ -1879,8 +1874,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(6))) void k_
 				cs.src = ent[cs.k].x;
 				cs.noff = ent[cs.k + 1].y;
 			}
-			const uint2 *src8 = P + cs.src + (f - cs.off);
-			t[u] = stream ? hj_load_stream8(src8) : hj_load8(src8);
+			t[u] = hj_load8(P + cs.src + (f - cs.off));
 		}
 	};
 	// The probe side is read in aligned pairs: the 16 bytes at tuple index
@@ -2010,7 +2004,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(6))) void k_
 					cursor_t cs = seek(fb);
 					for (uint32_t f0 = fb; f0 < fe; f0 += STEP) {
 						uint2 t[RU];
-						load_step(f0, fe, cs, t, false);
+						load_step(f0, fe, cs, t);
 #pragma unroll
 						for (uint32_t u = 0; u < RU; u++) {
 							const uint32_t f = f0 + u * HJ_WAVE + lane;
