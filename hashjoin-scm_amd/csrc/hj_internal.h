@@ -120,6 +120,10 @@ struct hj_ctx {
 	bool kernel_times = false;
 	int n_kt = 0;
 	uint32_t kt_id[HJ_MAX_KTIMES] = {};
+	// region i runs from event kt_s[i] to event kt_e[i] (slots of ev[]); a
+	// boundary between two regions is one event
+	int kt_s[HJ_MAX_KTIMES] = {}, kt_e[HJ_MAX_KTIMES] = {};
+	bool kt_open = false;
 };
 
 // Set ctx->last_error and return `code`.
@@ -142,9 +146,17 @@ int hj_noalloc_setup(hj_ctx_t *ctx);
 			               hipGetErrorString(e_), __FILE__, __LINE__);                       \
 	} while (0)
 
-// Per-kernel timing: bracket a launch with kt_begin/kt_end.
+// Per-kernel timing: bracket a launch with kt_begin/kt_end.  kt_next ends
+// the open region and opens the next one with a single event (an event
+// record costs a few microseconds of device time); kt_open_at / kt_close_at
+// take an event of ev[] the caller has just recorded as the region's start
+// or end.  kt_ms(i): region i's device milliseconds.
 void hj_kt_begin(hj_ctx_t *ctx, uint32_t kid);
 void hj_kt_end(hj_ctx_t *ctx);
+void hj_kt_next(hj_ctx_t *ctx, uint32_t kid);
+void hj_kt_open_at(hj_ctx_t *ctx, uint32_t kid, int ev_index);
+void hj_kt_close_at(hj_ctx_t *ctx, int ev_index);
+double hj_kt_ms(hj_ctx_t *ctx, int i);
 
 // Device counters block (lives in ctx->stats, or at the start of the fixed
 // region during a no-allocation join: see hj_stats_dev).

@@ -104,12 +104,13 @@ extern "C" int hj_join_noalloc(hj_ctx_t *ctx, const char *algo, const hj_relatio
 	ctx->fixed_off = 256;  // the counters (hj_stats_dev)
 	ctx->kernel_times = (prm.flags & HJ_F_KERNEL_TIMES) != 0;
 	ctx->n_kt = 0;
+	ctx->kt_open = false;
 	const int rc = fn(ctx, R, S, &prm, out);
 	if (rc == HJ_OK) {
 		out->n_ktimes = (uint32_t)ctx->n_kt;
 		for (int i = 0; i < ctx->n_kt; i++) {
 			out->ktime_id[i] = ctx->kt_id[i];
-			out->ktime_ms[i] = hj_event_seconds(ctx, 4 + 2 * i, 4 + 2 * i + 1) * 1e3;
+			out->ktime_ms[i] = hj_kt_ms(ctx, i);
 		}
 	}
 	ctx->kernel_times = false;

@@ -2838,17 +2838,17 @@ static int radix_engine(hj_ctx_t *ctx, const hj_relation_t *R, const hj_relation
 			if (sldsA + 1024 > ctx->prop.sharedMemPerBlock || sldsB + 1024 > ctx->prop.sharedMemPerBlock)
 				return hj_fail(ctx, HJ_EINVAL, "radix bits per pass too large for LDS");
 			const unsigned gA = (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(chunks1, (uint64_t)ctx->num_cu * 4));
+			// kernel times: the two-pass regions are chained (one event
+			// between two regions), the last one closing at ev[1]
 			hj_kt_begin(ctx, HJ_K_TILESORT);
 			launch_sortchunk(ctx, wv, pcA, gA, sldsA, segs1, cmap, 0, b1, ctx->part[0].p, runsA, cstA);
-			hj_kt_end(ctx);
 			// the count matrix: group sums, their scan, positions
-			hj_kt_begin(ctx, HJ_K_SCAN);
+			hj_kt_next(ctx, HJ_K_SCAN);
 			const dim3 gg(std::max(1u, GR + GS), (F1 + 63) / 64);
 			hipLaunchKernelGGL(k_runs_part, gg, dim3(256), 0, ctx->stream, runsA, segs1, F1, GR, GS, gpart);
 			rc = scan_u32(ctx, gpart, (uint64_t)F1 * (GR + GS), (uint64_t *)ctx->scan_tmp.p);
 			hipLaunchKernelGGL(k_runs_pos, gg, dim3(256), 0, ctx->stream, runsA, segs1, F1, GR, GS, gpart, hist32,
 			                   beg_t);
-			hj_kt_end(ctx);
 			if (rc) return rc;
 		}
 		if (P == 1)
@@ -2858,7 +2858,7 @@ static int radix_engine(hj_ctx_t *ctx, const hj_relation_t *R, const hj_relation
 		Rb = Sb = ctx->part[0].p;
 		if (P == 2) {
 			// second-pass segments, pass-B tile records and the join's work list
-			hj_kt_begin(ctx, HJ_K_PLAN);
+			hj_kt_next(ctx, HJ_K_PLAN);
 			allow_lds(ctx, (const void *)k_segs2);
 			hipLaunchKernelGGL(k_segs2, dim3(1), dim3(1024), (size_t)16 * F1, ctx->stream, segs1, gpart, F1, GR, GS,
 			                   ch2, (const uint4 *)ctx->part[0].p, wv, b2, cpt, segs2, pb1, dpre, dtiles, st, hist32,
@@ -2871,7 +2871,6 @@ static int radix_engine(hj_ctx_t *ctx, const hj_relation_t *R, const hj_relation
 			hipLaunchKernelGGL(k_first_run, dim3((unsigned)std::min<uint64_t>(4096, (chunks2_max + 255) / 256)),
 			                   dim3(256), (size_t)16 * F1, ctx->stream, segs1, segs2, st, b1, (uint32_t)ch2, hist32,
 			                   gt, b2, dpre, dtiles, pairs, tiles, std::min(pb_dblk, F1));
-			hj_kt_end(ctx);
 			// pass B: each first-level partition's tiles gathered from the
 			// pass-A runs and sorted by the second digit
 			// two workgroups per CU: with three the tiles in flight on an XCD
@@ -2881,10 +2880,9 @@ static int radix_engine(hj_ctx_t *ctx, const hj_relation_t *R, const hj_relation
 			// 1: all resident, 2: one per CU
 			const uint32_t gsel = (variant >> 14) & 3;
 			const unsigned sgrid = gsel == 1 ? ~0u : (unsigned)ctx->num_cu * (gsel == 2 ? 1u : 2u);
-			hj_kt_begin(ctx, HJ_K_GATHERSORT);
+			hj_kt_next(ctx, HJ_K_GATHERSORT);
 			launch_sortchunk_g(ctx, wv, pc2, sgrid, sldsB, gt, hist32, beg_t, b1, b2, ctx->part[0].p, ctx->part[1].p, runs,
 			                   cst, (uint32_t)pcA.tile);
-			hj_kt_end(ctx);
 			hipLaunchKernelGGL(k_run_transpose, dim3((unsigned)((chunks2_max + 63) / 64), (F2 + 1 + 63) / 64),
 			                   dim3(256), 0, ctx->stream, runs, st, F2, (uint32_t)chunks2_max, run_t);
 			Rb = Sb = ctx->part[1].p;
@@ -2892,6 +2890,7 @@ static int radix_engine(hj_ctx_t *ctx, const hj_relation_t *R, const hj_relation
 	}
 	HJ_HIP(ctx, hipGetLastError());
 	HJ_HIP(ctx, hipEventRecord(ctx->ev[1], ctx->stream));
+	if (P == 2) hj_kt_close_at(ctx, 1);  // the pass-B region (and the run transpose)
 
 	// join: tile the probe side so a skewed partition spreads over workgroups
 	const uint64_t avg_s = (ns >> B) + 1;
@@ -2918,15 +2917,17 @@ static int radix_engine(hj_ctx_t *ctx, const hj_relation_t *R, const hj_relation
 		hj_kt_end(ctx);
 		if (rc) return rc;
 	}
-	hj_kt_begin(ctx, HJ_K_JOIN);
 	if (P == 2) {
+		hj_kt_open_at(ctx, HJ_K_JOIN, 1);
 		launch_join2(ctx, wv, tk, jgrid, jlds, Rb, cst, run_t, (uint32_t)chunks2_max, b1, pairs, tiles, Q, B, cpt, cap);
 	} else {
+		hj_kt_begin(ctx, HJ_K_JOIN);
 		launch_join(ctx, wv, tk, jgrid, jlds, Rb, Sb, pairs, tiles, Q, B, s_tile, cap);
+		hj_kt_end(ctx);
 	}
-	hj_kt_end(ctx);
 	HJ_HIP(ctx, hipGetLastError());
 	HJ_HIP(ctx, hipEventRecord(ctx->ev[2], ctx->stream));
+	if (P == 2) hj_kt_close_at(ctx, 2);
 	hj_stats h;
 	if ((rc = hj_stats_read(ctx, &h))) return rc;
 	out->buildpart_time = hj_event_seconds(ctx, 0, 1);
@@ -4274,6 +4275,7 @@ extern "C" int hj_batch_run(hj_batch_t *b, hj_join_result_t *results, hj_batch_i
 	if (b->timed) {
 		ctx->kernel_times = true;
 		ctx->n_kt = 0;
+		ctx->kt_open = false;
 		b->launch();
 		HJ_HIP(ctx, hipGetLastError());
 	} else {
@@ -4305,7 +4307,7 @@ extern "C" int hj_batch_run(hj_batch_t *b, hj_join_result_t *results, hj_batch_i
 		b->info.n_ktimes = (uint32_t)ctx->n_kt;
 		for (int i = 0; i < ctx->n_kt; i++) {
 			b->info.ktime_id[i] = ctx->kt_id[i];
-			b->info.ktime_ms[i] = hj_event_seconds(ctx, 4 + 2 * i, 4 + 2 * i + 1) * 1e3;
+			b->info.ktime_ms[i] = hj_kt_ms(ctx, i);
 		}
 		ctx->kernel_times = false;
 	}

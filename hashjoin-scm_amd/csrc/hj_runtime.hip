@@ -65,15 +65,47 @@ int hj_scratch(hj_ctx_t *ctx, hj_buf &b, size_t bytes, void **p) {
 
 void hj_kt_begin(hj_ctx_t *ctx, uint32_t kid) {
 	if (!ctx->kernel_times || ctx->n_kt >= HJ_MAX_KTIMES) return;
-	ctx->kt_id[ctx->n_kt] = kid;
-	(void)hipEventRecord(ctx->ev[4 + 2 * ctx->n_kt], ctx->stream);
+	const int n = ctx->n_kt;
+	ctx->kt_id[n] = kid;
+	ctx->kt_s[n] = 4 + 2 * n;
+	ctx->kt_open = true;
+	(void)hipEventRecord(ctx->ev[ctx->kt_s[n]], ctx->stream);
 }
 
 void hj_kt_end(hj_ctx_t *ctx) {
-	if (!ctx->kernel_times || ctx->n_kt >= HJ_MAX_KTIMES) return;
-	(void)hipEventRecord(ctx->ev[4 + 2 * ctx->n_kt + 1], ctx->stream);
+	if (!ctx->kernel_times || !ctx->kt_open || ctx->n_kt >= HJ_MAX_KTIMES) return;
+	const int n = ctx->n_kt;
+	ctx->kt_e[n] = 4 + 2 * n + 1;
+	(void)hipEventRecord(ctx->ev[ctx->kt_e[n]], ctx->stream);
+	ctx->kt_open = false;
 	ctx->n_kt++;
 }
+
+void hj_kt_next(hj_ctx_t *ctx, uint32_t kid) {
+	if (!ctx->kernel_times || !ctx->kt_open) return;
+	hj_kt_end(ctx);
+	const int n = ctx->n_kt;
+	if (n == 0 || n >= HJ_MAX_KTIMES) return;
+	ctx->kt_id[n] = kid;
+	ctx->kt_s[n] = ctx->kt_e[n - 1];
+	ctx->kt_open = true;
+}
+
+void hj_kt_open_at(hj_ctx_t *ctx, uint32_t kid, int ev_index) {
+	if (!ctx->kernel_times || ctx->n_kt >= HJ_MAX_KTIMES) return;
+	ctx->kt_id[ctx->n_kt] = kid;
+	ctx->kt_s[ctx->n_kt] = ev_index;
+	ctx->kt_open = true;
+}
+
+void hj_kt_close_at(hj_ctx_t *ctx, int ev_index) {
+	if (!ctx->kernel_times || !ctx->kt_open || ctx->n_kt >= HJ_MAX_KTIMES) return;
+	ctx->kt_e[ctx->n_kt] = ev_index;
+	ctx->kt_open = false;
+	ctx->n_kt++;
+}
+
+double hj_kt_ms(hj_ctx_t *ctx, int i) { return hj_event_seconds(ctx, ctx->kt_s[i], ctx->kt_e[i]) * 1e3; }
 
 double hj_event_seconds(hj_ctx_t *ctx, int a, int b) {
 	float ms = 0.f;
@@ -336,12 +368,13 @@ int hj_join(hj_ctx_t *ctx, const char *algo, const hj_relation_t *R, const hj_re
 	}
 	ctx->kernel_times = (prm.flags & HJ_F_KERNEL_TIMES) != 0;
 	ctx->n_kt = 0;
+	ctx->kt_open = false;
 	int rc = e->fn(ctx, &rel[0], &rel[1], &prm, out);
 	if (rc == HJ_OK) {
 		out->n_ktimes = (uint32_t)ctx->n_kt;
 		for (int i = 0; i < ctx->n_kt; i++) {
 			out->ktime_id[i] = ctx->kt_id[i];
-			out->ktime_ms[i] = hj_event_seconds(ctx, 4 + 2 * i, 4 + 2 * i + 1) * 1e3;
+			out->ktime_ms[i] = hj_kt_ms(ctx, i);
 		}
 	}
 	ctx->kernel_times = false;

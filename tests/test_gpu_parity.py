@@ -264,6 +264,33 @@ def test_repeated_joins_reuse_scratch(ctx):
     ctx.free_relation(S)
 
 
+@pytest.mark.parametrize("passes", [1, 2])
+def test_kernel_times_cover_the_join(ctx, passes):
+    """kernel_times=True: the same result, every phase named, and the
+    phases' device times sum to no more than the join's own (the two-pass
+    phases are chained: one event between two, the last closing with the
+    join)."""
+    R, S = ctx.generate("pkfk", 1 << 20, 1 << 24, 16, seed=31)
+    bits = 13 if passes == 2 else 10
+    want = ctx.join("phj_radix_bc", R, S, radix_bits=bits, passes=passes)
+    for _ in range(2):
+        got = ctx.join("phj_radix_bc", R, S, radix_bits=bits, passes=passes, kernel_times=True)
+        assert got["passes"] == passes
+        _check(got, want, "phj_radix_bc")
+        names = set(got["kernel_ms"])
+        need = {"radix_tilesort", "radix_scan", "radix_plan", "radix_gathersort", "radix_join"} if passes == 2 \
+            else {"radix_plan", "radix_join"}
+        assert need <= names, names
+        total = sum(sum(v) for v in got["kernel_ms"].values())
+        assert all(x >= 0 for v in got["kernel_ms"].values() for x in v)
+        assert total <= got["total_time"] * 1e3 * 1.0001 + 1e-3, (total, got["total_time"])
+        if passes == 2:
+            # the chained regions span the join, from pass A to the join's end
+            assert total >= 0.9 * got["total_time"] * 1e3, (total, got["total_time"])
+    ctx.free_relation(R)
+    ctx.free_relation(S)
+
+
 ASYMM = ["phj_asymm_radix_bc", "phj_asymm_radix_sc", "phj_asymm_radix_hr", "phj_asymm_radix_hro"]
 
 
