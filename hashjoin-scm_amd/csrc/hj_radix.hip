@@ -990,9 +990,11 @@ static_assert(sizeof(hj_gtile) == 48, "pass-B tile record size");
 // lines in L2.  first = the
 // largest i with row[i] <= tile start (row = scanned counts of its digit).
 // The tile counts and first tile ids of all 2 * F1 second-level segments
-// are read into LDS once per workgroup (dynamic LDS: 16 * F1 bytes) for the
-// rank loop and for finding a tile's segment (the last one whose first tile
-// id is at most the tile's: empty segments share the next one's).
+// are read into LDS once per workgroup for the rank and for finding a tile's
+// segment (the last one whose first tile id is at most the tile's: empty
+// segments share the next one's), with each digit block's tile count
+// prefix, so that a tile's rank sums over its own block only (dynamic LDS:
+// 24 * F1 bytes).
 //
 // The same kernel writes the join's work list, items in (first digit,
 // second digit) order: item i is final partition q = (d2 << b1) | d1 with
@@ -1011,9 +1013,19 @@ __global__ void k_first_run(const hj_seg *__restrict__ segs1, const hj_seg *__re
 	extern __shared__ uint32_t nck[];
 	const uint32_t total = (uint32_t)st->total_chunks, F1 = 1u << b1;
 	uint32_t *cbs = nck + 2 * F1;
+	// bpre[seg * nblk + j]: tiles of segment seg's digits below j * dblk
+	uint32_t *bpre = cbs + 2 * F1;
+	const uint32_t nblk = F1 / dblk;
 	for (uint32_t i = threadIdx.x; i < 2 * F1; i += blockDim.x) {
 		nck[i] = segs2[i].nchunks;
 		cbs[i] = segs2[i].chunk_base;
+	}
+	__syncthreads();
+	for (uint32_t i = threadIdx.x; i < 2 * nblk; i += blockDim.x) {
+		const uint32_t seg = i / nblk, j = i % nblk;
+		uint32_t a = 0;
+		for (uint32_t d = 0; d < j * dblk; d++) a += nck[seg * F1 + d];
+		bpre[i] = a;
 	}
 	__syncthreads();
 	const uint32_t Q = F1 << b2;
@@ -1038,12 +1050,13 @@ __global__ void k_first_run(const hj_seg *__restrict__ segs1, const hj_seg *__re
 		// lie within a few places of each other and run together on one XCD,
 		// so a line that the two runs share is found in its L2 (C3: pass B
 		// 1.23 -> 1.08 ms with blocks of 8 against all digits at each k)
-		uint32_t rank = seg ? segs2[F1].chunk_base : 0u;  // segment 1 follows all of segment 0
-		const uint32_t *n_of = nck + seg * F1;
+		// segment 1 follows all of segment 0
 		const uint32_t blk = d1 & ~(dblk - 1);
-		for (uint32_t d = 0; d < F1; d++) {
+		uint32_t rank = (seg ? cbs[F1] : 0u) + bpre[seg * nblk + blk / dblk];
+		const uint32_t *n_of = nck + seg * F1 + blk;
+		for (uint32_t d = 0; d < dblk; d++) {
 			const uint32_t n = n_of[d];
-			rank += d < blk ? n : (d < blk + dblk ? min(n, k) + (d < d1 && n > k ? 1u : 0u) : 0u);
+			rank += min(n, k) + (blk + d < d1 && n > k ? 1u : 0u);
 		}
 		const uint64_t ts = g2.out_start + (uint64_t)k * tile;
 		const hj_seg &g1 = segs1[seg];
@@ -1110,7 +1123,11 @@ __global__ void k_first_run(const hj_seg *__restrict__ segs1, const hj_seg *__re
 
 // Pass B: every tile of a first-level partition is gathered from the pass-A
 // runs that cover it, sorted by the second digit in LDS and written to `out`
-// at its place, with the run table of k_sortchunk.  Staging writes each tile
+// at its place, with its run table written digit-major: run_t[p * rstride +
+// c] = where digit p starts in tile c (p = F: the tile's length), the layout
+// the join reads (a partition's runs over consecutive tiles are consecutive
+// u16s; the tiles sharing a line are sorted at about the same time, so their
+// 2-byte writes meet in L2).  Staging writes each tile
 // position's source (offset in its segment) into LDS (two buffers); the
 // next tile's loads are issued before the current tile is sorted, and every
 // thread issues exactly ITEMS*WV of them (past the tile end they re-read its
@@ -1127,8 +1144,9 @@ template <int WV, int NT, int ITEMS, bool NIN, bool DENSE>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_sortchunk_g(
     const hj_gtile *__restrict__ gt, const uint32_t *__restrict__ hist, const uint16_t *__restrict__ beg_t,
     const hj_stats *__restrict__ st, uint32_t b1, uint32_t bits,
-    const uint4 *__restrict__ in, uint4 *__restrict__ out, uint16_t *__restrict__ run, uint64_t *__restrict__ cstart,
-    bool narrow_ok, const uint8_t *__restrict__ tnarrow, uint32_t tileA, uint32_t grid2, uint32_t pmode) {
+    const uint4 *__restrict__ in, uint4 *__restrict__ out, uint16_t *__restrict__ run_t, uint32_t rstride,
+    uint64_t *__restrict__ cstart, bool narrow_ok, const uint8_t *__restrict__ tnarrow, uint32_t tileA, uint32_t grid2,
+    uint32_t pmode) {
 	constexpr int TILE = NT * ITEMS;
 	extern __shared__ uint4 lds_u4[];
 	const uint32_t F = 1u << bits, fmask = F - 1;
@@ -1366,10 +1384,10 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
 			const uint32_t r = hj_block_exscan<uint32_t>(cnt, red, &tot);
 			if (n && threadIdx.x < F) {
 				tbase[threadIdx.x] = r;
-				run[(uint64_t)c * (F + 1) + threadIdx.x] = (uint16_t)r;
+				run_t[(uint64_t)threadIdx.x * rstride + c] = (uint16_t)r;
 			}
 			if (threadIdx.x == 0 && n) {
-				run[(uint64_t)c * (F + 1) + F] = (uint16_t)n;
+				run_t[(uint64_t)F * rstride + c] = (uint16_t)n;
 				cstart[c] = pos;
 			}
 		} else {
@@ -1383,13 +1401,13 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
 				if (p0 + q < F) {
 					if (n) {
 						tbase[p0 + q] = r;
-						run[(uint64_t)c * (F + 1) + p0 + q] = (uint16_t)r;
+						run_t[(uint64_t)(p0 + q) * rstride + c] = (uint16_t)r;
 					}
 					r += tcnt[p0 + q];
 					tcnt[p0 + q] = 0;
 				}
 			if (threadIdx.x == 0 && n) {
-				run[(uint64_t)c * (F + 1) + F] = (uint16_t)n;
+				run_t[(uint64_t)F * rstride + c] = (uint16_t)n;
 				cstart[c] = pos;
 			}
 		}
@@ -1477,26 +1495,6 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
 		tb = td;
 		tc = te;
 		td = tf;
-	}
-}
-
-// Digit-major copy of a chunk run table: run_t[p * stride + c] =
-// run[c * (F + 1) + p] for p in [0, F], so that a partition's runs over
-// consecutive chunks are read as consecutive u16s.
-__global__ __launch_bounds__(256) void k_run_transpose(const uint16_t *__restrict__ run, const hj_stats *__restrict__ st,
-                                                       uint32_t F, uint32_t stride, uint16_t *__restrict__ run_t) {
-	__shared__ uint16_t t[64][66];
-	const uint32_t total = (uint32_t)st->total_chunks;
-	const uint32_t c0 = blockIdx.x * 64, p0 = blockIdx.y * 64;
-	if (c0 >= total) return;
-	for (uint32_t e = threadIdx.x; e < 64 * 64; e += 256) {
-		const uint32_t ci = e / 64, pj = e % 64, c = c0 + ci, p = p0 + pj;
-		t[ci][pj] = (c < total && p <= F) ? run[(uint64_t)c * (F + 1) + p] : (uint16_t)0;
-	}
-	__syncthreads();
-	for (uint32_t e = threadIdx.x; e < 64 * 64; e += 256) {
-		const uint32_t pj = e / 64, ci = e % 64, c = c0 + ci, p = p0 + pj;
-		if (c < total && p <= F) run_t[(uint64_t)p * stride + c] = t[ci][pj];
 	}
 }
 
@@ -2377,7 +2375,7 @@ static void launch_sortchunk(hj_ctx_t *ctx, uint32_t wv, const part_cfg &pc, uns
 template <int WV, int NT, int ITEMS, bool DENSE, bool NIN>
 static void launch_sortchunk_g_i(hj_ctx_t *ctx, unsigned grid, size_t lds, const hj_gtile *gt, const uint32_t *hist,
                                  const uint16_t *beg_t, uint32_t b1, uint32_t bits, const void *in, void *out,
-                                 uint16_t *run, uint64_t *cst, uint32_t tileA, uint32_t pmode) {
+                                 uint16_t *run_t, uint32_t rstride, uint64_t *cst, uint32_t tileA, uint32_t pmode) {
 	// grid: all resident workgroups up to the cap `grid`, a multiple of the
 	// 8 XCDs (the kernel's mapping); DENSE instances launch half as many
 	// again, and use them only for mostly positional inputs
@@ -2389,7 +2387,7 @@ static void launch_sortchunk_g_i(hj_ctx_t *ctx, unsigned grid, size_t lds, const
 	allow_lds(ctx, (const void *)kfn);
 	grid = std::max(8u, std::min(grid, resident_grid(ctx, (const void *)kfn, NT, l)) & ~7u);
 	hipLaunchKernelGGL(kfn, dim3(grid), dim3(NT), l, ctx->stream, gt, hist, beg_t, (const hj_stats *)ctx->stats.p, b1,
-	                   bits, (const uint4 *)in, (uint4 *)out, run, cst, ctx->narrow_ok,
+	                   bits, (const uint4 *)in, (uint4 *)out, run_t, rstride, cst, ctx->narrow_ok,
 	                   (const uint8_t *)ctx->tile_narrow, tileA, std::min(grid2, grid), pmode);
 }
 
@@ -2401,8 +2399,10 @@ static void launch_sortchunk_g_i(hj_ctx_t *ctx, unsigned grid, size_t lds, const
 template <int WV, int NT, int ITEMS>
 static void launch_sortchunk_g_t(hj_ctx_t *ctx, unsigned grid, size_t lds, const hj_gtile *gt, const uint32_t *hist,
                                  const uint16_t *beg_t, uint32_t b1, uint32_t bits, const void *in, void *out,
-                                 uint16_t *run, uint64_t *cst, uint32_t tileA) {
-#define GI(D, N, M) 	launch_sortchunk_g_i<WV, NT, ITEMS, D, N>(ctx, grid, lds, gt, hist, beg_t, b1, bits, in, out, run, cst, tileA, M)
+                                 uint16_t *run_t, uint32_t rstride, uint64_t *cst, uint32_t tileA) {
+#define GI(D, N, M) \
+	launch_sortchunk_g_i<WV, NT, ITEMS, D, N>(ctx, grid, lds, gt, hist, beg_t, b1, bits, in, out, run_t, rstride, cst, \
+	                                          tileA, M)
 	if constexpr (WV == 1) {
 		if (ctx->dense_ok) {
 			GI(true, true, 1);
@@ -2423,10 +2423,11 @@ static size_t gathersort_lds(const part_cfg &c, uint32_t wv, uint32_t F) {
 
 static void launch_sortchunk_g(hj_ctx_t *ctx, uint32_t wv, const part_cfg &pc, unsigned grid, size_t lds,
                                const hj_gtile *gt, const uint32_t *hist, const uint16_t *beg_t, uint32_t b1,
-                               uint32_t bits, const void *in, void *out, uint16_t *run, uint64_t *cst, uint32_t tileA) {
+                               uint32_t bits, const void *in, void *out, uint16_t *run_t, uint32_t rstride, uint64_t *cst,
+                               uint32_t tileA) {
 	// shapes must match the pass-B tile (cfg_for(wv, 0), or 1024 x 4)
 #define G(W, N, I) \
-	launch_sortchunk_g_t<W, N, I>(ctx, grid, lds, gt, hist, beg_t, b1, bits, in, out, run, cst, tileA)
+	launch_sortchunk_g_t<W, N, I>(ctx, grid, lds, gt, hist, beg_t, b1, bits, in, out, run_t, rstride, cst, tileA)
 	switch (wv) {
 	case 1:
 		if (pc.nt == 1024) G(1, 1024, 4);
@@ -2769,7 +2770,7 @@ static int radix_engine(hj_ctx_t *ctx, const hj_relation_t *R, const hj_relation
 	const uint64_t hist_len = std::max(hist1_len, hist2_len) + 1;
 	if ((rc = hj_reserve(ctx, ctx->hist, hist_len * 8))) return rc;
 	if ((rc = hj_reserve(ctx, ctx->scan_tmp, ((hist_len + Q + nseg2) / SCAN_CH + 2) * 8 + 4096))) return rc;
-	// meta: segs1[2] | segs2[nseg2] | cmap[max chunks] | pb1 | pairs | tiles | runs | chunk starts
+	// meta: segs1[2] | segs2[nseg2] | cmap[max chunks] | pb1 | pairs | tiles | chunk starts
 	const uint64_t cmap_n = std::max<uint64_t>(chunks1, chunks2_max);
 	size_t off = 0;
 	auto take = [&](size_t bytes) {
@@ -2780,7 +2781,6 @@ static int radix_engine(hj_ctx_t *ctx, const hj_relation_t *R, const hj_relation
 	const size_t o_seg1 = take(2 * sizeof(hj_seg)), o_seg2 = take(nseg2 * sizeof(hj_seg));
 	const size_t o_cmap = take(cmap_n * 4), o_pb1 = take(((uint64_t)2 * F1 + 1) * 8);
 	const size_t o_pairs = take((size_t)Q * sizeof(hj_pair)), o_tiles = take(((size_t)Q + 1) * 8);
-	const size_t o_run = take(P == 2 ? chunks2_max * (F2 + 1) * 2 : 8);
 	const size_t o_cst = take(P == 2 ? chunks2_max * 8 : 8);
 	const size_t o_runA = take(P == 2 ? chunks1 * (F1 + 1) * 2 : 8), o_cstA = take(P == 2 ? chunks1 * 8 : 8);
 	const size_t o_begt = take(P == 2 ? (hist1_len + 1) * 2 : 8);
@@ -2799,7 +2799,7 @@ static int radix_engine(hj_ctx_t *ctx, const hj_relation_t *R, const hj_relation
 	uint64_t *pb1 = (uint64_t *)(meta + o_pb1);
 	uint64_t *tiles = (uint64_t *)(meta + o_tiles);
 	hj_pair *pairs = (hj_pair *)(meta + o_pairs);
-	uint16_t *runs = (uint16_t *)(meta + o_run), *runsA = (uint16_t *)(meta + o_runA);
+	uint16_t *runsA = (uint16_t *)(meta + o_runA);
 	uint64_t *cst = (uint64_t *)(meta + o_cst), *cstA = (uint64_t *)(meta + o_cstA);
 	hj_gtile *gt = (hj_gtile *)(meta + o_gt);
 	uint16_t *beg_t = (uint16_t *)(meta + o_begt), *run_t = (uint16_t *)(meta + o_runt);
@@ -2869,7 +2869,7 @@ static int radix_engine(hj_ctx_t *ctx, const hj_relation_t *R, const hj_relation
 			const uint32_t pb_dblk = pb_dv ? 1u << (pb_dv - 1) : 8u;
 			allow_lds(ctx, (const void *)k_first_run);
 			hipLaunchKernelGGL(k_first_run, dim3((unsigned)std::min<uint64_t>(4096, (chunks2_max + 255) / 256)),
-			                   dim3(256), (size_t)16 * F1, ctx->stream, segs1, segs2, st, b1, (uint32_t)ch2, hist32,
+			                   dim3(256), (size_t)24 * F1, ctx->stream, segs1, segs2, st, b1, (uint32_t)ch2, hist32,
 			                   gt, b2, dpre, dtiles, pairs, tiles, std::min(pb_dblk, F1));
 			// pass B: each first-level partition's tiles gathered from the
 			// pass-A runs and sorted by the second digit
@@ -2881,16 +2881,14 @@ static int radix_engine(hj_ctx_t *ctx, const hj_relation_t *R, const hj_relation
 			const uint32_t gsel = (variant >> 14) & 3;
 			const unsigned sgrid = gsel == 1 ? ~0u : (unsigned)ctx->num_cu * (gsel == 2 ? 1u : 2u);
 			hj_kt_next(ctx, HJ_K_GATHERSORT);
-			launch_sortchunk_g(ctx, wv, pc2, sgrid, sldsB, gt, hist32, beg_t, b1, b2, ctx->part[0].p, ctx->part[1].p, runs,
-			                   cst, (uint32_t)pcA.tile);
-			hipLaunchKernelGGL(k_run_transpose, dim3((unsigned)((chunks2_max + 63) / 64), (F2 + 1 + 63) / 64),
-			                   dim3(256), 0, ctx->stream, runs, st, F2, (uint32_t)chunks2_max, run_t);
+			launch_sortchunk_g(ctx, wv, pc2, sgrid, sldsB, gt, hist32, beg_t, b1, b2, ctx->part[0].p, ctx->part[1].p, run_t,
+			                   (uint32_t)chunks2_max, cst, (uint32_t)pcA.tile);
 			Rb = Sb = ctx->part[1].p;
 		}
 	}
 	HJ_HIP(ctx, hipGetLastError());
 	HJ_HIP(ctx, hipEventRecord(ctx->ev[1], ctx->stream));
-	if (P == 2) hj_kt_close_at(ctx, 1);  // the pass-B region (and the run transpose)
+	if (P == 2) hj_kt_close_at(ctx, 1);  // the pass-B region
 
 	// join: tile the probe side so a skewed partition spreads over workgroups
 	const uint64_t avg_s = (ns >> B) + 1;
