@@ -1798,7 +1798,15 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
 // bucket (key >> B) & (nb - 1), fills of at most `cap` build tuples joined
 // one after the other), so the result is bc's.
 // LDS: kv u64[cap] | heads u32[np2(cap)] | next u16[cap]
-template <int NT>
+//
+// LPT: the same loop with HJ_TK_LP's table (k_join2's lp): slots u32[np2(cap)]
+// hold 1-based tuple indices, claimed by CAS in the fill's order after the
+// whole partition is in LDS, linear probing from (key >> B) & (nb - 1), the
+// first match only.  A second build tuple with the same key (HJ_DF_DUPKEY) or
+// a partition larger than one fill (HJ_DF_OVERSIZE) raises a flag and the
+// join is recomputed by the exact path, as after k_join2.
+// LDS: kv u64[cap] | slots u32[np2(cap)]
+template <int NT, bool LPT = false>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(6))) void k_join2n(const uint2 *__restrict__ P, const uint64_t *__restrict__ cstart,
                                                const uint16_t *__restrict__ run_t, uint32_t stride, uint32_t b1,
                                                const hj_pair *__restrict__ pairs,
@@ -1817,8 +1825,9 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(6))) void k_
 	__shared__ uint32_t pend[NT];  // staged probe run k: its end (tuple index)
 	const uint32_t nb = hj_np2u(cap), hmask = nb - 1;
 	uint64_t *kv = lds_u64;
-	uint32_t *heads = (uint32_t *)(kv + cap);
-	uint16_t *next = (uint16_t *)(heads + nb);
+	uint32_t *heads = (uint32_t *)(kv + cap);  // LPT: the slots
+	uint16_t *next = (uint16_t *)(heads + nb);  // (not used with LPT)
+	[[maybe_unused]] bool dup = false, oversize = false;
 	const uint32_t lane = threadIdx.x & (HJ_WAVE - 1), wid = threadIdx.x / HJ_WAVE;
 	const uint64_t total = tprefix[npairs];
 	const uint32_t b2 = B - b1, m2 = (1u << b2) - 1;
@@ -1943,8 +1952,37 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(6))) void k_
 			rid[2 * u] = t[u].y;
 			key[2 * u + 1] = t[u].z;
 			rid[2 * u + 1] = t[u].w;
-			c[2 * u] = live && ((ok >> (2 * u)) & 1u) ? heads[(t[u].x >> hs) & hmask] : 0u;
-			c[2 * u + 1] = live && ((ok >> (2 * u + 1)) & 1u) ? heads[(t[u].z >> hs) & hmask] : 0u;
+			// bc: the chain's first entry; lp: the home slot's index + 1
+			const uint32_t h0 = (t[u].x >> hs) & hmask, h1 = (t[u].z >> hs) & hmask;
+			c[2 * u] = live && ((ok >> (2 * u)) & 1u) ? (LPT ? h0 + 1 : heads[h0]) : 0u;
+			c[2 * u + 1] = live && ((ok >> (2 * u + 1)) & 1u) ? (LPT ? h1 + 1 : heads[h1]) : 0u;
+		}
+		if constexpr (LPT) {
+			// c[u] = the next slot's index + 1 (0: done); at most nb slots
+			// (the table always has an empty one: m <= cap <= nb)
+			bool any = true;
+			for (uint32_t step = 0; any && step < nb; step++) {
+				any = false;
+#pragma unroll
+				for (uint32_t u = 0; u < 2 * RP; u++) {
+					if (!c[u]) continue;
+					const uint32_t o = heads[c[u] - 1];
+					if (!o) {
+						c[u] = 0;
+						continue;
+					}
+					const uint64_t e = kv[o - 1];
+					if ((uint32_t)e == key[u]) {
+						cnt++;
+						sum += (e >> 32) + rid[u];
+						c[u] = 0;
+						continue;
+					}
+					c[u] = (c[u] & hmask) + 1;  // slot c[u] - 1 + 1
+					any = true;
+				}
+			}
+			return;
 		}
 		bool any = true;
 		while (any) {
@@ -2009,7 +2047,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(6))) void k_
 							if (f >= fe) continue;
 							const uint32_t i = m + f;
 							kv[i] = (uint64_t)t[u].x | (uint64_t)t[u].y << 32;
-							next[i] = (uint16_t)atomicExch(&heads[(t[u].x >> hs) & hmask], i + 1);
+							if constexpr (!LPT) next[i] = (uint16_t)atomicExch(&heads[(t[u].x >> hs) & hmask], i + 1);
 						}
 					}
 				}
@@ -2018,6 +2056,28 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(6))) void k_
 				roff = roff_new;
 				__syncthreads();
 				if (m >= cap || rc >= pr.r1) break;
+			}
+			if constexpr (LPT) {
+				if (rc < pr.r1) {
+					oversize = true;  // the host recomputes this join exactly
+					break;
+				}
+				// claim slots for the m tuples in LDS (every thread saw the
+				// fill's last barrier, so kv[0..m) is complete)
+				for (uint32_t i = threadIdx.x; i < m; i += NT) {
+					const uint32_t key = (uint32_t)kv[i];
+					uint32_t p = (key >> hs) & hmask;
+					for (uint32_t step = 0; step < nb; step++) {
+						const uint32_t old = atomicCAS(&heads[p], 0u, i + 1);
+						if (old == 0) break;
+						if ((uint32_t)kv[old - 1] == key) {
+							dup = true;
+							break;
+						}
+						p = (p + 1) & hmask;
+					}
+				}
+				__syncthreads();
 			}
 			// ---- probe the tile's runs, NT chunks at a time, in pairs
 			for (uint64_t kb = sb; m && kb < se; kb += NT) {
@@ -2042,11 +2102,17 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(6))) void k_
 		}
 		__syncthreads();
 	}
+	if constexpr (LPT) {
+		const unsigned fl = (__any(dup) ? HJ_DF_DUPKEY : 0u) | (__any(oversize) ? HJ_DF_OVERSIZE : 0u);
+		if (fl && (threadIdx.x & (HJ_WAVE - 1)) == 0) atomicOr(&st->flags, fl);
+	}
 	hj_block_add2(cnt, sum, red, &st->matched, &st->checksum);
 }
 
-// LDS bytes of k_join2n's table for `cap` build tuples
-static size_t join2n_lds(uint32_t cap) { return (size_t)cap * 10 + (size_t)hj_np2u(cap) * 4 + 8; }
+// LDS bytes of k_join2n's table for `cap` build tuples (lp: no chain links)
+static size_t join2n_lds(uint32_t cap, bool lp = false) {
+	return (size_t)cap * (lp ? 8 : 10) + (size_t)hj_np2u(cap) * 4 + 8;
+}
 
 // ------------------------------------------------------------ host side
 struct part_cfg {
@@ -2462,16 +2528,17 @@ static void launch_join2_t(hj_ctx_t *ctx, unsigned grid, size_t lds, const void 
 			}
 		} else {
 			if (ctx->narrow_ok) {
-				if constexpr (TK == HJ_TK_BC) {
+				if constexpr (TK == HJ_TK_BC || TK == HJ_TK_LP) {
 					// the compact table: as many workgroups per CU as its LDS allows
-					const size_t nl = join2n_lds(cap), all = nl + static_lds((const void *)k_join2n<512>);
+					constexpr bool L = TK == HJ_TK_LP;
+					auto kfn = k_join2n<512, L>;
+					const size_t nl = join2n_lds(cap, L), all = nl + static_lds((const void *)kfn);
 					const unsigned per_cu = (unsigned)std::max<size_t>(1, std::min<size_t>(4, (160 * 1024) / all));
 					const unsigned ng = (unsigned)std::max<uint64_t>(
 					    1, std::min<uint64_t>((uint64_t)ctx->num_cu * per_cu, (uint64_t)npairs * 64));
-					allow_lds(ctx, (const void *)k_join2n<512>);
-					hipLaunchKernelGGL(k_join2n<512>, dim3(ng), dim3(512), nl, ctx->stream,
-					                   (const uint2 *)P, cst, run_t, stride, b1, pairs, tpre, npairs, B, cpt, cap,
-					                   (hj_stats *)ctx->stats.p, ctx->narrow_ok);
+					allow_lds(ctx, (const void *)kfn);
+					hipLaunchKernelGGL(kfn, dim3(ng), dim3(512), nl, ctx->stream, (const uint2 *)P, cst, run_t, stride, b1,
+					                   pairs, tpre, npairs, B, cpt, cap, (hj_stats *)ctx->stats.p, ctx->narrow_ok);
 				} else {
 					launch(k_join2<1, 512, TK, false, true>);
 				}

@@ -100,6 +100,29 @@ def test_lp_fills_and_cross_fill_duplicates(ctx, bits, passes):
         ctx.free_relation(d)
 
 
+@pytest.mark.parametrize("bits", [10, 13])
+def test_lp_narrow_two_pass_duplicates(ctx, bits):
+    """Two passes over narrow tuples (keys and row ids below 2^32) join lp
+    partitions with the compact 8-byte table: unique keys need no re-run;
+    one repeated build key is caught when its partition's slots are claimed
+    and the join is recomputed, so the first build row of each key wins."""
+    nr, ns = 1 << 16, 1 << 20
+    R, S = O.generate("pkfk", nr, ns, 16, seed=43)
+    dR, dS = _dev(ctx, R, 16), _dev(ctx, S, 16)
+    got = ctx.join("phj_radix_lp", dR, dS, radix_bits=bits, passes=2)
+    _check(got, O.aggregate(R, S, 16, lp=True), ("unique", bits))
+    assert got["narrow"] == 1 and got["fallback"] == 0, got
+    R2 = R.copy()
+    w = R2.view(np.uint64)
+    w[2 * (nr - 1)] = w[2 * 5]  # the last R key := the key of R[5]
+    dR2 = _dev(ctx, R2, 16)
+    got2 = ctx.join("phj_radix_lp", dR2, dS, radix_bits=bits, passes=2)
+    _check(got2, O.aggregate(R2, S, 16, lp=True), ("duplicate", bits))
+    assert got2["fallback"] & hjmi.HJ_FB_LP_DUPLICATES, got2
+    for d in (dR, dS, dR2):
+        ctx.free_relation(d)
+
+
 @pytest.mark.parametrize("algo", ALGOS)
 @pytest.mark.parametrize("width", [16, 64])
 def test_duplicates_and_misses(ctx, algo, width):
