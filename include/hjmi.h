@@ -83,7 +83,8 @@ typedef struct {
 	uint32_t reserved[2];
 } hj_params_t;
 
-#define HJ_F_KERNEL_TIMES 1u  /* record a HIP event pair around every kernel */
+#define HJ_F_KERNEL_TIMES 1u  /* time each phase with HIP events (two-pass joins: one event between
+                                 two phases, so the phases tile the join's device time) */
 #define HJ_F_NO_NARROW 2u     /* keep 16-byte tuples through the second radix
                                  pass even when keys and row ids fit 32 bits */
 #define HJ_F_NO_POSITIONAL 4u /* two radix passes: store every narrow first-
